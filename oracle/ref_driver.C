@@ -1,0 +1,288 @@
+// Driver for the reference build in oracle/_ref (checker only; never shipped
+// as product code).  It wires the reference's own public call surface --
+// TFESpace3D, TSquareStructure3D::Sort, TSquareMatrix3D, TDiscreteForm3D,
+// TAssembleMat3D, TMGLevel3D, TMultiGrid3D, TMultiGridScaIte, TFgmresIte
+// (src/System/SystemCD3D.C:36-203,227-311 wires the same objects for
+// GALERKIN only; this driver also builds the SDFEM form from
+// BilinearAssemble_SD, src/FE/ConvDiff3D.C:124) -- and dumps what the
+// reference computes, for the committed golden fixtures.
+//
+// usage: cd3d_ref <file.dat> <outdir|-> [mode]
+//   mode "cd"  (default): assemble all levels, V-cycle probe, FGMRES solve
+//   mode "tcd": Crank-Nicolson steps of the TCD3D system (TSystemTCD3D)
+//
+// Coefficients come from the .dat P-parameters:
+//   P0 eps, P1..P3 convection b (P7>0: rotating field
+//   b = (-(y - P13), x - P12, 0.1 sin(2 pi t)) of the TCD3D config),
+//   P4 reaction c, P5 source kind (0: f = P6, 1: manufactured
+//   u = sin(pi x) sin(pi y) sin(pi z)), P8 number of CN steps,
+//   P9..P11 centre of the Gaussian pulse initial condition (sigma 0.05).
+// The mesh is what the reference builds from data/Wuerfel.GEO: ReadGeo.C:1753-1755
+// scales parametrised vertices by (1, 0.346, 0.346), so the domain is the box
+// [0,1] x [0,0.346] x [0,0.346].
+// Boundary: Dirichlet u = 0 on every face.
+
+#include <Domain.h>
+#include <Database.h>
+#include <FEDatabase3D.h>
+#include <FESpace3D.h>
+#include <SquareStructure3D.h>
+#include <SquareMatrix3D.h>
+#include <DiscreteForm3D.h>
+#include <AssembleMat3D.h>
+#include <AuxParam3D.h>
+#include <MultiGrid3D.h>
+#include <MGLevel3D.h>
+#include <MultiGridScaIte.h>
+#include <FgmresIte.h>
+// ConvDiff3D.h defines data objects; declare the two kernels used here (ConvDiff3D.C:61,124)
+void BilinearAssemble(double, double *, double, double **, int *, double ***, double **);
+void BilinearAssemble_SD(double, double *, double, double **, int *, double ***, double **);
+#include <SystemTCD3D.h>
+#include <LinAlg.h>
+#include <MainUtilities.h>
+#include <MooNMD_Io.h>
+#include <It_Finest.h>
+#include <TimeDiscRout.h>
+#include <FEFunction3D.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <iostream>
+#include <iomanip>
+
+void SetPolynomialDegree();
+
+static std::string g_out;
+
+
+static double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+static void dump(const std::string &name, const void *p, size_t bytes) {
+  if (g_out == "-") return;
+  FILE *f = fopen((g_out + "/" + name).c_str(), "wb");
+  if (!f) { perror(name.c_str()); exit(2); }
+  fwrite(p, 1, bytes, f);
+  fclose(f);
+}
+
+// ---------------------------------------------------------------- callbacks
+static void conv_field(double x, double y, double z, double t, double &b1, double &b2, double &b3) {
+  if (TDatabase::ParamDB->P7 > 0) {
+    b1 = -(y - TDatabase::ParamDB->P13);
+    b2 = x - TDatabase::ParamDB->P12;
+    b3 = 0.1 * sin(2 * Pi * t);
+  } else {
+    b1 = TDatabase::ParamDB->P1;
+    b2 = TDatabase::ParamDB->P2;
+    b3 = TDatabase::ParamDB->P3;
+  }
+}
+
+void DrvCoeffs(int n_points, double *X, double *Y, double *Z, double **parameters, double **coeffs) {
+  double eps = TDatabase::ParamDB->P0, c = TDatabase::ParamDB->P4;
+  double t = TDatabase::TimeDB->CURRENTTIME;
+  for (int i = 0; i < n_points; i++) {
+    double *co = coeffs[i], b1, b2, b3;
+    conv_field(X[i], Y[i], Z[i], t, b1, b2, b3);
+    co[0] = eps;
+    co[1] = b1; co[2] = b2; co[3] = b3;
+    co[4] = c;
+    if ((int)TDatabase::ParamDB->P5 == 1) {
+      double sx = sin(Pi * X[i]), sy = sin(Pi * Y[i]), sz = sin(Pi * Z[i]);
+      double cx = cos(Pi * X[i]), cy = cos(Pi * Y[i]), cz = cos(Pi * Z[i]);
+      double u = sx * sy * sz;
+      co[5] = eps * 3 * Pi * Pi * u + b1 * Pi * cx * sy * sz + b2 * Pi * sx * cy * sz +
+              b3 * Pi * sx * sy * cz + c * u;
+    } else {
+      co[5] = TDatabase::ParamDB->P6;
+    }
+    co[6] = sqrt(b1 * b1 + b2 * b2 + b3 * b3);
+  }
+}
+
+void DrvBoundCondition(int, double, double, double, BoundCond &cond) { cond = DIRICHLET; }
+void DrvBoundValue(int, double, double, double, double &value) { value = 0; }
+void DrvInitialCondition(double x, double y, double z, double *values) {
+  double cx = TDatabase::ParamDB->P9, cy = TDatabase::ParamDB->P10, cz = TDatabase::ParamDB->P11;
+  double s = 0.05, r2 = (x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz);
+  values[0] = exp(-r2 / (2 * s * s));
+}
+
+// ---------------------------------------------------------------- helpers
+static double norm2(int n, const double *v) {
+  double s = 0;
+  for (int i = 0; i < n; i++) s += v[i] * v[i];
+  return sqrt(s);
+}
+
+static void dump_level(int lev, TFESpace3D *sp, TSquareMatrix3D *A, double *rhs) {
+  char b[64];
+  TCollection *coll = sp->GetCollection();
+  int nc = coll->GetN_Cells();
+  std::vector<double> xyz(nc * 24);
+  for (int c = 0; c < nc; c++) {
+    TBaseCell *cell = coll->GetCell(c);
+    for (int v = 0; v < 8; v++)
+      cell->GetVertex(v)->GetCoords(xyz[c * 24 + 3 * v], xyz[c * 24 + 3 * v + 1], xyz[c * 24 + 3 * v + 2]);
+  }
+  int nb = sp->GetBeginIndex()[1];
+  snprintf(b, sizeof b, "L%d_cells.f64", lev); dump(b, xyz.data(), xyz.size() * 8);
+  snprintf(b, sizeof b, "L%d_gn.i32", lev); dump(b, sp->GetGlobalNumbers(), (size_t)nc * nb * 4);
+  int N = A->GetN_Rows(), nnz = A->GetN_Entries();
+  snprintf(b, sizeof b, "L%d_rowptr.i32", lev); dump(b, A->GetRowPtr(), (size_t)(N + 1) * 4);
+  snprintf(b, sizeof b, "L%d_kcol.i32", lev); dump(b, A->GetKCol(), (size_t)nnz * 4);
+  snprintf(b, sizeof b, "L%d_entries.f64", lev); dump(b, A->GetEntries(), (size_t)nnz * 8);
+  snprintf(b, sizeof b, "L%d_rhs.f64", lev); dump(b, rhs, (size_t)N * 8);
+  printf("LEVEL %d cells %d nbf %d ndof %d active %d hanging_bound %d dirichlet %d nnz %d\n", lev, nc, nb,
+         sp->GetN_DegreesOfFreedom(), sp->GetN_ActiveDegrees(), sp->GetHangingBound(), sp->GetN_Dirichlet(), nnz);
+}
+
+// ---------------------------------------------------------------- main
+int main(int argc, char **argv) {
+  if (argc < 3) { fprintf(stderr, "usage: %s file.dat outdir|- [cd|tcd]\n", argv[0]); return 1; }
+  g_out = argv[2];
+  std::string mode = argc > 3 ? argv[3] : "cd";
+  TDatabase *Database = new TDatabase();
+  TFEDatabase3D *FEDatabase = new TFEDatabase3D();
+  TDomain *Domain = new TDomain(argv[1]);
+  OpenFiles();
+  std::cout << std::setprecision(17);
+  OutFile << std::setprecision(17);
+  if (TDatabase::ParamDB->DISCTYPE == SDFEM) SetPolynomialDegree();
+
+  Domain->Init(TDatabase::ParamDB->BNDFILE, TDatabase::ParamDB->GEOFILE);
+  for (int i = 0; i < TDatabase::ParamDB->UNIFORM_STEPS; i++) Domain->RegRefineAll();
+
+  int L = TDatabase::ParamDB->LEVELS, ORDER = TDatabase::ParamDB->ANSATZ_ORDER;
+  char Name[] = "name", Desc[] = "desc";
+  std::vector<TFESpace3D *> sp(L);
+  std::vector<TSquareMatrix3D *> A(L), M(L);
+  std::vector<double *> sol(L), rhs(L), oldrhs(L);
+  double t0 = now_s();
+  for (int i = 0; i < L; i++) {
+    if (i) Domain->RegRefineAll();
+    TCollection *coll = Domain->GetCollection(It_Finest, 0);
+    sp[i] = new TFESpace3D(coll, Name, Desc, DrvBoundCondition, ORDER);
+    int N = sp[i]->GetN_DegreesOfFreedom();
+    sol[i] = new double[N]();
+    rhs[i] = new double[N]();
+  }
+  double t_space = now_s() - t0;
+
+  if (mode == "tcd") {
+    // TSystemTCD3D path (src/System/SystemTCD3D.C), Crank-Nicolson, following
+    // the time loop of 3DPrograms/TCD3D_ParMooN.C:539-637.
+    TDatabase::TimeDB->TIME_DISC = 2;
+    int Nf = sp[L - 1]->GetN_DegreesOfFreedom();
+    TFEFunction3D *u0 = new TFEFunction3D(sp[L - 1], Name, Desc, sol[L - 1], Nf);
+    u0->Interpolate(DrvInitialCondition);
+    printf("TCD interpolated |u0| %.17e\n", norm2(Nf, sol[L - 1]));
+    TSystemTCD3D *S = new TSystemTCD3D(L, sp.data(), sol.data(), rhs.data(), TDatabase::ParamDB->DISCTYPE,
+                                       TDatabase::ParamDB->SOLVER_TYPE);
+    TDatabase::TimeDB->CURRENTTIME = TDatabase::TimeDB->STARTTIME;
+    S->Init(DrvCoeffs, DrvBoundCondition, DrvBoundValue, NULL);
+    printf("TCD after Init |u0| %.17e\n", norm2(Nf, sol[L - 1]));
+    S->AssembleMRhs();
+    printf("TCD after AssembleMRhs |u0| %.17e\n", norm2(Nf, sol[L - 1]));
+    std::vector<double> old_rhs(rhs[L - 1], rhs[L - 1] + Nf), old_sol(Nf);
+    dump("tcd_u0.f64", sol[L - 1], (size_t)Nf * 8);
+    int nsteps = (int)TDatabase::ParamDB->P8;
+    double ts = now_s();
+    for (int m = 0; m < nsteps; m++) {
+      SetTimeDiscParameters(1);
+      memcpy(old_sol.data(), sol[L - 1], Nf * 8);
+      double tau = TDatabase::TimeDB->CURRENTTIMESTEPLENGTH;
+      TDatabase::TimeDB->CURRENTTIME += tau;
+      S->AssembleARhs();
+      S->AssembleSystMat(old_rhs.data(), old_sol.data(), rhs[L - 1], sol[L - 1]);
+      memcpy(old_rhs.data(), rhs[L - 1], Nf * 8);
+      S->Solve(sol[L - 1]);
+      S->RestoreMassMat();
+      char b[64];
+      snprintf(b, sizeof b, "tcd_u%d.f64", m + 1);
+      dump(b, sol[L - 1], (size_t)Nf * 8);
+      printf("TCD step %d t %.17e |u| %.17e\n", m + 1, TDatabase::TimeDB->CURRENTTIME, norm2(Nf, sol[L - 1]));
+    }
+    printf("TIME tcd_steps %.6f\n", now_s() - ts);
+    return 0;
+  }
+
+  // ---- CD3D: structures, matrices, discrete form (Galerkin or SDFEM)
+  t0 = now_s();
+  char cd[] = "Conv-Diff", gal[] = "form";
+  static MultiIndex3D Der[4] = {D100, D010, D001, D000};
+  static int SpNum[4] = {0, 0, 0, 0}, RowSp[1] = {0}, ColSp[1] = {0}, RhsSp[1] = {0};
+  AssembleFct3D *kern = TDatabase::ParamDB->DISCTYPE == SDFEM ? BilinearAssemble_SD : BilinearAssemble;
+  TDiscreteForm3D *form = new TDiscreteForm3D(cd, gal, 4, Der, SpNum, 1, 1, RowSp, ColSp, RhsSp, kern, DrvCoeffs, NULL);
+  BoundCondFunct3D *BC[1] = {DrvBoundCondition};
+  BoundValueFunct3D *BV[1] = {DrvBoundValue};
+  std::vector<TAssembleMat3D *> asmb(L);
+  for (int i = 0; i < L; i++) {
+    TSquareStructure3D *st = new TSquareStructure3D(sp[i]);
+    st->Sort();
+    A[i] = new TSquareMatrix3D(st);
+    TFESpace3D *fesp[1] = {sp[i]};
+    TSquareMatrix3D *SQ[1] = {A[i]};
+    double *R[1] = {rhs[i]};
+    TAuxParam3D *aux = new TAuxParam3D(1, 0, 0, 0, fesp, NULL, NULL, NULL, NULL, 0, NULL);
+    asmb[i] = new TAssembleMat3D(1, fesp, 1, SQ, 0, NULL, 1, R, fesp, form, BC, BV, aux);
+    asmb[i]->Init();
+  }
+  double t_struct = now_s() - t0;
+  t0 = now_s();
+  for (int i = 0; i < L; i++) {
+    int N = sp[i]->GetN_DegreesOfFreedom(), NA = sp[i]->GetActiveBound();
+    asmb[i]->Reset();
+    asmb[i]->Assemble3D();
+    memcpy(sol[i] + NA, rhs[i] + NA, (N - NA) * 8);  // SystemCD3D.C:330
+  }
+  double t_asm = now_s() - t0;
+  for (int i = 0; i < L; i++) dump_level(i, sp[i], A[i], rhs[i]);
+
+  // ---- multigrid hierarchy + FGMRES (SystemCD3D.C:151-203)
+  double Parameters[2] = {TDatabase::ParamDB->SC_SMOOTH_DAMP_FACTOR_SCALAR,
+                          TDatabase::ParamDB->SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR};
+  TMultiGrid3D *MG = new TMultiGrid3D(1, 2, Parameters);
+  for (int i = 0; i < L; i++) MG->AddLevel(new TMGLevel3D(i, A[i], rhs[i], sol[i], 2, NULL));
+  int N = sp[L - 1]->GetN_DegreesOfFreedom();
+  TSquareMatrix *SQM[1] = {A[L - 1]};
+  TMultiGridScaIte *prec = new TMultiGridScaIte(MatVect_Scalar, Defect_Scalar, NULL, 0, N, MG, 1);
+
+  // the solver works on its own copies (SystemCD3D.C:351-355)
+  std::vector<double> x(sol[L - 1], sol[L - 1] + N), b(rhs[L - 1], rhs[L - 1] + N), r(N), z(N);
+  double r0;
+  ScalarDefect(A[L - 1], x.data(), b.data(), r.data(), r0);
+
+  // V-cycle probe: z = V(r / |r|)
+  std::vector<double> rn(r);
+  for (int i = 0; i < N; i++) rn[i] /= r0;
+  std::vector<double> save_sol(sol[L - 1], sol[L - 1] + N), save_rhs(rhs[L - 1], rhs[L - 1] + N);
+  prec->Iterate(SQM, NULL, z.data(), rn.data());
+  dump("vcycle_in.f64", rn.data(), (size_t)N * 8);
+  dump("vcycle_out.f64", z.data(), (size_t)N * 8);
+  memcpy(sol[L - 1], save_sol.data(), N * 8);
+  memcpy(rhs[L - 1], save_rhs.data(), N * 8);
+
+  double rel = TDatabase::ParamDB->SC_LIN_RED_FACTOR_SCALAR;
+  TDatabase::ParamDB->SC_LIN_RES_NORM_MIN_SCALAR = rel * r0;
+  TDatabase::ParamDB->SC_LIN_RED_FACTOR_SCALAR = 0.0;
+  TFgmresIte *fg = new TFgmresIte(MatVect_Scalar, Defect_Scalar, prec, 0, N, 1);
+  double ts = now_s();
+  int its = fg->Iterate(SQM, NULL, x.data(), b.data());
+  double t_solve = now_s() - ts;
+  double rf;
+  ScalarDefect(A[L - 1], x.data(), b.data(), r.data(), rf);
+  dump("sol.f64", x.data(), (size_t)N * 8);
+  printf("SOLVE its %d r0 %.17e rfinal %.17e rel %.17e\n", its, r0, rf, rf / r0);
+  printf("TIME space %.6f struct %.6f assemble %.6f solve %.6f\n", t_space, t_struct, t_asm, t_solve);
+  return 0;
+}

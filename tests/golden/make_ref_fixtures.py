@@ -1,0 +1,133 @@
+#!/usr/bin/env python3
+"""Write the golden fixtures of the CD3D/TCD3D path from the reference build.
+
+Runs ``oracle/_ref/cd3d_ref`` (the reference compiled from its own sources by
+``oracle/build_ref.py``) on a set of small cases and stores what the reference
+computed -- cell vertex lists, cell-to-DOF maps, CSR pattern and entries, rhs,
+a V-cycle probe, the FGMRES solution and its iteration count, and TCD3D
+Crank-Nicolson states -- as compressed ``.npz`` files next to this script.
+
+Only data is stored; the reference itself never travels.  This script runs
+where the reference sources exist (it needs ``/root/reference/data/Wuerfel.*``).
+
+usage: python tests/golden/make_ref_fixtures.py [case ...]
+"""
+import os
+import re
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+# The reference reads data/Wuerfel.GEO into the box [0,1] x [0,0.346] x [0,0.346]
+# (ReadGeo.C:1753-1755 scales parametrised y and z by 0.346); every fixture
+# lives on that box.
+BOX = (1.0, 0.346, 0.346)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+EXE = os.path.join(REPO, "oracle", "_ref", "cd3d_ref")
+REF = os.environ.get("PARMOON_REF", "/root/reference")
+
+# Solver settings shared by every case (Jacobi 0.8,
+# 2 pre + 2 post, V-cycle, FGMRES(20); coarse smoother SSOR because the
+# reference's exact coarse solve needs an external LAPACK).
+SOLVER = dict(
+    SOLVER_TYPE=1, SC_MG_TYPE_SCALAR=0, SC_SOLVER_SCALAR=16, SC_PRECONDITIONER_SCALAR=5,
+    SC_GMRES_RESTART=20, SC_LIN_MAXIT_SCALAR=200, SC_LIN_RED_FACTOR_SCALAR=1e-10,
+    SC_LIN_RES_NORM_MIN_SCALAR=1e-30, SC_MG_CYCLE_SCALAR=1, SC_SMOOTHER_SCALAR=1,
+    SC_PRE_SMOOTH_SCALAR=2, SC_POST_SMOOTH_SCALAR=2, SC_SMOOTH_DAMP_FACTOR_SCALAR=0.8,
+    SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=0.8, SC_COARSE_SMOOTHER_SCALAR=3,
+    SC_COARSE_MAXIT_SCALAR=10, SC_COARSE_RED_FACTOR_SCALAR=0.1, SC_GMG_DAMP_FACTOR_SCALAR=1.0,
+    SC_GMG_DAMP_FACTOR_FINE_SCALAR=1.0, SC_STEP_LENGTH_CONTROL_FINE_SCALAR=0,
+    SC_STEP_LENGTH_CONTROL_ALL_SCALAR=0, SC_AMG_PREC_IT=1, SC_MINIT=0, SC_VERBOSE=1,
+    SC_FLEXIBLE_KRYLOV_SPACE_SOLVER=1,
+)
+
+# name -> (mode, .dat overrides)
+CASES = {
+    # Poisson with manufactured solution: configuration 1 at 2 and 3 levels
+    "q1_l2_poisson": ("cd", dict(ANSATZ_ORDER=1, LEVELS=2, UNIFORM_STEPS=2, DISCTYPE=1,
+                                 P0=1.0, P1=0, P2=0, P3=0, P4=0, P5=1)),
+    "q1_c1": ("cd", dict(ANSATZ_ORDER=1, LEVELS=3, UNIFORM_STEPS=2, DISCTYPE=1,
+                         P0=1.0, P1=0, P2=0, P3=0, P4=0, P5=1)),
+    # SUPG convection-diffusion, configuration 2 problem at reduced size
+    "q2_l2_supg": ("cd", dict(ANSATZ_ORDER=2, LEVELS=2, UNIFORM_STEPS=1, DISCTYPE=2,
+                              CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                              P0=1e-3, P1=1.0, P2=0.5, P3=0.25, P4=0, P5=0, P6=1.0)),
+    "q1_l3_supg": ("cd", dict(ANSATZ_ORDER=1, LEVELS=3, UNIFORM_STEPS=2, DISCTYPE=2,
+                              CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                              P0=1e-6, P1=-0.3, P2=0.7, P3=0.4, P4=0.5, P5=0, P6=1.0)),
+    "q2_l2_galerkin_conv": ("cd", dict(ANSATZ_ORDER=2, LEVELS=2, UNIFORM_STEPS=1, DISCTYPE=1,
+                                       P0=0.1, P1=1.0, P2=0.5, P3=0.25, P4=1.0, P5=1)),
+    # TCD3D, Crank-Nicolson with the rotating convection field (configuration 3 problem)
+    # (rotation centre and pulse centre placed inside the reference's box)
+    "q1_tcd": ("tcd", dict(ANSATZ_ORDER=1, LEVELS=2, UNIFORM_STEPS=2, DISCTYPE=1,
+                           P0=1e-4, P4=0, P5=0, P6=0.0, P7=1, P8=3,
+                           P9=0.3, P10=0.173, P11=0.173, P12=0.5, P13=0.173,
+                           TIME_DISC=2, STARTTIME=0.0, ENDTIME=1.0, TIMESTEPLENGTH=1e-3)),
+}
+
+
+def write_dat(path, overrides):
+    kv = dict(GEOFILE=f"{REF}/data/Wuerfel.GEO", BNDFILE=f"{REF}/data/Wuerfel.PRM",
+              MESH_TYPE=0, OUTFILE="ref.out", LOGFILE="ref.log", MEASURE_ERRORS=0,
+              WRITE_VTK=0, WRITE_PS=0, RE_NR=1.0, USE_ISOPARAMETRIC=0)
+    kv.update(SOLVER)
+    kv.update(overrides)
+    with open(path, "w") as f:
+        for k, v in kv.items():
+            f.write(f"{k}: {v}\n")
+
+
+def run_case(name):
+    mode, ov = CASES[name]
+    with tempfile.TemporaryDirectory() as td:
+        dat = os.path.join(td, "case.dat")
+        write_dat(dat, ov)
+        p = subprocess.run([EXE, dat, td, mode], cwd=td, capture_output=True, text=True)
+        if p.returncode:
+            print(p.stdout[-3000:], p.stderr[-3000:])
+            raise SystemExit(f"{name}: reference run failed ({p.returncode})")
+        out = {"params": np.array(repr(ov)), "box": np.array(BOX)}
+        levels = [tuple(map(int, re.findall(r"-?\d+", l)[:8]))
+                  for l in p.stdout.splitlines() if l.startswith("LEVEL ")]
+        for lev, nc, nbf, ndof, act, hb, ndir, nnz in levels:
+            pre = f"L{lev}_"
+            rd = lambda f, dt: np.fromfile(os.path.join(td, pre + f), dtype=dt)
+            out[pre + "meta"] = np.array([nc, nbf, ndof, act, hb, ndir, nnz], np.int64)
+            out[pre + "cells"] = rd("cells.f64", np.float64).reshape(nc, 8, 3)
+            out[pre + "gn"] = rd("gn.i32", np.int32).reshape(nc, nbf)
+            out[pre + "rowptr"] = rd("rowptr.i32", np.int32)
+            out[pre + "kcol"] = rd("kcol.i32", np.int32)
+            out[pre + "entries"] = rd("entries.f64", np.float64)
+            out[pre + "rhs"] = rd("rhs.f64", np.float64)
+        if mode == "cd":
+            for f in ("vcycle_in", "vcycle_out", "sol"):
+                out[f] = np.fromfile(os.path.join(td, f + ".f64"), dtype=np.float64)
+            m = re.search(r"SOLVE its (\d+) r0 (\S+) rfinal (\S+)", p.stdout)
+            out["its"] = np.array(int(m.group(1)))
+            out["r0"] = np.array(float(m.group(2)))
+            out["rfinal"] = np.array(float(m.group(3)))
+            hist = [float(x) for x in re.findall(r"fgmres iteration \d+ (\S+)", p.stdout)]
+            out["res_hist"] = np.array(hist)
+            t = re.search(r"TIME space (\S+) struct (\S+) assemble (\S+) solve (\S+)", p.stdout)
+            out["ref_times"] = np.array([float(x) for x in t.groups()])
+        else:
+            steps = int(ov["P8"])
+            for k in range(steps + 1):
+                out[f"u{k}"] = np.fromfile(os.path.join(td, f"tcd_u{k}.f64"), dtype=np.float64)
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
+        print(f"{name}: {[l[:8] for l in levels]}", "its" in out and int(out["its"]) or "")
+
+
+def main():
+    if not os.path.exists(EXE):
+        raise SystemExit("build oracle/_ref first: python oracle/build_ref.py")
+    for name in (sys.argv[1:] or CASES):
+        run_case(name)
+
+
+if __name__ == "__main__":
+    main()

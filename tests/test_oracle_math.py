@@ -1,0 +1,195 @@
+"""Pins the oracle where the reference's fixtures leave it open, with what the
+mathematics fixes: closed forms, invariants and textbook special cases.  Each
+check is chosen so that a plausible mistake (dropped term, wrong sign or
+index, transposed operand) fails it."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+
+def test_nnz_closed_forms():
+    # all-Dirichlet unit cube, n^3 cells: Q1 (3(n-1))^3 + (n+1)^3 - (n-1)^3,
+    # Q2 (8n-5)^3 + (2n+1)^3 - (2n-1)^3 (active rows hold every DOF of the
+    # surrounding cells, Dirichlet rows the diagonal only)
+    for order, k0, levels in ((1, 1, 3), (2, 1, 2)):
+        H = po.Hierarchy(order, 1, k0, levels, box=po.UNIT_BOX)
+        for l in range(levels):
+            n = 2 ** (k0 + l)
+            if order == 1:
+                expect = (3 * (n - 1)) ** 3 + (n + 1) ** 3 - (n - 1) ** 3
+                ndof = (n + 1) ** 3
+            else:
+                expect = (8 * n - 5) ** 3 + (2 * n + 1) ** 3 - (2 * n - 1) ** 3
+                ndof = (2 * n + 1) ** 3
+            i = H.info(l)
+            assert i["nnz"] == expect and i["ndof"] == ndof
+
+
+def test_macro_mesh_counts():
+    # m^3 macro cells (non-power-of-two n) keep the same closed forms
+    H = po.Hierarchy(1, 3, 1, 2, box=po.UNIT_BOX)
+    for l, n in enumerate((6, 12)):
+        i = H.info(l)
+        assert i["ndof"] == (n + 1) ** 3
+        assert i["nnz"] == (3 * (n - 1)) ** 3 + (n + 1) ** 3 - (n - 1) ** 3
+        assert i["nact"] == (n - 1) ** 3
+
+
+def brute_hconv(h, b):
+    # 8x8 system for the trilinear basis on the brick (ConvDiff.C:411-469),
+    # solved with numpy
+    xs = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float) * h
+    A = np.array([[1, x, y, z, x * y, x * z, y * z, x * y * z] for x, y, z in xs])
+    coef = np.linalg.solve(A, np.eye(8))  # column i: coefficients of basis function i
+    sx, sy, sz = xs.mean(0)
+    den = 0
+    for i in range(8):
+        c = coef[:, i]
+        g = np.array([c[1] + c[4] * sy + c[5] * sz + c[7] * sy * sz,
+                      c[2] + c[4] * sx + c[6] * sz + c[7] * sx * sz,
+                      c[3] + c[5] * sx + c[6] * sy + c[7] * sx * sy])
+        den += abs(np.dot(b, g))
+    return 2 * np.linalg.norm(b) / den
+
+
+@pytest.mark.parametrize("b", [(1, 0.5, 0.25), (-0.3, 0.7, 0.4), (1, 0, 0), (0.2, -0.9, 0.0)])
+@pytest.mark.parametrize("h", [(0.25, 0.25, 0.25), (0.125, 0.04325, 0.04325)])
+def test_hconv_matches_bilinear_fit(h, b):
+    assert abs(po.hconv_brick(h, b) - brute_hconv(np.array(h), np.array(b))) < 1e-13 * brute_hconv(np.array(h), np.array(b))
+
+
+def test_hconv_known_value():
+    # b = (1, 0.5, 0.25) on a cube of side h: h_conv = 8 h |b| / sum |+-1 +-.5 +-.25| = 1.1456 h
+    h = 0.1
+    v = po.hconv_brick((h, h, h), (1, 0.5, 0.25))
+    expect = 8 * h * np.linalg.norm([1, 0.5, 0.25]) / (2 * (1.75 + 1.25 + 0.75 + 0.25))
+    assert abs(v - expect) < 1e-15 and abs(v / h - 1.1456) < 1e-4
+
+
+def test_sdfem_delta_limits():
+    b = np.array([1.0, 0.5, 0.25])
+    nb = np.linalg.norm(b)
+    h = 0.01
+    for order in (1, 2):
+        nu = 1.0 / order
+        # convection dominated: coth -> 1, delta -> nu h / (2|b|) (1 - 1/alpha)
+        eps = 1e-9
+        alpha = nu * nb * h / (2 * eps)
+        assert abs(po.sdfem_delta(h, eps, b, order) - nu * h * (1 - 1 / alpha) / (2 * nb)) < 1e-15
+        # diffusion dominated: coth(a) - 1/a ~ a/3 -> delta ~ nu^2 h^2 / (12 eps)
+        eps = 10.0
+        d = po.sdfem_delta(h, eps, b, order)
+        assert abs(d / (nu * nu * h * h / (12 * eps)) - 1) < 1e-6
+    assert po.sdfem_delta(h, 1e-3, np.zeros(3), 1) == 0.0
+
+
+def test_prolongation_weights_and_transpose():
+    # nodal interpolation: a coarse function that is linear (Q1) or quadratic
+    # (Q2) per axis is reproduced exactly on the fine grid; restriction is
+    # the transpose of prolongation
+    rng = np.random.default_rng(0)
+    for order in (1, 2):
+        H = po.Hierarchy(order, 1, 1, 2, box=po.UNIT_BOX)
+        ic, fi = H.info(0), H.info(1)
+        # coordinates of the DOFs from the cell vertex lists and DOF maps
+        def coords(l):
+            cells, gn = H.cells(l), H.gn(l)
+            n1 = order + 1
+            X = np.zeros((H.info(l)["ndof"], 3))
+            for c in range(cells.shape[0]):
+                v = cells[c]
+                o, ex, ey, ez = v[0], v[1] - v[0], v[3] - v[0], v[4] - v[0]
+                for b in range(n1 ** 3):
+                    i, j, k = b % n1, (b // n1) % n1, b // (n1 * n1)
+                    X[gn[c, b]] = o + ex * i / order + ey * j / order + ez * k / order
+            return X
+        Xc, Xf = coords(0), coords(1)
+        f = lambda X: 1 + 2 * X[:, 0] - X[:, 1] + 0.5 * X[:, 2] + (order == 2) * (X[:, 0] ** 2 - X[:, 1] * X[:, 2])
+        pf = H.prolongate(1, f(Xc))
+        assert np.abs(pf - f(Xf)).max() < 1e-14
+        u = rng.standard_normal(ic["ndof"])
+        v = rng.standard_normal(fi["ndof"])
+        assert abs(np.dot(H.prolongate(1, u), v) - np.dot(u, H.restrict(1, v))) < 1e-12 * np.abs(v).sum()
+
+
+def test_vcycle_is_linear():
+    H = po.Hierarchy(1, 1, 1, 3, box=po.UNIT_BOX)
+    H.assemble(po.coeffs(eps=0.01, b=(1, 0.5, 0.25), f_val=1.0, supg=1))
+    sv = po.solver(coarse_smoother=17)
+    rng = np.random.default_rng(1)
+    n, na = H.info(2)["ndof"], H.info(2)["nact"]
+    r1, r2 = rng.standard_normal(n), rng.standard_normal(n)
+    r1[na:] = 0
+    r2[na:] = 0
+    a = 0.37
+    lhs = H.vcycle(sv, a * r1 + r2)
+    rhs = a * H.vcycle(sv, r1) + H.vcycle(sv, r2)
+    assert np.abs(lhs - rhs).max() < 1e-12 * np.abs(lhs).max()
+
+
+def test_exact_coarse_solve():
+    # one level only: the "V-cycle" is the exact coarse solve, so V(r) = A^-1 r
+    H = po.Hierarchy(2, 1, 1, 1, box=po.UNIT_BOX)
+    H.assemble(po.coeffs(eps=0.05, b=(1, -0.5, 0.3), c=0.2, f_val=1.0, supg=1))
+    sv = po.solver(coarse_smoother=17)
+    r = np.random.default_rng(2).standard_normal(H.info(0)["ndof"])
+    z = H.vcycle(sv, r)
+    rp, kc, e = H.csr(0)
+    A = np.zeros((len(r), len(r)))
+    for i in range(len(r)):
+        A[i, kc[rp[i]:rp[i + 1]]] = e[rp[i]:rp[i + 1]]
+    assert np.linalg.norm(A @ z - r) < 1e-13 * np.linalg.norm(r)
+
+
+def test_fgmres_residual_monotone_and_true_residual():
+    H = po.Hierarchy(2, 1, 1, 2, box=po.UNIT_BOX)
+    H.assemble(po.coeffs(eps=1e-3, b=(1, 0.5, 0.25), f_val=1.0, supg=1))
+    b = H.rhs(1)
+    _, r0 = H.defect(1, np.zeros_like(b), b)
+    sv = po.solver(coarse_smoother=17, res_norm_min=1e-10 * r0, red_factor=0.0)
+    x, its, hist = H.fgmres(sv, b)
+    assert its > 0 and np.all(np.diff(hist) <= 1e-15 * r0)
+    _, rf = H.defect(1, x, b)
+    assert rf <= 1.01e-10 * r0
+
+
+def test_poisson_discretisation_converges_h2():
+    # configuration 1 (eps = 1, u = sin sin sin) on the unit cube: the nodal
+    # error of the FE solution falls like h^2 (a dropped or mis-scaled
+    # stiffness or load term breaks this)
+    errs = []
+    for k in (2, 3, 4):
+        H = po.Hierarchy(1, 1, 1, k, box=po.UNIT_BOX)
+        H.assemble(po.coeffs(eps=1.0, f_kind=1))
+        l = k - 1
+        b = H.rhs(l)
+        _, r0 = H.defect(l, np.zeros_like(b), b)
+        x, its, _ = H.fgmres(po.solver(coarse_smoother=17, res_norm_min=1e-12 * r0, red_factor=0), b)
+        cells, gn = H.cells(l), H.gn(l)
+        X = np.zeros((len(b), 3))
+        for c in range(cells.shape[0]):
+            for bb, vi in enumerate((0, 1, 3, 2, 4, 5, 7, 6)):
+                X[gn[c, bb]] = cells[c, vi]
+        u = np.sin(np.pi * X[:, 0]) * np.sin(np.pi * X[:, 1]) * np.sin(np.pi * X[:, 2])
+        errs.append(np.abs(x - u).max())
+    rate = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert np.all(rate > 1.8), (errs, rate)
+
+
+def test_mass_matrix_integrates_volume():
+    # sum of all mass entries = |box| (partition of unity, exact quadrature);
+    # Dirichlet rows are identity rows, so compare the active block via M 1
+    box = (1.0, 0.346, 0.346)
+    for order in (1, 2):
+        H = po.Hierarchy(order, 1, 1, 1, box=box)
+        e, r = H.assemble_level(0, po.coeffs(f_val=1.0), what=1)
+        rp, kc, _ = H.csr(0)
+        na = H.info(0)["nact"]
+        # rhs of f = 1 is the integral of each basis function; their sum over
+        # active and Dirichlet DOFs is the volume, the Dirichlet part is zeroed,
+        # so sum(active rows of M applied to 1) equals sum(active rhs)
+        ones = np.ones(len(r))
+        M1 = np.array([e[rp[i]:rp[i + 1]].sum() for i in range(na)])
+        assert np.abs(M1 - r[:na]).max() < 1e-15
+        assert r[:na].sum() < np.prod(box)
