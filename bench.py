@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Benchmark of the CD3D hot path on one MI355X.
+
+Workload (one "step"): the configuration-2 problem -- unit cube, 64^3 hexahedra,
+Q2 (2 146 689 DOFs), SUPG, eps = 1e-3, b = (1, 0.5, 0.25), c = 0, f = 1,
+Dirichlet u = 0 -- assembled on all 5 multigrid levels (4^3 ... 64^3) and
+solved by GMG-preconditioned FGMRES(20) to a relative residual of 1e-10.
+Inputs (mesh, DOF maps, CSR patterns, transfers, Krylov workspace) are resident
+in HBM before the timed region; each step re-assembles and re-solves.
+
+Solver: V-cycle, damped Jacobi (omega 0.5, 3 pre + 3 post), exact coarse
+solve, FGMRES(20).  (With ParMooN's configuration-1 smoother, omega 0.8 and
+2 + 2 sweeps, the reference itself does not converge on this problem at 32^3.)
+
+value = solves/s of the mixed-precision form (FP32 V-cycle, FP64 FGMRES,
+residual, coarse solve and SUPG tau); the plain FP64 form, p50/p99 latency,
+the fine-level SpMV bandwidth (roofline of the dominant kernel) and the
+reference CPU path (timed on this host) are reported beside it.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "CD3D solves/sec at rel. residual 1e-10, mixed vs FP64; p50/p99 latency; SpMV GB/s"
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = dict(order=2, m=1, k0=2, levels=5, box=(1.0, 1.0, 1.0))
+COEFFS = dict(eps=1e-3, b=(1.0, 0.5, 0.25), c=0.0, f_val=1.0, supg=1)
+SOLVER = dict(omega=0.5, pre=3, post=3, coarse_smoother=17, restart=20, maxit=400, rel_tol=1e-10)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class HipEvents:
+    """hipEvent timing on the library's own stream."""
+
+    def __init__(self):
+        self.h = C.CDLL("libamdhip64.so")
+        self.h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.h.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def elapsed_ms(self, a, b):
+        self.h.hipEventSynchronize(b)
+        ms = C.c_float()
+        self.h.hipEventElapsedTime(C.byref(ms), a, b)
+        return ms.value
+
+
+def spmv_roofline(pm, P, reps=50):
+    """Average duration of the fine-level FP64 CSR SpMV kernel, HIP events on
+    the handle's stream; algorithmic bytes 12 nnz + 20 N (f64 values, int32
+    columns and row pointers, x and y once)."""
+    l = P.levels - 1
+    info = P.info(l)
+    x, y = P.device_ptr(l, 0), P.device_ptr(l, 2)
+    ev = HipEvents()
+    stream = C.c_void_p(P.L.cd3d_stream(P.h))
+    a, b = ev.event(), ev.event()
+    for _ in range(5):
+        P.matvec_dev(l, x, y)
+    ev.h.hipEventRecord(a, stream)
+    for _ in range(reps):
+        P.matvec_dev(l, x, y)
+    ev.h.hipEventRecord(b, stream)
+    ms = ev.elapsed_ms(a, b) / reps
+    nbytes = 12 * info["nnz"] + 20 * info["ndof"]
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4),
+                traffic=None, kernel="csr_rows<16,kSpmv,f64> fine level", us_per_launch=round(ms * 1e3, 2),
+                algorithmic_bytes=nbytes, nnz=info["nnz"], n=info["ndof"])
+
+
+def run_steps(pm, P, sp, co, warmup, steps):
+    lat, its, res = [], [], []
+    for k in range(warmup + steps):
+        t = time.perf_counter()
+        P.assemble(co)
+        st = P.solve_device(sp)
+        dt = time.perf_counter() - t
+        if st.status != 0:
+            raise RuntimeError(f"solve status {st.status}")
+        if k >= warmup:
+            lat.append(dt)
+            its.append(st.iterations)
+            res.append(st.rfinal / st.r0)
+        log(f"  step {k} ({'warmup' if k < warmup else 'timed'}): {dt * 1e3:.1f} ms, {st.iterations} its, "
+            f"rel res {st.rfinal / st.r0:.2e}")
+    return np.array(lat), its, res
+
+
+def cpu_baseline(sample_levels=4, timeout=240):
+    """Reference CPU path (oracle/_ref, compiled from the reference's sources)
+    on a bounded sample: the same problem at 32^3 cells (4 levels, 274 625
+    DOFs, 1/8 of the workload), one thread; solves/s scaled to the workload
+    by the DOF ratio."""
+    exe = os.path.join(REPO, "oracle", "_ref", "cd3d_ref")
+    if not os.path.exists(exe):
+        return None
+    data = os.path.join(REPO, "oracle", "data")
+    kv = dict(GEOFILE=os.path.join(data, "unitcube.GEO"), BNDFILE=os.path.join(data, "unitcube.PRM"), MESH_TYPE=0,
+              OUTFILE="ref.out", LOGFILE="ref.log", MEASURE_ERRORS=0, WRITE_VTK=0, WRITE_PS=0,
+              ANSATZ_ORDER=2, LEVELS=sample_levels, UNIFORM_STEPS=2, DISCTYPE=2, CELL_MEASURE=4, SDFEM_TYPE=2,
+              DELTA0=1.0, P0=1e-3, P1=1.0, P2=0.5, P3=0.25, P4=0, P5=0, P6=1.0,
+              SOLVER_TYPE=1, SC_MG_TYPE_SCALAR=0, SC_SOLVER_SCALAR=16, SC_PRECONDITIONER_SCALAR=5,
+              SC_GMRES_RESTART=20, SC_LIN_MAXIT_SCALAR=400, SC_LIN_RED_FACTOR_SCALAR=1e-10,
+              SC_LIN_RES_NORM_MIN_SCALAR=1e-30, SC_MG_CYCLE_SCALAR=1, SC_SMOOTHER_SCALAR=1,
+              SC_PRE_SMOOTH_SCALAR=3, SC_POST_SMOOTH_SCALAR=3, SC_SMOOTH_DAMP_FACTOR_SCALAR=0.5,
+              SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=0.5, SC_COARSE_SMOOTHER_SCALAR=3, SC_COARSE_MAXIT_SCALAR=10,
+              SC_COARSE_RED_FACTOR_SCALAR=0.1, SC_GMG_DAMP_FACTOR_SCALAR=1.0, SC_STEP_LENGTH_CONTROL_FINE_SCALAR=0,
+              SC_STEP_LENGTH_CONTROL_ALL_SCALAR=0, SC_AMG_PREC_IT=1, SC_MINIT=0, SC_VERBOSE=0,
+              SC_FLEXIBLE_KRYLOV_SPACE_SOLVER=1)
+    with tempfile.TemporaryDirectory() as td:
+        dat = os.path.join(td, "c2.dat")
+        with open(dat, "w") as f:
+            for k, v in kv.items():
+                f.write(f"{k}: {v}\n")
+        t = time.perf_counter()
+        try:
+            p = subprocess.run([exe, dat, "-", "cd"], cwd=td, capture_output=True, text=True, timeout=timeout)
+        except subprocess.TimeoutExpired:
+            return None
+        wall = time.perf_counter() - t
+    if p.returncode != 0:
+        log("reference run failed", p.stderr[-500:])
+        return None
+    tline = [l for l in p.stdout.splitlines() if l.startswith("TIME ")][-1].split()
+    sline = [l for l in p.stdout.splitlines() if l.startswith("SOLVE ")][-1].split()
+    t_asm, t_solve = float(tline[6]), float(tline[8])
+    ndof_sample = (2 * 2 ** (2 + sample_levels - 1) + 1) ** 3
+    ndof_work = (2 * 64 + 1) ** 3
+    per_solve = (t_asm + t_solve) * ndof_work / ndof_sample
+    return dict(value=round(1.0 / per_solve, 6), unit="solves/s", cores=1, kind="reference",
+                sample=f"reference (oracle/_ref, -O2, _SEQ, 1 thread) on the same problem at 32^3 cells "
+                       f"(Q2, 4 levels, {ndof_sample} DOFs): assembly {t_asm:.2f} s + solve {t_solve:.2f} s "
+                       f"({sline[2]} FGMRES its, SSOR coarse smoother: no LAPACK in the build), wall {wall:.1f} s; "
+                       f"scaled by the DOF ratio {ndof_work / ndof_sample:.2f} to the 64^3 workload")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every MI355X box here has one GPU)",
+              file=sys.stderr)
+        return 2
+    import parmoon_cmg_amd as pm
+
+    log("[bench] building the 64^3 Q2 hierarchy (5 levels)")
+    t = time.perf_counter()
+    P = pm.Problem(**WORKLOAD)
+    log(f"[bench] setup {time.perf_counter() - t:.1f} s, device memory {P.device_bytes() / 2**30:.2f} GiB")
+    co = pm.coeffs(**COEFFS)
+    res = {}
+    sols = {}
+    for name, prec in (("fp64", pm.FP64), ("mixed", pm.MIXED)):
+        sp = pm.solver_params(**SOLVER, precision=prec)
+        log(f"[bench] {name}: {args.warmup} warmup + {args.steps} timed steps")
+        lat, its, rr = run_steps(pm, P, sp, co, args.warmup, args.steps)
+        sols[name] = P.solution()
+        res[name] = dict(solves_per_s=args.steps / lat.sum(), ms_per_step=1e3 * lat.mean(),
+                         p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
+                         iterations=int(np.median(its)), max_rel_residual=float(max(rr)))
+    err = float(np.linalg.norm(sols["mixed"] - sols["fp64"]) / np.linalg.norm(sols["fp64"]))
+    log("[bench] SpMV roofline")
+    roof = spmv_roofline(pm, P)
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] reference CPU baseline (bounded sample)")
+        cpu = cpu_baseline()
+    m = res["mixed"]
+    out = {
+        "metric": METRIC, "value": round(m["solves_per_s"], 4), "unit": "solves/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(m["ms_per_step"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 V-cycle / f64 FGMRES",
+        "data": "synthetic (configuration-2 coefficients)",
+        "config": {"workload": "CD3D Q2 SUPG, unit cube 64^3 hexahedra, 2146689 DOFs, 5-level GMG-FGMRES(20) to 1e-10",
+                   "eps": 1e-3, "b": [1.0, 0.5, 0.25], "smoother": "Jacobi omega 0.5, 3+3, V-cycle, exact coarse",
+                   "parallelism": "1 GPU"},
+        "mixed": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in m.items()},
+        "fp64": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res["fp64"].items()},
+        "mixed_vs_fp64_speedup": round(m["solves_per_s"] / res["fp64"]["solves_per_s"], 3),
+        "mixed_solution_rel_err_vs_fp64": err,
+        "p50_ms": round(m["p50_ms"], 3), "p99_ms": round(m["p99_ms"], 3),
+        "spmv_gbs": roof["achieved"],
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    P.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

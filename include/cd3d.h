@@ -1,0 +1,173 @@
+/* cd3d.h -- C ABI of libcd3d_hip.so, the MI355X implementation of ParMooN's
+ * 3D scalar convection-diffusion hot path (CD3D / TCD3D): Q1/Q2 (SUPG)
+ * assembly into CSR, CSR SpMV, the geometric multigrid V-cycle and the
+ * FGMRES iteration that drives it.
+ *
+ * Plain C types only.  Every call returns a cd3d_status; no call exits the
+ * process.  A handle owns one multilevel problem resident in device memory
+ * (mesh, DOF maps, CSR matrices of every level, transfer operators, Krylov
+ * workspace); calls on one handle are serialised by the handle's mutex, calls
+ * on different handles may run concurrently from different threads.
+ *
+ * Pointer arguments named *_host are host memory; *_dev are device memory
+ * (hipMalloc'd, or obtained from cd3d_device_ptr) and are used on the
+ * handle's stream.
+ */
+#ifndef CD3D_H
+#define CD3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  CD3D_OK = 0,
+  CD3D_ERR_ARG = 1,          /* invalid argument (ParMooN: OutPut + exit(4711)) */
+  CD3D_ERR_HIP = 2,          /* HIP runtime error */
+  CD3D_ERR_NOMEM = 3,        /* device or host allocation failed / budget exceeded */
+  CD3D_NOT_CONVERGED = 4,    /* FGMRES reached maxit (ParMooN returns maxit) */
+  CD3D_DIVERGED = 5,         /* res > div_factor * res0 (ParMooN: exit(4711), FgmresIte.C:509) */
+  CD3D_NONFINITE = 6,        /* NaN/Inf in data or residual */
+  CD3D_ERR_STATE = 7         /* call out of order (e.g. solve before assemble) */
+} cd3d_status;
+
+typedef struct cd3d_problem cd3d_problem; /* opaque handle */
+
+/* Mesh hierarchy: box [0,box[0]] x [0,box[1]] x [0,box[2]] split into m^3
+ * macro bricks, refined k0 times for the coarsest level, `levels` levels in
+ * total (level l has n_l = m 2^(k0+l) cells per side).  m = 1 is ParMooN's
+ * data/Wuerfel.GEO refined by RegRefineAll (Domain.C:1037); order 1 (Q1) or
+ * 2 (Q2) conforming elements, Dirichlet boundary on every face. */
+typedef struct {
+  int order;
+  int m;
+  int k0;
+  int levels;
+  double box[3];
+} cd3d_mesh_desc;
+
+/* Problem coefficients (replaces the CoeffFct3D callback, Constants.h:123,
+ * which cannot run on the device): coeff[0] = eps, coeff[1..3] = b,
+ * coeff[4] = c, coeff[5] = f, coeff[6] = |b| at every quadrature point. */
+typedef struct {
+  double eps;
+  double b[3];          /* conv_kind 0: constant convection */
+  double c;             /* reaction */
+  int conv_kind;        /* 0 constant; 1 rotating b = (-(y-rot_c[1]), x-rot_c[0], 0.1 sin(2 pi t)) */
+  double rot_c[2];
+  double pulse_c[3];    /* centre of the Gaussian pulse initial condition (sigma 0.05) */
+  int f_kind;           /* 0 f = f_val; 1 manufactured u = sin(pi x) sin(pi y) sin(pi z) */
+  double f_val;
+  int supg;             /* 0 Galerkin (DISCTYPE 1); 1 SDFEM_TYPE 2 (DISCTYPE 2, CELL_MEASURE 4, DELTA0 1) */
+  double t;             /* time at which the coefficients are evaluated */
+} cd3d_coeffs;
+
+typedef enum { CD3D_FP64 = 0, CD3D_MIXED = 1 } cd3d_precision;
+
+/* Solver parameters: the ParamDB SC_* knobs of the GMG/FGMRES path
+ * (Database.C:648-757, FgmresIte.C:71-85, MultiGrid3D.C). */
+typedef struct {
+  int smoother;         /* SC_SMOOTHER_SCALAR: 1 Jacobi */
+  int pre, post;        /* SC_PRE_SMOOTH_SCALAR, SC_POST_SMOOTH_SCALAR */
+  double omega;         /* SC_SMOOTH_DAMP_FACTOR_SCALAR */
+  int coarse_smoother;  /* SC_COARSE_SMOOTHER_SCALAR: 17 exact (default), 3 SSOR, 1 Jacobi */
+  int coarse_maxit;     /* SC_COARSE_MAXIT_SCALAR */
+  double coarse_red;    /* SC_COARSE_RED_FACTOR_SCALAR */
+  double gmg_damp;      /* SC_GMG_DAMP_FACTOR_SCALAR */
+  int restart;          /* SC_GMRES_RESTART */
+  int maxit;            /* SC_LIN_MAXIT_SCALAR */
+  double res_norm_min;  /* SC_LIN_RES_NORM_MIN_SCALAR (absolute) */
+  double red_factor;    /* SC_LIN_RED_FACTOR_SCALAR (relative to the restart-cycle start) */
+  double div_factor;    /* SC_DIV_FACTOR */
+  double rel_tol;       /* > 0: stop at |b - A x| <= rel_tol |b - A x0| (overrides the two above) */
+  int precision;        /* cd3d_precision */
+  int cycle;            /* SC_MG_CYCLE_SCALAR: 1 V (default), 2 W, 0 F */
+} cd3d_solver_params;
+
+typedef struct {
+  int iterations;
+  int status;           /* cd3d_status of the solve */
+  double r0;            /* |b - A x0| */
+  double rfinal;        /* |b - A x| recomputed in FP64 after the solve */
+  double ms_assemble;
+  double ms_solve;
+} cd3d_stats;
+
+/* defaults = ParMooN defaults of the configuration-1 solver (Jacobi 0.8,
+ * 2+2, V-cycle, exact coarse solve, FGMRES(20), maxit 200, rel_tol 1e-10) */
+void cd3d_default_solver(cd3d_solver_params *p);
+
+/* Build the multilevel problem on the current HIP device:
+ * TDomain::Init + RegRefineAll (Domain.C:818,1037), TFESpace3D (FESpace3D.C:55),
+ * TSquareStructure3D + Sort (SquareStructure3D.C:32, SquareStructure.C:65),
+ * TSquareMatrix3D (SquareMatrix3D.C), TMGLevel3D/TMultiGrid3D::AddLevel
+ * (MultiGrid3D.C:61-68), TFgmresIte ctor (FgmresIte.C:44). */
+cd3d_status cd3d_create(const cd3d_mesh_desc *mesh, cd3d_problem **out);
+cd3d_status cd3d_destroy(cd3d_problem *p);
+const char *cd3d_last_error(const cd3d_problem *p);
+/* bytes of device memory held by the handle */
+size_t cd3d_device_bytes(const cd3d_problem *p);
+
+/* sizes of level l (TFESpace3D::GetN_DegreesOfFreedom / GetActiveBound,
+ * TSquareMatrix::GetN_Entries, TCollection::GetN_Cells) */
+cd3d_status cd3d_level_info(const cd3d_problem *p, int level, int *n_dof, int *n_active, int64_t *nnz,
+                            int *n_cells);
+/* host copies: TFESpace3D::GetGlobalNumbers (n_cells x n_basis),
+ * TSquareMatrix::GetRowPtr/GetKCol/GetEntries, rhs and vertex coordinates
+ * (n_cells x 8 x 3) */
+cd3d_status cd3d_get_dofmap(const cd3d_problem *p, int level, int *gn_host);
+cd3d_status cd3d_get_cells(const cd3d_problem *p, int level, double *xyz_host);
+cd3d_status cd3d_get_csr(cd3d_problem *p, int level, int *rowptr_host, int *kcol_host, double *entries_host);
+cd3d_status cd3d_get_rhs(cd3d_problem *p, int level, double *rhs_host);
+cd3d_status cd3d_set_entries(cd3d_problem *p, int level, const double *entries_host);
+
+/* TSystemCD3D::Assemble (SystemCD3D.C:314): TAssembleMat3D::Assemble3D
+ * (AssembleMat3D.C:230) on every level with BilinearAssemble /
+ * BilinearAssemble_SD (ConvDiff3D.C:61,124); rhs on Dirichlet DOFs = 0. */
+cd3d_status cd3d_assemble(cd3d_problem *p, const cd3d_coeffs *co);
+
+/* MatVect / ScalarDefect on level l (Blas2.C:60,11), host vectors */
+cd3d_status cd3d_matvec(cd3d_problem *p, int level, const double *x_host, double *y_host);
+/* same on device buffers, on the handle's stream (asynchronous) */
+cd3d_status cd3d_matvec_dev(cd3d_problem *p, int level, const double *x_dev, double *y_dev);
+/* Prolongate / DefectRestriction between level l-1 and l (MGComponents3D.C:182,706) */
+cd3d_status cd3d_prolongate(cd3d_problem *p, int level, const double *coarse_host, double *fine_host);
+cd3d_status cd3d_restrict(cd3d_problem *p, int level, const double *fine_host, double *coarse_host);
+
+/* one preconditioner application z = V(r), TMultiGridScaIte::Iterate
+ * (MultiGridScaIte.C:63) with TMultiGrid3D::Cycle (MultiGrid3D.C:423) */
+cd3d_status cd3d_vcycle(cd3d_problem *p, const cd3d_solver_params *sp, const double *r_host, double *z_host);
+
+/* TSystemCD3D::Solve -> TFgmresIte::Iterate (SystemCD3D.C:342, FgmresIte.C:219).
+ * rhs_host NULL: the assembled rhs; sol_host NULL: keep the solution on the
+ * device (read it with cd3d_get_solution).  Initial guess: 0 on active DOFs,
+ * the rhs (Dirichlet values) on Dirichlet DOFs, or sol_host's values when
+ * use_guess != 0. */
+cd3d_status cd3d_solve(cd3d_problem *p, const cd3d_solver_params *sp, const double *rhs_host, double *sol_host,
+                       int use_guess, cd3d_stats *st);
+cd3d_status cd3d_get_solution(cd3d_problem *p, double *sol_host);
+
+/* TCD3D Crank-Nicolson (3DPrograms/TCD3D_ParMooN.C:539-637 over
+ * TSystemTCD3D, SystemTCD3D.C:176-376): the mass matrix and rhs at t0
+ * (AssembleMRhs), then per step AssembleARhs at t+tau, AssembleSystMat,
+ * Solve, RestoreMassMat.  u_host: in = initial state (finest level), out =
+ * final state; states_host (nsteps x N) optional. */
+cd3d_status cd3d_tcd_run(cd3d_problem *p, const cd3d_coeffs *co, const cd3d_solver_params *sp, double t0,
+                         double tau, int nsteps, double *u_host, double *states_host, cd3d_stats *st);
+/* nodal interpolation of the Gaussian pulse (TFEFunction3D::Interpolate, FEFunction3D.C:1446) */
+cd3d_status cd3d_interp_pulse(cd3d_problem *p, int level, const cd3d_coeffs *co, double *u_host);
+
+/* device pointers of level vectors for timing and zero-copy callers:
+ * which 0 = solution, 1 = rhs, 2 = scratch vector, 3 = scratch vector */
+cd3d_status cd3d_device_ptr(cd3d_problem *p, int level, int which, double **dev);
+/* the handle's HIP stream (hipStream_t) */
+void *cd3d_stream(cd3d_problem *p);
+cd3d_status cd3d_synchronize(cd3d_problem *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

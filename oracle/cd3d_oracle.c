@@ -664,10 +664,13 @@ static void smooth(orc_hier *h, level_t *L, const orc_solver *s, int n) {
   }
 }
 
-/* TMultiGrid3D::Cycle (MultiGrid3D.C:423-669), V-cycle, no step-length
- * control; the Defect calls whose result only feeds verbose output are
- * omitted (they change no state that is read afterwards). */
-static void cycle(orc_hier *h, int i, const orc_solver *s) {
+/* TMultiGrid3D::Cycle (MultiGrid3D.C:423-669), no step-length control; the
+ * Defect calls whose result only feeds verbose output are omitted (they
+ * change no state that is read afterwards).  rec[i] is mg_recursions[i]
+ * (SetRecursion, MultiGrid3D.C:690-703): 1 on the level above the coarsest,
+ * SC_MG_CYCLE_SCALAR (V 1, W 2) elsewhere; the F-cycle starts with 2 and
+ * drops to 1 after a level's first coarse-grid correction. */
+static void cycle(orc_hier *h, int i, const orc_solver *s, int *rec) {
   level_t *L = h->lv + i;
   if (i == 0) {
     /* TMultiGrid3D::Smooth, CoarseSmooth (MultiGrid3D.C:197-260) */
@@ -689,7 +692,8 @@ static void cycle(orc_hier *h, int i, const orc_solver *s) {
   orc_restrict(h, i, L->aux0, Cl->rhs);
   memset(Cl->rhs + Cl->nact, 0, sizeof(double) * (Cl->ndof - Cl->nact)); /* CorrectDefect */
   memset(Cl->sol, 0, sizeof(double) * Cl->ndof);                         /* Reset */
-  cycle(h, i - 1, s);
+  for (int r = 0; r < rec[i]; r++) cycle(h, i - 1, s, rec);
+  if (s->cycle < 1) rec[i] = 1;
   orc_prolongate(h, i, Cl->sol, L->aux0);
   memset(L->aux0 + L->nact, 0, sizeof(double) * (L->ndof - L->nact)); /* CorrectNodes */
   for (int k = 0; k < L->ndof; k++) L->sol[k] += s->gmg_damp * L->aux0[k]; /* Update */
@@ -702,7 +706,10 @@ void orc_vcycle(orc_hier *h, const orc_solver *s, const double *r, double *z) {
   memset(L->sol, 0, sizeof(double) * L->ndof);
   memcpy(L->rhs, r, sizeof(double) * L->ndof);
   memcpy(L->sol + L->nact, L->rhs + L->nact, sizeof(double) * (L->ndof - L->nact)); /* SetDirichletNodes */
-  cycle(h, h->L - 1, s);
+  int rec[32];
+  rec[1] = 1;
+  for (int k = 2; k < h->L; k++) rec[k] = s->cycle > 0 ? s->cycle : 2;
+  cycle(h, h->L - 1, s, rec);
   memcpy(z, L->sol, sizeof(double) * L->ndof);
 }
 

@@ -42,6 +42,7 @@ typedef struct {
   double res_norm_min;   /* SC_LIN_RES_NORM_MIN_SCALAR (absolute) */
   double red_factor;     /* SC_LIN_RED_FACTOR_SCALAR (relative to the restart cycle start) */
   double div_factor;     /* SC_DIV_FACTOR */
+  int cycle;             /* SC_MG_CYCLE_SCALAR: 1 V, 2 W, 0 F */
 } orc_solver;
 
 typedef struct orc_hier orc_hier;

@@ -1,0 +1,269 @@
+"""MI355X implementation of ParMooN's CD3D/TCD3D hot path (Q1/Q2 SUPG assembly
+into CSR, CSR SpMV, geometric multigrid V-cycle, FGMRES).
+
+The compute lives in ``libcd3d_hip.so`` (hand-written HIP for gfx950 behind a
+C ABI, ``include/cd3d.h``).  This module is a thin ctypes mirror of that ABI
+for Python callers; it never falls back to a CPU path: if the library is
+missing or no GPU is present, calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcd3d_hip.so")
+
+CD3D_OK, CD3D_ERR_ARG, CD3D_ERR_HIP, CD3D_ERR_NOMEM = 0, 1, 2, 3
+CD3D_NOT_CONVERGED, CD3D_DIVERGED, CD3D_NONFINITE, CD3D_ERR_STATE = 4, 5, 6, 7
+STATUS_NAMES = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_NOMEM", 4: "NOT_CONVERGED", 5: "DIVERGED",
+                6: "NONFINITE", 7: "ERR_STATE"}
+FP64, MIXED = 0, 1
+
+# every entry point declared in include/cd3d.h
+EXPORTS = ["cd3d_default_solver", "cd3d_create", "cd3d_destroy", "cd3d_last_error", "cd3d_device_bytes",
+           "cd3d_level_info", "cd3d_get_dofmap", "cd3d_get_cells", "cd3d_get_csr", "cd3d_get_rhs",
+           "cd3d_set_entries", "cd3d_assemble", "cd3d_matvec", "cd3d_matvec_dev", "cd3d_prolongate",
+           "cd3d_restrict", "cd3d_vcycle", "cd3d_solve", "cd3d_get_solution", "cd3d_tcd_run",
+           "cd3d_interp_pulse", "cd3d_device_ptr", "cd3d_stream", "cd3d_synchronize"]
+
+
+class CD3DError(RuntimeError):
+    def __init__(self, status, msg=""):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {msg}")
+        self.status = status
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("order", C.c_int), ("m", C.c_int), ("k0", C.c_int), ("levels", C.c_int),
+                ("box", C.c_double * 3)]
+
+
+class Coeffs(C.Structure):
+    _fields_ = [("eps", C.c_double), ("b", C.c_double * 3), ("c", C.c_double), ("conv_kind", C.c_int),
+                ("rot_c", C.c_double * 2), ("pulse_c", C.c_double * 3), ("f_kind", C.c_int),
+                ("f_val", C.c_double), ("supg", C.c_int), ("t", C.c_double)]
+
+
+class SolverParams(C.Structure):
+    _fields_ = [("smoother", C.c_int), ("pre", C.c_int), ("post", C.c_int), ("omega", C.c_double),
+                ("coarse_smoother", C.c_int), ("coarse_maxit", C.c_int), ("coarse_red", C.c_double),
+                ("gmg_damp", C.c_double), ("restart", C.c_int), ("maxit", C.c_int),
+                ("res_norm_min", C.c_double), ("red_factor", C.c_double), ("div_factor", C.c_double),
+                ("rel_tol", C.c_double), ("precision", C.c_int), ("cycle", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("status", C.c_int), ("r0", C.c_double), ("rfinal", C.c_double),
+                ("ms_assemble", C.c_double), ("ms_solve", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libcd3d_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run __graft_entry__.build() (hipcc, gfx950)")
+    L = C.CDLL(LIB_PATH)
+    P, I, D, I64 = C.c_void_p, C.c_int, C.c_double, C.c_int64
+    dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+    L.cd3d_default_solver.argtypes = [C.POINTER(SolverParams)]
+    L.cd3d_default_solver.restype = None
+    L.cd3d_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(P)]
+    L.cd3d_destroy.argtypes = [P]
+    L.cd3d_last_error.argtypes = [P]
+    L.cd3d_last_error.restype = C.c_char_p
+    L.cd3d_device_bytes.argtypes = [P]
+    L.cd3d_device_bytes.restype = C.c_size_t
+    L.cd3d_level_info.argtypes = [P, I, ip, ip, C.POINTER(I64), ip]
+    L.cd3d_get_dofmap.argtypes = [P, I, ip]
+    L.cd3d_get_cells.argtypes = [P, I, dp]
+    L.cd3d_get_csr.argtypes = [P, I, ip, ip, dp]
+    L.cd3d_get_rhs.argtypes = [P, I, dp]
+    L.cd3d_set_entries.argtypes = [P, I, dp]
+    L.cd3d_assemble.argtypes = [P, C.POINTER(Coeffs)]
+    L.cd3d_matvec.argtypes = [P, I, dp, dp]
+    L.cd3d_matvec_dev.argtypes = [P, I, C.c_void_p, C.c_void_p]
+    L.cd3d_prolongate.argtypes = [P, I, dp, dp]
+    L.cd3d_restrict.argtypes = [P, I, dp, dp]
+    L.cd3d_vcycle.argtypes = [P, C.POINTER(SolverParams), dp, dp]
+    L.cd3d_solve.argtypes = [P, C.POINTER(SolverParams), dp, dp, I, C.POINTER(Stats)]
+    L.cd3d_get_solution.argtypes = [P, dp]
+    L.cd3d_tcd_run.argtypes = [P, C.POINTER(Coeffs), C.POINTER(SolverParams), D, D, I, dp, dp, C.POINTER(Stats)]
+    L.cd3d_interp_pulse.argtypes = [P, I, C.POINTER(Coeffs), dp]
+    L.cd3d_device_ptr.argtypes = [P, I, I, C.POINTER(C.c_void_p)]
+    L.cd3d_stream.argtypes = [P]
+    L.cd3d_stream.restype = C.c_void_p
+    L.cd3d_synchronize.argtypes = [P]
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def coeffs(eps=1.0, b=(0, 0, 0), c=0.0, conv_kind=0, rot_c=(0.5, 0.5), pulse_c=(0.3, 0.5, 0.5), f_kind=0,
+           f_val=1.0, supg=0, t=0.0):
+    return Coeffs(eps, (C.c_double * 3)(*b), c, conv_kind, (C.c_double * 2)(*rot_c), (C.c_double * 3)(*pulse_c),
+                  f_kind, f_val, supg, t)
+
+
+def solver_params(**kw):
+    sp = SolverParams()
+    lib().cd3d_default_solver(C.byref(sp))
+    for k, v in kw.items():
+        if not hasattr(sp, k):
+            raise KeyError(k)
+        setattr(sp, k, v)
+    return sp
+
+
+class Problem:
+    """One multilevel CD3D problem resident on the GPU (a cd3d_problem handle).
+
+    Mirrors the reference's TSystemCD3D flow: construct (spaces, structures,
+    multigrid levels), assemble(coeffs), solve(params)."""
+
+    def __init__(self, order, m=1, k0=1, levels=2, box=(1.0, 1.0, 1.0)):
+        self.L = lib()
+        self.order, self.levels = order, levels
+        self.nb = (order + 1) ** 3
+        md = MeshDesc(order, m, k0, levels, (C.c_double * 3)(*box))
+        h = C.c_void_p()
+        st = self.L.cd3d_create(C.byref(md), C.byref(h))
+        if st != CD3D_OK:
+            raise CD3DError(st, "cd3d_create failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cd3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, st, what):
+        if st != CD3D_OK:
+            raise CD3DError(st, f"{what}: {self.L.cd3d_last_error(self.h).decode()}")
+
+    def info(self, l):
+        n, na, nc, nnz = C.c_int(), C.c_int(), C.c_int(), C.c_int64()
+        self._chk(self.L.cd3d_level_info(self.h, l, C.byref(n), C.byref(na), C.byref(nnz), C.byref(nc)), "info")
+        return dict(ndof=n.value, nact=na.value, nnz=nnz.value, ncell=nc.value)
+
+    def device_bytes(self):
+        return self.L.cd3d_device_bytes(self.h)
+
+    def gn(self, l):
+        i = self.info(l)
+        a = np.zeros((i["ncell"], self.nb), dtype=np.int32)
+        self._chk(self.L.cd3d_get_dofmap(self.h, l, _ip(a)), "get_dofmap")
+        return a
+
+    def cells(self, l):
+        i = self.info(l)
+        a = np.zeros((i["ncell"], 8, 3))
+        self._chk(self.L.cd3d_get_cells(self.h, l, _dp(a)), "get_cells")
+        return a
+
+    def csr(self, l):
+        i = self.info(l)
+        rp = np.zeros(i["ndof"] + 1, dtype=np.int32)
+        kc = np.zeros(i["nnz"], dtype=np.int32)
+        e = np.zeros(i["nnz"])
+        self._chk(self.L.cd3d_get_csr(self.h, l, _ip(rp), _ip(kc), _dp(e)), "get_csr")
+        return rp, kc, e
+
+    def rhs(self, l):
+        a = np.zeros(self.info(l)["ndof"])
+        self._chk(self.L.cd3d_get_rhs(self.h, l, _dp(a)), "get_rhs")
+        return a
+
+    def set_entries(self, l, e):
+        e = np.ascontiguousarray(e, dtype=np.float64)
+        self._chk(self.L.cd3d_set_entries(self.h, l, _dp(e)), "set_entries")
+
+    def assemble(self, co):
+        self._chk(self.L.cd3d_assemble(self.h, C.byref(co)), "assemble")
+
+    def matvec(self, l, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        self._chk(self.L.cd3d_matvec(self.h, l, _dp(x), _dp(y)), "matvec")
+        return y
+
+    def prolongate(self, l, coarse):
+        coarse = np.ascontiguousarray(coarse, dtype=np.float64)
+        fine = np.zeros(self.info(l)["ndof"])
+        self._chk(self.L.cd3d_prolongate(self.h, l, _dp(coarse), _dp(fine)), "prolongate")
+        return fine
+
+    def restrict(self, l, fine):
+        fine = np.ascontiguousarray(fine, dtype=np.float64)
+        coarse = np.zeros(self.info(l - 1)["ndof"])
+        self._chk(self.L.cd3d_restrict(self.h, l, _dp(fine), _dp(coarse)), "restrict")
+        return coarse
+
+    def vcycle(self, sp, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.zeros_like(r)
+        self._chk(self.L.cd3d_vcycle(self.h, C.byref(sp), _dp(r), _dp(z)), "vcycle")
+        return z
+
+    def solve(self, sp, rhs=None, x0=None, check=True):
+        """FGMRES + V-cycle; returns (solution, Stats)."""
+        n = self.info(self.levels - 1)["ndof"]
+        x = np.zeros(n) if x0 is None else np.array(x0, dtype=np.float64)
+        st = Stats()
+        r = None if rhs is None else np.ascontiguousarray(rhs, dtype=np.float64)
+        rc = self.L.cd3d_solve(self.h, C.byref(sp), None if r is None else _dp(r), _dp(x), int(x0 is not None),
+                               C.byref(st))
+        if check:
+            self._chk(rc, "solve")
+        return x, st
+
+    def solve_device(self, sp):
+        """solve with rhs and solution kept on the device (timing path)"""
+        st = Stats()
+        rc = self.L.cd3d_solve(self.h, C.byref(sp), None, None, 0, C.byref(st))
+        self._chk(rc, "solve")
+        return st
+
+    def solution(self):
+        x = np.zeros(self.info(self.levels - 1)["ndof"])
+        self._chk(self.L.cd3d_get_solution(self.h, _dp(x)), "get_solution")
+        return x
+
+    def interp_pulse(self, l, co):
+        u = np.zeros(self.info(l)["ndof"])
+        self._chk(self.L.cd3d_interp_pulse(self.h, l, C.byref(co), _dp(u)), "interp_pulse")
+        return u
+
+    def tcd_run(self, co, sp, t0, tau, nsteps, u0):
+        u = np.array(u0, dtype=np.float64)
+        states = np.zeros((nsteps, len(u)))
+        st = Stats()
+        self._chk(self.L.cd3d_tcd_run(self.h, C.byref(co), C.byref(sp), t0, tau, nsteps, _dp(u), _dp(states),
+                                      C.byref(st)), "tcd_run")
+        return u, states, st
+
+    def device_ptr(self, l, which):
+        p = C.c_void_p()
+        self._chk(self.L.cd3d_device_ptr(self.h, l, which, C.byref(p)), "device_ptr")
+        return p.value
+
+    def matvec_dev(self, l, x_dev, y_dev):
+        self._chk(self.L.cd3d_matvec_dev(self.h, l, C.c_void_p(x_dev), C.c_void_p(y_dev)), "matvec_dev")
+
+    def synchronize(self):
+        self._chk(self.L.cd3d_synchronize(self.h), "synchronize")

@@ -1,0 +1,884 @@
+// C ABI of libcd3d_hip.so and the host drivers behind it: problem setup and
+// upload, device assembly, the multigrid V-cycle (FP64 and mixed FP32) and the
+// flexible GMRES iteration, plus the TCD3D Crank-Nicolson step.
+//
+// The host only sequences kernels on the handle's stream; it reads back one
+// column of the Hessenberg matrix per FGMRES iteration (the Givens rotations
+// and the stopping test run on the host, as in TFgmresIte::Iterate).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <new>
+#include <stdexcept>
+
+#include "cd3d_internal.hpp"
+
+using namespace cd3d;
+
+struct cd3d_problem : public Problem {};
+
+namespace {
+
+struct HipError : std::runtime_error {
+  hipError_t code;
+  HipError(hipError_t c, const char *what) : std::runtime_error(what), code(c) {}
+};
+
+#define HIPCHK(x)                                                                                   \
+  do {                                                                                              \
+    hipError_t e_ = (x);                                                                            \
+    if (e_ != hipSuccess) {                                                                         \
+      char b_[256];                                                                                 \
+      snprintf(b_, sizeof b_, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+      throw HipError(e_, b_);                                                                       \
+    }                                                                                               \
+  } while (0)
+
+struct ArgError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+template <class T>
+T *dalloc(Problem &P, size_t count) {
+  if (count == 0) count = 1;
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, count * sizeof(T));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw std::bad_alloc();
+  }
+  P.allocs.push_back(p);
+  P.dev_bytes += count * sizeof(T);
+  return (T *)p;
+}
+
+template <class T>
+void upload(Problem &P, T *dst, const std::vector<T> &src) {
+  if (!src.empty()) HIPCHK(hipMemcpyAsync(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice, P.stream));
+}
+
+double now_ms() {
+  using namespace std::chrono;
+  return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// frame code of a cell: local axis a -> (global axis, sign) in 3 bits
+int frame_code(const int8_t *E) {
+  int code = 0;
+  for (int a = 0; a < 3; a++)
+    for (int d = 0; d < 3; d++)
+      if (E[3 * a + d] != 0) code |= (d | (E[3 * a + d] < 0 ? 4 : 0)) << (3 * a);
+  return code;
+}
+
+cd3d_status run(cd3d_problem *p, const std::function<void()> &f);
+
+}  // namespace
+
+// ============================================================== setup
+
+static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
+  if (m.order != 1 && m.order != 2) throw ArgError("order must be 1 or 2");
+  if (m.m < 1 || m.k0 < 0 || m.levels < 1 || m.levels > 12) throw ArgError("invalid mesh hierarchy");
+  for (int d = 0; d < 3; d++)
+    if (!(m.box[d] > 0)) throw ArgError("box sides must be positive");
+  int64_t nfine = (int64_t)m.m << (m.k0 + m.levels - 1);
+  if (nfine * m.order + 1 > 1290) throw ArgError("mesh too large (nodes per side > 1290)");
+  P.mesh = m;
+  P.nb = (m.order + 1) * (m.order + 1) * (m.order + 1);
+  HIPCHK(hipGetDevice(&P.device));
+  HIPCHK(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
+  const int L = m.levels;
+  P.hl.resize(L);
+  P.dl.resize(L);
+  for (int l = 0; l < L; l++) build_host_level(P.hl[l], m.order, m.m, m.k0 + l);
+  for (int l = 1; l < L; l++) build_transfer(P.hl[l], P.hl[l - 1], m.order);
+  std::vector<double> tab;
+  basis_tables(m.order, tab);
+  P.tab = dalloc<double>(P, tab.size());
+  upload(P, P.tab, tab);
+  for (int l = 0; l < L; l++) {
+    HostLevel &H = P.hl[l];
+    DevLevel &D = P.dl[l];
+    D.ndof = H.ndof;
+    D.nact = H.nact;
+    D.nnz = H.nnz;
+    D.ncell = H.ncell;
+    for (int d = 0; d < 3; d++) {
+      D.h[d] = m.box[d] / H.n;
+      D.dn[d] = m.box[d] / ((double)m.order * H.n);
+    }
+    D.rowptr = dalloc<int>(P, H.ndof + 1);
+    D.col = dalloc<int>(P, H.nnz);
+    D.diag = dalloc<int>(P, H.ndof);
+    D.val = dalloc<double>(P, H.nnz);
+    D.val32 = dalloc<float>(P, H.nnz);
+    upload(P, D.rowptr, H.rowptr);
+    upload(P, D.col, H.kcol);
+    upload(P, D.diag, H.diag);
+    for (double **v : {&D.sol, &D.rhs, &D.aux, &D.tmp}) {
+      *v = dalloc<double>(P, H.ndof);
+      HIPCHK(hipMemsetAsync(*v, 0, sizeof(double) * H.ndof, P.stream));
+    }
+    for (float **v : {&D.fsol, &D.frhs, &D.faux, &D.ftmp}) *v = dalloc<float>(P, H.ndof);
+    if (l > 0) {
+      D.p_nnz = (int64_t)H.p_col.size();
+      D.r_nnz = (int64_t)H.r_col.size();
+      D.p_rowptr = dalloc<int>(P, H.p_rowptr.size());
+      D.p_col = dalloc<int>(P, H.p_col.size());
+      D.p_val = dalloc<double>(P, H.p_val.size());
+      D.r_rowptr = dalloc<int>(P, H.r_rowptr.size());
+      D.r_col = dalloc<int>(P, H.r_col.size());
+      D.r_val = dalloc<double>(P, H.r_val.size());
+      D.p_val32 = dalloc<float>(P, H.p_val.size());
+      D.r_val32 = dalloc<float>(P, H.r_val.size());
+      upload(P, D.p_rowptr, H.p_rowptr);
+      upload(P, D.p_col, H.p_col);
+      upload(P, D.p_val, H.p_val);
+      upload(P, D.r_rowptr, H.r_rowptr);
+      upload(P, D.r_col, H.r_col);
+      upload(P, D.r_val, H.r_val);
+      k_to_f32(P.stream, D.p_nnz, D.p_val, D.p_val32);
+      k_to_f32(P.stream, D.r_nnz, D.r_val, D.r_val32);
+    }
+    // assembly data
+    std::vector<int4> cells(H.ncell);
+    for (int c = 0; c < H.ncell; c++)
+      cells[c] = make_int4(H.org[3 * c], H.org[3 * c + 1], H.org[3 * c + 2], frame_code(&H.ax[9 * c]));
+    D.cells = dalloc<int4>(P, H.ncell);
+    upload(P, D.cells, cells);
+    std::vector<int> slots;
+    build_slots(H, m.order, slots);
+    D.slots = dalloc<int>(P, slots.size());
+    upload(P, D.slots, slots);
+    D.gn = dalloc<int>(P, H.gn.size());
+    upload(P, D.gn, H.gn);
+    D.colour_cells = dalloc<int>(P, H.colour_cells.size());
+    upload(P, D.colour_cells, H.colour_cells);
+    D.colour_start = H.colour_start;
+    HIPCHK(hipStreamSynchronize(P.stream));  // host vectors above go out of scope
+  }
+  // exact coarse solve storage
+  const int na = P.dl[0].nact;
+  P.cinv = dalloc<double>(P, (size_t)na * na);
+  // Krylov workspace (allocated for restart 20; grown on demand)
+  P.red = dalloc<double>(P, 4 * kRedBlocks);
+  P.dscal = dalloc<double>(P, 256);
+  HIPCHK(hipHostMalloc((void **)&P.hscal, 256 * sizeof(double), hipHostMallocDefault));
+}
+
+static void ensure_krylov(cd3d_problem &P, int restart) {
+  if (restart <= P.krylov_restart) return;
+  const int N = P.dl.back().ndof;
+  P.V = dalloc<double>(P, (size_t)(restart + 1) * N);
+  P.Z = dalloc<double>(P, (size_t)(restart + 1) * N);
+  P.w = dalloc<double>(P, N);
+  P.xs = dalloc<double>(P, N);
+  P.bs = dalloc<double>(P, N);
+  P.krylov_restart = restart;
+}
+
+// ============================================================== assembly
+
+static double sdfem_delta_host(double hconv, double eps, const double *b, int order) {
+  if (fabs(eps) < 1e-20) eps = 1e-20;
+  const double nb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  if (!(nb > 0)) return 0.0;
+  const double nu = 1.0 / order, alpha = nu * nb * hconv / (2 * eps);
+  return nu * hconv * (1 / tanh(alpha) - 1 / alpha) / (2 * nb);
+}
+
+static double hconv_host(const double *h, const double *b) {
+  const double nb = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+  double den = 0;
+  for (int s = 0; s < 8; s++) {
+    const double v = ((s & 1) ? -b[0] : b[0]) / h[0] + ((s & 2) ? -b[1] : b[1]) / h[1] + ((s & 4) ? -b[2] : b[2]) / h[2];
+    den += fabs(v) / 4;
+  }
+  if (den < 1e-10) return sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+  return 2 * nb / den;
+}
+
+static void check_coeffs(const cd3d_coeffs &co) {
+  const double v[] = {co.eps, co.b[0], co.b[1], co.b[2], co.c, co.f_val, co.t, co.rot_c[0], co.rot_c[1]};
+  for (double x : v)
+    if (!std::isfinite(x)) throw ArgError("non-finite coefficient");
+  if (co.conv_kind != 0 && co.conv_kind != 1) throw ArgError("conv_kind must be 0 or 1");
+  if (co.f_kind != 0 && co.f_kind != 1) throw ArgError("f_kind must be 0 or 1");
+}
+
+// assemble level l: what 0 -> conv-diff into `val`, 1 -> mass into `val`;
+// rhs into D.rhs (TAssembleMat3D::Reset + Assemble3D, AssembleMat3D.C:204,230)
+static void assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int what, double *val) {
+  DevLevel &D = P.dl[l];
+  AsmParams ap{};
+  ap.what = what;
+  ap.eps = co.eps;
+  for (int d = 0; d < 3; d++) ap.b[d] = co.b[d];
+  ap.c = co.c;
+  ap.f_val = co.f_val;
+  ap.conv_kind = co.conv_kind;
+  ap.f_kind = co.f_kind;
+  ap.supg = co.supg;
+  ap.order = P.mesh.order;
+  ap.rot_c[0] = co.rot_c[0];
+  ap.rot_c[1] = co.rot_c[1];
+  ap.sin2pit = sin(2 * M_PI * co.t);
+  for (int d = 0; d < 3; d++) {
+    ap.h[d] = D.h[d];
+    ap.dn[d] = D.dn[d];
+  }
+  ap.delta_const = 0;
+  if (co.supg && co.conv_kind == 0)
+    ap.delta_const = sdfem_delta_host(hconv_host(D.h, co.b), co.eps, co.b, P.mesh.order);
+  HIPCHK(hipMemsetAsync(val, 0, sizeof(double) * D.nnz, P.stream));
+  HIPCHK(hipMemsetAsync(D.rhs, 0, sizeof(double) * D.ndof, P.stream));
+  for (int k = 0; k < 8; k++)
+    k_assemble(P.stream, D, P.tab, ap, val, D.rhs, D.colour_start[k], D.colour_start[k + 1] - D.colour_start[k]);
+  k_dirichlet_rows(P.stream, D, val, D.rhs);
+  HIPCHK(hipGetLastError());
+}
+
+static void refresh_coarse_and_f32(cd3d_problem &P) {
+  DevLevel &C = P.dl[0];
+  const int na = C.nact;
+  if (na > 0) {
+    double *scratch = dalloc<double>(P, (size_t)2 * na * na + na);
+    k_invert_active(P.stream, C, scratch, P.cinv);
+    HIPCHK(hipStreamSynchronize(P.stream));
+    HIPCHK(hipFree(scratch));
+    P.allocs.pop_back();
+    P.dev_bytes -= ((size_t)2 * na * na + na) * sizeof(double);
+  }
+  P.cinv_valid = true;
+  for (auto &D : P.dl) k_to_f32(P.stream, D.nnz, D.val, D.val32);
+}
+
+// ============================================================== V-cycle
+
+namespace {
+
+struct VC {
+  cd3d_problem &P;
+  const cd3d_solver_params &sp;
+  int rec[32];  // mg_recursions (TMultiGrid3D::SetRecursion, MultiGrid3D.C:690-703)
+  VC(cd3d_problem &p, const cd3d_solver_params &s) : P(p), sp(s) { reset(); }
+  void reset() {
+    rec[1] = 1;
+    for (int k = 2; k < 32; k++) rec[k] = sp.cycle > 0 ? sp.cycle : 2;
+  }
+};
+
+// coarse level: x starts at zero (zero_start) or holds an iterate
+void coarse64(VC &v, bool zero_start) {
+  DevLevel &C = v.P.dl[0];
+  if (zero_start) HIPCHK(hipMemsetAsync(C.sol, 0, sizeof(double) * C.ndof, v.P.stream));
+  if (v.sp.coarse_smoother == 17) {
+    // SolveExact once: x_A += A_AA^-1 (f - A x)_A, x_D = f_D; the loop of
+    // TMultiGrid3D::Smooth ends after one exact solve (MultiGrid3D.C:205-258)
+    k_copy_range(v.P.stream, C.nact, C.ndof, C.rhs, C.sol);
+    k_defect(v.P.stream, C, C.sol, C.rhs, C.aux);
+    k_gemv_dense(v.P.stream, C.nact, v.P.cinv, C.aux, C.sol);
+  } else {
+    k_coarse_smooth(v.P.stream, C, C.sol, C.rhs, C.aux, v.sp.coarse_smoother == 1 ? 1 : 3, v.sp.omega,
+                    v.sp.coarse_maxit, v.sp.coarse_red);
+  }
+}
+
+// TMultiGrid3D::Cycle (MultiGrid3D.C:423-669), V-cycle, FP64.  On entry the
+// level's rhs is set; zero_start: the iterate is zero and the Dirichlet part
+// of the rhs is zero, so the first Jacobi sweep needs only the diagonal.
+void cycle64(VC &v, int i, bool zero_start) {
+  cd3d_problem &P = v.P;
+  if (i == 0) {
+    coarse64(v, zero_start);
+    return;
+  }
+  DevLevel &L = P.dl[i], &C = P.dl[i - 1];
+  double *x = L.sol, *t = L.tmp;
+  for (int k = 0; k < v.sp.pre; k++) {
+    if (k == 0 && zero_start) k_jacobi_zero(P.stream, L, L.rhs, t, v.sp.omega);
+    else k_jacobi(P.stream, L, x, L.rhs, t, v.sp.omega);
+    std::swap(x, t);
+  }
+  if (v.sp.pre == 0 && zero_start) {
+    HIPCHK(hipMemsetAsync(x, 0, sizeof(double) * L.ndof, P.stream));
+    k_copy_range(P.stream, L.nact, L.ndof, L.rhs, x);
+  }
+  k_defect(P.stream, L, x, L.rhs, L.aux);
+  k_restrict(P.stream, L, C, L.aux, C.rhs);
+  for (int r = 0; r < v.rec[i]; r++) cycle64(v, i - 1, r == 0);
+  if (v.sp.cycle < 1) v.rec[i] = 1;  // F-cycle
+  k_prolong_update(P.stream, L, C, C.sol, x, v.sp.gmg_damp);
+  for (int k = 0; k < v.sp.post; k++) {
+    k_jacobi(P.stream, L, x, L.rhs, t, v.sp.omega);
+    std::swap(x, t);
+  }
+  if (x != L.sol) k_copy(P.stream, L.ndof, x, L.sol);
+}
+
+// mixed precision: FP32 smoothing, transfers and defects with FP32
+// accumulation; the coarsest solve stays FP64
+void coarse32(VC &v, bool zero_start) {
+  DevLevel &C = v.P.dl[0];
+  k_cast_f2d(v.P.stream, C.ndof, C.frhs, C.rhs);
+  if (!zero_start) k_cast_f2d(v.P.stream, C.ndof, C.fsol, C.sol);
+  coarse64(v, zero_start);
+  k_cast_d2f(v.P.stream, C.ndof, C.sol, C.fsol);
+}
+
+void cycle32(VC &v, int i, bool zero_start) {
+  cd3d_problem &P = v.P;
+  if (i == 0) {
+    coarse32(v, zero_start);
+    return;
+  }
+  DevLevel &L = P.dl[i], &C = P.dl[i - 1];
+  float *x = L.fsol, *t = L.ftmp;
+  const float om = (float)v.sp.omega;
+  for (int k = 0; k < v.sp.pre; k++) {
+    if (k == 0 && zero_start) k_jacobi_zero32(P.stream, L, L.frhs, t, om);
+    else k_jacobi32(P.stream, L, x, L.frhs, t, om);
+    std::swap(x, t);
+  }
+  k_defect32(P.stream, L, x, L.frhs, L.faux);
+  k_restrict32(P.stream, L, C, L.faux, C.frhs);
+  for (int r = 0; r < v.rec[i]; r++) cycle32(v, i - 1, r == 0);
+  if (v.sp.cycle < 1) v.rec[i] = 1;  // F-cycle
+  k_prolong_update32(P.stream, L, C, C.fsol, x, (float)v.sp.gmg_damp);
+  for (int k = 0; k < v.sp.post; k++) {
+    k_jacobi32(P.stream, L, x, L.frhs, t, om);
+    std::swap(x, t);
+  }
+  if (x != L.fsol) HIPCHK(hipMemcpyAsync(L.fsol, x, sizeof(float) * L.ndof, hipMemcpyDeviceToDevice, P.stream));
+}
+
+// TMultiGridScaIte::Iterate (MultiGridScaIte.C:63-104): z = V(r), zero start.
+// rhs_dirichlet_zero: r vanishes on Dirichlet DOFs (always true for the
+// Krylov vectors of FGMRES), which lets the first sweep skip the matrix.
+void precondition(VC &v, const double *r, double *z, bool rhs_dirichlet_zero) {
+  cd3d_problem &P = v.P;
+  v.reset();
+  const int top = (int)P.dl.size() - 1;
+  DevLevel &F = P.dl[top];
+  if (v.sp.precision == CD3D_MIXED && top > 0 && rhs_dirichlet_zero) {
+    k_cast_d2f(P.stream, F.ndof, r, F.frhs);
+    cycle32(v, top, true);
+    k_cast_f2d(P.stream, F.ndof, F.fsol, z);
+    return;
+  }
+  k_copy(P.stream, F.ndof, r, F.rhs);
+  if (rhs_dirichlet_zero) {
+    cycle64(v, top, true);
+  } else {
+    // SetDirichletNodes (MultiGrid3D.C:671-688): x = 0, x_D = r_D
+    HIPCHK(hipMemsetAsync(F.sol, 0, sizeof(double) * F.ndof, P.stream));
+    k_copy_range(P.stream, F.nact, F.ndof, F.rhs, F.sol);
+    cycle64(v, top, false);
+  }
+  k_copy(P.stream, F.ndof, F.sol, z);
+}
+
+// GeneratePlaneRotation / ApplyPlaneRotation (FgmresIte.C:168-195)
+void gen_rot(double dx, double dy, double &cs, double &sn) {
+  if (dy == 0.0) { cs = 1.0; sn = 0.0; }
+  else if (fabs(dy) > fabs(dx)) { double t = dx / dy; sn = 1.0 / sqrt(1.0 + t * t); cs = t * sn; }
+  else { double t = dy / dx; cs = 1.0 / sqrt(1.0 + t * t); sn = t * cs; }
+}
+void app_rot(double &dx, double &dy, double cs, double sn) {
+  const double t = cs * dx + sn * dy;
+  dy = -sn * dx + cs * dy;
+  dx = t;
+}
+
+double dev_norm(cd3d_problem &P, int n, const double *a) {
+  k_dot(P.stream, n, a, a, P.red, P.dscal);
+  HIPCHK(hipMemcpyAsync(P.hscal, P.dscal, sizeof(double), hipMemcpyDeviceToHost, P.stream));
+  HIPCHK(hipStreamSynchronize(P.stream));
+  return sqrt(P.hscal[0]);
+}
+
+// TFgmresIte::Iterate, flexible branch (FgmresIte.C:219-595) on the finest
+// level: x (P.xs) and b (P.bs) on the device.  Returns the iteration count.
+int fgmres(cd3d_problem &P, const cd3d_solver_params &sp, double &r0, cd3d_status &status) {
+  DevLevel &F = P.dl.back();
+  const int N = F.ndof;
+  int R = std::min(sp.restart, sp.maxit);
+  if (R <= 0) R = 10;
+  ensure_krylov(P, R);
+  VC v{P, sp};
+  double *V = P.V, *Z = P.Z, *w = P.w;
+  auto Vk = [&](int k) { return V + (int64_t)k * N; };
+  auto Zk = [&](int k) { return Z + (int64_t)k * N; };
+  std::vector<double> H((size_t)(R + 1) * R, 0.0), s(R + 1), cs(R + 1), sn(R + 1);
+  double *pa = P.red, *pb = P.red + kRedBlocks;
+  k_defect(P.stream, F, P.xs, P.bs, Vk(0));
+  double beta = dev_norm(P, N, Vk(0));
+  r0 = beta;
+  if (!std::isfinite(beta)) { status = CD3D_NONFINITE; return 0; }
+  double res_norm_min = sp.res_norm_min, red_factor = sp.red_factor;
+  if (sp.rel_tol > 0) { res_norm_min = sp.rel_tol * beta; red_factor = 0.0; }
+  status = CD3D_OK;
+  if (beta <= res_norm_min) return 0;
+  double res0 = beta;
+  int j = 1;
+  auto update_x = [&](int k) {
+    // back substitution (UpdateGmresIterate, FgmresIte.C:198-216), then x += Z y
+    std::vector<double> y(s.begin(), s.begin() + k + 1);
+    for (int i = k; i >= 0; i--) {
+      y[i] /= H[(size_t)i * R + i];
+      for (int jj = i - 1; jj >= 0; jj--) y[jj] -= H[(size_t)jj * R + i] * y[i];
+    }
+    memcpy(P.hscal + 64, y.data(), sizeof(double) * (k + 1));
+    HIPCHK(hipMemcpyAsync(P.dscal + 64, P.hscal + 64, sizeof(double) * (k + 1), hipMemcpyHostToDevice, P.stream));
+    k_multi_axpy(P.stream, N, k + 1, P.dscal + 64, Z, N, P.xs);
+    HIPCHK(hipStreamSynchronize(P.stream));  // pinned y is reused next time
+  };
+  while (j <= sp.maxit) {
+    k_scale(P.stream, N, 1.0 / beta, Vk(0));  // Dscal(N, 1/beta, v[0])
+    std::fill(s.begin(), s.end(), 0.0);
+    s[0] = beta;
+    int i;
+    for (i = 0; i < R && j <= sp.maxit; i++, j++) {
+      precondition(v, Vk(i), Zk(i), true);
+      k_spmv(P.stream, F, Zk(i), w);
+      k_dot(P.stream, N, w, Vk(0), pa, nullptr);
+      // modified Gram-Schmidt: H[k][i] = <w, v_k>; w -= H[k][i] v_k, fused with the next dot
+      for (int k = 0; k <= i; k++) {
+        double *in = (k % 2 == 0) ? pa : pb, *out = (k % 2 == 0) ? pb : pa;
+        k_axpy_dot(P.stream, N, in, -1.0, Vk(k), w, k < i ? Vk(k + 1) : nullptr, out, P.dscal + k);
+      }
+      double *last = (i % 2 == 0) ? pb : pa;
+      k_scale_copy(P.stream, N, last, P.dscal + i + 1, w, Vk(i + 1));
+      HIPCHK(hipMemcpyAsync(P.hscal, P.dscal, sizeof(double) * (i + 2), hipMemcpyDeviceToHost, P.stream));
+      HIPCHK(hipStreamSynchronize(P.stream));
+      for (int k = 0; k <= i + 1; k++) H[(size_t)k * R + i] = P.hscal[k];
+      if (!std::isfinite(H[(size_t)(i + 1) * R + i])) { status = CD3D_NONFINITE; return j; }
+      for (int k = 0; k < i; k++) app_rot(H[(size_t)k * R + i], H[(size_t)(k + 1) * R + i], cs[k], sn[k]);
+      gen_rot(H[(size_t)i * R + i], H[(size_t)(i + 1) * R + i], cs[i], sn[i]);
+      app_rot(H[(size_t)i * R + i], H[(size_t)(i + 1) * R + i], cs[i], sn[i]);
+      app_rot(s[i], s[i + 1], cs[i], sn[i]);
+      const double res = fabs(s[i + 1]);
+      if (res < res_norm_min || res < red_factor * res0) {
+        update_x(i);
+        return j;
+      }
+      if (res > sp.div_factor * res0) { status = CD3D_DIVERGED; return j; }
+    }
+    update_x(R - 1);
+    k_defect(P.stream, F, P.xs, P.bs, Vk(0));
+    beta = dev_norm(P, N, Vk(0));
+    res0 = beta;
+    if (!std::isfinite(beta)) { status = CD3D_NONFINITE; return j; }
+  }
+  status = CD3D_NOT_CONVERGED;
+  return sp.maxit;
+}
+
+void check_params(const cd3d_solver_params &sp) {
+  if (sp.smoother != 1) throw ArgError("smoother: only 1 (Jacobi) is implemented");
+  if (sp.coarse_smoother != 17 && sp.coarse_smoother != 3 && sp.coarse_smoother != 1)
+    throw ArgError("coarse_smoother must be 17, 3 or 1");
+  if (sp.pre < 0 || sp.post < 0 || sp.restart < 1 || sp.maxit < 1) throw ArgError("invalid solver counts");
+  if (sp.precision != CD3D_FP64 && sp.precision != CD3D_MIXED) throw ArgError("precision must be 0 or 1");
+  if (sp.cycle < 0 || sp.cycle > 2) throw ArgError("cycle must be 0 (F), 1 (V) or 2 (W)");
+  if (!(sp.omega > 0) || !(sp.gmg_damp > 0)) throw ArgError("damping factors must be positive");
+}
+
+cd3d_status run(cd3d_problem *p, const std::function<void()> &f) {
+  if (!p) return CD3D_ERR_ARG;
+  try {
+    f();
+    return CD3D_OK;
+  } catch (const ArgError &e) {
+    p->err = e.what();
+    return CD3D_ERR_ARG;
+  } catch (const HipError &e) {
+    p->err = e.what();
+    return CD3D_ERR_HIP;
+  } catch (const std::bad_alloc &) {
+    p->err = "allocation failed";
+    return CD3D_ERR_NOMEM;
+  } catch (const std::exception &e) {
+    p->err = e.what();
+    return CD3D_ERR_STATE;
+  }
+}
+
+}  // namespace
+
+// ============================================================== C ABI
+
+extern "C" {
+
+void cd3d_default_solver(cd3d_solver_params *p) {
+  if (!p) return;
+  p->smoother = 1;
+  p->pre = 2;
+  p->post = 2;
+  p->omega = 0.8;
+  p->coarse_smoother = 17;
+  p->coarse_maxit = 10;
+  p->coarse_red = 0.1;
+  p->gmg_damp = 1.0;
+  p->restart = 20;
+  p->maxit = 200;
+  p->res_norm_min = 1e-30;
+  p->red_factor = 0.0;
+  p->div_factor = 1e10;
+  p->rel_tol = 1e-10;
+  p->precision = CD3D_FP64;
+  p->cycle = 1;
+}
+
+cd3d_status cd3d_create(const cd3d_mesh_desc *mesh, cd3d_problem **out) {
+  if (!mesh || !out) return CD3D_ERR_ARG;
+  *out = nullptr;
+  cd3d_problem *p = new (std::nothrow) cd3d_problem();
+  if (!p) return CD3D_ERR_NOMEM;
+  cd3d_status st = run(p, [&]() { setup_problem(*p, *mesh); });
+  if (st != CD3D_OK) {
+    fprintf(stderr, "cd3d_create: %s\n", p->err.c_str());
+    cd3d_destroy(p);
+    return st;
+  }
+  *out = p;
+  return CD3D_OK;
+}
+
+cd3d_status cd3d_destroy(cd3d_problem *p) {
+  if (!p) return CD3D_ERR_ARG;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  for (void *a : p->allocs) (void)hipFree(a);
+  if (p->hscal) (void)hipHostFree(p->hscal);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+  return CD3D_OK;
+}
+
+const char *cd3d_last_error(const cd3d_problem *p) { return p ? p->err.c_str() : "null handle"; }
+size_t cd3d_device_bytes(const cd3d_problem *p) { return p ? p->dev_bytes : 0; }
+
+cd3d_status cd3d_level_info(const cd3d_problem *p, int level, int *n_dof, int *n_active, int64_t *nnz, int *n_cells) {
+  if (!p || level < 0 || level >= (int)p->hl.size()) return CD3D_ERR_ARG;
+  const HostLevel &H = p->hl[level];
+  if (n_dof) *n_dof = H.ndof;
+  if (n_active) *n_active = H.nact;
+  if (nnz) *nnz = H.nnz;
+  if (n_cells) *n_cells = H.ncell;
+  return CD3D_OK;
+}
+
+cd3d_status cd3d_get_dofmap(const cd3d_problem *p, int level, int *gn) {
+  if (!p || !gn || level < 0 || level >= (int)p->hl.size()) return CD3D_ERR_ARG;
+  memcpy(gn, p->hl[level].gn.data(), sizeof(int) * p->hl[level].gn.size());
+  return CD3D_OK;
+}
+
+cd3d_status cd3d_get_cells(const cd3d_problem *p, int level, double *xyz) {
+  if (!p || !xyz || level < 0 || level >= (int)p->hl.size()) return CD3D_ERR_ARG;
+  static const int hexv[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  const HostLevel &H = p->hl[level];
+  const int S = p->mesh.order;
+  for (int c = 0; c < H.ncell; c++)
+    for (int v = 0; v < 8; v++)
+      for (int d = 0; d < 3; d++) {
+        const int8_t *E = &H.ax[9 * c];
+        const int X = H.org[3 * c + d] + S * (hexv[v][0] * E[d] + hexv[v][1] * E[3 + d] + hexv[v][2] * E[6 + d]);
+        xyz[24 * c + 3 * v + d] = X * (p->mesh.box[d] / ((double)S * H.n));
+      }
+  return CD3D_OK;
+}
+
+cd3d_status cd3d_get_csr(cd3d_problem *p, int level, int *rowptr, int *kcol, double *entries) {
+  if (!p || level < 0 || level >= (int)p->hl.size()) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  return run(p, [&]() {
+    const HostLevel &H = p->hl[level];
+    if (rowptr) memcpy(rowptr, H.rowptr.data(), sizeof(int) * (H.ndof + 1));
+    if (kcol) memcpy(kcol, H.kcol.data(), sizeof(int) * H.nnz);
+    if (entries) {
+      HIPCHK(hipMemcpyAsync(entries, p->dl[level].val, sizeof(double) * H.nnz, hipMemcpyDeviceToHost, p->stream));
+      HIPCHK(hipStreamSynchronize(p->stream));
+    }
+  });
+}
+
+cd3d_status cd3d_get_rhs(cd3d_problem *p, int level, double *rhs) {
+  if (!p || !rhs || level < 0 || level >= (int)p->hl.size()) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  return run(p, [&]() {
+    HIPCHK(hipMemcpyAsync(rhs, p->dl[level].rhs, sizeof(double) * p->dl[level].ndof, hipMemcpyDeviceToHost,
+                          p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+  });
+}
+
+cd3d_status cd3d_set_entries(cd3d_problem *p, int level, const double *entries) {
+  if (!p || !entries || level < 0 || level >= (int)p->hl.size()) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  return run(p, [&]() {
+    HIPCHK(hipMemcpyAsync(p->dl[level].val, entries, sizeof(double) * p->dl[level].nnz, hipMemcpyHostToDevice,
+                          p->stream));
+    refresh_coarse_and_f32(*p);
+    HIPCHK(hipStreamSynchronize(p->stream));
+  });
+}
+
+cd3d_status cd3d_assemble(cd3d_problem *p, const cd3d_coeffs *co) {
+  if (!p || !co) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  return run(p, [&]() {
+    check_coeffs(*co);
+    for (int l = 0; l < (int)p->dl.size(); l++) {
+      DevLevel &D = p->dl[l];
+      assemble_level(*p, l, *co, 0, D.val);
+      // SystemCD3D.C:330: Dirichlet part of the rhs into the level solution
+      k_copy_range(p->stream, D.nact, D.ndof, D.rhs, D.sol);
+    }
+    refresh_coarse_and_f32(*p);
+    HIPCHK(hipStreamSynchronize(p->stream));
+    p->assembled = true;
+  });
+}
+
+static void host_vec_op(cd3d_problem *p, int level, const double *in, double *out, int nin, int nout,
+                        const std::function<void(double *, double *)> &op) {
+  DevLevel &D = p->dl[level];
+  (void)D;
+  double *din = dalloc<double>(*p, nin), *dout = dalloc<double>(*p, nout);
+  HIPCHK(hipMemcpyAsync(din, in, sizeof(double) * nin, hipMemcpyHostToDevice, p->stream));
+  HIPCHK(hipMemsetAsync(dout, 0, sizeof(double) * nout, p->stream));
+  op(din, dout);
+  HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, p->stream));
+  HIPCHK(hipStreamSynchronize(p->stream));
+  for (int k = 0; k < 2; k++) {
+    (void)hipFree(p->allocs.back());
+    p->allocs.pop_back();
+  }
+  p->dev_bytes -= sizeof(double) * (nin + nout);
+}
+
+cd3d_status cd3d_matvec(cd3d_problem *p, int level, const double *x, double *y) {
+  if (!p || !x || !y || level < 0 || level >= (int)p->dl.size()) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  return run(p, [&]() {
+    const int n = p->dl[level].ndof;
+    host_vec_op(p, level, x, y, n, n, [&](double *a, double *b) { k_spmv(p->stream, p->dl[level], a, b); });
+  });
+}
+
+cd3d_status cd3d_matvec_dev(cd3d_problem *p, int level, const double *x, double *y) {
+  if (!p || !x || !y || level < 0 || level >= (int)p->dl.size()) return CD3D_ERR_ARG;
+  return run(p, [&]() {
+    k_spmv(p->stream, p->dl[level], x, y);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+cd3d_status cd3d_prolongate(cd3d_problem *p, int level, const double *coarse, double *fine) {
+  if (!p || !coarse || !fine || level < 1 || level >= (int)p->dl.size()) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  return run(p, [&]() {
+    host_vec_op(p, level, coarse, fine, p->dl[level - 1].ndof, p->dl[level].ndof, [&](double *a, double *b) {
+      k_prolong(p->stream, p->dl[level], p->dl[level - 1], a, b);
+    });
+  });
+}
+
+cd3d_status cd3d_restrict(cd3d_problem *p, int level, const double *fine, double *coarse) {
+  if (!p || !coarse || !fine || level < 1 || level >= (int)p->dl.size()) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  return run(p, [&]() {
+    // plain R = P^T (without zeroing the Dirichlet rows of the result)
+    DevLevel C = p->dl[level - 1];
+    C.nact = C.ndof;
+    host_vec_op(p, level, fine, coarse, p->dl[level].ndof, C.ndof, [&](double *a, double *b) {
+      k_restrict(p->stream, p->dl[level], C, a, b);
+    });
+  });
+}
+
+cd3d_status cd3d_vcycle(cd3d_problem *p, const cd3d_solver_params *sp, const double *r, double *z) {
+  if (!p || !sp || !r || !z) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  return run(p, [&]() {
+    check_params(*sp);
+    if (!p->assembled) throw std::logic_error("assemble before vcycle");
+    const int n = p->dl.back().ndof;
+    VC v{*p, *sp};
+    host_vec_op(p, (int)p->dl.size() - 1, r, z, n, n, [&](double *a, double *b) {
+      // a general rhs may carry Dirichlet data: SetDirichletNodes path
+      std::vector<double> rd(r + p->dl.back().nact, r + n);
+      bool dz = std::all_of(rd.begin(), rd.end(), [](double x) { return x == 0.0; });
+      precondition(v, a, b, dz);
+    });
+  });
+}
+
+cd3d_status cd3d_solve(cd3d_problem *p, const cd3d_solver_params *sp, const double *rhs, double *sol, int use_guess,
+                       cd3d_stats *st) {
+  if (!p || !sp) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  cd3d_status solve_status = CD3D_OK;
+  cd3d_status rc = run(p, [&]() {
+    check_params(*sp);
+    if (!p->assembled) throw std::logic_error("assemble before solve");
+    DevLevel &F = p->dl.back();
+    const int N = F.ndof;
+    ensure_krylov(*p, std::min(sp->restart, sp->maxit));
+    const double t0 = now_ms();
+    if (rhs) HIPCHK(hipMemcpyAsync(p->bs, rhs, sizeof(double) * N, hipMemcpyHostToDevice, p->stream));
+    else k_copy(p->stream, N, F.rhs, p->bs);
+    // initial guess: 0 on active DOFs, Dirichlet values from the rhs
+    // (SystemCD3D.C:330), or the caller's guess with its Dirichlet part reset
+    if (use_guess && sol) HIPCHK(hipMemcpyAsync(p->xs, sol, sizeof(double) * N, hipMemcpyHostToDevice, p->stream));
+    else HIPCHK(hipMemsetAsync(p->xs, 0, sizeof(double) * N, p->stream));
+    k_copy_range(p->stream, F.nact, N, p->bs, p->xs);
+    double r0 = 0;
+    const int its = fgmres(*p, *sp, r0, solve_status);
+    double rf = 0;
+    if (st) {
+      k_defect(p->stream, F, p->xs, p->bs, p->w);
+      rf = dev_norm(*p, N, p->w);
+    }
+    k_copy(p->stream, N, p->xs, F.sol);
+    if (sol) HIPCHK(hipMemcpyAsync(sol, p->xs, sizeof(double) * N, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    if (st) {
+      st->iterations = its;
+      st->status = solve_status;
+      st->r0 = r0;
+      st->rfinal = rf;
+      st->ms_solve = now_ms() - t0;
+    }
+  });
+  return rc != CD3D_OK ? rc : solve_status;
+}
+
+cd3d_status cd3d_get_solution(cd3d_problem *p, double *sol) {
+  if (!p || !sol) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  return run(p, [&]() {
+    DevLevel &F = p->dl.back();
+    HIPCHK(hipMemcpyAsync(sol, F.sol, sizeof(double) * F.ndof, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+  });
+}
+
+cd3d_status cd3d_interp_pulse(cd3d_problem *p, int level, const cd3d_coeffs *co, double *u) {
+  if (!p || !co || !u || level < 0 || level >= (int)p->hl.size()) return CD3D_ERR_ARG;
+  const HostLevel &H = p->hl[level];
+  const double *pc = co->pulse_c;
+  for (int d = 0; d < H.ndof; d++) {
+    const int node = H.dof_node[d];
+    const double den = (double)p->mesh.order * H.n;
+    const double x = (node % H.nn) * (p->mesh.box[0] / den), y = ((node / H.nn) % H.nn) * (p->mesh.box[1] / den),
+                 z = (node / (H.nn * H.nn)) * (p->mesh.box[2] / den);
+    const double r2 = (x - pc[0]) * (x - pc[0]) + (y - pc[1]) * (y - pc[1]) + (z - pc[2]) * (z - pc[2]);
+    u[d] = exp(-r2 / (2 * 0.05 * 0.05));
+  }
+  return CD3D_OK;
+}
+
+cd3d_status cd3d_tcd_run(cd3d_problem *p, const cd3d_coeffs *co0, const cd3d_solver_params *sp, double t0, double tau,
+                         int nsteps, double *u, double *states, cd3d_stats *st) {
+  if (!p || !co0 || !sp || !u || nsteps < 0 || !(tau > 0)) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  cd3d_status solve_status = CD3D_OK;
+  cd3d_status rc = run(p, [&]() {
+    check_params(*sp);
+    check_coeffs(*co0);
+    if (co0->supg) throw ArgError("TCD3D is Galerkin only, as TSystemTCD3D::Init (SystemTCD3D.C:114-131)");
+    cd3d_coeffs co = *co0;
+    const int Lf = (int)p->dl.size() - 1;
+    DevLevel &F = p->dl[Lf];
+    const int N = F.ndof, NA = F.nact;
+    const double th1 = 0.5, th2 = 0.5, th3 = 0.5, th4 = 0.5;  // Crank-Nicolson (TimeDiscRout.C)
+    ensure_krylov(*p, std::min(sp->restart, sp->maxit));
+    for (auto &D : p->dl)
+      if (!D.aconv) D.aconv = dalloc<double>(*p, D.nnz);
+    double *oldrhs = dalloc<double>(*p, N), *oldsol = dalloc<double>(*p, N), *def = dalloc<double>(*p, N);
+    const double tstart = now_ms();
+    // AssembleMRhs at t0 on every level (SystemTCD3D.C:176-202)
+    co.t = t0;
+    for (int l = 0; l <= Lf; l++) assemble_level(*p, l, co, 1, p->dl[l].val);
+    HIPCHK(hipMemcpyAsync(p->xs, u, sizeof(double) * N, hipMemcpyHostToDevice, p->stream));
+    k_copy_range(p->stream, NA, N, F.rhs, p->xs);
+    k_copy(p->stream, N, F.rhs, oldrhs);
+    double t = t0;
+    int total_its = 0;
+    for (int m = 0; m < nsteps; m++) {
+      k_copy(p->stream, N, p->xs, oldsol);
+      t += tau;
+      co.t = t;
+      // AssembleARhs at t_new (SystemTCD3D.C:204-226)
+      for (int l = 0; l <= Lf; l++) assemble_level(*p, l, co, 0, p->dl[l].aconv);
+      k_copy_range(p->stream, NA, N, F.rhs, p->xs);
+      // AssembleSystMat (SystemTCD3D.C:228-294)
+      HIPCHK(hipMemsetAsync(p->bs, 0, sizeof(double) * N, p->stream));
+      k_axpy(p->stream, NA, tau * th3, oldrhs, p->bs);
+      k_axpy(p->stream, NA, tau * th4, F.rhs, p->bs);
+      k_matadd(p->stream, p->hl[Lf].rowptr[NA], F.val, F.aconv, -tau * th2);
+      double gamma = -tau * th2;
+      HIPCHK(hipMemsetAsync(def, 0, sizeof(double) * N, p->stream));
+      k_matvec_active(p->stream, F, oldsol, def);
+      k_axpy(p->stream, NA, 1.0, def, p->bs);
+      k_copy_range(p->stream, NA, N, F.rhs, p->bs);
+      k_copy_range(p->stream, NA, N, F.rhs, p->xs);
+      for (int l = 0; l <= Lf; l++) {
+        DevLevel &D = p->dl[l];
+        k_matadd(p->stream, p->hl[l].rowptr[D.nact], D.val, D.aconv, l == Lf ? -gamma + tau * th1 : tau * th1);
+      }
+      gamma = tau * th1;
+      k_copy(p->stream, N, F.rhs, oldrhs);
+      refresh_coarse_and_f32(*p);
+      // Solve (SystemTCD3D.C:317-376)
+      double r0;
+      const int its = fgmres(*p, *sp, r0, solve_status);
+      total_its += its;
+      // RestoreMassMat (SystemTCD3D.C:296-315)
+      for (int l = 0; l <= Lf; l++) {
+        DevLevel &D = p->dl[l];
+        k_matadd(p->stream, p->hl[l].rowptr[D.nact], D.val, D.aconv, -gamma);
+      }
+      if (states)
+        HIPCHK(hipMemcpyAsync(states + (size_t)m * N, p->xs, sizeof(double) * N, hipMemcpyDeviceToHost, p->stream));
+      if (solve_status != CD3D_OK) break;
+    }
+    HIPCHK(hipMemcpyAsync(u, p->xs, sizeof(double) * N, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    for (int k = 0; k < 3; k++) {
+      (void)hipFree(p->allocs.back());
+      p->allocs.pop_back();
+    }
+    p->dev_bytes -= 3 * sizeof(double) * N;
+    p->assembled = false;  // the level matrices now hold the mass matrices
+    if (st) {
+      st->iterations = total_its;
+      st->status = solve_status;
+      st->ms_solve = now_ms() - tstart;
+    }
+  });
+  return rc != CD3D_OK ? rc : solve_status;
+}
+
+cd3d_status cd3d_device_ptr(cd3d_problem *p, int level, int which, double **dev) {
+  if (!p || !dev || level < 0 || level >= (int)p->dl.size() || which < 0 || which > 3) return CD3D_ERR_ARG;
+  DevLevel &D = p->dl[level];
+  double *v[4] = {D.sol, D.rhs, D.aux, D.tmp};
+  *dev = v[which];
+  return CD3D_OK;
+}
+
+void *cd3d_stream(cd3d_problem *p) { return p ? (void *)p->stream : nullptr; }
+
+cd3d_status cd3d_synchronize(cd3d_problem *p) {
+  if (!p) return CD3D_ERR_ARG;
+  return hipStreamSynchronize(p->stream) == hipSuccess ? CD3D_OK : CD3D_ERR_HIP;
+}
+
+}  // extern "C"

@@ -1,0 +1,161 @@
+// Internal data structures of libcd3d_hip.so.
+//
+// Layout in device memory (HBM), per multigrid level l:
+//   CSR  rowptr int32[N+1], col int32[nnz], val f64[nnz] (+ val32 f32[nnz] for
+//        the mixed-precision V-cycle), diag int32[N] (position of a_ii)
+//   vectors f64[N]: sol, rhs, aux (defect / correction), tmp (Jacobi buffer)
+//            f32[N]: the same four for the mixed-precision V-cycle
+//   transfer from level l-1: P (fine rows, CSR) and R = P^T (coarse rows, CSR)
+//   assembly: cell geometry int4[ncell] (origin in node units + frame code),
+//            slot map int32[ncell][nb][nb] (CSR position of each local entry,
+//            -1 for Dirichlet rows), DOF map int32[ncell][nb], cells grouped by
+//            the 8 parity colours (no two cells of one colour share a DOF)
+// DOF numbering, CSR pattern and row pointers are those of the reference
+// (FESpace3D.C / SquareStructure3D.C); vectors are in that DOF order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/cd3d.h"
+
+namespace cd3d {
+
+struct HostLevel {
+  int n = 0;        // cells per side
+  int ncell = 0;
+  int nn = 0;       // nodes per side (p n + 1)
+  int ndof = 0, nact = 0;
+  int64_t nnz = 0;
+  std::vector<int> org;         // 3 per cell, node-lattice units
+  std::vector<int8_t> ax;       // 9 per cell: ax[9c + 3a + d]
+  std::vector<int> gn;          // nb per cell
+  std::vector<int> dof_node;    // DOF -> lattice node
+  std::vector<int> rowptr, kcol, diag;
+  // transfer from the next coarser level
+  std::vector<int> p_rowptr, p_col;
+  std::vector<double> p_val;
+  std::vector<int> r_rowptr, r_col;
+  std::vector<double> r_val;
+  std::vector<int> colour_start;  // 9 offsets into colour_cells
+  std::vector<int> colour_cells;
+};
+
+struct DevLevel {
+  int ndof = 0, nact = 0;
+  int64_t nnz = 0;
+  int *rowptr = nullptr, *col = nullptr, *diag = nullptr;
+  double *val = nullptr;
+  float *val32 = nullptr;
+  double *sol = nullptr, *rhs = nullptr, *aux = nullptr, *tmp = nullptr;
+  float *fsol = nullptr, *frhs = nullptr, *faux = nullptr, *ftmp = nullptr;
+  // TCD3D: conv-diff matrix next to the system (mass) matrix in `val`
+  double *aconv = nullptr;
+  // transfer (level >= 1)
+  int *p_rowptr = nullptr, *p_col = nullptr, *r_rowptr = nullptr, *r_col = nullptr;
+  double *p_val = nullptr, *r_val = nullptr;
+  float *p_val32 = nullptr, *r_val32 = nullptr;
+  int64_t p_nnz = 0, r_nnz = 0;
+  // assembly
+  int4 *cells = nullptr;
+  int *slots = nullptr;
+  int *gn = nullptr;
+  int *colour_cells = nullptr;
+  std::vector<int> colour_start;
+  int ncell = 0;
+  double h[3] = {0, 0, 0};   // cell sides
+  double dn[3] = {0, 0, 0};  // node spacing
+};
+
+struct Problem {
+  cd3d_mesh_desc mesh{};
+  int nb = 0;
+  std::vector<HostLevel> hl;
+  std::vector<DevLevel> dl;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  size_t dev_bytes = 0;
+  std::vector<void *> allocs;
+  std::string err;
+  std::mutex mu;
+  bool assembled = false;
+  // exact coarse solve: explicit inverse of the coarsest matrix (f64)
+  double *cinv = nullptr;
+  bool cinv_valid = false;
+  // FGMRES workspace
+  int krylov_restart = 0;
+  double *V = nullptr, *Z = nullptr, *w = nullptr, *xs = nullptr, *bs = nullptr;
+  double *red = nullptr;      // reduction partials
+  double *dscal = nullptr;    // device scalars (H column, norms)
+  double *hscal = nullptr;    // pinned host mirror of dscal
+  int red_blocks = 0;
+  // device tables for assembly (reference basis at quadrature points)
+  double *tab = nullptr;
+};
+
+// ---------------------------------------------------------------- setup (setup.cpp)
+void build_host_level(HostLevel &L, int order, int m, int depth);
+void build_transfer(HostLevel &fine, const HostLevel &coarse, int order);
+void build_slots(const HostLevel &L, int order, std::vector<int> &slots);
+void build_colours(HostLevel &L, int order);
+void basis_tables(int order, std::vector<double> &tab);
+int quad_points_1d(int order);
+
+// ---------------------------------------------------------------- kernels (kernels.hip)
+constexpr int kRedBlocks = 1024;
+
+void k_spmv(hipStream_t s, const DevLevel &L, const double *x, double *y);
+void k_defect(hipStream_t s, const DevLevel &L, const double *x, const double *f, double *d);
+void k_jacobi(hipStream_t s, const DevLevel &L, const double *x, const double *f, double *out, double omega);
+void k_jacobi_zero(hipStream_t s, const DevLevel &L, const double *f, double *out, double omega);
+void k_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, const double *fine, double *coarse);
+void k_prolong_update(hipStream_t s, const DevLevel &F, const DevLevel &C, const double *coarse, double *x,
+                      double damp);
+void k_prolong(hipStream_t s, const DevLevel &F, const DevLevel &C, const double *coarse, double *fine);
+void k_gemv_dense(hipStream_t s, int n, const double *A, const double *x, double *y);
+void k_coarse_smooth(hipStream_t s, const DevLevel &L, double *x, const double *f, double *d, int smoother,
+                     double omega, int maxit, double red);
+// float variants (mixed-precision V-cycle)
+void k_jacobi32(hipStream_t s, const DevLevel &L, const float *x, const float *f, float *out, float omega);
+void k_jacobi_zero32(hipStream_t s, const DevLevel &L, const float *f, float *out, float omega);
+void k_defect32(hipStream_t s, const DevLevel &L, const float *x, const float *f, float *d);
+void k_restrict32(hipStream_t s, const DevLevel &F, const DevLevel &C, const float *fine, float *coarse);
+void k_prolong_update32(hipStream_t s, const DevLevel &F, const DevLevel &C, const float *coarse, float *x,
+                        float damp);
+void k_cast_d2f(hipStream_t s, int n, const double *a, float *b);
+void k_cast_f2d(hipStream_t s, int n, const float *a, double *b);
+void k_to_f32(hipStream_t s, int64_t n, const double *a, float *b);
+// BLAS-1 with deterministic two-stage reductions
+void k_dot(hipStream_t s, int n, const double *a, const double *b, double *partials, double *out);
+void k_axpy_dot(hipStream_t s, int n, const double *alpha_partials, double sign, const double *x, double *y,
+                const double *z, double *partials, double *alpha_out);
+void k_scale_copy(hipStream_t s, int n, const double *partials, double *norm_out, const double *x, double *y);
+void k_axpy(hipStream_t s, int n, double alpha, const double *x, double *y);
+void k_scale(hipStream_t s, int n, double alpha, double *x);
+void k_axpy_dev(hipStream_t s, int n, const double *alpha_dev, int idx, const double *x, double *y);
+void k_copy(hipStream_t s, int n, const double *x, double *y);
+void k_fill(hipStream_t s, int64_t n, double v, double *y);
+void k_matadd(hipStream_t s, int64_t n, double *C, const double *D, double alpha);
+void k_matvec_active(hipStream_t s, const DevLevel &L, const double *x, double *y);
+// assembly
+struct AsmParams {
+  int what;  // 0 conv-diff, 1 mass
+  double eps, b[3], c, f_val;
+  int conv_kind, f_kind, supg, order;
+  double rot_c[2], sin2pit;
+  double h[3], dn[3];
+  double delta_const;  // SUPG delta for constant b (conv_kind 0)
+};
+void k_assemble(hipStream_t s, const DevLevel &L, const double *tab, const AsmParams &ap, double *val,
+                double *rhs, int colour_begin, int colour_count);
+void k_dirichlet_rows(hipStream_t s, const DevLevel &L, double *val, double *rhs);
+// scratch: 2 na^2 + na doubles; inv: na^2 doubles
+void k_invert_active(hipStream_t s, const DevLevel &L, double *scratch, double *inv);
+void k_multi_axpy(hipStream_t s, int n, int m, const double *coef, const double *Z, int64_t ldz, double *x);
+void k_copy_range(hipStream_t s, int b, int e, const double *src, double *dst);
+
+}  // namespace cd3d

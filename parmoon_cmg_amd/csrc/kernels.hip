@@ -1,0 +1,663 @@
+// HIP kernels for gfx950 (MI355X): CSR row kernels (SpMV, defect, damped
+// Jacobi), grid transfers, deterministic reductions, coarse solves and the
+// element assembly.  Everything on this path is HBM-bound sparse work or FP64
+// vector arithmetic; nothing is a dense contraction, so no MFMA is used.
+//
+// CSR row kernels map one row to a group of LANES consecutive lanes of a
+// 64-wide wavefront (64/LANES rows per wavefront): the row's column indices
+// and values are read as contiguous, coalesced segments and the group reduces
+// with DPP/permute shuffles.  LANES is chosen from the mean row length.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "cd3d_internal.hpp"
+
+namespace cd3d {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+enum RowMode { kSpmv = 0, kDefect = 1, kJacobi = 2, kRestrict = 3, kProlongUpd = 4, kActiveSpmv = 5 };
+
+template <int LANES, typename T>
+__device__ inline T group_sum(T v) {
+#pragma unroll
+  for (int off = LANES / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, LANES);
+  return v;
+}
+
+// One kernel body for every CSR row operation.  MODE:
+//   kSpmv       out = A x
+//   kDefect     out = f - A x                                   (ScalarDefect)
+//   kJacobi     out = (1-w) x + w (f - sum_{j!=i} a_ij x_j)/a_ii on active rows,
+//               out = f on Dirichlet rows                       (TMGLevel3D::Jacobi)
+//   kRestrict   out = R x on active coarse rows, 0 on the others (DefectRestriction + CorrectDefect)
+//   kProlongUpd out += w (P x) on active fine rows               (Prolongate + CorrectNodes + Update)
+//   kActiveSpmv out = A x on active rows only                   (MatVectActive)
+template <int LANES, int MODE, typename T, typename V>
+__global__ __launch_bounds__(kBlock) void csr_rows(int n, int nact, const int *__restrict__ rowptr,
+                                                   const int *__restrict__ col, const V *__restrict__ val,
+                                                   const T *__restrict__ x, const T *__restrict__ f,
+                                                   T *__restrict__ out, T omega) {
+  const int gid = blockIdx.x * kBlock + threadIdx.x;
+  const int row = gid / LANES, lane = gid % LANES;
+  if (row >= n) return;  // whole groups leave together
+  if ((MODE == kRestrict || MODE == kProlongUpd || MODE == kActiveSpmv) && row >= nact) {
+    if (MODE == kRestrict && lane == 0) out[row] = T(0);
+    return;
+  }
+  const int b = rowptr[row], e = rowptr[row + 1];
+  T s = T(0), diag = T(0);
+  for (int j = b + lane; j < e; j += LANES) {
+    const int c = col[j];
+    const T a = (T)val[j];
+    if (MODE == kJacobi) {
+      if (c == row) diag = a;
+      else s += a * x[c];
+    } else {
+      s += a * x[c];
+    }
+  }
+  s = group_sum<LANES>(s);
+  if (MODE == kJacobi) diag = group_sum<LANES>(diag);
+  if (lane != 0) return;
+  if (MODE == kSpmv || MODE == kRestrict || MODE == kActiveSpmv) out[row] = s;
+  else if (MODE == kDefect) out[row] = f[row] - s;
+  else if (MODE == kJacobi) out[row] = row < nact ? (T(1) - omega) * x[row] + omega * (f[row] - s) / diag : f[row];
+  else if (MODE == kProlongUpd) out[row] += omega * s;
+}
+
+template <int MODE, typename T, typename V>
+void launch_rows(hipStream_t st, int lanes, int n, int nact, const int *rp, const int *col, const V *val, const T *x,
+                 const T *f, T *out, T omega) {
+  if (n <= 0) return;
+  const int64_t threads = (int64_t)n * lanes;
+  const int grid = (int)((threads + kBlock - 1) / kBlock);
+  switch (lanes) {
+    case 4: csr_rows<4, MODE, T, V><<<grid, kBlock, 0, st>>>(n, nact, rp, col, val, x, f, out, omega); break;
+    case 8: csr_rows<8, MODE, T, V><<<grid, kBlock, 0, st>>>(n, nact, rp, col, val, x, f, out, omega); break;
+    case 16: csr_rows<16, MODE, T, V><<<grid, kBlock, 0, st>>>(n, nact, rp, col, val, x, f, out, omega); break;
+    default: csr_rows<32, MODE, T, V><<<grid, kBlock, 0, st>>>(n, nact, rp, col, val, x, f, out, omega); break;
+  }
+}
+
+// lanes per row from the mean row length: about four entries per lane
+inline int lanes_for(int64_t nnz, int n) {
+  double m = n > 0 ? (double)nnz / n : 1;
+  if (m <= 12) return 4;
+  if (m <= 40) return 8;
+  if (m <= 96) return 16;
+  return 32;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void jacobi_zero_k(int n, int nact, const int *__restrict__ diag,
+                                                        const double *__restrict__ val64,
+                                                        const float *__restrict__ val32, const T *__restrict__ f,
+                                                        T *__restrict__ out, T omega) {
+  // first sweep from a zero iterate with zero Dirichlet data: every product
+  // with the old iterate vanishes, so out = w f / a_ii exactly as the full sweep
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  if (i < nact) {
+    const T d = val32 ? (T)val32[diag[i]] : (T)val64[diag[i]];
+    out[i] = omega * f[i] / d;
+  } else {
+    out[i] = f[i];
+  }
+}
+
+// ---------------------------------------------------------------- reductions
+// Deterministic two-stage sums: each block writes one partial; the consumer
+// sums the kRedBlocks partials in a fixed order (every block of the consumer
+// does the same sum, so all see the same value).
+
+__device__ inline double block_sum(double v, double *sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double t = 0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) t += sh[k];
+  return t;  // valid in thread 0
+}
+
+__device__ inline double sum_partials(const double *__restrict__ p, double *sh) {
+  double v = 0;
+  for (int k = threadIdx.x; k < kRedBlocks; k += blockDim.x) v += p[k];
+  v = block_sum(v, sh);
+  __syncthreads();
+  if (threadIdx.x == 0) sh[16] = v;
+  __syncthreads();
+  return sh[16];
+}
+
+__global__ __launch_bounds__(kBlock) void dot_k(int n, const double *__restrict__ a, const double *__restrict__ b,
+                                                double *__restrict__ partials) {
+  __shared__ double sh[17];
+  double v = 0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) v += a[i] * b[i];
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void finish_k(const double *__restrict__ partials, double *__restrict__ out) {
+  __shared__ double sh[17];
+  const double v = sum_partials(partials, sh);
+  if (threadIdx.x == 0) *out = v;
+}
+
+// y += sign * alpha * x with alpha = *alpha_dev (or the sum of alpha partials
+// when alpha_dev is the partial array), then partial dot of the new y with z
+__global__ __launch_bounds__(kBlock) void axpy_dot_k(int n, const double *__restrict__ alpha_partials,
+                                                     double *__restrict__ alpha_out, double sign,
+                                                     const double *__restrict__ x, double *__restrict__ y,
+                                                     const double *__restrict__ z, double *__restrict__ partials) {
+  __shared__ double sh[17];
+  const double alpha = sum_partials(alpha_partials, sh);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && alpha_out) *alpha_out = alpha;
+  const double a = sign * alpha;
+  double v = 0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double yi = y[i] + a * x[i];
+    y[i] = yi;
+    v += yi * (z ? z[i] : yi);
+  }
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = v;
+}
+
+// y = x / sqrt(sum partials); writes the norm to *norm_out
+__global__ __launch_bounds__(kBlock) void scale_copy_k(int n, const double *__restrict__ partials,
+                                                       double *__restrict__ norm_out, const double *__restrict__ x,
+                                                       double *__restrict__ y) {
+  __shared__ double sh[17];
+  const double nrm = sqrt(sum_partials(partials, sh));
+  if (blockIdx.x == 0 && threadIdx.x == 0) *norm_out = nrm;
+  const double s = nrm != 0.0 ? 1.0 / nrm : 1.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] = x[i] * s;
+}
+
+__global__ void scale_k(int n, double alpha, double *__restrict__ x) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) x[i] *= alpha;
+}
+
+__global__ void axpy_k(int n, double alpha, const double *__restrict__ x, double *__restrict__ y) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] += alpha * x[i];
+}
+
+__global__ void axpy_dev_k(int n, const double *__restrict__ alpha, int idx, const double *__restrict__ x,
+                           double *__restrict__ y) {
+  const double a = alpha[idx];
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] += a * x[i];
+}
+
+__global__ void copy_k(int n, const double *__restrict__ x, double *__restrict__ y) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] = x[i];
+}
+
+__global__ void fill_k(int64_t n, double v, double *__restrict__ y) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) y[i] = v;
+}
+
+__global__ void matadd_k(int64_t n, double *__restrict__ C, const double *__restrict__ D, double alpha) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    C[i] += alpha * D[i];
+}
+
+__global__ void d2f_k(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    b[i] = (float)a[i];
+}
+
+__global__ void f2d_k(int64_t n, const float *__restrict__ a, double *__restrict__ b) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    b[i] = (double)a[i];
+}
+
+inline int grid_for(int64_t n) {
+  int64_t g = (n + kBlock - 1) / kBlock;
+  return (int)(g < 2048 ? (g < 1 ? 1 : g) : 2048);
+}
+
+// ---------------------------------------------------------------- coarse level
+
+// x_A += Ainv_AA d_A : one wavefront per row of the explicit inverse of the
+// active block (exact coarse solve as one defect correction)
+__global__ __launch_bounds__(kBlock) void coarse_update_k(int na, const double *__restrict__ Ainv,
+                                                          const double *__restrict__ d, double *__restrict__ x) {
+  const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  if (row >= na) return;
+  double s = 0;
+  for (int j = lane; j < na; j += 64) s += Ainv[(int64_t)row * na + j] * d[j];
+  s = group_sum<64>(s);
+  if (lane == 0) x[row] += s;
+}
+
+// Coarse-level smoothing loop of TMultiGrid3D::Smooth(CoarseSmooth)
+// (MultiGrid3D.C:197-260) with SSOR (MGLevel3D.C:337-441) or Jacobi sweeps,
+// run by a single wavefront: rows are updated one after another, each row's
+// dot product spread over the 64 lanes; the iterate lives in LDS.
+__global__ __launch_bounds__(64) void coarse_smooth_k(int n, int nact, const int *__restrict__ rp,
+                                                      const int *__restrict__ col, const double *__restrict__ val,
+                                                      double *__restrict__ xg, const double *__restrict__ f,
+                                                      double *__restrict__ aux, int smoother, double omega,
+                                                      int maxit, double red) {
+  extern __shared__ double x[];
+  const int lane = threadIdx.x;
+  for (int i = lane; i < n; i += 64) x[i] = xg[i];
+  __syncthreads();
+  auto defect_norm = [&]() {
+    double s2 = 0;
+    for (int i = 0; i < n; i++) {
+      double s = 0;
+      for (int j = rp[i] + lane; j < rp[i + 1]; j += 64) s += val[j] * x[col[j]];
+      s = group_sum<64>(s);
+      const double d = f[i] - s;
+      s2 += d * d;
+    }
+    return sqrt(s2);
+  };
+  auto row_update = [&](int i, const double *xr) {  // (f_i - sum_{j!=i} a_ij xr_j) / a_ii
+    double s = 0, diag = 0;
+    const int b = rp[i], e = rp[i + 1];
+    for (int j = b + lane; j < e; j += 64) {
+      const int c = col[j];
+      if (c == i) diag += val[j];
+      else s += val[j] * xr[c];
+    }
+    s = group_sum<64>(s);
+    diag = group_sum<64>(diag);
+    return (f[i] - s) / diag;
+  };
+  double res = defect_norm();
+  const double reduction = red * res;
+  int it = 0;
+  while (res > reduction && it < maxit) {
+    if (smoother == 1) {  // Jacobi: aux = old iterate, then Dirichlet rows = f
+      for (int i = lane; i < n; i += 64) aux[i] = x[i];
+      __syncthreads();
+      for (int i = nact + lane; i < n; i += 64) x[i] = f[i];
+      __syncthreads();
+      for (int i = 0; i < nact; i++) {
+        const double q = row_update(i, aux);
+        if (lane == 0) x[i] = (1 - omega) * aux[i] + omega * q;
+      }
+      __syncthreads();
+    } else {  // SSOR: Dirichlet rows = f, forward then backward relaxed Gauss-Seidel
+      for (int i = nact + lane; i < n; i += 64) x[i] = f[i];
+      __syncthreads();
+      for (int i = 0; i < nact; i++) {
+        const double q = row_update(i, x);
+        __syncthreads();
+        if (lane == 0) x[i] = omega * (q - x[i]) + x[i];
+        __syncthreads();
+      }
+      for (int i = nact - 1; i >= 0; i--) {
+        const double q = row_update(i, x);
+        __syncthreads();
+        if (lane == 0) x[i] = omega * (q - x[i]) + x[i];
+        __syncthreads();
+      }
+    }
+    res = defect_norm();
+    it++;
+  }
+  __syncthreads();
+  for (int i = lane; i < n; i += 64) xg[i] = x[i];
+}
+
+// ---------------------------------------------------------------- assembly
+
+// cell frame code: for local axis a, bits 3a..3a+1 = global axis, bit 3a+2 = negative
+__device__ inline void decode_frame(int code, int *dax, int *sgn) {
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    dax[a] = (code >> (3 * a)) & 3;
+    sgn[a] = ((code >> (3 * a + 2)) & 1) ? -1 : 1;
+  }
+}
+
+__device__ inline double sdfem_delta(double hconv, double eps, double nb, double nu) {
+  // Compute_SDFEM_delta, SDFEM_TYPE 2 (ConvDiff.C:540-556), DELTA0 = 1
+  if (fabs(eps) < 1e-20) eps = 1e-20;
+  if (!(nb > 0)) return 0.0;
+  const double alpha = nu * nb * hconv / (2 * eps);
+  return nu * hconv * (1 / tanh(alpha) - 1 / alpha) / (2 * nb);
+}
+
+__device__ inline double hconv_brick(const double *h, double b0, double b1, double b2) {
+  // Mesh_size_in_convection_direction on an axis-aligned brick (ConvDiff.C:411-469)
+  const double nb = sqrt(b0 * b0 + b1 * b1 + b2 * b2);
+  double den = 0;
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+    const double v = ((s & 1) ? -b0 : b0) / h[0] + ((s & 2) ? -b1 : b1) / h[1] + ((s & 4) ? -b2 : b2) / h[2];
+    den += fabs(v) / 4;
+  }
+  if (den < 1e-10) return sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]);
+  return 2 * nb / den;
+}
+
+// One thread per (cell, local row i); NB rows of a cell are consecutive
+// threads.  Per quadrature point the thread forms, in the cell's reference
+// frame, u_i = kappa .* grad phi_i (diffusion), t_i = phi_i + delta beta.grad
+// phi_i (test function with SUPG term) and s_i = beta.grad phi_i + c phi_i
+// (convection-reaction applied to the ansatz function), shares s_i through
+// LDS and accumulates row i of the local matrix in registers:
+//   A_ij += w |J| (u_i . grad phi_j + t_i s_j)
+// (BilinearAssemble_SD / BilinearAssemble, ConvDiff3D.C:61-190, with the
+// physical gradients J^-T grad phi of THexaAffin folded into kappa and beta:
+// the cell frames are signed axis permutations).  Then the row is added into
+// the CSR values through the slot map; cells of one colour share no DOF.
+template <int P>
+__global__ __launch_bounds__(kBlock) void assemble_k(int ncells, const int *__restrict__ list,
+                                                     const int4 *__restrict__ cells, const int *__restrict__ slots,
+                                                     const int *__restrict__ gn, int nact,
+                                                     const double *__restrict__ tab, AsmParams ap,
+                                                     double *__restrict__ val, double *__restrict__ rhs) {
+  constexpr int NB = (P + 1) * (P + 1) * (P + 1);
+  constexpr int N1 = P + 2;
+  constexpr int NQ = N1 * N1 * N1;
+  constexpr int STRIDE = 4 + 4 * NB;
+  constexpr int CPB = kBlock / NB;
+  __shared__ double s_sh[CPB][NB];
+  const int t = threadIdx.x, lc = t / NB, i = t % NB;
+  const int ci = blockIdx.x * CPB + lc;
+  const bool live = lc < CPB && ci < ncells;
+  const int c = live ? list[ci] : 0;
+  int4 cg = live ? cells[c] : make_int4(0, 0, 0, 0);
+  int dax[3], sg[3];
+  decode_frame(cg.w, dax, sg);
+  const int O[3] = {cg.x, cg.y, cg.z};
+  const double detj = (ap.h[0] / 2) * (ap.h[1] / 2) * (ap.h[2] / 2);
+  double kap[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) kap[a] = ap.eps * 4.0 / (ap.h[dax[a]] * ap.h[dax[a]]);
+  const double nu = 1.0 / P;
+  double acc[NB];
+#pragma unroll
+  for (int j = 0; j < NB; j++) acc[j] = 0;
+  double racc = 0;
+  double hconv = -1;
+  for (int q = 0; q < NQ; q++) {
+    const double *T = tab + q * STRIDE;
+    const double w = T[0] * detj;
+    const double ref[3] = {T[1], T[2], T[3]};
+    // physical quadrature point (only variable coefficients need it)
+    double X[3] = {0, 0, 0};
+    if (ap.conv_kind == 1 || ap.f_kind == 1) {
+#pragma unroll
+      for (int a = 0; a < 3; a++) X[dax[a]] = ap.dn[dax[a]] * (O[dax[a]] + P * sg[a] * (ref[a] + 1) * 0.5);
+    }
+    double b0 = ap.b[0], b1 = ap.b[1], b2 = ap.b[2];
+    if (ap.conv_kind == 1) {
+      b0 = -(X[1] - ap.rot_c[1]);
+      b1 = X[0] - ap.rot_c[0];
+      b2 = 0.1 * ap.sin2pit;
+    }
+    double f = ap.f_val;
+    if (ap.f_kind == 1) {
+      const double sx = sin(M_PI * X[0]), sy = sin(M_PI * X[1]), sz = sin(M_PI * X[2]);
+      const double cx = cos(M_PI * X[0]), cy = cos(M_PI * X[1]), cz = cos(M_PI * X[2]), u = sx * sy * sz;
+      f = ap.eps * 3 * M_PI * M_PI * u + b0 * M_PI * cx * sy * sz + b1 * M_PI * sx * cy * sz +
+          b2 * M_PI * sx * sy * cz + ap.c * u;
+    }
+    double delta = 0;
+    if (ap.supg && ap.what == 0) {
+      if (ap.conv_kind == 0) {
+        delta = ap.delta_const;
+      } else {
+        if (hconv < 0) hconv = hconv_brick(ap.h, b0, b1, b2);
+        delta = sdfem_delta(hconv, ap.eps, sqrt(b0 * b0 + b1 * b1 + b2 * b2), nu);
+      }
+    }
+    const double bphys[3] = {b0, b1, b2};
+    double beta[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) beta[a] = sg[a] * bphys[dax[a]] * 2.0 / ap.h[dax[a]];
+    const double phi_i = T[4 + 4 * i], g0 = T[5 + 4 * i], g1 = T[6 + 4 * i], g2 = T[7 + 4 * i];
+    const double bg = beta[0] * g0 + beta[1] * g1 + beta[2] * g2;
+    if (ap.what == 1) {
+      __syncthreads();
+      if (live) s_sh[lc][i] = phi_i;
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < NB; j++) acc[j] += w * (phi_i * s_sh[lc][j]);
+      racc += w * phi_i * f;
+      continue;
+    }
+    const double u0 = kap[0] * g0, u1 = kap[1] * g1, u2 = kap[2] * g2;
+    const double ti = phi_i + delta * bg;
+    __syncthreads();
+    if (live) s_sh[lc][i] = bg + ap.c * phi_i;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      const double v = u0 * T[5 + 4 * j] + u1 * T[6 + 4 * j] + u2 * T[7 + 4 * j] + ti * s_sh[lc][j];
+      acc[j] += w * v;
+    }
+    racc += w * f * ti;
+  }
+  if (!live) return;
+  const int r = gn[c * NB + i];
+  if (r >= nact) return;
+  const int *sl = slots + ((int64_t)c * NB + i) * NB;
+#pragma unroll
+  for (int j = 0; j < NB; j++) val[sl[j]] += acc[j];
+  rhs[r] += racc;
+}
+
+// Explicit inverse of the active block of the coarsest matrix by Gauss-Jordan
+// elimination with partial pivoting, one workgroup; M is na x 2na scratch
+// ([A_AA | I]), fac na scratch.  Ties in the pivot search go to the lower row
+// so the result is deterministic.
+__global__ __launch_bounds__(1024) void gj_invert_k(int na, const int *__restrict__ rp, const int *__restrict__ col,
+                                                    const double *__restrict__ val, double *__restrict__ M,
+                                                    double *__restrict__ fac, double *__restrict__ inv) {
+  __shared__ double sv[1024];
+  __shared__ int si[1024];
+  const int tid = threadIdx.x, nt = blockDim.x, w2 = 2 * na;
+  for (int64_t e = tid; e < (int64_t)na * w2; e += nt) M[e] = 0.0;
+  __syncthreads();
+  for (int r = tid; r < na; r += nt) {
+    for (int j = rp[r]; j < rp[r + 1]; j++)
+      if (col[j] < na) M[(int64_t)r * w2 + col[j]] = val[j];
+    M[(int64_t)r * w2 + na + r] = 1.0;
+  }
+  __syncthreads();
+  for (int k = 0; k < na; k++) {
+    double best = -1.0;
+    int bi = k;
+    for (int r = k + tid; r < na; r += nt) {
+      const double v = fabs(M[(int64_t)r * w2 + k]);
+      if (v > best) { best = v; bi = r; }
+    }
+    sv[tid] = best;
+    si[tid] = bi;
+    __syncthreads();
+    for (int s = nt / 2; s > 0; s >>= 1) {
+      if (tid < s && (sv[tid + s] > sv[tid] || (sv[tid + s] == sv[tid] && si[tid + s] < si[tid]))) {
+        sv[tid] = sv[tid + s];
+        si[tid] = si[tid + s];
+      }
+      __syncthreads();
+    }
+    const int p = si[0];
+    if (p != k)
+      for (int j = tid; j < w2; j += nt) {
+        const double t = M[(int64_t)k * w2 + j];
+        M[(int64_t)k * w2 + j] = M[(int64_t)p * w2 + j];
+        M[(int64_t)p * w2 + j] = t;
+      }
+    __syncthreads();
+    const double ip = 1.0 / M[(int64_t)k * w2 + k];
+    __syncthreads();
+    for (int j = k + tid; j < w2; j += nt) M[(int64_t)k * w2 + j] *= ip;
+    for (int r = tid; r < na; r += nt) fac[r] = r == k ? 0.0 : M[(int64_t)r * w2 + k];
+    __syncthreads();
+    const int wk = w2 - k;
+    for (int64_t e = tid; e < (int64_t)na * wk; e += nt) {
+      const int r = (int)(e / wk), j = k + (int)(e % wk);
+      M[(int64_t)r * w2 + j] -= fac[r] * M[(int64_t)k * w2 + j];
+    }
+    __syncthreads();
+  }
+  for (int64_t e = tid; e < (int64_t)na * na; e += nt) inv[e] = M[(e / na) * w2 + na + (e % na)];
+}
+
+// x += sum_k coef[k] Z_k  (FGMRES update x += Z y, UpdateGmresIterate)
+__global__ void multi_axpy_k(int n, int m, const double *__restrict__ coef, const double *__restrict__ Z,
+                             int64_t ldz, double *__restrict__ x) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    double s = 0;
+    for (int k = 0; k < m; k++) s += coef[k] * Z[k * ldz + i];
+    x[i] += s;
+  }
+}
+
+// x_i = v for i in [b, e)  (Dirichlet part of an iterate)
+__global__ void copy_range_k(int b, int e, const double *__restrict__ src, double *__restrict__ dst) {
+  const int i = b + blockIdx.x * kBlock + threadIdx.x;
+  if (i < e) dst[i] = src[i];
+}
+
+__global__ void dirichlet_rows_k(int n, int nact, const int *__restrict__ rp, double *__restrict__ val,
+                                 double *__restrict__ rhs) {
+  const int r = nact + blockIdx.x * kBlock + threadIdx.x;
+  if (r >= n) return;
+  val[rp[r]] = 1.0;  // AddLocalSqMatToGlobal: Dirichlet row = identity row
+  rhs[r] = 0.0;      // AddLocalRhsToGlobal: Dirichlet value g = 0
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+
+void k_spmv(hipStream_t s, const DevLevel &L, const double *x, double *y) {
+  launch_rows<kSpmv, double, double>(s, lanes_for(L.nnz, L.ndof), L.ndof, L.nact, L.rowptr, L.col, L.val, x,
+                                     nullptr, y, 0.0);
+}
+void k_defect(hipStream_t s, const DevLevel &L, const double *x, const double *f, double *d) {
+  launch_rows<kDefect, double, double>(s, lanes_for(L.nnz, L.ndof), L.ndof, L.nact, L.rowptr, L.col, L.val, x, f,
+                                       d, 0.0);
+}
+void k_jacobi(hipStream_t s, const DevLevel &L, const double *x, const double *f, double *out, double omega) {
+  launch_rows<kJacobi, double, double>(s, lanes_for(L.nnz, L.ndof), L.ndof, L.nact, L.rowptr, L.col, L.val, x, f,
+                                       out, omega);
+}
+void k_jacobi_zero(hipStream_t s, const DevLevel &L, const double *f, double *out, double omega) {
+  jacobi_zero_k<double><<<(L.ndof + kBlock - 1) / kBlock, kBlock, 0, s>>>(L.ndof, L.nact, L.diag, L.val, nullptr,
+                                                                         f, out, omega);
+}
+void k_matvec_active(hipStream_t s, const DevLevel &L, const double *x, double *y) {
+  launch_rows<kActiveSpmv, double, double>(s, lanes_for(L.nnz, L.ndof), L.ndof, L.nact, L.rowptr, L.col, L.val, x,
+                                           nullptr, y, 0.0);
+}
+void k_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, const double *fine, double *coarse) {
+  launch_rows<kRestrict, double, double>(s, lanes_for(F.r_nnz, C.ndof), C.ndof, C.nact, F.r_rowptr, F.r_col,
+                                         F.r_val, fine, nullptr, coarse, 0.0);
+}
+void k_prolong_update(hipStream_t s, const DevLevel &F, const DevLevel &C, const double *coarse, double *x,
+                      double damp) {
+  launch_rows<kProlongUpd, double, double>(s, lanes_for(F.p_nnz, F.ndof), F.ndof, F.nact, F.p_rowptr, F.p_col,
+                                           F.p_val, coarse, nullptr, x, damp);
+}
+void k_prolong(hipStream_t s, const DevLevel &F, const DevLevel &C, const double *coarse, double *fine) {
+  (void)C;
+  launch_rows<kSpmv, double, double>(s, lanes_for(F.p_nnz, F.ndof), F.ndof, F.nact, F.p_rowptr, F.p_col, F.p_val,
+                                     coarse, nullptr, fine, 0.0);
+}
+void k_jacobi32(hipStream_t s, const DevLevel &L, const float *x, const float *f, float *out, float omega) {
+  launch_rows<kJacobi, float, float>(s, lanes_for(L.nnz, L.ndof), L.ndof, L.nact, L.rowptr, L.col, L.val32, x, f,
+                                     out, omega);
+}
+void k_jacobi_zero32(hipStream_t s, const DevLevel &L, const float *f, float *out, float omega) {
+  jacobi_zero_k<float><<<(L.ndof + kBlock - 1) / kBlock, kBlock, 0, s>>>(L.ndof, L.nact, L.diag, nullptr, L.val32,
+                                                                        f, out, omega);
+}
+void k_defect32(hipStream_t s, const DevLevel &L, const float *x, const float *f, float *d) {
+  launch_rows<kDefect, float, float>(s, lanes_for(L.nnz, L.ndof), L.ndof, L.nact, L.rowptr, L.col, L.val32, x, f,
+                                     d, 0.0f);
+}
+void k_restrict32(hipStream_t s, const DevLevel &F, const DevLevel &C, const float *fine, float *coarse) {
+  launch_rows<kRestrict, float, float>(s, lanes_for(F.r_nnz, C.ndof), C.ndof, C.nact, F.r_rowptr, F.r_col,
+                                       F.r_val32, fine, nullptr, coarse, 0.0f);
+}
+void k_prolong_update32(hipStream_t s, const DevLevel &F, const DevLevel &C, const float *coarse, float *x,
+                        float damp) {
+  (void)C;
+  launch_rows<kProlongUpd, float, float>(s, lanes_for(F.p_nnz, F.ndof), F.ndof, F.nact, F.p_rowptr, F.p_col,
+                                         F.p_val32, coarse, nullptr, x, damp);
+}
+void k_cast_d2f(hipStream_t s, int n, const double *a, float *b) { d2f_k<<<grid_for(n), kBlock, 0, s>>>(n, a, b); }
+void k_cast_f2d(hipStream_t s, int n, const float *a, double *b) { f2d_k<<<grid_for(n), kBlock, 0, s>>>(n, a, b); }
+void k_to_f32(hipStream_t s, int64_t n, const double *a, float *b) { d2f_k<<<grid_for(n), kBlock, 0, s>>>(n, a, b); }
+
+void k_dot(hipStream_t s, int n, const double *a, const double *b, double *partials, double *out) {
+  dot_k<<<kRedBlocks, kBlock, 0, s>>>(n, a, b, partials);
+  if (out) finish_k<<<1, kBlock, 0, s>>>(partials, out);
+}
+void k_axpy_dot(hipStream_t s, int n, const double *alpha_partials, double sign, const double *x, double *y,
+                const double *z, double *partials, double *alpha_out) {
+  axpy_dot_k<<<kRedBlocks, kBlock, 0, s>>>(n, alpha_partials, alpha_out, sign, x, y, z, partials);
+}
+void k_scale_copy(hipStream_t s, int n, const double *partials, double *norm_out, const double *x, double *y) {
+  scale_copy_k<<<kRedBlocks, kBlock, 0, s>>>(n, partials, norm_out, x, y);
+}
+void k_scale(hipStream_t s, int n, double alpha, double *x) { scale_k<<<grid_for(n), kBlock, 0, s>>>(n, alpha, x); }
+void k_axpy(hipStream_t s, int n, double alpha, const double *x, double *y) {
+  axpy_k<<<grid_for(n), kBlock, 0, s>>>(n, alpha, x, y);
+}
+void k_axpy_dev(hipStream_t s, int n, const double *alpha_dev, int idx, const double *x, double *y) {
+  axpy_dev_k<<<grid_for(n), kBlock, 0, s>>>(n, alpha_dev, idx, x, y);
+}
+void k_copy(hipStream_t s, int n, const double *x, double *y) { copy_k<<<grid_for(n), kBlock, 0, s>>>(n, x, y); }
+void k_fill(hipStream_t s, int64_t n, double v, double *y) { fill_k<<<grid_for(n), kBlock, 0, s>>>(n, v, y); }
+void k_matadd(hipStream_t s, int64_t n, double *C, const double *D, double alpha) {
+  matadd_k<<<grid_for(n), kBlock, 0, s>>>(n, C, D, alpha);
+}
+void k_gemv_dense(hipStream_t s, int n, const double *A, const double *x, double *y) {
+  coarse_update_k<<<(n * 64 + kBlock - 1) / kBlock, kBlock, 0, s>>>(n, A, x, y);
+}
+void k_coarse_smooth(hipStream_t s, const DevLevel &L, double *x, const double *f, double *d, int smoother,
+                     double omega, int maxit, double red) {
+  coarse_smooth_k<<<1, 64, sizeof(double) * L.ndof, s>>>(L.ndof, L.nact, L.rowptr, L.col, L.val, x, f, d, smoother,
+                                                         omega, maxit, red);
+}
+void k_assemble(hipStream_t s, const DevLevel &L, const double *tab, const AsmParams &ap, double *val, double *rhs,
+                int colour_begin, int colour_count) {
+  if (colour_count <= 0) return;
+  const int *list = L.colour_cells + colour_begin;
+  if (ap.order == 1) {
+    constexpr int cpb = kBlock / 8;
+    assemble_k<1><<<(colour_count + cpb - 1) / cpb, kBlock, 0, s>>>(colour_count, list, L.cells, L.slots, L.gn,
+                                                                      L.nact, tab, ap, val, rhs);
+  } else {
+    constexpr int cpb = kBlock / 27;
+    assemble_k<2><<<(colour_count + cpb - 1) / cpb, kBlock, 0, s>>>(colour_count, list, L.cells, L.slots, L.gn,
+                                                                      L.nact, tab, ap, val, rhs);
+  }
+}
+void k_invert_active(hipStream_t s, const DevLevel &L, double *scratch, double *inv) {
+  const int na = L.nact;
+  if (na <= 0) return;
+  gj_invert_k<<<1, 1024, 0, s>>>(na, L.rowptr, L.col, L.val, scratch, scratch + (int64_t)2 * na * na, inv);
+}
+void k_multi_axpy(hipStream_t s, int n, int m, const double *coef, const double *Z, int64_t ldz, double *x) {
+  multi_axpy_k<<<grid_for(n), kBlock, 0, s>>>(n, m, coef, Z, ldz, x);
+}
+void k_copy_range(hipStream_t s, int b, int e, const double *src, double *dst) {
+  if (e > b) copy_range_k<<<(e - b + kBlock - 1) / kBlock, kBlock, 0, s>>>(b, e, src, dst);
+}
+void k_dirichlet_rows(hipStream_t s, const DevLevel &L, double *val, double *rhs) {
+  const int nd = L.ndof - L.nact;
+  if (nd <= 0) return;
+  dirichlet_rows_k<<<(nd + kBlock - 1) / kBlock, kBlock, 0, s>>>(L.ndof, L.nact, L.rowptr, val, rhs);
+}
+
+}  // namespace cd3d

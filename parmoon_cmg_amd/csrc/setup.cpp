@@ -1,0 +1,323 @@
+// Host-side construction of the multilevel problem: mesh, DOF numbering, CSR
+// pattern, grid-transfer operators, element slot maps and colours.
+//
+// The mesh is the uniform refinement of m^3 macro bricks in depth-first child
+// order with the rotated child frames of the regular hexahedron refinement
+// (RefHexaRegDesc.C:29-33).  Everything is generated on the integer node
+// lattice of each level, so shared DOFs, boundary DOFs and matrix columns are
+// found by lattice arithmetic instead of the reference's joint/mapper graph.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <thread>
+
+#include "cd3d_internal.hpp"
+
+namespace cd3d {
+
+namespace {
+
+// child frames of the regular refinement: child j starts at parent vertex j;
+// its local axes (v1-v0, v3-v0, v4-v0) in the parent's local axes
+// (RefHexaRegDesc.C:29-33, DatChildVertex)
+const int kChildAxes[8][3][3] = {
+    {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},   {{0, 1, 0}, {-1, 0, 0}, {0, 0, 1}},
+    {{-1, 0, 0}, {0, -1, 0}, {0, 0, 1}}, {{0, -1, 0}, {1, 0, 0}, {0, 0, 1}},
+    {{0, 1, 0}, {1, 0, 0}, {0, 0, -1}},  {{-1, 0, 0}, {0, 1, 0}, {0, 0, -1}},
+    {{0, -1, 0}, {-1, 0, 0}, {0, 0, -1}}, {{1, 0, 0}, {0, -1, 0}, {0, 0, -1}}};
+// hexahedron vertex j in local coordinates (bottom 0-3 counter-clockwise, 4-7 above)
+const int kHexV[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                         {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+
+struct Frame {
+  int O[3];
+  int E[3][3];
+};
+
+void emit(HostLevel &L, int &cnt, const Frame &f, int S, int depth) {
+  if (depth == 0) {
+    for (int d = 0; d < 3; d++) L.org[3 * cnt + d] = f.O[d];
+    for (int a = 0; a < 3; a++)
+      for (int d = 0; d < 3; d++) L.ax[9 * cnt + 3 * a + d] = (int8_t)f.E[a][d];
+    cnt++;
+    return;
+  }
+  for (int j = 0; j < 8; j++) {
+    Frame c;
+    for (int d = 0; d < 3; d++)
+      c.O[d] = f.O[d] + S * (kHexV[j][0] * f.E[0][d] + kHexV[j][1] * f.E[1][d] + kHexV[j][2] * f.E[2][d]);
+    for (int a = 0; a < 3; a++)
+      for (int d = 0; d < 3; d++)
+        c.E[a][d] = kChildAxes[j][a][0] * f.E[0][d] + kChildAxes[j][a][1] * f.E[1][d] +
+                    kChildAxes[j][a][2] * f.E[2][d];
+    emit(L, cnt, c, S / 2, depth - 1);
+  }
+}
+
+template <class F>
+void parallel_for(int n, F f) {
+  int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 4096) nt = 1;
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; t++)
+    th.emplace_back([=]() {
+      int b = (int)((int64_t)n * t / nt), e = (int)((int64_t)n * (t + 1) / nt);
+      for (int i = b; i < e; i++) f(i);
+    });
+  for (auto &x : th) x.join();
+}
+
+inline int cell_node(const HostLevel &L, int c, int i, int j, int k) {
+  const int *O = &L.org[3 * c];
+  const int8_t *E = &L.ax[9 * c];
+  int X = O[0] + i * E[0] + j * E[3] + k * E[6];
+  int Y = O[1] + i * E[1] + j * E[4] + k * E[7];
+  int Z = O[2] + i * E[2] + j * E[5] + k * E[8];
+  return X + L.nn * (Y + L.nn * Z);
+}
+
+}  // namespace
+
+int quad_points_1d(int order) { return order + 2; }  // Gauss3 (Q1), Gauss4 (Q2): FEDatabase3D.C:1009-1016
+
+void build_host_level(HostLevel &L, int order, int m, int depth) {
+  const int p = order, nb = (p + 1) * (p + 1) * (p + 1);
+  L.n = m << depth;
+  L.ncell = L.n * L.n * L.n;
+  L.nn = p * L.n + 1;
+  L.org.assign(3 * (size_t)L.ncell, 0);
+  L.ax.assign(9 * (size_t)L.ncell, 0);
+  int cnt = 0, S = p << depth;
+  for (int k = 0; k < m; k++)
+    for (int j = 0; j < m; j++)
+      for (int i = 0; i < m; i++) {
+        Frame f{{i * S, j * S, k * S}, {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
+        emit(L, cnt, f, S, depth);
+      }
+
+  // DOF numbering: first touch in (cell, local DOF) order, inner DOFs first,
+  // then Dirichlet DOFs (boundary nodes), each class counted separately --
+  // the result of the union-find marks and final pass of
+  // TFESpace3D::ConstructSpace (FESpace3D.C:459-2035, FE3DMapper.C:134-230).
+  const int nn = L.nn;
+  const int64_t nnodes = (int64_t)nn * nn * nn;
+  std::vector<int> node_dof(nnodes, -1);
+  L.gn.resize((size_t)nb * L.ncell);
+  int n_in = 0, n_dir = 0;
+  for (int c = 0; c < L.ncell; c++)
+    for (int b = 0; b < nb; b++) {
+      int i = b % (p + 1), j = (b / (p + 1)) % (p + 1), k = b / ((p + 1) * (p + 1));
+      int node = cell_node(L, c, i, j, k);
+      if (node_dof[node] == -1) {
+        int X = node % nn, Y = (node / nn) % nn, Z = node / (nn * nn);
+        bool bd = X == 0 || Y == 0 || Z == 0 || X == nn - 1 || Y == nn - 1 || Z == nn - 1;
+        node_dof[node] = bd ? -2 - (n_dir++) : n_in++;
+      }
+      L.gn[(size_t)c * nb + b] = node;
+    }
+  L.nact = n_in;
+  L.ndof = n_in + n_dir;
+  L.dof_node.assign(L.ndof, 0);
+  for (int64_t i = 0; i < nnodes; i++) {
+    int v = node_dof[i];
+    if (v == -1) continue;
+    int d = v >= 0 ? v : n_in + (-2 - v);
+    node_dof[i] = d;
+    L.dof_node[d] = (int)i;
+  }
+  for (auto &g : L.gn) g = node_dof[g];
+
+  // CSR pattern (TSquareStructure3D + Sort): an active row holds every DOF of
+  // the cells around its node, sorted; a Dirichlet row holds its diagonal.
+  // The cells around lattice node X along one axis span [X-p, X+p] when X is
+  // on a cell face plane, otherwise the single cell containing X.
+  std::vector<int> len(L.ndof);
+  auto range = [&](int X, int &lo, int &hi) {
+    if (X % p == 0) {
+      lo = std::max(0, X - p);
+      hi = std::min(nn - 1, X + p);
+    } else {
+      lo = p * (X / p);
+      hi = lo + p;
+    }
+  };
+  parallel_for(L.ndof, [&](int r) {
+    if (r >= L.nact) { len[r] = 1; return; }
+    int node = L.dof_node[r], X = node % nn, Y = (node / nn) % nn, Z = node / (nn * nn), a[3], b[3];
+    range(X, a[0], b[0]);
+    range(Y, a[1], b[1]);
+    range(Z, a[2], b[2]);
+    len[r] = (b[0] - a[0] + 1) * (b[1] - a[1] + 1) * (b[2] - a[2] + 1);
+  });
+  L.rowptr.assign(L.ndof + 1, 0);
+  for (int r = 0; r < L.ndof; r++) L.rowptr[r + 1] = L.rowptr[r] + len[r];
+  L.nnz = L.rowptr[L.ndof];
+  L.kcol.assign(L.nnz, 0);
+  L.diag.assign(L.ndof, 0);
+  parallel_for(L.ndof, [&](int r) {
+    int *out = &L.kcol[L.rowptr[r]];
+    if (r >= L.nact) {
+      out[0] = r;
+      L.diag[r] = L.rowptr[r];
+      return;
+    }
+    int node = L.dof_node[r], X = node % nn, Y = (node / nn) % nn, Z = node / (nn * nn), a[3], b[3];
+    range(X, a[0], b[0]);
+    range(Y, a[1], b[1]);
+    range(Z, a[2], b[2]);
+    int t = 0;
+    for (int z = a[2]; z <= b[2]; z++)
+      for (int y = a[1]; y <= b[1]; y++)
+        for (int x = a[0]; x <= b[0]; x++) out[t++] = node_dof[x + nn * (y + (int64_t)nn * z)];
+    std::sort(out, out + t);
+    L.diag[r] = (int)(std::lower_bound(out, out + t, r) - &L.kcol[0]);
+  });
+  build_colours(L, order);
+}
+
+// Grid transfer between coarse level and the next finer one: P(i,k) is the
+// value of coarse basis function k at fine node i -- what Prolongate
+// computes cell by cell with the prolongation matrices and the multiplicity
+// average (MGComponents3D.C:182-463, FEDatabase3D.C:1630-1790); the weights
+// are dyadic (Q1: 1, 1/2; Q2: 1, 3/4, 3/8, -1/8 per axis) so the products are
+// exact.  Restriction is R = P^T (DefectRestriction, MGComponents3D.C:706-989).
+void build_transfer(HostLevel &F, const HostLevel &C, int order) {
+  const int p = order, nnf = F.nn, nnc = C.nn, nc = C.n;
+  std::vector<int> cnode_dof((size_t)nnc * nnc * nnc, -1);
+  for (int d = 0; d < C.ndof; d++) cnode_dof[C.dof_node[d]] = d;
+  struct W { int idx[3]; double w[3]; int n; };
+  auto w1d = [&](int X, W &o) {
+    int c = std::min(X / (2 * p), nc - 1), t = X - 2 * p * c;
+    double xi = (double)t / p - 1.0;
+    double v[3];
+    if (p == 1) { v[0] = 0.5 - 0.5 * xi; v[1] = 0.5 + 0.5 * xi; }
+    else { v[0] = 0.5 * xi * (xi - 1); v[1] = 1 - xi * xi; v[2] = 0.5 * xi * (xi + 1); }
+    o.n = 0;
+    for (int k = 0; k <= p; k++)
+      if (v[k] != 0.0) { o.idx[o.n] = c * p + k; o.w[o.n] = v[k]; o.n++; }
+  };
+  std::vector<int> len(F.ndof);
+  std::vector<std::vector<std::pair<int, double>>> rows(F.ndof);
+  parallel_for(F.ndof, [&](int i) {
+    int node = F.dof_node[i], X = node % nnf, Y = (node / nnf) % nnf, Z = node / (nnf * nnf);
+    W a, b, c;
+    w1d(X, a); w1d(Y, b); w1d(Z, c);
+    auto &r = rows[i];
+    for (int kz = 0; kz < c.n; kz++)
+      for (int ky = 0; ky < b.n; ky++)
+        for (int kx = 0; kx < a.n; kx++) {
+          int cn = a.idx[kx] + nnc * (b.idx[ky] + nnc * c.idx[kz]);
+          r.push_back({cnode_dof[cn], a.w[kx] * b.w[ky] * c.w[kz]});
+        }
+    std::sort(r.begin(), r.end());
+  });
+  F.p_rowptr.assign(F.ndof + 1, 0);
+  for (int i = 0; i < F.ndof; i++) F.p_rowptr[i + 1] = F.p_rowptr[i] + (int)rows[i].size();
+  F.p_col.resize(F.p_rowptr[F.ndof]);
+  F.p_val.resize(F.p_rowptr[F.ndof]);
+  for (int i = 0; i < F.ndof; i++)
+    for (size_t t = 0; t < rows[i].size(); t++) {
+      F.p_col[F.p_rowptr[i] + t] = rows[i][t].first;
+      F.p_val[F.p_rowptr[i] + t] = rows[i][t].second;
+    }
+  // transpose
+  F.r_rowptr.assign(C.ndof + 1, 0);
+  for (int c : F.p_col) F.r_rowptr[c + 1]++;
+  for (int k = 0; k < C.ndof; k++) F.r_rowptr[k + 1] += F.r_rowptr[k];
+  F.r_col.resize(F.p_col.size());
+  F.r_val.resize(F.p_col.size());
+  std::vector<int> pos(F.r_rowptr.begin(), F.r_rowptr.end() - 1);
+  for (int i = 0; i < F.ndof; i++)
+    for (int t = F.p_rowptr[i]; t < F.p_rowptr[i + 1]; t++) {
+      int k = F.p_col[t], q = pos[k]++;
+      F.r_col[q] = i;
+      F.r_val[q] = F.p_val[t];
+    }
+}
+
+// CSR position of every local matrix entry (row DOF i, column DOF j of the
+// cell), -1 on Dirichlet rows -- the column search of AddLocalSqMatToGlobal
+// (AssembleMat3D.C:765-787) done once at setup.
+void build_slots(const HostLevel &L, int order, std::vector<int> &slots) {
+  const int nb = (order + 1) * (order + 1) * (order + 1);
+  slots.assign((size_t)L.ncell * nb * nb, -1);
+  parallel_for(L.ncell, [&](int c) {
+    const int *g = &L.gn[(size_t)c * nb];
+    for (int i = 0; i < nb; i++) {
+      int r = g[i];
+      if (r >= L.nact) continue;
+      const int *b = &L.kcol[L.rowptr[r]], *e = &L.kcol[L.rowptr[r + 1]];
+      for (int j = 0; j < nb; j++)
+        slots[((size_t)c * nb + i) * nb + j] = (int)(std::lower_bound(b, e, g[j]) - &L.kcol[0]);
+    }
+  });
+}
+
+// 8 colours by the parity of the cell's integer position: two cells of one
+// colour never share a node, so one colour can be scattered without atomics.
+void build_colours(HostLevel &L, int order) {
+  const int p = order;
+  std::vector<int> col(L.ncell);
+  std::vector<int> cnt(8, 0);
+  for (int c = 0; c < L.ncell; c++) {
+    int par = 0;
+    for (int d = 0; d < 3; d++) {
+      int s = L.ax[9 * c + d] + L.ax[9 * c + 3 + d] + L.ax[9 * c + 6 + d];  // sign of the axis along d
+      int lo = L.org[3 * c + d] + (s < 0 ? -p : 0);
+      par |= ((lo / p) & 1) << d;
+    }
+    col[c] = par;
+    cnt[par]++;
+  }
+  L.colour_start.assign(9, 0);
+  for (int k = 0; k < 8; k++) L.colour_start[k + 1] = L.colour_start[k] + cnt[k];
+  L.colour_cells.resize(L.ncell);
+  std::vector<int> pos(L.colour_start.begin(), L.colour_start.end() - 1);
+  for (int c = 0; c < L.ncell; c++) L.colour_cells[pos[col[c]]++] = c;
+}
+
+// Reference basis at the quadrature points, per point q:
+//   [w_q, xi, eta, zeta, then per basis b: phi, dphi/dxi, dphi/deta, dphi/dzeta]
+// Gauss rules in the point order of QuadFormulaHexa.C:62-120; tensor-product
+// Lagrange basis, x fastest (BF_C_H_Q1_3D.h, BF_C_H_Q2_3D.h).
+void basis_tables(int order, std::vector<double> &tab) {
+  static const double g3x[3] = {-0.7745966692414833770358530, 0, 0.7745966692414833770358530};
+  static const double g3w[3] = {0.5555555555555555555555558, 0.8888888888888888888888888,
+                                0.5555555555555555555555558};
+  static const double g4x[4] = {0.33998104358485626480266576, 0.86113631159405257522394649,
+                                -0.86113631159405257522394649, -0.33998104358485626480266576};
+  static const double g4w[4] = {0.652145154862546142626936051, 0.347854845137453857373063949,
+                                0.347854845137453857373063949, 0.652145154862546142626936051};
+  const int p = order, n1 = p + 2, nb = (p + 1) * (p + 1) * (p + 1), stride = 4 + 4 * nb;
+  const double *gx = p == 1 ? g3x : g4x, *gw = p == 1 ? g3w : g4w;
+  tab.assign((size_t)n1 * n1 * n1 * stride, 0.0);
+  auto lag = [&](double x, double *v, double *d) {
+    if (p == 1) { v[0] = 0.5 - 0.5 * x; v[1] = 0.5 + 0.5 * x; d[0] = -0.5; d[1] = 0.5; }
+    else {
+      v[0] = 0.5 * x * (x - 1); v[1] = 1 - x * x; v[2] = 0.5 * x * (x + 1);
+      d[0] = x - 0.5; d[1] = -2 * x; d[2] = x + 0.5;
+    }
+  };
+  int q = 0;
+  for (int qi = 0; qi < n1; qi++)
+    for (int qj = 0; qj < n1; qj++)
+      for (int qk = 0; qk < n1; qk++, q++) {
+        double xi = gx[qk], eta = gx[qj], zeta = gx[qi];
+        double *t = &tab[(size_t)q * stride];
+        t[0] = gw[qi] * gw[qj] * gw[qk];
+        t[1] = xi; t[2] = eta; t[3] = zeta;
+        double vx[3], dx[3], vy[3], dy[3], vz[3], dz[3];
+        lag(xi, vx, dx); lag(eta, vy, dy); lag(zeta, vz, dz);
+        int b = 0;
+        for (int k = 0; k <= p; k++)
+          for (int j = 0; j <= p; j++)
+            for (int i = 0; i <= p; i++, b++) {
+              t[4 + 4 * b + 0] = vx[i] * vy[j] * vz[k];
+              t[4 + 4 * b + 1] = dx[i] * vy[j] * vz[k];
+              t[4 + 4 * b + 2] = vx[i] * dy[j] * vz[k];
+              t[4 + 4 * b + 3] = vx[i] * vy[j] * dz[k];
+            }
+      }
+}
+
+}  // namespace cd3d

@@ -18,6 +18,10 @@ CD_CASES = {
                        co=dict(eps=1e-6, b=(-0.3, 0.7, 0.4), c=0.5, f_val=1.0, supg=1)),
     "q2_l2_galerkin_conv": dict(order=2, k0=1, levels=2,
                                 co=dict(eps=0.1, b=(1.0, 0.5, 0.25), c=1.0, f_kind=1)),
+    "unit_c1": dict(order=1, k0=2, levels=3, box=(1.0, 1.0, 1.0),
+                    co=dict(eps=1.0, f_kind=1)),
+    "unit_c2_n8": dict(order=2, k0=1, levels=3, box=(1.0, 1.0, 1.0),
+                       co=dict(eps=1e-3, b=(1.0, 0.5, 0.25), f_val=1.0, supg=1)),
 }
 
 TCD_CASE = dict(name="q1_tcd", order=1, k0=2, levels=2, tau=1e-3, nsteps=3,

@@ -21,9 +21,11 @@ import tempfile
 import numpy as np
 
 # The reference reads data/Wuerfel.GEO into the box [0,1] x [0,0.346] x [0,0.346]
-# (ReadGeo.C:1753-1755 scales parametrised y and z by 0.346); every fixture
-# lives on that box.
+# (ReadGeo.C:1753-1755 scales parametrised y and z by 0.346).  Cases marked
+# "unit" use oracle/data/unitcube.{GEO,PRM} instead: the same cube with
+# explicit vertex coordinates (KNPR 0), which the reader does not scale.
 BOX = (1.0, 0.346, 0.346)
+UNIT = (1.0, 1.0, 1.0)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
@@ -63,6 +65,13 @@ CASES = {
                                        P0=0.1, P1=1.0, P2=0.5, P3=0.25, P4=1.0, P5=1)),
     # TCD3D, Crank-Nicolson with the rotating convection field (configuration 3 problem)
     # (rotation centre and pulse centre placed inside the reference's box)
+    # on the unit cube: configuration 1 exactly, and the configuration 2
+    # problem (Q2, SUPG, eps 1e-3, b = (1, 0.5, 0.25), f = 1) at 8^3 cells
+    "unit_c1": ("cd", dict(geo="unit", ANSATZ_ORDER=1, LEVELS=3, UNIFORM_STEPS=2, DISCTYPE=1,
+                           P0=1.0, P1=0, P2=0, P3=0, P4=0, P5=1)),
+    "unit_c2_n8": ("cd", dict(geo="unit", ANSATZ_ORDER=2, LEVELS=3, UNIFORM_STEPS=1, DISCTYPE=2,
+                              CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                              P0=1e-3, P1=1.0, P2=0.5, P3=0.25, P4=0, P5=0, P6=1.0)),
     "q1_tcd": ("tcd", dict(ANSATZ_ORDER=1, LEVELS=2, UNIFORM_STEPS=2, DISCTYPE=1,
                            P0=1e-4, P4=0, P5=0, P6=0.0, P7=1, P8=3,
                            P9=0.3, P10=0.173, P11=0.173, P12=0.5, P13=0.173,
@@ -71,7 +80,13 @@ CASES = {
 
 
 def write_dat(path, overrides):
-    kv = dict(GEOFILE=f"{REF}/data/Wuerfel.GEO", BNDFILE=f"{REF}/data/Wuerfel.PRM",
+    overrides = dict(overrides)
+    if overrides.pop("geo", None) == "unit":
+        data = os.path.join(REPO, "oracle", "data")
+        geo, prm = os.path.join(data, "unitcube.GEO"), os.path.join(data, "unitcube.PRM")
+    else:
+        geo, prm = f"{REF}/data/Wuerfel.GEO", f"{REF}/data/Wuerfel.PRM"
+    kv = dict(GEOFILE=geo, BNDFILE=prm,
               MESH_TYPE=0, OUTFILE="ref.out", LOGFILE="ref.log", MEASURE_ERRORS=0,
               WRITE_VTK=0, WRITE_PS=0, RE_NR=1.0, USE_ISOPARAMETRIC=0)
     kv.update(SOLVER)
@@ -90,7 +105,7 @@ def run_case(name):
         if p.returncode:
             print(p.stdout[-3000:], p.stderr[-3000:])
             raise SystemExit(f"{name}: reference run failed ({p.returncode})")
-        out = {"params": np.array(repr(ov)), "box": np.array(BOX)}
+        out = {"params": np.array(repr(ov)), "box": np.array(UNIT if ov.get("geo") == "unit" else BOX)}
         levels = [tuple(map(int, re.findall(r"-?\d+", l)[:8]))
                   for l in p.stdout.splitlines() if l.startswith("LEVEL ")]
         for lev, nc, nbf, ndof, act, hb, ndir, nnz in levels:

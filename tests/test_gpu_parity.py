@@ -1,0 +1,178 @@
+"""Parity of the HIP path (through the C ABI) with the reference and the oracle.
+
+Bars: DOF maps, CSR pattern and row pointers bit-exact; assembled entries
+within 1e-12 of the row's largest entry; V-cycle probe 1e-12 relative;
+FGMRES the same iteration count as the reference and the solution within
+1e-10 relative L2; mixed precision: relative residual 1e-10 in FP64 and
+solution within 1e-9 of the FP64 solution."""
+import sys
+
+import numpy as np
+import pytest
+
+import parmoon_cmg_amd as pm
+from oracle import pyoracle as po
+from tests.cases import CD_CASES, REF_BOX, REF_SOLVER, TCD_CASE
+
+pytestmark = pytest.mark.gpu
+
+ENTRY_TOL = 1e-12
+SOL_TOL = 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def row_scaled_err(e, er, rp):
+    rowmax = np.maximum.reduceat(np.abs(er), rp[:-1])
+    return (np.abs(e - er) / np.repeat(rowmax, np.diff(rp))).max()
+
+
+_problems = {}
+
+
+def problem(name, golden):
+    if name not in _problems:
+        c = CD_CASES[name]
+        say(f"[gpu] create {name} order={c['order']} k0={c['k0']} levels={c['levels']}")
+        P = pm.Problem(c["order"], 1, c["k0"], c["levels"], box=c.get("box", REF_BOX))
+        P.assemble(pm.coeffs(**c["co"]))
+        _problems[name] = P
+    return _problems[name]
+
+
+@pytest.mark.parametrize("name", list(CD_CASES))
+def test_structure_bit_exact(name, golden):
+    d = golden(name)
+    P = problem(name, golden)
+    for l in range(CD_CASES[name]["levels"]):
+        nc, nbf, ndof, act, hb, ndir, nnz = d[f"L{l}_meta"].tolist()
+        i = P.info(l)
+        assert (i["ncell"], i["ndof"], i["nact"], i["nnz"]) == (nc, ndof, act, nnz)
+        assert np.array_equal(P.gn(l), d[f"L{l}_gn"])
+        assert np.abs(P.cells(l) - d[f"L{l}_cells"]).max() <= 1e-15
+        rp, kc, _ = P.csr(l)
+        assert np.array_equal(rp, d[f"L{l}_rowptr"]) and np.array_equal(kc, d[f"L{l}_kcol"])
+
+
+@pytest.mark.parametrize("name", list(CD_CASES))
+def test_assembly_matches_reference(name, golden):
+    d = golden(name)
+    P = problem(name, golden)
+    for l in range(CD_CASES[name]["levels"]):
+        rp, _, e = P.csr(l)
+        err = row_scaled_err(e, d[f"L{l}_entries"], rp)
+        say(f"[gpu] {name} level {l} entry err {err:.2e}")
+        assert err < ENTRY_TOL
+        rr = d[f"L{l}_rhs"]
+        assert np.abs(P.rhs(l) - rr).max() <= ENTRY_TOL * np.abs(rr).max()
+
+
+@pytest.mark.parametrize("name", list(CD_CASES))
+def test_vcycle_matches_reference(name, golden):
+    d = golden(name)
+    P = problem(name, golden)
+    sp = pm.solver_params(**REF_SOLVER)
+    z = P.vcycle(sp, d["vcycle_in"])
+    err = np.abs(z - d["vcycle_out"]).max() / np.abs(d["vcycle_out"]).max()
+    say(f"[gpu] {name} vcycle err {err:.2e}")
+    assert err < 1e-12
+
+
+@pytest.mark.parametrize("name", list(CD_CASES))
+def test_fgmres_matches_reference(name, golden):
+    d = golden(name)
+    P = problem(name, golden)
+    sp = pm.solver_params(**REF_SOLVER, rel_tol=1e-10)
+    x, st = P.solve(sp)
+    err = np.linalg.norm(x - d["sol"]) / np.linalg.norm(d["sol"])
+    say(f"[gpu] {name} its {st.iterations} (ref {int(d['its'])}) sol err {err:.2e} rel res {st.rfinal / st.r0:.2e}")
+    assert st.iterations == int(d["its"])
+    assert err < SOL_TOL
+    assert st.rfinal <= 1.0001e-10 * st.r0
+
+
+@pytest.mark.parametrize("name", list(CD_CASES))
+def test_fgmres_exact_coarse_matches_oracle(name, golden):
+    c = CD_CASES[name]
+    P = problem(name, golden)
+    H = po.Hierarchy(c["order"], 1, c["k0"], c["levels"], box=c.get("box", REF_BOX))
+    H.assemble(po.coeffs(**c["co"]))
+    b = H.rhs(c["levels"] - 1)
+    _, r0 = H.defect(c["levels"] - 1, np.zeros_like(b), b)
+    xo, its_o, _ = H.fgmres(po.solver(**dict(REF_SOLVER, coarse_smoother=17, res_norm_min=1e-10 * r0,
+                                             red_factor=0.0)), b)
+    x, st = P.solve(pm.solver_params(**dict(REF_SOLVER, coarse_smoother=17), rel_tol=1e-10))
+    err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    say(f"[gpu] {name} exact coarse: its {st.iterations} (oracle {its_o}) err {err:.2e}")
+    assert st.iterations == its_o
+    assert err < SOL_TOL
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_transfers_and_matvec_match_oracle(order):
+    say(f"[gpu] transfers order {order}")
+    P = pm.Problem(order, 1, 1, 3, box=REF_BOX)
+    H = po.Hierarchy(order, 1, 1, 3, box=REF_BOX)
+    co = dict(eps=0.01, b=(1.0, -0.5, 0.3), c=0.2, f_val=1.0, supg=1)
+    P.assemble(pm.coeffs(**co))
+    H.assemble(po.coeffs(**co))
+    rng = np.random.default_rng(7)
+    for l in (1, 2):
+        nc, nf = H.info(l - 1)["ndof"], H.info(l)["ndof"]
+        u, v = rng.standard_normal(nc), rng.standard_normal(nf)
+        assert np.abs(P.prolongate(l, u) - H.prolongate(l, u)).max() < 1e-14 * np.abs(u).max() * 8
+        rg, ro = P.restrict(l, v), H.restrict(l, v)
+        assert np.abs(rg - ro).max() < 1e-13 * np.abs(ro).max()
+        x = rng.standard_normal(nf)
+        yg, yo = P.matvec(l, x), H.matvec(l, x)
+        assert np.abs(yg - yo).max() < 1e-12 * np.abs(yo).max()
+
+
+def test_tcd_crank_nicolson_matches_reference(golden):
+    c = TCD_CASE
+    d = golden(c["name"])
+    say("[gpu] tcd")
+    P = pm.Problem(c["order"], 1, c["k0"], c["levels"], box=REF_BOX)
+    co = pm.coeffs(**c["co"])
+    u0 = P.interp_pulse(c["levels"] - 1, co)
+    sp = pm.solver_params(**REF_SOLVER, rel_tol=0.0, res_norm_min=1e-30, red_factor=1e-10)
+    _, states, st = P.tcd_run(co, sp, 0.0, c["tau"], c["nsteps"], u0)
+    for k in range(c["nsteps"]):
+        ref = d[f"u{k + 1}"]
+        err = np.abs(states[k] - ref).max() / np.abs(ref).max()
+        say(f"[gpu] tcd step {k + 1} err {err:.2e}")
+        assert err < SOL_TOL
+
+
+@pytest.mark.parametrize("name", ["q2_l2_supg", "q1_l3_supg", "q1_c1"])
+def test_mixed_precision_against_fp64(name, golden):
+    P = problem(name, golden)
+    x64, st64 = P.solve(pm.solver_params(**dict(REF_SOLVER, coarse_smoother=17), rel_tol=1e-10))
+    xm, stm = P.solve(pm.solver_params(**dict(REF_SOLVER, coarse_smoother=17), rel_tol=1e-10,
+                                       precision=pm.MIXED))
+    err = np.linalg.norm(xm - x64) / np.linalg.norm(x64)
+    say(f"[gpu] {name} mixed its {stm.iterations} vs fp64 {st64.iterations}; rel res {stm.rfinal / stm.r0:.2e};"
+        f" err vs fp64 {err:.2e}")
+    assert stm.rfinal <= 1.0001e-10 * stm.r0
+    assert err < 1e-9
+
+
+@pytest.mark.parametrize("order,k0,levels", [(1, 2, 4), (2, 2, 3)])
+def test_structure_closed_forms_larger(order, k0, levels):
+    # size-independent check at sizes the reference fixtures do not cover
+    P = pm.Problem(order, 1, k0, levels, box=(1.0, 1.0, 1.0))
+    for l in range(levels):
+        n = 2 ** (k0 + l)
+        i = P.info(l)
+        if order == 1:
+            assert i["nnz"] == (3 * (n - 1)) ** 3 + (n + 1) ** 3 - (n - 1) ** 3
+        else:
+            assert i["nnz"] == (8 * n - 5) ** 3 + (2 * n + 1) ** 3 - (2 * n - 1) ** 3
+    rp, kc, _ = P.csr(levels - 1)
+    assert np.all(np.diff(rp) > 0)
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    same_row = rows[1:] == rows[:-1]
+    assert np.all(kc[1:][same_row] > kc[:-1][same_row])  # sorted, unique columns
