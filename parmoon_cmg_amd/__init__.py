@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcd3d_hip.so")
+LIB_PATH = os.environ.get("CD3D_LIB") or os.path.join(HERE, "libcd3d_hip.so")
 
 CD3D_OK, CD3D_ERR_ARG, CD3D_ERR_HIP, CD3D_ERR_NOMEM = 0, 1, 2, 3
 CD3D_NOT_CONVERGED, CD3D_DIVERGED, CD3D_NONFINITE, CD3D_ERR_STATE = 4, 5, 6, 7

@@ -119,7 +119,7 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
     upload(P, D.rowptr, H.rowptr);
     upload(P, D.col, H.kcol);
     upload(P, D.diag, H.diag);
-    for (double **v : {&D.sol, &D.rhs, &D.aux, &D.tmp}) {
+    for (double **v : {&D.sol, &D.rhs, &D.aux, &D.tmp, &D.rhs_asm}) {
       *v = dalloc<double>(P, H.ndof);
       HIPCHK(hipMemsetAsync(*v, 0, sizeof(double) * H.ndof, P.stream));
     }
@@ -211,7 +211,7 @@ static void check_coeffs(const cd3d_coeffs &co) {
 }
 
 // assemble level l: what 0 -> conv-diff into `val`, 1 -> mass into `val`;
-// rhs into D.rhs (TAssembleMat3D::Reset + Assemble3D, AssembleMat3D.C:204,230)
+// rhs into D.rhs_asm (TAssembleMat3D::Reset + Assemble3D, AssembleMat3D.C:204,230)
 static void assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int what, double *val) {
   DevLevel &D = P.dl[l];
   AsmParams ap{};
@@ -235,10 +235,10 @@ static void assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int wh
   if (co.supg && co.conv_kind == 0)
     ap.delta_const = sdfem_delta_host(hconv_host(D.h, co.b), co.eps, co.b, P.mesh.order);
   HIPCHK(hipMemsetAsync(val, 0, sizeof(double) * D.nnz, P.stream));
-  HIPCHK(hipMemsetAsync(D.rhs, 0, sizeof(double) * D.ndof, P.stream));
+  HIPCHK(hipMemsetAsync(D.rhs_asm, 0, sizeof(double) * D.ndof, P.stream));
   for (int k = 0; k < 8; k++)
-    k_assemble(P.stream, D, P.tab, ap, val, D.rhs, D.colour_start[k], D.colour_start[k + 1] - D.colour_start[k]);
-  k_dirichlet_rows(P.stream, D, val, D.rhs);
+    k_assemble(P.stream, D, P.tab, ap, val, D.rhs_asm, D.colour_start[k], D.colour_start[k + 1] - D.colour_start[k]);
+  k_dirichlet_rows(P.stream, D, val, D.rhs_asm);
   HIPCHK(hipGetLastError());
 }
 
@@ -246,12 +246,9 @@ static void refresh_coarse_and_f32(cd3d_problem &P) {
   DevLevel &C = P.dl[0];
   const int na = C.nact;
   if (na > 0) {
-    double *scratch = dalloc<double>(P, (size_t)2 * na * na + na);
-    k_invert_active(P.stream, C, scratch, P.cinv);
-    HIPCHK(hipStreamSynchronize(P.stream));
-    HIPCHK(hipFree(scratch));
-    P.allocs.pop_back();
-    P.dev_bytes -= ((size_t)2 * na * na + na) * sizeof(double);
+    if (!P.gj_scratch) P.gj_scratch = dalloc<double>(P, (size_t)gj_scratch_doubles(na));
+    k_invert_active(P.stream, C, P.gj_scratch, P.cinv);
+    HIPCHK(hipGetLastError());
   }
   P.cinv_valid = true;
   for (auto &D : P.dl) k_to_f32(P.stream, D.nnz, D.val, D.val32);
@@ -611,7 +608,7 @@ cd3d_status cd3d_get_rhs(cd3d_problem *p, int level, double *rhs) {
   if (!p || !rhs || level < 0 || level >= (int)p->hl.size()) return CD3D_ERR_ARG;
   std::lock_guard<std::mutex> g(p->mu);
   return run(p, [&]() {
-    HIPCHK(hipMemcpyAsync(rhs, p->dl[level].rhs, sizeof(double) * p->dl[level].ndof, hipMemcpyDeviceToHost,
+    HIPCHK(hipMemcpyAsync(rhs, p->dl[level].rhs_asm, sizeof(double) * p->dl[level].ndof, hipMemcpyDeviceToHost,
                           p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
   });
@@ -637,7 +634,7 @@ cd3d_status cd3d_assemble(cd3d_problem *p, const cd3d_coeffs *co) {
       DevLevel &D = p->dl[l];
       assemble_level(*p, l, *co, 0, D.val);
       // SystemCD3D.C:330: Dirichlet part of the rhs into the level solution
-      k_copy_range(p->stream, D.nact, D.ndof, D.rhs, D.sol);
+      k_copy_range(p->stream, D.nact, D.ndof, D.rhs_asm, D.sol);
     }
     refresh_coarse_and_f32(*p);
     HIPCHK(hipStreamSynchronize(p->stream));
@@ -732,7 +729,7 @@ cd3d_status cd3d_solve(cd3d_problem *p, const cd3d_solver_params *sp, const doub
     ensure_krylov(*p, std::min(sp->restart, sp->maxit));
     const double t0 = now_ms();
     if (rhs) HIPCHK(hipMemcpyAsync(p->bs, rhs, sizeof(double) * N, hipMemcpyHostToDevice, p->stream));
-    else k_copy(p->stream, N, F.rhs, p->bs);
+    else k_copy(p->stream, N, F.rhs_asm, p->bs);
     // initial guess: 0 on active DOFs, Dirichlet values from the rhs
     // (SystemCD3D.C:330), or the caller's guess with its Dirichlet part reset
     if (use_guess && sol) HIPCHK(hipMemcpyAsync(p->xs, sol, sizeof(double) * N, hipMemcpyHostToDevice, p->stream));
@@ -807,8 +804,8 @@ cd3d_status cd3d_tcd_run(cd3d_problem *p, const cd3d_coeffs *co0, const cd3d_sol
     co.t = t0;
     for (int l = 0; l <= Lf; l++) assemble_level(*p, l, co, 1, p->dl[l].val);
     HIPCHK(hipMemcpyAsync(p->xs, u, sizeof(double) * N, hipMemcpyHostToDevice, p->stream));
-    k_copy_range(p->stream, NA, N, F.rhs, p->xs);
-    k_copy(p->stream, N, F.rhs, oldrhs);
+    k_copy_range(p->stream, NA, N, F.rhs_asm, p->xs);
+    k_copy(p->stream, N, F.rhs_asm, oldrhs);
     double t = t0;
     int total_its = 0;
     for (int m = 0; m < nsteps; m++) {
@@ -817,24 +814,24 @@ cd3d_status cd3d_tcd_run(cd3d_problem *p, const cd3d_coeffs *co0, const cd3d_sol
       co.t = t;
       // AssembleARhs at t_new (SystemTCD3D.C:204-226)
       for (int l = 0; l <= Lf; l++) assemble_level(*p, l, co, 0, p->dl[l].aconv);
-      k_copy_range(p->stream, NA, N, F.rhs, p->xs);
+      k_copy_range(p->stream, NA, N, F.rhs_asm, p->xs);
       // AssembleSystMat (SystemTCD3D.C:228-294)
       HIPCHK(hipMemsetAsync(p->bs, 0, sizeof(double) * N, p->stream));
       k_axpy(p->stream, NA, tau * th3, oldrhs, p->bs);
-      k_axpy(p->stream, NA, tau * th4, F.rhs, p->bs);
+      k_axpy(p->stream, NA, tau * th4, F.rhs_asm, p->bs);
       k_matadd(p->stream, p->hl[Lf].rowptr[NA], F.val, F.aconv, -tau * th2);
       double gamma = -tau * th2;
       HIPCHK(hipMemsetAsync(def, 0, sizeof(double) * N, p->stream));
       k_matvec_active(p->stream, F, oldsol, def);
       k_axpy(p->stream, NA, 1.0, def, p->bs);
-      k_copy_range(p->stream, NA, N, F.rhs, p->bs);
-      k_copy_range(p->stream, NA, N, F.rhs, p->xs);
+      k_copy_range(p->stream, NA, N, F.rhs_asm, p->bs);
+      k_copy_range(p->stream, NA, N, F.rhs_asm, p->xs);
       for (int l = 0; l <= Lf; l++) {
         DevLevel &D = p->dl[l];
         k_matadd(p->stream, p->hl[l].rowptr[D.nact], D.val, D.aconv, l == Lf ? -gamma + tau * th1 : tau * th1);
       }
       gamma = tau * th1;
-      k_copy(p->stream, N, F.rhs, oldrhs);
+      k_copy(p->stream, N, F.rhs_asm, oldrhs);
       refresh_coarse_and_f32(*p);
       // Solve (SystemTCD3D.C:317-376)
       double r0;

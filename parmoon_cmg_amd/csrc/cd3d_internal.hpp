@@ -52,6 +52,7 @@ struct DevLevel {
   double *val = nullptr;
   float *val32 = nullptr;
   double *sol = nullptr, *rhs = nullptr, *aux = nullptr, *tmp = nullptr;
+  double *rhs_asm = nullptr;  // assembled rhs (the multigrid works in `rhs`)
   float *fsol = nullptr, *frhs = nullptr, *faux = nullptr, *ftmp = nullptr;
   // TCD3D: conv-diff matrix next to the system (mass) matrix in `val`
   double *aconv = nullptr;
@@ -85,6 +86,7 @@ struct Problem {
   bool assembled = false;
   // exact coarse solve: explicit inverse of the coarsest matrix (f64)
   double *cinv = nullptr;
+  double *gj_scratch = nullptr;
   bool cinv_valid = false;
   // FGMRES workspace
   int krylov_restart = 0;
@@ -153,8 +155,9 @@ struct AsmParams {
 void k_assemble(hipStream_t s, const DevLevel &L, const double *tab, const AsmParams &ap, double *val,
                 double *rhs, int colour_begin, int colour_count);
 void k_dirichlet_rows(hipStream_t s, const DevLevel &L, double *val, double *rhs);
-// scratch: 2 na^2 + na doubles; inv: na^2 doubles
+// scratch: gj_scratch_doubles(na) doubles; inv: na^2 doubles
 void k_invert_active(hipStream_t s, const DevLevel &L, double *scratch, double *inv);
+int64_t gj_scratch_doubles(int na);
 void k_multi_axpy(hipStream_t s, int n, int m, const double *coef, const double *Z, int64_t ldz, double *x);
 void k_copy_range(hipStream_t s, int b, int e, const double *src, double *dst);
 

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <algorithm>
+#include <cstdlib>
 
 #include "cd3d_internal.hpp"
 
@@ -36,7 +38,9 @@ __device__ inline T group_sum(T v) {
 //   kRestrict   out = R x on active coarse rows, 0 on the others (DefectRestriction + CorrectDefect)
 //   kProlongUpd out += w (P x) on active fine rows               (Prolongate + CorrectNodes + Update)
 //   kActiveSpmv out = A x on active rows only                   (MatVectActive)
-template <int LANES, int MODE, typename T, typename V>
+// UNROLL independent (column, value, x) load chains per lane and iteration
+// keep more HBM requests in flight.
+template <int LANES, int UNROLL, int MODE, typename T, typename V>
 __global__ __launch_bounds__(kBlock) void csr_rows(int n, int nact, const int *__restrict__ rowptr,
                                                    const int *__restrict__ col, const V *__restrict__ val,
                                                    const T *__restrict__ x, const T *__restrict__ f,
@@ -50,14 +54,24 @@ __global__ __launch_bounds__(kBlock) void csr_rows(int n, int nact, const int *_
   }
   const int b = rowptr[row], e = rowptr[row + 1];
   T s = T(0), diag = T(0);
-  for (int j = b + lane; j < e; j += LANES) {
-    const int c = col[j];
-    const T a = (T)val[j];
-    if (MODE == kJacobi) {
-      if (c == row) diag = a;
-      else s += a * x[c];
-    } else {
-      s += a * x[c];
+  for (int j0 = b + lane; j0 < e; j0 += LANES * UNROLL) {
+    int c[UNROLL];
+    T a[UNROLL];
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++) {
+      const int j = j0 + u * LANES;
+      c[u] = j < e ? col[j] : row;
+      a[u] = j < e ? (T)val[j] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++) {
+      const T xv = x[c[u]];
+      if (MODE == kJacobi) {
+        if (c[u] == row) diag += a[u];
+        else s += a[u] * xv;
+      } else {
+        s += a[u] * xv;
+      }
     }
   }
   s = group_sum<LANES>(s);
@@ -69,27 +83,42 @@ __global__ __launch_bounds__(kBlock) void csr_rows(int n, int nact, const int *_
   else if (MODE == kProlongUpd) out[row] += omega * s;
 }
 
+// lanes per row from the mean row length (overridable for tuning through
+// CD3D_CSR_LANES / CD3D_CSR_UNROLL)
+inline void row_shape(int64_t nnz, int n, int &lanes, int &unroll) {
+  const double m = n > 0 ? (double)nnz / n : 1;
+  lanes = m <= 12 ? 4 : m <= 40 ? 8 : m <= 96 ? 16 : 32;
+  unroll = m <= 12 ? 1 : 4;
+  static const int env_l = getenv("CD3D_CSR_LANES") ? atoi(getenv("CD3D_CSR_LANES")) : 0;
+  static const int env_u = getenv("CD3D_CSR_UNROLL") ? atoi(getenv("CD3D_CSR_UNROLL")) : 0;
+  if (env_l == 4 || env_l == 8 || env_l == 16 || env_l == 32 || env_l == 64) lanes = env_l;
+  if (env_u == 1 || env_u == 2 || env_u == 4) unroll = env_u;
+}
+
+template <int LANES, int MODE, typename T, typename V>
+void launch_lanes(hipStream_t st, int unroll, int grid, int n, int nact, const int *rp, const int *col, const V *val,
+                  const T *x, const T *f, T *out, T omega) {
+  if (unroll == 4) csr_rows<LANES, 4, MODE, T, V><<<grid, kBlock, 0, st>>>(n, nact, rp, col, val, x, f, out, omega);
+  else if (unroll == 2)
+    csr_rows<LANES, 2, MODE, T, V><<<grid, kBlock, 0, st>>>(n, nact, rp, col, val, x, f, out, omega);
+  else csr_rows<LANES, 1, MODE, T, V><<<grid, kBlock, 0, st>>>(n, nact, rp, col, val, x, f, out, omega);
+}
+
 template <int MODE, typename T, typename V>
-void launch_rows(hipStream_t st, int lanes, int n, int nact, const int *rp, const int *col, const V *val, const T *x,
-                 const T *f, T *out, T omega) {
+void launch_rows(hipStream_t st, int64_t nnz, int n, int nact, const int *rp, const int *col, const V *val,
+                 const T *x, const T *f, T *out, T omega) {
   if (n <= 0) return;
+  int lanes, unroll;
+  row_shape(nnz, n, lanes, unroll);
   const int64_t threads = (int64_t)n * lanes;
   const int grid = (int)((threads + kBlock - 1) / kBlock);
   switch (lanes) {
-    case 4: csr_rows<4, MODE, T, V><<<grid, kBlock, 0, st>>>(n, nact, rp, col, val, x, f, out, omega); break;
-    case 8: csr_rows<8, MODE, T, V><<<grid, kBlock, 0, st>>>(n, nact, rp, col, val, x, f, out, omega); break;
-    case 16: csr_rows<16, MODE, T, V><<<grid, kBlock, 0, st>>>(n, nact, rp, col, val, x, f, out, omega); break;
-    default: csr_rows<32, MODE, T, V><<<grid, kBlock, 0, st>>>(n, nact, rp, col, val, x, f, out, omega); break;
+    case 4: launch_lanes<4, MODE, T, V>(st, unroll, grid, n, nact, rp, col, val, x, f, out, omega); break;
+    case 8: launch_lanes<8, MODE, T, V>(st, unroll, grid, n, nact, rp, col, val, x, f, out, omega); break;
+    case 16: launch_lanes<16, MODE, T, V>(st, unroll, grid, n, nact, rp, col, val, x, f, out, omega); break;
+    case 32: launch_lanes<32, MODE, T, V>(st, unroll, grid, n, nact, rp, col, val, x, f, out, omega); break;
+    default: launch_lanes<64, MODE, T, V>(st, unroll, grid, n, nact, rp, col, val, x, f, out, omega); break;
   }
-}
-
-// lanes per row from the mean row length: about four entries per lane
-inline int lanes_for(int64_t nnz, int n) {
-  double m = n > 0 ? (double)nnz / n : 1;
-  if (m <= 12) return 4;
-  if (m <= 40) return 8;
-  if (m <= 96) return 16;
-  return 32;
 }
 
 template <typename T>
@@ -366,9 +395,10 @@ __global__ __launch_bounds__(kBlock) void assemble_k(int ncells, const int *__re
   constexpr int STRIDE = 4 + 4 * NB;
   constexpr int CPB = kBlock / NB;
   __shared__ double s_sh[CPB][NB];
-  const int t = threadIdx.x, lc = t / NB, i = t % NB;
-  const int ci = blockIdx.x * CPB + lc;
-  const bool live = lc < CPB && ci < ncells;
+  const int t = threadIdx.x, lc0 = t / NB, i = t % NB;
+  const int lc = lc0 < CPB ? lc0 : 0;  // tail threads of a Q2 block (t >= 243) work on cell slot 0, write nothing
+  const int ci = blockIdx.x * CPB + lc0;
+  const bool live = lc0 < CPB && ci < ncells;
   const int c = live ? list[ci] : 0;
   int4 cg = live ? cells[c] : make_int4(0, 0, 0, 0);
   int dax[3], sg[3];
@@ -453,61 +483,102 @@ __global__ __launch_bounds__(kBlock) void assemble_k(int ncells, const int *__re
 }
 
 // Explicit inverse of the active block of the coarsest matrix by Gauss-Jordan
-// elimination with partial pivoting, one workgroup; M is na x 2na scratch
-// ([A_AA | I]), fac na scratch.  Ties in the pivot search go to the lower row
-// so the result is deterministic.
-__global__ __launch_bounds__(1024) void gj_invert_k(int na, const int *__restrict__ rp, const int *__restrict__ col,
+// elimination with partial pivoting, spread over many workgroups: one launch
+// per pivot column.  Rows are never swapped or scaled during the sweep: the
+// pivot row of column k (the unused row with the largest |M[r][k]|, ties to
+// the lower index) is recorded in perm[k]; every other row subtracts its
+// multiple of the (unmodified) pivot row.  At the end row perm[k] holds
+// d_k e_k in the left half, and row k of the inverse is its right half / d_k.
+// M is na x 2na ([A_AA | I]); one wavefront per row, lanes over columns.
+constexpr int kGjWaves = kBlock / 64;
+
+__global__ __launch_bounds__(kBlock) void gj_init_k(int na, const int *__restrict__ rp, const int *__restrict__ col,
                                                     const double *__restrict__ val, double *__restrict__ M,
-                                                    double *__restrict__ fac, double *__restrict__ inv) {
-  __shared__ double sv[1024];
-  __shared__ int si[1024];
-  const int tid = threadIdx.x, nt = blockDim.x, w2 = 2 * na;
-  for (int64_t e = tid; e < (int64_t)na * w2; e += nt) M[e] = 0.0;
-  __syncthreads();
-  for (int r = tid; r < na; r += nt) {
-    for (int j = rp[r]; j < rp[r + 1]; j++)
-      if (col[j] < na) M[(int64_t)r * w2 + col[j]] = val[j];
-    M[(int64_t)r * w2 + na + r] = 1.0;
-  }
-  __syncthreads();
-  for (int k = 0; k < na; k++) {
-    double best = -1.0;
-    int bi = k;
-    for (int r = k + tid; r < na; r += nt) {
-      const double v = fabs(M[(int64_t)r * w2 + k]);
+                                                    int *__restrict__ used, double *__restrict__ pv,
+                                                    int *__restrict__ pi) {
+  const int w2 = 2 * na, wave = blockIdx.x * kGjWaves + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  __shared__ double sv[kGjWaves];
+  __shared__ int si[kGjWaves];
+  double best = -1.0;
+  int bi = na;
+  for (int r = wave; r < na; r += gridDim.x * kGjWaves) {
+    for (int j = lane; j < w2; j += 64) M[(int64_t)r * w2 + j] = (j == na + r) ? 1.0 : 0.0;
+    __syncwarp();
+    if (lane == 0) {
+      used[r] = 0;
+      for (int t = rp[r]; t < rp[r + 1]; t++)
+        if (col[t] < na) M[(int64_t)r * w2 + col[t]] = val[t];
+      const double v = fabs(M[(int64_t)r * w2]);
       if (v > best) { best = v; bi = r; }
     }
-    sv[tid] = best;
-    si[tid] = bi;
-    __syncthreads();
-    for (int s = nt / 2; s > 0; s >>= 1) {
-      if (tid < s && (sv[tid + s] > sv[tid] || (sv[tid + s] == sv[tid] && si[tid + s] < si[tid]))) {
-        sv[tid] = sv[tid + s];
-        si[tid] = si[tid + s];
-      }
-      __syncthreads();
-    }
-    const int p = si[0];
-    if (p != k)
-      for (int j = tid; j < w2; j += nt) {
-        const double t = M[(int64_t)k * w2 + j];
-        M[(int64_t)k * w2 + j] = M[(int64_t)p * w2 + j];
-        M[(int64_t)p * w2 + j] = t;
-      }
-    __syncthreads();
-    const double ip = 1.0 / M[(int64_t)k * w2 + k];
-    __syncthreads();
-    for (int j = k + tid; j < w2; j += nt) M[(int64_t)k * w2 + j] *= ip;
-    for (int r = tid; r < na; r += nt) fac[r] = r == k ? 0.0 : M[(int64_t)r * w2 + k];
-    __syncthreads();
-    const int wk = w2 - k;
-    for (int64_t e = tid; e < (int64_t)na * wk; e += nt) {
-      const int r = (int)(e / wk), j = k + (int)(e % wk);
-      M[(int64_t)r * w2 + j] -= fac[r] * M[(int64_t)k * w2 + j];
-    }
-    __syncthreads();
   }
-  for (int64_t e = tid; e < (int64_t)na * na; e += nt) inv[e] = M[(e / na) * w2 + na + (e % na)];
+  if (lane == 0) { sv[threadIdx.x >> 6] = best; si[threadIdx.x >> 6] = bi; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kGjWaves; k++)
+      if (sv[k] > sv[0] || (sv[k] == sv[0] && si[k] < si[0])) { sv[0] = sv[k]; si[0] = si[k]; }
+    pv[blockIdx.x] = sv[0];
+    pi[blockIdx.x] = si[0];
+  }
+}
+
+__device__ inline int gj_pivot(const double *__restrict__ pv, const int *__restrict__ pi, int nparts) {
+  double best = -1.0;
+  int bi = 1 << 30;
+  for (int t = 0; t < nparts; t++)
+    if (pv[t] > best || (pv[t] == best && pi[t] < bi)) { best = pv[t]; bi = pi[t]; }
+  return bi;
+}
+
+__global__ __launch_bounds__(kBlock) void gj_step_k(int na, int k, double *__restrict__ M, int *__restrict__ used,
+                                                    int *__restrict__ perm, const double *__restrict__ pv_in,
+                                                    const int *__restrict__ pi_in, double *__restrict__ pv_out,
+                                                    int *__restrict__ pi_out) {
+  const int w2 = 2 * na, wave = blockIdx.x * kGjWaves + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int p = gj_pivot(pv_in, pi_in, gridDim.x);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    perm[k] = p;
+    used[p] = 1;
+  }
+  __shared__ double sv[kGjWaves];
+  __shared__ int si[kGjWaves];
+  const double *__restrict__ prow = M + (int64_t)p * w2;
+  const double pk = prow[k];
+  double best = -1.0;
+  int bi = na;
+  for (int r = wave; r < na; r += gridDim.x * kGjWaves) {
+    if (r == p) continue;
+    double *__restrict__ row = M + (int64_t)r * w2;
+    const double f = row[k] / pk;
+    if (f != 0.0)
+      for (int j = k + 1 + lane; j < w2; j += 64) row[j] -= f * prow[j];
+    __syncwarp();
+    if (lane == 0) {
+      row[k] = 0.0;
+      if (k + 1 < na && !used[r]) {
+        const double v = fabs(row[k + 1]);
+        if (v > best) { best = v; bi = r; }
+      }
+    }
+  }
+  if (lane == 0) { sv[threadIdx.x >> 6] = best; si[threadIdx.x >> 6] = bi; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int t = 1; t < kGjWaves; t++)
+      if (sv[t] > sv[0] || (sv[t] == sv[0] && si[t] < si[0])) { sv[0] = sv[t]; si[0] = si[t]; }
+    pv_out[blockIdx.x] = sv[0];
+    pi_out[blockIdx.x] = si[0];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void gj_final_k(int na, const double *__restrict__ M,
+                                                     const int *__restrict__ perm, double *__restrict__ inv) {
+  const int w2 = 2 * na, wave = blockIdx.x * kGjWaves + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  for (int k = wave; k < na; k += gridDim.x * kGjWaves) {
+    const double *row = M + (int64_t)perm[k] * w2;
+    const double d = 1.0 / row[k];
+    for (int j = lane; j < na; j += 64) inv[(int64_t)k * na + j] = row[na + j] * d;
+  }
 }
 
 // x += sum_k coef[k] Z_k  (FGMRES update x += Z y, UpdateGmresIterate)
@@ -539,15 +610,15 @@ __global__ void dirichlet_rows_k(int n, int nact, const int *__restrict__ rp, do
 // ---------------------------------------------------------------- launchers
 
 void k_spmv(hipStream_t s, const DevLevel &L, const double *x, double *y) {
-  launch_rows<kSpmv, double, double>(s, lanes_for(L.nnz, L.ndof), L.ndof, L.nact, L.rowptr, L.col, L.val, x,
+  launch_rows<kSpmv, double, double>(s, L.nnz, L.ndof, L.nact, L.rowptr, L.col, L.val, x,
                                      nullptr, y, 0.0);
 }
 void k_defect(hipStream_t s, const DevLevel &L, const double *x, const double *f, double *d) {
-  launch_rows<kDefect, double, double>(s, lanes_for(L.nnz, L.ndof), L.ndof, L.nact, L.rowptr, L.col, L.val, x, f,
+  launch_rows<kDefect, double, double>(s, L.nnz, L.ndof, L.nact, L.rowptr, L.col, L.val, x, f,
                                        d, 0.0);
 }
 void k_jacobi(hipStream_t s, const DevLevel &L, const double *x, const double *f, double *out, double omega) {
-  launch_rows<kJacobi, double, double>(s, lanes_for(L.nnz, L.ndof), L.ndof, L.nact, L.rowptr, L.col, L.val, x, f,
+  launch_rows<kJacobi, double, double>(s, L.nnz, L.ndof, L.nact, L.rowptr, L.col, L.val, x, f,
                                        out, omega);
 }
 void k_jacobi_zero(hipStream_t s, const DevLevel &L, const double *f, double *out, double omega) {
@@ -555,25 +626,25 @@ void k_jacobi_zero(hipStream_t s, const DevLevel &L, const double *f, double *ou
                                                                          f, out, omega);
 }
 void k_matvec_active(hipStream_t s, const DevLevel &L, const double *x, double *y) {
-  launch_rows<kActiveSpmv, double, double>(s, lanes_for(L.nnz, L.ndof), L.ndof, L.nact, L.rowptr, L.col, L.val, x,
+  launch_rows<kActiveSpmv, double, double>(s, L.nnz, L.ndof, L.nact, L.rowptr, L.col, L.val, x,
                                            nullptr, y, 0.0);
 }
 void k_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, const double *fine, double *coarse) {
-  launch_rows<kRestrict, double, double>(s, lanes_for(F.r_nnz, C.ndof), C.ndof, C.nact, F.r_rowptr, F.r_col,
+  launch_rows<kRestrict, double, double>(s, F.r_nnz, C.ndof, C.nact, F.r_rowptr, F.r_col,
                                          F.r_val, fine, nullptr, coarse, 0.0);
 }
 void k_prolong_update(hipStream_t s, const DevLevel &F, const DevLevel &C, const double *coarse, double *x,
                       double damp) {
-  launch_rows<kProlongUpd, double, double>(s, lanes_for(F.p_nnz, F.ndof), F.ndof, F.nact, F.p_rowptr, F.p_col,
+  launch_rows<kProlongUpd, double, double>(s, F.p_nnz, F.ndof, F.nact, F.p_rowptr, F.p_col,
                                            F.p_val, coarse, nullptr, x, damp);
 }
 void k_prolong(hipStream_t s, const DevLevel &F, const DevLevel &C, const double *coarse, double *fine) {
   (void)C;
-  launch_rows<kSpmv, double, double>(s, lanes_for(F.p_nnz, F.ndof), F.ndof, F.nact, F.p_rowptr, F.p_col, F.p_val,
+  launch_rows<kSpmv, double, double>(s, F.p_nnz, F.ndof, F.nact, F.p_rowptr, F.p_col, F.p_val,
                                      coarse, nullptr, fine, 0.0);
 }
 void k_jacobi32(hipStream_t s, const DevLevel &L, const float *x, const float *f, float *out, float omega) {
-  launch_rows<kJacobi, float, float>(s, lanes_for(L.nnz, L.ndof), L.ndof, L.nact, L.rowptr, L.col, L.val32, x, f,
+  launch_rows<kJacobi, float, float>(s, L.nnz, L.ndof, L.nact, L.rowptr, L.col, L.val32, x, f,
                                      out, omega);
 }
 void k_jacobi_zero32(hipStream_t s, const DevLevel &L, const float *f, float *out, float omega) {
@@ -581,17 +652,17 @@ void k_jacobi_zero32(hipStream_t s, const DevLevel &L, const float *f, float *ou
                                                                         f, out, omega);
 }
 void k_defect32(hipStream_t s, const DevLevel &L, const float *x, const float *f, float *d) {
-  launch_rows<kDefect, float, float>(s, lanes_for(L.nnz, L.ndof), L.ndof, L.nact, L.rowptr, L.col, L.val32, x, f,
+  launch_rows<kDefect, float, float>(s, L.nnz, L.ndof, L.nact, L.rowptr, L.col, L.val32, x, f,
                                      d, 0.0f);
 }
 void k_restrict32(hipStream_t s, const DevLevel &F, const DevLevel &C, const float *fine, float *coarse) {
-  launch_rows<kRestrict, float, float>(s, lanes_for(F.r_nnz, C.ndof), C.ndof, C.nact, F.r_rowptr, F.r_col,
+  launch_rows<kRestrict, float, float>(s, F.r_nnz, C.ndof, C.nact, F.r_rowptr, F.r_col,
                                        F.r_val32, fine, nullptr, coarse, 0.0f);
 }
 void k_prolong_update32(hipStream_t s, const DevLevel &F, const DevLevel &C, const float *coarse, float *x,
                         float damp) {
   (void)C;
-  launch_rows<kProlongUpd, float, float>(s, lanes_for(F.p_nnz, F.ndof), F.ndof, F.nact, F.p_rowptr, F.p_col,
+  launch_rows<kProlongUpd, float, float>(s, F.p_nnz, F.ndof, F.nact, F.p_rowptr, F.p_col,
                                          F.p_val32, coarse, nullptr, x, damp);
 }
 void k_cast_d2f(hipStream_t s, int n, const double *a, float *b) { d2f_k<<<grid_for(n), kBlock, 0, s>>>(n, a, b); }
@@ -644,10 +715,22 @@ void k_assemble(hipStream_t s, const DevLevel &L, const double *tab, const AsmPa
   }
 }
 void k_invert_active(hipStream_t s, const DevLevel &L, double *scratch, double *inv) {
+  // scratch layout: M (2 na^2 doubles), two partial (value, index) buffers,
+  // used[na], perm[na]  (see gj_scratch_doubles)
   const int na = L.nact;
   if (na <= 0) return;
-  gj_invert_k<<<1, 1024, 0, s>>>(na, L.rowptr, L.col, L.val, scratch, scratch + (int64_t)2 * na * na, inv);
+  const int nblk = std::min((na + kGjWaves - 1) / kGjWaves, 256);
+  double *M = scratch, *pv0 = M + (int64_t)2 * na * na, *pv1 = pv0 + 256;
+  int *pi0 = (int *)(pv1 + 256), *pi1 = pi0 + 256, *used = pi1 + 256, *perm = used + na;
+  gj_init_k<<<nblk, kBlock, 0, s>>>(na, L.rowptr, L.col, L.val, M, used, pv0, pi0);
+  for (int k = 0; k < na; k++) {
+    const bool even = (k % 2) == 0;
+    gj_step_k<<<nblk, kBlock, 0, s>>>(na, k, M, used, perm, even ? pv0 : pv1, even ? pi0 : pi1, even ? pv1 : pv0,
+                                      even ? pi1 : pi0);
+  }
+  gj_final_k<<<nblk, kBlock, 0, s>>>(na, M, perm, inv);
 }
+int64_t gj_scratch_doubles(int na) { return (int64_t)2 * na * na + 512 + (512 + 2 * (int64_t)na + 1) / 2 + 1; }
 void k_multi_axpy(hipStream_t s, int n, int m, const double *coef, const double *Z, int64_t ldz, double *x) {
   multi_axpy_k<<<grid_for(n), kBlock, 0, s>>>(n, m, coef, Z, ldz, x);
 }
