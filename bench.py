@@ -86,7 +86,7 @@ def spmv_roofline(pm, P, reps=50):
     nbytes = 12 * info["nnz"] + 20 * info["ndof"]
     gbs = nbytes / (ms * 1e-3) / 1e9
     return dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4),
-                traffic=None, kernel="csr_rows<16,kSpmv,f64> fine level", us_per_launch=round(ms * 1e3, 2),
+                traffic=None, kernel="csr_stream<kSpmv,f64> fine level", us_per_launch=round(ms * 1e3, 2),
                 algorithmic_bytes=nbytes, nnz=info["nnz"], n=info["ndof"])
 
 
@@ -200,8 +200,8 @@ def main():
         "config": {"workload": "CD3D Q2 SUPG, unit cube 64^3 hexahedra, 2146689 DOFs, 5-level GMG-FGMRES(20) to 1e-10",
                    "eps": 1e-3, "b": [1.0, 0.5, 0.25], "smoother": "Jacobi omega 0.5, 3+3, V-cycle, exact coarse",
                    "parallelism": "1 GPU"},
-        "mixed": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in m.items()},
-        "fp64": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in res["fp64"].items()},
+        "mixed": {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in m.items()},
+        "fp64": {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in res["fp64"].items()},
         "mixed_vs_fp64_speedup": round(m["solves_per_s"] / res["fp64"]["solves_per_s"], 3),
         "mixed_solution_rel_err_vs_fp64": err,
         "p50_ms": round(m["p50_ms"], 3), "p99_ms": round(m["p99_ms"], 3),

@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <new>
@@ -87,6 +88,13 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
     if (!(m.box[d] > 0)) throw ArgError("box sides must be positive");
   int64_t nfine = (int64_t)m.m << (m.k0 + m.levels - 1);
   if (nfine * m.order + 1 > 1290) throw ArgError("mesh too large (nodes per side > 1290)");
+  {
+    // CSR row pointers and column indices are int32: bound the finest nnz
+    // (interior rows hold 27 (Q1) or on average 64 (Q2) entries)
+    const double nn = (double)(nfine * m.order + 1);
+    if (nn * nn * nn * (m.order == 1 ? 27 : 64) >= 2147483647.0)
+      throw ArgError("mesh too large (finest level exceeds 2^31 matrix entries)");
+  }
   P.mesh = m;
   P.nb = (m.order + 1) * (m.order + 1) * (m.order + 1);
   HIPCHK(hipGetDevice(&P.device));
@@ -119,6 +127,11 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
     upload(P, D.rowptr, H.rowptr);
     upload(P, D.col, H.kcol);
     upload(P, D.diag, H.diag);
+    D.nblk = (int)H.blk.size() - 1;
+    D.blk = dalloc<int>(P, H.blk.size());
+    upload(P, D.blk, H.blk);
+    D.invdiag = dalloc<double>(P, H.ndof);
+    D.invdiag32 = dalloc<float>(P, H.ndof);
     for (double **v : {&D.sol, &D.rhs, &D.aux, &D.tmp, &D.rhs_asm}) {
       *v = dalloc<double>(P, H.ndof);
       HIPCHK(hipMemsetAsync(*v, 0, sizeof(double) * H.ndof, P.stream));
@@ -159,6 +172,27 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
     D.colour_cells = dalloc<int>(P, H.colour_cells.size());
     upload(P, D.colour_cells, H.colour_cells);
     D.colour_start = H.colour_start;
+    // lattice-stencil form of the mixed-precision V-cycle
+    D.nn = H.nn;
+    D.nclass = (int)H.lat.size();
+    for (int c = 0; c < D.nclass; c++) D.cls[c] = H.lat[c];
+    D.lat_nval = H.lat_nval;
+    D.lat_nint = (int64_t)H.lat_node2dof.size();
+    D.ldof2node = dalloc<int>(P, H.dof_node.size());
+    upload(P, D.ldof2node, H.dof_node);
+    if (l > 0) {  // the coarsest level is solved in FP64 from the DOF vectors
+      D.lval = dalloc<float>(P, H.lat_perm.size());
+      D.lperm = dalloc<int>(P, H.lat_perm.size());
+      upload(P, D.lperm, H.lat_perm);
+      D.linv = dalloc<float>(P, H.lat_node2dof.size());
+      D.lnode2dof = dalloc<int>(P, H.lat_node2dof.size());
+      upload(P, D.lnode2dof, H.lat_node2dof);
+    }
+    const size_t nn3 = (size_t)H.nn * H.nn * H.nn;
+    for (float **v : {&D.lx, &D.lf, &D.la, &D.lt}) {
+      *v = dalloc<float>(P, nn3);  // boundary entries stay zero for good
+      HIPCHK(hipMemsetAsync(*v, 0, sizeof(float) * nn3, P.stream));
+    }
     HIPCHK(hipStreamSynchronize(P.stream));  // host vectors above go out of scope
   }
   // exact coarse solve storage
@@ -251,7 +285,12 @@ static void refresh_coarse_and_f32(cd3d_problem &P) {
     HIPCHK(hipGetLastError());
   }
   P.cinv_valid = true;
-  for (auto &D : P.dl) k_to_f32(P.stream, D.nnz, D.val, D.val32);
+  for (size_t l = 0; l < P.dl.size(); l++) {
+    DevLevel &D = P.dl[l];
+    k_to_f32(P.stream, D.nnz, D.val, D.val32);
+    k_invdiag(P.stream, D);
+    if (l > 0) kl_refresh(P.stream, D);
+  }
 }
 
 // ============================================================== V-cycle
@@ -353,6 +392,54 @@ void cycle32(VC &v, int i, bool zero_start) {
   if (x != L.fsol) HIPCHK(hipMemcpyAsync(L.fsol, x, sizeof(float) * L.ndof, hipMemcpyDeviceToDevice, P.stream));
 }
 
+// mixed precision on the lattice-stencil form: the same cycle, with vectors
+// in node-lattice order whose boundary entries are zero (the Dirichlet part
+// of every preconditioner rhs and iterate vanishes)
+void coarseL(VC &v, bool zero_start) {
+  DevLevel &C = v.P.dl[0];
+  kl_to_dof(v.P.stream, C, C.lf, C.rhs);
+  if (!zero_start) kl_to_dof(v.P.stream, C, C.lx, C.sol);
+  coarse64(v, zero_start);
+  kl_from_dof(v.P.stream, C, C.sol, C.lx);
+}
+
+void cycleL(VC &v, int i, bool zero_start) {
+  cd3d_problem &P = v.P;
+  if (i == 0) {
+    coarseL(v, zero_start);
+    return;
+  }
+  DevLevel &L = P.dl[i], &C = P.dl[i - 1];
+  const int p = P.mesh.order;
+  float *x = L.lx, *t = L.lt;
+  const float om = (float)v.sp.omega;
+  for (int k = 0; k < v.sp.pre; k++) {
+    if (k == 0 && zero_start) kl_jacobi_zero(P.stream, L, p, L.lf, t, om);
+    else kl_jacobi(P.stream, L, p, x, L.lf, t, om);
+    std::swap(x, t);
+  }
+  const size_t nn3 = (size_t)L.nn * L.nn * L.nn;
+  if (v.sp.pre == 0 && zero_start) HIPCHK(hipMemsetAsync(x, 0, sizeof(float) * nn3, P.stream));
+  kl_defect(P.stream, L, p, x, L.lf, L.la);
+  kl_restrict(P.stream, L, C, p, L.la, C.lf);
+  for (int r = 0; r < v.rec[i]; r++) cycleL(v, i - 1, r == 0);
+  if (v.sp.cycle < 1) v.rec[i] = 1;  // F-cycle
+  kl_prolong_update(P.stream, L, C, p, C.lx, x, (float)v.sp.gmg_damp);
+  for (int k = 0; k < v.sp.post; k++) {
+    kl_jacobi(P.stream, L, p, x, L.lf, t, om);
+    std::swap(x, t);
+  }
+  if (x != L.lx) HIPCHK(hipMemcpyAsync(L.lx, x, sizeof(float) * nn3, hipMemcpyDeviceToDevice, P.stream));
+}
+
+bool mixed_csr() {
+  static const bool v = [] {
+    const char *e = getenv("CD3D_MIXED_CSR");
+    return e && atoi(e) != 0;
+  }();
+  return v;
+}
+
 // TMultiGridScaIte::Iterate (MultiGridScaIte.C:63-104): z = V(r), zero start.
 // rhs_dirichlet_zero: r vanishes on Dirichlet DOFs (always true for the
 // Krylov vectors of FGMRES), which lets the first sweep skip the matrix.
@@ -362,9 +449,15 @@ void precondition(VC &v, const double *r, double *z, bool rhs_dirichlet_zero) {
   const int top = (int)P.dl.size() - 1;
   DevLevel &F = P.dl[top];
   if (v.sp.precision == CD3D_MIXED && top > 0 && rhs_dirichlet_zero) {
-    k_cast_d2f(P.stream, F.ndof, r, F.frhs);
-    cycle32(v, top, true);
-    k_cast_f2d(P.stream, F.ndof, F.fsol, z);
+    if (mixed_csr()) {
+      k_cast_d2f(P.stream, F.ndof, r, F.frhs);
+      cycle32(v, top, true);
+      k_cast_f2d(P.stream, F.ndof, F.fsol, z);
+    } else {
+      kl_from_dof(P.stream, F, r, F.lf);
+      cycleL(v, top, true);
+      kl_to_dof(P.stream, F, F.lx, z);
+    }
     return;
   }
   k_copy(P.stream, F.ndof, r, F.rhs);

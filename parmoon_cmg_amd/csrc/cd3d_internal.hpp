@@ -25,6 +25,22 @@
 
 namespace cd3d {
 
+// Lattice-stencil form of a level (mixed-precision V-cycle): interior nodes
+// of the node lattice split into parity classes; class c holds the box of
+// nodes with X % 2 == px (Q2; one class for Q1), each with the same stencil
+// (per axis offsets -p..p on cell-face planes, the cell's p+1 nodes inside a
+// cell).  Values are stored [class][stencil entry][node of the class].
+struct LatClass {
+  int px[3] = {0, 0, 0};   // parity per axis (Q2), 0 for Q1
+  int lo[3] = {0, 0, 0};   // first lattice coordinate per axis
+  int cnt[3] = {0, 0, 0};  // nodes per axis (step 2 for Q2, 1 for Q1)
+  int step = 1;
+  int64_t nodes = 0;       // cnt0 cnt1 cnt2
+  int nsten = 0;           // stencil entries
+  int64_t off = 0;         // offset of the class in the stencil value array
+  int64_t first = 0;       // index of the class's first node in the interior-node numbering
+};
+
 struct HostLevel {
   int n = 0;        // cells per side
   int ncell = 0;
@@ -36,6 +52,7 @@ struct HostLevel {
   std::vector<int> gn;          // nb per cell
   std::vector<int> dof_node;    // DOF -> lattice node
   std::vector<int> rowptr, kcol, diag;
+  std::vector<int> blk;  // row blocks for the stream kernels
   // transfer from the next coarser level
   std::vector<int> p_rowptr, p_col;
   std::vector<double> p_val;
@@ -43,6 +60,11 @@ struct HostLevel {
   std::vector<double> r_val;
   std::vector<int> colour_start;  // 9 offsets into colour_cells
   std::vector<int> colour_cells;
+  // lattice-stencil form
+  std::vector<LatClass> lat;
+  int64_t lat_nval = 0;             // sum over classes of nsten * nodes
+  std::vector<int> lat_perm;        // CSR position of every stencil value
+  std::vector<int> lat_node2dof;    // interior node (class numbering) -> DOF
 };
 
 struct DevLevel {
@@ -51,6 +73,11 @@ struct DevLevel {
   int *rowptr = nullptr, *col = nullptr, *diag = nullptr;
   double *val = nullptr;
   float *val32 = nullptr;
+  // row blocks of the stream kernels (at most 2048 nonzeros and 256 rows)
+  int *blk = nullptr;
+  int nblk = 0;
+  double *invdiag = nullptr;  // 1 / a_ii
+  float *invdiag32 = nullptr;
   double *sol = nullptr, *rhs = nullptr, *aux = nullptr, *tmp = nullptr;
   double *rhs_asm = nullptr;  // assembled rhs (the multigrid works in `rhs`)
   float *fsol = nullptr, *frhs = nullptr, *faux = nullptr, *ftmp = nullptr;
@@ -70,6 +97,17 @@ struct DevLevel {
   int ncell = 0;
   double h[3] = {0, 0, 0};   // cell sides
   double dn[3] = {0, 0, 0};  // node spacing
+  // lattice-stencil form (mixed-precision V-cycle)
+  int nn = 0;                  // nodes per side
+  int nclass = 0;
+  LatClass cls[8];
+  int64_t lat_nval = 0, lat_nint = 0;
+  float *lval = nullptr;       // stencil values f32
+  int *lperm = nullptr;        // CSR position of each stencil value
+  float *linv = nullptr;       // 1/a_ii per interior node (class numbering)
+  int *lnode2dof = nullptr;    // interior node -> DOF
+  int *ldof2node = nullptr;    // DOF -> lattice node
+  float *lx = nullptr, *lf = nullptr, *la = nullptr, *lt = nullptr;  // lattice vectors, nn^3 each
 };
 
 struct Problem {
@@ -104,6 +142,8 @@ void build_host_level(HostLevel &L, int order, int m, int depth);
 void build_transfer(HostLevel &fine, const HostLevel &coarse, int order);
 void build_slots(const HostLevel &L, int order, std::vector<int> &slots);
 void build_colours(HostLevel &L, int order);
+void build_row_blocks(const std::vector<int> &rowptr, int max_nnz, int max_rows, std::vector<int> &blk);
+void build_lattice(HostLevel &L, int order);
 void basis_tables(int order, std::vector<double> &tab);
 int quad_points_1d(int order);
 
@@ -114,6 +154,7 @@ void k_spmv(hipStream_t s, const DevLevel &L, const double *x, double *y);
 void k_defect(hipStream_t s, const DevLevel &L, const double *x, const double *f, double *d);
 void k_jacobi(hipStream_t s, const DevLevel &L, const double *x, const double *f, double *out, double omega);
 void k_jacobi_zero(hipStream_t s, const DevLevel &L, const double *f, double *out, double omega);
+void k_invdiag(hipStream_t s, const DevLevel &L);
 void k_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, const double *fine, double *coarse);
 void k_prolong_update(hipStream_t s, const DevLevel &F, const DevLevel &C, const double *coarse, double *x,
                       double damp);
@@ -155,6 +196,19 @@ struct AsmParams {
 void k_assemble(hipStream_t s, const DevLevel &L, const double *tab, const AsmParams &ap, double *val,
                 double *rhs, int colour_begin, int colour_count);
 void k_dirichlet_rows(hipStream_t s, const DevLevel &L, double *val, double *rhs);
+// ---------------------------------------------------------------- lattice form (lattice.hip)
+void kl_jacobi(hipStream_t s, const DevLevel &L, int order, const float *x, const float *f, float *out,
+               float omega);
+void kl_jacobi_zero(hipStream_t s, const DevLevel &L, int order, const float *f, float *out, float omega);
+void kl_defect(hipStream_t s, const DevLevel &L, int order, const float *x, const float *f, float *d);
+void kl_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, int order, const float *fine, float *coarse);
+void kl_prolong_update(hipStream_t s, const DevLevel &F, const DevLevel &C, int order, const float *coarse, float *x,
+                       float damp);
+// stencil values and 1/a_ii from the FP64 CSR matrix
+void kl_refresh(hipStream_t s, const DevLevel &L);
+// DOF-ordered f64 <-> lattice f32 (Dirichlet DOFs map to zero)
+void kl_from_dof(hipStream_t s, const DevLevel &L, const double *v, float *l);
+void kl_to_dof(hipStream_t s, const DevLevel &L, const float *l, double *v);
 // scratch: gj_scratch_doubles(na) doubles; inv: na^2 doubles
 void k_invert_active(hipStream_t s, const DevLevel &L, double *scratch, double *inv);
 int64_t gj_scratch_doubles(int na);

@@ -121,6 +121,105 @@ void launch_rows(hipStream_t st, int64_t nnz, int n, int nact, const int *rp, co
   }
 }
 
+// ---------------------------------------------------------------- CSR stream
+// Row-block kernel for the level matrices.  Workgroup b owns rows
+// [blk[b], blk[b+1]) whose nonzeros (at most kStreamNnz) are contiguous in
+// the CSR arrays: phase 1 streams them with coalesced, unrolled loads and
+// stores a_ij x_j into LDS; phase 2 sums each row from LDS with G lanes per
+// row (G = the largest power of two with G * rows <= 256).  Memory access no
+// longer depends on the row lengths.  MODE: kSpmv out = A x; kDefect
+// out = f - A x; kJacobi out = x + w (f - A x) / a_ii on active rows, f on
+// Dirichlet rows (the damped Jacobi step of TMGLevel3D::Jacobi,
+// MGLevel3D.C:443-537, written with the full row sum).
+constexpr int kStreamNnz = 2048;
+constexpr int kStreamPer = kStreamNnz / kBlock;
+
+template <int G, int MODE, typename T>
+__device__ inline void stream_rows(int nr, int r0, int j0, int nact, const int *__restrict__ rowptr,
+                                   const T *__restrict__ prod, const T *__restrict__ x, const T *__restrict__ f,
+                                   const T *__restrict__ invdiag, T *__restrict__ out, T omega) {
+  const int g = threadIdx.x / G, l = threadIdx.x % G;
+  for (int r = g; r < nr; r += kBlock / G) {
+    const int row = r0 + r;
+    const int a = rowptr[row] - j0, e = rowptr[row + 1] - j0;
+    T s = T(0);
+    for (int t = a + l; t < e; t += G) s += prod[t];
+    s = group_sum<G>(s);
+    if (l == 0) {
+      if (MODE == kSpmv) out[row] = s;
+      else if (MODE == kDefect) out[row] = f[row] - s;
+      else out[row] = row < nact ? x[row] + omega * (f[row] - s) * invdiag[row] : f[row];
+    }
+  }
+}
+
+template <int MODE, typename T, typename V>
+__global__ __launch_bounds__(kBlock) void csr_stream(const int *__restrict__ blk, int nact,
+                                                     const int *__restrict__ rowptr, const int *__restrict__ col,
+                                                     const V *__restrict__ val, const T *__restrict__ x,
+                                                     const T *__restrict__ f, const T *__restrict__ invdiag,
+                                                     T *__restrict__ out, T omega) {
+  __shared__ T prod[kStreamNnz];
+  const int r0 = blk[blockIdx.x], r1 = blk[blockIdx.x + 1];
+  const int j0 = rowptr[r0], nz = rowptr[r1] - j0;
+  const int nr = r1 - r0;
+  if (nz > kStreamNnz) {
+    // a single row longer than a block: strided partial sums and a block reduction
+    __shared__ T red[kBlock / 64];
+    T s = T(0);
+    for (int t = threadIdx.x; t < nz; t += kBlock) s += (T)val[j0 + t] * x[col[j0 + t]];
+    s = group_sum<64>(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      T tot = T(0);
+      for (int k = 0; k < kBlock / 64; k++) tot += red[k];
+      const int row = r0;
+      if (MODE == kSpmv) out[row] = tot;
+      else if (MODE == kDefect) out[row] = f[row] - tot;
+      else out[row] = row < nact ? x[row] + omega * (f[row] - tot) * invdiag[row] : f[row];
+    }
+    return;
+  }
+  int c[kStreamPer];
+  T a[kStreamPer];
+#pragma unroll
+  for (int k = 0; k < kStreamPer; k++) {
+    const int t = threadIdx.x + k * kBlock;
+    c[k] = t < nz ? col[j0 + t] : 0;
+    a[k] = t < nz ? (T)val[j0 + t] : T(0);
+  }
+#pragma unroll
+  for (int k = 0; k < kStreamPer; k++) {
+    const int t = threadIdx.x + k * kBlock;
+    if (t < nz) prod[t] = a[k] * x[c[k]];
+  }
+  __syncthreads();
+  const int G = nr <= 4 ? 64 : nr <= 8 ? 32 : nr <= 16 ? 16 : nr <= 32 ? 8 : nr <= 64 ? 4 : nr <= 128 ? 2 : 1;
+  switch (G) {
+    case 64: stream_rows<64, MODE, T>(nr, r0, j0, nact, rowptr, prod, x, f, invdiag, out, omega); break;
+    case 32: stream_rows<32, MODE, T>(nr, r0, j0, nact, rowptr, prod, x, f, invdiag, out, omega); break;
+    case 16: stream_rows<16, MODE, T>(nr, r0, j0, nact, rowptr, prod, x, f, invdiag, out, omega); break;
+    case 8: stream_rows<8, MODE, T>(nr, r0, j0, nact, rowptr, prod, x, f, invdiag, out, omega); break;
+    case 4: stream_rows<4, MODE, T>(nr, r0, j0, nact, rowptr, prod, x, f, invdiag, out, omega); break;
+    case 2: stream_rows<2, MODE, T>(nr, r0, j0, nact, rowptr, prod, x, f, invdiag, out, omega); break;
+    default: stream_rows<1, MODE, T>(nr, r0, j0, nact, rowptr, prod, x, f, invdiag, out, omega); break;
+  }
+}
+
+template <typename T>
+__global__ void invdiag_k(int n, const int *__restrict__ diag, const double *__restrict__ val, T *__restrict__ inv) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) inv[i] = (T)(1.0 / val[diag[i]]);
+}
+
+template <int MODE, typename T, typename V>
+void launch_stream(hipStream_t st, const DevLevel &L, const V *val, const T *x, const T *f, const T *invdiag, T *out,
+                   T omega) {
+  if (L.nblk <= 0) return;
+  csr_stream<MODE, T, V><<<L.nblk, kBlock, 0, st>>>(L.blk, L.nact, L.rowptr, L.col, val, x, f, invdiag, out, omega);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void jacobi_zero_k(int n, int nact, const int *__restrict__ diag,
                                                         const double *__restrict__ val64,
@@ -610,16 +709,17 @@ __global__ void dirichlet_rows_k(int n, int nact, const int *__restrict__ rp, do
 // ---------------------------------------------------------------- launchers
 
 void k_spmv(hipStream_t s, const DevLevel &L, const double *x, double *y) {
-  launch_rows<kSpmv, double, double>(s, L.nnz, L.ndof, L.nact, L.rowptr, L.col, L.val, x,
-                                     nullptr, y, 0.0);
+  launch_stream<kSpmv, double, double>(s, L, L.val, x, nullptr, nullptr, y, 0.0);
 }
 void k_defect(hipStream_t s, const DevLevel &L, const double *x, const double *f, double *d) {
-  launch_rows<kDefect, double, double>(s, L.nnz, L.ndof, L.nact, L.rowptr, L.col, L.val, x, f,
-                                       d, 0.0);
+  launch_stream<kDefect, double, double>(s, L, L.val, x, f, nullptr, d, 0.0);
 }
 void k_jacobi(hipStream_t s, const DevLevel &L, const double *x, const double *f, double *out, double omega) {
-  launch_rows<kJacobi, double, double>(s, L.nnz, L.ndof, L.nact, L.rowptr, L.col, L.val, x, f,
-                                       out, omega);
+  launch_stream<kJacobi, double, double>(s, L, L.val, x, f, L.invdiag, out, omega);
+}
+void k_invdiag(hipStream_t s, const DevLevel &L) {
+  invdiag_k<double><<<(L.ndof + kBlock - 1) / kBlock, kBlock, 0, s>>>(L.ndof, L.diag, L.val, L.invdiag);
+  invdiag_k<float><<<(L.ndof + kBlock - 1) / kBlock, kBlock, 0, s>>>(L.ndof, L.diag, L.val, L.invdiag32);
 }
 void k_jacobi_zero(hipStream_t s, const DevLevel &L, const double *f, double *out, double omega) {
   jacobi_zero_k<double><<<(L.ndof + kBlock - 1) / kBlock, kBlock, 0, s>>>(L.ndof, L.nact, L.diag, L.val, nullptr,
@@ -644,16 +744,14 @@ void k_prolong(hipStream_t s, const DevLevel &F, const DevLevel &C, const double
                                      coarse, nullptr, fine, 0.0);
 }
 void k_jacobi32(hipStream_t s, const DevLevel &L, const float *x, const float *f, float *out, float omega) {
-  launch_rows<kJacobi, float, float>(s, L.nnz, L.ndof, L.nact, L.rowptr, L.col, L.val32, x, f,
-                                     out, omega);
+  launch_stream<kJacobi, float, float>(s, L, L.val32, x, f, L.invdiag32, out, omega);
 }
 void k_jacobi_zero32(hipStream_t s, const DevLevel &L, const float *f, float *out, float omega) {
   jacobi_zero_k<float><<<(L.ndof + kBlock - 1) / kBlock, kBlock, 0, s>>>(L.ndof, L.nact, L.diag, nullptr, L.val32,
                                                                         f, out, omega);
 }
 void k_defect32(hipStream_t s, const DevLevel &L, const float *x, const float *f, float *d) {
-  launch_rows<kDefect, float, float>(s, L.nnz, L.ndof, L.nact, L.rowptr, L.col, L.val32, x, f,
-                                     d, 0.0f);
+  launch_stream<kDefect, float, float>(s, L, L.val32, x, f, nullptr, d, 0.0f);
 }
 void k_restrict32(hipStream_t s, const DevLevel &F, const DevLevel &C, const float *fine, float *coarse) {
   launch_rows<kRestrict, float, float>(s, F.r_nnz, C.ndof, C.nact, F.r_rowptr, F.r_col,

@@ -7,8 +7,10 @@
 // lattice of each level, so shared DOFs, boundary DOFs and matrix columns are
 // found by lattice arithmetic instead of the reference's joint/mapper graph.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <stdexcept>
 #include <thread>
 
 #include "cd3d_internal.hpp"
@@ -173,6 +175,83 @@ void build_host_level(HostLevel &L, int order, int m, int depth) {
     L.diag[r] = (int)(std::lower_bound(out, out + t, r) - &L.kcol[0]);
   });
   build_colours(L, order);
+  build_row_blocks(L.rowptr, 2048, 256, L.blk);
+  build_lattice(L, order);
+}
+
+// Lattice-stencil form: classes, the CSR position of every stencil value and
+// the interior node -> DOF map.  Stencil entry order within a class: z, y, x
+// offsets ascending (x fastest).
+void build_lattice(HostLevel &L, int order) {
+  const int p = order, nn = L.nn;
+  std::vector<int> node_dof((size_t)nn * nn * nn, -1);
+  for (int d = 0; d < L.ndof; d++) node_dof[L.dof_node[d]] = d;
+  L.lat.clear();
+  const int ncls = p == 1 ? 1 : 8;
+  int64_t off = 0, first = 0;
+  for (int c = 0; c < ncls; c++) {
+    LatClass C;
+    C.step = p == 1 ? 1 : 2;
+    int sten = 1;
+    for (int d = 0; d < 3; d++) {
+      C.px[d] = p == 1 ? 0 : (c >> d) & 1;
+      // interior coordinates 1..nn-2 with the class parity
+      C.lo[d] = p == 1 ? 1 : (C.px[d] ? 1 : 2);
+      C.cnt[d] = p == 1 ? nn - 2 : (C.px[d] ? (nn - 1) / 2 : (nn - 3) / 2);
+      sten *= (p == 1 || C.px[d] == 0) ? 2 * p + 1 : 3;
+    }
+    C.nodes = (int64_t)C.cnt[0] * C.cnt[1] * C.cnt[2];
+    C.nsten = sten;
+    C.off = off;
+    C.first = first;
+    off += C.nodes * sten;
+    first += C.nodes;
+    L.lat.push_back(C);
+  }
+  L.lat_nval = off;
+  L.lat_perm.assign(off, -1);
+  L.lat_node2dof.assign(first, -1);
+  std::atomic<int> missing{0};
+  for (const LatClass &C : L.lat) {
+    parallel_for((int)C.nodes, [&](int k) {
+      const int kx = k % C.cnt[0], ky = (k / C.cnt[0]) % C.cnt[1], kz = k / (C.cnt[0] * C.cnt[1]);
+      const int X = C.lo[0] + C.step * kx, Y = C.lo[1] + C.step * ky, Z = C.lo[2] + C.step * kz;
+      const int row = node_dof[X + nn * (Y + (int64_t)nn * Z)];
+      L.lat_node2dof[C.first + k] = row;
+      int rng[3][2];
+      const int co[3] = {X, Y, Z};
+      for (int d = 0; d < 3; d++) {
+        if (co[d] % p == 0) { rng[d][0] = -p; rng[d][1] = p; }
+        else { rng[d][0] = -(co[d] % p); rng[d][1] = p - co[d] % p; }
+      }
+      const int *b = &L.kcol[L.rowptr[row]], *e = &L.kcol[L.rowptr[row + 1]];
+      int s = 0;
+      for (int dz = rng[2][0]; dz <= rng[2][1]; dz++)
+        for (int dy = rng[1][0]; dy <= rng[1][1]; dy++)
+          for (int dx = rng[0][0]; dx <= rng[0][1]; dx++, s++) {
+            const int cdof = node_dof[(X + dx) + nn * ((Y + dy) + (int64_t)nn * (Z + dz))];
+            const int *q = std::lower_bound(b, e, cdof);
+            if (q == e || *q != cdof) missing++;
+            L.lat_perm[C.off + (int64_t)s * C.nodes + k] = (int)(q - &L.kcol[0]);
+          }
+    });
+  }
+  if (missing) throw std::logic_error("lattice stencil entry missing from the CSR pattern");
+}
+
+// Consecutive rows grouped so that each group holds at most max_nnz nonzeros
+// and max_rows rows (a longer row forms a group of its own).
+void build_row_blocks(const std::vector<int> &rowptr, int max_nnz, int max_rows, std::vector<int> &blk) {
+  const int n = (int)rowptr.size() - 1;
+  blk.clear();
+  blk.push_back(0);
+  int r = 0;
+  while (r < n) {
+    int e = r + 1;
+    while (e < n && e - r < max_rows && rowptr[e + 1] - rowptr[r] <= max_nnz) e++;
+    blk.push_back(e);
+    r = e;
+  }
 }
 
 // Grid transfer between coarse level and the next finer one: P(i,k) is the

@@ -176,3 +176,24 @@ def test_structure_closed_forms_larger(order, k0, levels):
     rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
     same_row = rows[1:] == rows[:-1]
     assert np.all(kc[1:][same_row] > kc[:-1][same_row])  # sorted, unique columns
+
+
+@pytest.mark.parametrize("order,m,k0,levels", [(2, 1, 1, 3), (1, 2, 1, 3), (2, 3, 0, 3), (1, 1, 2, 4)])
+def test_mixed_vcycle_against_fp64(order, m, k0, levels):
+    # the FP32 lattice-stencil V-cycle is the FP64 CSR V-cycle up to rounding
+    P = pm.Problem(order, m, k0, levels, box=(1.0, 0.7, 0.5))
+    P.assemble(pm.coeffs(eps=1e-2, b=(1.0, 0.5, 0.25), supg=1, f_val=1.0))
+    i = P.info(levels - 1)
+    r = np.random.default_rng(3).standard_normal(i["ndof"])
+    r[i["nact"]:] = 0.0
+    sp = dict(REF_SOLVER, coarse_smoother=17)
+    z64 = P.vcycle(pm.solver_params(**sp), r)
+    z32 = P.vcycle(pm.solver_params(**sp, precision=pm.MIXED), r)
+    err = np.linalg.norm(z32 - z64) / np.linalg.norm(z64)
+    say(f"[gpu] mixed vcycle order={order} m={m} k0={k0} levels={levels} err vs fp64 {err:.2e}")
+    assert err < 1e-5
+    assert np.all(z32[i["nact"]:] == 0.0)
+    x, st = P.solve(pm.solver_params(**sp, rel_tol=1e-10, precision=pm.MIXED))
+    x64, st64 = P.solve(pm.solver_params(**sp, rel_tol=1e-10))
+    assert st.rfinal <= 1.0001e-10 * st.r0
+    assert np.linalg.norm(x - x64) / np.linalg.norm(x64) < 1e-9
