@@ -157,6 +157,20 @@ cd3d_status cd3d_get_solution(cd3d_problem *p, double *sol_host);
  * final state; states_host (nsteps x N) optional. */
 cd3d_status cd3d_tcd_run(cd3d_problem *p, const cd3d_coeffs *co, const cd3d_solver_params *sp, double t0,
                          double tau, int nsteps, double *u_host, double *states_host, cd3d_stats *st);
+/* Step-wise TCD3D with the theta-scheme of SetTimeDiscParameters
+ * (TimeDiscRout.C; theta 0.5 Crank-Nicolson, 1 backward Euler).
+ * cd3d_tcd_init: TSystemTCD3D::AssembleMRhs at t0 (SystemTCD3D.C:176-202)
+ * and the initial state u_host (finest level; Dirichlet values from the rhs).
+ * cd3d_tcd_step: one step t -> t + tau: AssembleARhs at t + tau,
+ * AssembleSystMat, Solve, RestoreMassMat (SystemTCD3D.C:204-376, driven as
+ * in TCD3D_ParMooN.C:569-622); u_host (optional) receives the new state.
+ * The coefficients' t is ignored (the handle keeps the time).  After
+ * cd3d_tcd_init the level matrices hold the mass matrices: cd3d_solve needs a
+ * new cd3d_assemble. */
+cd3d_status cd3d_tcd_init(cd3d_problem *p, const cd3d_coeffs *co, double t0, const double *u_host);
+cd3d_status cd3d_tcd_step(cd3d_problem *p, const cd3d_coeffs *co, const cd3d_solver_params *sp, double tau,
+                          double theta, double *u_host, cd3d_stats *st);
+cd3d_status cd3d_tcd_get_state(cd3d_problem *p, double *u_host);
 /* nodal interpolation of the Gaussian pulse (TFEFunction3D::Interpolate, FEFunction3D.C:1446) */
 cd3d_status cd3d_interp_pulse(cd3d_problem *p, int level, const cd3d_coeffs *co, double *u_host);
 

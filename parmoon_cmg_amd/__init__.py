@@ -25,7 +25,11 @@ EXPORTS = ["cd3d_default_solver", "cd3d_create", "cd3d_destroy", "cd3d_last_erro
            "cd3d_level_info", "cd3d_get_dofmap", "cd3d_get_cells", "cd3d_get_csr", "cd3d_get_rhs",
            "cd3d_set_entries", "cd3d_assemble", "cd3d_matvec", "cd3d_matvec_dev", "cd3d_prolongate",
            "cd3d_restrict", "cd3d_vcycle", "cd3d_solve", "cd3d_get_solution", "cd3d_tcd_run",
-           "cd3d_interp_pulse", "cd3d_device_ptr", "cd3d_stream", "cd3d_synchronize"]
+           "cd3d_tcd_init", "cd3d_tcd_step", "cd3d_tcd_get_state",
+           "cd3d_interp_pulse", "cd3d_device_ptr", "cd3d_stream", "cd3d_synchronize",
+           # cd3d_server.h
+           "cd3d_server_default_params", "cd3d_estimate_bytes", "cd3d_server_create", "cd3d_server_submit",
+           "cd3d_server_wait", "cd3d_server_drain", "cd3d_server_get_stats", "cd3d_server_destroy"]
 
 
 class CD3DError(RuntimeError):
@@ -57,6 +61,33 @@ class Stats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("status", C.c_int), ("r0", C.c_double), ("rfinal", C.c_double),
                 ("ms_assemble", C.c_double), ("ms_solve", C.c_double)]
 
+
+class ServerParams(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_inflight", C.c_int), ("workers", C.c_int),
+                ("keep_resident", C.c_int), ("max_bypass", C.c_int), ("simulate", C.c_int),
+                ("sim_us_per_dof", C.c_double)]
+
+
+class Request(C.Structure):
+    _fields_ = [("kind", C.c_int), ("mesh", MeshDesc), ("coeffs", Coeffs), ("solver", SolverParams),
+                ("t0", C.c_double), ("tau", C.c_double), ("theta", C.c_double), ("nsteps", C.c_int),
+                ("want_solution", C.c_int)]
+
+
+class Result(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int), ("iterations", C.c_int), ("rel_residual", C.c_double),
+                ("queue_ms", C.c_double), ("run_ms", C.c_double), ("latency_ms", C.c_double),
+                ("device_bytes", C.c_size_t), ("reused_hierarchy", C.c_int), ("n_dof", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("rejected", C.c_uint64), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
+                ("solves_per_s", C.c_double), ("bytes_in_use", C.c_size_t), ("peak_bytes", C.c_size_t),
+                ("budget", C.c_size_t), ("inflight", C.c_int), ("queued", C.c_int), ("resident", C.c_int)]
+
+
+REQ_CD, REQ_TCD = 0, 1
 
 _lib = None
 
@@ -94,11 +125,24 @@ def lib():
     L.cd3d_solve.argtypes = [P, C.POINTER(SolverParams), dp, dp, I, C.POINTER(Stats)]
     L.cd3d_get_solution.argtypes = [P, dp]
     L.cd3d_tcd_run.argtypes = [P, C.POINTER(Coeffs), C.POINTER(SolverParams), D, D, I, dp, dp, C.POINTER(Stats)]
+    L.cd3d_tcd_init.argtypes = [P, C.POINTER(Coeffs), D, dp]
+    L.cd3d_tcd_step.argtypes = [P, C.POINTER(Coeffs), C.POINTER(SolverParams), D, D, dp, C.POINTER(Stats)]
+    L.cd3d_tcd_get_state.argtypes = [P, dp]
     L.cd3d_interp_pulse.argtypes = [P, I, C.POINTER(Coeffs), dp]
     L.cd3d_device_ptr.argtypes = [P, I, I, C.POINTER(C.c_void_p)]
     L.cd3d_stream.argtypes = [P]
     L.cd3d_stream.restype = C.c_void_p
     L.cd3d_synchronize.argtypes = [P]
+    L.cd3d_server_default_params.argtypes = [C.POINTER(ServerParams)]
+    L.cd3d_server_default_params.restype = None
+    L.cd3d_estimate_bytes.argtypes = [C.POINTER(MeshDesc), I, I, I]
+    L.cd3d_estimate_bytes.restype = C.c_size_t
+    L.cd3d_server_create.argtypes = [C.POINTER(ServerParams), C.POINTER(P)]
+    L.cd3d_server_submit.argtypes = [P, C.POINTER(Request), C.POINTER(C.c_uint64)]
+    L.cd3d_server_wait.argtypes = [P, C.c_uint64, D, C.POINTER(Result), dp, I64]
+    L.cd3d_server_drain.argtypes = [P]
+    L.cd3d_server_get_stats.argtypes = [P, C.POINTER(ServerStats)]
+    L.cd3d_server_destroy.argtypes = [P]
     _lib = L
     return L
 
@@ -257,6 +301,25 @@ class Problem:
                                       C.byref(st)), "tcd_run")
         return u, states, st
 
+    def tcd_init(self, co, t0, u0):
+        """TSystemTCD3D::AssembleMRhs at t0 and the initial state."""
+        u = np.ascontiguousarray(u0, dtype=np.float64)
+        self._chk(self.L.cd3d_tcd_init(self.h, C.byref(co), t0, _dp(u)), "tcd_init")
+
+    def tcd_step(self, co, sp, tau, theta=0.5, want_state=True):
+        """One theta-scheme step; returns (state or None, Stats)."""
+        u = np.zeros(self.info(self.levels - 1)["ndof"]) if want_state else None
+        st = Stats()
+        rc = self.L.cd3d_tcd_step(self.h, C.byref(co), C.byref(sp), tau, theta, None if u is None else _dp(u),
+                                  C.byref(st))
+        self._chk(rc, "tcd_step")
+        return u, st
+
+    def tcd_state(self):
+        u = np.zeros(self.info(self.levels - 1)["ndof"])
+        self._chk(self.L.cd3d_tcd_get_state(self.h, _dp(u)), "tcd_get_state")
+        return u
+
     def device_ptr(self, l, which):
         p = C.c_void_p()
         self._chk(self.L.cd3d_device_ptr(self.h, l, which, C.byref(p)), "device_ptr")
@@ -267,3 +330,65 @@ class Problem:
 
     def synchronize(self):
         self._chk(self.L.cd3d_synchronize(self.h), "synchronize")
+
+
+def mesh_desc(order, m=1, k0=1, levels=2, box=(1.0, 1.0, 1.0)):
+    return MeshDesc(order, m, k0, levels, (C.c_double * 3)(*box))
+
+
+def estimate_bytes(mesh, restart=20, cellwise=False, tcd=False):
+    return lib().cd3d_estimate_bytes(C.byref(mesh), restart, int(cellwise), int(tcd))
+
+
+class Server:
+    """Serving loop (cd3d_server.h): requests admitted by an HBM budget and run
+    concurrently, each on its own resident hierarchy and stream."""
+
+    def __init__(self, hbm_budget=0, max_inflight=8, workers=0, keep_resident=True, max_bypass=16,
+                 simulate=False, sim_us_per_dof=0.01):
+        self.L = lib()
+        prm = ServerParams()
+        self.L.cd3d_server_default_params(C.byref(prm))
+        prm.hbm_budget, prm.max_inflight, prm.workers = hbm_budget, max_inflight, workers
+        prm.keep_resident, prm.max_bypass = int(keep_resident), max_bypass
+        prm.simulate, prm.sim_us_per_dof = int(simulate), sim_us_per_dof
+        h = C.c_void_p()
+        st = self.L.cd3d_server_create(C.byref(prm), C.byref(h))
+        if st != CD3D_OK:
+            raise CD3DError(st, "cd3d_server_create failed")
+        self.h = h
+
+    def submit(self, mesh, co, sp, kind=REQ_CD, t0=0.0, tau=1e-3, theta=0.5, nsteps=0, want_solution=False):
+        """Returns the request id (raises CD3DError when it is refused)."""
+        r = Request(kind, mesh, co, sp, t0, tau, theta, nsteps, int(want_solution))
+        rid = C.c_uint64()
+        st = self.L.cd3d_server_submit(self.h, C.byref(r), C.byref(rid))
+        if st != CD3D_OK:
+            raise CD3DError(st, "cd3d_server_submit")
+        return rid.value
+
+    def wait(self, rid, timeout_ms=-1.0, n_dof=0):
+        """Returns (Result, solution or None)."""
+        res = Result()
+        sol = np.zeros(n_dof) if n_dof else None
+        st = self.L.cd3d_server_wait(self.h, rid, timeout_ms, C.byref(res), None if sol is None else _dp(sol),
+                                     n_dof)
+        if st not in (CD3D_OK, 4):
+            raise CD3DError(st, f"request {rid}")
+        return res, sol
+
+    def drain(self):
+        self.L.cd3d_server_drain(self.h)
+
+    def stats(self):
+        st = ServerStats()
+        self.L.cd3d_server_get_stats(self.h, C.byref(st))
+        return st
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.cd3d_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

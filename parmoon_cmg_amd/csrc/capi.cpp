@@ -15,6 +15,7 @@
 #include <new>
 #include <stdexcept>
 
+#include "../../include/cd3d_server.h"
 #include "cd3d_internal.hpp"
 
 using namespace cd3d;
@@ -52,8 +53,20 @@ T *dalloc(Problem &P, size_t count) {
     throw std::bad_alloc();
   }
   P.allocs.push_back(p);
+  P.alloc_bytes.push_back(count * sizeof(T));
   P.dev_bytes += count * sizeof(T);
   return (T *)p;
+}
+
+void dfree(Problem &P, void *ptr) {
+  for (size_t i = 0; i < P.allocs.size(); i++)
+    if (P.allocs[i] == ptr) {
+      (void)hipFree(ptr);
+      P.dev_bytes -= P.alloc_bytes[i];
+      P.allocs.erase(P.allocs.begin() + i);
+      P.alloc_bytes.erase(P.alloc_bytes.begin() + i);
+      return;
+    }
 }
 
 template <class T>
@@ -123,7 +136,6 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
     D.col = dalloc<int>(P, H.nnz);
     D.diag = dalloc<int>(P, H.ndof);
     D.val = dalloc<double>(P, H.nnz);
-    D.val32 = dalloc<float>(P, H.nnz);
     upload(P, D.rowptr, H.rowptr);
     upload(P, D.col, H.kcol);
     upload(P, D.diag, H.diag);
@@ -131,12 +143,10 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
     D.blk = dalloc<int>(P, H.blk.size());
     upload(P, D.blk, H.blk);
     D.invdiag = dalloc<double>(P, H.ndof);
-    D.invdiag32 = dalloc<float>(P, H.ndof);
     for (double **v : {&D.sol, &D.rhs, &D.aux, &D.tmp, &D.rhs_asm}) {
       *v = dalloc<double>(P, H.ndof);
       HIPCHK(hipMemsetAsync(*v, 0, sizeof(double) * H.ndof, P.stream));
     }
-    for (float **v : {&D.fsol, &D.frhs, &D.faux, &D.ftmp}) *v = dalloc<float>(P, H.ndof);
     if (l > 0) {
       D.p_nnz = (int64_t)H.p_col.size();
       D.r_nnz = (int64_t)H.r_col.size();
@@ -146,16 +156,12 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
       D.r_rowptr = dalloc<int>(P, H.r_rowptr.size());
       D.r_col = dalloc<int>(P, H.r_col.size());
       D.r_val = dalloc<double>(P, H.r_val.size());
-      D.p_val32 = dalloc<float>(P, H.p_val.size());
-      D.r_val32 = dalloc<float>(P, H.r_val.size());
       upload(P, D.p_rowptr, H.p_rowptr);
       upload(P, D.p_col, H.p_col);
       upload(P, D.p_val, H.p_val);
       upload(P, D.r_rowptr, H.r_rowptr);
       upload(P, D.r_col, H.r_col);
       upload(P, D.r_val, H.r_val);
-      k_to_f32(P.stream, D.p_nnz, D.p_val, D.p_val32);
-      k_to_f32(P.stream, D.r_nnz, D.r_val, D.r_val32);
     }
     // assembly data
     std::vector<int4> cells(H.ncell);
@@ -163,10 +169,6 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
       cells[c] = make_int4(H.org[3 * c], H.org[3 * c + 1], H.org[3 * c + 2], frame_code(&H.ax[9 * c]));
     D.cells = dalloc<int4>(P, H.ncell);
     upload(P, D.cells, cells);
-    std::vector<int> slots;
-    build_slots(H, m.order, slots);
-    D.slots = dalloc<int>(P, slots.size());
-    upload(P, D.slots, slots);
     D.gn = dalloc<int>(P, H.gn.size());
     upload(P, D.gn, H.gn);
     D.colour_cells = dalloc<int>(P, H.colour_cells.size());
@@ -180,14 +182,15 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
     D.lat_nint = (int64_t)H.lat_node2dof.size();
     D.ldof2node = dalloc<int>(P, H.dof_node.size());
     upload(P, D.ldof2node, H.dof_node);
-    if (l > 0) {  // the coarsest level is solved in FP64 from the DOF vectors
-      D.lval = dalloc<float>(P, H.lat_perm.size());
-      D.lperm = dalloc<int>(P, H.lat_perm.size());
-      upload(P, D.lperm, H.lat_perm);
-      D.linv = dalloc<float>(P, H.lat_node2dof.size());
-      D.lnode2dof = dalloc<int>(P, H.lat_node2dof.size());
-      upload(P, D.lnode2dof, H.lat_node2dof);
-    }
+    D.lperm = dalloc<int>(P, H.lat_perm.size());
+    upload(P, D.lperm, H.lat_perm);
+    D.lnode2dof = dalloc<int>(P, H.lat_node2dof.size());
+    upload(P, D.lnode2dof, H.lat_node2dof);
+    D.lcst = dalloc<float>(P, 8 * 128);
+    D.lscale = dalloc<double>(P, 8);
+    D.lflag = dalloc<int>(P, 8);
+    D.lelem = dalloc<double>(P, (size_t)P.nb * P.nb + P.nb);
+    D.lst64 = dalloc<double>(P, 8 * 128 + 8);
     const size_t nn3 = (size_t)H.nn * H.nn * H.nn;
     for (float **v : {&D.lx, &D.lf, &D.la, &D.lt}) {
       *v = dalloc<float>(P, nn3);  // boundary entries stay zero for good
@@ -202,20 +205,38 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
   P.red = dalloc<double>(P, 4 * kRedBlocks);
   P.dscal = dalloc<double>(P, 256);
   HIPCHK(hipHostMalloc((void **)&P.hscal, 256 * sizeof(double), hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void **)&P.hflag, 64 * sizeof(int), hipHostMallocDefault));
 }
 
 static void ensure_krylov(cd3d_problem &P, int restart) {
   if (restart <= P.krylov_restart) return;
   const int N = P.dl.back().ndof;
+  // the basis grows; w, xs (which carries the TCD3D state) and bs stay
+  if (P.V) {
+    dfree(P, P.V);
+    dfree(P, P.Z);
+  }
   P.V = dalloc<double>(P, (size_t)(restart + 1) * N);
   P.Z = dalloc<double>(P, (size_t)(restart + 1) * N);
-  P.w = dalloc<double>(P, N);
-  P.xs = dalloc<double>(P, N);
-  P.bs = dalloc<double>(P, N);
+  if (!P.w) {
+    P.w = dalloc<double>(P, N);
+    P.xs = dalloc<double>(P, N);
+    P.bs = dalloc<double>(P, N);
+  }
   P.krylov_restart = restart;
 }
 
 // ============================================================== assembly
+
+// CD3D_CELLWISE_ASSEMBLY=1 assembles constant coefficients cell by cell too
+// (the general kernel), for comparisons
+static bool uniform_assembly_off() {
+  static const bool v = [] {
+    const char *e = getenv("CD3D_CELLWISE_ASSEMBLY");
+    return e && atoi(e) != 0;
+  }();
+  return v;
+}
 
 static double sdfem_delta_host(double hconv, double eps, const double *b, int order) {
   if (fabs(eps) < 1e-20) eps = 1e-20;
@@ -245,8 +266,10 @@ static void check_coeffs(const cd3d_coeffs &co) {
 }
 
 // assemble level l: what 0 -> conv-diff into `val`, 1 -> mass into `val`;
-// rhs into D.rhs_asm (TAssembleMat3D::Reset + Assemble3D, AssembleMat3D.C:204,230)
-static void assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int what, double *val) {
+// rhs into D.rhs_asm (TAssembleMat3D::Reset + Assemble3D, AssembleMat3D.C:204,230).
+// Returns true when the constant-coefficient path ran (it leaves the FP32
+// compact stencils of `val` behind).
+static bool assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int what, double *val) {
   DevLevel &D = P.dl[l];
   AsmParams ap{};
   ap.what = what;
@@ -268,12 +291,27 @@ static void assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int wh
   ap.delta_const = 0;
   if (co.supg && co.conv_kind == 0)
     ap.delta_const = sdfem_delta_host(hconv_host(D.h, co.b), co.eps, co.b, P.mesh.order);
+  if (co.conv_kind == 0 && co.f_kind == 0 && D.nclass > 0 && !uniform_assembly_off()) {
+    // every active row is written: no clearing pass
+    kl_assemble_uniform(P.stream, D, P.mesh.order, P.tab, ap, val, D.rhs_asm);
+    k_dirichlet_rows(P.stream, D, val, D.rhs_asm);
+    HIPCHK(hipGetLastError());
+    return true;
+  }
+  if (!D.slots) {  // slot maps only for the cell-by-cell path (nb^2 ints per cell)
+    std::vector<int> slots;
+    build_slots(P.hl[l], P.mesh.order, slots);
+    D.slots = dalloc<int>(P, slots.size());
+    upload(P, D.slots, slots);
+    HIPCHK(hipStreamSynchronize(P.stream));
+  }
   HIPCHK(hipMemsetAsync(val, 0, sizeof(double) * D.nnz, P.stream));
   HIPCHK(hipMemsetAsync(D.rhs_asm, 0, sizeof(double) * D.ndof, P.stream));
   for (int k = 0; k < 8; k++)
     k_assemble(P.stream, D, P.tab, ap, val, D.rhs_asm, D.colour_start[k], D.colour_start[k + 1] - D.colour_start[k]);
   k_dirichlet_rows(P.stream, D, val, D.rhs_asm);
   HIPCHK(hipGetLastError());
+  return false;
 }
 
 static void refresh_coarse_and_f32(cd3d_problem &P) {
@@ -287,9 +325,19 @@ static void refresh_coarse_and_f32(cd3d_problem &P) {
   P.cinv_valid = true;
   for (size_t l = 0; l < P.dl.size(); l++) {
     DevLevel &D = P.dl[l];
-    k_to_f32(P.stream, D.nnz, D.val, D.val32);
     k_invdiag(P.stream, D);
-    if (l > 0) kl_refresh(P.stream, D);
+    if (l == 0) D.lat_fresh = false;  // the coarsest level has no FP32 stencils
+    if (l > 0) {
+      bool all_uniform = false;
+      HIPCHK(kl_detect_uniform(P.stream, D, P.hflag, &all_uniform));
+      if (!all_uniform) {  // per-node FP32 stencils (4 bytes per nonzero), allocated on first need
+        if (!D.lval) {
+          D.lval = dalloc<float>(P, (size_t)D.lat_nval);
+          D.linv = dalloc<float>(P, (size_t)D.lat_nint);
+        }
+        HIPCHK(kl_convert(P.stream, D));
+      }
+    }
   }
 }
 
@@ -356,42 +404,6 @@ void cycle64(VC &v, int i, bool zero_start) {
   if (x != L.sol) k_copy(P.stream, L.ndof, x, L.sol);
 }
 
-// mixed precision: FP32 smoothing, transfers and defects with FP32
-// accumulation; the coarsest solve stays FP64
-void coarse32(VC &v, bool zero_start) {
-  DevLevel &C = v.P.dl[0];
-  k_cast_f2d(v.P.stream, C.ndof, C.frhs, C.rhs);
-  if (!zero_start) k_cast_f2d(v.P.stream, C.ndof, C.fsol, C.sol);
-  coarse64(v, zero_start);
-  k_cast_d2f(v.P.stream, C.ndof, C.sol, C.fsol);
-}
-
-void cycle32(VC &v, int i, bool zero_start) {
-  cd3d_problem &P = v.P;
-  if (i == 0) {
-    coarse32(v, zero_start);
-    return;
-  }
-  DevLevel &L = P.dl[i], &C = P.dl[i - 1];
-  float *x = L.fsol, *t = L.ftmp;
-  const float om = (float)v.sp.omega;
-  for (int k = 0; k < v.sp.pre; k++) {
-    if (k == 0 && zero_start) k_jacobi_zero32(P.stream, L, L.frhs, t, om);
-    else k_jacobi32(P.stream, L, x, L.frhs, t, om);
-    std::swap(x, t);
-  }
-  k_defect32(P.stream, L, x, L.frhs, L.faux);
-  k_restrict32(P.stream, L, C, L.faux, C.frhs);
-  for (int r = 0; r < v.rec[i]; r++) cycle32(v, i - 1, r == 0);
-  if (v.sp.cycle < 1) v.rec[i] = 1;  // F-cycle
-  k_prolong_update32(P.stream, L, C, C.fsol, x, (float)v.sp.gmg_damp);
-  for (int k = 0; k < v.sp.post; k++) {
-    k_jacobi32(P.stream, L, x, L.frhs, t, om);
-    std::swap(x, t);
-  }
-  if (x != L.fsol) HIPCHK(hipMemcpyAsync(L.fsol, x, sizeof(float) * L.ndof, hipMemcpyDeviceToDevice, P.stream));
-}
-
 // mixed precision on the lattice-stencil form: the same cycle, with vectors
 // in node-lattice order whose boundary entries are zero (the Dirichlet part
 // of every preconditioner rhs and iterate vanishes)
@@ -432,14 +444,6 @@ void cycleL(VC &v, int i, bool zero_start) {
   if (x != L.lx) HIPCHK(hipMemcpyAsync(L.lx, x, sizeof(float) * nn3, hipMemcpyDeviceToDevice, P.stream));
 }
 
-bool mixed_csr() {
-  static const bool v = [] {
-    const char *e = getenv("CD3D_MIXED_CSR");
-    return e && atoi(e) != 0;
-  }();
-  return v;
-}
-
 // TMultiGridScaIte::Iterate (MultiGridScaIte.C:63-104): z = V(r), zero start.
 // rhs_dirichlet_zero: r vanishes on Dirichlet DOFs (always true for the
 // Krylov vectors of FGMRES), which lets the first sweep skip the matrix.
@@ -449,15 +453,9 @@ void precondition(VC &v, const double *r, double *z, bool rhs_dirichlet_zero) {
   const int top = (int)P.dl.size() - 1;
   DevLevel &F = P.dl[top];
   if (v.sp.precision == CD3D_MIXED && top > 0 && rhs_dirichlet_zero) {
-    if (mixed_csr()) {
-      k_cast_d2f(P.stream, F.ndof, r, F.frhs);
-      cycle32(v, top, true);
-      k_cast_f2d(P.stream, F.ndof, F.fsol, z);
-    } else {
-      kl_from_dof(P.stream, F, r, F.lf);
-      cycleL(v, top, true);
-      kl_to_dof(P.stream, F, F.lx, z);
-    }
+    kl_from_dof(P.stream, F, r, F.lf);
+    cycleL(v, top, true);
+    kl_to_dof(P.stream, F, F.lx, z);
     return;
   }
   k_copy(P.stream, F.ndof, r, F.rhs);
@@ -624,6 +622,44 @@ void cd3d_default_solver(cd3d_solver_params *p) {
   p->cycle = 1;
 }
 
+size_t cd3d_estimate_bytes(const cd3d_mesh_desc *mesh, int restart, int cellwise_assembly, int tcd) {
+  if (!mesh || (mesh->order != 1 && mesh->order != 2) || mesh->m < 1 || mesh->k0 < 0 || mesh->levels < 1 ||
+      mesh->levels > 12 || restart < 1)
+    return 0;
+  const double p = mesh->order, nb = (p + 1) * (p + 1) * (p + 1);
+  double total = 0, N_fine = 0, prevN = 0;
+  for (int l = 0; l < mesh->levels; l++) {
+    const double n = (double)mesh->m * std::ldexp(1.0, mesh->k0 + l), nn = p * n + 1, N = nn * nn * nn;
+    const double na = (nn - 2) * (nn - 2) * (nn - 2), ncell = n * n * n;
+    const double row = p == 1 ? 3 * (n - 1) : 8 * n - 5;  // active nonzeros per row and axis, summed
+    const double nact_nnz = row * row * row, nnz = nact_nnz + (N - na);
+    double b = 0;
+    b += 4 * (N + 1) + 4 * nnz + 4 * N + 8 * nnz;  // rowptr, col, diag, val
+    b += 4 * (nnz / 2048 + N / 256 + 2);           // row blocks
+    b += 8 * N + 5 * 8 * N;                        // 1/a_ii, sol rhs aux tmp rhs_asm
+    if (l > 0) {
+      const double pnnz = (p == 1 ? 3.375 : 8.0) * N;
+      b += 4 * (N + 1) + 4 * (prevN + 1) + 2 * 12 * pnnz;  // P and R = P^T
+    }
+    b += 16 * ncell + 4 * nb * ncell + 4 * ncell;   // cell geometry, DOF map, colours
+    b += 4 * N + 4 * nact_nnz + 4 * na + 4 * 4 * N;  // lattice: maps, stencil positions, vectors
+    b += 8 * (nb * nb + nb) + 8 * 8 * 129 + 4 * 8 * 128;
+    if (cellwise_assembly) {
+      b += 4 * nb * nb * ncell;                     // slot maps
+      if (l > 0) b += 4 * nact_nnz + 4 * na;        // per-node FP32 stencils
+    }
+    if (tcd) b += 8 * nnz;                          // conv-diff matrix beside the mass matrix
+    total += b + (2 << 20);                         // allocation granularity of ~20 buffers
+    if (l == 0) total += 8.0 * na * na + 8.0 * (2 * na * na + 1024);  // coarse inverse + Gauss-Jordan scratch
+    prevN = N;
+    N_fine = N;
+  }
+  total += 8.0 * N_fine * (2.0 * (restart + 1) + 3);  // Krylov basis V, Z and w, xs, bs
+  if (tcd) total += 3 * 8.0 * N_fine;
+  total += 8.0 * (4 * 1024 + 256) + 8.0 * 4096;     // reductions, scalars, basis tables
+  return (size_t)total;
+}
+
 cd3d_status cd3d_create(const cd3d_mesh_desc *mesh, cd3d_problem **out) {
   if (!mesh || !out) return CD3D_ERR_ARG;
   *out = nullptr;
@@ -644,6 +680,7 @@ cd3d_status cd3d_destroy(cd3d_problem *p) {
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   for (void *a : p->allocs) (void)hipFree(a);
   if (p->hscal) (void)hipHostFree(p->hscal);
+  if (p->hflag) (void)hipHostFree(p->hflag);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
   return CD3D_OK;
@@ -725,13 +762,14 @@ cd3d_status cd3d_assemble(cd3d_problem *p, const cd3d_coeffs *co) {
     check_coeffs(*co);
     for (int l = 0; l < (int)p->dl.size(); l++) {
       DevLevel &D = p->dl[l];
-      assemble_level(*p, l, *co, 0, D.val);
+      D.lat_fresh = assemble_level(*p, l, *co, 0, D.val);
       // SystemCD3D.C:330: Dirichlet part of the rhs into the level solution
       k_copy_range(p->stream, D.nact, D.ndof, D.rhs_asm, D.sol);
     }
     refresh_coarse_and_f32(*p);
     HIPCHK(hipStreamSynchronize(p->stream));
     p->assembled = true;
+    p->tcd_ready = false;  // the level matrices no longer hold the mass matrices
   });
 }
 
@@ -745,11 +783,8 @@ static void host_vec_op(cd3d_problem *p, int level, const double *in, double *ou
   op(din, dout);
   HIPCHK(hipMemcpyAsync(out, dout, sizeof(double) * nout, hipMemcpyDeviceToHost, p->stream));
   HIPCHK(hipStreamSynchronize(p->stream));
-  for (int k = 0; k < 2; k++) {
-    (void)hipFree(p->allocs.back());
-    p->allocs.pop_back();
-  }
-  p->dev_bytes -= sizeof(double) * (nin + nout);
+  dfree(*p, din);
+  dfree(*p, dout);
 }
 
 cd3d_status cd3d_matvec(cd3d_problem *p, int level, const double *x, double *y) {
@@ -874,85 +909,157 @@ cd3d_status cd3d_interp_pulse(cd3d_problem *p, int level, const cd3d_coeffs *co,
   return CD3D_OK;
 }
 
-cd3d_status cd3d_tcd_run(cd3d_problem *p, const cd3d_coeffs *co0, const cd3d_solver_params *sp, double t0, double tau,
-                         int nsteps, double *u, double *states, cd3d_stats *st) {
-  if (!p || !co0 || !sp || !u || nsteps < 0 || !(tau > 0)) return CD3D_ERR_ARG;
+// ---------------------------------------------------------------- TCD3D
+// TSystemTCD3D (SystemTCD3D.C:176-376) over the theta-scheme of
+// SetTimeDiscParameters (TimeDiscRout.C): theta1 = theta, theta2 = 1 - theta,
+// theta3 = 1 - theta, theta4 = theta (Crank-Nicolson 0.5, backward Euler 1).
+
+static void tcd_init_impl(cd3d_problem &P, const cd3d_coeffs &co0, double t0, const double *u) {
+  check_coeffs(co0);
+  if (co0.supg) throw ArgError("TCD3D is Galerkin only, as TSystemTCD3D::Init (SystemTCD3D.C:114-131)");
+  const int Lf = (int)P.dl.size() - 1;
+  DevLevel &F = P.dl[Lf];
+  const int N = F.ndof, NA = F.nact;
+  for (auto &D : P.dl)
+    if (!D.aconv) D.aconv = dalloc<double>(P, D.nnz);
+  if (!P.tcd_oldrhs) {
+    P.tcd_oldrhs = dalloc<double>(P, N);
+    P.tcd_oldsol = dalloc<double>(P, N);
+    P.tcd_def = dalloc<double>(P, N);
+  }
+  ensure_krylov(P, 1);
+  // AssembleMRhs at t0 on every level (SystemTCD3D.C:176-202): the level
+  // matrices hold the mass matrices from here on
+  cd3d_coeffs co = co0;
+  co.t = t0;
+  P.assembled = false;
+  for (int l = 0; l <= Lf; l++) assemble_level(P, l, co, 1, P.dl[l].val);
+  HIPCHK(hipMemcpyAsync(P.xs, u, sizeof(double) * N, hipMemcpyHostToDevice, P.stream));
+  k_copy_range(P.stream, NA, N, F.rhs_asm, P.xs);
+  k_copy(P.stream, N, F.rhs_asm, P.tcd_oldrhs);
+  HIPCHK(hipStreamSynchronize(P.stream));
+  P.tcd_t = t0;
+  P.tcd_ready = true;
+}
+
+// one step t -> t + tau; returns the FGMRES iterations
+static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_solver_params &sp, double tau,
+                         double theta, cd3d_status &solve_status) {
+  check_coeffs(co0);
+  if (co0.supg) throw ArgError("TCD3D is Galerkin only, as TSystemTCD3D::Init (SystemTCD3D.C:114-131)");
+  if (!P.tcd_ready) throw std::logic_error("cd3d_tcd_init before cd3d_tcd_step");
+  const double th1 = theta, th2 = 1 - theta, th3 = 1 - theta, th4 = theta;
+  const int Lf = (int)P.dl.size() - 1;
+  DevLevel &F = P.dl[Lf];
+  const int N = F.ndof, NA = F.nact;
+  ensure_krylov(P, std::min(sp.restart, sp.maxit));
+  k_copy(P.stream, N, P.xs, P.tcd_oldsol);
+  cd3d_coeffs co = co0;
+  co.t = P.tcd_t + tau;
+  // AssembleARhs at t_new (SystemTCD3D.C:204-226)
+  for (int l = 0; l <= Lf; l++) assemble_level(P, l, co, 0, P.dl[l].aconv);
+  k_copy_range(P.stream, NA, N, F.rhs_asm, P.xs);
+  // AssembleSystMat (SystemTCD3D.C:228-294): B = tau th3 f_old + tau th4 f_new
+  // + (M - tau th2 A) u_old on the active rows, system matrix M + tau th1 A
+  HIPCHK(hipMemsetAsync(P.bs, 0, sizeof(double) * N, P.stream));
+  k_axpy(P.stream, NA, tau * th3, P.tcd_oldrhs, P.bs);
+  k_axpy(P.stream, NA, tau * th4, F.rhs_asm, P.bs);
+  k_matadd(P.stream, P.hl[Lf].rowptr[NA], F.val, F.aconv, -tau * th2);
+  const double gamma0 = -tau * th2;
+  HIPCHK(hipMemsetAsync(P.tcd_def, 0, sizeof(double) * N, P.stream));
+  k_matvec_active(P.stream, F, P.tcd_oldsol, P.tcd_def);
+  k_axpy(P.stream, NA, 1.0, P.tcd_def, P.bs);
+  k_copy_range(P.stream, NA, N, F.rhs_asm, P.bs);
+  k_copy_range(P.stream, NA, N, F.rhs_asm, P.xs);
+  for (int l = 0; l <= Lf; l++) {
+    DevLevel &D = P.dl[l];
+    k_matadd(P.stream, P.hl[l].rowptr[D.nact], D.val, D.aconv, l == Lf ? -gamma0 + tau * th1 : tau * th1);
+  }
+  k_copy(P.stream, N, F.rhs_asm, P.tcd_oldrhs);
+  refresh_coarse_and_f32(P);
+  // Solve (SystemTCD3D.C:317-376)
+  double r0;
+  const int its = fgmres(P, sp, r0, solve_status);
+  // RestoreMassMat (SystemTCD3D.C:296-315)
+  for (int l = 0; l <= Lf; l++) {
+    DevLevel &D = P.dl[l];
+    k_matadd(P.stream, P.hl[l].rowptr[D.nact], D.val, D.aconv, -tau * th1);
+  }
+  P.tcd_t += tau;
+  return its;
+}
+
+static void check_theta(double theta) {
+  if (!(theta >= 0.5 && theta <= 1.0)) throw ArgError("theta must lie in [0.5, 1] (Crank-Nicolson .. backward Euler)");
+}
+
+cd3d_status cd3d_tcd_init(cd3d_problem *p, const cd3d_coeffs *co, double t0, const double *u_host) {
+  if (!p || !co || !u_host) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  return run(p, [&]() { tcd_init_impl(*p, *co, t0, u_host); });
+}
+
+cd3d_status cd3d_tcd_step(cd3d_problem *p, const cd3d_coeffs *co, const cd3d_solver_params *sp, double tau,
+                          double theta, double *u_host, cd3d_stats *st) {
+  if (!p || !co || !sp || !(tau > 0)) return CD3D_ERR_ARG;
   std::lock_guard<std::mutex> g(p->mu);
   cd3d_status solve_status = CD3D_OK;
+  const double t0 = now_ms();
+  int its = 0;
   cd3d_status rc = run(p, [&]() {
     check_params(*sp);
-    check_coeffs(*co0);
-    if (co0->supg) throw ArgError("TCD3D is Galerkin only, as TSystemTCD3D::Init (SystemTCD3D.C:114-131)");
-    cd3d_coeffs co = *co0;
-    const int Lf = (int)p->dl.size() - 1;
-    DevLevel &F = p->dl[Lf];
-    const int N = F.ndof, NA = F.nact;
-    const double th1 = 0.5, th2 = 0.5, th3 = 0.5, th4 = 0.5;  // Crank-Nicolson (TimeDiscRout.C)
-    ensure_krylov(*p, std::min(sp->restart, sp->maxit));
-    for (auto &D : p->dl)
-      if (!D.aconv) D.aconv = dalloc<double>(*p, D.nnz);
-    double *oldrhs = dalloc<double>(*p, N), *oldsol = dalloc<double>(*p, N), *def = dalloc<double>(*p, N);
-    const double tstart = now_ms();
-    // AssembleMRhs at t0 on every level (SystemTCD3D.C:176-202)
-    co.t = t0;
-    for (int l = 0; l <= Lf; l++) assemble_level(*p, l, co, 1, p->dl[l].val);
-    HIPCHK(hipMemcpyAsync(p->xs, u, sizeof(double) * N, hipMemcpyHostToDevice, p->stream));
-    k_copy_range(p->stream, NA, N, F.rhs_asm, p->xs);
-    k_copy(p->stream, N, F.rhs_asm, oldrhs);
-    double t = t0;
-    int total_its = 0;
-    for (int m = 0; m < nsteps; m++) {
-      k_copy(p->stream, N, p->xs, oldsol);
-      t += tau;
-      co.t = t;
-      // AssembleARhs at t_new (SystemTCD3D.C:204-226)
-      for (int l = 0; l <= Lf; l++) assemble_level(*p, l, co, 0, p->dl[l].aconv);
-      k_copy_range(p->stream, NA, N, F.rhs_asm, p->xs);
-      // AssembleSystMat (SystemTCD3D.C:228-294)
-      HIPCHK(hipMemsetAsync(p->bs, 0, sizeof(double) * N, p->stream));
-      k_axpy(p->stream, NA, tau * th3, oldrhs, p->bs);
-      k_axpy(p->stream, NA, tau * th4, F.rhs_asm, p->bs);
-      k_matadd(p->stream, p->hl[Lf].rowptr[NA], F.val, F.aconv, -tau * th2);
-      double gamma = -tau * th2;
-      HIPCHK(hipMemsetAsync(def, 0, sizeof(double) * N, p->stream));
-      k_matvec_active(p->stream, F, oldsol, def);
-      k_axpy(p->stream, NA, 1.0, def, p->bs);
-      k_copy_range(p->stream, NA, N, F.rhs_asm, p->bs);
-      k_copy_range(p->stream, NA, N, F.rhs_asm, p->xs);
-      for (int l = 0; l <= Lf; l++) {
-        DevLevel &D = p->dl[l];
-        k_matadd(p->stream, p->hl[l].rowptr[D.nact], D.val, D.aconv, l == Lf ? -gamma + tau * th1 : tau * th1);
-      }
-      gamma = tau * th1;
-      k_copy(p->stream, N, F.rhs_asm, oldrhs);
-      refresh_coarse_and_f32(*p);
-      // Solve (SystemTCD3D.C:317-376)
-      double r0;
-      const int its = fgmres(*p, *sp, r0, solve_status);
-      total_its += its;
-      // RestoreMassMat (SystemTCD3D.C:296-315)
-      for (int l = 0; l <= Lf; l++) {
-        DevLevel &D = p->dl[l];
-        k_matadd(p->stream, p->hl[l].rowptr[D.nact], D.val, D.aconv, -gamma);
-      }
+    check_theta(theta);
+    its = tcd_step_impl(*p, *co, *sp, tau, theta, solve_status);
+    const int N = p->dl.back().ndof;
+    if (u_host) HIPCHK(hipMemcpyAsync(u_host, p->xs, sizeof(double) * N, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+  });
+  if (st) {
+    st->iterations = its;
+    st->status = rc != CD3D_OK ? rc : solve_status;
+    st->r0 = st->rfinal = 0;
+    st->ms_assemble = 0;
+    st->ms_solve = now_ms() - t0;
+  }
+  return rc != CD3D_OK ? rc : solve_status;
+}
+
+cd3d_status cd3d_tcd_get_state(cd3d_problem *p, double *u_host) {
+  if (!p || !u_host) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  return run(p, [&]() {
+    if (!p->tcd_ready) throw std::logic_error("cd3d_tcd_init before cd3d_tcd_get_state");
+    HIPCHK(hipMemcpyAsync(u_host, p->xs, sizeof(double) * p->dl.back().ndof, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+  });
+}
+
+cd3d_status cd3d_tcd_run(cd3d_problem *p, const cd3d_coeffs *co, const cd3d_solver_params *sp, double t0, double tau,
+                         int nsteps, double *u, double *states, cd3d_stats *st) {
+  if (!p || !co || !sp || !u || nsteps < 0 || !(tau > 0)) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  cd3d_status solve_status = CD3D_OK;
+  int total_its = 0;
+  const double tstart = now_ms();
+  cd3d_status rc = run(p, [&]() {
+    check_params(*sp);
+    tcd_init_impl(*p, *co, t0, u);
+    const int N = p->dl.back().ndof;
+    for (int m = 0; m < nsteps && solve_status == CD3D_OK; m++) {
+      total_its += tcd_step_impl(*p, *co, *sp, tau, 0.5, solve_status);
       if (states)
         HIPCHK(hipMemcpyAsync(states + (size_t)m * N, p->xs, sizeof(double) * N, hipMemcpyDeviceToHost, p->stream));
-      if (solve_status != CD3D_OK) break;
     }
     HIPCHK(hipMemcpyAsync(u, p->xs, sizeof(double) * N, hipMemcpyDeviceToHost, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
-    for (int k = 0; k < 3; k++) {
-      (void)hipFree(p->allocs.back());
-      p->allocs.pop_back();
-    }
-    p->dev_bytes -= 3 * sizeof(double) * N;
-    p->assembled = false;  // the level matrices now hold the mass matrices
-    if (st) {
-      st->iterations = total_its;
-      st->status = solve_status;
-      st->ms_solve = now_ms() - tstart;
-    }
   });
+  if (st) {
+    st->iterations = total_its;
+    st->status = rc != CD3D_OK ? rc : solve_status;
+    st->r0 = st->rfinal = 0;
+    st->ms_assemble = 0;
+    st->ms_solve = now_ms() - tstart;
+  }
   return rc != CD3D_OK ? rc : solve_status;
 }
 

@@ -72,21 +72,17 @@ struct DevLevel {
   int64_t nnz = 0;
   int *rowptr = nullptr, *col = nullptr, *diag = nullptr;
   double *val = nullptr;
-  float *val32 = nullptr;
   // row blocks of the stream kernels (at most 2048 nonzeros and 256 rows)
   int *blk = nullptr;
   int nblk = 0;
   double *invdiag = nullptr;  // 1 / a_ii
-  float *invdiag32 = nullptr;
   double *sol = nullptr, *rhs = nullptr, *aux = nullptr, *tmp = nullptr;
   double *rhs_asm = nullptr;  // assembled rhs (the multigrid works in `rhs`)
-  float *fsol = nullptr, *frhs = nullptr, *faux = nullptr, *ftmp = nullptr;
   // TCD3D: conv-diff matrix next to the system (mass) matrix in `val`
   double *aconv = nullptr;
   // transfer (level >= 1)
   int *p_rowptr = nullptr, *p_col = nullptr, *r_rowptr = nullptr, *r_col = nullptr;
   double *p_val = nullptr, *r_val = nullptr;
-  float *p_val32 = nullptr, *r_val32 = nullptr;
   int64_t p_nnz = 0, r_nnz = 0;
   // assembly
   int4 *cells = nullptr;
@@ -108,6 +104,16 @@ struct DevLevel {
   int *lnode2dof = nullptr;    // interior node -> DOF
   int *ldof2node = nullptr;    // DOF -> lattice node
   float *lx = nullptr, *lf = nullptr, *la = nullptr, *lt = nullptr;  // lattice vectors, nn^3 each
+  // classes whose stencil is the same at every node (constant coefficients on
+  // the uniform mesh) use one compact stencil: 128 floats per class, the last 1/a_ii
+  int uni[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  float *lcst = nullptr;
+  double *lscale = nullptr;
+  int *lflag = nullptr;
+  // constant-coefficient assembly: element matrix (NB x NB + element rhs) and
+  // FP64 class stencils (8 x 128 + 8 class rhs)
+  double *lelem = nullptr, *lst64 = nullptr;
+  bool lat_fresh = false;  // lcst already holds the stencils of `val`
 };
 
 struct Problem {
@@ -119,6 +125,7 @@ struct Problem {
   int device = 0;
   size_t dev_bytes = 0;
   std::vector<void *> allocs;
+  std::vector<size_t> alloc_bytes;
   std::string err;
   std::mutex mu;
   bool assembled = false;
@@ -132,7 +139,12 @@ struct Problem {
   double *red = nullptr;      // reduction partials
   double *dscal = nullptr;    // device scalars (H column, norms)
   double *hscal = nullptr;    // pinned host mirror of dscal
+  int *hflag = nullptr;       // pinned, lattice uniformity flags
   int red_blocks = 0;
+  // TCD3D state: f(t_old), u_old, a defect buffer; the current state is in xs
+  double *tcd_oldrhs = nullptr, *tcd_oldsol = nullptr, *tcd_def = nullptr;
+  double tcd_t = 0;
+  bool tcd_ready = false;
   // device tables for assembly (reference basis at quadrature points)
   double *tab = nullptr;
 };
@@ -162,16 +174,6 @@ void k_prolong(hipStream_t s, const DevLevel &F, const DevLevel &C, const double
 void k_gemv_dense(hipStream_t s, int n, const double *A, const double *x, double *y);
 void k_coarse_smooth(hipStream_t s, const DevLevel &L, double *x, const double *f, double *d, int smoother,
                      double omega, int maxit, double red);
-// float variants (mixed-precision V-cycle)
-void k_jacobi32(hipStream_t s, const DevLevel &L, const float *x, const float *f, float *out, float omega);
-void k_jacobi_zero32(hipStream_t s, const DevLevel &L, const float *f, float *out, float omega);
-void k_defect32(hipStream_t s, const DevLevel &L, const float *x, const float *f, float *d);
-void k_restrict32(hipStream_t s, const DevLevel &F, const DevLevel &C, const float *fine, float *coarse);
-void k_prolong_update32(hipStream_t s, const DevLevel &F, const DevLevel &C, const float *coarse, float *x,
-                        float damp);
-void k_cast_d2f(hipStream_t s, int n, const double *a, float *b);
-void k_cast_f2d(hipStream_t s, int n, const float *a, double *b);
-void k_to_f32(hipStream_t s, int64_t n, const double *a, float *b);
 // BLAS-1 with deterministic two-stage reductions
 void k_dot(hipStream_t s, int n, const double *a, const double *b, double *partials, double *out);
 void k_axpy_dot(hipStream_t s, int n, const double *alpha_partials, double sign, const double *x, double *y,
@@ -204,8 +206,16 @@ void kl_defect(hipStream_t s, const DevLevel &L, int order, const float *x, cons
 void kl_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, int order, const float *fine, float *coarse);
 void kl_prolong_update(hipStream_t s, const DevLevel &F, const DevLevel &C, int order, const float *coarse, float *x,
                        float damp);
-// stencil values and 1/a_ii from the FP64 CSR matrix
-void kl_refresh(hipStream_t s, const DevLevel &L);
+// classes whose stencil is the same at every node (within 1e-12 of the
+// largest entry) get the compact form; synchronises the stream (flag_host:
+// pinned, 8 ints).  all_uniform false: kl_convert must fill lval / linv.
+hipError_t kl_detect_uniform(hipStream_t s, DevLevel &L, int *flag_host, bool *all_uniform);
+// per-node FP32 stencil values and 1/a_ii from the FP64 CSR matrix
+hipError_t kl_convert(hipStream_t s, const DevLevel &L);
+// constant-coefficient assembly of the active rows into val / rhs through
+// the class stencils (also writes the FP32 compact stencils)
+void kl_assemble_uniform(hipStream_t s, DevLevel &L, int order, const double *tab, const AsmParams &ap,
+                         double *val, double *rhs);
 // DOF-ordered f64 <-> lattice f32 (Dirichlet DOFs map to zero)
 void kl_from_dof(hipStream_t s, const DevLevel &L, const double *v, float *l);
 void kl_to_dof(hipStream_t s, const DevLevel &L, const float *l, double *v);

@@ -337,16 +337,6 @@ __global__ void matadd_k(int64_t n, double *__restrict__ C, const double *__rest
     C[i] += alpha * D[i];
 }
 
-__global__ void d2f_k(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-    b[i] = (float)a[i];
-}
-
-__global__ void f2d_k(int64_t n, const float *__restrict__ a, double *__restrict__ b) {
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-    b[i] = (double)a[i];
-}
-
 inline int grid_for(int64_t n) {
   int64_t g = (n + kBlock - 1) / kBlock;
   return (int)(g < 2048 ? (g < 1 ? 1 : g) : 2048);
@@ -621,11 +611,22 @@ __global__ __launch_bounds__(kBlock) void gj_init_k(int na, const int *__restric
   }
 }
 
+// arg-max over the block partials (largest value, ties to the lower row),
+// computed by every wavefront: lanes stride the partials, then a butterfly
 __device__ inline int gj_pivot(const double *__restrict__ pv, const int *__restrict__ pi, int nparts) {
   double best = -1.0;
   int bi = 1 << 30;
-  for (int t = 0; t < nparts; t++)
-    if (pv[t] > best || (pv[t] == best && pi[t] < bi)) { best = pv[t]; bi = pi[t]; }
+  for (int t = threadIdx.x & 63; t < nparts; t += 64) {
+    const double v = pv[t];
+    const int r = pi[t];
+    if (v > best || (v == best && r < bi)) { best = v; bi = r; }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const double v = __shfl_xor(best, o);
+    const int r = __shfl_xor(bi, o);
+    if (v > best || (v == best && r < bi)) { best = v; bi = r; }
+  }
   return bi;
 }
 
@@ -719,7 +720,6 @@ void k_jacobi(hipStream_t s, const DevLevel &L, const double *x, const double *f
 }
 void k_invdiag(hipStream_t s, const DevLevel &L) {
   invdiag_k<double><<<(L.ndof + kBlock - 1) / kBlock, kBlock, 0, s>>>(L.ndof, L.diag, L.val, L.invdiag);
-  invdiag_k<float><<<(L.ndof + kBlock - 1) / kBlock, kBlock, 0, s>>>(L.ndof, L.diag, L.val, L.invdiag32);
 }
 void k_jacobi_zero(hipStream_t s, const DevLevel &L, const double *f, double *out, double omega) {
   jacobi_zero_k<double><<<(L.ndof + kBlock - 1) / kBlock, kBlock, 0, s>>>(L.ndof, L.nact, L.diag, L.val, nullptr,
@@ -743,30 +743,6 @@ void k_prolong(hipStream_t s, const DevLevel &F, const DevLevel &C, const double
   launch_rows<kSpmv, double, double>(s, F.p_nnz, F.ndof, F.nact, F.p_rowptr, F.p_col, F.p_val,
                                      coarse, nullptr, fine, 0.0);
 }
-void k_jacobi32(hipStream_t s, const DevLevel &L, const float *x, const float *f, float *out, float omega) {
-  launch_stream<kJacobi, float, float>(s, L, L.val32, x, f, L.invdiag32, out, omega);
-}
-void k_jacobi_zero32(hipStream_t s, const DevLevel &L, const float *f, float *out, float omega) {
-  jacobi_zero_k<float><<<(L.ndof + kBlock - 1) / kBlock, kBlock, 0, s>>>(L.ndof, L.nact, L.diag, nullptr, L.val32,
-                                                                        f, out, omega);
-}
-void k_defect32(hipStream_t s, const DevLevel &L, const float *x, const float *f, float *d) {
-  launch_stream<kDefect, float, float>(s, L, L.val32, x, f, nullptr, d, 0.0f);
-}
-void k_restrict32(hipStream_t s, const DevLevel &F, const DevLevel &C, const float *fine, float *coarse) {
-  launch_rows<kRestrict, float, float>(s, F.r_nnz, C.ndof, C.nact, F.r_rowptr, F.r_col,
-                                       F.r_val32, fine, nullptr, coarse, 0.0f);
-}
-void k_prolong_update32(hipStream_t s, const DevLevel &F, const DevLevel &C, const float *coarse, float *x,
-                        float damp) {
-  (void)C;
-  launch_rows<kProlongUpd, float, float>(s, F.p_nnz, F.ndof, F.nact, F.p_rowptr, F.p_col,
-                                         F.p_val32, coarse, nullptr, x, damp);
-}
-void k_cast_d2f(hipStream_t s, int n, const double *a, float *b) { d2f_k<<<grid_for(n), kBlock, 0, s>>>(n, a, b); }
-void k_cast_f2d(hipStream_t s, int n, const float *a, double *b) { f2d_k<<<grid_for(n), kBlock, 0, s>>>(n, a, b); }
-void k_to_f32(hipStream_t s, int64_t n, const double *a, float *b) { d2f_k<<<grid_for(n), kBlock, 0, s>>>(n, a, b); }
-
 void k_dot(hipStream_t s, int n, const double *a, const double *b, double *partials, double *out) {
   dot_k<<<kRedBlocks, kBlock, 0, s>>>(n, a, b, partials);
   if (out) finish_k<<<1, kBlock, 0, s>>>(partials, out);

@@ -14,6 +14,8 @@
 // 3/8 -1/8 and 3/4 1 3/4), so they need no matrix at all.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "cd3d_internal.hpp"
 
 namespace cd3d {
@@ -31,6 +33,7 @@ struct LatArgs {
   int nn, nclass;
   LatDev c[8];
   int bstart[9];  // first block of each class
+  int uni[8];     // class stencil the same at every node: compact values
 };
 
 enum LatMode { kLJacobi = 0, kLJacobiZero = 1, kLDefect = 2 };
@@ -40,15 +43,22 @@ __host__ __device__ inline constexpr int lo_off() { return (P == 1 || PXS == 0) 
 template <int P, int PXS>
 __host__ __device__ inline constexpr int hi_off() { return (P == 1 || PXS == 0) ? P : 1; }
 
-template <int P, int PX, int PY, int PZ, int MODE>
+// Compact class stencil: kCst floats per class, entry kCst - 1 is 1 / a_ii.
+constexpr int kCst = 128;
+
+// UNI: the class's stencil (and diagonal) are the same at every node and are
+// read from cst (uniform across the block); otherwise per-node values.
+template <int P, int PX, int PY, int PZ, int MODE, bool UNI>
 __host__ __device__ inline void lat_node(const LatArgs &a, const LatDev &C, long long k, const float *__restrict__ val,
-                                const float *__restrict__ inv, const float *__restrict__ x,
-                                const float *__restrict__ f, float *__restrict__ out, float omega) {
+                                         const float *__restrict__ inv, const float *__restrict__ cst,
+                                         const float *__restrict__ x, const float *__restrict__ f,
+                                         float *__restrict__ out, float omega) {
   const int kx = (int)(k % C.cnt[0]), ky = (int)((k / C.cnt[0]) % C.cnt[1]), kz = (int)(k / ((long long)C.cnt[0] * C.cnt[1]));
   const int X = C.lo[0] + C.step * kx, Y = C.lo[1] + C.step * ky, Z = C.lo[2] + C.step * kz;
   const long long nn = a.nn, node = X + nn * (Y + nn * Z);
+  const float di = UNI ? cst[kCst - 1] : (MODE == kLDefect ? 0.f : inv[C.first + k]);
   if (MODE == kLJacobiZero) {
-    out[node] = omega * f[node] * inv[C.first + k];
+    out[node] = omega * f[node] * di;
     return;
   }
   const float *__restrict__ v = val + C.off + k;
@@ -60,41 +70,52 @@ __host__ __device__ inline void lat_node(const LatArgs &a, const LatDev &C, long
     for (int dy = lo_off<P, PY>(); dy <= hi_off<P, PY>(); dy++)
 #pragma unroll
       for (int dx = lo_off<P, PX>(); dx <= hi_off<P, PX>(); dx++, si++)
-        s += v[(long long)si * C.nodes] * x[node + dx + nn * (dy + nn * dz)];
+        s += (UNI ? cst[si] : v[(long long)si * C.nodes]) * x[node + dx + nn * (dy + nn * dz)];
   if (MODE == kLDefect) out[node] = f[node] - s;
-  else out[node] = x[node] + omega * (f[node] - s) * inv[C.first + k];
+  else out[node] = x[node] + omega * (f[node] - s) * di;
+}
+
+// node k of class c (the class fixes the parities, hence the stencil shape)
+template <int P, int PX, int PY, int PZ, int MODE>
+__host__ __device__ inline void lat_node_any(const LatArgs &a, int c, long long k, const float *__restrict__ val,
+                                             const float *__restrict__ inv, const float *__restrict__ cst,
+                                             const float *__restrict__ x, const float *__restrict__ f,
+                                             float *__restrict__ out, float omega) {
+  if (a.uni[c]) lat_node<P, PX, PY, PZ, MODE, true>(a, a.c[c], k, val, inv, cst + c * kCst, x, f, out, omega);
+  else lat_node<P, PX, PY, PZ, MODE, false>(a, a.c[c], k, val, inv, cst, x, f, out, omega);
 }
 
 // node k of class c (the class fixes the parities, hence the stencil shape)
 template <int P, int MODE>
 __host__ __device__ inline void lat_class_node(const LatArgs &a, int c, long long k, const float *__restrict__ val,
-                                               const float *__restrict__ inv, const float *__restrict__ x,
-                                               const float *__restrict__ f, float *__restrict__ out, float omega) {
-  const LatDev &C = a.c[c];
+                                               const float *__restrict__ inv, const float *__restrict__ cst,
+                                               const float *__restrict__ x, const float *__restrict__ f,
+                                               float *__restrict__ out, float omega) {
   if (P == 1) {
-    lat_node<1, 0, 0, 0, MODE>(a, C, k, val, inv, x, f, out, omega);
+    lat_node_any<1, 0, 0, 0, MODE>(a, c, k, val, inv, cst, x, f, out, omega);
   } else {
     switch (c) {
-      case 0: lat_node<2, 0, 0, 0, MODE>(a, C, k, val, inv, x, f, out, omega); break;
-      case 1: lat_node<2, 1, 0, 0, MODE>(a, C, k, val, inv, x, f, out, omega); break;
-      case 2: lat_node<2, 0, 1, 0, MODE>(a, C, k, val, inv, x, f, out, omega); break;
-      case 3: lat_node<2, 1, 1, 0, MODE>(a, C, k, val, inv, x, f, out, omega); break;
-      case 4: lat_node<2, 0, 0, 1, MODE>(a, C, k, val, inv, x, f, out, omega); break;
-      case 5: lat_node<2, 1, 0, 1, MODE>(a, C, k, val, inv, x, f, out, omega); break;
-      case 6: lat_node<2, 0, 1, 1, MODE>(a, C, k, val, inv, x, f, out, omega); break;
-      default: lat_node<2, 1, 1, 1, MODE>(a, C, k, val, inv, x, f, out, omega); break;
+      case 0: lat_node_any<2, 0, 0, 0, MODE>(a, c, k, val, inv, cst, x, f, out, omega); break;
+      case 1: lat_node_any<2, 1, 0, 0, MODE>(a, c, k, val, inv, cst, x, f, out, omega); break;
+      case 2: lat_node_any<2, 0, 1, 0, MODE>(a, c, k, val, inv, cst, x, f, out, omega); break;
+      case 3: lat_node_any<2, 1, 1, 0, MODE>(a, c, k, val, inv, cst, x, f, out, omega); break;
+      case 4: lat_node_any<2, 0, 0, 1, MODE>(a, c, k, val, inv, cst, x, f, out, omega); break;
+      case 5: lat_node_any<2, 1, 0, 1, MODE>(a, c, k, val, inv, cst, x, f, out, omega); break;
+      case 6: lat_node_any<2, 0, 1, 1, MODE>(a, c, k, val, inv, cst, x, f, out, omega); break;
+      default: lat_node_any<2, 1, 1, 1, MODE>(a, c, k, val, inv, cst, x, f, out, omega); break;
     }
   }
 }
 
 template <int P, int MODE>
 __global__ __launch_bounds__(kB) void lat_sweep_k(LatArgs a, const float *__restrict__ val,
-                                                  const float *__restrict__ inv, const float *__restrict__ x,
-                                                  const float *__restrict__ f, float *__restrict__ out, float omega) {
+                                                  const float *__restrict__ inv, const float *__restrict__ cst,
+                                                  const float *__restrict__ x, const float *__restrict__ f,
+                                                  float *__restrict__ out, float omega) {
   int c = 0;
   while (c + 1 < a.nclass && (int)blockIdx.x >= a.bstart[c + 1]) c++;
   const long long k = (long long)(blockIdx.x - a.bstart[c]) * kB + threadIdx.x;
-  if (k < a.c[c].nodes) lat_class_node<P, MODE>(a, c, k, val, inv, x, f, out, omega);
+  if (k < a.c[c].nodes) lat_class_node<P, MODE>(a, c, k, val, inv, cst, x, f, out, omega);
 }
 
 // 1D transfer weights.  Restriction gathers, for a coarse node Xc, the fine
@@ -196,10 +217,169 @@ __global__ void lat_convert_k(long long n, const int *__restrict__ perm, const d
     out[i] = (float)val[perm[i]];
 }
 
+// Per class: the largest |a| of the first node's stencil (the scale of the
+// uniformity test) and the compact stencil of that node; clears the flag.
+__global__ void lat_first_node_k(LatArgs a, const int *__restrict__ perm, const int *__restrict__ node2dof,
+                                 const int *__restrict__ diag, const double *__restrict__ val,
+                                 double *__restrict__ scale, int *__restrict__ flag, float *__restrict__ cst) {
+  const int c = blockIdx.x;
+  const LatDev &C = a.c[c];
+  double m = 0.0;
+  if (C.nodes > 0)
+    for (int s = threadIdx.x; s < C.nsten; s += 64) {
+      const double v = val[perm[C.off + s * C.nodes]];
+      cst[c * kCst + s] = (float)v;
+      m = fmax(m, fabs(v));
+    }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o));
+  if (threadIdx.x == 0) {
+    scale[c] = m;
+    flag[c] = C.nodes > 0 ? 0 : 1;
+    if (C.nodes > 0) cst[c * kCst + kCst - 1] = (float)(1.0 / val[diag[node2dof[C.first]]]);
+  }
+}
+
+// flag[c] = 1 when some node of class c has a stencil entry that differs from
+// the first node's by more than 1e-12 of the stencil's largest entry
+__global__ void lat_uniform_k(LatArgs a, long long n, const int *__restrict__ perm, const double *__restrict__ val,
+                              const double *__restrict__ scale, int *__restrict__ flag) {
+  for (long long i = (long long)blockIdx.x * kB + threadIdx.x; i < n; i += (long long)gridDim.x * kB) {
+    int c = 0;
+    while (c + 1 < a.nclass && i >= a.c[c + 1].off) c++;
+    const LatDev &C = a.c[c];
+    const long long s = (i - C.off) / C.nodes;
+    const double v = val[perm[i]], v0 = val[perm[C.off + s * C.nodes]];
+    if (fabs(v - v0) > 1e-12 * scale[c]) flag[c] = 1;
+  }
+}
+
 __global__ void lat_invdiag_k(long long n, const int *__restrict__ node2dof, const int *__restrict__ diag,
                               const double *__restrict__ val, float *__restrict__ inv) {
   const long long i = (long long)blockIdx.x * kB + threadIdx.x;
   if (i < n) inv[i] = (float)(1.0 / val[diag[node2dof[i]]]);
+}
+
+// ------------------------------------------------------------------------
+// Assembly with constant coefficients (TAssembleMat3D::Assemble3D with
+// BilinearAssemble / BilinearAssemble_SD, AssembleMat3D.C:230,
+// ConvDiff3D.C:61,124).  On the uniform brick mesh every cell has the same
+// physical element matrix -- the child frames only renumber its rows and
+// columns -- so the element matrix is integrated once (elem_k, the integrand
+// of the per-cell kernel in the identity frame), folded into one stencil per
+// parity class (the at most 8 cells around an interior node), and written to
+// the CSR values of every interior row; Dirichlet rows are set apart.
+
+// E[i][j] (NB x NB, cell-local lexicographic numbering) and the element rhs
+// E[NB*NB + i]; one thread per entry
+template <int P>
+__global__ __launch_bounds__(1024) void elem_k(const double *__restrict__ tab, AsmParams ap, double *__restrict__ E) {
+  constexpr int NB = (P + 1) * (P + 1) * (P + 1), N1 = P + 2, NQ = N1 * N1 * N1, STRIDE = 4 + 4 * NB;
+  const int t = threadIdx.x;
+  if (t >= NB * NB) return;
+  const int i = t / NB, j = t % NB;
+  const double detj = (ap.h[0] / 2) * (ap.h[1] / 2) * (ap.h[2] / 2);
+  double kap[3], beta[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    kap[a] = ap.eps * 4.0 / (ap.h[a] * ap.h[a]);
+    beta[a] = ap.b[a] * 2.0 / ap.h[a];
+  }
+  const double delta = (ap.supg && ap.what == 0) ? ap.delta_const : 0.0;
+  double acc = 0.0, racc = 0.0;
+  for (int q = 0; q < NQ; q++) {
+    const double *T = tab + q * STRIDE;
+    const double w = T[0] * detj;
+    const double pi_ = T[4 + 4 * i], gi0 = T[5 + 4 * i], gi1 = T[6 + 4 * i], gi2 = T[7 + 4 * i];
+    const double pj = T[4 + 4 * j], gj0 = T[5 + 4 * j], gj1 = T[6 + 4 * j], gj2 = T[7 + 4 * j];
+    if (ap.what == 1) {
+      acc += w * (pi_ * pj);
+      racc += w * pi_ * ap.f_val;
+    } else {
+      const double bgi = beta[0] * gi0 + beta[1] * gi1 + beta[2] * gi2;
+      const double bgj = beta[0] * gj0 + beta[1] * gj1 + beta[2] * gj2;
+      const double ti = pi_ + delta * bgi;
+      const double v = kap[0] * gi0 * gj0 + kap[1] * gi1 * gj1 + kap[2] * gi2 * gj2 + ti * (bgj + ap.c * pj);
+      acc += w * v;
+      racc += w * ap.f_val * ti;
+    }
+  }
+  E[i * NB + j] = acc;
+  if (j == 0) E[NB * NB + i] = racc;
+}
+
+// Per axis, the cell-local coordinates of a node of parity px in the cells
+// that contain it (two cells on a cell-face plane, one inside a Q2 cell), and
+// the stencil offsets of the class.
+template <int P>
+__host__ __device__ inline void class_axis(int px, int *nl, int *l, int *lo, int *len) {
+  if (P == 2 && px == 1) { *nl = 1; l[0] = 1; l[1] = 1; *lo = -1; *len = 3; }
+  else { *nl = 2; l[0] = P; l[1] = 0; *lo = -P; *len = 2 * P + 1; }
+}
+
+template <int P>
+__host__ __device__ inline double stencil_entry(const double *__restrict__ E, const int (&nl)[3], const int (&l)[3][2],
+                                       const int (&d)[3]) {
+  constexpr int P1 = P + 1, NB = P1 * P1 * P1;
+  double v = 0.0;
+  for (int az = 0; az < nl[2]; az++)
+    for (int ay = 0; ay < nl[1]; ay++)
+      for (int ax = 0; ax < nl[0]; ax++) {
+        const int lx = l[0][ax], ly = l[1][ay], lz = l[2][az];
+        const int mx = lx + d[0], my = ly + d[1], mz = lz + d[2];
+        if (mx < 0 || mx > P || my < 0 || my > P || mz < 0 || mz > P) continue;
+        v += E[(lx + P1 * (ly + P1 * lz)) * NB + (mx + P1 * (my + P1 * mz))];
+      }
+  return v;
+}
+
+// st[c*kCst + s]: FP64 class stencils; st[8*kCst + c]: class rhs;
+// cst: the FP32 compact stencils of the V-cycle (last entry 1/a_ii)
+template <int P>
+__global__ __launch_bounds__(kCst) void class_stencil_k(LatArgs a, const double *__restrict__ E,
+                                                        double *__restrict__ st, float *__restrict__ cst) {
+  constexpr int P1 = P + 1, NB = P1 * P1 * P1;
+  const int c = blockIdx.x, s = threadIdx.x;
+  const LatDev &C = a.c[c];
+  int nl[3], l[3][2], lo[3], len[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) class_axis<P>(C.px[d], &nl[d], l[d], &lo[d], &len[d]);
+  if (s < C.nsten) {
+    const int d[3] = {lo[0] + s % len[0], lo[1] + (s / len[0]) % len[1], lo[2] + s / (len[0] * len[1])};
+    const double v = stencil_entry<P>(E, nl, l, d);
+    st[c * kCst + s] = v;
+    cst[c * kCst + s] = (float)v;
+  }
+  if (s == kCst - 1) {
+    const int d0[3] = {0, 0, 0};
+    cst[c * kCst + kCst - 1] = (float)(1.0 / stencil_entry<P>(E, nl, l, d0));
+    double r = 0.0;
+    for (int az = 0; az < nl[2]; az++)
+      for (int ay = 0; ay < nl[1]; ay++)
+        for (int ax = 0; ax < nl[0]; ax++) r += E[NB * NB + l[0][ax] + P1 * (l[1][ay] + P1 * l[2][az])];
+    st[8 * kCst + c] = r;
+  }
+}
+
+// val at the CSR position of (class z, stencil entry y, node x) = class stencil
+__global__ void csr_fill_k(LatArgs a, const int *__restrict__ perm, const double *__restrict__ st,
+                           double *__restrict__ val) {
+  const int c = blockIdx.z, s = blockIdx.y;
+  const LatDev &C = a.c[c];
+  if (s >= C.nsten) return;
+  const double v = st[c * kCst + s];
+  const int *__restrict__ pp = perm + C.off + (long long)s * C.nodes;
+  for (long long k = (long long)blockIdx.x * kB + threadIdx.x; k < C.nodes; k += (long long)gridDim.x * kB)
+    val[pp[k]] = v;
+}
+
+__global__ void rhs_fill_k(LatArgs a, long long n, const int *__restrict__ node2dof, const double *__restrict__ st,
+                           double *__restrict__ rhs) {
+  const long long i = (long long)blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  int c = 0;
+  while (c + 1 < a.nclass && i >= a.c[c + 1].first) c++;
+  rhs[node2dof[i]] = st[8 * kCst + c];
 }
 
 // DOF-ordered f64 <-> lattice-ordered f32 (the boundary of the lattice vector
@@ -234,6 +414,7 @@ LatArgs lat_args(const DevLevel &L) {
     a.c[c].off = C.off;
     a.c[c].first = C.first;
     a.bstart[c] = b;
+    a.uni[c] = L.uni[c];
     b += (int)((C.nodes + kB - 1) / kB);
   }
   a.bstart[L.nclass] = b;
@@ -246,8 +427,8 @@ void lat_sweep(hipStream_t s, const DevLevel &L, int order, const float *x, cons
   const LatArgs a = lat_args(L);
   const int grid = a.bstart[L.nclass];
   if (grid <= 0) return;
-  if (order == 1) lat_sweep_k<1, MODE><<<grid, kB, 0, s>>>(a, L.lval, L.linv, x, f, out, omega);
-  else lat_sweep_k<2, MODE><<<grid, kB, 0, s>>>(a, L.lval, L.linv, x, f, out, omega);
+  if (order == 1) lat_sweep_k<1, MODE><<<grid, kB, 0, s>>>(a, L.lval, L.linv, L.lcst, x, f, out, omega);
+  else lat_sweep_k<2, MODE><<<grid, kB, 0, s>>>(a, L.lval, L.linv, L.lcst, x, f, out, omega);
 }
 
 }  // namespace
@@ -275,12 +456,60 @@ void kl_prolong_update(hipStream_t s, const DevLevel &F, const DevLevel &C, int 
   if (order == 1) lat_prolong_k<1><<<(int)((n + kB - 1) / kB), kB, 0, s>>>(F.nn, C.nn, coarse, x, damp);
   else lat_prolong_k<2><<<(int)((n + kB - 1) / kB), kB, 0, s>>>(F.nn, C.nn, coarse, x, damp);
 }
-void kl_refresh(hipStream_t s, const DevLevel &L) {
-  if (L.lat_nval > 0)
-    lat_convert_k<<<2048, kB, 0, s>>>(L.lat_nval, L.lperm, L.val, L.lval);
-  if (L.lat_nint > 0)
-    lat_invdiag_k<<<(int)((L.lat_nint + kB - 1) / kB), kB, 0, s>>>(L.lat_nint, L.lnode2dof, L.diag, L.val, L.linv);
+hipError_t kl_detect_uniform(hipStream_t s, DevLevel &L, int *flag_host, bool *all_uniform) {
+  *all_uniform = true;
+  if (L.nclass <= 0) return hipSuccess;
+  if (L.lat_fresh) {  // kl_assemble_uniform wrote the compact stencils of this matrix
+    L.lat_fresh = false;
+    for (int c = 0; c < L.nclass; c++) L.uni[c] = 1;
+    return hipSuccess;
+  }
+  for (int c = 0; c < L.nclass; c++) L.uni[c] = 0;
+  const LatArgs a = lat_args(L);
+  lat_first_node_k<<<L.nclass, 64, 0, s>>>(a, L.lperm, L.lnode2dof, L.diag, L.val, L.lscale, L.lflag, L.lcst);
+  lat_uniform_k<<<2048, kB, 0, s>>>(a, L.lat_nval, L.lperm, L.val, L.lscale, L.lflag);
+  hipError_t e = hipMemcpyAsync(flag_host, L.lflag, sizeof(int) * L.nclass, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    *all_uniform = false;
+    return e;
+  }
+  for (int c = 0; c < L.nclass; c++) {
+    L.uni[c] = flag_host[c] == 0;
+    *all_uniform = *all_uniform && L.uni[c];
+  }
+  return hipGetLastError();
 }
+
+hipError_t kl_convert(hipStream_t s, const DevLevel &L) {
+  lat_convert_k<<<2048, kB, 0, s>>>(L.lat_nval, L.lperm, L.val, L.lval);
+  lat_invdiag_k<<<(int)((L.lat_nint + kB - 1) / kB), kB, 0, s>>>(L.lat_nint, L.lnode2dof, L.diag, L.val, L.linv);
+  return hipGetLastError();
+}
+
+void kl_assemble_uniform(hipStream_t s, DevLevel &L, int order, const double *tab, const AsmParams &ap,
+                         double *val, double *rhs) {
+  const LatArgs a = lat_args(L);
+  if (order == 1) {
+    elem_k<1><<<1, 64, 0, s>>>(tab, ap, L.lelem);
+    class_stencil_k<1><<<L.nclass, kCst, 0, s>>>(a, L.lelem, L.lst64, L.lcst);
+  } else {
+    elem_k<2><<<1, 768, 0, s>>>(tab, ap, L.lelem);
+    class_stencil_k<2><<<L.nclass, kCst, 0, s>>>(a, L.lelem, L.lst64, L.lcst);
+  }
+  int64_t maxn = 0;
+  int maxs = 0;
+  for (int c = 0; c < L.nclass; c++) {
+    maxn = std::max<int64_t>(maxn, L.cls[c].nodes);
+    maxs = std::max(maxs, L.cls[c].nsten);
+  }
+  const int gx = (int)std::min<int64_t>((maxn + kB - 1) / kB, 64);
+  if (gx > 0) {
+    csr_fill_k<<<dim3(gx, maxs, L.nclass), kB, 0, s>>>(a, L.lperm, L.lst64, val);
+    rhs_fill_k<<<(int)((L.lat_nint + kB - 1) / kB), kB, 0, s>>>(a, L.lat_nint, L.lnode2dof, L.lst64, rhs);
+  }
+}
+
 void kl_from_dof(hipStream_t s, const DevLevel &L, const double *v, float *l) {
   dof_to_lat_k<<<(L.ndof + kB - 1) / kB, kB, 0, s>>>(L.ndof, L.nact, L.ldof2node, v, l);
 }

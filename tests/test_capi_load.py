@@ -1,5 +1,5 @@
 """The C-ABI library loads on a CPU-only host and exports every entry point
-that include/cd3d.h declares (no compute call: there is no GPU here)."""
+that include/cd3d.h and include/cd3d_server.h declare (no compute call: there is no GPU here)."""
 import ctypes
 import os
 import re
@@ -10,9 +10,12 @@ from tests.conftest import REPO
 
 
 def declared_functions():
-    src = open(os.path.join(REPO, "include", "cd3d.h")).read()
-    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(cd3d_\w+)\s*\(", src)))
+    names = set()
+    for h in ("cd3d.h", "cd3d_server.h"):
+        src = open(os.path.join(REPO, "include", h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        names |= set(re.findall(r"\b(cd3d_\w+)\s*\(", src))
+    return sorted(names)
 
 
 def test_header_matches_python_mirror():

@@ -147,6 +147,38 @@ def test_tcd_crank_nicolson_matches_reference(golden):
         assert err < SOL_TOL
 
 
+def test_tcd_stepwise_matches_run_and_backward_euler_residual(golden):
+    import scipy.sparse as sps
+    c = TCD_CASE
+    P = pm.Problem(c["order"], 1, c["k0"], c["levels"], box=REF_BOX)
+    co = pm.coeffs(**c["co"])
+    u0 = P.interp_pulse(c["levels"] - 1, co)
+    sp = pm.solver_params(**REF_SOLVER, rel_tol=0.0, res_norm_min=1e-30, red_factor=1e-10)
+    _, states, _ = P.tcd_run(co, sp, 0.0, c["tau"], c["nsteps"], u0)
+    P.tcd_init(co, 0.0, u0)
+    for k in range(c["nsteps"]):
+        u, _ = P.tcd_step(co, sp, c["tau"])
+        assert np.array_equal(u, states[k])
+    assert np.array_equal(P.tcd_state(), states[-1])
+    # backward Euler (theta 1): (M + tau A(tau)) u1 = M u0 on the active rows (f = 0)
+    P.tcd_init(co, 0.0, u0)
+    u1, _ = P.tcd_step(co, sp, c["tau"], theta=1.0)
+    lf = c["levels"] - 1
+    rp, kc, M = P.csr(lf)  # the mass matrix, restored after the step
+    Q = pm.Problem(c["order"], 1, c["k0"], c["levels"], box=REF_BOX)
+    Q.assemble(pm.coeffs(**dict(c["co"], t=c["tau"])))
+    _, _, A = Q.csr(lf)
+    n, na = len(rp) - 1, P.info(lf)["nact"]
+    Mm = sps.csr_matrix((M, kc, rp), shape=(n, n))
+    Am = sps.csr_matrix((A, kc, rp), shape=(n, n))
+    u0d = u0.copy()
+    u0d[na:] = 0.0  # Dirichlet value g = 0 from the rhs
+    r = (Mm @ u1 + c["tau"] * (Am @ u1) - Mm @ u0d)[:na]
+    rel = np.linalg.norm(r) / np.linalg.norm((Mm @ u0d)[:na])
+    say(f"[gpu] tcd backward Euler residual {rel:.2e}")
+    assert rel < 1e-9
+
+
 @pytest.mark.parametrize("name", ["q2_l2_supg", "q1_l3_supg", "q1_c1"])
 def test_mixed_precision_against_fp64(name, golden):
     P = problem(name, golden)
@@ -178,11 +210,14 @@ def test_structure_closed_forms_larger(order, k0, levels):
     assert np.all(kc[1:][same_row] > kc[:-1][same_row])  # sorted, unique columns
 
 
-@pytest.mark.parametrize("order,m,k0,levels", [(2, 1, 1, 3), (1, 2, 1, 3), (2, 3, 0, 3), (1, 1, 2, 4)])
-def test_mixed_vcycle_against_fp64(order, m, k0, levels):
-    # the FP32 lattice-stencil V-cycle is the FP64 CSR V-cycle up to rounding
+@pytest.mark.parametrize("order,m,k0,levels,conv_kind", [(2, 1, 1, 3, 0), (1, 2, 1, 3, 0), (2, 3, 0, 3, 0),
+                                                          (1, 1, 2, 4, 0), (2, 1, 1, 3, 1), (1, 2, 1, 3, 1)])
+def test_mixed_vcycle_against_fp64(order, m, k0, levels, conv_kind):
+    # the FP32 lattice-stencil V-cycle is the FP64 CSR V-cycle up to rounding;
+    # constant coefficients give the same stencil at every node of a class
+    # (compact form), the rotating field a different one at every node
     P = pm.Problem(order, m, k0, levels, box=(1.0, 0.7, 0.5))
-    P.assemble(pm.coeffs(eps=1e-2, b=(1.0, 0.5, 0.25), supg=1, f_val=1.0))
+    P.assemble(pm.coeffs(eps=1e-2, b=(1.0, 0.5, 0.25), supg=1, f_val=1.0, conv_kind=conv_kind, t=0.1))
     i = P.info(levels - 1)
     r = np.random.default_rng(3).standard_normal(i["ndof"])
     r[i["nact"]:] = 0.0
@@ -190,7 +225,7 @@ def test_mixed_vcycle_against_fp64(order, m, k0, levels):
     z64 = P.vcycle(pm.solver_params(**sp), r)
     z32 = P.vcycle(pm.solver_params(**sp, precision=pm.MIXED), r)
     err = np.linalg.norm(z32 - z64) / np.linalg.norm(z64)
-    say(f"[gpu] mixed vcycle order={order} m={m} k0={k0} levels={levels} err vs fp64 {err:.2e}")
+    say(f"[gpu] mixed vcycle order={order} m={m} k0={k0} levels={levels} conv_kind={conv_kind} err vs fp64 {err:.2e}")
     assert err < 1e-5
     assert np.all(z32[i["nact"]:] == 0.0)
     x, st = P.solve(pm.solver_params(**sp, rel_tol=1e-10, precision=pm.MIXED))

@@ -1,0 +1,183 @@
+"""Serving loop (include/cd3d_server.h).
+
+CPU: admission control, eviction, overtaking limit and statistics, with the
+server in simulate mode (no device work).  GPU: every result is the same
+alone and in the mix (bitwise: no state is shared between requests and every
+reduction is deterministic), equals the direct handle API, and the device
+footprint stays within the admission estimate."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import parmoon_cmg_amd as pm
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def solver(**kw):
+    return pm.solver_params(**dict(dict(omega=0.5, pre=3, post=3, coarse_smoother=17, restart=20, maxit=200,
+                                        rel_tol=1e-10), **kw))
+
+
+def test_estimate_grows_with_the_problem():
+    small, big = pm.mesh_desc(1, 1, 2, 4), pm.mesh_desc(1, 1, 2, 5)
+    e_small, e_big = pm.estimate_bytes(small), pm.estimate_bytes(big)
+    assert 0 < e_small < e_big
+    # the finest level dominates: about 8x per extra level
+    assert 5 < e_big / e_small < 9
+    assert pm.estimate_bytes(big, restart=40) > e_big
+    assert pm.estimate_bytes(big, cellwise=True) > e_big
+    assert pm.estimate_bytes(big, tcd=True) > e_big
+    assert pm.estimate_bytes(pm.mesh_desc(3, 1, 2, 3)) == 0  # invalid order
+
+
+def test_simulated_admission_keeps_the_budget_and_reuses_hierarchies():
+    mesh = pm.mesh_desc(1, 1, 2, 3)
+    est = pm.estimate_bytes(mesh)
+    srv = pm.Server(hbm_budget=int(2.5 * est), max_inflight=8, simulate=True, sim_us_per_dof=20.0)
+    ids = [srv.submit(mesh, pm.coeffs(), solver()) for _ in range(6)]
+    res = [srv.wait(i)[0] for i in ids]
+    st = srv.stats()
+    assert st.completed == 6 and st.failed == 0
+    assert st.peak_bytes <= st.budget
+    assert sum(r.reused_hierarchy for r in res) >= 4  # two hierarchies fit; later requests reuse them
+    assert st.resident == 2
+    assert 0 < st.p50_ms <= st.p99_ms
+    assert st.solves_per_s > 0
+    srv.close()
+
+
+def test_request_larger_than_the_budget_is_refused():
+    mesh = pm.mesh_desc(2, 1, 2, 4)
+    srv = pm.Server(hbm_budget=pm.estimate_bytes(mesh) // 2, simulate=True)
+    with pytest.raises(pm.CD3DError) as e:
+        srv.submit(mesh, pm.coeffs(), solver())
+    assert e.value.status == 3  # CD3D_ERR_NOMEM
+    assert srv.stats().rejected == 1
+    with pytest.raises(pm.CD3DError) as e:
+        srv.submit(pm.mesh_desc(3, 1, 2, 3), pm.coeffs(), solver())
+    assert e.value.status == 1  # CD3D_ERR_ARG
+    srv.close()
+
+
+def test_overtaking_is_bounded():
+    small, big = pm.mesh_desc(1, 1, 1, 3), pm.mesh_desc(1, 1, 2, 3)
+    es, eb = pm.estimate_bytes(small), pm.estimate_bytes(big)
+    # the big request fits alone, not beside a small one
+    srv = pm.Server(hbm_budget=eb + es // 2, max_inflight=4, keep_resident=False, max_bypass=1, simulate=True,
+                    sim_us_per_dof=200.0)
+    a = srv.submit(small, pm.coeffs(), solver())
+    time.sleep(0.02)  # a is on the device
+    b = srv.submit(big, pm.coeffs(), solver())
+    c = srv.submit(small, pm.coeffs(), solver())
+    d = srv.submit(small, pm.coeffs(), solver())
+    r = {k: srv.wait(v)[0] for k, v in dict(a=a, b=b, c=c, d=d).items()}
+    admit = {k: v.latency_ms - v.run_ms for k, v in r.items()}  # time from submit to admission
+    assert admit["c"] < admit["b"]  # c overtook b once
+    assert admit["d"] >= admit["b"]  # then b's limit held d back
+    assert srv.stats().peak_bytes <= eb + es // 2
+    srv.close()
+
+
+def test_destroy_drops_queued_requests():
+    mesh = pm.mesh_desc(1, 1, 2, 3)
+    srv = pm.Server(hbm_budget=2 * pm.estimate_bytes(mesh), max_inflight=1, simulate=True, sim_us_per_dof=10.0)
+    for _ in range(5):
+        srv.submit(mesh, pm.coeffs(), solver())
+    t = time.perf_counter()
+    srv.close()
+    assert time.perf_counter() - t < 5.0
+
+
+# ---------------------------------------------------------------- GPU
+
+MIX = [
+    dict(mesh=(1, 1, 2, 3), co=dict(eps=1e-2, b=(1.0, 0.5, 0.25), supg=1)),
+    dict(mesh=(2, 1, 1, 3), co=dict(eps=1e-3, b=(0.2, -0.7, 0.4), supg=1)),
+    dict(mesh=(1, 3, 1, 3), co=dict(eps=1.0, f_kind=1)),
+    dict(mesh=(1, 1, 2, 3), co=dict(eps=1e-4, b=(-0.3, 0.1, 0.9), supg=1), mixed=True),
+    dict(mesh=(2, 1, 1, 2), co=dict(eps=0.1, c=1.0)),
+    dict(mesh=(1, 1, 2, 3), co=dict(eps=1e-4, conv_kind=1), tcd=True),
+]
+
+
+def submit_mix(srv):
+    ids = []
+    for c in MIX:
+        sp = solver(precision=pm.MIXED if c.get("mixed") else pm.FP64)
+        mesh = pm.mesh_desc(*c["mesh"])
+        if c.get("tcd"):
+            sp = solver(rel_tol=0.0, res_norm_min=1e-30, red_factor=1e-10)
+            ids.append(srv.submit(mesh, pm.coeffs(**c["co"]), sp, kind=pm.REQ_TCD, tau=1e-3, nsteps=3,
+                                  want_solution=True))
+        else:
+            ids.append(srv.submit(mesh, pm.coeffs(**c["co"]), sp, want_solution=True))
+    return ids
+
+
+def n_dof(c):
+    o, m, k0, lv = c["mesh"]
+    return (o * m * 2 ** (k0 + lv - 1) + 1) ** 3
+
+
+@pytest.mark.gpu
+def test_results_are_the_same_alone_and_in_the_mix():
+    say("[gpu] server alone")
+    alone = []
+    for c in MIX:
+        srv = pm.Server(max_inflight=1, keep_resident=False)
+        saved = MIX[:]
+        MIX[:] = [c]
+        try:
+            (i,) = submit_mix(srv)
+        finally:
+            MIX[:] = saved
+        alone.append(srv.wait(i, n_dof=n_dof(c)))
+        srv.close()
+    say("[gpu] server mix")
+    srv = pm.Server(max_inflight=4)
+    ids = submit_mix(srv) + submit_mix(srv)  # the second round reuses resident hierarchies
+    mix = [srv.wait(i, n_dof=n_dof(MIX[k % len(MIX)])) for k, i in enumerate(ids)]
+    st = srv.stats()
+    say(f"[gpu] server mix: {st.completed} done, p50 {st.p50_ms:.1f} ms p99 {st.p99_ms:.1f} ms, "
+        f"peak {st.peak_bytes / 2**20:.0f} MiB, reused {sum(r.reused_hierarchy for r, _ in mix)}")
+    srv.close()
+    for k, (r, x) in enumerate(mix):
+        ra, xa = alone[k % len(MIX)]
+        assert r.status == 0 and ra.status == 0
+        assert r.iterations == ra.iterations
+        assert np.array_equal(x, xa), k
+    assert sum(r.reused_hierarchy for r, _ in mix) >= 1
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mesh", [(2, 1, 2, 4), (1, 1, 2, 5)])
+def test_device_footprint_stays_within_the_estimate(mesh):
+    P = pm.Problem(*mesh)
+    P.assemble(pm.coeffs(eps=1e-3, b=(1.0, 0.5, 0.25), supg=1))
+    P.solve(solver())
+    used = P.L.cd3d_device_bytes(P.h)
+    est = pm.estimate_bytes(pm.mesh_desc(*mesh))
+    say(f"[gpu] mesh {mesh}: device bytes {used / 2**20:.1f} MiB, estimate {est / 2**20:.1f} MiB")
+    assert used <= est <= 1.3 * used
+    P.close()
+
+
+@pytest.mark.gpu
+def test_server_result_equals_the_handle_api():
+    c = MIX[1]
+    co, sp = pm.coeffs(**c["co"]), solver()
+    P = pm.Problem(*c["mesh"])
+    P.assemble(co)
+    x, st = P.solve(sp)
+    P.close()
+    srv = pm.Server(max_inflight=2)
+    r, xs = srv.wait(srv.submit(pm.mesh_desc(*c["mesh"]), co, sp, want_solution=True), n_dof=len(x))
+    srv.close()
+    assert r.iterations == st.iterations
+    assert np.array_equal(xs, x)
