@@ -156,12 +156,155 @@ def cpu_baseline(sample_levels=4, timeout=240):
                        f"scaled by the DOF ratio {ndof_work / ndof_sample:.2f} to the 64^3 workload")
 
 
+# ---------------------------------------------------------------- configurations 3-5
+
+def mesh_for_n(n):
+    """Nearest n = m 2^k with 2 <= m <= 7 (coarsest level m^3 cells); returns
+    (n, m, levels), preferring more levels."""
+    best = None
+    for m in range(2, 8):
+        for k in range(0, 7):
+            v = m * 2 ** k
+            key = (abs(v - n), -k)
+            if best is None or key < best[0]:
+                best = (key, (v, m, k + 1))
+    return best[1]
+
+
+def random_coeffs(pm, rng, supg=1, conv_kind=0):
+    eps = 10 ** rng.uniform(-6, 0)
+    d = rng.standard_normal(3)
+    b = d / np.linalg.norm(d) * rng.uniform(0, 1)
+    return pm.coeffs(eps=eps, b=tuple(b), f_val=1.0, supg=supg, conv_kind=conv_kind), eps
+
+
+def tcd_run(pm, precision, steps, n=128, levels=5):
+    """Configuration 3: Q1 128^3, eps 1e-4, rotating field, Crank-Nicolson
+    dt 1e-3, convection matrix re-assembled every step on every level."""
+    P = pm.Problem(1, 1, int(np.log2(n)) - levels + 1, levels, box=(1.0, 1.0, 1.0))
+    co = pm.coeffs(eps=1e-4, conv_kind=1, rot_c=(0.5, 0.5), pulse_c=(0.3, 0.5, 0.5), f_val=0.0)
+    sp = pm.solver_params(**dict(SOLVER, rel_tol=0.0, res_norm_min=1e-30, red_factor=1e-10), precision=precision)
+    u0 = P.interp_pulse(levels - 1, co)
+    P.tcd_init(co, 0.0, u0)
+    P.tcd_step(co, sp, 1e-3, want_state=False)  # warm-up step (allocations)
+    P.tcd_init(co, 0.0, u0)
+    its = 0
+    P.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        _, st = P.tcd_step(co, sp, 1e-3, want_state=False)
+        its += st.iterations
+    dt = time.perf_counter() - t
+    u = P.tcd_state()
+    P.close()
+    return dict(steps=steps, steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps, iterations=its), u
+
+
+def batched_uneven(pm, precision, count=256, seed=20261202, passes=2):
+    """Configuration 4: `count` independent Q1 SUPG problems, n log-uniform in
+    [8, 48] (snapped to m 2^k), eps log-uniform in [1e-6, 1], b a random unit
+    direction times U(0, 1), f = 1; all submitted at once to the server.  The
+    first pass builds the resident hierarchies, the last one is reported."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(count):
+        n, m, levels = mesh_for_n(np.exp(rng.uniform(np.log(8), np.log(48))))
+        co, _ = random_coeffs(pm, rng)
+        reqs.append((pm.mesh_desc(1, m, 0, levels), co))
+    sp = pm.solver_params(**dict(SOLVER, maxit=1000), precision=precision)
+    srv = pm.Server(max_inflight=16)
+    out = None
+    for _ in range(passes):
+        t = time.perf_counter()
+        ids = [srv.submit(mesh, co, sp) for mesh, co in reqs]
+        res = [srv.wait(i)[0] for i in ids]
+        wall = time.perf_counter() - t
+        lat = np.array([r.latency_ms for r in res])
+        out = dict(problems=count, solves_per_s=count / wall, wall_s=wall, p50_ms=float(np.percentile(lat, 50)),
+                   p99_ms=float(np.percentile(lat, 99)), not_converged=sum(r.status != 0 for r in res),
+                   max_rel_residual=float(max(r.rel_residual for r in res)),
+                   median_iterations=int(np.median([r.iterations for r in res])),
+                   reused=sum(r.reused_hierarchy for r in res))
+    st = srv.stats()
+    out["peak_device_gib"] = st.peak_bytes / 2**30
+    srv.close()
+    return out
+
+
+def serving_stream(pm, precision, rates, duration_s=4.0, seed=7, tcd_share=0.2):
+    """Configuration 5: Poisson arrivals of CD3D / TCD3D requests, n in
+    [16, 96] (snapped), Q1 or Q2 with equal probability, eps and b as in
+    configuration 4; TCD3D requests run 10 Crank-Nicolson steps of the
+    rotating field.  Per rate: offered load, completed solves/s, p50/p99."""
+    import threading
+    srv = pm.Server(max_inflight=16)
+    out = []
+    for ri, rate in enumerate(rates):
+        rng = np.random.default_rng(seed + ri)
+        arrivals, t = [], 0.0
+        while True:
+            t += rng.exponential(1.0 / rate)
+            if t > duration_s:
+                break
+            n, m, levels = mesh_for_n(np.exp(rng.uniform(np.log(16), np.log(96))))
+            order = 1 if rng.uniform() < 0.5 else 2
+            tcd = rng.uniform() < tcd_share
+            co, _ = random_coeffs(pm, rng, supg=0 if tcd else 1, conv_kind=1 if tcd else 0)
+            arrivals.append((t, pm.mesh_desc(order, m, 0, levels), co, tcd))
+        sp_cd = pm.solver_params(**dict(SOLVER, maxit=1000), precision=precision)
+        sp_tcd = pm.solver_params(**dict(SOLVER, rel_tol=0.0, res_norm_min=1e-30, red_factor=1e-10, maxit=1000),
+                                  precision=precision)
+
+        def submit_all(stream, ids, t0):
+            for (ta, mesh, co, tcd) in stream:
+                d = t0 + ta - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                if tcd:
+                    ids.append(srv.submit(mesh, co, sp_tcd, kind=pm.REQ_TCD, tau=1e-3, nsteps=10))
+                else:
+                    ids.append(srv.submit(mesh, co, sp_cd))
+
+        for phase in ("warm", "timed"):
+            ids = []
+            t0 = time.perf_counter()
+            th = threading.Thread(target=submit_all, args=(arrivals, ids, t0))
+            th.start()
+            th.join()
+            res = [srv.wait(i)[0] for i in ids]
+            wall = time.perf_counter() - t0
+        lat = np.array([r.latency_ms for r in res])
+        out.append(dict(rate_per_s=rate, requests=len(res), duration_s=duration_s, wall_s=wall,
+                        solves_per_s=len(res) / wall, p50_ms=float(np.percentile(lat, 50)),
+                        p99_ms=float(np.percentile(lat, 99)), failed=sum(r.status != 0 for r in res),
+                        tcd_requests=sum(a[3] for a in arrivals)))
+        log(f"[bench]   rate {rate}/s: {len(res)} requests, p50 {out[-1]['p50_ms']:.1f} ms, "
+            f"p99 {out[-1]['p99_ms']:.1f} ms, {out[-1]['solves_per_s']:.1f} solves/s")
+    srv.close()
+    return out
+
+
+def r5(d):
+    if isinstance(d, dict):
+        return {k: r5(v) for k, v in d.items()}
+    if isinstance(d, list):
+        return [r5(v) for v in d]
+    if isinstance(d, float):
+        return float(f"{d:.5g}")
+    return d
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="only the headline configuration")
+    ap.add_argument("--c3-steps", type=int, default=100)
+    ap.add_argument("--c4-problems", type=int, default=256)
+    ap.add_argument("--c5-rates", type=str, default="10,50")
+    ap.add_argument("--c5-seconds", type=float, default=4.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every MI355X box here has one GPU)",
@@ -191,6 +334,23 @@ def main():
     if not args.no_cpu:
         log("[bench] reference CPU baseline (bounded sample)")
         cpu = cpu_baseline()
+    extra = {}
+    if not args.no_extra:
+        for name, prec in (("fp64", pm.FP64), ("mixed", pm.MIXED)):
+            log(f"[bench] configuration 3 (TCD3D Q1 128^3, {args.c3_steps} CN steps), {name}")
+            r, u = tcd_run(pm, prec, args.c3_steps)
+            extra.setdefault("tcd3d_q1_128", {})[name] = r
+            extra["tcd3d_q1_128"].setdefault("_u", {})[name] = u
+        u = extra["tcd3d_q1_128"].pop("_u")
+        extra["tcd3d_q1_128"]["mixed_state_rel_err_vs_fp64"] = float(
+            np.linalg.norm(u["mixed"] - u["fp64"]) / np.linalg.norm(u["fp64"]))
+        for name, prec in (("fp64", pm.FP64), ("mixed", pm.MIXED)):
+            log(f"[bench] configuration 4 ({args.c4_problems} batched uneven Q1 problems), {name}")
+            extra.setdefault("batched_uneven_q1", {})[name] = batched_uneven(pm, prec, args.c4_problems)
+        rates = [float(x) for x in args.c5_rates.split(",") if x]
+        for name, prec in (("fp64", pm.FP64), ("mixed", pm.MIXED)):
+            log(f"[bench] configuration 5 (Poisson serving stream), {name}")
+            extra.setdefault("serving_stream", {})[name] = serving_stream(pm, prec, rates, args.c5_seconds)
     m = res["mixed"]
     out = {
         "metric": METRIC, "value": round(m["solves_per_s"], 4), "unit": "solves/s", "n_gpus": 1,
@@ -209,6 +369,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    out.update(r5(extra))
     print(json.dumps(out))
     P.close()
     return 0

@@ -491,9 +491,50 @@ double dev_norm(cd3d_problem &P, int n, const double *a) {
 
 // TFgmresIte::Iterate, flexible branch (FgmresIte.C:219-595) on the finest
 // level: x (P.xs) and b (P.bs) on the device.  Returns the iteration count.
-int fgmres(cd3d_problem &P, const cd3d_solver_params &sp, double &r0, cd3d_status &status) {
+// The vector space FGMRES runs in: DOF order with the CSR matrix (the plain
+// form), or node-lattice order with the class stencils (mixed form on a
+// constant-coefficient level); x and b are device vectors of that space.
+struct Space {
+  int n;
+  double *x, *b;
+  std::function<void(const double *, const double *, double *)> defect;
+  std::function<void(const double *, double *)> matvec;
+  std::function<void(VC &, const double *, double *)> precond;
+};
+
+Space dof_space(cd3d_problem &P) {
   DevLevel &F = P.dl.back();
-  const int N = F.ndof;
+  Space S;
+  S.n = F.ndof;
+  S.x = P.xs;
+  S.b = P.bs;
+  S.defect = [&P, &F](const double *x, const double *f, double *d) { k_defect(P.stream, F, x, f, d); };
+  S.matvec = [&P, &F](const double *x, double *y) { k_spmv(P.stream, F, x, y); };
+  S.precond = [](VC &v, const double *r, double *z) { precondition(v, r, z, true); };
+  return S;
+}
+
+Space lattice_space(cd3d_problem &P) {
+  DevLevel &F = P.dl.back();
+  const int p = P.mesh.order, top = (int)P.dl.size() - 1;
+  Space S;
+  S.n = F.ndof;  // every lattice node is a DOF of the finest level
+  S.x = P.xl;
+  S.b = P.bl;
+  S.defect = [&P, &F, p](const double *x, const double *f, double *d) { kl_defect64(P.stream, F, p, x, f, d); };
+  S.matvec = [&P, &F, p](const double *x, double *y) { kl_spmv64(P.stream, F, p, x, y); };
+  // the Krylov vectors vanish on the boundary, as the V-cycle's rhs must
+  S.precond = [&P, &F, top](VC &v, const double *r, double *z) {
+    v.reset();
+    kl_cast_d2f(P.stream, F.ndof, r, F.lf);
+    cycleL(v, top, true);
+    kl_cast_f2d(P.stream, F.ndof, F.lx, z);
+  };
+  return S;
+}
+
+int fgmres(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S, double &r0, cd3d_status &status) {
+  const int N = S.n;
   int R = std::min(sp.restart, sp.maxit);
   if (R <= 0) R = 10;
   ensure_krylov(P, R);
@@ -503,7 +544,7 @@ int fgmres(cd3d_problem &P, const cd3d_solver_params &sp, double &r0, cd3d_statu
   auto Zk = [&](int k) { return Z + (int64_t)k * N; };
   std::vector<double> H((size_t)(R + 1) * R, 0.0), s(R + 1), cs(R + 1), sn(R + 1);
   double *pa = P.red, *pb = P.red + kRedBlocks;
-  k_defect(P.stream, F, P.xs, P.bs, Vk(0));
+  S.defect(S.x, S.b, Vk(0));
   double beta = dev_norm(P, N, Vk(0));
   r0 = beta;
   if (!std::isfinite(beta)) { status = CD3D_NONFINITE; return 0; }
@@ -522,7 +563,7 @@ int fgmres(cd3d_problem &P, const cd3d_solver_params &sp, double &r0, cd3d_statu
     }
     memcpy(P.hscal + 64, y.data(), sizeof(double) * (k + 1));
     HIPCHK(hipMemcpyAsync(P.dscal + 64, P.hscal + 64, sizeof(double) * (k + 1), hipMemcpyHostToDevice, P.stream));
-    k_multi_axpy(P.stream, N, k + 1, P.dscal + 64, Z, N, P.xs);
+    k_multi_axpy(P.stream, N, k + 1, P.dscal + 64, Z, N, S.x);
     HIPCHK(hipStreamSynchronize(P.stream));  // pinned y is reused next time
   };
   while (j <= sp.maxit) {
@@ -531,8 +572,8 @@ int fgmres(cd3d_problem &P, const cd3d_solver_params &sp, double &r0, cd3d_statu
     s[0] = beta;
     int i;
     for (i = 0; i < R && j <= sp.maxit; i++, j++) {
-      precondition(v, Vk(i), Zk(i), true);
-      k_spmv(P.stream, F, Zk(i), w);
+      S.precond(v, Vk(i), Zk(i));
+      S.matvec(Zk(i), w);
       k_dot(P.stream, N, w, Vk(0), pa, nullptr);
       // modified Gram-Schmidt: H[k][i] = <w, v_k>; w -= H[k][i] v_k, fused with the next dot
       for (int k = 0; k <= i; k++) {
@@ -557,7 +598,7 @@ int fgmres(cd3d_problem &P, const cd3d_solver_params &sp, double &r0, cd3d_statu
       if (res > sp.div_factor * res0) { status = CD3D_DIVERGED; return j; }
     }
     update_x(R - 1);
-    k_defect(P.stream, F, P.xs, P.bs, Vk(0));
+    S.defect(S.x, S.b, Vk(0));
     beta = dev_norm(P, N, Vk(0));
     res0 = beta;
     if (!std::isfinite(beta)) { status = CD3D_NONFINITE; return j; }
@@ -864,8 +905,21 @@ cd3d_status cd3d_solve(cd3d_problem *p, const cd3d_solver_params *sp, const doub
     else HIPCHK(hipMemsetAsync(p->xs, 0, sizeof(double) * N, p->stream));
     k_copy_range(p->stream, F.nact, N, p->bs, p->xs);
     double r0 = 0;
-    const int its = fgmres(*p, *sp, r0, solve_status);
-    double rf = 0;
+    int its;
+    const int top = (int)p->dl.size() - 1;
+    if (sp->precision == CD3D_MIXED && top > 0 && F.lat_exact64) {
+      if (!p->xl) {
+        p->xl = dalloc<double>(*p, N);
+        p->bl = dalloc<double>(*p, N);
+      }
+      kl_dof_to_lat64(p->stream, F, p->bs, p->bl);
+      kl_dof_to_lat64(p->stream, F, p->xs, p->xl);
+      its = fgmres(*p, *sp, lattice_space(*p), r0, solve_status);
+      kl_lat64_to_dof(p->stream, F, p->xl, p->xs);
+    } else {
+      its = fgmres(*p, *sp, dof_space(*p), r0, solve_status);
+    }
+    double rf = 0;  // recomputed with the CSR matrix in DOF order
     if (st) {
       k_defect(p->stream, F, p->xs, p->bs, p->w);
       rf = dev_norm(*p, N, p->w);
@@ -979,7 +1033,7 @@ static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_sol
   refresh_coarse_and_f32(P);
   // Solve (SystemTCD3D.C:317-376)
   double r0;
-  const int its = fgmres(P, sp, r0, solve_status);
+  const int its = fgmres(P, sp, dof_space(P), r0, solve_status);
   // RestoreMassMat (SystemTCD3D.C:296-315)
   for (int l = 0; l <= Lf; l++) {
     DevLevel &D = P.dl[l];

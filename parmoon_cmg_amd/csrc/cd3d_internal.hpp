@@ -114,6 +114,7 @@ struct DevLevel {
   // FP64 class stencils (8 x 128 + 8 class rhs)
   double *lelem = nullptr, *lst64 = nullptr;
   bool lat_fresh = false;  // lcst already holds the stencils of `val`
+  bool lat_exact64 = false;  // lst64 holds exactly the CSR values of the interior rows
 };
 
 struct Problem {
@@ -140,6 +141,7 @@ struct Problem {
   double *dscal = nullptr;    // device scalars (H column, norms)
   double *hscal = nullptr;    // pinned host mirror of dscal
   int *hflag = nullptr;       // pinned, lattice uniformity flags
+  double *xl = nullptr, *bl = nullptr;  // lattice-ordered iterate and rhs of the lattice FGMRES
   int red_blocks = 0;
   // TCD3D state: f(t_old), u_old, a defect buffer; the current state is in xs
   double *tcd_oldrhs = nullptr, *tcd_oldsol = nullptr, *tcd_def = nullptr;
@@ -216,6 +218,14 @@ hipError_t kl_convert(hipStream_t s, const DevLevel &L);
 // the class stencils (also writes the FP32 compact stencils)
 void kl_assemble_uniform(hipStream_t s, DevLevel &L, int order, const double *tab, const AsmParams &ap,
                          double *val, double *rhs);
+// FP64 operator in lattice order (class stencils lst64, identity rows on the
+// boundary) and the order / precision changes of the lattice FGMRES
+void kl_spmv64(hipStream_t s, const DevLevel &L, int order, const double *x, double *y);
+void kl_defect64(hipStream_t s, const DevLevel &L, int order, const double *x, const double *f, double *d);
+void kl_dof_to_lat64(hipStream_t s, const DevLevel &L, const double *v, double *l);
+void kl_lat64_to_dof(hipStream_t s, const DevLevel &L, const double *l, double *v);
+void kl_cast_d2f(hipStream_t s, int64_t n, const double *a, float *b);
+void kl_cast_f2d(hipStream_t s, int64_t n, const float *a, double *b);
 // DOF-ordered f64 <-> lattice f32 (Dirichlet DOFs map to zero)
 void kl_from_dof(hipStream_t s, const DevLevel &L, const double *v, float *l);
 void kl_to_dof(hipStream_t s, const DevLevel &L, const float *l, double *v);

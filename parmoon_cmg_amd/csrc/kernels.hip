@@ -767,7 +767,7 @@ void k_matadd(hipStream_t s, int64_t n, double *C, const double *D, double alpha
   matadd_k<<<grid_for(n), kBlock, 0, s>>>(n, C, D, alpha);
 }
 void k_gemv_dense(hipStream_t s, int n, const double *A, const double *x, double *y) {
-  coarse_update_k<<<(n * 64 + kBlock - 1) / kBlock, kBlock, 0, s>>>(n, A, x, y);
+  if (n > 0) coarse_update_k<<<(n * 64 + kBlock - 1) / kBlock, kBlock, 0, s>>>(n, A, x, y);
 }
 void k_coarse_smooth(hipStream_t s, const DevLevel &L, double *x, const double *f, double *d, int smoother,
                      double omega, int maxit, double red) {

@@ -382,6 +382,84 @@ __global__ void rhs_fill_k(LatArgs a, long long n, const int *__restrict__ node2
   rhs[node2dof[i]] = st[8 * kCst + c];
 }
 
+// ------------------------------------------------------------------------
+// FP64 operator in lattice order for the outer FGMRES of the mixed form,
+// when the class stencils are exactly the assembled CSR values: interior
+// rows apply the class stencil, boundary (Dirichlet) rows are identity rows.
+
+enum Lat64Mode { kL64Spmv = 0, kL64Defect = 1 };
+
+template <int P, int PX, int PY, int PZ, int MODE>
+__device__ inline void lat64_node(const LatArgs &a, const LatDev &C, long long k, const double *__restrict__ st,
+                                  const double *__restrict__ x, const double *__restrict__ f,
+                                  double *__restrict__ out) {
+  const int kx = (int)(k % C.cnt[0]), ky = (int)((k / C.cnt[0]) % C.cnt[1]), kz = (int)(k / ((long long)C.cnt[0] * C.cnt[1]));
+  const int X = C.lo[0] + C.step * kx, Y = C.lo[1] + C.step * ky, Z = C.lo[2] + C.step * kz;
+  const long long nn = a.nn, node = X + nn * (Y + nn * Z);
+  double s = 0.0;
+  int si = 0;
+#pragma unroll
+  for (int dz = lo_off<P, PZ>(); dz <= hi_off<P, PZ>(); dz++)
+#pragma unroll
+    for (int dy = lo_off<P, PY>(); dy <= hi_off<P, PY>(); dy++)
+#pragma unroll
+      for (int dx = lo_off<P, PX>(); dx <= hi_off<P, PX>(); dx++, si++)
+        s += st[si] * x[node + dx + nn * (dy + nn * dz)];
+  out[node] = MODE == kL64Defect ? f[node] - s : s;
+}
+
+template <int P, int MODE>
+__global__ __launch_bounds__(kB) void lat64_k(LatArgs a, const double *__restrict__ st64, const double *__restrict__ x,
+                                              const double *__restrict__ f, double *__restrict__ out) {
+  int c = 0;
+  while (c + 1 < a.nclass && (int)blockIdx.x >= a.bstart[c + 1]) c++;
+  const long long k = (long long)(blockIdx.x - a.bstart[c]) * kB + threadIdx.x;
+  if (k >= a.c[c].nodes) return;
+  const double *st = st64 + c * kCst;
+  if (P == 1) {
+    lat64_node<1, 0, 0, 0, MODE>(a, a.c[c], k, st, x, f, out);
+  } else {
+    switch (c) {
+      case 0: lat64_node<2, 0, 0, 0, MODE>(a, a.c[c], k, st, x, f, out); break;
+      case 1: lat64_node<2, 1, 0, 0, MODE>(a, a.c[c], k, st, x, f, out); break;
+      case 2: lat64_node<2, 0, 1, 0, MODE>(a, a.c[c], k, st, x, f, out); break;
+      case 3: lat64_node<2, 1, 1, 0, MODE>(a, a.c[c], k, st, x, f, out); break;
+      case 4: lat64_node<2, 0, 0, 1, MODE>(a, a.c[c], k, st, x, f, out); break;
+      case 5: lat64_node<2, 1, 0, 1, MODE>(a, a.c[c], k, st, x, f, out); break;
+      case 6: lat64_node<2, 0, 1, 1, MODE>(a, a.c[c], k, st, x, f, out); break;
+      default: lat64_node<2, 1, 1, 1, MODE>(a, a.c[c], k, st, x, f, out); break;
+    }
+  }
+}
+
+// Dirichlet rows (DOFs nact..ndof-1): identity
+__global__ void lat64_boundary_k(int nact, int ndof, const int *__restrict__ dof2node, const double *__restrict__ x,
+                                 const double *__restrict__ f, double *__restrict__ out, int defect) {
+  const int d = nact + blockIdx.x * kB + threadIdx.x;
+  if (d >= ndof) return;
+  const int node = dof2node[d];
+  out[node] = defect ? f[node] - x[node] : x[node];
+}
+
+// DOF order <-> lattice order, f64, every DOF (dir 0: to lattice)
+__global__ void lat64_perm_k(int ndof, const int *__restrict__ dof2node, const double *__restrict__ in,
+                             double *__restrict__ out, int dir) {
+  const int d = blockIdx.x * kB + threadIdx.x;
+  if (d >= ndof) return;
+  if (dir == 0) out[dof2node[d]] = in[d];
+  else out[d] = in[dof2node[d]];
+}
+
+__global__ void cast_d2f_k(long long n, const double *__restrict__ a, float *__restrict__ b) {
+  for (long long i = (long long)blockIdx.x * kB + threadIdx.x; i < n; i += (long long)gridDim.x * kB)
+    b[i] = (float)a[i];
+}
+
+__global__ void cast_f2d_k(long long n, const float *__restrict__ a, double *__restrict__ b) {
+  for (long long i = (long long)blockIdx.x * kB + threadIdx.x; i < n; i += (long long)gridDim.x * kB)
+    b[i] = (double)a[i];
+}
+
 // DOF-ordered f64 <-> lattice-ordered f32 (the boundary of the lattice vector
 // is left untouched: it stays zero)
 __global__ void dof_to_lat_k(int n, int nact, const int *__restrict__ dof2node, const double *__restrict__ v,
@@ -461,9 +539,11 @@ hipError_t kl_detect_uniform(hipStream_t s, DevLevel &L, int *flag_host, bool *a
   if (L.nclass <= 0) return hipSuccess;
   if (L.lat_fresh) {  // kl_assemble_uniform wrote the compact stencils of this matrix
     L.lat_fresh = false;
+    L.lat_exact64 = true;
     for (int c = 0; c < L.nclass; c++) L.uni[c] = 1;
     return hipSuccess;
   }
+  L.lat_exact64 = false;
   for (int c = 0; c < L.nclass; c++) L.uni[c] = 0;
   const LatArgs a = lat_args(L);
   lat_first_node_k<<<L.nclass, 64, 0, s>>>(a, L.lperm, L.lnode2dof, L.diag, L.val, L.lscale, L.lflag, L.lcst);
@@ -508,6 +588,39 @@ void kl_assemble_uniform(hipStream_t s, DevLevel &L, int order, const double *ta
     csr_fill_k<<<dim3(gx, maxs, L.nclass), kB, 0, s>>>(a, L.lperm, L.lst64, val);
     rhs_fill_k<<<(int)((L.lat_nint + kB - 1) / kB), kB, 0, s>>>(a, L.lat_nint, L.lnode2dof, L.lst64, rhs);
   }
+}
+
+template <int MODE>
+void lat64_apply(hipStream_t s, const DevLevel &L, int order, const double *x, const double *f, double *out) {
+  const LatArgs a = lat_args(L);
+  const int grid = a.bstart[L.nclass];
+  if (grid > 0) {
+    if (order == 1) lat64_k<1, MODE><<<grid, kB, 0, s>>>(a, L.lst64, x, f, out);
+    else lat64_k<2, MODE><<<grid, kB, 0, s>>>(a, L.lst64, x, f, out);
+  }
+  const int nd = L.ndof - L.nact;
+  if (nd > 0)
+    lat64_boundary_k<<<(nd + kB - 1) / kB, kB, 0, s>>>(L.nact, L.ndof, L.ldof2node, x, f, out,
+                                                      MODE == kL64Defect ? 1 : 0);
+}
+
+void kl_spmv64(hipStream_t s, const DevLevel &L, int order, const double *x, double *y) {
+  lat64_apply<kL64Spmv>(s, L, order, x, nullptr, y);
+}
+void kl_defect64(hipStream_t s, const DevLevel &L, int order, const double *x, const double *f, double *d) {
+  lat64_apply<kL64Defect>(s, L, order, x, f, d);
+}
+void kl_dof_to_lat64(hipStream_t s, const DevLevel &L, const double *v, double *l) {
+  lat64_perm_k<<<(L.ndof + kB - 1) / kB, kB, 0, s>>>(L.ndof, L.ldof2node, v, l, 0);
+}
+void kl_lat64_to_dof(hipStream_t s, const DevLevel &L, const double *l, double *v) {
+  lat64_perm_k<<<(L.ndof + kB - 1) / kB, kB, 0, s>>>(L.ndof, L.ldof2node, l, v, 1);
+}
+void kl_cast_d2f(hipStream_t s, int64_t n, const double *a, float *b) {
+  cast_d2f_k<<<(int)std::min<int64_t>((n + kB - 1) / kB, 2048), kB, 0, s>>>(n, a, b);
+}
+void kl_cast_f2d(hipStream_t s, int64_t n, const float *a, double *b) {
+  cast_f2d_k<<<(int)std::min<int64_t>((n + kB - 1) / kB, 2048), kB, 0, s>>>(n, a, b);
 }
 
 void kl_from_dof(hipStream_t s, const DevLevel &L, const double *v, float *l) {
