@@ -44,50 +44,29 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-class HipEvents:
-    """hipEvent timing on the library's own stream."""
-
-    def __init__(self):
-        self.h = C.CDLL("libamdhip64.so")
-        self.h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
-        self.h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
-        self.h.hipEventSynchronize.argtypes = [C.c_void_p]
-        self.h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
-
-    def event(self):
-        e = C.c_void_p()
-        assert self.h.hipEventCreate(C.byref(e)) == 0
-        return e
-
-    def elapsed_ms(self, a, b):
-        self.h.hipEventSynchronize(b)
-        ms = C.c_float()
-        self.h.hipEventElapsedTime(C.byref(ms), a, b)
-        return ms.value
+KERNEL_NAMES = {0: "csr_stream<kSpmv,f64>", 1: "lat_sweep_k<kLJacobi,f32> (compact class stencils)",
+                2: "lat64_k<kL64Spmv,f64>", 3: "lat_restrict_k<f32>", 4: "lat_prolong_k<f32>", 5: "csr_fill_k",
+                6: "copy_k (f64 vector copy, traffic calibration)"}
 
 
-def spmv_roofline(pm, P, reps=50):
-    """Average duration of the fine-level FP64 CSR SpMV kernel, HIP events on
-    the handle's stream; algorithmic bytes 12 nnz + 20 N (f64 values, int32
-    columns and row pointers, x and y once)."""
+def kernel_table(P, reps=50):
+    """Every kernel of the path on the finest level, timed with HIP events on
+    the handle's stream by the library (cd3d_kernel_bench): mean duration and
+    algorithmic bytes per launch."""
     l = P.levels - 1
-    info = P.info(l)
-    x, y = P.device_ptr(l, 0), P.device_ptr(l, 2)
-    ev = HipEvents()
-    stream = C.c_void_p(P.L.cd3d_stream(P.h))
-    a, b = ev.event(), ev.event()
-    for _ in range(5):
-        P.matvec_dev(l, x, y)
-    ev.h.hipEventRecord(a, stream)
-    for _ in range(reps):
-        P.matvec_dev(l, x, y)
-    ev.h.hipEventRecord(b, stream)
-    ms = ev.elapsed_ms(a, b) / reps
-    nbytes = 12 * info["nnz"] + 20 * info["ndof"]
-    gbs = nbytes / (ms * 1e-3) / 1e9
-    return dict(bound="hbm", achieved=round(gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(gbs / HBM_PEAK_GBS, 4),
-                traffic=None, kernel="csr_stream<kSpmv,f64> fine level", us_per_launch=round(ms * 1e3, 2),
-                algorithmic_bytes=nbytes, nnz=info["nnz"], n=info["ndof"])
+    out = {}
+    for which, name in KERNEL_NAMES.items():
+        ms, nbytes = P.kernel_bench(which, l, reps)
+        out[name] = dict(us_per_launch=round(ms * 1e3, 2), algorithmic_bytes=int(nbytes),
+                         achieved_gbs=round(nbytes / (ms * 1e-3) / 1e9, 1))
+    return out
+
+
+def roofline_entry(table, which, kernel_note):
+    k = table[KERNEL_NAMES[which]]
+    return dict(bound="hbm", achieved=k["achieved_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(k["achieved_gbs"] / HBM_PEAK_GBS, 4), traffic=None, kernel=KERNEL_NAMES[which],
+                note=kernel_note, us_per_launch=k["us_per_launch"], algorithmic_bytes=k["algorithmic_bytes"])
 
 
 def run_steps(pm, P, sp, co, warmup, steps):
@@ -305,6 +284,10 @@ def main():
     ap.add_argument("--c4-problems", type=int, default=256)
     ap.add_argument("--c5-rates", type=str, default="10,50")
     ap.add_argument("--c5-seconds", type=float, default=4.0)
+    ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both",
+                    help="run one form only (profiling)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="assemble once and time the path's kernels (for rocprofv3 counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every MI355X box here has one GPU)",
@@ -317,9 +300,15 @@ def main():
     P = pm.Problem(**WORKLOAD)
     log(f"[bench] setup {time.perf_counter() - t:.1f} s, device memory {P.device_bytes() / 2**30:.2f} GiB")
     co = pm.coeffs(**COEFFS)
+    if args.kernels_only:
+        P.assemble(co)
+        print(json.dumps(kernel_table(P)))
+        P.close()
+        return 0
     res = {}
     sols = {}
-    for name, prec in (("fp64", pm.FP64), ("mixed", pm.MIXED)):
+    forms = [f for f in (("fp64", pm.FP64), ("mixed", pm.MIXED)) if args.precision in ("both", f[0])]
+    for name, prec in forms:
         sp = pm.solver_params(**SOLVER, precision=prec)
         log(f"[bench] {name}: {args.warmup} warmup + {args.steps} timed steps")
         lat, its, rr = run_steps(pm, P, sp, co, args.warmup, args.steps)
@@ -327,9 +316,17 @@ def main():
         res[name] = dict(solves_per_s=args.steps / lat.sum(), ms_per_step=1e3 * lat.mean(),
                          p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
                          iterations=int(np.median(its)), max_rel_residual=float(max(rr)))
+    if args.precision != "both":
+        print(json.dumps({k: r5(v) for k, v in res.items()}))
+        P.close()
+        return 0
     err = float(np.linalg.norm(sols["mixed"] - sols["fp64"]) / np.linalg.norm(sols["fp64"]))
-    log("[bench] SpMV roofline")
-    roof = spmv_roofline(pm, P)
+    log("[bench] kernel table (HIP events on the library stream)")
+    table = kernel_table(P)
+    roof = roofline_entry(table, 1, "dominant kernel of the mixed step: the fine-level FP32 Jacobi sweep "
+                                    "(algorithmic bytes: x, f and the result once, 12 bytes per node)")
+    spmv = roofline_entry(table, 0, "FP64 CSR SpMV of the fine level (12 nnz + 20 N bytes); the FP64 form's "
+                                    "smoother, defect and FGMRES product")
     cpu = None
     if not args.no_cpu:
         log("[bench] reference CPU baseline (bounded sample)")
@@ -365,8 +362,10 @@ def main():
         "mixed_vs_fp64_speedup": round(m["solves_per_s"] / res["fp64"]["solves_per_s"], 3),
         "mixed_solution_rel_err_vs_fp64": err,
         "p50_ms": round(m["p50_ms"], 3), "p99_ms": round(m["p99_ms"], 3),
-        "spmv_gbs": roof["achieved"],
+        "spmv_gbs": spmv["achieved"],
         "roofline": roof,
+        "spmv_fp64_csr": spmv,
+        "kernels_fine_level": table,
         "cpu_baseline": cpu,
     }
     out.update(r5(extra))

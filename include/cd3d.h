@@ -174,6 +174,20 @@ cd3d_status cd3d_tcd_get_state(cd3d_problem *p, double *u_host);
 /* nodal interpolation of the Gaussian pulse (TFEFunction3D::Interpolate, FEFunction3D.C:1446) */
 cd3d_status cd3d_interp_pulse(cd3d_problem *p, int level, const cd3d_coeffs *co, double *u_host);
 
+/* Measurement: launch one kernel of the path `reps` times on the handle's
+ * stream between two HIP events (after one untimed launch) and return the
+ * mean duration per launch and the kernel's algorithmic bytes per launch.
+ * which: 0 FP64 CSR SpMV (12 nnz + 20 N bytes); 1 FP32 lattice Jacobi sweep
+ * (compact class stencils: 12 bytes per node; per-node stencils: + 4 per
+ * nonzero + 4 per node); 2 FP64 lattice operator (16 bytes per node);
+ * 3 FP32 lattice restriction to level-1 (4 N_fine + 4 N_coarse);
+ * 4 FP32 lattice prolongation + update from level-1 (8 N_fine + 4 N_coarse);
+ * 5 CSR values from the class stencils (12 bytes per nonzero); 6 copy of an
+ * FP64 level vector (16 N: the traffic-counter calibration).
+ * Needs an assembled problem; the level vectors are overwritten. */
+cd3d_status cd3d_kernel_bench(cd3d_problem *p, int which, int level, int reps, double *ms_per_launch,
+                              double *bytes_per_launch);
+
 /* device pointers of level vectors for timing and zero-copy callers:
  * which 0 = solution, 1 = rhs, 2 = scratch vector, 3 = scratch vector */
 cd3d_status cd3d_device_ptr(cd3d_problem *p, int level, int which, double **dev);

@@ -26,7 +26,7 @@ EXPORTS = ["cd3d_default_solver", "cd3d_create", "cd3d_destroy", "cd3d_last_erro
            "cd3d_set_entries", "cd3d_assemble", "cd3d_matvec", "cd3d_matvec_dev", "cd3d_prolongate",
            "cd3d_restrict", "cd3d_vcycle", "cd3d_solve", "cd3d_get_solution", "cd3d_tcd_run",
            "cd3d_tcd_init", "cd3d_tcd_step", "cd3d_tcd_get_state",
-           "cd3d_interp_pulse", "cd3d_device_ptr", "cd3d_stream", "cd3d_synchronize",
+           "cd3d_interp_pulse", "cd3d_kernel_bench", "cd3d_device_ptr", "cd3d_stream", "cd3d_synchronize",
            # cd3d_server.h
            "cd3d_server_default_params", "cd3d_estimate_bytes", "cd3d_server_create", "cd3d_server_submit",
            "cd3d_server_wait", "cd3d_server_drain", "cd3d_server_get_stats", "cd3d_server_destroy"]
@@ -130,6 +130,7 @@ def lib():
     L.cd3d_tcd_get_state.argtypes = [P, dp]
     L.cd3d_interp_pulse.argtypes = [P, I, C.POINTER(Coeffs), dp]
     L.cd3d_device_ptr.argtypes = [P, I, I, C.POINTER(C.c_void_p)]
+    L.cd3d_kernel_bench.argtypes = [P, I, I, I, dp, dp]
     L.cd3d_stream.argtypes = [P]
     L.cd3d_stream.restype = C.c_void_p
     L.cd3d_synchronize.argtypes = [P]
@@ -319,6 +320,16 @@ class Problem:
         u = np.zeros(self.info(self.levels - 1)["ndof"])
         self._chk(self.L.cd3d_tcd_get_state(self.h, _dp(u)), "tcd_get_state")
         return u
+
+    KERNELS = {0: "csr_stream<kSpmv> FP64", 1: "lat_sweep_k<kLJacobi> FP32", 2: "lat64_k<kL64Spmv> FP64",
+               3: "lat_restrict_k FP32", 4: "lat_prolong_k FP32", 5: "csr_fill_k", 6: "copy_k FP64"}
+
+    def kernel_bench(self, which, level, reps=50):
+        """(ms per launch, algorithmic bytes per launch) of one kernel of the path,
+        timed with HIP events on the handle's stream."""
+        ms, b = C.c_double(), C.c_double()
+        self._chk(self.L.cd3d_kernel_bench(self.h, which, level, reps, C.byref(ms), C.byref(b)), "kernel_bench")
+        return ms.value, b.value
 
     def device_ptr(self, l, which):
         p = C.c_void_p()

@@ -182,8 +182,10 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
     D.lat_nint = (int64_t)H.lat_node2dof.size();
     D.ldof2node = dalloc<int>(P, H.dof_node.size());
     upload(P, D.ldof2node, H.dof_node);
-    D.lperm = dalloc<int>(P, H.lat_perm.size());
-    upload(P, D.lperm, H.lat_perm);
+    D.lsid = dalloc<uint16_t>(P, H.lat_sid.size());
+    upload(P, D.lsid, H.lat_sid);
+    D.lfirst = dalloc<int>(P, H.lat_first.size());
+    upload(P, D.lfirst, H.lat_first);
     D.lnode2dof = dalloc<int>(P, H.lat_node2dof.size());
     upload(P, D.lnode2dof, H.lat_node2dof);
     D.lcst = dalloc<float>(P, 8 * 128);
@@ -334,6 +336,8 @@ static void refresh_coarse_and_f32(cd3d_problem &P) {
         if (!D.lval) {
           D.lval = dalloc<float>(P, (size_t)D.lat_nval);
           D.linv = dalloc<float>(P, (size_t)D.lat_nint);
+          D.lperm = dalloc<int>(P, P.hl[l].lat_perm.size());
+          upload(P, D.lperm, P.hl[l].lat_perm);
         }
         HIPCHK(kl_convert(P.stream, D));
       }
@@ -683,11 +687,11 @@ size_t cd3d_estimate_bytes(const cd3d_mesh_desc *mesh, int restart, int cellwise
       b += 4 * (N + 1) + 4 * (prevN + 1) + 2 * 12 * pnnz;  // P and R = P^T
     }
     b += 16 * ncell + 4 * nb * ncell + 4 * ncell;   // cell geometry, DOF map, colours
-    b += 4 * N + 4 * nact_nnz + 4 * na + 4 * 4 * N;  // lattice: maps, stencil positions, vectors
-    b += 8 * (nb * nb + nb) + 8 * 8 * 129 + 4 * 8 * 128;
+    b += 4 * N + 2 * nact_nnz + 4 * na + 4 * 4 * N;  // lattice: maps, stencil ids, vectors
+    b += 8 * (nb * nb + nb) + 8 * 8 * 129 + 4 * 8 * 128 * 2;
     if (cellwise_assembly) {
-      b += 4 * nb * nb * ncell;                     // slot maps
-      if (l > 0) b += 4 * nact_nnz + 4 * na;        // per-node FP32 stencils
+      b += 4 * nb * nb * ncell;                       // slot maps
+      if (l > 0) b += 8 * nact_nnz + 4 * na;          // per-node FP32 stencils and their CSR positions
     }
     if (tcd) b += 8 * nnz;                          // conv-diff matrix beside the mass matrix
     total += b + (2 << 20);                         // allocation granularity of ~20 buffers
@@ -1123,6 +1127,77 @@ cd3d_status cd3d_device_ptr(cd3d_problem *p, int level, int which, double **dev)
   double *v[4] = {D.sol, D.rhs, D.aux, D.tmp};
   *dev = v[which];
   return CD3D_OK;
+}
+
+cd3d_status cd3d_kernel_bench(cd3d_problem *p, int which, int level, int reps, double *ms, double *bytes) {
+  if (!p || !ms || !bytes || reps < 1 || level < 0 || level >= (int)p->dl.size() || which < 0 || which > 6)
+    return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  return run(p, [&]() {
+    if (!p->assembled) throw std::logic_error("assemble before cd3d_kernel_bench");
+    DevLevel &L = p->dl[level];
+    const int o = p->mesh.order;
+    const double N = L.ndof, nnz = (double)L.nnz, nint = (double)L.lat_nint;
+    std::function<void()> launch;
+    double b = 0;
+    switch (which) {
+      case 0:
+        launch = [&] { k_spmv(p->stream, L, L.sol, L.aux); };
+        b = 12 * nnz + 20 * N;
+        break;
+      case 1: {
+        if (level == 0) throw ArgError("the coarsest level has no FP32 lattice stencils");
+        bool uni = true;
+        for (int c = 0; c < L.nclass; c++) uni = uni && L.uni[c];
+        launch = [&] { kl_jacobi(p->stream, L, o, L.lx, L.lf, L.lt, 0.5f); };
+        b = uni ? 12 * N : 12 * N + 4 * (double)L.lat_nval + 4 * nint;
+        break;
+      }
+      case 2:
+        if (!L.lat_exact64) throw ArgError("the FP64 lattice operator needs constant-coefficient assembly");
+        launch = [&] { kl_spmv64(p->stream, L, o, L.sol, L.aux); };
+        b = 16 * N;
+        break;
+      case 3:
+      case 4: {
+        if (level == 0) throw ArgError("transfers need level >= 1");
+        DevLevel &C = p->dl[level - 1];
+        const double nc = (double)C.nn * C.nn * C.nn;
+        if (which == 3) {
+          launch = [&] { kl_restrict(p->stream, L, C, o, L.la, C.lf); };
+          b = 4 * N + 4 * nc;
+        } else {
+          launch = [&] { kl_prolong_update(p->stream, L, C, o, C.lx, L.lt, 1.0f); };
+          b = 8 * N + 4 * nc;
+        }
+        break;
+      }
+      case 5:
+        if (!L.lat_exact64) throw ArgError("the class stencils need constant-coefficient assembly");
+        launch = [&] { kl_fill_csr(p->stream, L, L.val); };
+        b = 12 * nnz;
+        break;
+      case 6:
+        launch = [&] { k_copy(p->stream, L.ndof, L.sol, L.aux); };
+        b = 16 * N;
+        break;
+    }
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    launch();
+    HIPCHK(hipEventRecord(e0, p->stream));
+    for (int r = 0; r < reps; r++) launch();
+    HIPCHK(hipEventRecord(e1, p->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(hipGetLastError());
+    *ms = t / reps;
+    *bytes = b;
+  });
 }
 
 void *cd3d_stream(cd3d_problem *p) { return p ? (void *)p->stream : nullptr; }

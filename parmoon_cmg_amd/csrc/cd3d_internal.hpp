@@ -64,6 +64,8 @@ struct HostLevel {
   std::vector<LatClass> lat;
   int64_t lat_nval = 0;             // sum over classes of nsten * nodes
   std::vector<int> lat_perm;        // CSR position of every stencil value
+  std::vector<uint16_t> lat_sid;    // per active-row CSR entry: class * 128 + stencil entry
+  std::vector<int> lat_first;       // CSR position of entry s of class c's first node (c * 128 + s)
   std::vector<int> lat_node2dof;    // interior node (class numbering) -> DOF
 };
 
@@ -99,7 +101,9 @@ struct DevLevel {
   LatClass cls[8];
   int64_t lat_nval = 0, lat_nint = 0;
   float *lval = nullptr;       // stencil values f32
-  int *lperm = nullptr;        // CSR position of each stencil value
+  int *lperm = nullptr;        // CSR position of each stencil value (per-node FP32 path only)
+  uint16_t *lsid = nullptr;    // stencil id of each active-row CSR entry
+  int *lfirst = nullptr;       // CSR positions of each class's first node (8 x 128)
   float *linv = nullptr;       // 1/a_ii per interior node (class numbering)
   int *lnode2dof = nullptr;    // interior node -> DOF
   int *ldof2node = nullptr;    // DOF -> lattice node
@@ -218,6 +222,8 @@ hipError_t kl_convert(hipStream_t s, const DevLevel &L);
 // the class stencils (also writes the FP32 compact stencils)
 void kl_assemble_uniform(hipStream_t s, DevLevel &L, int order, const double *tab, const AsmParams &ap,
                          double *val, double *rhs);
+// the interior-row CSR values from the FP64 class stencils lst64
+void kl_fill_csr(hipStream_t s, const DevLevel &L, double *val);
 // FP64 operator in lattice order (class stencils lst64, identity rows on the
 // boundary) and the order / precision changes of the lattice FGMRES
 void kl_spmv64(hipStream_t s, const DevLevel &L, int order, const double *x, double *y);

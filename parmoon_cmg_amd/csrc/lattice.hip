@@ -118,97 +118,166 @@ __global__ __launch_bounds__(kB) void lat_sweep_k(LatArgs a, const float *__rest
   if (k < a.c[c].nodes) lat_class_node<P, MODE>(a, c, k, val, inv, cst, x, f, out, omega);
 }
 
-// 1D transfer weights.  Restriction gathers, for a coarse node Xc, the fine
-// nodes around 2 Xc; prolongation gathers, for a fine node X, the coarse nodes
-// whose basis functions do not vanish at X.
-template <int P>
-__host__ __device__ inline int restr_1d(int Xc, int *off, float *w) {
-  if (P == 1) {
-    off[0] = -1; w[0] = 0.5f; off[1] = 0; w[1] = 1.f; off[2] = 1; w[2] = 0.5f;
-    return 3;
+// 1D transfer weights (nodal interpolation of the coarse basis at the fine
+// nodes: Q1 1, 1/2; Q2 1, 3/4, 3/8, -1/8).
+//
+// Restriction R = P^T, one thread per coarse interior node: a coarse node Xc
+// sits on fine node 2 Xc and gathers, per axis, Q1: offsets -1 0 1 with
+// weights 1/2 1 1/2; Q2 even Xc (cell vertex): -3 -1 0 1 3 with -1/8 3/8 1
+// 3/8 -1/8; Q2 odd Xc (cell midpoint): -1 0 1 with 3/4 1 3/4.  The parity of
+// the coarse node fixes the pattern, so the coarse level's parity classes
+// give loops with compile-time bounds (all loads in flight at once).
+template <int P, int PXS>
+struct Restr1D;
+template <int PXS>
+struct Restr1D<1, PXS> {
+  static constexpr int n = 3;
+  __host__ __device__ static constexpr int off(int i) { return i - 1; }
+  __host__ __device__ static constexpr float w(int i) { return i == 1 ? 1.0f : 0.5f; }
+};
+template <>
+struct Restr1D<2, 0> {
+  static constexpr int n = 5;
+  __host__ __device__ static constexpr int off(int i) { return i == 0 ? -3 : i == 1 ? -1 : i == 2 ? 0 : i == 3 ? 1 : 3; }
+  __host__ __device__ static constexpr float w(int i) {
+    return i == 0 || i == 4 ? -0.125f : i == 2 ? 1.0f : 0.375f;
   }
-  if (Xc % 2 == 0) {
-    off[0] = -3; w[0] = -0.125f; off[1] = -1; w[1] = 0.375f; off[2] = 0; w[2] = 1.f;
-    off[3] = 1; w[3] = 0.375f; off[4] = 3; w[4] = -0.125f;
-    return 5;
-  }
-  off[0] = -1; w[0] = 0.75f; off[1] = 0; w[1] = 1.f; off[2] = 1; w[2] = 0.75f;
-  return 3;
-}
+};
+template <>
+struct Restr1D<2, 1> {
+  static constexpr int n = 3;
+  __host__ __device__ static constexpr int off(int i) { return i - 1; }
+  __host__ __device__ static constexpr float w(int i) { return i == 1 ? 1.0f : 0.75f; }
+};
 
-template <int P>
-__host__ __device__ inline int prol_1d(int X, int *idx, float *w) {
-  if (P == 1) {
-    if ((X & 1) == 0) { idx[0] = X / 2; w[0] = 1.f; return 1; }
-    idx[0] = (X - 1) / 2; w[0] = 0.5f; idx[1] = (X + 1) / 2; w[1] = 0.5f;
-    return 2;
-  }
-  const int t = X & 3;
-  if (t == 0 || t == 2) { idx[0] = X / 2; w[0] = 1.f; return 1; }
-  if (t == 1) {
-    idx[0] = (X - 1) / 2; w[0] = 0.375f; idx[1] = (X + 1) / 2; w[1] = 0.75f; idx[2] = (X + 3) / 2; w[2] = -0.125f;
-    return 3;
-  }
-  idx[0] = (X - 3) / 2; w[0] = -0.125f; idx[1] = (X - 1) / 2; w[1] = 0.75f; idx[2] = (X + 1) / 2; w[2] = 0.375f;
-  return 3;
-}
-
-// coarse interior node t <- R fine (DefectRestriction + CorrectDefect: the
-// coarse boundary stays zero)
-template <int P>
-__host__ __device__ inline void restrict_node(long long t, int nnf, int nnc, const float *__restrict__ fine,
-                                              float *__restrict__ coarse) {
-  const int m = nnc - 2;
-  const int Xc = 1 + (int)(t % m), Yc = 1 + (int)((t / m) % m), Zc = 1 + (int)(t / ((long long)m * m));
-  int ox[5], oy[5], oz[5];
-  float wx[5], wy[5], wz[5];
-  const int nx = restr_1d<P>(Xc, ox, wx), ny = restr_1d<P>(Yc, oy, wy), nz = restr_1d<P>(Zc, oz, wz);
-  const long long F = nnf;
+template <int P, int PX, int PY, int PZ>
+__host__ __device__ inline void restrict_node(const LatArgs &ca, const LatDev &C, long long k, int nnf,
+                                              const float *__restrict__ fine, float *__restrict__ coarse) {
+  using RX = Restr1D<P, PX>;
+  using RY = Restr1D<P, PY>;
+  using RZ = Restr1D<P, PZ>;
+  const int kx = (int)(k % C.cnt[0]), ky = (int)((k / C.cnt[0]) % C.cnt[1]), kz = (int)(k / ((long long)C.cnt[0] * C.cnt[1]));
+  const int Xc = C.lo[0] + C.step * kx, Yc = C.lo[1] + C.step * ky, Zc = C.lo[2] + C.step * kz;
+  const long long F = nnf, centre = 2 * Xc + F * (2 * Yc + F * 2 * Zc);
   float s = 0.f;
-  for (int c = 0; c < nz; c++) {
-    const long long zrow = (long long)(2 * Zc + oz[c]) * F * F;
-    for (int b = 0; b < ny; b++) {
-      const long long yrow = zrow + (long long)(2 * Yc + oy[b]) * F;
-      float sy = 0.f;
-      for (int a = 0; a < nx; a++) sy += wx[a] * fine[yrow + 2 * Xc + ox[a]];
-      s += wz[c] * wy[b] * sy;
+#pragma unroll
+  for (int c = 0; c < RZ::n; c++) {
+    float sy = 0.f;
+#pragma unroll
+    for (int b = 0; b < RY::n; b++) {
+      float sx = 0.f;
+#pragma unroll
+      for (int a = 0; a < RX::n; a++) sx += RX::w(a) * fine[centre + RX::off(a) + F * (RY::off(b) + F * RZ::off(c))];
+      sy += RY::w(b) * sx;
+    }
+    s += RZ::w(c) * sy;
+  }
+  coarse[Xc + (long long)ca.nn * (Yc + (long long)ca.nn * Zc)] = s;
+}
+
+template <int P>
+__host__ __device__ inline void restrict_class_node(const LatArgs &ca, int c, long long k, int nnf,
+                                                    const float *__restrict__ fine, float *__restrict__ coarse) {
+  const LatDev &C = ca.c[c];
+  if (P == 1) {
+    restrict_node<1, 0, 0, 0>(ca, C, k, nnf, fine, coarse);
+  } else {
+    switch (c) {
+      case 0: restrict_node<2, 0, 0, 0>(ca, C, k, nnf, fine, coarse); break;
+      case 1: restrict_node<2, 1, 0, 0>(ca, C, k, nnf, fine, coarse); break;
+      case 2: restrict_node<2, 0, 1, 0>(ca, C, k, nnf, fine, coarse); break;
+      case 3: restrict_node<2, 1, 1, 0>(ca, C, k, nnf, fine, coarse); break;
+      case 4: restrict_node<2, 0, 0, 1>(ca, C, k, nnf, fine, coarse); break;
+      case 5: restrict_node<2, 1, 0, 1>(ca, C, k, nnf, fine, coarse); break;
+      case 6: restrict_node<2, 0, 1, 1>(ca, C, k, nnf, fine, coarse); break;
+      default: restrict_node<2, 1, 1, 1>(ca, C, k, nnf, fine, coarse); break;
     }
   }
-  coarse[Xc + (long long)nnc * (Yc + (long long)nnc * Zc)] = s;
 }
 
+// coarse interior <- R fine (DefectRestriction + CorrectDefect: the coarse
+// boundary stays zero); ca: the coarse level's classes
 template <int P>
-__global__ __launch_bounds__(kB) void lat_restrict_k(int nnf, int nnc, const float *__restrict__ fine,
+__global__ __launch_bounds__(kB) void lat_restrict_k(LatArgs ca, int nnf, const float *__restrict__ fine,
                                                      float *__restrict__ coarse) {
-  const long long m = nnc - 2, t = (long long)blockIdx.x * kB + threadIdx.x;
-  if (t < m * m * m) restrict_node<P>(t, nnf, nnc, fine, coarse);
+  int c = 0;
+  while (c + 1 < ca.nclass && (int)blockIdx.x >= ca.bstart[c + 1]) c++;
+  const long long k = (long long)(blockIdx.x - ca.bstart[c]) * kB + threadIdx.x;
+  if (k < ca.c[c].nodes) restrict_class_node<P>(ca, c, k, nnf, fine, coarse);
 }
 
-// fine interior node t: x += damp (P e)  (Prolongate + CorrectNodes + Update)
+// Prolongation + CorrectNodes + Update, one thread per coarse cell: the cell's
+// (P+1)^3 coarse values interpolated to its fine nodes with local index
+// 0..2P-1 per axis (the last plane belongs to the next cell) by three 1D
+// passes; fine boundary nodes are left alone.  Local fine position t in the
+// cell: Q1 t = 0: (1, 0), t = 1: (1/2, 1/2); Q2 t = 0: (1, 0, 0),
+// t = 1: (3/8, 3/4, -1/8), t = 2: (0, 1, 0), t = 3: (-1/8, 3/4, 3/8).
 template <int P>
-__host__ __device__ inline void prolong_node(long long t, int nnf, int nnc, const float *__restrict__ coarse,
+__host__ __device__ constexpr float prol_w(int t, int a) {
+  return P == 1 ? (t == 0 ? (a == 0 ? 1.0f : 0.0f) : 0.5f)
+                : (t == 0   ? (a == 0 ? 1.0f : 0.0f)
+                   : t == 2 ? (a == 1 ? 1.0f : 0.0f)
+                   : t == 1 ? (a == 0 ? 0.375f : a == 1 ? 0.75f : -0.125f)
+                            : (a == 0 ? -0.125f : a == 1 ? 0.75f : 0.375f));
+}
+
+template <int P>
+__host__ __device__ inline void prolong_cell(long long t, int ncc, int nnf, int nnc, const float *__restrict__ coarse,
                                              float *__restrict__ x, float damp) {
-  const int m = nnf - 2;
-  const int X = 1 + (int)(t % m), Y = 1 + (int)((t / m) % m), Z = 1 + (int)(t / ((long long)m * m));
-  int ix[3], iy[3], iz[3];
-  float wx[3], wy[3], wz[3];
-  const int nx = prol_1d<P>(X, ix, wx), ny = prol_1d<P>(Y, iy, wy), nz = prol_1d<P>(Z, iz, wz);
-  const long long C = nnc;
-  float s = 0.f;
-  for (int c = 0; c < nz; c++)
-    for (int b = 0; b < ny; b++) {
-      float sy = 0.f;
-      for (int a = 0; a < nx; a++) sy += wx[a] * coarse[ix[a] + C * (iy[b] + C * iz[c])];
-      s += wz[c] * wy[b] * sy;
+  constexpr int P1 = P + 1, T = 2 * P;
+  const int cx = (int)(t % ncc), cy = (int)((t / ncc) % ncc), cz = (int)(t / ((long long)ncc * ncc));
+  const long long C = nnc, F = nnf;
+  float cv[P1][P1][P1];
+#pragma unroll
+  for (int c = 0; c < P1; c++)
+#pragma unroll
+    for (int b = 0; b < P1; b++)
+#pragma unroll
+      for (int a = 0; a < P1; a++) cv[c][b][a] = coarse[(P * cx + a) + C * ((P * cy + b) + C * (P * cz + c))];
+#pragma unroll
+  for (int tz = 0; tz < T; tz++) {
+    const int Z = T * cz + tz;
+    if (Z == 0) continue;
+    float pz[P1][P1];
+#pragma unroll
+    for (int b = 0; b < P1; b++)
+#pragma unroll
+      for (int a = 0; a < P1; a++) {
+        float v = 0.f;
+#pragma unroll
+        for (int c = 0; c < P1; c++) v += prol_w<P>(tz, c) * cv[c][b][a];
+        pz[b][a] = v;
+      }
+#pragma unroll
+    for (int ty = 0; ty < T; ty++) {
+      const int Y = T * cy + ty;
+      if (Y == 0) continue;
+      float py[P1];
+#pragma unroll
+      for (int a = 0; a < P1; a++) {
+        float v = 0.f;
+#pragma unroll
+        for (int b = 0; b < P1; b++) v += prol_w<P>(ty, b) * pz[b][a];
+        py[a] = v;
+      }
+#pragma unroll
+      for (int tx = 0; tx < T; tx++) {
+        const int X = T * cx + tx;
+        if (X == 0) continue;
+        float v = 0.f;
+#pragma unroll
+        for (int a = 0; a < P1; a++) v += prol_w<P>(tx, a) * py[a];
+        x[X + F * (Y + F * (long long)Z)] += damp * v;
+      }
     }
-  x[X + (long long)nnf * (Y + (long long)nnf * Z)] += damp * s;
+  }
 }
 
 template <int P>
-__global__ __launch_bounds__(kB) void lat_prolong_k(int nnf, int nnc, const float *__restrict__ coarse,
+__global__ __launch_bounds__(kB) void lat_prolong_k(int ncc, int nnf, int nnc, const float *__restrict__ coarse,
                                                     float *__restrict__ x, float damp) {
-  const long long m = nnf - 2, t = (long long)blockIdx.x * kB + threadIdx.x;
-  if (t < m * m * m) prolong_node<P>(t, nnf, nnc, coarse, x, damp);
+  const long long t = (long long)blockIdx.x * kB + threadIdx.x;
+  if (t < (long long)ncc * ncc * ncc) prolong_cell<P>(t, ncc, nnf, nnc, coarse, x, damp);
 }
 
 __global__ void lat_convert_k(long long n, const int *__restrict__ perm, const double *__restrict__ val,
@@ -219,7 +288,7 @@ __global__ void lat_convert_k(long long n, const int *__restrict__ perm, const d
 
 // Per class: the largest |a| of the first node's stencil (the scale of the
 // uniformity test) and the compact stencil of that node; clears the flag.
-__global__ void lat_first_node_k(LatArgs a, const int *__restrict__ perm, const int *__restrict__ node2dof,
+__global__ void lat_first_node_k(LatArgs a, const int *__restrict__ first, const int *__restrict__ node2dof,
                                  const int *__restrict__ diag, const double *__restrict__ val,
                                  double *__restrict__ scale, int *__restrict__ flag, float *__restrict__ cst) {
   const int c = blockIdx.x;
@@ -227,7 +296,7 @@ __global__ void lat_first_node_k(LatArgs a, const int *__restrict__ perm, const 
   double m = 0.0;
   if (C.nodes > 0)
     for (int s = threadIdx.x; s < C.nsten; s += 64) {
-      const double v = val[perm[C.off + s * C.nodes]];
+      const double v = val[first[c * kCst + s]];
       cst[c * kCst + s] = (float)v;
       m = fmax(m, fabs(v));
     }
@@ -242,15 +311,13 @@ __global__ void lat_first_node_k(LatArgs a, const int *__restrict__ perm, const 
 
 // flag[c] = 1 when some node of class c has a stencil entry that differs from
 // the first node's by more than 1e-12 of the stencil's largest entry
-__global__ void lat_uniform_k(LatArgs a, long long n, const int *__restrict__ perm, const double *__restrict__ val,
-                              const double *__restrict__ scale, int *__restrict__ flag) {
+// (active-row CSR entries in CSR order, each with its stencil id)
+__global__ void lat_uniform_k(long long n, const uint16_t *__restrict__ sid, const int *__restrict__ first,
+                              const double *__restrict__ val, const double *__restrict__ scale,
+                              int *__restrict__ flag) {
   for (long long i = (long long)blockIdx.x * kB + threadIdx.x; i < n; i += (long long)gridDim.x * kB) {
-    int c = 0;
-    while (c + 1 < a.nclass && i >= a.c[c + 1].off) c++;
-    const LatDev &C = a.c[c];
-    const long long s = (i - C.off) / C.nodes;
-    const double v = val[perm[i]], v0 = val[perm[C.off + s * C.nodes]];
-    if (fabs(v - v0) > 1e-12 * scale[c]) flag[c] = 1;
+    const int id = sid[i], c = id >> 7;
+    if (fabs(val[i] - val[first[id]]) > 1e-12 * scale[c]) flag[c] = 1;
   }
 }
 
@@ -362,15 +429,11 @@ __global__ __launch_bounds__(kCst) void class_stencil_k(LatArgs a, const double 
 }
 
 // val at the CSR position of (class z, stencil entry y, node x) = class stencil
-__global__ void csr_fill_k(LatArgs a, const int *__restrict__ perm, const double *__restrict__ st,
+// the active-row CSR entries in CSR order (coalesced writes): val = class stencil entry
+__global__ void csr_fill_k(long long n, const uint16_t *__restrict__ sid, const double *__restrict__ st,
                            double *__restrict__ val) {
-  const int c = blockIdx.z, s = blockIdx.y;
-  const LatDev &C = a.c[c];
-  if (s >= C.nsten) return;
-  const double v = st[c * kCst + s];
-  const int *__restrict__ pp = perm + C.off + (long long)s * C.nodes;
-  for (long long k = (long long)blockIdx.x * kB + threadIdx.x; k < C.nodes; k += (long long)gridDim.x * kB)
-    val[pp[k]] = v;
+  for (long long i = (long long)blockIdx.x * kB + threadIdx.x; i < n; i += (long long)gridDim.x * kB)
+    val[i] = st[sid[i]];
 }
 
 __global__ void rhs_fill_k(LatArgs a, long long n, const int *__restrict__ node2dof, const double *__restrict__ st,
@@ -522,17 +585,18 @@ void kl_defect(hipStream_t s, const DevLevel &L, int order, const float *x, cons
   lat_sweep<kLDefect>(s, L, order, x, f, d, 0.f);
 }
 void kl_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, int order, const float *fine, float *coarse) {
-  const long long m = C.nn - 2, n = m * m * m;
-  if (n <= 0) return;
-  if (order == 1) lat_restrict_k<1><<<(int)((n + kB - 1) / kB), kB, 0, s>>>(F.nn, C.nn, fine, coarse);
-  else lat_restrict_k<2><<<(int)((n + kB - 1) / kB), kB, 0, s>>>(F.nn, C.nn, fine, coarse);
+  const LatArgs ca = lat_args(C);
+  const int grid = ca.bstart[C.nclass];
+  if (grid <= 0) return;
+  if (order == 1) lat_restrict_k<1><<<grid, kB, 0, s>>>(ca, F.nn, fine, coarse);
+  else lat_restrict_k<2><<<grid, kB, 0, s>>>(ca, F.nn, fine, coarse);
 }
 void kl_prolong_update(hipStream_t s, const DevLevel &F, const DevLevel &C, int order, const float *coarse, float *x,
                        float damp) {
-  const long long m = F.nn - 2, n = m * m * m;
+  const long long ncc = (C.nn - 1) / order, n = ncc * ncc * ncc;  // coarse cells
   if (n <= 0) return;
-  if (order == 1) lat_prolong_k<1><<<(int)((n + kB - 1) / kB), kB, 0, s>>>(F.nn, C.nn, coarse, x, damp);
-  else lat_prolong_k<2><<<(int)((n + kB - 1) / kB), kB, 0, s>>>(F.nn, C.nn, coarse, x, damp);
+  if (order == 1) lat_prolong_k<1><<<(int)((n + kB - 1) / kB), kB, 0, s>>>((int)ncc, F.nn, C.nn, coarse, x, damp);
+  else lat_prolong_k<2><<<(int)((n + kB - 1) / kB), kB, 0, s>>>((int)ncc, F.nn, C.nn, coarse, x, damp);
 }
 hipError_t kl_detect_uniform(hipStream_t s, DevLevel &L, int *flag_host, bool *all_uniform) {
   *all_uniform = true;
@@ -546,8 +610,8 @@ hipError_t kl_detect_uniform(hipStream_t s, DevLevel &L, int *flag_host, bool *a
   L.lat_exact64 = false;
   for (int c = 0; c < L.nclass; c++) L.uni[c] = 0;
   const LatArgs a = lat_args(L);
-  lat_first_node_k<<<L.nclass, 64, 0, s>>>(a, L.lperm, L.lnode2dof, L.diag, L.val, L.lscale, L.lflag, L.lcst);
-  lat_uniform_k<<<2048, kB, 0, s>>>(a, L.lat_nval, L.lperm, L.val, L.lscale, L.lflag);
+  lat_first_node_k<<<L.nclass, 64, 0, s>>>(a, L.lfirst, L.lnode2dof, L.diag, L.val, L.lscale, L.lflag, L.lcst);
+  lat_uniform_k<<<2048, kB, 0, s>>>(L.lat_nval, L.lsid, L.lfirst, L.val, L.lscale, L.lflag);
   hipError_t e = hipMemcpyAsync(flag_host, L.lflag, sizeof(int) * L.nclass, hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) {
@@ -577,17 +641,16 @@ void kl_assemble_uniform(hipStream_t s, DevLevel &L, int order, const double *ta
     elem_k<2><<<1, 768, 0, s>>>(tab, ap, L.lelem);
     class_stencil_k<2><<<L.nclass, kCst, 0, s>>>(a, L.lelem, L.lst64, L.lcst);
   }
-  int64_t maxn = 0;
-  int maxs = 0;
-  for (int c = 0; c < L.nclass; c++) {
-    maxn = std::max<int64_t>(maxn, L.cls[c].nodes);
-    maxs = std::max(maxs, L.cls[c].nsten);
-  }
-  const int gx = (int)std::min<int64_t>((maxn + kB - 1) / kB, 64);
-  if (gx > 0) {
-    csr_fill_k<<<dim3(gx, maxs, L.nclass), kB, 0, s>>>(a, L.lperm, L.lst64, val);
+  kl_fill_csr(s, L, val);
+  if (L.lat_nint > 0)
     rhs_fill_k<<<(int)((L.lat_nint + kB - 1) / kB), kB, 0, s>>>(a, L.lat_nint, L.lnode2dof, L.lst64, rhs);
-  }
+}
+
+void kl_fill_csr(hipStream_t s, const DevLevel &L, double *val) {
+  // active rows come first in the CSR, and their entries are exactly the stencil entries
+  if (L.lat_nval > 0)
+    csr_fill_k<<<(int)std::min<int64_t>((L.lat_nval + kB - 1) / kB, 8192), kB, 0, s>>>(L.lat_nval, L.lsid, L.lst64,
+                                                                                        val);
 }
 
 template <int MODE>

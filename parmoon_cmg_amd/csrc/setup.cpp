@@ -211,8 +211,11 @@ void build_lattice(HostLevel &L, int order) {
   L.lat_nval = off;
   L.lat_perm.assign(off, -1);
   L.lat_node2dof.assign(first, -1);
+  L.lat_sid.assign(off, 0);
+  L.lat_first.assign(8 * 128, 0);
   std::atomic<int> missing{0};
-  for (const LatClass &C : L.lat) {
+  for (int c = 0; c < (int)L.lat.size(); c++) {
+    const LatClass &C = L.lat[c];
     parallel_for((int)C.nodes, [&](int k) {
       const int kx = k % C.cnt[0], ky = (k / C.cnt[0]) % C.cnt[1], kz = k / (C.cnt[0] * C.cnt[1]);
       const int X = C.lo[0] + C.step * kx, Y = C.lo[1] + C.step * ky, Z = C.lo[2] + C.step * kz;
@@ -231,8 +234,14 @@ void build_lattice(HostLevel &L, int order) {
           for (int dx = rng[0][0]; dx <= rng[0][1]; dx++, s++) {
             const int cdof = node_dof[(X + dx) + nn * ((Y + dy) + (int64_t)nn * (Z + dz))];
             const int *q = std::lower_bound(b, e, cdof);
-            if (q == e || *q != cdof) missing++;
-            L.lat_perm[C.off + (int64_t)s * C.nodes + k] = (int)(q - &L.kcol[0]);
+            if (q == e || *q != cdof) {
+              missing++;
+              continue;
+            }
+            const int pos = (int)(q - &L.kcol[0]);
+            L.lat_perm[C.off + (int64_t)s * C.nodes + k] = pos;
+            L.lat_sid[pos] = (uint16_t)(c * 128 + s);
+            if (k == 0) L.lat_first[c * 128 + s] = pos;
           }
     });
   }
