@@ -62,11 +62,28 @@ def kernel_table(P, reps=50):
     return out
 
 
+# HBM bytes per launch of the fine-level kernels from the rocprofv3 counter
+# runs (FETCH_SIZE x 2.0 + WRITE_SIZE, calibrated on a copy kernel)
+TRAFFIC_FILE = os.path.join(REPO, "profiles", "cd3d_q2_64_fine_level_traffic.json")
+TRAFFIC_KERNEL = {0: "csr_stream<0, double, double>", 1: "void lat_sweep_k<2, 0>", 2: "void lat64_k<2, 0>",
+                  3: "void lat_restrict_k<2>", 4: "void lat_prolong_k<2>", 5: "csr_fill_k", 6: "copy_k"}
+
+
+def measured_traffic(which):
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            k = json.load(fh)["kernels"].get(TRAFFIC_KERNEL[which])
+        return int(k["traffic_bytes"]) if k else None
+    except (OSError, ValueError, KeyError):
+        return None
+
+
 def roofline_entry(table, which, kernel_note):
     k = table[KERNEL_NAMES[which]]
     return dict(bound="hbm", achieved=k["achieved_gbs"], peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(k["achieved_gbs"] / HBM_PEAK_GBS, 4), traffic=None, kernel=KERNEL_NAMES[which],
-                note=kernel_note, us_per_launch=k["us_per_launch"], algorithmic_bytes=k["algorithmic_bytes"])
+                frac=round(k["achieved_gbs"] / HBM_PEAK_GBS, 4), traffic=measured_traffic(which),
+                kernel=KERNEL_NAMES[which], note=kernel_note, us_per_launch=k["us_per_launch"],
+                algorithmic_bytes=k["algorithmic_bytes"])
 
 
 def run_steps(pm, P, sp, co, warmup, steps):

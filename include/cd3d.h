@@ -182,7 +182,7 @@ cd3d_status cd3d_interp_pulse(cd3d_problem *p, int level, const cd3d_coeffs *co,
  * nonzero + 4 per node); 2 FP64 lattice operator (16 bytes per node);
  * 3 FP32 lattice restriction to level-1 (4 N_fine + 4 N_coarse);
  * 4 FP32 lattice prolongation + update from level-1 (8 N_fine + 4 N_coarse);
- * 5 CSR values from the class stencils (12 bytes per nonzero); 6 copy of an
+ * 5 CSR values from the class stencils (10 bytes per active-row nonzero); 6 copy of an
  * FP64 level vector (16 N: the traffic-counter calibration).
  * Needs an assembled problem; the level vectors are overwritten. */
 cd3d_status cd3d_kernel_bench(cd3d_problem *p, int which, int level, int reps, double *ms_per_launch,

@@ -1175,7 +1175,7 @@ cd3d_status cd3d_kernel_bench(cd3d_problem *p, int which, int level, int reps, d
       case 5:
         if (!L.lat_exact64) throw ArgError("the class stencils need constant-coefficient assembly");
         launch = [&] { kl_fill_csr(p->stream, L, L.val); };
-        b = 12 * nnz;
+        b = 10 * (double)L.lat_nval;  // 2-byte stencil id in, 8-byte value out, active rows
         break;
       case 6:
         launch = [&] { k_copy(p->stream, L.ndof, L.sol, L.aux); };

@@ -167,6 +167,17 @@ int quad_points_1d(int order);
 
 // ---------------------------------------------------------------- kernels (kernels.hip)
 constexpr int kRedBlocks = 1024;
+// CSR-stream row blocks: at most kStreamNnz nonzeros and kStreamRows rows,
+// processed by kStreamBlock threads
+#ifndef CD3D_STREAM_NNZ
+#define CD3D_STREAM_NNZ 2048
+#endif
+#ifndef CD3D_STREAM_BLOCK
+#define CD3D_STREAM_BLOCK 256
+#endif
+constexpr int kStreamNnz = CD3D_STREAM_NNZ;
+constexpr int kStreamBlock = CD3D_STREAM_BLOCK;
+constexpr int kStreamRows = 256;
 
 void k_spmv(hipStream_t s, const DevLevel &L, const double *x, double *y);
 void k_defect(hipStream_t s, const DevLevel &L, const double *x, const double *f, double *d);

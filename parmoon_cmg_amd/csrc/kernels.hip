@@ -131,15 +131,14 @@ void launch_rows(hipStream_t st, int64_t nnz, int n, int nact, const int *rp, co
 // out = f - A x; kJacobi out = x + w (f - A x) / a_ii on active rows, f on
 // Dirichlet rows (the damped Jacobi step of TMGLevel3D::Jacobi,
 // MGLevel3D.C:443-537, written with the full row sum).
-constexpr int kStreamNnz = 2048;
-constexpr int kStreamPer = kStreamNnz / kBlock;
+constexpr int kStreamPer = kStreamNnz / kStreamBlock;
 
 template <int G, int MODE, typename T>
 __device__ inline void stream_rows(int nr, int r0, int j0, int nact, const int *__restrict__ rowptr,
                                    const T *__restrict__ prod, const T *__restrict__ x, const T *__restrict__ f,
                                    const T *__restrict__ invdiag, T *__restrict__ out, T omega) {
   const int g = threadIdx.x / G, l = threadIdx.x % G;
-  for (int r = g; r < nr; r += kBlock / G) {
+  for (int r = g; r < nr; r += kStreamBlock / G) {
     const int row = r0 + r;
     const int a = rowptr[row] - j0, e = rowptr[row + 1] - j0;
     T s = T(0);
@@ -154,7 +153,7 @@ __device__ inline void stream_rows(int nr, int r0, int j0, int nact, const int *
 }
 
 template <int MODE, typename T, typename V>
-__global__ __launch_bounds__(kBlock) void csr_stream(const int *__restrict__ blk, int nact,
+__global__ __launch_bounds__(kStreamBlock) void csr_stream(const int *__restrict__ blk, int nact,
                                                      const int *__restrict__ rowptr, const int *__restrict__ col,
                                                      const V *__restrict__ val, const T *__restrict__ x,
                                                      const T *__restrict__ f, const T *__restrict__ invdiag,
@@ -165,15 +164,15 @@ __global__ __launch_bounds__(kBlock) void csr_stream(const int *__restrict__ blk
   const int nr = r1 - r0;
   if (nz > kStreamNnz) {
     // a single row longer than a block: strided partial sums and a block reduction
-    __shared__ T red[kBlock / 64];
+    __shared__ T red[kStreamBlock / 64];
     T s = T(0);
-    for (int t = threadIdx.x; t < nz; t += kBlock) s += (T)val[j0 + t] * x[col[j0 + t]];
+    for (int t = threadIdx.x; t < nz; t += kStreamBlock) s += (T)val[j0 + t] * x[col[j0 + t]];
     s = group_sum<64>(s);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
     if (threadIdx.x == 0) {
       T tot = T(0);
-      for (int k = 0; k < kBlock / 64; k++) tot += red[k];
+      for (int k = 0; k < kStreamBlock / 64; k++) tot += red[k];
       const int row = r0;
       if (MODE == kSpmv) out[row] = tot;
       else if (MODE == kDefect) out[row] = f[row] - tot;
@@ -185,13 +184,13 @@ __global__ __launch_bounds__(kBlock) void csr_stream(const int *__restrict__ blk
   T a[kStreamPer];
 #pragma unroll
   for (int k = 0; k < kStreamPer; k++) {
-    const int t = threadIdx.x + k * kBlock;
+    const int t = threadIdx.x + k * kStreamBlock;
     c[k] = t < nz ? col[j0 + t] : 0;
     a[k] = t < nz ? (T)val[j0 + t] : T(0);
   }
 #pragma unroll
   for (int k = 0; k < kStreamPer; k++) {
-    const int t = threadIdx.x + k * kBlock;
+    const int t = threadIdx.x + k * kStreamBlock;
     if (t < nz) prod[t] = a[k] * x[c[k]];
   }
   __syncthreads();
@@ -217,7 +216,8 @@ template <int MODE, typename T, typename V>
 void launch_stream(hipStream_t st, const DevLevel &L, const V *val, const T *x, const T *f, const T *invdiag, T *out,
                    T omega) {
   if (L.nblk <= 0) return;
-  csr_stream<MODE, T, V><<<L.nblk, kBlock, 0, st>>>(L.blk, L.nact, L.rowptr, L.col, val, x, f, invdiag, out, omega);
+  csr_stream<MODE, T, V><<<L.nblk, kStreamBlock, 0, st>>>(L.blk, L.nact, L.rowptr, L.col, val, x, f, invdiag, out,
+                                                          omega);
 }
 
 template <typename T>

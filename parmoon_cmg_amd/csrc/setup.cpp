@@ -175,7 +175,7 @@ void build_host_level(HostLevel &L, int order, int m, int depth) {
     L.diag[r] = (int)(std::lower_bound(out, out + t, r) - &L.kcol[0]);
   });
   build_colours(L, order);
-  build_row_blocks(L.rowptr, 2048, 256, L.blk);
+  build_row_blocks(L.rowptr, kStreamNnz, kStreamRows, L.blk);
   build_lattice(L, order);
 }
 
