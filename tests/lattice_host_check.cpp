@@ -1,0 +1,209 @@
+// Host check of the lattice form's index arithmetic (the __host__ __device__
+// node functions of lattice.hip) against the CSR matrices and transfer
+// operators that setup.cpp builds: class stencils against CSR rows, compact
+// against per-node stencils, restriction and prolongation against R and P, stencil ids against the lattice map, and the folding of one
+// element matrix into class stencils against cell-by-cell accumulation.
+// Built by tests/test_lattice_host.py with hipcc (host code only runs).
+#include "lattice.hip"
+#include <cstdio>
+#include <random>
+using namespace cd3d;
+
+static double rnd(std::mt19937 &g) { return std::uniform_real_distribution<double>(-1, 1)(g); }
+
+template <int P>
+int check(int m, int k0) {
+  HostLevel C, F;
+  build_host_level(C, P, m, k0);
+  build_host_level(F, P, m, k0 + 1);
+  build_transfer(F, C, P);
+  std::mt19937 g(7);
+  int bad = 0;
+  // ---- stencil: defect with f = 0 gives -A x on interior nodes
+  const int nn = F.nn;
+  const size_t nn3 = (size_t)nn * nn * nn;
+  std::vector<double> val(F.nnz);
+  for (auto &v : val) v = rnd(g);
+  std::vector<float> lval(F.lat_nval);
+  for (int64_t i = 0; i < F.lat_nval; i++) {
+    if (F.lat_perm[i] < 0 || F.lat_perm[i] >= F.nnz) { printf("perm out of range\n"); return 1; }
+    lval[i] = (float)val[F.lat_perm[i]];
+  }
+  std::vector<float> x(nn3, 0.f), f(nn3, 0.f), out(nn3, 0.f);
+  std::vector<double> xd(F.ndof, 0.0);
+  for (int d = 0; d < F.nact; d++) { xd[d] = rnd(g); x[F.dof_node[d]] = (float)xd[d]; }
+  DevLevel D;
+  D.nn = nn;
+  D.nclass = (int)F.lat.size();
+  for (int c = 0; c < D.nclass; c++) D.cls[c] = F.lat[c];
+  LatArgs a = lat_args(D);
+  // node2dof consistency
+  for (int c = 0; c < D.nclass; c++) {
+    const LatClass &L = F.lat[c];
+    for (int64_t k = 0; k < L.nodes; k++) {
+      int kx = k % L.cnt[0], ky = (k / L.cnt[0]) % L.cnt[1], kz = k / ((int64_t)L.cnt[0] * L.cnt[1]);
+      int X = L.lo[0] + L.step * kx, Y = L.lo[1] + L.step * ky, Z = L.lo[2] + L.step * kz;
+      int node = X + nn * (Y + nn * Z);
+      int dof = F.lat_node2dof[L.first + k];
+      if (dof < 0 || dof >= F.nact || F.dof_node[dof] != node) bad++;
+    }
+    for (int64_t k = 0; k < L.nodes; k++) lat_class_node<P, kLDefect>(a, c, k, lval.data(), nullptr, nullptr, x.data(), f.data(), out.data(), 0.f);
+  }
+  // stencil ids and first-node positions agree with the lattice map
+  for (int c = 0; c < D.nclass; c++) {
+    const LatClass &L = F.lat[c];
+    for (int s2 = 0; s2 < L.nsten; s2++) {
+      if (F.lat_first[c * 128 + s2] != F.lat_perm[L.off + (int64_t)s2 * L.nodes]) bad++;
+      for (int64_t k = 0; k < L.nodes; k++)
+        if (F.lat_sid[F.lat_perm[L.off + (int64_t)s2 * L.nodes + k]] != c * 128 + s2) bad++;
+    }
+  }
+  if ((int64_t)F.lat_sid.size() != F.rowptr[F.nact]) { printf("sid size\n"); bad++; }
+  int64_t nint = 0;
+  for (auto &L : F.lat) nint += L.nodes;
+  if (nint != F.nact) { printf("interior count %lld != nact %d\n", (long long)nint, F.nact); bad++; }
+  double emax = 0;
+  for (int d = 0; d < F.nact; d++) {
+    double y = 0;
+    for (int k = F.rowptr[d]; k < F.rowptr[d + 1]; k++) y += val[k] * xd[F.kcol[k]];
+    emax = std::max(emax, fabs(-y - out[F.dof_node[d]]));
+  }
+  // boundary entries untouched
+  for (int d = F.nact; d < F.ndof; d++) if (out[F.dof_node[d]] != 0.f) bad++;
+  printf("P%d m%d k0=%d nn=%d stencil max err %.3g  (nval %lld nnz %lld)\n", P, m, k0, nn, emax,
+         (long long)F.lat_nval, (long long)F.nnz);
+  if (emax > 1e-4) bad++;
+  // ---- compact stencil: values that depend only on (class, offset)
+  {
+    auto cls_of = [&](int X, int Y, int Z) { return P == 1 ? 0 : (X & 1) + 2 * (Y & 1) + 4 * (Z & 1); };
+    std::vector<double> uval(F.nnz, 0.0);
+    for (int d = 0; d < F.nact; d++) {
+      int nd = F.dof_node[d], X = nd % nn, Y = (nd / nn) % nn, Z = nd / (nn * nn);
+      for (int k = F.rowptr[d]; k < F.rowptr[d + 1]; k++) {
+        int nc = F.dof_node[F.kcol[k]], dx = nc % nn - X, dy = (nc / nn) % nn - Y, dz = nc / (nn * nn) - Z;
+        uval[k] = 1.0 + cls_of(X, Y, Z) * 0.37 + dx * 0.11 + dy * 0.013 + dz * 0.0017 + (dx == 0 && dy == 0 && dz == 0) * 5;
+      }
+    }
+    std::vector<float> cst(8 * kCst, 0.f);
+    for (int c = 0; c < D.nclass; c++) {
+      const LatClass &L = F.lat[c];
+      for (int s2 = 0; s2 < L.nsten; s2++) cst[c * kCst + s2] = (float)uval[F.lat_perm[L.off + (int64_t)s2 * L.nodes]];
+      a.uni[c] = 1;
+    }
+    std::fill(out.begin(), out.end(), 0.f);
+    for (int c = 0; c < D.nclass; c++)
+      for (int64_t k = 0; k < F.lat[c].nodes; k++)
+        lat_class_node<P, kLDefect>(a, c, k, nullptr, nullptr, cst.data(), x.data(), f.data(), out.data(), 0.f);
+    double e2 = 0;
+    for (int d = 0; d < F.nact; d++) {
+      double y = 0;
+      for (int k = F.rowptr[d]; k < F.rowptr[d + 1]; k++) y += uval[k] * xd[F.kcol[k]];
+      e2 = std::max(e2, fabs(-y - out[F.dof_node[d]]) / (1 + fabs(y)));
+    }
+    printf("   compact stencil max rel err %.3g\n", e2);
+    if (e2 > 1e-5) bad++;
+    for (int c = 0; c < D.nclass; c++) a.uni[c] = 0;
+  }
+  // ---- class stencils folded from one element matrix == cell-by-cell assembly
+  {
+    constexpr int P1 = P + 1, NB = P1 * P1 * P1;
+    std::vector<double> E(NB * NB + NB);
+    for (auto &v : E) v = rnd(g);
+    std::vector<double> aval(F.nnz, 0.0), arhs(F.ndof, 0.0);
+    std::vector<int> nd2dof((size_t)nn * nn * nn, -1);
+    for (int d = 0; d < F.ndof; d++) nd2dof[F.dof_node[d]] = d;
+    const int ncs = (nn - 1) / P;
+    for (int cz = 0; cz < ncs; cz++) for (int cy = 0; cy < ncs; cy++) for (int cx = 0; cx < ncs; cx++)
+      for (int li = 0; li < NB; li++) {
+        int ix = li % P1, iy = (li / P1) % P1, iz = li / (P1 * P1);
+        int r = nd2dof[(P * cx + ix) + nn * ((P * cy + iy) + nn * (P * cz + iz))];
+        if (r >= F.nact) continue;
+        arhs[r] += E[NB * NB + li];
+        for (int lj = 0; lj < NB; lj++) {
+          int jx = lj % P1, jy = (lj / P1) % P1, jz = lj / (P1 * P1);
+          int cc = nd2dof[(P * cx + jx) + nn * ((P * cy + jy) + nn * (P * cz + jz))];
+          const int *b = &F.kcol[F.rowptr[r]], *e = &F.kcol[F.rowptr[r + 1]];
+          const int *q = std::lower_bound(b, e, cc);
+          if (q == e || *q != cc) { printf("missing column\n"); bad++; continue; }
+          aval[q - &F.kcol[0]] += E[li * NB + lj];
+        }
+      }
+    std::vector<double> st(8 * kCst + 8, 0.0), fval(F.nnz, -99.0), frhs(F.ndof, -99.0);
+    for (int c = 0; c < D.nclass; c++) {
+      const LatClass &L = F.lat[c];
+      int nl[3], l[3][2], lo[3], len[3];
+      for (int d = 0; d < 3; d++) class_axis<P>(L.px[d], &nl[d], l[d], &lo[d], &len[d]);
+      if (len[0] * len[1] * len[2] != L.nsten) { printf("nsten mismatch\n"); bad++; }
+      for (int s2 = 0; s2 < L.nsten; s2++) {
+        const int dd[3] = {lo[0] + s2 % len[0], lo[1] + (s2 / len[0]) % len[1], lo[2] + s2 / (len[0] * len[1])};
+        st[c * kCst + s2] = stencil_entry<P>(E.data(), nl, l, dd);
+      }
+      double r = 0;
+      for (int az = 0; az < nl[2]; az++) for (int ay = 0; ay < nl[1]; ay++) for (int ax = 0; ax < nl[0]; ax++)
+        r += E[NB * NB + l[0][ax] + P1 * (l[1][ay] + P1 * l[2][az])];
+      st[8 * kCst + c] = r;
+      for (int s2 = 0; s2 < L.nsten; s2++)
+        for (int64_t k = 0; k < L.nodes; k++) fval[F.lat_perm[L.off + s2 * L.nodes + k]] = st[c * kCst + s2];
+      for (int64_t k = 0; k < L.nodes; k++) frhs[F.lat_node2dof[L.first + k]] = st[8 * kCst + c];
+    }
+    double e3 = 0;
+    for (int d = 0; d < F.nact; d++) {
+      for (int k = F.rowptr[d]; k < F.rowptr[d + 1]; k++) e3 = std::max(e3, fabs(fval[k] - aval[k]));
+      e3 = std::max(e3, fabs(frhs[d] - arhs[d]));
+    }
+    printf("   folded class stencils max err %.3g\n", e3);
+    if (e3 > 1e-12) bad++;
+  }
+  // ---- restriction: R = P^T, Dirichlet coarse rows zero
+  const int nc = C.nn;
+  std::vector<float> fl(nn3, 0.f), cl((size_t)nc * nc * nc, 0.f);
+  std::vector<double> fd(F.ndof, 0.0);
+  for (int d = 0; d < F.nact; d++) { fd[d] = rnd(g); fl[F.dof_node[d]] = (float)fd[d]; }
+  {
+    DevLevel DC;
+    DC.nn = C.nn;
+    DC.nclass = (int)C.lat.size();
+    for (int c = 0; c < DC.nclass; c++) DC.cls[c] = C.lat[c];
+    LatArgs ca = lat_args(DC);
+    for (int c = 0; c < DC.nclass; c++)
+      for (int64_t k = 0; k < C.lat[c].nodes; k++) restrict_class_node<P>(ca, c, k, nn, fl.data(), cl.data());
+  }
+  emax = 0;
+  for (int d = 0; d < C.ndof; d++) {
+    double y = 0;
+    for (int k = F.r_rowptr[d]; k < F.r_rowptr[d + 1]; k++) y += F.r_val[k] * fd[F.r_col[k]];
+    if (d >= C.nact) y = 0;
+    emax = std::max(emax, fabs(y - cl[C.dof_node[d]]));
+  }
+  printf("   restriction max err %.3g\n", emax);
+  if (emax > 1e-4) bad++;
+  // ---- prolongation: fine interior += P c (coarse Dirichlet zero)
+  std::vector<double> cd(C.ndof, 0.0);
+  std::fill(cl.begin(), cl.end(), 0.f);
+  for (int d = 0; d < C.nact; d++) { cd[d] = rnd(g); cl[C.dof_node[d]] = (float)cd[d]; }
+  std::vector<float> xl(nn3, 0.f);
+  const long long ncc = (nc - 1) / P;
+  for (long long t = 0; t < ncc * ncc * ncc; t++) prolong_cell<P>(t, (int)ncc, nn, nc, cl.data(), xl.data(), 1.f);
+  emax = 0;
+  for (int d = 0; d < F.ndof; d++) {
+    double y = 0;
+    for (int k = F.p_rowptr[d]; k < F.p_rowptr[d + 1]; k++) y += F.p_val[k] * cd[F.p_col[k]];
+    if (d >= F.nact) y = 0;
+    emax = std::max(emax, fabs(y - xl[F.dof_node[d]]));
+  }
+  printf("   prolongation max err %.3g\n", emax);
+  if (emax > 1e-4) bad++;
+  return bad;
+}
+
+int main() {
+  int bad = 0;
+  bad += check<1>(1, 1);
+  bad += check<1>(2, 1);
+  bad += check<2>(1, 1);
+  bad += check<2>(1, 2);
+  bad += check<2>(3, 0);
+  bad += check<2>(2, 1);
+  printf(bad ? "FAIL %d\n" : "OK\n", bad);
+  return bad != 0;
+}
