@@ -305,12 +305,28 @@ def main():
                     help="run one form only (profiling)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="assemble once and time the path's kernels (for rocprofv3 counter runs)")
+    ap.add_argument("--only", choices=["tcd", "batched", "serving"],
+                    help="run one of configurations 3-5 alone (profiling) and print its result")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every MI355X box here has one GPU)",
               file=sys.stderr)
         return 2
     import parmoon_cmg_amd as pm
+
+    if args.only:
+        forms = [f for f in (("fp64", pm.FP64), ("mixed", pm.MIXED)) if args.precision in ("both", f[0])]
+        out = {}
+        for name, prec in forms:
+            if args.only == "tcd":
+                out[name] = tcd_run(pm, prec, args.c3_steps)[0]
+            elif args.only == "batched":
+                out[name] = batched_uneven(pm, prec, args.c4_problems)
+            else:
+                out[name] = serving_stream(pm, prec, [float(x) for x in args.c5_rates.split(",") if x],
+                                           args.c5_seconds)
+        print(json.dumps(r5(out)))
+        return 0
 
     log("[bench] building the 64^3 Q2 hierarchy (5 levels)")
     t = time.perf_counter()

@@ -681,6 +681,82 @@ __global__ __launch_bounds__(kBlock) void gj_final_k(int na, const double *__res
   }
 }
 
+// Small coarse blocks (na <= kGjLdsMax): the whole in-place Gauss-Jordan
+// inversion in one workgroup with the matrix in LDS.  Partial pivoting with
+// row interchanges (the largest |a_ik| for i >= k, ties to the lower row);
+// the interchanges are undone as column interchanges in reverse order at the
+// end.  One launch instead of na + 2.
+constexpr int kGjLdsMax = 128;
+
+__global__ __launch_bounds__(1024) void gj_lds_k(int na, const int *__restrict__ rp, const int *__restrict__ col,
+                                                 const double *__restrict__ val, double *__restrict__ inv) {
+  extern __shared__ double A[];  // na x na, row-major
+  __shared__ double fcol[kGjLdsMax];
+  __shared__ int ipiv[kGjLdsMax];
+  __shared__ int pivrow;
+  const int t = threadIdx.x, nt = blockDim.x, nn = na * na;
+  for (int i = t; i < nn; i += nt) A[i] = 0.0;
+  __syncthreads();
+  for (int r = t; r < na; r += nt)
+    for (int q = rp[r]; q < rp[r + 1]; q++)
+      if (col[q] < na) A[r * na + col[q]] = val[q];
+  __syncthreads();
+  for (int k = 0; k < na; k++) {
+    if (t < 64) {  // pivot search by the first wavefront
+      double best = -1.0;
+      int bi = na;
+      for (int i = k + t; i < na; i += 64) {
+        const double v = fabs(A[i * na + k]);
+        if (v > best || (v == best && i < bi)) { best = v; bi = i; }
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        const double v = __shfl_xor(best, o);
+        const int r = __shfl_xor(bi, o);
+        if (v > best || (v == best && r < bi)) { best = v; bi = r; }
+      }
+      if (t == 0) {
+        pivrow = bi;
+        ipiv[k] = bi;
+      }
+    }
+    __syncthreads();
+    const int p = pivrow;
+    if (p != k)
+      for (int j = t; j < na; j += nt) {
+        const double a = A[k * na + j];
+        A[k * na + j] = A[p * na + j];
+        A[p * na + j] = a;
+      }
+    __syncthreads();
+    const double dinv = 1.0 / A[k * na + k];
+    for (int j = t; j < na; j += nt) {
+      if (j != k) A[k * na + j] *= dinv;
+      fcol[j] = A[j * na + k];  // column k before the elimination
+    }
+    __syncthreads();
+    if (t == 0) A[k * na + k] = dinv;
+    for (int e = t; e < nn; e += nt) {
+      const int i = e / na, j = e % na;
+      if (i == k) continue;
+      if (j == k) A[e] = -fcol[i] * dinv;
+      else A[e] -= fcol[i] * A[k * na + j];
+    }
+    __syncthreads();
+  }
+  for (int k = na - 1; k >= 0; k--) {
+    const int p = ipiv[k];
+    if (p != k)
+      for (int i = t; i < na; i += nt) {
+        const double a = A[i * na + k];
+        A[i * na + k] = A[i * na + p];
+        A[i * na + p] = a;
+      }
+    __syncthreads();
+  }
+  for (int i = t; i < nn; i += nt) inv[i] = A[i];
+}
+
 // x += sum_k coef[k] Z_k  (FGMRES update x += Z y, UpdateGmresIterate)
 __global__ void multi_axpy_k(int n, int m, const double *__restrict__ coef, const double *__restrict__ Z,
                              int64_t ldz, double *__restrict__ x) {
@@ -793,6 +869,18 @@ void k_invert_active(hipStream_t s, const DevLevel &L, double *scratch, double *
   // used[na], perm[na]  (see gj_scratch_doubles)
   const int na = L.nact;
   if (na <= 0) return;
+  if (na <= kGjLdsMax) {
+    static int lds_ok = -1;  // the larger-than-64 KiB LDS opt-in, once per process
+    const size_t bytes = sizeof(double) * na * na;
+    if (lds_ok < 0)
+      lds_ok = hipFuncSetAttribute((const void *)gj_lds_k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)(sizeof(double) * kGjLdsMax * kGjLdsMax)) == hipSuccess;
+    if (lds_ok || bytes <= 65536) {
+      gj_lds_k<<<1, 1024, bytes, s>>>(na, L.rowptr, L.col, L.val, inv);
+      return;
+    }
+    (void)hipGetLastError();
+  }
   const int nblk = std::min((na + kGjWaves - 1) / kGjWaves, 256);
   double *M = scratch, *pv0 = M + (int64_t)2 * na * na, *pv1 = pv0 + 256;
   int *pi0 = (int *)(pv1 + 256), *pi1 = pi0 + 256, *used = pi1 + 256, *perm = used + na;

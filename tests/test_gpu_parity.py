@@ -232,3 +232,19 @@ def test_mixed_vcycle_against_fp64(order, m, k0, levels, conv_kind):
     x64, st64 = P.solve(pm.solver_params(**sp, rel_tol=1e-10))
     assert st.rfinal <= 1.0001e-10 * st.r0
     assert np.linalg.norm(x - x64) / np.linalg.norm(x64) < 1e-9
+
+
+@pytest.mark.parametrize("order,m,k0", [(1, 1, 2), (2, 3, 0), (1, 1, 3), (2, 1, 1)])
+def test_coarse_inverse_solves_the_coarse_system(order, m, k0):
+    # one level: the preconditioner is the exact coarse solve (SolveExact);
+    # 27 and 125 unknowns take the single-workgroup LDS elimination, 343 the
+    # one-launch-per-pivot elimination
+    P = pm.Problem(order, m, k0, 1, box=(1.0, 0.8, 0.6))
+    P.assemble(pm.coeffs(eps=1e-3, b=(1.0, -0.4, 0.3), c=0.5, supg=1, f_val=1.0))
+    i = P.info(0)
+    r = np.random.default_rng(5).standard_normal(i["ndof"])
+    r[i["nact"]:] = 0.0
+    z = P.vcycle(pm.solver_params(coarse_smoother=17), r)
+    res = np.linalg.norm(P.matvec(0, z) - r) / np.linalg.norm(r)
+    say(f"[gpu] coarse inverse order={order} na={i['nact']}: |A z - r| / |r| = {res:.2e}")
+    assert res < 1e-12
