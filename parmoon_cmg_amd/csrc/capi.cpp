@@ -191,8 +191,8 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
     D.lcst = dalloc<float>(P, 8 * 128);
     D.lscale = dalloc<double>(P, 8);
     D.lflag = dalloc<int>(P, 8);
-    D.lelem = dalloc<double>(P, (size_t)P.nb * P.nb + P.nb);
-    D.lst64 = dalloc<double>(P, 8 * 128 + 8);
+    D.lelem = dalloc<double>(P, (size_t)3 * P.nb * P.nb + P.nb);
+    D.lst64 = dalloc<double>(P, kLst64);
     const size_t nn3 = (size_t)H.nn * H.nn * H.nn;
     for (float **v : {&D.lx, &D.lf, &D.la, &D.lt}) {
       *v = dalloc<float>(P, nn3);  // boundary entries stay zero for good
@@ -269,9 +269,10 @@ static void check_coeffs(const cd3d_coeffs &co) {
 
 // assemble level l: what 0 -> conv-diff into `val`, 1 -> mass into `val`;
 // rhs into D.rhs_asm (TAssembleMat3D::Reset + Assemble3D, AssembleMat3D.C:204,230).
-// Returns true when the constant-coefficient path ran (it leaves the FP32
-// compact stencils of `val` behind).
-static bool assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int what, double *val) {
+// Returns which path ran: 1 constant coefficients (it leaves the FP32 compact
+// stencils of `val` behind), 2 rotating field (affine class stencils in
+// lst64), 0 cell by cell.
+static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int what, double *val) {
   DevLevel &D = P.dl[l];
   AsmParams ap{};
   ap.what = what;
@@ -293,12 +294,18 @@ static bool assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int wh
   ap.delta_const = 0;
   if (co.supg && co.conv_kind == 0)
     ap.delta_const = sdfem_delta_host(hconv_host(D.h, co.b), co.eps, co.b, P.mesh.order);
-  if (co.conv_kind == 0 && co.f_kind == 0 && D.nclass > 0 && !uniform_assembly_off()) {
+  if ((co.conv_kind == 0 || what == 1) && co.f_kind == 0 && D.nclass > 0 && !uniform_assembly_off()) {
     // every active row is written: no clearing pass
     kl_assemble_uniform(P.stream, D, P.mesh.order, P.tab, ap, val, D.rhs_asm);
     k_dirichlet_rows(P.stream, D, val, D.rhs_asm);
     HIPCHK(hipGetLastError());
-    return true;
+    return 1;
+  }
+  if (co.conv_kind == 1 && co.f_kind == 0 && !co.supg && D.nclass > 0 && !uniform_assembly_off()) {
+    kl_assemble_affine(P.stream, D, P.mesh.order, P.tab, ap, val, D.rhs_asm);
+    k_dirichlet_rows(P.stream, D, val, D.rhs_asm);
+    HIPCHK(hipGetLastError());
+    return 2;
   }
   if (!D.slots) {  // slot maps only for the cell-by-cell path (nb^2 ints per cell)
     std::vector<int> slots;
@@ -313,7 +320,7 @@ static bool assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int wh
     k_assemble(P.stream, D, P.tab, ap, val, D.rhs_asm, D.colour_start[k], D.colour_start[k + 1] - D.colour_start[k]);
   k_dirichlet_rows(P.stream, D, val, D.rhs_asm);
   HIPCHK(hipGetLastError());
-  return false;
+  return 0;
 }
 
 static void refresh_coarse_and_f32(cd3d_problem &P) {
@@ -328,18 +335,30 @@ static void refresh_coarse_and_f32(cd3d_problem &P) {
   for (size_t l = 0; l < P.dl.size(); l++) {
     DevLevel &D = P.dl[l];
     k_invdiag(P.stream, D);
-    if (l == 0) D.lat_fresh = false;  // the coarsest level has no FP32 stencils
+    if (l == 0) D.lat_fresh = 0;  // the coarsest level has no FP32 stencils
     if (l > 0) {
-      bool all_uniform = false;
-      HIPCHK(kl_detect_uniform(P.stream, D, P.hflag, &all_uniform));
-      if (!all_uniform) {  // per-node FP32 stencils (4 bytes per nonzero), allocated on first need
+      auto per_node = [&](bool with_map) {  // per-node FP32 stencils (4 bytes per nonzero), on first need
         if (!D.lval) {
           D.lval = dalloc<float>(P, (size_t)D.lat_nval);
           D.linv = dalloc<float>(P, (size_t)D.lat_nint);
+        }
+        if (with_map && !D.lperm) {
           D.lperm = dalloc<int>(P, P.hl[l].lat_perm.size());
           upload(P, D.lperm, P.hl[l].lat_perm);
         }
-        HIPCHK(kl_convert(P.stream, D));
+      };
+      if (D.lat_fresh == 2) {  // straight from the affine class stencils
+        per_node(false);
+        HIPCHK(kl_fill_lattice_affine(P.stream, D, D.lat_alpha_m, D.lat_alpha));
+        D.lat_fresh = 0;
+        D.lat_exact64 = false;
+      } else {
+        bool all_uniform = false;
+        HIPCHK(kl_detect_uniform(P.stream, D, P.hflag, &all_uniform));
+        if (!all_uniform) {
+          per_node(true);
+          HIPCHK(kl_convert(P.stream, D));
+        }
       }
     }
   }
@@ -808,6 +827,8 @@ cd3d_status cd3d_assemble(cd3d_problem *p, const cd3d_coeffs *co) {
     for (int l = 0; l < (int)p->dl.size(); l++) {
       DevLevel &D = p->dl[l];
       D.lat_fresh = assemble_level(*p, l, *co, 0, D.val);
+      D.lat_alpha_m = 0.0;
+      D.lat_alpha = 1.0;
       // SystemCD3D.C:330: Dirichlet part of the rhs into the level solution
       k_copy_range(p->stream, D.nact, D.ndof, D.rhs_asm, D.sol);
     }
@@ -991,7 +1012,13 @@ static void tcd_init_impl(cd3d_problem &P, const cd3d_coeffs &co0, double t0, co
   cd3d_coeffs co = co0;
   co.t = t0;
   P.assembled = false;
-  for (int l = 0; l <= Lf; l++) assemble_level(P, l, co, 1, P.dl[l].val);
+  for (int l = 0; l <= Lf; l++) {
+    DevLevel &D = P.dl[l];
+    D.tcd_mass_uniform = assemble_level(P, l, co, 1, D.val) == 1;
+    if (D.tcd_mass_uniform)  // keep the mass class stencils beside the conv-diff ones
+      HIPCHK(hipMemcpyAsync(D.lst64 + 3 * 8 * 128 + 8, D.lst64, sizeof(double) * 8 * 128, hipMemcpyDeviceToDevice,
+                            P.stream));
+  }
   HIPCHK(hipMemcpyAsync(P.xs, u, sizeof(double) * N, hipMemcpyHostToDevice, P.stream));
   k_copy_range(P.stream, NA, N, F.rhs_asm, P.xs);
   k_copy(P.stream, N, F.rhs_asm, P.tcd_oldrhs);
@@ -1015,7 +1042,8 @@ static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_sol
   cd3d_coeffs co = co0;
   co.t = P.tcd_t + tau;
   // AssembleARhs at t_new (SystemTCD3D.C:204-226)
-  for (int l = 0; l <= Lf; l++) assemble_level(P, l, co, 0, P.dl[l].aconv);
+  std::vector<int> conv_path(Lf + 1);
+  for (int l = 0; l <= Lf; l++) conv_path[l] = assemble_level(P, l, co, 0, P.dl[l].aconv);
   k_copy_range(P.stream, NA, N, F.rhs_asm, P.xs);
   // AssembleSystMat (SystemTCD3D.C:228-294): B = tau th3 f_old + tau th4 f_new
   // + (M - tau th2 A) u_old on the active rows, system matrix M + tau th1 A
@@ -1034,6 +1062,15 @@ static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_sol
     k_matadd(P.stream, P.hl[l].rowptr[D.nact], D.val, D.aconv, l == Lf ? -gamma0 + tau * th1 : tau * th1);
   }
   k_copy(P.stream, N, F.rhs_asm, P.tcd_oldrhs);
+  for (int l = 0; l <= Lf; l++) {
+    // system matrix M + tau th1 A: the FP32 lattice values come straight from
+    // the class stencils when M is uniform and A affine
+    DevLevel &D = P.dl[l];
+    const bool affine = D.tcd_mass_uniform && conv_path[l] == 2;
+    D.lat_fresh = affine ? 2 : 0;
+    D.lat_alpha_m = 1.0;
+    D.lat_alpha = tau * th1;
+  }
   refresh_coarse_and_f32(P);
   // Solve (SystemTCD3D.C:317-376)
   double r0;

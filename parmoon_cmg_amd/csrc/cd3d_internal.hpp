@@ -114,11 +114,17 @@ struct DevLevel {
   float *lcst = nullptr;
   double *lscale = nullptr;
   int *lflag = nullptr;
-  // constant-coefficient assembly: element matrix (NB x NB + element rhs) and
-  // FP64 class stencils (8 x 128 + 8 class rhs)
+  // assembly through class stencils: element matrices (3 NB x NB + NB) and
+  // FP64 class stencils: [S0 8x128][class rhs 8][SX 8x128][SY 8x128]
+  // [mass stencils 8x128][centre entry per class, 8 ints] (kLst64 doubles)
   double *lelem = nullptr, *lst64 = nullptr;
-  bool lat_fresh = false;  // lcst already holds the stencils of `val`
+  // what the last assembly left for the FP32 lattice form of `val`:
+  // 0 nothing (detect on the device), 1 compact stencils in lcst,
+  // 2 affine stencils: lval = lat_alpha_m * mass + lat_alpha * (S0 + X SX + Y SY)
+  int lat_fresh = 0;
+  double lat_alpha_m = 0.0, lat_alpha = 1.0;
   bool lat_exact64 = false;  // lst64 holds exactly the CSR values of the interior rows
+  bool tcd_mass_uniform = false;  // TCD3D: the mass class stencils are in lst64 (mass slot)
 };
 
 struct Problem {
@@ -235,6 +241,14 @@ void kl_assemble_uniform(hipStream_t s, DevLevel &L, int order, const double *ta
                          double *val, double *rhs);
 // the interior-row CSR values from the FP64 class stencils lst64
 void kl_fill_csr(hipStream_t s, const DevLevel &L, double *val);
+// rotating field (affine in space), Galerkin, constant f: affine class
+// stencils S0 + X SX + Y SY, CSR values of the active rows and the rhs
+void kl_assemble_affine(hipStream_t s, DevLevel &L, int order, const double *tab, const AsmParams &ap,
+                        double *val, double *rhs);
+// FP32 per-node lattice values and 1/a_ii from the affine stencils
+// (alpha_m times the mass class stencils plus alpha times the affine ones)
+hipError_t kl_fill_lattice_affine(hipStream_t s, DevLevel &L, double alpha_m, double alpha);
+constexpr int kLst64 = 4112;
 // FP64 operator in lattice order (class stencils lst64, identity rows on the
 // boundary) and the order / precision changes of the lattice FGMRES
 void kl_spmv64(hipStream_t s, const DevLevel &L, int order, const double *x, double *y);

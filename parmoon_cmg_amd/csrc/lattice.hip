@@ -446,6 +446,162 @@ __global__ void rhs_fill_k(LatArgs a, long long n, const int *__restrict__ node2
 }
 
 // ------------------------------------------------------------------------
+// Assembly with the rotating field b = (-(y - r1), x - r0, b3), Galerkin
+// (TCD3D's MatrixARhsAssemble, TCD3D.C:242-295; BilinearAssemble,
+// ConvDiff3D.C:61).  b is affine in space and the quadrature integrates the
+// element integrand exactly, so a cell with centre (xc, yc) has the matrix
+//   A_const + (-(yc - r1)) Dx + (xc - r0) Dy,
+//   A_const = eps K + c M + b3 Dz - int (y - yc) d_x phi_j phi_i + int (x - xc) d_y phi_j phi_i,
+//   Dx = int d_x phi_j phi_i,  Dy = int d_y phi_j phi_i,
+// and the stencil of an interior node (X, Y) is S0 + X SX + Y SY per parity
+// class.  The rhs (constant f) is one value per class.
+
+// E: [A_const NB x NB][rhs NB][Dx NB x NB][Dy NB x NB]; one thread per entry
+template <int P>
+__global__ __launch_bounds__(1024) void elem_affine_k(const double *__restrict__ tab, AsmParams ap,
+                                                      double *__restrict__ E) {
+  constexpr int NB = (P + 1) * (P + 1) * (P + 1), N1 = P + 2, NQ = N1 * N1 * N1, STRIDE = 4 + 4 * NB;
+  const int t = threadIdx.x;
+  if (t >= NB * NB) return;
+  const int i = t / NB, j = t % NB;
+  const double detj = (ap.h[0] / 2) * (ap.h[1] / 2) * (ap.h[2] / 2);
+  double kap[3], g2[3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    kap[a] = ap.eps * 4.0 / (ap.h[a] * ap.h[a]);
+    g2[a] = 2.0 / ap.h[a];
+  }
+  const double b3 = 0.1 * ap.sin2pit;
+  double a0 = 0.0, ax = 0.0, ay = 0.0, racc = 0.0;
+  for (int q = 0; q < NQ; q++) {
+    const double *T = tab + q * STRIDE;
+    const double w = T[0] * detj;
+    const double dxq = T[1] * ap.h[0] / 2, dyq = T[2] * ap.h[1] / 2;  // qp - cell centre
+    const double pi_ = T[4 + 4 * i], gi0 = T[5 + 4 * i], gi1 = T[6 + 4 * i], gi2 = T[7 + 4 * i];
+    const double pj = T[4 + 4 * j], gj0 = T[5 + 4 * j] * g2[0], gj1 = T[6 + 4 * j] * g2[1], gj2 = T[7 + 4 * j] * g2[2];
+    const double v = kap[0] * gi0 * T[5 + 4 * j] + kap[1] * gi1 * T[6 + 4 * j] + kap[2] * gi2 * T[7 + 4 * j] +
+                     pi_ * (ap.c * pj + b3 * gj2 - dyq * gj0 + dxq * gj1);
+    a0 += w * v;
+    ax += w * gj0 * pi_;
+    ay += w * gj1 * pi_;
+    racc += w * ap.f_val * pi_;
+  }
+  E[i * NB + j] = a0;
+  E[NB * NB + NB + i * NB + j] = ax;
+  E[2 * NB * NB + NB + i * NB + j] = ay;
+  if (j == 0) E[NB * NB + i] = racc;
+}
+
+// one stencil entry (offset d) of a node of the class: the sum over the
+// cells around the node of A_const + beta1 Dx + beta2 Dy with beta at the
+// cell centre, split into the parts constant, linear in X and linear in Y
+template <int P>
+__host__ __device__ inline void affine_stencil_entry(const double *__restrict__ E, const double *__restrict__ Dx,
+                                                     const double *__restrict__ Dy, const int (&nl)[3],
+                                                     const int (&l)[3][2], const int (&d)[3], double dnx, double dny,
+                                                     double r0, double r1, double *s0, double *sx, double *sy) {
+  constexpr int P1 = P + 1, NB = P1 * P1 * P1;
+  double a = 0.0, bx = 0.0, by = 0.0;
+  for (int az = 0; az < nl[2]; az++)
+    for (int ay = 0; ay < nl[1]; ay++)
+      for (int ax = 0; ax < nl[0]; ax++) {
+        const int lx = l[0][ax], ly = l[1][ay], lz = l[2][az];
+        const int mx = lx + d[0], my = ly + d[1], mz = lz + d[2];
+        if (mx < 0 || mx > P || my < 0 || my > P || mz < 0 || mz > P) continue;
+        const int e = (lx + P1 * (ly + P1 * lz)) * NB + (mx + P1 * (my + P1 * mz));
+        // cell centre = node + (P/2 - l) node spacings per axis
+        a += E[e] + (r1 - dny * (0.5 * P - ly)) * Dx[e] + (dnx * (0.5 * P - lx) - r0) * Dy[e];
+        bx += dnx * Dy[e];
+        by -= dny * Dx[e];
+      }
+  *s0 = a;
+  *sx = bx;
+  *sy = by;
+}
+
+// st: [S0 8 x 128][class rhs 8][SX 8 x 128][SY 8 x 128] (node units: X, Y
+// are lattice coordinates, dn the node spacing)
+template <int P>
+__global__ __launch_bounds__(kCst) void class_stencil_affine_k(LatArgs a, const double *__restrict__ E, double dnx,
+                                                               double dny, double r0, double r1,
+                                                               double *__restrict__ st) {
+  constexpr int P1 = P + 1, NB = P1 * P1 * P1;
+  const double *Dx = E + NB * NB + NB, *Dy = E + 2 * NB * NB + NB;
+  const int c = blockIdx.x, s = threadIdx.x;
+  const LatDev &C = a.c[c];
+  int nl[3], l[3][2], lo[3], len[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) class_axis<P>(C.px[d], &nl[d], l[d], &lo[d], &len[d]);
+  if (s < C.nsten) {
+    const int d[3] = {lo[0] + s % len[0], lo[1] + (s / len[0]) % len[1], lo[2] + s / (len[0] * len[1])};
+    double s0, sx, sy;
+    affine_stencil_entry<P>(E, Dx, Dy, nl, l, d, dnx, dny, r0, r1, &s0, &sx, &sy);
+    st[c * kCst + s] = s0;
+    st[8 * kCst + 8 + c * kCst + s] = sx;
+    st[2 * 8 * kCst + 8 + c * kCst + s] = sy;
+  }
+  if (s == kCst - 1) {
+    double r = 0.0;
+    for (int az = 0; az < nl[2]; az++)
+      for (int ay = 0; ay < nl[1]; ay++)
+        for (int ax = 0; ax < nl[0]; ax++) r += E[NB * NB + l[0][ax] + P1 * (l[1][ay] + P1 * l[2][az])];
+    st[8 * kCst + c] = r;
+  }
+}
+
+__host__ __device__ inline double affine_entry(const double *__restrict__ st, int id, int X, int Y) {
+  return st[id] + X * st[8 * kCst + 8 + id] + Y * st[2 * 8 * kCst + 8 + id];
+}
+
+// CSR values of the active rows, one row block (at most kStreamNnz entries)
+// per workgroup: entries in CSR order, each finds its row in the block's row
+// pointers (LDS)
+__global__ __launch_bounds__(kB) void csr_fill_affine_k(const int *__restrict__ blk, int nact,
+                                                        const int *__restrict__ rowptr,
+                                                        const int *__restrict__ dof2node, int nn,
+                                                        const uint16_t *__restrict__ sid,
+                                                        const double *__restrict__ st, double *__restrict__ val) {
+  __shared__ int rp[kStreamRows + 1];
+  const int r0 = blk[blockIdx.x], r1 = min(blk[blockIdx.x + 1], nact);
+  if (r1 <= r0) return;
+  const int nr = r1 - r0;
+  for (int i = threadIdx.x; i <= nr; i += kB) rp[i] = rowptr[r0 + i];
+  __syncthreads();
+  const int j0 = rp[0], nz = rp[nr] - j0;
+  for (int t = threadIdx.x; t < nz; t += kB) {
+    const int k = j0 + t;
+    int lo = 0, hi = nr - 1;  // last row with rp <= k
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (rp[mid] <= k) lo = mid;
+      else hi = mid - 1;
+    }
+    const int node = dof2node[r0 + lo];
+    val[k] = affine_entry(st, sid[k], node % nn, (node / nn) % nn);
+  }
+}
+
+// FP32 per-node lattice values and 1/a_ii straight from the affine stencils:
+// alpha_m M + alpha (S0 + X SX + Y SY), M the mass class stencils in stm
+// (alpha_m = 0: the conv-diff matrix alone)
+__global__ void lat_fill_affine_k(LatArgs a, long long n, const double *__restrict__ st,
+                                  const double *__restrict__ stm, double alpha_m, double alpha,
+                                  const int *__restrict__ centre, float *__restrict__ lval,
+                                  float *__restrict__ linv) {
+  for (long long i = (long long)blockIdx.x * kB + threadIdx.x; i < n; i += (long long)gridDim.x * kB) {
+    int c = 0;
+    while (c + 1 < a.nclass && i >= a.c[c + 1].off) c++;
+    const LatDev &C = a.c[c];
+    const long long r = i - C.off, s = r / C.nodes, k = r % C.nodes;
+    const int X = C.lo[0] + C.step * (int)(k % C.cnt[0]), Y = C.lo[1] + C.step * (int)((k / C.cnt[0]) % C.cnt[1]);
+    const int id = c * kCst + (int)s;
+    const double v = alpha * affine_entry(st, id, X, Y) + (alpha_m != 0.0 ? alpha_m * stm[id] : 0.0);
+    lval[i] = (float)v;
+    if (s == centre[c]) linv[C.first + k] = (float)(1.0 / v);
+  }
+}
+
+// ------------------------------------------------------------------------
 // FP64 operator in lattice order for the outer FGMRES of the mixed form,
 // when the class stencils are exactly the assembled CSR values: interior
 // rows apply the class stencil, boundary (Dirichlet) rows are identity rows.
@@ -601,8 +757,8 @@ void kl_prolong_update(hipStream_t s, const DevLevel &F, const DevLevel &C, int 
 hipError_t kl_detect_uniform(hipStream_t s, DevLevel &L, int *flag_host, bool *all_uniform) {
   *all_uniform = true;
   if (L.nclass <= 0) return hipSuccess;
-  if (L.lat_fresh) {  // kl_assemble_uniform wrote the compact stencils of this matrix
-    L.lat_fresh = false;
+  if (L.lat_fresh == 1) {  // kl_assemble_uniform wrote the compact stencils of this matrix
+    L.lat_fresh = 0;
     L.lat_exact64 = true;
     for (int c = 0; c < L.nclass; c++) L.uni[c] = 1;
     return hipSuccess;
@@ -644,6 +800,47 @@ void kl_assemble_uniform(hipStream_t s, DevLevel &L, int order, const double *ta
   kl_fill_csr(s, L, val);
   if (L.lat_nint > 0)
     rhs_fill_k<<<(int)((L.lat_nint + kB - 1) / kB), kB, 0, s>>>(a, L.lat_nint, L.lnode2dof, L.lst64, rhs);
+}
+
+void kl_assemble_affine(hipStream_t s, DevLevel &L, int order, const double *tab, const AsmParams &ap,
+                        double *val, double *rhs) {
+  const LatArgs a = lat_args(L);
+  if (order == 1) {
+    elem_affine_k<1><<<1, 64, 0, s>>>(tab, ap, L.lelem);
+    class_stencil_affine_k<1><<<L.nclass, kCst, 0, s>>>(a, L.lelem, ap.dn[0], ap.dn[1], ap.rot_c[0], ap.rot_c[1],
+                                                        L.lst64);
+  } else {
+    elem_affine_k<2><<<1, 768, 0, s>>>(tab, ap, L.lelem);
+    class_stencil_affine_k<2><<<L.nclass, kCst, 0, s>>>(a, L.lelem, ap.dn[0], ap.dn[1], ap.rot_c[0], ap.rot_c[1],
+                                                        L.lst64);
+  }
+  if (L.nblk > 0)
+    csr_fill_affine_k<<<L.nblk, kB, 0, s>>>(L.blk, L.nact, L.rowptr, L.ldof2node, L.nn, L.lsid, L.lst64, val);
+  if (L.lat_nint > 0)
+    rhs_fill_k<<<(int)((L.lat_nint + kB - 1) / kB), kB, 0, s>>>(a, L.lat_nint, L.lnode2dof, L.lst64, rhs);
+}
+
+hipError_t kl_fill_lattice_affine(hipStream_t s, DevLevel &L, double alpha_m, double alpha) {
+  if (L.nclass <= 0 || L.lat_nval <= 0) return hipSuccess;
+  int centre[8];
+  for (int c = 0; c < L.nclass; c++) {  // stencil index of offset (0, 0, 0)
+    const LatClass &C = L.cls[c];
+    int lo[3], len[3];
+    for (int d = 0; d < 3; d++) {
+      const int p = C.step == 1 ? 1 : 2;  // Q1 classes step 1, Q2 classes step 2
+      const bool mid = p == 2 && C.px[d] == 1;
+      lo[d] = mid ? -1 : -p;
+      len[d] = mid ? 3 : 2 * p + 1;
+    }
+    centre[c] = -lo[0] + len[0] * (-lo[1] + len[1] * -lo[2]);
+  }
+  int *dcentre = (int *)(L.lst64 + 3 * 8 * kCst + 8 + 8 * kCst);  // spare tail of the stencil buffer
+  hipError_t e = hipMemcpyAsync(dcentre, centre, sizeof(int) * 8, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return e;
+  for (int c = 0; c < L.nclass; c++) L.uni[c] = 0;
+  lat_fill_affine_k<<<2048, kB, 0, s>>>(lat_args(L), L.lat_nval, L.lst64, L.lst64 + 3 * 8 * kCst + 8, alpha_m, alpha,
+                                        dcentre, L.lval, L.linv);
+  return hipGetLastError();
 }
 
 void kl_fill_csr(hipStream_t s, const DevLevel &L, double *val) {

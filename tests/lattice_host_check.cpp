@@ -154,6 +154,58 @@ int check(int m, int k0) {
     printf("   folded class stencils max err %.3g\n", e3);
     if (e3 > 1e-12) bad++;
   }
+  // ---- affine class stencils == cell-by-cell accumulation with coefficients
+  //      at the cell centres: A_cell = E0 + beta1(cell) Dx + beta2(cell) Dy
+  {
+    constexpr int P1 = P + 1, NB = P1 * P1 * P1;
+    std::vector<double> E0(NB * NB), Dx(NB * NB), Dy(NB * NB);
+    for (auto *v : {&E0, &Dx, &Dy})
+      for (auto &e : *v) e = rnd(g);
+    const double dnx = 0.37, dny = 0.29, r0 = 0.41, r1 = 0.53;
+    std::vector<double> aval(F.nnz, 0.0);
+    std::vector<int> nd2dof((size_t)nn * nn * nn, -1);
+    for (int d = 0; d < F.ndof; d++) nd2dof[F.dof_node[d]] = d;
+    const int ncs = (nn - 1) / P;
+    for (int cz = 0; cz < ncs; cz++) for (int cy = 0; cy < ncs; cy++) for (int cx = 0; cx < ncs; cx++) {
+      const double xc = dnx * (P * cx + 0.5 * P), yc = dny * (P * cy + 0.5 * P);
+      const double b1 = -(yc - r1), b2 = xc - r0;
+      for (int li = 0; li < NB; li++) {
+        int ix = li % P1, iy = (li / P1) % P1, iz = li / (P1 * P1);
+        int r = nd2dof[(P * cx + ix) + nn * ((P * cy + iy) + nn * (P * cz + iz))];
+        if (r >= F.nact) continue;
+        for (int lj = 0; lj < NB; lj++) {
+          int jx = lj % P1, jy = (lj / P1) % P1, jz = lj / (P1 * P1);
+          int cc = nd2dof[(P * cx + jx) + nn * ((P * cy + jy) + nn * (P * cz + jz))];
+          const int *b = &F.kcol[F.rowptr[r]], *e = &F.kcol[F.rowptr[r + 1]];
+          const int *q = std::lower_bound(b, e, cc);
+          const int ent = li * NB + lj;
+          aval[q - &F.kcol[0]] += E0[ent] + b1 * Dx[ent] + b2 * Dy[ent];
+        }
+      }
+    }
+    std::vector<double> st(3 * 8 * kCst + 8, 0.0);
+    for (int c = 0; c < D.nclass; c++) {
+      const LatClass &L = F.lat[c];
+      int nl[3], l[3][2], lo[3], len[3];
+      for (int d = 0; d < 3; d++) class_axis<P>(L.px[d], &nl[d], l[d], &lo[d], &len[d]);
+      for (int s2 = 0; s2 < L.nsten; s2++) {
+        const int dd[3] = {lo[0] + s2 % len[0], lo[1] + (s2 / len[0]) % len[1], lo[2] + s2 / (len[0] * len[1])};
+        affine_stencil_entry<P>(E0.data(), Dx.data(), Dy.data(), nl, l, dd, dnx, dny, r0, r1, &st[c * kCst + s2],
+                                &st[8 * kCst + 8 + c * kCst + s2], &st[2 * 8 * kCst + 8 + c * kCst + s2]);
+      }
+    }
+    double e6 = 0, scale = 0;
+    for (int d = 0; d < F.nact; d++) {
+      const int node = F.dof_node[d];
+      for (int k = F.rowptr[d]; k < F.rowptr[d + 1]; k++) {
+        const double v = affine_entry(st.data(), F.lat_sid[k], node % nn, (node / nn) % nn);
+        e6 = std::max(e6, fabs(v - aval[k]));
+        scale = std::max(scale, fabs(aval[k]));
+      }
+    }
+    printf("   affine class stencils max err %.3g (scale %.3g)\n", e6, scale);
+    if (e6 > 1e-12 * scale) bad++;
+  }
   // ---- restriction: R = P^T, Dirichlet coarse rows zero
   const int nc = C.nn;
   std::vector<float> fl(nn3, 0.f), cl((size_t)nc * nc * nc, 0.f);

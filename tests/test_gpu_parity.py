@@ -248,3 +248,22 @@ def test_coarse_inverse_solves_the_coarse_system(order, m, k0):
     res = np.linalg.norm(P.matvec(0, z) - r) / np.linalg.norm(r)
     say(f"[gpu] coarse inverse order={order} na={i['nact']}: |A z - r| / |r| = {res:.2e}")
     assert res < 1e-12
+
+
+@pytest.mark.parametrize("order,m,k0", [(1, 1, 2), (2, 1, 1), (1, 2, 1), (2, 3, 0)])
+def test_rotating_field_assembly_matches_oracle(order, m, k0):
+    # rotating convection, Galerkin: assembled through affine class stencils
+    # on the device, cell by cell in the oracle (TCD3D.C:242-295)
+    co = dict(eps=1e-2, conv_kind=1, rot_c=(0.45, 0.55), c=0.3, f_val=2.0, t=0.3)
+    P = pm.Problem(order, m, k0, 2, box=(1.0, 0.8, 0.6))
+    P.assemble(pm.coeffs(**co))
+    H = po.Hierarchy(order, m, k0, 2, box=(1.0, 0.8, 0.6))
+    H.assemble(po.coeffs(**co))
+    for l in range(2):
+        rp, kc, e = P.csr(l)
+        rpo, kco, eo = H.csr(l)
+        assert np.array_equal(rp, rpo) and np.array_equal(kc, kco)
+        err = row_scaled_err(e, eo, rp)
+        rerr = np.abs(P.rhs(l) - H.rhs(l)).max() / np.abs(H.rhs(l)).max()
+        say(f"[gpu] rotating field order={order} m={m} level {l}: entry err {err:.2e}, rhs err {rerr:.2e}")
+        assert err < ENTRY_TOL and rerr < ENTRY_TOL
