@@ -112,6 +112,9 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
   P.nb = (m.order + 1) * (m.order + 1) * (m.order + 1);
   HIPCHK(hipGetDevice(&P.device));
   HIPCHK(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&P.side, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&P.ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&P.ev_join, hipEventDisableTiming));
   const int L = m.levels;
   P.hl.resize(L);
   P.dl.resize(L);
@@ -323,15 +326,41 @@ static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int wha
   return 0;
 }
 
-static void refresh_coarse_and_f32(cd3d_problem &P) {
+// explicit inverse of the coarsest matrix on the side stream.  fork_coarse
+// marks the point of the main stream where the level-0 matrix is final;
+// launch_coarse_inverse enqueues the inversion after that point (called once
+// the main stream has other work queued, so that it runs beside that work).
+static void fork_coarse(cd3d_problem &P) {
+  HIPCHK(hipEventRecord(P.ev_fork, P.stream));
+  P.fork_recorded = true;
+}
+
+static void launch_coarse_inverse(cd3d_problem &P) {
   DevLevel &C = P.dl[0];
   const int na = C.nact;
+  if (!P.fork_recorded) fork_coarse(P);
+  P.fork_recorded = false;
   if (na > 0) {
     if (!P.gj_scratch) P.gj_scratch = dalloc<double>(P, (size_t)gj_scratch_doubles(na));
-    k_invert_active(P.stream, C, P.gj_scratch, P.cinv);
+    HIPCHK(hipStreamWaitEvent(P.side, P.ev_fork, 0));
+    k_invert_active(P.side, C, P.gj_scratch, P.cinv);
     HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(P.ev_join, P.side));
+    P.coarse_pending = true;
   }
   P.cinv_valid = true;
+}
+
+// the main stream waits for the coarse inverse (before it is used, before
+// the level-0 matrix changes again, and at the end of every call)
+static void coarse_join(cd3d_problem &P) {
+  if (!P.coarse_pending) return;
+  HIPCHK(hipStreamWaitEvent(P.stream, P.ev_join, 0));
+  P.coarse_pending = false;
+}
+
+// 1/a_ii and the FP32 lattice values of every level
+static void refresh_f32(cd3d_problem &P) {
   for (size_t l = 0; l < P.dl.size(); l++) {
     DevLevel &D = P.dl[l];
     k_invdiag(P.stream, D);
@@ -379,9 +408,15 @@ struct VC {
   }
 };
 
+static void refresh_coarse_and_f32(cd3d_problem &P) {
+  launch_coarse_inverse(P);
+  refresh_f32(P);
+}
+
 // coarse level: x starts at zero (zero_start) or holds an iterate
 void coarse64(VC &v, bool zero_start) {
   DevLevel &C = v.P.dl[0];
+  coarse_join(v.P);
   if (zero_start) HIPCHK(hipMemsetAsync(C.sol, 0, sizeof(double) * C.ndof, v.P.stream));
   if (v.sp.coarse_smoother == 17) {
     // SolveExact once: x_A += A_AA^-1 (f - A x)_A, x_D = f_D; the loop of
@@ -643,7 +678,10 @@ void check_params(const cd3d_solver_params &sp) {
 cd3d_status run(cd3d_problem *p, const std::function<void()> &f) {
   if (!p) return CD3D_ERR_ARG;
   try {
+    p->fork_recorded = false;  // a fork point never outlives its call
+    coarse_join(*p);           // left pending by a call that failed
     f();
+    coarse_join(*p);
     return CD3D_OK;
   } catch (const ArgError &e) {
     p->err = e.what();
@@ -714,7 +752,7 @@ size_t cd3d_estimate_bytes(const cd3d_mesh_desc *mesh, int restart, int cellwise
     }
     if (tcd) b += 8 * nnz;                          // conv-diff matrix beside the mass matrix
     total += b + (2 << 20);                         // allocation granularity of ~20 buffers
-    if (l == 0) total += 8.0 * na * na + 8.0 * (2 * na * na + 1024);  // coarse inverse + Gauss-Jordan scratch
+    if (l == 0) total += 8.0 * na * na + 8.0 * (double)gj_scratch_doubles(na);  // coarse inverse + Gauss-Jordan scratch
     prevN = N;
     N_fine = N;
   }
@@ -742,9 +780,13 @@ cd3d_status cd3d_create(const cd3d_mesh_desc *mesh, cd3d_problem **out) {
 cd3d_status cd3d_destroy(cd3d_problem *p) {
   if (!p) return CD3D_ERR_ARG;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
+  if (p->side) (void)hipStreamSynchronize(p->side);
   for (void *a : p->allocs) (void)hipFree(a);
   if (p->hscal) (void)hipHostFree(p->hscal);
   if (p->hflag) (void)hipHostFree(p->hflag);
+  if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+  if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+  if (p->side) (void)hipStreamDestroy(p->side);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
   return CD3D_OK;
@@ -831,8 +873,10 @@ cd3d_status cd3d_assemble(cd3d_problem *p, const cd3d_coeffs *co) {
       D.lat_alpha = 1.0;
       // SystemCD3D.C:330: Dirichlet part of the rhs into the level solution
       k_copy_range(p->stream, D.nact, D.ndof, D.rhs_asm, D.sol);
+      if (l == 0) fork_coarse(*p);
     }
-    refresh_coarse_and_f32(*p);
+    launch_coarse_inverse(*p);  // beside the finer levels' assembly
+    refresh_f32(*p);
     HIPCHK(hipStreamSynchronize(p->stream));
     p->assembled = true;
     p->tcd_ready = false;  // the level matrices no longer hold the mass matrices
@@ -1060,7 +1104,9 @@ static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_sol
   for (int l = 0; l <= Lf; l++) {
     DevLevel &D = P.dl[l];
     k_matadd(P.stream, P.hl[l].rowptr[D.nact], D.val, D.aconv, l == Lf ? -gamma0 + tau * th1 : tau * th1);
+    if (l == 0) fork_coarse(P);
   }
+  launch_coarse_inverse(P);  // beside the finer levels' updates
   k_copy(P.stream, N, F.rhs_asm, P.tcd_oldrhs);
   for (int l = 0; l <= Lf; l++) {
     // system matrix M + tau th1 A: the FP32 lattice values come straight from
@@ -1071,11 +1117,12 @@ static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_sol
     D.lat_alpha_m = 1.0;
     D.lat_alpha = tau * th1;
   }
-  refresh_coarse_and_f32(P);
+  refresh_f32(P);
   // Solve (SystemTCD3D.C:317-376)
   double r0;
   const int its = fgmres(P, sp, dof_space(P), r0, solve_status);
   // RestoreMassMat (SystemTCD3D.C:296-315)
+  coarse_join(P);
   for (int l = 0; l <= Lf; l++) {
     DevLevel &D = P.dl[l];
     k_matadd(P.stream, P.hl[l].rowptr[D.nact], D.val, D.aconv, -tau * th1);

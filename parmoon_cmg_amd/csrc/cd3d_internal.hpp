@@ -133,6 +133,12 @@ struct Problem {
   std::vector<HostLevel> hl;
   std::vector<DevLevel> dl;
   hipStream_t stream = nullptr;
+  // the coarse inverse runs on a stream of its own, beside the assembly of
+  // the finer levels; the main stream waits for it before the first coarse
+  // solve and at the end of every call (coarse_join)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  bool coarse_pending = false, fork_recorded = false;
   int device = 0;
   size_t dev_bytes = 0;
   std::vector<void *> allocs;

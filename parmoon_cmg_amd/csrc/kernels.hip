@@ -581,6 +581,16 @@ __global__ __launch_bounds__(kBlock) void assemble_k(int ncells, const int *__re
 // M is na x 2na ([A_AA | I]); one wavefront per row, lanes over columns.
 constexpr int kGjWaves = kBlock / 64;
 
+__device__ inline void gj_better(double &bv, int &bi, double v, int r) {
+  if (v > bv || (v == bv && r < bi)) { bv = v; bi = r; }
+}
+// |x| as a pivot score; a NaN scores below every number, so that some unused
+// row is always chosen and the pivot index stays in range
+__device__ inline double gj_score(double x) {
+  const double a = fabs(x);
+  return a >= 0.0 ? a : -0.5;
+}
+
 __global__ __launch_bounds__(kBlock) void gj_init_k(int na, const int *__restrict__ rp, const int *__restrict__ col,
                                                     const double *__restrict__ val, double *__restrict__ M,
                                                     int *__restrict__ used, double *__restrict__ pv,
@@ -592,14 +602,17 @@ __global__ __launch_bounds__(kBlock) void gj_init_k(int na, const int *__restric
   int bi = na;
   for (int r = wave; r < na; r += gridDim.x * kGjWaves) {
     for (int j = lane; j < w2; j += 64) M[(int64_t)r * w2 + j] = (j == na + r) ? 1.0 : 0.0;
-    __syncwarp();
-    if (lane == 0) {
-      used[r] = 0;
-      for (int t = rp[r]; t < rp[r + 1]; t++)
-        if (col[t] < na) M[(int64_t)r * w2 + col[t]] = val[t];
-      const double v = fabs(M[(int64_t)r * w2]);
-      if (v > best) { best = v; bi = r; }
+    __threadfence_block();  // the zeros land before the entries below
+    double s0 = -3.0;       // score of column 0 (-3: no entry, i.e. 0)
+    for (int t = rp[r] + lane; t < rp[r + 1]; t += 64) {
+      const int c = col[t];
+      if (c < na) M[(int64_t)r * w2 + c] = val[t];
+      if (c == 0) s0 = gj_score(val[t]);
     }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s0 = fmax(s0, __shfl_xor(s0, o));
+    if (lane == 0) used[r] = 0;
+    gj_better(best, bi, s0 == -3.0 ? 0.0 : s0, r);
   }
   if (lane == 0) { sv[threadIdx.x >> 6] = best; si[threadIdx.x >> 6] = bi; }
   __syncthreads();
@@ -656,7 +669,7 @@ __global__ __launch_bounds__(kBlock) void gj_step_k(int na, int k, double *__res
     if (lane == 0) {
       row[k] = 0.0;
       if (k + 1 < na && !used[r]) {
-        const double v = fabs(row[k + 1]);
+        const double v = gj_score(row[k + 1]);
         if (v > best) { best = v; bi = r; }
       }
     }
@@ -671,12 +684,14 @@ __global__ __launch_bounds__(kBlock) void gj_step_k(int na, int k, double *__res
   }
 }
 
+// dval: the pivot values (blocked form), else M[perm[k]][k]
 __global__ __launch_bounds__(kBlock) void gj_final_k(int na, const double *__restrict__ M,
-                                                     const int *__restrict__ perm, double *__restrict__ inv) {
+                                                     const int *__restrict__ perm, const double *__restrict__ dval,
+                                                     double *__restrict__ inv) {
   const int w2 = 2 * na, wave = blockIdx.x * kGjWaves + (threadIdx.x >> 6), lane = threadIdx.x & 63;
   for (int k = wave; k < na; k += gridDim.x * kGjWaves) {
     const double *row = M + (int64_t)perm[k] * w2;
-    const double d = 1.0 / row[k];
+    const double d = 1.0 / (dval ? dval[k] : row[k]);
     for (int j = lane; j < na; j += 64) inv[(int64_t)k * na + j] = row[na + j] * d;
   }
 }
@@ -704,9 +719,9 @@ __global__ __launch_bounds__(1024) void gj_lds_k(int na, const int *__restrict__
   for (int k = 0; k < na; k++) {
     if (t < 64) {  // pivot search by the first wavefront
       double best = -1.0;
-      int bi = na;
+      int bi = k;
       for (int i = k + t; i < na; i += 64) {
-        const double v = fabs(A[i * na + k]);
+        const double v = gj_score(A[i * na + k]);
         if (v > best || (v == best && i < bi)) { best = v; bi = i; }
       }
 #pragma unroll
@@ -755,6 +770,144 @@ __global__ __launch_bounds__(1024) void gj_lds_k(int na, const int *__restrict__
     __syncthreads();
   }
   for (int i = t; i < nn; i += nt) inv[i] = A[i];
+}
+
+// Blocked form of the same elimination for kGjLdsMax < na <= kGjBlockedMax:
+// the same pivots (largest unused |m_rk|, ties to the lower row), the same
+// multipliers and the same extraction, reorganised by panels of NB pivot
+// columns.  The panel kernel (one workgroup; each thread holds RPT rows of the
+// panel in registers) finds the NB pivots of the panel and every row's
+// multipliers W[r][j] (0 for the pivot row of step j).  The update kernel then
+// applies their combined effect to the columns right of the panel as one
+// rank-NB update,
+//   U_j  = M[p_j] - sum_{i<j} W[p_j][i] U_i   (pivot row p_j as it was at step j)
+//   M'[r] = M[r] - sum_j W[r][j] U_j           (every row r),
+// reading M and writing M' (two buffers, so that the U rows are formed from
+// the old values while M' is written).  The pivot values go to dval[k].  Two
+// launches per panel instead of NB.
+constexpr int kGjPanel = 16;        // stride of the multiplier rows W
+constexpr int kGjBlockedMax = 4096;  // 4 rows per thread of the panel kernel
+constexpr int kGjUpdRows = 16;      // rows per workgroup of the update kernel
+
+template <int NB, int RPT>
+__global__ __launch_bounds__(1024) void gj_panel_k(int na, int k0, int b, const double *__restrict__ M,
+                                                   int *__restrict__ used, int *__restrict__ perm,
+                                                   double *__restrict__ dval, double *__restrict__ W) {
+  __shared__ double sv[2][16];
+  __shared__ int si[2][16];
+  __shared__ double srow[2][16][NB];  // each wave's candidate row, double-buffered by step
+  const int w2 = 2 * na, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  double v[RPT][NB];
+  bool un[RPT];
+  int pj[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    const int r = t + q * 1024;
+    pj[q] = -1;
+    un[q] = r < na && !used[r];
+#pragma unroll
+    for (int j = 0; j < NB; j++) v[q][j] = (r < na && j < b) ? M[(int64_t)r * w2 + k0 + j] : 0.0;
+  }
+  // the wave's best unused row for column j; its owner lane stages the row
+  auto propose = [&](const int j, const int buf) {
+    double bv = -1.0;
+    int bi = 1 << 30;
+#pragma unroll
+    for (int q = 0; q < RPT; q++)
+      if (un[q]) gj_better(bv, bi, gj_score(v[q][j]), t + q * 1024);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const double ov = __shfl_xor(bv, o);
+      const int oi = __shfl_xor(bi, o);
+      gj_better(bv, bi, ov, oi);
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; q++)
+      if (un[q] && t + q * 1024 == bi) {
+#pragma unroll
+        for (int jj = j; jj < NB; jj++) srow[buf][wv][jj] = v[q][jj];
+      }
+    if (lane == 0) {
+      sv[buf][wv] = bv;
+      si[buf][wv] = bi;
+    }
+  };
+  const int nw = blockDim.x >> 6;
+  propose(0, 0);
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    if (j < b) {
+      const int buf = j & 1;
+      __syncthreads();
+      double gv = lane < nw ? sv[buf][lane] : -2.0;
+      int gi = lane < nw ? si[buf][lane] : (1 << 30), gs = lane;
+#pragma unroll
+      for (int o = 8; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(gv, o);
+        const int oi = __shfl_xor(gi, o), os = __shfl_xor(gs, o);
+        if (ov > gv || (ov == gv && oi < gi)) { gv = ov; gi = oi; gs = os; }
+      }
+      const int p = __shfl(gi, 0), ps = __shfl(gs, 0);
+      const double d = srow[buf][ps][j];
+      if (t == 0) {
+        perm[k0 + j] = p;
+        used[p] = 1;
+        dval[k0 + j] = d;
+      }
+#pragma unroll
+      for (int q = 0; q < RPT; q++) {
+        const int r = t + q * 1024;
+        if (r == p) {
+          un[q] = false;
+          pj[q] = j;
+        } else if (r < na) {
+          const double f = v[q][j] / d;
+          if (f != 0.0)
+#pragma unroll
+            for (int jj = j + 1; jj < NB; jj++) v[q][jj] -= f * srow[buf][ps][jj];
+          v[q][j] = f;
+        }
+      }
+      if (j + 1 < b) propose(j + 1, buf ^ 1);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    const int r = t + q * 1024;
+    if (r < na)
+#pragma unroll
+      for (int j = 0; j < NB; j++) W[(int64_t)r * kGjPanel + j] = (j < b && j != pj[q]) ? v[q][j] : 0.0;
+  }
+}
+
+// rank-NB update of the columns right of the panel; a workgroup forms U for
+// its 64 columns (each wave its own copy, in registers) and updates
+// kGjUpdRows rows of them
+template <int NB>
+__global__ __launch_bounds__(256) void gj_update_k(int na, int k0, int b, const double *__restrict__ Min,
+                                                   double *__restrict__ Mout, const int *__restrict__ perm,
+                                                   const double *__restrict__ W) {
+  const int w2 = 2 * na, c = k0 + b + blockIdx.x * 64 + (threadIdx.x & 63), wv = threadIdx.x >> 6;
+  if (c >= w2) return;
+  double u[NB];
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    u[j] = 0.0;
+    if (j < b) {
+      const int p = perm[k0 + j];
+      double s = Min[(int64_t)p * w2 + c];
+#pragma unroll
+      for (int i = 0; i < j; i++) s -= W[(int64_t)p * kGjPanel + i] * u[i];
+      u[j] = s;
+    }
+  }
+  const int r1 = min(na, (int)(blockIdx.y + 1) * kGjUpdRows);
+  for (int r = blockIdx.y * kGjUpdRows + wv; r < r1; r += 4) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < NB; j++) s += W[(int64_t)r * kGjPanel + j] * u[j];
+    Mout[(int64_t)r * w2 + c] = Min[(int64_t)r * w2 + c] - s;
+  }
 }
 
 // x += sum_k coef[k] Z_k  (FGMRES update x += Z y, UpdateGmresIterate)
@@ -864,9 +1017,33 @@ void k_assemble(hipStream_t s, const DevLevel &L, const double *tab, const AsmPa
                                                                       L.nact, tab, ap, val, rhs);
   }
 }
+// blocked elimination (panels of NB pivots); returns the buffer holding the result
+template <int NB, int RPT>
+const double *gj_blocked(hipStream_t s, int na, double *M, double *M1, int *used, int *perm, double *dval,
+                         double *W) {
+  double *Mi = M, *Mo = M1;
+  for (int k0 = 0; k0 < na; k0 += NB) {
+    const int b = std::min(NB, na - k0), ncol = 2 * na - k0 - b;  // ncol >= na > 0
+    gj_panel_k<NB, RPT><<<1, RPT == 1 ? (na + 63) / 64 * 64 : 1024, 0, s>>>(na, k0, b, Mi, used, perm, dval, W);
+    const dim3 grid((ncol + 63) / 64, (na + kGjUpdRows - 1) / kGjUpdRows);
+    gj_update_k<NB><<<grid, 256, 0, s>>>(na, k0, b, Mi, Mo, perm, W);
+    std::swap(Mi, Mo);
+  }
+  return Mi;
+}
+
+// CD3D_GJ_UNBLOCKED=1: one launch per pivot also for large blocks (comparison runs)
+static bool gj_unblocked() {
+  static const bool v = [] {
+    const char *e = getenv("CD3D_GJ_UNBLOCKED");
+    return e && atoi(e) != 0;
+  }();
+  return v;
+}
+
 void k_invert_active(hipStream_t s, const DevLevel &L, double *scratch, double *inv) {
-  // scratch layout: M (2 na^2 doubles), two partial (value, index) buffers,
-  // used[na], perm[na]  (see gj_scratch_doubles)
+  // scratch layout: M, M' (2 na^2 doubles each), W (kGjPanel na), dval[na],
+  // two partial (value, index) buffers, used[na], perm[na]
   const int na = L.nact;
   if (na <= 0) return;
   if (na <= kGjLdsMax) {
@@ -882,18 +1059,30 @@ void k_invert_active(hipStream_t s, const DevLevel &L, double *scratch, double *
     (void)hipGetLastError();
   }
   const int nblk = std::min((na + kGjWaves - 1) / kGjWaves, 256);
-  double *M = scratch, *pv0 = M + (int64_t)2 * na * na, *pv1 = pv0 + 256;
+  double *M = scratch, *M1 = M + (int64_t)2 * na * na, *W = M1 + (int64_t)2 * na * na, *dval = W + (int64_t)kGjPanel * na;
+  double *pv0 = dval + na, *pv1 = pv0 + 256;
   int *pi0 = (int *)(pv1 + 256), *pi1 = pi0 + 256, *used = pi1 + 256, *perm = used + na;
   gj_init_k<<<nblk, kBlock, 0, s>>>(na, L.rowptr, L.col, L.val, M, used, pv0, pi0);
+  if (na <= kGjBlockedMax && !gj_unblocked()) {
+    const int rpt = (na + 1023) / 1024;
+    const double *Mf = rpt == 1   ? gj_blocked<16, 1>(s, na, M, M1, used, perm, dval, W)
+                       : rpt == 2 ? gj_blocked<8, 2>(s, na, M, M1, used, perm, dval, W)
+                       : rpt == 3 ? gj_blocked<8, 3>(s, na, M, M1, used, perm, dval, W)
+                                  : gj_blocked<8, 4>(s, na, M, M1, used, perm, dval, W);
+    gj_final_k<<<nblk, kBlock, 0, s>>>(na, Mf, perm, dval, inv);
+    return;
+  }
   for (int k = 0; k < na; k++) {
     const bool even = (k % 2) == 0;
     gj_step_k<<<nblk, kBlock, 0, s>>>(na, k, M, used, perm, even ? pv0 : pv1, even ? pi0 : pi1, even ? pv1 : pv0,
                                       even ? pi1 : pi0);
   }
-  gj_final_k<<<nblk, kBlock, 0, s>>>(na, M, perm, inv);
+  gj_final_k<<<nblk, kBlock, 0, s>>>(na, M, perm, nullptr, inv);
 }
-int64_t gj_scratch_doubles(int na) { return (int64_t)2 * na * na + 512 + (512 + 2 * (int64_t)na + 1) / 2 + 1; }
-void k_multi_axpy(hipStream_t s, int n, int m, const double *coef, const double *Z, int64_t ldz, double *x) {
+int64_t gj_scratch_doubles(int na) {
+  // M and M' (na x 2na each), W (na x kGjPanel), dval, partials, used + perm (ints)
+  return (int64_t)4 * na * na + (int64_t)(kGjPanel + 1) * na + 512 + (512 + 2 * (int64_t)na + 2) / 2 + 1;
+}void k_multi_axpy(hipStream_t s, int n, int m, const double *coef, const double *Z, int64_t ldz, double *x) {
   multi_axpy_k<<<grid_for(n), kBlock, 0, s>>>(n, m, coef, Z, ldz, x);
 }
 void k_copy_range(hipStream_t s, int b, int e, const double *src, double *dst) {

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 
 #include "cd3d_internal.hpp"
 
@@ -34,7 +35,12 @@ struct LatArgs {
   LatDev c[8];
   int bstart[9];  // first block of each class
   int uni[8];     // class stencil the same at every node: compact values
+  int pstart[5];  // Q2: first block of each vertex/midpoint pair class (py, pz)
 };
+
+// 8-byte vector at a 4-byte aligned address (global_load_dwordx2 needs dword
+// alignment only)
+typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
 
 enum LatMode { kLJacobi = 0, kLJacobiZero = 1, kLDefect = 2 };
 
@@ -105,6 +111,120 @@ __host__ __device__ inline void lat_class_node(const LatArgs &a, int c, long lon
       default: lat_node_any<2, 1, 1, 1, MODE>(a, c, k, val, inv, cst, x, f, out, omega); break;
     }
   }
+}
+
+// Q2 sweeps by vertex/midpoint pairs along X.  A thread takes the nodes
+// X0 = 2 ix (cell vertex in X, class (0, py, pz)) and X0 + 1 (cell midpoint,
+// class (1, py, pz)).  Both have the same Y/Z stencil extent, and together
+// they need the segment X0-2 .. X0+2 of each neighbour line: two 8-byte and
+// one 4-byte load, consecutive lanes on consecutive pairs, instead of 5 + 3
+// loads at stride 2 (half of every cache line touched) for the two classes
+// separately.  ix = 0 is a boundary vertex: only its midpoint is written.
+// Each node's sum runs over its stencil in the per-class order, so results
+// are bitwise those of lat_node.
+// cs0, cs1: the compact stencils of the two classes (UNI)
+template <int PY, int PZ, int MODE, bool UNI>
+__host__ __device__ inline void lat_pair(const LatArgs &a, long long k, const float *__restrict__ val,
+                                         const float *__restrict__ inv, const float *cs0, const float *cs1,
+                                         const float *__restrict__ x, const float *__restrict__ f,
+                                         float *__restrict__ out, float omega) {
+  constexpr int c0 = 2 * PY + 4 * PZ, c1 = c0 + 1;
+  const LatDev &V = a.c[c0], &M = a.c[c1];
+  const int n = M.cnt[0];  // pairs per line; k is also the midpoint's index in its class
+  const int ix = (int)(k % n);
+  const long long r = k / n;  // ky + cnt[1] kz, the same for both classes
+  const int Y = M.lo[1] + 2 * (int)(r % M.cnt[1]), Z = M.lo[2] + 2 * (int)(r / M.cnt[1]);
+  const long long nn = a.nn, nv = 2 * ix + nn * (Y + nn * (long long)Z);  // the vertex node
+  const bool hv = ix > 0;
+  const long long kv = r * (n - 1) + ix - 1;  // the vertex's index in its class (hv)
+  const f2u fp = *(const f2u *)(f + nv);
+  float d0 = 0.f, d1 = 0.f;
+  if (UNI) {
+    d0 = cs0[kCst - 1];
+    d1 = cs1[kCst - 1];
+  } else if (MODE != kLDefect) {
+    d0 = hv ? inv[V.first + kv] : 0.f;
+    d1 = inv[M.first + k];
+  }
+  if (MODE == kLJacobiZero) {
+    if (hv) out[nv] = omega * fp.x * d0;
+    out[nv + 1] = omega * fp.y * d1;
+    return;
+  }
+  const float *__restrict__ v0 = val + V.off + (hv ? kv : 0);
+  const float *__restrict__ v1 = val + M.off + k;
+  float s0 = 0.f, s1 = 0.f, xv = 0.f, xm = 0.f;
+  constexpr int ny = hi_off<2, PY>() - lo_off<2, PY>() + 1;
+  // planes in a rolled loop: one plane's loads (ny lines) in flight at a time
+  // keeps the registers of both stencils' coefficients and x values bounded
+#pragma unroll 1
+  for (int dz = lo_off<2, PZ>(); dz <= hi_off<2, PZ>(); dz++) {
+    const int l0 = (dz - lo_off<2, PZ>()) * ny;
+#pragma unroll
+    for (int dy = lo_off<2, PY>(); dy <= hi_off<2, PY>(); dy++) {
+      const int li = l0 + dy - lo_off<2, PY>();
+      const float *xl = x + nv + nn * (dy + nn * dz);
+      const f2u lft = hv ? *(const f2u *)(xl - 2) : f2u{0.f, 0.f};
+      const f2u ctr = *(const f2u *)xl;
+      const float rgt = xl[2];
+      if (dy == 0 && dz == 0) {
+        xv = ctr.x;
+        xm = ctr.y;
+      }
+      const float w[5] = {lft.x, lft.y, ctr.x, ctr.y, rgt};
+#pragma unroll
+      for (int j = 0; j < 5; j++)
+        s0 += (UNI ? cs0[li * 5 + j] : (hv ? v0[(long long)(li * 5 + j) * V.nodes] : 0.f)) * w[j];
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+        s1 += (UNI ? cs1[li * 3 + j] : v1[(long long)(li * 3 + j) * M.nodes]) * w[j + 2];
+    }
+  }
+  if (MODE == kLDefect) {
+    if (hv) out[nv] = fp.x - s0;
+    out[nv + 1] = fp.y - s1;
+  } else {
+    if (hv) out[nv] = xv + omega * (fp.x - s0) * d0;
+    out[nv + 1] = xm + omega * (fp.y - s1) * d1;
+  }
+}
+
+// pair k of pair class pc = py + 2 pz; UNI: every class has compact values
+// (cs0, cs1 those of the pair class), else every class has per-node values
+template <int MODE, bool UNI>
+__host__ __device__ inline void lat_pair_node(const LatArgs &a, int pc, long long k, const float *__restrict__ val,
+                                              const float *__restrict__ inv, const float *cs0, const float *cs1,
+                                              const float *__restrict__ x, const float *__restrict__ f,
+                                              float *__restrict__ out, float omega) {
+  switch (pc) {
+    case 0: lat_pair<0, 0, MODE, UNI>(a, k, val, inv, cs0, cs1, x, f, out, omega); break;
+    case 1: lat_pair<1, 0, MODE, UNI>(a, k, val, inv, cs0, cs1, x, f, out, omega); break;
+    case 2: lat_pair<0, 1, MODE, UNI>(a, k, val, inv, cs0, cs1, x, f, out, omega); break;
+    default: lat_pair<1, 1, MODE, UNI>(a, k, val, inv, cs0, cs1, x, f, out, omega); break;
+  }
+}
+
+// UNI: the pair class's two compact stencils are staged in LDS (200 values
+// do not fit the scalar registers)
+template <int MODE, bool UNI>
+__global__ __launch_bounds__(kB) void lat_pair_k(LatArgs a, const float *__restrict__ val,
+                                                 const float *__restrict__ inv, const float *__restrict__ cst,
+                                                 const float *__restrict__ x, const float *__restrict__ f,
+                                                 float *__restrict__ out, float omega) {
+  __shared__ float cs[2 * kCst];
+  // blocks in pair-class order: a CU sweeps one class's code path at a time
+  // (measured: interleaving the four classes per tile, also per XCD, saves
+  // HBM traffic but is slower)
+  int pc = 0;
+  while (pc < 3 && (int)blockIdx.x >= a.pstart[pc + 1]) pc++;
+  const int tile = blockIdx.x - a.pstart[pc];
+  if (UNI) {
+    static_assert(kB == 2 * kCst, "one stencil value per thread");
+    cs[threadIdx.x] = cst[2 * pc * kCst + threadIdx.x];  // classes 2 pc and 2 pc + 1 are adjacent
+    __syncthreads();
+  }
+  const long long k = (long long)tile * kB + threadIdx.x;
+  if (k < a.c[2 * pc + 1].nodes) lat_pair_node<MODE, UNI>(a, pc, k, val, inv, cs, cs + kCst, x, f, out, omega);
 }
 
 template <int P, int MODE>
@@ -715,7 +835,33 @@ LatArgs lat_args(const DevLevel &L) {
     b += (int)((C.nodes + kB - 1) / kB);
   }
   a.bstart[L.nclass] = b;
+  b = 0;
+  if (L.nclass == 8)
+    for (int pc = 0; pc < 4; pc++) {
+      a.pstart[pc] = b;
+      b += (int)((L.cls[2 * pc + 1].nodes + kB - 1) / kB);
+    }
+  a.pstart[4] = b;
   return a;
+}
+
+// small lattices (16^3 cells and below) sweep faster by class: too few
+// workgroups for the pair form's longer per-thread chains to pay off
+constexpr int kPairMinNN = 40;
+
+// CD3D_LAT_PAIRS=0: Q2 sweeps by class instead of by vertex/midpoint pairs
+static bool lat_pairs() {
+  static const bool v = [] {
+    const char *e = getenv("CD3D_LAT_PAIRS");
+    return !(e && atoi(e) == 0);
+  }();
+  return v;
+}
+
+static bool all_uni(const LatArgs &a) {
+  for (int c = 0; c < a.nclass; c++)
+    if (!a.uni[c]) return false;
+  return true;
 }
 
 template <int MODE>
@@ -725,6 +871,10 @@ void lat_sweep(hipStream_t s, const DevLevel &L, int order, const float *x, cons
   const int grid = a.bstart[L.nclass];
   if (grid <= 0) return;
   if (order == 1) lat_sweep_k<1, MODE><<<grid, kB, 0, s>>>(a, L.lval, L.linv, L.lcst, x, f, out, omega);
+  else if (lat_pairs() && L.nn > kPairMinNN && all_uni(a))
+    lat_pair_k<MODE, true><<<a.pstart[4], kB, 0, s>>>(a, L.lval, L.linv, L.lcst, x, f, out, omega);
+  else if (lat_pairs() && L.nn > kPairMinNN)
+    lat_pair_k<MODE, false><<<a.pstart[4], kB, 0, s>>>(a, L.lval, L.linv, L.lcst, x, f, out, omega);
   else lat_sweep_k<2, MODE><<<grid, kB, 0, s>>>(a, L.lval, L.linv, L.lcst, x, f, out, omega);
 }
 

@@ -6,6 +6,7 @@
 // Built by tests/test_lattice_host.py with hipcc (host code only runs).
 #include "lattice.hip"
 #include <cstdio>
+#include <cstring>
 #include <random>
 using namespace cd3d;
 
@@ -248,8 +249,67 @@ int check(int m, int k0) {
   return bad;
 }
 
+// Q2: the vertex/midpoint pair sweep writes exactly what the per-class sweep
+// writes (bitwise), in every mode, with compact and with per-node stencils;
+// x, f and the output's untouched entries hold arbitrary values everywhere.
+int check_pairs(int m, int k0) {
+  HostLevel F;
+  build_host_level(F, 2, m, k0);
+  const int nn = F.nn;
+  const size_t nn3 = (size_t)nn * nn * nn;
+  std::mt19937 g(11);
+  std::vector<float> x(nn3), f(nn3), lval(F.lat_nval), inv(F.nact), cst(8 * kCst);
+  for (auto &v : x) v = (float)rnd(g);
+  for (auto &v : f) v = (float)rnd(g);
+  for (auto &v : lval) v = (float)rnd(g);
+  for (auto &v : inv) v = (float)rnd(g);
+  for (auto &v : cst) v = (float)rnd(g);
+  DevLevel D;
+  D.nn = nn;
+  D.nclass = (int)F.lat.size();
+  for (int c = 0; c < D.nclass; c++) D.cls[c] = F.lat[c];
+  int bad = 0;
+  for (int uni = 0; uni < 2; uni++) {
+    LatArgs a = lat_args(D);
+    for (int c = 0; c < 8; c++) a.uni[c] = uni;
+    int64_t npairs = 0;
+    for (int pc = 0; pc < 4; pc++) npairs += a.c[2 * pc + 1].nodes;
+    if (a.pstart[4] != (int)((a.c[1].nodes + kB - 1) / kB + (a.c[3].nodes + kB - 1) / kB +
+                             (a.c[5].nodes + kB - 1) / kB + (a.c[7].nodes + kB - 1) / kB))
+      bad++;
+    for (int mode = 0; mode < 3; mode++) {
+      std::vector<float> o1(nn3, 7.f), o2(nn3, 7.f);
+      for (int c = 0; c < 8; c++)
+        for (int64_t k = 0; k < a.c[c].nodes; k++) {
+          if (mode == kLJacobi) lat_class_node<2, kLJacobi>(a, c, k, lval.data(), inv.data(), cst.data(), x.data(), f.data(), o1.data(), 0.7f);
+          if (mode == kLJacobiZero) lat_class_node<2, kLJacobiZero>(a, c, k, lval.data(), inv.data(), cst.data(), x.data(), f.data(), o1.data(), 0.7f);
+          if (mode == kLDefect) lat_class_node<2, kLDefect>(a, c, k, lval.data(), inv.data(), cst.data(), x.data(), f.data(), o1.data(), 0.7f);
+        }
+      for (int pc = 0; pc < 4; pc++)
+        for (int64_t k = 0; k < a.c[2 * pc + 1].nodes; k++) {
+          if (mode == kLJacobi) (uni ? lat_pair_node<kLJacobi, true> : lat_pair_node<kLJacobi, false>)(a, pc, k, lval.data(), inv.data(), cst.data() + 2 * pc * kCst, cst.data() + (2 * pc + 1) * kCst, x.data(), f.data(), o2.data(), 0.7f);
+          if (mode == kLJacobiZero) (uni ? lat_pair_node<kLJacobiZero, true> : lat_pair_node<kLJacobiZero, false>)(a, pc, k, lval.data(), inv.data(), cst.data() + 2 * pc * kCst, cst.data() + (2 * pc + 1) * kCst, x.data(), f.data(), o2.data(), 0.7f);
+          if (mode == kLDefect) (uni ? lat_pair_node<kLDefect, true> : lat_pair_node<kLDefect, false>)(a, pc, k, lval.data(), inv.data(), cst.data() + 2 * pc * kCst, cst.data() + (2 * pc + 1) * kCst, x.data(), f.data(), o2.data(), 0.7f);
+        }
+      int64_t diff = 0, written = 0;
+      for (size_t i = 0; i < nn3; i++) {
+        diff += memcmp(&o1[i], &o2[i], sizeof(float)) != 0;
+        written += o1[i] != 7.f;
+      }
+      printf("   pairs m%d k0=%d uni %d mode %d: %lld pairs, %lld nodes written, %lld differ\n", m, k0, uni, mode,
+             (long long)npairs, (long long)written, (long long)diff);
+      if (diff || written < F.nact - 2) bad++;
+    }
+  }
+  return bad;
+}
+
 int main() {
   int bad = 0;
+  bad += check_pairs(1, 0);
+  bad += check_pairs(1, 2);
+  bad += check_pairs(3, 1);
+  bad += check_pairs(1, 5);  // 32^3 cells: many tiles per XCD
   bad += check<1>(1, 1);
   bad += check<1>(2, 1);
   bad += check<2>(1, 1);
