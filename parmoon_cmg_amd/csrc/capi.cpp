@@ -213,6 +213,10 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
   HIPCHK(hipHostMalloc((void **)&P.hflag, 64 * sizeof(int), hipHostMallocDefault));
 }
 
+// basis column stride: a multiple of 32 doubles, so that every column is
+// 256-byte aligned (the vector kernels stream 16-byte pairs)
+static int64_t krylov_ld(int64_t N) { return (N + 31) / 32 * 32; }
+
 static void ensure_krylov(cd3d_problem &P, int restart) {
   if (restart <= P.krylov_restart) return;
   const int N = P.dl.back().ndof;
@@ -221,8 +225,9 @@ static void ensure_krylov(cd3d_problem &P, int restart) {
     dfree(P, P.V);
     dfree(P, P.Z);
   }
-  P.V = dalloc<double>(P, (size_t)(restart + 1) * N);
-  P.Z = dalloc<double>(P, (size_t)(restart + 1) * N);
+  P.krylov_ld = krylov_ld(N);
+  P.V = dalloc<double>(P, (size_t)(restart + 1) * P.krylov_ld);
+  P.Z = dalloc<double>(P, (size_t)(restart + 1) * P.krylov_ld);
   if (!P.w) {
     P.w = dalloc<double>(P, N);
     P.xs = dalloc<double>(P, N);
@@ -598,8 +603,10 @@ int fgmres(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S, double
   ensure_krylov(P, R);
   VC v{P, sp};
   double *V = P.V, *Z = P.Z, *w = P.w;
-  auto Vk = [&](int k) { return V + (int64_t)k * N; };
-  auto Zk = [&](int k) { return Z + (int64_t)k * N; };
+  const int64_t ld = P.krylov_ld;
+  if (ld < N) throw std::logic_error("Krylov basis smaller than the space");
+  auto Vk = [&](int k) { return V + (int64_t)k * ld; };
+  auto Zk = [&](int k) { return Z + (int64_t)k * ld; };
   std::vector<double> H((size_t)(R + 1) * R, 0.0), s(R + 1), cs(R + 1), sn(R + 1);
   double *pa = P.red, *pb = P.red + kRedBlocks;
   S.defect(S.x, S.b, Vk(0));
@@ -621,7 +628,7 @@ int fgmres(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S, double
     }
     memcpy(P.hscal + 64, y.data(), sizeof(double) * (k + 1));
     HIPCHK(hipMemcpyAsync(P.dscal + 64, P.hscal + 64, sizeof(double) * (k + 1), hipMemcpyHostToDevice, P.stream));
-    k_multi_axpy(P.stream, N, k + 1, P.dscal + 64, Z, N, S.x);
+    k_multi_axpy(P.stream, N, k + 1, P.dscal + 64, Z, ld, S.x);
     HIPCHK(hipStreamSynchronize(P.stream));  // pinned y is reused next time
   };
   while (j <= sp.maxit) {
@@ -756,7 +763,7 @@ size_t cd3d_estimate_bytes(const cd3d_mesh_desc *mesh, int restart, int cellwise
     prevN = N;
     N_fine = N;
   }
-  total += 8.0 * N_fine * (2.0 * (restart + 1) + 3);  // Krylov basis V, Z and w, xs, bs
+  total += 8.0 * (2.0 * (restart + 1) * krylov_ld(N_fine) + 3.0 * N_fine);  // Krylov basis V, Z and w, xs, bs
   if (tcd) total += 3 * 8.0 * N_fine;
   total += 8.0 * (4 * 1024 + 256) + 8.0 * 4096;     // reductions, scalars, basis tables
   return (size_t)total;

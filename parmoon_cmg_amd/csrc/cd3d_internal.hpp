@@ -152,6 +152,7 @@ struct Problem {
   bool cinv_valid = false;
   // FGMRES workspace
   int krylov_restart = 0;
+  int64_t krylov_ld = 0;  // column stride of V and Z
   double *V = nullptr, *Z = nullptr, *w = nullptr, *xs = nullptr, *bs = nullptr;
   double *red = nullptr;      // reduction partials
   double *dscal = nullptr;    // device scalars (H column, norms)

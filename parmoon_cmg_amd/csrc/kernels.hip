@@ -273,6 +273,33 @@ __global__ __launch_bounds__(kBlock) void dot_k(int n, const double *__restrict_
   if (threadIdx.x == 0) partials[blockIdx.x] = v;
 }
 
+// The level vectors are streamed as 16-byte pairs (global_load_dwordx4), two
+// pairs in flight per thread, when every operand is 16-byte aligned; an odd
+// last element goes to thread 0 of block 0.
+__global__ __launch_bounds__(kBlock) void dot2_k(int n, const double *__restrict__ a, const double *__restrict__ b,
+                                                 double *__restrict__ partials) {
+  __shared__ double sh[17];
+  const int n2 = n >> 1, gs = gridDim.x * kBlock;
+  const double2 *a2 = (const double2 *)a, *b2 = (const double2 *)b;
+  double v = 0;
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  for (; i + gs < n2; i += 2 * gs) {
+    const double2 p = a2[i], q = b2[i], r = a2[i + gs], t = b2[i + gs];
+    v += p.x * q.x;
+    v += p.y * q.y;
+    v += r.x * t.x;
+    v += r.y * t.y;
+  }
+  if (i < n2) {
+    const double2 p = a2[i], q = b2[i];
+    v += p.x * q.x;
+    v += p.y * q.y;
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) v += a[n - 1] * b[n - 1];
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = v;
+}
+
 __global__ __launch_bounds__(kBlock) void finish_k(const double *__restrict__ partials, double *__restrict__ out) {
   __shared__ double sh[17];
   const double v = sum_partials(partials, sh);
@@ -299,6 +326,60 @@ __global__ __launch_bounds__(kBlock) void axpy_dot_k(int n, const double *__rest
   if (threadIdx.x == 0) partials[blockIdx.x] = v;
 }
 
+// axpy_dot_k on 16-byte pairs (Z: a separate z vector, else the dot is <y, y>).
+// The loads of a thread's first two pairs do not depend on alpha: they are
+// issued before the reduction of the alpha partials, which hides its latency.
+template <bool Z>
+__global__ __launch_bounds__(kBlock) void axpy_dot2_k(int n, const double *__restrict__ alpha_partials,
+                                                      double *__restrict__ alpha_out, double sign,
+                                                      const double *__restrict__ x, double *__restrict__ y,
+                                                      const double *__restrict__ z, double *__restrict__ partials) {
+  __shared__ double sh[17];
+  const int n2 = n >> 1, gs = gridDim.x * kBlock;
+  const double2 *x2 = (const double2 *)x, *z2 = (const double2 *)z;
+  double2 *y2 = (double2 *)y;
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  const bool ha = i < n2, hb = i + gs < n2;
+  const double2 zero = {0, 0};
+  const double2 xa = ha ? x2[i] : zero, xb = hb ? x2[i + gs] : zero;
+  double2 ya = ha ? y2[i] : zero, yb = hb ? y2[i + gs] : zero;
+  const double2 za = (Z && ha) ? z2[i] : zero, zb = (Z && hb) ? z2[i + gs] : zero;
+  const double alpha = sum_partials(alpha_partials, sh);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && alpha_out) *alpha_out = alpha;
+  const double a = sign * alpha;
+  double v = 0;
+  if (ha) {
+    ya.x += a * xa.x;
+    ya.y += a * xa.y;
+    y2[i] = ya;
+    v += ya.x * (Z ? za.x : ya.x);
+    v += ya.y * (Z ? za.y : ya.y);
+  }
+  if (hb) {
+    yb.x += a * xb.x;
+    yb.y += a * xb.y;
+    y2[i + gs] = yb;
+    v += yb.x * (Z ? zb.x : yb.x);
+    v += yb.y * (Z ? zb.y : yb.y);
+  }
+  for (i += 2 * gs; i < n2; i += gs) {
+    double2 yi = y2[i];
+    const double2 xi = x2[i], zi = Z ? z2[i] : zero;
+    yi.x += a * xi.x;
+    yi.y += a * xi.y;
+    y2[i] = yi;
+    v += yi.x * (Z ? zi.x : yi.x);
+    v += yi.y * (Z ? zi.y : yi.y);
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    const double yi = y[n - 1] + a * x[n - 1];
+    y[n - 1] = yi;
+    v += yi * (Z ? z[n - 1] : yi);
+  }
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = v;
+}
+
 // y = x / sqrt(sum partials); writes the norm to *norm_out
 __global__ __launch_bounds__(kBlock) void scale_copy_k(int n, const double *__restrict__ partials,
                                                        double *__restrict__ norm_out, const double *__restrict__ x,
@@ -308,6 +389,26 @@ __global__ __launch_bounds__(kBlock) void scale_copy_k(int n, const double *__re
   if (blockIdx.x == 0 && threadIdx.x == 0) *norm_out = nrm;
   const double s = nrm != 0.0 ? 1.0 / nrm : 1.0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] = x[i] * s;
+}
+
+__global__ __launch_bounds__(kBlock) void scale_copy2_k(int n, const double *__restrict__ partials,
+                                                        double *__restrict__ norm_out, const double *__restrict__ x,
+                                                        double *__restrict__ y) {
+  __shared__ double sh[17];
+  const int n2 = n >> 1, gs = gridDim.x * kBlock;
+  const double2 *x2 = (const double2 *)x;
+  double2 *y2 = (double2 *)y;
+  int i = blockIdx.x * kBlock + threadIdx.x;
+  const double2 v0 = i < n2 ? x2[i] : double2{0, 0};  // issued before the reduction
+  const double nrm = sqrt(sum_partials(partials, sh));
+  if (blockIdx.x == 0 && threadIdx.x == 0) *norm_out = nrm;
+  const double s = nrm != 0.0 ? 1.0 / nrm : 1.0;
+  if (i < n2) y2[i] = double2{v0.x * s, v0.y * s};
+  for (i += gs; i < n2; i += gs) {
+    const double2 v = x2[i];
+    y2[i] = double2{v.x * s, v.y * s};
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) y[n - 1] = x[n - 1] * s;
 }
 
 __global__ void scale_k(int n, double alpha, double *__restrict__ x) {
@@ -336,6 +437,9 @@ __global__ void matadd_k(int64_t n, double *__restrict__ C, const double *__rest
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
     C[i] += alpha * D[i];
 }
+
+// 16-byte aligned (a null pointer counts as aligned)
+static inline bool al16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
 inline int grid_for(int64_t n) {
   int64_t g = (n + kBlock - 1) / kBlock;
@@ -911,6 +1015,30 @@ __global__ __launch_bounds__(256) void gj_update_k(int na, int k0, int b, const 
 }
 
 // x += sum_k coef[k] Z_k  (FGMRES update x += Z y, UpdateGmresIterate)
+// multi_axpy_k on 16-byte pairs (Z, x and ldz 16-byte aligned)
+__global__ void multi_axpy2_k(int n, int m, const double *__restrict__ coef, const double *__restrict__ Z,
+                              int64_t ldz, double *__restrict__ x) {
+  const int n2 = n >> 1;
+  double2 *x2 = (double2 *)x;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n2; i += gridDim.x * kBlock) {
+    double2 t = {0, 0};
+    for (int k = 0; k < m; k++) {
+      const double2 z = ((const double2 *)(Z + k * ldz))[i];
+      t.x += coef[k] * z.x;
+      t.y += coef[k] * z.y;
+    }
+    double2 v = x2[i];
+    v.x += t.x;
+    v.y += t.y;
+    x2[i] = v;
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    double t = 0;
+    for (int k = 0; k < m; k++) t += coef[k] * Z[k * ldz + n - 1];
+    x[n - 1] += t;
+  }
+}
+
 __global__ void multi_axpy_k(int n, int m, const double *__restrict__ coef, const double *__restrict__ Z,
                              int64_t ldz, double *__restrict__ x) {
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
@@ -973,15 +1101,22 @@ void k_prolong(hipStream_t s, const DevLevel &F, const DevLevel &C, const double
                                      coarse, nullptr, fine, 0.0);
 }
 void k_dot(hipStream_t s, int n, const double *a, const double *b, double *partials, double *out) {
-  dot_k<<<kRedBlocks, kBlock, 0, s>>>(n, a, b, partials);
+  if (al16(a) && al16(b)) dot2_k<<<kRedBlocks, kBlock, 0, s>>>(n, a, b, partials);
+  else dot_k<<<kRedBlocks, kBlock, 0, s>>>(n, a, b, partials);
   if (out) finish_k<<<1, kBlock, 0, s>>>(partials, out);
 }
 void k_axpy_dot(hipStream_t s, int n, const double *alpha_partials, double sign, const double *x, double *y,
                 const double *z, double *partials, double *alpha_out) {
-  axpy_dot_k<<<kRedBlocks, kBlock, 0, s>>>(n, alpha_partials, alpha_out, sign, x, y, z, partials);
+  if (al16(x) && al16(y) && al16(z)) {
+    if (z) axpy_dot2_k<true><<<kRedBlocks, kBlock, 0, s>>>(n, alpha_partials, alpha_out, sign, x, y, z, partials);
+    else axpy_dot2_k<false><<<kRedBlocks, kBlock, 0, s>>>(n, alpha_partials, alpha_out, sign, x, y, z, partials);
+  } else {
+    axpy_dot_k<<<kRedBlocks, kBlock, 0, s>>>(n, alpha_partials, alpha_out, sign, x, y, z, partials);
+  }
 }
 void k_scale_copy(hipStream_t s, int n, const double *partials, double *norm_out, const double *x, double *y) {
-  scale_copy_k<<<kRedBlocks, kBlock, 0, s>>>(n, partials, norm_out, x, y);
+  if (al16(x) && al16(y)) scale_copy2_k<<<kRedBlocks, kBlock, 0, s>>>(n, partials, norm_out, x, y);
+  else scale_copy_k<<<kRedBlocks, kBlock, 0, s>>>(n, partials, norm_out, x, y);
 }
 void k_scale(hipStream_t s, int n, double alpha, double *x) { scale_k<<<grid_for(n), kBlock, 0, s>>>(n, alpha, x); }
 void k_axpy(hipStream_t s, int n, double alpha, const double *x, double *y) {
@@ -1083,7 +1218,8 @@ int64_t gj_scratch_doubles(int na) {
   // M and M' (na x 2na each), W (na x kGjPanel), dval, partials, used + perm (ints)
   return (int64_t)4 * na * na + (int64_t)(kGjPanel + 1) * na + 512 + (512 + 2 * (int64_t)na + 2) / 2 + 1;
 }void k_multi_axpy(hipStream_t s, int n, int m, const double *coef, const double *Z, int64_t ldz, double *x) {
-  multi_axpy_k<<<grid_for(n), kBlock, 0, s>>>(n, m, coef, Z, ldz, x);
+  if (al16(Z) && al16(x) && ldz % 2 == 0) multi_axpy2_k<<<grid_for((n + 1) / 2), kBlock, 0, s>>>(n, m, coef, Z, ldz, x);
+  else multi_axpy_k<<<grid_for(n), kBlock, 0, s>>>(n, m, coef, Z, ldz, x);
 }
 void k_copy_range(hipStream_t s, int b, int e, const double *src, double *dst) {
   if (e > b) copy_range_k<<<(e - b + kBlock - 1) / kBlock, kBlock, 0, s>>>(b, e, src, dst);

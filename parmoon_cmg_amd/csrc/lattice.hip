@@ -116,44 +116,58 @@ __host__ __device__ inline void lat_class_node(const LatArgs &a, int c, long lon
 // Q2 sweeps by vertex/midpoint pairs along X.  A thread takes the nodes
 // X0 = 2 ix (cell vertex in X, class (0, py, pz)) and X0 + 1 (cell midpoint,
 // class (1, py, pz)).  Both have the same Y/Z stencil extent, and together
-// they need the segment X0-2 .. X0+2 of each neighbour line: two 8-byte and
-// one 4-byte load, consecutive lanes on consecutive pairs, instead of 5 + 3
-// loads at stride 2 (half of every cache line touched) for the two classes
-// separately.  ix = 0 is a boundary vertex: only its midpoint is written.
-// Each node's sum runs over its stencil in the per-class order, so results
-// are bitwise those of lat_node.
-// cs0, cs1: the compact stencils of the two classes (UNI)
-template <int PY, int PZ, int MODE, bool UNI>
-__host__ __device__ inline void lat_pair(const LatArgs &a, long long k, const float *__restrict__ val,
-                                         const float *__restrict__ inv, const float *cs0, const float *cs1,
-                                         const float *__restrict__ x, const float *__restrict__ f,
-                                         float *__restrict__ out, float omega) {
-  constexpr int c0 = 2 * PY + 4 * PZ, c1 = c0 + 1;
-  const LatDev &V = a.c[c0], &M = a.c[c1];
-  const int n = M.cnt[0];  // pairs per line; k is also the midpoint's index in its class
+// they need the segment X0-2 .. X0+2 of each neighbour line: two 2-element
+// and one 1-element load, consecutive lanes on consecutive pairs, instead of
+// 5 + 3 loads at stride 2 (half of every cache line touched) for the two
+// classes separately.  ix = 0 is a boundary vertex: only its midpoint is
+// written.  Each node's sum runs over its stencil in the per-class order, so
+// results are bitwise those of the per-class kernels.
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<float> {
+  typedef f2u type;
+};
+template <>
+struct Vec2<double> {
+  typedef double type __attribute__((ext_vector_type(2), aligned(8)));
+};
+
+struct PairPos {
+  long long nv;  // lattice index of the vertex node (the midpoint is nv + 1)
+  long long kv;  // the vertex's index in its class (when hv)
+  bool hv;       // the vertex is interior (ix > 0)
+};
+
+// pair k of pair class (PY, PZ); k is also the midpoint's index in its class
+template <int PY, int PZ>
+__host__ __device__ inline PairPos pair_pos(const LatArgs &a, long long k) {
+  const LatDev &M = a.c[2 * PY + 4 * PZ + 1];
+  const int n = M.cnt[0];  // pairs per line
   const int ix = (int)(k % n);
   const long long r = k / n;  // ky + cnt[1] kz, the same for both classes
   const int Y = M.lo[1] + 2 * (int)(r % M.cnt[1]), Z = M.lo[2] + 2 * (int)(r / M.cnt[1]);
-  const long long nn = a.nn, nv = 2 * ix + nn * (Y + nn * (long long)Z);  // the vertex node
-  const bool hv = ix > 0;
-  const long long kv = r * (n - 1) + ix - 1;  // the vertex's index in its class (hv)
-  const f2u fp = *(const f2u *)(f + nv);
-  float d0 = 0.f, d1 = 0.f;
-  if (UNI) {
-    d0 = cs0[kCst - 1];
-    d1 = cs1[kCst - 1];
-  } else if (MODE != kLDefect) {
-    d0 = hv ? inv[V.first + kv] : 0.f;
-    d1 = inv[M.first + k];
-  }
-  if (MODE == kLJacobiZero) {
-    if (hv) out[nv] = omega * fp.x * d0;
-    out[nv + 1] = omega * fp.y * d1;
-    return;
-  }
-  const float *__restrict__ v0 = val + V.off + (hv ? kv : 0);
-  const float *__restrict__ v1 = val + M.off + k;
-  float s0 = 0.f, s1 = 0.f, xv = 0.f, xm = 0.f;
+  PairPos q;
+  q.nv = 2 * ix + (long long)a.nn * (Y + (long long)a.nn * Z);
+  q.kv = r * (n - 1) + ix - 1;
+  q.hv = ix > 0;
+  return q;
+}
+
+// the stencil sums s0 (vertex) and s1 (midpoint) of a pair and the pair's
+// own x values; UNI: compact stencils cs0, cs1, else per-node values val
+// ([class][entry][node], FP32)
+template <typename T, int PY, int PZ, bool UNI>
+__host__ __device__ inline void pair_sums(const LatArgs &a, const PairPos &q, long long k,
+                                          const float *__restrict__ val, const T *cs0, const T *cs1,
+                                          const T *__restrict__ x, T &s0, T &s1, T &xv, T &xm) {
+  typedef typename Vec2<T>::type V2;
+  constexpr int c0 = 2 * PY + 4 * PZ, c1 = c0 + 1;
+  const LatDev &V = a.c[c0], &M = a.c[c1];
+  const long long nn = a.nn;
+  const float *__restrict__ v0 = UNI ? nullptr : val + V.off + (q.hv ? q.kv : 0);
+  const float *__restrict__ v1 = UNI ? nullptr : val + M.off + k;
+  s0 = s1 = xv = xm = T(0);
   constexpr int ny = hi_off<2, PY>() - lo_off<2, PY>() + 1;
   // planes in a rolled loop: one plane's loads (ny lines) in flight at a time
   // keeps the registers of both stencils' coefficients and x values bounded
@@ -163,23 +177,51 @@ __host__ __device__ inline void lat_pair(const LatArgs &a, long long k, const fl
 #pragma unroll
     for (int dy = lo_off<2, PY>(); dy <= hi_off<2, PY>(); dy++) {
       const int li = l0 + dy - lo_off<2, PY>();
-      const float *xl = x + nv + nn * (dy + nn * dz);
-      const f2u lft = hv ? *(const f2u *)(xl - 2) : f2u{0.f, 0.f};
-      const f2u ctr = *(const f2u *)xl;
-      const float rgt = xl[2];
+      const T *xl = x + q.nv + nn * (dy + nn * dz);
+      const V2 lft = q.hv ? *(const V2 *)(xl - 2) : V2{T(0), T(0)};
+      const V2 ctr = *(const V2 *)xl;
+      const T rgt = xl[2];
       if (dy == 0 && dz == 0) {
         xv = ctr.x;
         xm = ctr.y;
       }
-      const float w[5] = {lft.x, lft.y, ctr.x, ctr.y, rgt};
+      const T w[5] = {lft.x, lft.y, ctr.x, ctr.y, rgt};
 #pragma unroll
       for (int j = 0; j < 5; j++)
-        s0 += (UNI ? cs0[li * 5 + j] : (hv ? v0[(long long)(li * 5 + j) * V.nodes] : 0.f)) * w[j];
+        s0 += (UNI ? cs0[li * 5 + j] : (q.hv ? T(v0[(long long)(li * 5 + j) * V.nodes]) : T(0))) * w[j];
 #pragma unroll
       for (int j = 0; j < 3; j++)
-        s1 += (UNI ? cs1[li * 3 + j] : v1[(long long)(li * 3 + j) * M.nodes]) * w[j + 2];
+        s1 += (UNI ? cs1[li * 3 + j] : T(v1[(long long)(li * 3 + j) * M.nodes])) * w[j + 2];
     }
   }
+}
+
+// FP32 sweep of pair k (modes of lat_node); cs0, cs1: the compact stencils of
+// the two classes (UNI)
+template <int PY, int PZ, int MODE, bool UNI>
+__host__ __device__ inline void lat_pair(const LatArgs &a, long long k, const float *__restrict__ val,
+                                         const float *__restrict__ inv, const float *cs0, const float *cs1,
+                                         const float *__restrict__ x, const float *__restrict__ f,
+                                         float *__restrict__ out, float omega) {
+  const PairPos q = pair_pos<PY, PZ>(a, k);
+  const long long nv = q.nv;
+  const bool hv = q.hv;
+  const f2u fp = *(const f2u *)(f + nv);
+  float d0 = 0.f, d1 = 0.f;
+  if (UNI) {
+    d0 = cs0[kCst - 1];
+    d1 = cs1[kCst - 1];
+  } else if (MODE != kLDefect) {
+    d0 = hv ? inv[a.c[2 * PY + 4 * PZ].first + q.kv] : 0.f;
+    d1 = inv[a.c[2 * PY + 4 * PZ + 1].first + k];
+  }
+  if (MODE == kLJacobiZero) {
+    if (hv) out[nv] = omega * fp.x * d0;
+    out[nv + 1] = omega * fp.y * d1;
+    return;
+  }
+  float s0, s1, xv, xm;
+  pair_sums<float, PY, PZ, UNI>(a, q, k, val, cs0, cs1, x, s0, s1, xv, xm);
   if (MODE == kLDefect) {
     if (hv) out[nv] = fp.x - s0;
     out[nv + 1] = fp.y - s1;
@@ -729,7 +771,7 @@ __global__ void lat_fill_affine_k(LatArgs a, long long n, const double *__restri
 enum Lat64Mode { kL64Spmv = 0, kL64Defect = 1 };
 
 template <int P, int PX, int PY, int PZ, int MODE>
-__device__ inline void lat64_node(const LatArgs &a, const LatDev &C, long long k, const double *__restrict__ st,
+__host__ __device__ inline void lat64_node(const LatArgs &a, const LatDev &C, long long k, const double *__restrict__ st,
                                   const double *__restrict__ x, const double *__restrict__ f,
                                   double *__restrict__ out) {
   const int kx = (int)(k % C.cnt[0]), ky = (int)((k / C.cnt[0]) % C.cnt[1]), kz = (int)(k / ((long long)C.cnt[0] * C.cnt[1]));
@@ -769,6 +811,44 @@ __global__ __launch_bounds__(kB) void lat64_k(LatArgs a, const double *__restric
       default: lat64_node<2, 1, 1, 1, MODE>(a, a.c[c], k, st, x, f, out); break;
     }
   }
+}
+
+// Q2 FP64 operator by vertex/midpoint pairs (the gathers of lat_pair; the
+// class stencils cs0, cs1 of the pair class)
+template <int PY, int PZ, int MODE>
+__host__ __device__ inline void lat64_pair(const LatArgs &a, long long k, const double *cs0, const double *cs1,
+                                           const double *__restrict__ x, const double *__restrict__ f,
+                                           double *__restrict__ out) {
+  const PairPos q = pair_pos<PY, PZ>(a, k);
+  double s0, s1, xv, xm;
+  pair_sums<double, PY, PZ, true>(a, q, k, nullptr, cs0, cs1, x, s0, s1, xv, xm);
+  if (q.hv) out[q.nv] = MODE == kL64Defect ? f[q.nv] - s0 : s0;
+  out[q.nv + 1] = MODE == kL64Defect ? f[q.nv + 1] - s1 : s1;
+}
+
+template <int MODE>
+__host__ __device__ inline void lat64_pair_node(const LatArgs &a, int pc, long long k, const double *cs0,
+                                                const double *cs1, const double *__restrict__ x,
+                                                const double *__restrict__ f, double *__restrict__ out) {
+  switch (pc) {
+    case 0: lat64_pair<0, 0, MODE>(a, k, cs0, cs1, x, f, out); break;
+    case 1: lat64_pair<1, 0, MODE>(a, k, cs0, cs1, x, f, out); break;
+    case 2: lat64_pair<0, 1, MODE>(a, k, cs0, cs1, x, f, out); break;
+    default: lat64_pair<1, 1, MODE>(a, k, cs0, cs1, x, f, out); break;
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kB) void lat64_pair_k(LatArgs a, const double *__restrict__ st64,
+                                                   const double *__restrict__ x, const double *__restrict__ f,
+                                                   double *__restrict__ out) {
+  __shared__ double cs[2 * kCst];
+  int pc = 0;
+  while (pc < 3 && (int)blockIdx.x >= a.pstart[pc + 1]) pc++;
+  cs[threadIdx.x] = st64[2 * pc * kCst + threadIdx.x];  // classes 2 pc and 2 pc + 1
+  __syncthreads();
+  const long long k = (long long)(blockIdx.x - a.pstart[pc]) * kB + threadIdx.x;
+  if (k < a.c[2 * pc + 1].nodes) lat64_pair_node<MODE>(a, pc, k, cs, cs + kCst, x, f, out);
 }
 
 // Dirichlet rows (DOFs nact..ndof-1): identity
@@ -1006,6 +1086,7 @@ void lat64_apply(hipStream_t s, const DevLevel &L, int order, const double *x, c
   const int grid = a.bstart[L.nclass];
   if (grid > 0) {
     if (order == 1) lat64_k<1, MODE><<<grid, kB, 0, s>>>(a, L.lst64, x, f, out);
+    else if (lat_pairs() && L.nn > kPairMinNN) lat64_pair_k<MODE><<<a.pstart[4], kB, 0, s>>>(a, L.lst64, x, f, out);
     else lat64_k<2, MODE><<<grid, kB, 0, s>>>(a, L.lst64, x, f, out);
   }
   const int nd = L.ndof - L.nact;

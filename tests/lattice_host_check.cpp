@@ -301,6 +301,43 @@ int check_pairs(int m, int k0) {
       if (diff || written < F.nact - 2) bad++;
     }
   }
+  // FP64 operator: pairs against the per-class nodes, both modes
+  {
+    std::vector<double> xd(nn3), fd(nn3), st(8 * kCst);
+    for (auto &v : xd) v = rnd(g);
+    for (auto &v : fd) v = rnd(g);
+    for (auto &v : st) v = rnd(g);
+    const LatArgs a = lat_args(D);
+    for (int mode = 0; mode < 2; mode++) {
+      std::vector<double> o1(nn3, 7.0), o2(nn3, 7.0);
+      for (int c = 0; c < 8; c++)
+        for (int64_t k = 0; k < a.c[c].nodes; k++) {
+          const double *sc = st.data() + c * kCst;
+#define CD3D_L64(PX, PY, PZ)                                                                          \
+  (mode ? lat64_node<2, PX, PY, PZ, kL64Defect> : lat64_node<2, PX, PY, PZ, kL64Spmv>)(a, a.c[c], k, sc, \
+                                                                                      xd.data(), fd.data(), o1.data())
+          switch (c) {
+            case 0: CD3D_L64(0, 0, 0); break;
+            case 1: CD3D_L64(1, 0, 0); break;
+            case 2: CD3D_L64(0, 1, 0); break;
+            case 3: CD3D_L64(1, 1, 0); break;
+            case 4: CD3D_L64(0, 0, 1); break;
+            case 5: CD3D_L64(1, 0, 1); break;
+            case 6: CD3D_L64(0, 1, 1); break;
+            default: CD3D_L64(1, 1, 1); break;
+          }
+#undef CD3D_L64
+        }
+      for (int pc = 0; pc < 4; pc++)
+        for (int64_t k = 0; k < a.c[2 * pc + 1].nodes; k++)
+          (mode ? lat64_pair_node<kL64Defect> : lat64_pair_node<kL64Spmv>)(
+              a, pc, k, st.data() + 2 * pc * kCst, st.data() + (2 * pc + 1) * kCst, xd.data(), fd.data(), o2.data());
+      int64_t diff = 0;
+      for (size_t i = 0; i < nn3; i++) diff += memcmp(&o1[i], &o2[i], sizeof(double)) != 0;
+      printf("   f64 pairs m%d k0=%d mode %d: %lld differ\n", m, k0, mode, (long long)diff);
+      if (diff) bad++;
+    }
+  }
   return bad;
 }
 
