@@ -112,9 +112,6 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
   P.nb = (m.order + 1) * (m.order + 1) * (m.order + 1);
   HIPCHK(hipGetDevice(&P.device));
   HIPCHK(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
-  HIPCHK(hipStreamCreateWithFlags(&P.side, hipStreamNonBlocking));
-  HIPCHK(hipEventCreateWithFlags(&P.ev_fork, hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&P.ev_join, hipEventDisableTiming));
   const int L = m.levels;
   P.hl.resize(L);
   P.dl.resize(L);
@@ -336,6 +333,15 @@ static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int wha
 // launch_coarse_inverse enqueues the inversion after that point (called once
 // the main stream has other work queued, so that it runs beside that work).
 static void fork_coarse(cd3d_problem &P) {
+  if (!k_invert_multi_launch(P.dl[0].nact)) return;  // inverted on the main stream
+  if (!P.side) {
+    // created on first need only: HIP hands streams to its hardware queues
+    // round-robin, and a side stream per handle would leave every other
+    // queue to the side streams of concurrently served handles
+    HIPCHK(hipStreamCreateWithFlags(&P.side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&P.ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&P.ev_join, hipEventDisableTiming));
+  }
   HIPCHK(hipEventRecord(P.ev_fork, P.stream));
   P.fork_recorded = true;
 }
@@ -344,8 +350,13 @@ static void launch_coarse_inverse(cd3d_problem &P) {
   DevLevel &C = P.dl[0];
   const int na = C.nact;
   if (!P.fork_recorded) fork_coarse(P);
-  P.fork_recorded = false;
-  if (na > 0) {
+  P.fork_recorded = false;  // (fork_coarse records nothing when the inverse stays on the main stream)
+  if (na > 0 && !k_invert_multi_launch(na)) {
+    // one short launch: the fork and join would cost more than they hide
+    if (!P.gj_scratch) P.gj_scratch = dalloc<double>(P, (size_t)gj_scratch_doubles(na));
+    k_invert_active(P.stream, C, P.gj_scratch, P.cinv);
+    HIPCHK(hipGetLastError());
+  } else if (na > 0) {
     if (!P.gj_scratch) P.gj_scratch = dalloc<double>(P, (size_t)gj_scratch_doubles(na));
     HIPCHK(hipStreamWaitEvent(P.side, P.ev_fork, 0));
     k_invert_active(P.side, C, P.gj_scratch, P.cinv);

@@ -270,6 +270,8 @@ void kl_to_dof(hipStream_t s, const DevLevel &L, const float *l, double *v);
 // scratch: gj_scratch_doubles(na) doubles; inv: na^2 doubles
 void k_invert_active(hipStream_t s, const DevLevel &L, double *scratch, double *inv);
 int64_t gj_scratch_doubles(int na);
+// the inversion takes more than one launch (larger than the one-workgroup LDS form)
+bool k_invert_multi_launch(int na);
 void k_multi_axpy(hipStream_t s, int n, int m, const double *coef, const double *Z, int64_t ldz, double *x);
 void k_copy_range(hipStream_t s, int b, int e, const double *src, double *dst);
 

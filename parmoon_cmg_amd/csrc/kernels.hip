@@ -132,6 +132,8 @@ void launch_rows(hipStream_t st, int64_t nnz, int n, int nact, const int *rp, co
 // Dirichlet rows (the damped Jacobi step of TMGLevel3D::Jacobi,
 // MGLevel3D.C:443-537, written with the full row sum).
 constexpr int kStreamPer = kStreamNnz / kStreamBlock;
+// matrices above the 256 MB last-level (Infinity) cache are streamed non-temporally
+constexpr double kStreamNtBytes = 256.0 * (1 << 20);
 
 template <int G, int MODE, typename T>
 __device__ inline void stream_rows(int nr, int r0, int j0, int nact, const int *__restrict__ rowptr,
@@ -152,7 +154,14 @@ __device__ inline void stream_rows(int nr, int r0, int j0, int nact, const int *
   }
 }
 
-template <int MODE, typename T, typename V>
+template <bool NT, typename P>
+__device__ inline P stream_load(const P *p) {
+  if (NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
+
+// NT: non-temporal matrix loads (matrices larger than the caches)
+template <int MODE, typename T, typename V, bool NT>
 __global__ __launch_bounds__(kStreamBlock) void csr_stream(const int *__restrict__ blk, int nact,
                                                      const int *__restrict__ rowptr, const int *__restrict__ col,
                                                      const V *__restrict__ val, const T *__restrict__ x,
@@ -180,18 +189,36 @@ __global__ __launch_bounds__(kStreamBlock) void csr_stream(const int *__restrict
     }
     return;
   }
+  // two consecutive nonzeros per thread and load (dword alignment suffices);
+  // a matrix larger than the caches is read non-temporally, so that it does
+  // not displace the gathered x from L2
+  typedef int i2u __attribute__((ext_vector_type(2), aligned(4)));
+  typedef V v2u __attribute__((ext_vector_type(2), aligned(sizeof(V))));
+  static_assert(kStreamPer % 2 == 0, "pairs of nonzeros per thread");
   int c[kStreamPer];
   T a[kStreamPer];
 #pragma unroll
-  for (int k = 0; k < kStreamPer; k++) {
-    const int t = threadIdx.x + k * kStreamBlock;
-    c[k] = t < nz ? col[j0 + t] : 0;
-    a[k] = t < nz ? (T)val[j0 + t] : T(0);
+  for (int k = 0; k < kStreamPer / 2; k++) {
+    const int t = 2 * threadIdx.x + k * 2 * kStreamBlock;
+    if (t + 1 < nz) {
+      const i2u cc = stream_load<NT>((const i2u *)(col + j0 + t));
+      const v2u vv = stream_load<NT>((const v2u *)(val + j0 + t));
+      c[2 * k] = cc.x;
+      c[2 * k + 1] = cc.y;
+      a[2 * k] = (T)vv.x;
+      a[2 * k + 1] = (T)vv.y;
+    } else {
+      c[2 * k] = t < nz ? col[j0 + t] : 0;
+      a[2 * k] = t < nz ? (T)val[j0 + t] : T(0);
+      c[2 * k + 1] = 0;
+      a[2 * k + 1] = T(0);
+    }
   }
 #pragma unroll
-  for (int k = 0; k < kStreamPer; k++) {
-    const int t = threadIdx.x + k * kStreamBlock;
-    if (t < nz) prod[t] = a[k] * x[c[k]];
+  for (int k = 0; k < kStreamPer / 2; k++) {
+    const int t = 2 * threadIdx.x + k * 2 * kStreamBlock;
+    if (t < nz) prod[t] = a[2 * k] * x[c[2 * k]];
+    if (t + 1 < nz) prod[t + 1] = a[2 * k + 1] * x[c[2 * k + 1]];
   }
   __syncthreads();
   const int G = nr <= 4 ? 64 : nr <= 8 ? 32 : nr <= 16 ? 16 : nr <= 32 ? 8 : nr <= 64 ? 4 : nr <= 128 ? 2 : 1;
@@ -216,8 +243,12 @@ template <int MODE, typename T, typename V>
 void launch_stream(hipStream_t st, const DevLevel &L, const V *val, const T *x, const T *f, const T *invdiag, T *out,
                    T omega) {
   if (L.nblk <= 0) return;
-  csr_stream<MODE, T, V><<<L.nblk, kStreamBlock, 0, st>>>(L.blk, L.nact, L.rowptr, L.col, val, x, f, invdiag, out,
-                                                          omega);
+  if ((double)L.nnz * (sizeof(V) + sizeof(int)) > kStreamNtBytes)
+    csr_stream<MODE, T, V, true><<<L.nblk, kStreamBlock, 0, st>>>(L.blk, L.nact, L.rowptr, L.col, val, x, f, invdiag,
+                                                                  out, omega);
+  else
+    csr_stream<MODE, T, V, false><<<L.nblk, kStreamBlock, 0, st>>>(L.blk, L.nact, L.rowptr, L.col, val, x, f, invdiag,
+                                                                   out, omega);
 }
 
 template <typename T>
@@ -1214,6 +1245,8 @@ void k_invert_active(hipStream_t s, const DevLevel &L, double *scratch, double *
   }
   gj_final_k<<<nblk, kBlock, 0, s>>>(na, M, perm, nullptr, inv);
 }
+bool k_invert_multi_launch(int na) { return na > kGjLdsMax; }
+
 int64_t gj_scratch_doubles(int na) {
   // M and M' (na x 2na each), W (na x kGjPanel), dval, partials, used + perm (ints)
   return (int64_t)4 * na * na + (int64_t)(kGjPanel + 1) * na + 512 + (512 + 2 * (int64_t)na + 2) / 2 + 1;

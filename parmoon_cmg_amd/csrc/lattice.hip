@@ -598,6 +598,21 @@ __global__ void csr_fill_k(long long n, const uint16_t *__restrict__ sid, const 
     val[i] = st[sid[i]];
 }
 
+// the same, four entries per thread (8-byte id loads, 16-byte value stores;
+// measured: non-temporal stores are slower here); sid and val 16-byte
+// aligned, entries 4 i .. 4 i + 3 for i < n4
+__global__ void csr_fill4_k(long long n4, const uint16_t *__restrict__ sid, const double *__restrict__ st,
+                            double *__restrict__ val) {
+  typedef unsigned short us4 __attribute__((ext_vector_type(4)));
+  typedef double d2 __attribute__((ext_vector_type(2)));
+  for (long long i = (long long)blockIdx.x * kB + threadIdx.x; i < n4; i += (long long)gridDim.x * kB) {
+    const us4 s4 = ((const us4 *)sid)[i];
+    const d2 lo = {st[s4.x], st[s4.y]}, hi = {st[s4.z], st[s4.w]};
+    ((d2 *)val)[2 * i] = lo;
+    ((d2 *)val)[2 * i + 1] = hi;
+  }
+}
+
 __global__ void rhs_fill_k(LatArgs a, long long n, const int *__restrict__ node2dof, const double *__restrict__ st,
                            double *__restrict__ rhs) {
   const long long i = (long long)blockIdx.x * kB + threadIdx.x;
@@ -1075,9 +1090,18 @@ hipError_t kl_fill_lattice_affine(hipStream_t s, DevLevel &L, double alpha_m, do
 
 void kl_fill_csr(hipStream_t s, const DevLevel &L, double *val) {
   // active rows come first in the CSR, and their entries are exactly the stencil entries
-  if (L.lat_nval > 0)
-    csr_fill_k<<<(int)std::min<int64_t>((L.lat_nval + kB - 1) / kB, 8192), kB, 0, s>>>(L.lat_nval, L.lsid, L.lst64,
-                                                                                        val);
+  const long long n = L.lat_nval;
+  if (n <= 0) return;
+  long long done = 0;
+  if ((((uintptr_t)val | (uintptr_t)L.lsid) & 15) == 0) {
+    const long long n4 = n / 4;
+    if (n4 > 0)
+      csr_fill4_k<<<(int)std::min<long long>((n4 + kB - 1) / kB, 4096), kB, 0, s>>>(n4, L.lsid, L.lst64, val);
+    done = 4 * n4;
+  }
+  if (done < n)
+    csr_fill_k<<<(int)std::min<long long>((n - done + kB - 1) / kB, 8192), kB, 0, s>>>(n - done, L.lsid + done,
+                                                                                      L.lst64, val + done);
 }
 
 template <int MODE>
