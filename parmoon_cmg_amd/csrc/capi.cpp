@@ -278,6 +278,7 @@ static void check_coeffs(const cd3d_coeffs &co) {
 // stencils of `val` behind), 2 rotating field (affine class stencils in
 // lst64), 0 cell by cell.
 static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int what, double *val) {
+  if (val == P.dl[l].val) P.dl[l].invdiag_ok = false;
   DevLevel &D = P.dl[l];
   AsmParams ap{};
   ap.what = what;
@@ -375,11 +376,12 @@ static void coarse_join(cd3d_problem &P) {
   P.coarse_pending = false;
 }
 
-// 1/a_ii and the FP32 lattice values of every level
+// after a change of the level matrices: the FP32 lattice values of every
+// level (1/a_ii of the FP64 form is recomputed on its first use)
 static void refresh_f32(cd3d_problem &P) {
   for (size_t l = 0; l < P.dl.size(); l++) {
     DevLevel &D = P.dl[l];
-    k_invdiag(P.stream, D);
+    D.invdiag_ok = false;  // only the FP64 smoother reads it: computed there on first use
     if (l == 0) D.lat_fresh = 0;  // the coarsest level has no FP32 stencils
     if (l > 0) {
       auto per_node = [&](bool with_map) {  // per-node FP32 stencils (4 bytes per nonzero), on first need
@@ -457,6 +459,10 @@ void cycle64(VC &v, int i, bool zero_start) {
   }
   DevLevel &L = P.dl[i], &C = P.dl[i - 1];
   double *x = L.sol, *t = L.tmp;
+  if (!L.invdiag_ok) {
+    k_invdiag(P.stream, L);
+    L.invdiag_ok = true;
+  }
   for (int k = 0; k < v.sp.pre; k++) {
     if (k == 0 && zero_start) k_jacobi_zero(P.stream, L, L.rhs, t, v.sp.omega);
     else k_jacobi(P.stream, L, x, L.rhs, t, v.sp.omega);
@@ -483,6 +489,13 @@ void cycle64(VC &v, int i, bool zero_start) {
 // of every preconditioner rhs and iterate vanishes)
 void coarseL(VC &v, bool zero_start) {
   DevLevel &C = v.P.dl[0];
+  if (zero_start && v.sp.coarse_smoother == 17) {
+    // inside the preconditioner the Dirichlet data are zero, so the defect of
+    // the zero iterate is the rhs: one launch, same arithmetic as below
+    coarse_join(v.P);
+    k_coarse_from_lattice(v.P.stream, C, v.P.cinv, C.lf, C.lx);
+    return;
+  }
   kl_to_dof(v.P.stream, C, C.lf, C.rhs);
   if (!zero_start) kl_to_dof(v.P.stream, C, C.lx, C.sol);
   coarse64(v, zero_start);
@@ -1144,6 +1157,7 @@ static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_sol
   for (int l = 0; l <= Lf; l++) {
     DevLevel &D = P.dl[l];
     k_matadd(P.stream, P.hl[l].rowptr[D.nact], D.val, D.aconv, -tau * th1);
+    D.invdiag_ok = false;
   }
   P.tcd_t += tau;
   return its;

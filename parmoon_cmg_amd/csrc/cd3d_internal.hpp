@@ -77,7 +77,8 @@ struct DevLevel {
   // row blocks of the stream kernels (at most 2048 nonzeros and 256 rows)
   int *blk = nullptr;
   int nblk = 0;
-  double *invdiag = nullptr;  // 1 / a_ii
+  double *invdiag = nullptr;  // 1 / a_ii (valid when invdiag_ok)
+  bool invdiag_ok = false;    // computed on first use after each change of val
   double *sol = nullptr, *rhs = nullptr, *aux = nullptr, *tmp = nullptr;
   double *rhs_asm = nullptr;  // assembled rhs (the multigrid works in `rhs`)
   // TCD3D: conv-diff matrix next to the system (mass) matrix in `val`
@@ -270,6 +271,9 @@ void kl_to_dof(hipStream_t s, const DevLevel &L, const float *l, double *v);
 // scratch: gj_scratch_doubles(na) doubles; inv: na^2 doubles
 void k_invert_active(hipStream_t s, const DevLevel &L, double *scratch, double *inv);
 int64_t gj_scratch_doubles(int na);
+// x_A = A_AA^-1 f_A on the FP32 lattice vectors of the coarsest level (zero
+// iterate, zero Dirichlet data)
+void k_coarse_from_lattice(hipStream_t s, const DevLevel &C, const double *Ainv, const float *lf, float *lx);
 // the inversion takes more than one launch (larger than the one-workgroup LDS form)
 bool k_invert_multi_launch(int na);
 void k_multi_axpy(hipStream_t s, int n, int m, const double *coef, const double *Z, int64_t ldz, double *x);

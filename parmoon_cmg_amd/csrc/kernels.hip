@@ -491,6 +491,23 @@ __global__ __launch_bounds__(kBlock) void coarse_update_k(int na, const double *
   if (lane == 0) x[row] += s;
 }
 
+// Mixed-form exact coarse solve from a zero iterate with zero Dirichlet data
+// (the V-cycle's visit of the coarsest level): x_A = A_AA^-1 f_A, read from
+// and written to the FP32 lattice vectors.  The defect f - A x of a zero
+// iterate is f itself, so these are the products and the summation order of
+// lat_to_dof_k, csr_stream<kDefect> and coarse_update_k into a zeroed x and
+// dof_to_lat_k, in one launch instead of six.
+__global__ __launch_bounds__(kBlock) void coarse_lat_k(int na, const double *__restrict__ Ainv,
+                                                       const int *__restrict__ dof2node, const float *__restrict__ lf,
+                                                       float *__restrict__ lx) {
+  const int row = (blockIdx.x * kBlock + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  if (row >= na) return;
+  double s = 0;
+  for (int j = lane; j < na; j += 64) s += Ainv[(int64_t)row * na + j] * (double)lf[dof2node[j]];
+  s = group_sum<64>(s);
+  if (lane == 0) lx[dof2node[row]] = (float)(0.0 + s);
+}
+
 // Coarse-level smoothing loop of TMultiGrid3D::Smooth(CoarseSmooth)
 // (MultiGrid3D.C:197-260) with SSOR (MGLevel3D.C:337-441) or Jacobi sweeps,
 // run by a single wavefront: rows are updated one after another, each row's
@@ -1160,6 +1177,10 @@ void k_copy(hipStream_t s, int n, const double *x, double *y) { copy_k<<<grid_fo
 void k_fill(hipStream_t s, int64_t n, double v, double *y) { fill_k<<<grid_for(n), kBlock, 0, s>>>(n, v, y); }
 void k_matadd(hipStream_t s, int64_t n, double *C, const double *D, double alpha) {
   matadd_k<<<grid_for(n), kBlock, 0, s>>>(n, C, D, alpha);
+}
+void k_coarse_from_lattice(hipStream_t s, const DevLevel &C, const double *Ainv, const float *lf, float *lx) {
+  if (C.nact > 0)
+    coarse_lat_k<<<(C.nact * 64 + kBlock - 1) / kBlock, kBlock, 0, s>>>(C.nact, Ainv, C.ldof2node, lf, lx);
 }
 void k_gemv_dense(hipStream_t s, int n, const double *A, const double *x, double *y) {
   if (n > 0) coarse_update_k<<<(n * 64 + kBlock - 1) / kBlock, kBlock, 0, s>>>(n, A, x, y);
