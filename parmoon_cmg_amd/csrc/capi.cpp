@@ -415,10 +415,35 @@ static void refresh_f32(cd3d_problem &P) {
 
 namespace {
 
+// Mixed V-cycles: the bottom levels, up to the highest level with at most
+// this many lattice nodes (and at most kSmallMaxLevels of them above an exact
+// coarse solve of at most kSmallCoarseMax unknowns), run as one launch
+// (kl_small_vcycle).  One workgroup pays off only while a level's nodes take
+// about one round of the CU (measured: 4913-node Q2 levels and a 343-unknown
+// coarse solve are several times slower than the launch-per-step form).  CD3D_SMALL_NODES overrides the node bound (0: off),
+// read once per solve.
+constexpr int kSmallMaxLevels = 3, kSmallCoarseMax = 128;
+static long long small_nodes_bound() {
+  const char *e = getenv("CD3D_SMALL_NODES");
+  return e ? atoll(e) : 3000LL;
+}
+
+static int small_top(const cd3d_problem &P, const cd3d_solver_params &sp) {
+  if (sp.cycle != 1 || sp.coarse_smoother != 17 || P.dl.empty() || P.dl[0].nact > kSmallCoarseMax) return 0;
+  int top = 0;
+  for (int i = 1; i < (int)P.dl.size() && i <= kSmallMaxLevels; i++) {
+    const long long nn = P.dl[i].nn;
+    if (nn * nn * nn > small_nodes_bound() || P.dl[i].nclass <= 0) break;
+    top = i;
+  }
+  return top;
+}
+
 struct VC {
   cd3d_problem &P;
   const cd3d_solver_params &sp;
-  int rec[32];  // mg_recursions (TMultiGrid3D::SetRecursion, MultiGrid3D.C:690-703)
+  int rec[32];     // mg_recursions (TMultiGrid3D::SetRecursion, MultiGrid3D.C:690-703)
+  int small = -1;  // the level whose zero-start visit runs as one launch (mixed V-cycle; 0: none)
   VC(cd3d_problem &p, const cd3d_solver_params &s) : P(p), sp(s) { reset(); }
   void reset() {
     rec[1] = 1;
@@ -506,6 +531,13 @@ void cycleL(VC &v, int i, bool zero_start) {
   cd3d_problem &P = v.P;
   if (i == 0) {
     coarseL(v, zero_start);
+    return;
+  }
+  if (v.small < 0) v.small = small_top(P, v.sp);
+  if (zero_start && i == v.small) {
+    coarse_join(P);
+    kl_small_vcycle(P.stream, P.dl.data(), i, P.mesh.order, v.sp.pre, v.sp.post, (float)v.sp.omega,
+                    (float)v.sp.gmg_damp, P.cinv);
     return;
   }
   DevLevel &L = P.dl[i], &C = P.dl[i - 1];

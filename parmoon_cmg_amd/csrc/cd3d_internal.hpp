@@ -267,6 +267,10 @@ void kl_cast_d2f(hipStream_t s, int64_t n, const double *a, float *b);
 void kl_cast_f2d(hipStream_t s, int64_t n, const float *a, double *b);
 // DOF-ordered f64 <-> lattice f32 (Dirichlet DOFs map to zero)
 void kl_from_dof(hipStream_t s, const DevLevel &L, const double *v, float *l);
+// zero-start mixed V-cycle of levels top..1 and the exact coarse solve in
+// one launch (levels[0..top]; 1 <= top <= 3)
+void kl_small_vcycle(hipStream_t s, const DevLevel *levels, int top, int order, int pre, int post, float omega,
+                     float damp, const double *Ainv);
 void kl_to_dof(hipStream_t s, const DevLevel &L, const float *l, double *v);
 // scratch: gj_scratch_doubles(na) doubles; inv: na^2 doubles
 void k_invert_active(hipStream_t s, const DevLevel &L, double *scratch, double *inv);

@@ -442,6 +442,129 @@ __global__ __launch_bounds__(kB) void lat_prolong_k(int ncc, int nnf, int nnc, c
   if (t < (long long)ncc * ncc * ncc) prolong_cell<P>(t, ncc, nnf, nnc, coarse, x, damp);
 }
 
+// ---------------------------------------------------------------- small levels
+// The bottom of a mixed V-cycle in one launch: one workgroup runs the
+// pre-smoothing, defect and restriction of the small FP32 levels top..1, the
+// exact coarse solve x_A = A_AA^-1 f_A (zero iterate, zero Dirichlet data, as
+// coarse_lat_k), and the prolongation and post-smoothing back up, with a
+// workgroup barrier between steps.  Each step is the per-node code of the
+// multi-workgroup kernels (lat_class_node, restrict_class_node,
+// prolong_cell) over the same nodes, and the iterates ping-pong between lx
+// and lt exactly as the host loop does, so the results are bitwise those of
+// the launch-per-step V-cycle.  On small levels every launch is mostly fixed
+// cost; one launch replaces 9 per level plus the coarse solve.
+constexpr int kSmallMax = 3;  // FP32 levels in one program
+// workgroup size: Q2's 125-point stencils need more than 128 registers per
+// lane (1024 threads) without spilling
+template <int P>
+constexpr int small_threads() {
+  return P == 1 ? 1024 : 512;
+}
+
+struct SmallLevel {
+  LatArgs a;
+  const float *val, *inv, *cst;
+  float *lx, *lt, *la, *lf;
+};
+
+struct SmallArgs {
+  SmallLevel L[kSmallMax + 1];  // L[0]: the coarsest level (a, lx, lf), L[1..top]: FP32 levels
+  int top, pre, post, na;
+  float omega, damp;
+  const double *Ainv;
+  const int *dof2node0;
+};
+static_assert(sizeof(SmallArgs) <= 4096, "kernel arguments are limited to 4 KiB");
+
+// UNI: every class of the level has compact stencils (else per-node values)
+template <int P, int MODE, bool UNI>
+__device__ inline void small_sweep(const SmallLevel &L, const float *x, const float *f, float *out, float omega) {
+  for (int c = 0; c < L.a.nclass; c++) {
+    const float *cst = L.cst + c * kCst;
+    for (long long k = threadIdx.x; k < L.a.c[c].nodes; k += small_threads<P>()) {
+      if (P == 1) {
+        lat_node<1, 0, 0, 0, MODE, UNI>(L.a, L.a.c[c], k, L.val, L.inv, cst, x, f, out, omega);
+      } else {
+        switch (c) {
+          case 0: lat_node<2, 0, 0, 0, MODE, UNI>(L.a, L.a.c[c], k, L.val, L.inv, cst, x, f, out, omega); break;
+          case 1: lat_node<2, 1, 0, 0, MODE, UNI>(L.a, L.a.c[c], k, L.val, L.inv, cst, x, f, out, omega); break;
+          case 2: lat_node<2, 0, 1, 0, MODE, UNI>(L.a, L.a.c[c], k, L.val, L.inv, cst, x, f, out, omega); break;
+          case 3: lat_node<2, 1, 1, 0, MODE, UNI>(L.a, L.a.c[c], k, L.val, L.inv, cst, x, f, out, omega); break;
+          case 4: lat_node<2, 0, 0, 1, MODE, UNI>(L.a, L.a.c[c], k, L.val, L.inv, cst, x, f, out, omega); break;
+          case 5: lat_node<2, 1, 0, 1, MODE, UNI>(L.a, L.a.c[c], k, L.val, L.inv, cst, x, f, out, omega); break;
+          case 6: lat_node<2, 0, 1, 1, MODE, UNI>(L.a, L.a.c[c], k, L.val, L.inv, cst, x, f, out, omega); break;
+          default: lat_node<2, 1, 1, 1, MODE, UNI>(L.a, L.a.c[c], k, L.val, L.inv, cst, x, f, out, omega); break;
+        }
+      }
+    }
+  }
+}
+
+// UNI: every small level has compact stencils in every class
+template <int P, bool UNI>
+__global__ __launch_bounds__(small_threads<P>()) void lat_small_k(SmallArgs s) {
+  constexpr int NT = small_threads<P>();
+  float *xs[kSmallMax + 1];
+  // down: levels top .. 1
+  for (int i = s.top; i >= 1; i--) {
+    const SmallLevel &L = s.L[i], &C = s.L[i - 1];
+    float *x = L.lx, *t = L.lt;
+    for (int k = 0; k < s.pre; k++) {
+      if (k == 0) small_sweep<P, kLJacobiZero, UNI>(L, nullptr, L.lf, t, s.omega);
+      else small_sweep<P, kLJacobi, UNI>(L, x, L.lf, t, s.omega);
+      float *u = x;
+      x = t;
+      t = u;
+      __syncthreads();
+    }
+    if (s.pre == 0) {
+      const long long n3 = (long long)L.a.nn * L.a.nn * L.a.nn;
+      for (long long k = threadIdx.x; k < n3; k += NT) x[k] = 0.f;
+      __syncthreads();
+    }
+    small_sweep<P, kLDefect, UNI>(L, x, L.lf, L.la, 0.f);
+    __syncthreads();
+    for (int c = 0; c < C.a.nclass; c++)
+      for (long long k = threadIdx.x; k < C.a.c[c].nodes; k += NT)
+        restrict_class_node<P>(C.a, c, k, L.a.nn, L.la, C.lf);
+    xs[i] = x;
+    __syncthreads();
+  }
+  // coarsest: one wavefront per row, the lanes over the columns
+  {
+    const int lane = threadIdx.x & 63;
+    for (int row = threadIdx.x >> 6; row < s.na; row += NT / 64) {
+      double v = 0;
+      for (int j = lane; j < s.na; j += 64) v += s.Ainv[(long long)row * s.na + j] * (double)s.L[0].lf[s.dof2node0[j]];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (lane == 0) s.L[0].lx[s.dof2node0[row]] = (float)(0.0 + v);
+    }
+    __syncthreads();
+  }
+  // up: levels 1 .. top
+  for (int i = 1; i <= s.top; i++) {
+    const SmallLevel &L = s.L[i], &C = s.L[i - 1];
+    float *x = xs[i], *t = x == L.lx ? L.lt : L.lx;
+    const int ncc = (C.a.nn - 1) / P;
+    for (long long c = threadIdx.x; c < (long long)ncc * ncc * ncc; c += NT)
+      prolong_cell<P>(c, ncc, L.a.nn, C.a.nn, C.lx, x, s.damp);
+    __syncthreads();
+    for (int k = 0; k < s.post; k++) {
+      small_sweep<P, kLJacobi, UNI>(L, x, L.lf, t, s.omega);
+      float *u = x;
+      x = t;
+      t = u;
+      __syncthreads();
+    }
+    if (x != L.lx) {
+      const long long n3 = (long long)L.a.nn * L.a.nn * L.a.nn;
+      for (long long k = threadIdx.x; k < n3; k += NT) L.lx[k] = x[k];
+      __syncthreads();
+    }
+  }
+}
+
 __global__ void lat_convert_k(long long n, const int *__restrict__ perm, const double *__restrict__ val,
                               float *__restrict__ out) {
   for (long long i = (long long)blockIdx.x * kB + threadIdx.x; i < n; i += (long long)gridDim.x * kB)
@@ -1136,6 +1259,38 @@ void kl_cast_d2f(hipStream_t s, int64_t n, const double *a, float *b) {
 }
 void kl_cast_f2d(hipStream_t s, int64_t n, const float *a, double *b) {
   cast_f2d_k<<<(int)std::min<int64_t>((n + kB - 1) / kB, 2048), kB, 0, s>>>(n, a, b);
+}
+
+void kl_small_vcycle(hipStream_t st, const DevLevel *levels, int top, int order, int pre, int post, float omega,
+                     float damp, const double *Ainv) {
+  if (top < 1 || top > kSmallMax) return;
+  SmallArgs s{};
+  for (int i = 0; i <= top; i++) {
+    const DevLevel &D = levels[i];
+    s.L[i].a = lat_args(D);
+    s.L[i].val = D.lval;
+    s.L[i].inv = D.linv;
+    s.L[i].cst = D.lcst;
+    s.L[i].lx = D.lx;
+    s.L[i].lt = D.lt;
+    s.L[i].la = D.la;
+    s.L[i].lf = D.lf;
+  }
+  s.top = top;
+  s.pre = pre;
+  s.post = post;
+  s.na = levels[0].nact;
+  s.omega = omega;
+  s.damp = damp;
+  s.Ainv = Ainv;
+  s.dof2node0 = levels[0].ldof2node;
+  bool uni = true;
+  for (int i = 1; i <= top; i++)
+    for (int c = 0; c < s.L[i].a.nclass; c++) uni = uni && s.L[i].a.uni[c];
+  if (order == 1 && uni) lat_small_k<1, true><<<1, small_threads<1>(), 0, st>>>(s);
+  else if (order == 1) lat_small_k<1, false><<<1, small_threads<1>(), 0, st>>>(s);
+  else if (uni) lat_small_k<2, true><<<1, small_threads<2>(), 0, st>>>(s);
+  else lat_small_k<2, false><<<1, small_threads<2>(), 0, st>>>(s);
 }
 
 void kl_from_dof(hipStream_t s, const DevLevel &L, const double *v, float *l) {

@@ -234,11 +234,40 @@ def test_mixed_vcycle_against_fp64(order, m, k0, levels, conv_kind):
     assert np.linalg.norm(x - x64) / np.linalg.norm(x64) < 1e-9
 
 
+@pytest.mark.parametrize("order,m,k0,levels,conv_kind", [(2, 1, 1, 3, 0), (2, 1, 1, 3, 1), (1, 1, 2, 4, 0),
+                                                          (1, 2, 1, 3, 1), (2, 1, 2, 4, 0)])
+def test_small_level_program_matches_launch_per_step(order, m, k0, levels, conv_kind, monkeypatch):
+    # the bottom of the mixed V-cycle as one workgroup program (levels of at
+    # most CD3D_SMALL_NODES lattice nodes and the exact coarse solve) against
+    # the launch-per-step V-cycle: the same arithmetic, so bitwise equal.
+    # Cases: the whole V-cycle in one launch (Q2 compact / per-node stencils),
+    # three Q1 levels, the Q1 rotating field, and a fused bottom below
+    # launch-per-step levels
+    P = pm.Problem(order, m, k0, levels, box=(1.0, 0.7, 0.5))
+    P.assemble(pm.coeffs(eps=1e-2, b=(1.0, 0.5, 0.25), supg=1, f_val=1.0, conv_kind=conv_kind, t=0.1))
+    i = P.info(levels - 1)
+    r = np.random.default_rng(9).standard_normal(i["ndof"])
+    r[i["nact"]:] = 0.0
+    sp = pm.solver_params(**dict(REF_SOLVER, omega=0.5, pre=3, post=3, coarse_smoother=17, rel_tol=1e-10),
+                          precision=pm.MIXED)
+    out = {}
+    for bound in ("0", "100000"):
+        monkeypatch.setenv("CD3D_SMALL_NODES", bound)
+        z = P.vcycle(sp, r)
+        x, st = P.solve(sp)
+        out[bound] = (z, x, st.iterations)
+    say(f"[gpu] small-level program order={order} m={m} k0={k0} levels={levels} conv_kind={conv_kind}: "
+        f"its {out['0'][2]} / {out['100000'][2]}")
+    assert np.array_equal(out["0"][0], out["100000"][0])
+    assert np.array_equal(out["0"][1], out["100000"][1])
+    assert out["0"][2] == out["100000"][2]
+
+
 @pytest.mark.parametrize("order,m,k0", [(1, 1, 2), (2, 3, 0), (1, 1, 3), (2, 1, 1)])
 def test_coarse_inverse_solves_the_coarse_system(order, m, k0):
     # one level: the preconditioner is the exact coarse solve (SolveExact);
     # 27 and 125 unknowns take the single-workgroup LDS elimination, 343 the
-    # one-launch-per-pivot elimination
+    # blocked elimination (panels of 16 pivots)
     P = pm.Problem(order, m, k0, 1, box=(1.0, 0.8, 0.6))
     P.assemble(pm.coeffs(eps=1e-3, b=(1.0, -0.4, 0.3), c=0.5, supg=1, f_val=1.0))
     i = P.info(0)
