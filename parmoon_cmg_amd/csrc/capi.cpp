@@ -618,7 +618,10 @@ struct Space {
   double *x, *b;
   std::function<void(const double *, const double *, double *)> defect;
   std::function<void(const double *, double *)> matvec;
-  std::function<void(VC &, const double *, double *)> precond;
+  // precond(v, r, z, r32_ready): z = M^-1 r; r32_ready: r32 already holds
+  // (float) r, written by the kernel that produced r
+  std::function<void(VC &, const double *, double *, bool)> precond;
+  float *r32 = nullptr;  // FP32 copy of the preconditioner input (lattice space)
 };
 
 Space dof_space(cd3d_problem &P) {
@@ -629,7 +632,7 @@ Space dof_space(cd3d_problem &P) {
   S.b = P.bs;
   S.defect = [&P, &F](const double *x, const double *f, double *d) { k_defect(P.stream, F, x, f, d); };
   S.matvec = [&P, &F](const double *x, double *y) { k_spmv(P.stream, F, x, y); };
-  S.precond = [](VC &v, const double *r, double *z) { precondition(v, r, z, true); };
+  S.precond = [](VC &v, const double *r, double *z, bool) { precondition(v, r, z, true); };
   return S;
 }
 
@@ -643,9 +646,10 @@ Space lattice_space(cd3d_problem &P) {
   S.defect = [&P, &F, p](const double *x, const double *f, double *d) { kl_defect64(P.stream, F, p, x, f, d); };
   S.matvec = [&P, &F, p](const double *x, double *y) { kl_spmv64(P.stream, F, p, x, y); };
   // the Krylov vectors vanish on the boundary, as the V-cycle's rhs must
-  S.precond = [&P, &F, top](VC &v, const double *r, double *z) {
+  S.r32 = F.lf;
+  S.precond = [&P, &F, top](VC &v, const double *r, double *z, bool r32_ready) {
     v.reset();
-    kl_cast_d2f(P.stream, F.ndof, r, F.lf);
+    if (!r32_ready) kl_cast_d2f(P.stream, F.ndof, r, F.lf);
     cycleL(v, top, true);
     kl_cast_f2d(P.stream, F.ndof, F.lx, z);
   };
@@ -688,12 +692,12 @@ int fgmres(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S, double
     HIPCHK(hipStreamSynchronize(P.stream));  // pinned y is reused next time
   };
   while (j <= sp.maxit) {
-    k_scale(P.stream, N, 1.0 / beta, Vk(0));  // Dscal(N, 1/beta, v[0])
+    k_scale_cast(P.stream, N, 1.0 / beta, Vk(0), S.r32);  // Dscal(N, 1/beta, v[0]), and its FP32 copy
     std::fill(s.begin(), s.end(), 0.0);
     s[0] = beta;
     int i;
     for (i = 0; i < R && j <= sp.maxit; i++, j++) {
-      S.precond(v, Vk(i), Zk(i));
+      S.precond(v, Vk(i), Zk(i), S.r32 != nullptr);
       S.matvec(Zk(i), w);
       k_dot(P.stream, N, w, Vk(0), pa, nullptr);
       // modified Gram-Schmidt: H[k][i] = <w, v_k>; w -= H[k][i] v_k, fused with the next dot
@@ -702,7 +706,7 @@ int fgmres(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S, double
         k_axpy_dot(P.stream, N, in, -1.0, Vk(k), w, k < i ? Vk(k + 1) : nullptr, out, P.dscal + k);
       }
       double *last = (i % 2 == 0) ? pb : pa;
-      k_scale_copy(P.stream, N, last, P.dscal + i + 1, w, Vk(i + 1));
+      k_scale_copy(P.stream, N, last, P.dscal + i + 1, w, Vk(i + 1), S.r32);
       HIPCHK(hipMemcpyAsync(P.hscal, P.dscal, sizeof(double) * (i + 2), hipMemcpyDeviceToHost, P.stream));
       HIPCHK(hipStreamSynchronize(P.stream));
       for (int k = 0; k <= i + 1; k++) H[(size_t)k * R + i] = P.hscal[k];

@@ -209,7 +209,11 @@ void k_coarse_smooth(hipStream_t s, const DevLevel &L, double *x, const double *
 void k_dot(hipStream_t s, int n, const double *a, const double *b, double *partials, double *out);
 void k_axpy_dot(hipStream_t s, int n, const double *alpha_partials, double sign, const double *x, double *y,
                 const double *z, double *partials, double *alpha_out);
-void k_scale_copy(hipStream_t s, int n, const double *partials, double *norm_out, const double *x, double *y);
+// y32 (optional): also (float) y
+void k_scale_copy(hipStream_t s, int n, const double *partials, double *norm_out, const double *x, double *y,
+                  float *y32 = nullptr);
+// x *= alpha, and y32 = (float) x when y32 is given
+void k_scale_cast(hipStream_t s, int n, double alpha, double *x, float *y32);
 void k_axpy(hipStream_t s, int n, double alpha, const double *x, double *y);
 void k_scale(hipStream_t s, int n, double alpha, double *x);
 void k_axpy_dev(hipStream_t s, int n, const double *alpha_dev, int idx, const double *x, double *y);

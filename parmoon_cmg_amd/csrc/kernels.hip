@@ -424,7 +424,7 @@ __global__ __launch_bounds__(kBlock) void scale_copy_k(int n, const double *__re
 
 __global__ __launch_bounds__(kBlock) void scale_copy2_k(int n, const double *__restrict__ partials,
                                                         double *__restrict__ norm_out, const double *__restrict__ x,
-                                                        double *__restrict__ y) {
+                                                        double *__restrict__ y, float *__restrict__ y32) {
   __shared__ double sh[17];
   const int n2 = n >> 1, gs = gridDim.x * kBlock;
   const double2 *x2 = (const double2 *)x;
@@ -434,12 +434,31 @@ __global__ __launch_bounds__(kBlock) void scale_copy2_k(int n, const double *__r
   const double nrm = sqrt(sum_partials(partials, sh));
   if (blockIdx.x == 0 && threadIdx.x == 0) *norm_out = nrm;
   const double s = nrm != 0.0 ? 1.0 / nrm : 1.0;
-  if (i < n2) y2[i] = double2{v0.x * s, v0.y * s};
+  typedef float f2 __attribute__((ext_vector_type(2)));
+  if (i < n2) {
+    const double2 r = {v0.x * s, v0.y * s};
+    y2[i] = r;
+    if (y32) ((f2 *)y32)[i] = f2{(float)r.x, (float)r.y};
+  }
   for (i += gs; i < n2; i += gs) {
     const double2 v = x2[i];
-    y2[i] = double2{v.x * s, v.y * s};
+    const double2 r = {v.x * s, v.y * s};
+    y2[i] = r;
+    if (y32) ((f2 *)y32)[i] = f2{(float)r.x, (float)r.y};
   }
-  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) y[n - 1] = x[n - 1] * s;
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    y[n - 1] = x[n - 1] * s;
+    if (y32) y32[n - 1] = (float)(x[n - 1] * s);
+  }
+}
+
+// x *= alpha and, when y32 is given, y32 = (float) x (scalar loop)
+__global__ void scale_cast_k(int n, double alpha, double *__restrict__ x, float *__restrict__ y32) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const double v = x[i] * alpha;
+    x[i] = v;
+    if (y32) y32[i] = (float)v;
+  }
 }
 
 __global__ void scale_k(int n, double alpha, double *__restrict__ x) {
@@ -1162,9 +1181,17 @@ void k_axpy_dot(hipStream_t s, int n, const double *alpha_partials, double sign,
     axpy_dot_k<<<kRedBlocks, kBlock, 0, s>>>(n, alpha_partials, alpha_out, sign, x, y, z, partials);
   }
 }
-void k_scale_copy(hipStream_t s, int n, const double *partials, double *norm_out, const double *x, double *y) {
-  if (al16(x) && al16(y)) scale_copy2_k<<<kRedBlocks, kBlock, 0, s>>>(n, partials, norm_out, x, y);
-  else scale_copy_k<<<kRedBlocks, kBlock, 0, s>>>(n, partials, norm_out, x, y);
+void k_scale_copy(hipStream_t s, int n, const double *partials, double *norm_out, const double *x, double *y,
+                  float *y32) {
+  if (al16(x) && al16(y) && ((uintptr_t)y32 & 7) == 0) {
+    scale_copy2_k<<<kRedBlocks, kBlock, 0, s>>>(n, partials, norm_out, x, y, y32);
+  } else {
+    scale_copy_k<<<kRedBlocks, kBlock, 0, s>>>(n, partials, norm_out, x, y);
+    if (y32) scale_cast_k<<<grid_for(n), kBlock, 0, s>>>(n, 1.0, y, y32);
+  }
+}
+void k_scale_cast(hipStream_t s, int n, double alpha, double *x, float *y32) {
+  scale_cast_k<<<grid_for(n), kBlock, 0, s>>>(n, alpha, x, y32);
 }
 void k_scale(hipStream_t s, int n, double alpha, double *x) { scale_k<<<grid_for(n), kBlock, 0, s>>>(n, alpha, x); }
 void k_axpy(hipStream_t s, int n, double alpha, const double *x, double *y) {

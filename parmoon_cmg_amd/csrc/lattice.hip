@@ -627,6 +627,11 @@ __global__ void lat_invdiag_k(long long n, const int *__restrict__ node2dof, con
 template <int P>
 __global__ __launch_bounds__(1024) void elem_k(const double *__restrict__ tab, AsmParams ap, double *__restrict__ E) {
   constexpr int NB = (P + 1) * (P + 1) * (P + 1), N1 = P + 2, NQ = N1 * N1 * N1, STRIDE = 4 + 4 * NB;
+  // the basis table (values and reference gradients at the quadrature
+  // points) is read by every entry: staged in LDS once
+  __shared__ double Ts[NQ * STRIDE];
+  for (int k = threadIdx.x; k < NQ * STRIDE; k += blockDim.x) Ts[k] = tab[k];
+  __syncthreads();
   const int t = threadIdx.x;
   if (t >= NB * NB) return;
   const int i = t / NB, j = t % NB;
@@ -640,7 +645,7 @@ __global__ __launch_bounds__(1024) void elem_k(const double *__restrict__ tab, A
   const double delta = (ap.supg && ap.what == 0) ? ap.delta_const : 0.0;
   double acc = 0.0, racc = 0.0;
   for (int q = 0; q < NQ; q++) {
-    const double *T = tab + q * STRIDE;
+    const double *T = Ts + q * STRIDE;
     const double w = T[0] * detj;
     const double pi_ = T[4 + 4 * i], gi0 = T[5 + 4 * i], gi1 = T[6 + 4 * i], gi2 = T[7 + 4 * i];
     const double pj = T[4 + 4 * j], gj0 = T[5 + 4 * j], gj1 = T[6 + 4 * j], gj2 = T[7 + 4 * j];
@@ -761,6 +766,11 @@ template <int P>
 __global__ __launch_bounds__(1024) void elem_affine_k(const double *__restrict__ tab, AsmParams ap,
                                                       double *__restrict__ E) {
   constexpr int NB = (P + 1) * (P + 1) * (P + 1), N1 = P + 2, NQ = N1 * N1 * N1, STRIDE = 4 + 4 * NB;
+  // the basis table (values and reference gradients at the quadrature
+  // points) is read by every entry: staged in LDS once
+  __shared__ double Ts[NQ * STRIDE];
+  for (int k = threadIdx.x; k < NQ * STRIDE; k += blockDim.x) Ts[k] = tab[k];
+  __syncthreads();
   const int t = threadIdx.x;
   if (t >= NB * NB) return;
   const int i = t / NB, j = t % NB;
@@ -774,7 +784,7 @@ __global__ __launch_bounds__(1024) void elem_affine_k(const double *__restrict__
   const double b3 = 0.1 * ap.sin2pit;
   double a0 = 0.0, ax = 0.0, ay = 0.0, racc = 0.0;
   for (int q = 0; q < NQ; q++) {
-    const double *T = tab + q * STRIDE;
+    const double *T = Ts + q * STRIDE;
     const double w = T[0] * detj;
     const double dxq = T[1] * ap.h[0] / 2, dyq = T[2] * ap.h[1] / 2;  // qp - cell centre
     const double pi_ = T[4 + 4 * i], gi0 = T[5 + 4 * i], gi1 = T[6 + 4 * i], gi2 = T[7 + 4 * i];
@@ -976,11 +986,22 @@ __host__ __device__ inline void lat64_pair_node(const LatArgs &a, int pc, long l
   }
 }
 
+// blocks pstart[4].. take the Dirichlet rows (DOFs nact..ndof-1, identity),
+// as lat64_boundary_k
 template <int MODE>
 __global__ __launch_bounds__(kB) void lat64_pair_k(LatArgs a, const double *__restrict__ st64,
                                                    const double *__restrict__ x, const double *__restrict__ f,
-                                                   double *__restrict__ out) {
+                                                   double *__restrict__ out, int nact, int ndof,
+                                                   const int *__restrict__ dof2node) {
   __shared__ double cs[2 * kCst];
+  if ((int)blockIdx.x >= a.pstart[4]) {
+    const int d = nact + (blockIdx.x - a.pstart[4]) * kB + threadIdx.x;
+    if (d < ndof) {
+      const int node = dof2node[d];
+      out[node] = MODE == kL64Defect ? f[node] - x[node] : x[node];
+    }
+    return;
+  }
   int pc = 0;
   while (pc < 3 && (int)blockIdx.x >= a.pstart[pc + 1]) pc++;
   cs[threadIdx.x] = st64[2 * pc * kCst + threadIdx.x];  // classes 2 pc and 2 pc + 1
@@ -1231,12 +1252,16 @@ template <int MODE>
 void lat64_apply(hipStream_t s, const DevLevel &L, int order, const double *x, const double *f, double *out) {
   const LatArgs a = lat_args(L);
   const int grid = a.bstart[L.nclass];
+  const int nd = L.ndof - L.nact;
+  if (order == 2 && grid > 0 && lat_pairs() && L.nn > kPairMinNN) {  // interior and Dirichlet rows in one launch
+    lat64_pair_k<MODE><<<a.pstart[4] + (nd + kB - 1) / kB, kB, 0, s>>>(a, L.lst64, x, f, out, L.nact, L.ndof,
+                                                                       L.ldof2node);
+    return;
+  }
   if (grid > 0) {
     if (order == 1) lat64_k<1, MODE><<<grid, kB, 0, s>>>(a, L.lst64, x, f, out);
-    else if (lat_pairs() && L.nn > kPairMinNN) lat64_pair_k<MODE><<<a.pstart[4], kB, 0, s>>>(a, L.lst64, x, f, out);
     else lat64_k<2, MODE><<<grid, kB, 0, s>>>(a, L.lst64, x, f, out);
   }
-  const int nd = L.ndof - L.nact;
   if (nd > 0)
     lat64_boundary_k<<<(nd + kB - 1) / kB, kB, 0, s>>>(L.nact, L.ndof, L.ldof2node, x, f, out,
                                                       MODE == kL64Defect ? 1 : 0);
