@@ -263,11 +263,14 @@ def test_small_level_program_matches_launch_per_step(order, m, k0, levels, conv_
     assert out["0"][2] == out["100000"][2]
 
 
-@pytest.mark.parametrize("order,m,k0", [(1, 1, 2), (2, 3, 0), (1, 1, 3), (2, 1, 1)])
+@pytest.mark.parametrize("order,m,k0", [(1, 1, 2), (2, 3, 0), (1, 1, 3), (2, 1, 1), (1, 12, 0), (2, 7, 0),
+                                         (1, 16, 0)])
 def test_coarse_inverse_solves_the_coarse_system(order, m, k0):
     # one level: the preconditioner is the exact coarse solve (SolveExact);
     # 27 and 125 unknowns take the single-workgroup LDS elimination, 343 the
-    # blocked elimination (panels of 16 pivots)
+    # blocked elimination (panels of 16 pivots, one panel row per thread),
+    # 1331 / 2197 / 3375 the blocked elimination with panels of 8 pivots and
+    # 2 / 3 / 4 rows per thread
     P = pm.Problem(order, m, k0, 1, box=(1.0, 0.8, 0.6))
     P.assemble(pm.coeffs(eps=1e-3, b=(1.0, -0.4, 0.3), c=0.5, supg=1, f_val=1.0))
     i = P.info(0)
