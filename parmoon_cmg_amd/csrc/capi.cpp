@@ -376,6 +376,13 @@ static void coarse_join(cd3d_problem &P) {
   P.coarse_pending = false;
 }
 
+// CD3D_SMALL_NODES overrides the node bound of the one-workgroup V-cycle
+// program (0: off), read once per solve and at each refresh
+static long long small_nodes_bound() {
+  const char *e = getenv("CD3D_SMALL_NODES");
+  return e ? atoll(e) : 3000LL;
+}
+
 // after a change of the level matrices: the FP32 lattice values of every
 // level (1/a_ii of the FP64 form is recomputed on its first use)
 static void refresh_f32(cd3d_problem &P) {
@@ -394,12 +401,21 @@ static void refresh_f32(cd3d_problem &P) {
           upload(P, D.lperm, P.hl[l].lat_perm);
         }
       };
-      if (D.lat_fresh == 2) {  // straight from the affine class stencils
+      const long long nn = D.nn;
+      if (D.lat_fresh == 2 && nn * nn * nn > small_nodes_bound()) {
+        // affine class stencils evaluated by the sweeps node by node (a level
+        // this large never runs in the one-workgroup program, which reads
+        // per-node values)
+        HIPCHK(kl_set_lattice_affine(P.stream, D, D.lat_alpha_m, D.lat_alpha));
+        D.lat_fresh = 0;
+        D.lat_exact64 = false;
+      } else if (D.lat_fresh == 2) {  // straight from the affine class stencils
         per_node(false);
         HIPCHK(kl_fill_lattice_affine(P.stream, D, D.lat_alpha_m, D.lat_alpha));
         D.lat_fresh = 0;
         D.lat_exact64 = false;
       } else {
+        D.lat_affine = false;
         bool all_uniform = false;
         HIPCHK(kl_detect_uniform(P.stream, D, P.hflag, &all_uniform));
         if (!all_uniform) {
@@ -420,20 +436,17 @@ namespace {
 // coarse solve of at most kSmallCoarseMax unknowns), run as one launch
 // (kl_small_vcycle).  One workgroup pays off only while a level's nodes take
 // about one round of the CU (measured: 4913-node Q2 levels and a 343-unknown
-// coarse solve are several times slower than the launch-per-step form).  CD3D_SMALL_NODES overrides the node bound (0: off),
-// read once per solve.
+// coarse solve are several times slower than the launch-per-step form).
+// The node bound is small_nodes_bound().
 constexpr int kSmallMaxLevels = 3, kSmallCoarseMax = 128;
-static long long small_nodes_bound() {
-  const char *e = getenv("CD3D_SMALL_NODES");
-  return e ? atoll(e) : 3000LL;
-}
 
 static int small_top(const cd3d_problem &P, const cd3d_solver_params &sp) {
   if (sp.cycle != 1 || sp.coarse_smoother != 17 || P.dl.empty() || P.dl[0].nact > kSmallCoarseMax) return 0;
   int top = 0;
   for (int i = 1; i < (int)P.dl.size() && i <= kSmallMaxLevels; i++) {
     const long long nn = P.dl[i].nn;
-    if (nn * nn * nn > small_nodes_bound() || P.dl[i].nclass <= 0) break;
+    // the program reads compact or per-node FP32 values (not the node-by-node affine form)
+    if (nn * nn * nn > small_nodes_bound() || P.dl[i].nclass <= 0 || P.dl[i].lat_affine) break;
     top = i;
   }
   return top;
@@ -1302,7 +1315,7 @@ cd3d_status cd3d_kernel_bench(cd3d_problem *p, int which, int level, int reps, d
         bool uni = true;
         for (int c = 0; c < L.nclass; c++) uni = uni && L.uni[c];
         launch = [&] { kl_jacobi(p->stream, L, o, L.lx, L.lf, L.lt, 0.5f); };
-        b = uni ? 12 * N : 12 * N + 4 * (double)L.lat_nval + 4 * nint;
+        b = uni || L.lat_affine ? 12 * N : 12 * N + 4 * (double)L.lat_nval + 4 * nint;
         break;
       }
       case 2:

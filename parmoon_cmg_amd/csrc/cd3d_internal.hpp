@@ -125,6 +125,10 @@ struct DevLevel {
   int lat_fresh = 0;
   double lat_alpha_m = 0.0, lat_alpha = 1.0;
   bool lat_exact64 = false;  // lst64 holds exactly the CSR values of the interior rows
+  // the FP32 sweeps evaluate the affine stencils node by node (lat_fresh 2
+  // on a level too large for the one-workgroup program): lval / linv unused
+  bool lat_affine = false;
+  int lat_centre[8] = {0};  // stencil index of the offset (0, 0, 0) per class
   bool tcd_mass_uniform = false;  // TCD3D: the mass class stencils are in lst64 (mass slot)
 };
 
@@ -260,6 +264,9 @@ void kl_assemble_affine(hipStream_t s, DevLevel &L, int order, const double *tab
 // FP32 per-node lattice values and 1/a_ii from the affine stencils
 // (alpha_m times the mass class stencils plus alpha times the affine ones)
 hipError_t kl_fill_lattice_affine(hipStream_t s, DevLevel &L, double alpha_m, double alpha);
+// the same operator without per-node values: the sweeps evaluate the affine
+// stencils at each node (bitwise the values the fill would store)
+hipError_t kl_set_lattice_affine(hipStream_t s, DevLevel &L, double alpha_m, double alpha);
 constexpr int kLst64 = 4112;
 // FP64 operator in lattice order (class stencils lst64, identity rows on the
 // boundary) and the order / precision changes of the lattice FGMRES

@@ -52,6 +52,18 @@ __host__ __device__ inline constexpr int hi_off() { return (P == 1 || PXS == 0) 
 // Compact class stencil: kCst floats per class, entry kCst - 1 is 1 / a_ii.
 constexpr int kCst = 128;
 
+// The FP64 value of the lattice system matrix alpha (S0 + X SX + Y SY) +
+// alpha_m M at entry s of a node of coordinates (X, Y), with the rounding
+// spelled out (explicit fma), so that the per-node fill and the sweeps that
+// evaluate it on the fly produce the same bits.  s0, sx, sy, sm: the class's
+// S0, SX, SY and mass stencils.
+__host__ __device__ inline double affine_value(const double *s0, const double *sx, const double *sy,
+                                               const double *sm, double alpha, double alpha_m, int s, int X,
+                                               int Y) {
+  const double e = fma((double)Y, sy[s], fma((double)X, sx[s], s0[s]));
+  return alpha_m != 0.0 ? fma(alpha_m, sm[s], alpha * e) : alpha * e;
+}
+
 // UNI: the class's stencil (and diagonal) are the same at every node and are
 // read from cst (uniform across the block); otherwise per-node values.
 template <int P, int PX, int PY, int PZ, int MODE, bool UNI>
@@ -111,6 +123,88 @@ __host__ __device__ inline void lat_class_node(const LatArgs &a, int c, long lon
       default: lat_node_any<2, 1, 1, 1, MODE>(a, c, k, val, inv, cst, x, f, out, omega); break;
     }
   }
+}
+
+// Sweep with affine stencils evaluated at each node (rotating convection):
+// the class's S0, SX, SY and mass stencils are staged in LDS and each value
+// is affine_value rounded to FP32, exactly what lat_fill_affine_k stores, so
+// the results are bitwise those of lat_node on the filled per-node values,
+// without their 4 bytes per nonzero of HBM traffic.
+struct LatAff {
+  const double *st;   // lst64: S0 at 0, SX at 8 kCst + 8, SY at 2 (8 kCst) + 8
+  const double *stm;  // mass stencils (8 kCst) or null
+  double alpha, alpha_m;
+  int centre[8];      // stencil index of the offset (0, 0, 0) per class
+};
+
+template <int P, int PX, int PY, int PZ, int MODE>
+__host__ __device__ inline void lat_node_aff(const LatArgs &a, const LatDev &C, long long k, const double *s0,
+                                             const double *sx, const double *sy, const double *sm, double alpha,
+                                             double alpha_m, int centre, const float *__restrict__ x,
+                                             const float *__restrict__ f, float *__restrict__ out, float omega) {
+  const int kx = (int)(k % C.cnt[0]), ky = (int)((k / C.cnt[0]) % C.cnt[1]), kz = (int)(k / ((long long)C.cnt[0] * C.cnt[1]));
+  const int X = C.lo[0] + C.step * kx, Y = C.lo[1] + C.step * ky, Z = C.lo[2] + C.step * kz;
+  const long long nn = a.nn, node = X + nn * (Y + nn * Z);
+  const float di = MODE == kLDefect ? 0.f : (float)(1.0 / affine_value(s0, sx, sy, sm, alpha, alpha_m, centre, X, Y));
+  if (MODE == kLJacobiZero) {
+    out[node] = omega * f[node] * di;
+    return;
+  }
+  float s = 0.f;
+  int si = 0;
+#pragma unroll
+  for (int dz = lo_off<P, PZ>(); dz <= hi_off<P, PZ>(); dz++)
+#pragma unroll
+    for (int dy = lo_off<P, PY>(); dy <= hi_off<P, PY>(); dy++)
+#pragma unroll
+      for (int dx = lo_off<P, PX>(); dx <= hi_off<P, PX>(); dx++, si++)
+        s += (float)affine_value(s0, sx, sy, sm, alpha, alpha_m, si, X, Y) * x[node + dx + nn * (dy + nn * dz)];
+  if (MODE == kLDefect) out[node] = f[node] - s;
+  else out[node] = x[node] + omega * (f[node] - s) * di;
+}
+
+template <int P, int MODE>
+__host__ __device__ inline void lat_class_node_aff(const LatArgs &a, int c, long long k, const double *s0,
+                                                   const double *sx, const double *sy, const double *sm,
+                                                   double alpha, double alpha_m, int centre, const float *x,
+                                                   const float *f, float *out, float omega) {
+#define CD3D_AFF(PX, PY, PZ) \
+  lat_node_aff<P, PX, PY, PZ, MODE>(a, a.c[c], k, s0, sx, sy, sm, alpha, alpha_m, centre, x, f, out, omega)
+  if (P == 1) {
+    CD3D_AFF(0, 0, 0);
+  } else {
+    switch (c) {
+      case 0: CD3D_AFF(0, 0, 0); break;
+      case 1: CD3D_AFF(1, 0, 0); break;
+      case 2: CD3D_AFF(0, 1, 0); break;
+      case 3: CD3D_AFF(1, 1, 0); break;
+      case 4: CD3D_AFF(0, 0, 1); break;
+      case 5: CD3D_AFF(1, 0, 1); break;
+      case 6: CD3D_AFF(0, 1, 1); break;
+      default: CD3D_AFF(1, 1, 1); break;
+    }
+  }
+#undef CD3D_AFF
+}
+
+template <int P, int MODE>
+__global__ __launch_bounds__(kB) void lat_aff_sweep_k(LatArgs a, LatAff af, const float *__restrict__ x,
+                                                      const float *__restrict__ f, float *__restrict__ out,
+                                                      float omega) {
+  __shared__ double st[4][kCst];  // S0, SX, SY, mass of the block's class
+  int c = 0;
+  while (c + 1 < a.nclass && (int)blockIdx.x >= a.bstart[c + 1]) c++;
+  for (int i = threadIdx.x; i < kCst; i += kB) {
+    st[0][i] = af.st[c * kCst + i];
+    st[1][i] = af.st[8 * kCst + 8 + c * kCst + i];
+    st[2][i] = af.st[2 * 8 * kCst + 8 + c * kCst + i];
+    st[3][i] = af.stm ? af.stm[c * kCst + i] : 0.0;
+  }
+  __syncthreads();
+  const long long k = (long long)(blockIdx.x - a.bstart[c]) * kB + threadIdx.x;
+  if (k < a.c[c].nodes)
+    lat_class_node_aff<P, MODE>(a, c, k, st[0], st[1], st[2], st[3], af.alpha, af.alpha_m, af.centre[c], x, f, out,
+                                omega);
 }
 
 // Q2 sweeps by vertex/midpoint pairs along X.  A thread takes the nodes
@@ -863,6 +957,7 @@ __host__ __device__ inline double affine_entry(const double *__restrict__ st, in
   return st[id] + X * st[8 * kCst + 8 + id] + Y * st[2 * 8 * kCst + 8 + id];
 }
 
+
 // CSR values of the active rows, one row block (at most kStreamNnz entries)
 // per workgroup: entries in CSR order, each finds its row in the block's row
 // pointers (LDS)
@@ -904,8 +999,9 @@ __global__ void lat_fill_affine_k(LatArgs a, long long n, const double *__restri
     const LatDev &C = a.c[c];
     const long long r = i - C.off, s = r / C.nodes, k = r % C.nodes;
     const int X = C.lo[0] + C.step * (int)(k % C.cnt[0]), Y = C.lo[1] + C.step * (int)((k / C.cnt[0]) % C.cnt[1]);
-    const int id = c * kCst + (int)s;
-    const double v = alpha * affine_entry(st, id, X, Y) + (alpha_m != 0.0 ? alpha_m * stm[id] : 0.0);
+    const int id = c * kCst;
+    const double v = affine_value(st + id, st + 8 * kCst + 8 + id, st + 2 * 8 * kCst + 8 + id, stm + id, alpha,
+                                  alpha_m, (int)s, X, Y);
     lval[i] = (float)v;
     if (s == centre[c]) linv[C.first + k] = (float)(1.0 / v);
   }
@@ -1097,6 +1193,16 @@ static bool lat_pairs() {
   return v;
 }
 
+static LatAff lat_aff(const DevLevel &L) {
+  LatAff af{};
+  af.st = L.lst64;
+  af.stm = L.lat_alpha_m != 0.0 ? L.lst64 + 3 * 8 * kCst + 8 : nullptr;
+  af.alpha = L.lat_alpha;
+  af.alpha_m = L.lat_alpha_m;
+  for (int c = 0; c < 8; c++) af.centre[c] = L.lat_centre[c];
+  return af;
+}
+
 static bool all_uni(const LatArgs &a) {
   for (int c = 0; c < a.nclass; c++)
     if (!a.uni[c]) return false;
@@ -1109,6 +1215,12 @@ void lat_sweep(hipStream_t s, const DevLevel &L, int order, const float *x, cons
   const LatArgs a = lat_args(L);
   const int grid = a.bstart[L.nclass];
   if (grid <= 0) return;
+  if (L.lat_affine) {
+    const LatAff af = lat_aff(L);
+    if (order == 1) lat_aff_sweep_k<1, MODE><<<grid, kB, 0, s>>>(a, af, x, f, out, omega);
+    else lat_aff_sweep_k<2, MODE><<<grid, kB, 0, s>>>(a, af, x, f, out, omega);
+    return;
+  }
   if (order == 1) lat_sweep_k<1, MODE><<<grid, kB, 0, s>>>(a, L.lval, L.linv, L.lcst, x, f, out, omega);
   else if (lat_pairs() && L.nn > kPairMinNN && all_uni(a))
     lat_pair_k<MODE, true><<<a.pstart[4], kB, 0, s>>>(a, L.lval, L.linv, L.lcst, x, f, out, omega);
@@ -1209,10 +1321,9 @@ void kl_assemble_affine(hipStream_t s, DevLevel &L, int order, const double *tab
     rhs_fill_k<<<(int)((L.lat_nint + kB - 1) / kB), kB, 0, s>>>(a, L.lat_nint, L.lnode2dof, L.lst64, rhs);
 }
 
-hipError_t kl_fill_lattice_affine(hipStream_t s, DevLevel &L, double alpha_m, double alpha) {
-  if (L.nclass <= 0 || L.lat_nval <= 0) return hipSuccess;
-  int centre[8];
-  for (int c = 0; c < L.nclass; c++) {  // stencil index of offset (0, 0, 0)
+// stencil index of the offset (0, 0, 0) of every class
+static void lat_centres(DevLevel &L) {
+  for (int c = 0; c < L.nclass; c++) {
     const LatClass &C = L.cls[c];
     int lo[3], len[3];
     for (int d = 0; d < 3; d++) {
@@ -1221,15 +1332,30 @@ hipError_t kl_fill_lattice_affine(hipStream_t s, DevLevel &L, double alpha_m, do
       lo[d] = mid ? -1 : -p;
       len[d] = mid ? 3 : 2 * p + 1;
     }
-    centre[c] = -lo[0] + len[0] * (-lo[1] + len[1] * -lo[2]);
+    L.lat_centre[c] = -lo[0] + len[0] * (-lo[1] + len[1] * -lo[2]);
   }
+}
+
+hipError_t kl_fill_lattice_affine(hipStream_t s, DevLevel &L, double alpha_m, double alpha) {
+  if (L.nclass <= 0 || L.lat_nval <= 0) return hipSuccess;
+  lat_centres(L);
   int *dcentre = (int *)(L.lst64 + 3 * 8 * kCst + 8 + 8 * kCst);  // spare tail of the stencil buffer
-  hipError_t e = hipMemcpyAsync(dcentre, centre, sizeof(int) * 8, hipMemcpyHostToDevice, s);
+  hipError_t e = hipMemcpyAsync(dcentre, L.lat_centre, sizeof(int) * 8, hipMemcpyHostToDevice, s);
   if (e != hipSuccess) return e;
   for (int c = 0; c < L.nclass; c++) L.uni[c] = 0;
+  L.lat_affine = false;
   lat_fill_affine_k<<<2048, kB, 0, s>>>(lat_args(L), L.lat_nval, L.lst64, L.lst64 + 3 * 8 * kCst + 8, alpha_m, alpha,
                                         dcentre, L.lval, L.linv);
   return hipGetLastError();
+}
+
+hipError_t kl_set_lattice_affine(hipStream_t, DevLevel &L, double alpha_m, double alpha) {
+  lat_centres(L);
+  for (int c = 0; c < L.nclass; c++) L.uni[c] = 0;
+  L.lat_alpha_m = alpha_m;
+  L.lat_alpha = alpha;
+  L.lat_affine = L.nclass > 0;
+  return hipSuccess;
 }
 
 void kl_fill_csr(hipStream_t s, const DevLevel &L, double *val) {

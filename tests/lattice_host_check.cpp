@@ -301,6 +301,59 @@ int check_pairs(int m, int k0) {
       if (diff || written < F.nact - 2) bad++;
     }
   }
+  // affine stencils evaluated node by node against per-node values filled
+  // with the same formula (affine_value), every class and mode
+  {
+    std::vector<double> st(kLst64), stm(8 * kCst);
+    for (auto &v : st) v = rnd(g);
+    for (auto &v : stm) v = rnd(g);
+    LatArgs a = lat_args(D);
+    for (int c = 0; c < 8; c++) a.uni[c] = 0;
+    int centre[8];
+    for (int c = 0; c < 8; c++) {
+      int lo[3], len[3];
+      for (int d = 0; d < 3; d++) {
+        const bool mid = a.c[c].px[d] == 1;
+        lo[d] = mid ? -1 : -2;
+        len[d] = mid ? 3 : 5;
+      }
+      centre[c] = -lo[0] + len[0] * (-lo[1] + len[1] * -lo[2]);
+    }
+    const double alpha = 0.37, alpha_m = 1.0;
+    std::vector<float> av(F.lat_nval), ai(F.nact);
+    for (int c = 0; c < 8; c++) {
+      const LatDev &C = a.c[c];
+      const double *s0 = st.data() + c * kCst, *sx = st.data() + 8 * kCst + 8 + c * kCst,
+                   *sy = st.data() + 2 * 8 * kCst + 8 + c * kCst, *sm = stm.data() + c * kCst;
+      for (int s2 = 0; s2 < C.nsten; s2++)
+        for (int64_t k = 0; k < C.nodes; k++) {
+          const int X = C.lo[0] + C.step * (int)(k % C.cnt[0]), Y = C.lo[1] + C.step * (int)((k / C.cnt[0]) % C.cnt[1]);
+          const double v = affine_value(s0, sx, sy, sm, alpha, alpha_m, s2, X, Y);
+          av[C.off + (int64_t)s2 * C.nodes + k] = (float)v;
+          if (s2 == centre[c]) ai[C.first + k] = (float)(1.0 / v);
+        }
+    }
+    for (int mode = 0; mode < 3; mode++) {
+      std::vector<float> o1(nn3, 7.f), o2(nn3, 7.f);
+      for (int c = 0; c < 8; c++) {
+        const double *s0 = st.data() + c * kCst, *sx = st.data() + 8 * kCst + 8 + c * kCst,
+                     *sy = st.data() + 2 * 8 * kCst + 8 + c * kCst, *sm = stm.data() + c * kCst;
+        for (int64_t k = 0; k < a.c[c].nodes; k++) {
+#define CD3D_BOTH(M)                                                                                       \
+  lat_class_node<2, M>(a, c, k, av.data(), ai.data(), nullptr, x.data(), f.data(), o1.data(), 0.7f);       \
+  lat_class_node_aff<2, M>(a, c, k, s0, sx, sy, sm, alpha, alpha_m, centre[c], x.data(), f.data(), o2.data(), 0.7f)
+          if (mode == kLJacobi) { CD3D_BOTH(kLJacobi); }
+          if (mode == kLJacobiZero) { CD3D_BOTH(kLJacobiZero); }
+          if (mode == kLDefect) { CD3D_BOTH(kLDefect); }
+#undef CD3D_BOTH
+        }
+      }
+      int64_t diff = 0;
+      for (size_t i = 0; i < nn3; i++) diff += memcmp(&o1[i], &o2[i], sizeof(float)) != 0;
+      printf("   affine m%d k0=%d mode %d: %lld differ\n", m, k0, mode, (long long)diff);
+      if (diff) bad++;
+    }
+  }
   // FP64 operator: pairs against the per-class nodes, both modes
   {
     std::vector<double> xd(nn3), fd(nn3), st(8 * kCst);

@@ -263,6 +263,29 @@ def test_small_level_program_matches_launch_per_step(order, m, k0, levels, conv_
     assert out["0"][2] == out["100000"][2]
 
 
+@pytest.mark.parametrize("order,m,k0,levels", [(1, 1, 2, 4), (2, 1, 1, 3)])
+def test_affine_sweeps_match_per_node_values(order, m, k0, levels, monkeypatch):
+    # rotating field, mixed form: on levels of more than CD3D_SMALL_NODES
+    # lattice nodes the FP32 sweeps evaluate the affine class stencils node by
+    # node, on the others they read per-node values filled from the same
+    # formula (and the bottom levels run as one workgroup): the same bits
+    co = pm.coeffs(eps=1e-3, conv_kind=1, rot_c=(0.45, 0.55), pulse_c=(0.3, 0.4, 0.5), c=0.0, f_val=0.0)
+    sp = pm.solver_params(**dict(REF_SOLVER, omega=0.5, pre=3, post=3, coarse_smoother=17, rel_tol=0.0,
+                                 res_norm_min=1e-30, red_factor=1e-10), precision=pm.MIXED)
+    out = {}
+    for bound in ("0", "1000000000"):
+        monkeypatch.setenv("CD3D_SMALL_NODES", bound)
+        P = pm.Problem(order, m, k0, levels, box=(1.0, 1.0, 1.0))
+        u0 = P.interp_pulse(levels - 1, co)
+        P.tcd_init(co, 0.0, u0)
+        its = [P.tcd_step(co, sp, 1e-3)[1].iterations for _ in range(3)]
+        out[bound] = (P.tcd_state(), its)
+        P.close()
+    say(f"[gpu] affine sweeps order={order} levels={levels}: its {out['0'][1]} / {out['1000000000'][1]}")
+    assert out["0"][1] == out["1000000000"][1]
+    assert np.array_equal(out["0"][0], out["1000000000"][0])
+
+
 @pytest.mark.parametrize("order,m,k0", [(1, 1, 2), (2, 3, 0), (1, 1, 3), (2, 1, 1), (1, 12, 0), (2, 7, 0),
                                          (1, 16, 0)])
 def test_coarse_inverse_solves_the_coarse_system(order, m, k0):
