@@ -1170,20 +1170,39 @@ static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_sol
   for (int l = 0; l <= Lf; l++) conv_path[l] = assemble_level(P, l, co, 0, P.dl[l].aconv);
   k_copy_range(P.stream, NA, N, F.rhs_asm, P.xs);
   // AssembleSystMat (SystemTCD3D.C:228-294): B = tau th3 f_old + tau th4 f_new
-  // + (M - tau th2 A) u_old on the active rows, system matrix M + tau th1 A
+  // + (M - tau th2 A) u_old on the active rows, system matrix M + tau th1 A.
+  // The reference forms M - tau th2 A in place, adds tau (th1 + th2) A and
+  // after the solve restores M (RestoreMassMat).  Here (M - tau th2 A) u_old
+  // is one pass over both matrices, and the system matrix is written over A
+  // (assembled again next step), so M is never changed and needs no restore:
+  // two passes over the matrices less per step, the same operators up to
+  // rounding.
   HIPCHK(hipMemsetAsync(P.bs, 0, sizeof(double) * N, P.stream));
   k_axpy(P.stream, NA, tau * th3, P.tcd_oldrhs, P.bs);
   k_axpy(P.stream, NA, tau * th4, F.rhs_asm, P.bs);
-  k_matadd(P.stream, P.hl[Lf].rowptr[NA], F.val, F.aconv, -tau * th2);
-  const double gamma0 = -tau * th2;
   HIPCHK(hipMemsetAsync(P.tcd_def, 0, sizeof(double) * N, P.stream));
-  k_matvec_active(P.stream, F, P.tcd_oldsol, P.tcd_def);
+  k_matvec_pair_active(P.stream, F, F.val, F.aconv, 1.0, -tau * th2, P.tcd_oldsol, P.tcd_def);
   k_axpy(P.stream, NA, 1.0, P.tcd_def, P.bs);
   k_copy_range(P.stream, NA, N, F.rhs_asm, P.bs);
   k_copy_range(P.stream, NA, N, F.rhs_asm, P.xs);
+  // the level matrices are the system matrices until the guard swaps back
+  struct SystemSwap {
+    cd3d_problem &P;
+    int n = 0;
+    explicit SystemSwap(cd3d_problem &p) : P(p) {}
+    void swap_level(int l) {
+      std::swap(P.dl[l].val, P.dl[l].aconv);
+      P.dl[l].invdiag_ok = false;
+    }
+    ~SystemSwap() {
+      for (int l = 0; l < n; l++) swap_level(l);
+    }
+  } sys(P);
   for (int l = 0; l <= Lf; l++) {
     DevLevel &D = P.dl[l];
-    k_matadd(P.stream, P.hl[l].rowptr[D.nact], D.val, D.aconv, l == Lf ? -gamma0 + tau * th1 : tau * th1);
+    k_matadd_into(P.stream, P.hl[l].rowptr[D.nact], D.aconv, D.val, tau * th1);  // A <- M + tau th1 A
+    sys.swap_level(l);
+    sys.n = l + 1;
     if (l == 0) fork_coarse(P);
   }
   launch_coarse_inverse(P);  // beside the finer levels' updates
@@ -1201,13 +1220,9 @@ static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_sol
   // Solve (SystemTCD3D.C:317-376)
   double r0;
   const int its = fgmres(P, sp, dof_space(P), r0, solve_status);
-  // RestoreMassMat (SystemTCD3D.C:296-315)
+  // the level matrices are the mass matrices again (RestoreMassMat,
+  // SystemTCD3D.C:296-315, without arithmetic: M was not changed)
   coarse_join(P);
-  for (int l = 0; l <= Lf; l++) {
-    DevLevel &D = P.dl[l];
-    k_matadd(P.stream, P.hl[l].rowptr[D.nact], D.val, D.aconv, -tau * th1);
-    D.invdiag_ok = false;
-  }
   P.tcd_t += tau;
   return its;
 }

@@ -224,6 +224,11 @@ void k_axpy_dev(hipStream_t s, int n, const double *alpha_dev, int idx, const do
 void k_copy(hipStream_t s, int n, const double *x, double *y);
 void k_fill(hipStream_t s, int64_t n, double v, double *y);
 void k_matadd(hipStream_t s, int64_t n, double *C, const double *D, double alpha);
+// C = D + alpha C
+void k_matadd_into(hipStream_t s, int64_t n, double *C, const double *D, double alpha);
+// y = alpha (A x) + beta (B x) on the active rows (A, B: value arrays of L's pattern)
+void k_matvec_pair_active(hipStream_t s, const DevLevel &L, const double *a, const double *b, double alpha,
+                          double beta, const double *x, double *y);
 void k_matvec_active(hipStream_t s, const DevLevel &L, const double *x, double *y);
 // assembly
 struct AsmParams {

@@ -491,6 +491,33 @@ __global__ void matadd_k(int64_t n, double *__restrict__ C, const double *__rest
 // 16-byte aligned (a null pointer counts as aligned)
 static inline bool al16(const void *p) { return ((uintptr_t)p & 15) == 0; }
 
+// y = alpha (A x) + beta (B x) on the active rows; A and B share the CSR
+// pattern, so one pass reads the pattern and both value arrays once
+template <int LANES>
+__global__ __launch_bounds__(kBlock) void csr_pair_active_k(int nact, const int *__restrict__ rowptr,
+                                                          const int *__restrict__ col, const double *__restrict__ a,
+                                                          const double *__restrict__ b, double alpha, double beta,
+                                                          const double *__restrict__ x, double *__restrict__ y) {
+  const int gid = blockIdx.x * kBlock + threadIdx.x;
+  const int row = gid / LANES, lane = gid % LANES;
+  if (row >= nact) return;  // whole groups leave together
+  double sa = 0, sb = 0;
+  for (int j = rowptr[row] + lane; j < rowptr[row + 1]; j += LANES) {
+    const double xv = x[col[j]];
+    sa += a[j] * xv;
+    sb += b[j] * xv;
+  }
+  sa = group_sum<LANES>(sa);
+  sb = group_sum<LANES>(sb);
+  if (lane == 0) y[row] = alpha * sa + beta * sb;
+}
+
+// C = D + alpha C (C and D of n entries)
+__global__ void matadd_into_k(int64_t n, double *__restrict__ C, const double *__restrict__ D, double alpha) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+    C[i] = D[i] + alpha * C[i];
+}
+
 inline int grid_for(int64_t n) {
   int64_t g = (n + kBlock - 1) / kBlock;
   return (int)(g < 2048 ? (g < 1 ? 1 : g) : 2048);
@@ -1148,6 +1175,19 @@ void k_invdiag(hipStream_t s, const DevLevel &L) {
 void k_jacobi_zero(hipStream_t s, const DevLevel &L, const double *f, double *out, double omega) {
   jacobi_zero_k<double><<<(L.ndof + kBlock - 1) / kBlock, kBlock, 0, s>>>(L.ndof, L.nact, L.diag, L.val, nullptr,
                                                                          f, out, omega);
+}
+void k_matvec_pair_active(hipStream_t s, const DevLevel &L, const double *a, const double *b, double alpha,
+                          double beta, const double *x, double *y) {
+  if (L.nact <= 0) return;
+  const double m = (double)L.nnz / L.ndof;
+  const int lanes = m <= 12 ? 4 : m <= 40 ? 8 : 16;
+  const int grid = (int)(((int64_t)L.nact * lanes + kBlock - 1) / kBlock);
+  if (lanes == 4) csr_pair_active_k<4><<<grid, kBlock, 0, s>>>(L.nact, L.rowptr, L.col, a, b, alpha, beta, x, y);
+  else if (lanes == 8) csr_pair_active_k<8><<<grid, kBlock, 0, s>>>(L.nact, L.rowptr, L.col, a, b, alpha, beta, x, y);
+  else csr_pair_active_k<16><<<grid, kBlock, 0, s>>>(L.nact, L.rowptr, L.col, a, b, alpha, beta, x, y);
+}
+void k_matadd_into(hipStream_t s, int64_t n, double *C, const double *D, double alpha) {
+  matadd_into_k<<<grid_for(n), kBlock, 0, s>>>(n, C, D, alpha);
 }
 void k_matvec_active(hipStream_t s, const DevLevel &L, const double *x, double *y) {
   launch_rows<kActiveSpmv, double, double>(s, L.nnz, L.ndof, L.nact, L.rowptr, L.col, L.val, x,
