@@ -409,13 +409,16 @@ static void refresh_f32(cd3d_problem &P) {
         HIPCHK(kl_set_lattice_affine(P.stream, D, D.lat_alpha_m, D.lat_alpha));
         D.lat_fresh = 0;
         D.lat_exact64 = false;
+        D.lat_aff64 = D.lat_alpha_m == 0.0 || D.lat_alpha_m == 1.0;
       } else if (D.lat_fresh == 2) {  // straight from the affine class stencils
         per_node(false);
         HIPCHK(kl_fill_lattice_affine(P.stream, D, D.lat_alpha_m, D.lat_alpha));
         D.lat_fresh = 0;
         D.lat_exact64 = false;
+        D.lat_aff64 = D.lat_alpha_m == 0.0 || D.lat_alpha_m == 1.0;
       } else {
         D.lat_affine = false;
+        D.lat_aff64 = false;
         bool all_uniform = false;
         HIPCHK(kl_detect_uniform(P.stream, D, P.hflag, &all_uniform));
         if (!all_uniform) {
@@ -1056,7 +1059,7 @@ cd3d_status cd3d_solve(cd3d_problem *p, const cd3d_solver_params *sp, const doub
     double r0 = 0;
     int its;
     const int top = (int)p->dl.size() - 1;
-    if (sp->precision == CD3D_MIXED && top > 0 && F.lat_exact64) {
+    if (sp->precision == CD3D_MIXED && top > 0 && (F.lat_exact64 || F.lat_aff64)) {
       if (!p->xl) {
         p->xl = dalloc<double>(*p, N);
         p->bl = dalloc<double>(*p, N);
@@ -1217,9 +1220,22 @@ static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_sol
     D.lat_alpha = tau * th1;
   }
   refresh_f32(P);
-  // Solve (SystemTCD3D.C:317-376)
+  // Solve (SystemTCD3D.C:317-376); the mixed form runs FGMRES in lattice
+  // order when the FP64 lattice operator is exactly the system matrix
   double r0;
-  const int its = fgmres(P, sp, dof_space(P), r0, solve_status);
+  int its;
+  if (sp.precision == CD3D_MIXED && Lf > 0 && (F.lat_exact64 || F.lat_aff64)) {
+    if (!P.xl) {
+      P.xl = dalloc<double>(P, N);
+      P.bl = dalloc<double>(P, N);
+    }
+    kl_dof_to_lat64(P.stream, F, P.bs, P.bl);
+    kl_dof_to_lat64(P.stream, F, P.xs, P.xl);
+    its = fgmres(P, sp, lattice_space(P), r0, solve_status);
+    kl_lat64_to_dof(P.stream, F, P.xl, P.xs);
+  } else {
+    its = fgmres(P, sp, dof_space(P), r0, solve_status);
+  }
   // the level matrices are the mass matrices again (RestoreMassMat,
   // SystemTCD3D.C:296-315, without arithmetic: M was not changed)
   coarse_join(P);
@@ -1334,7 +1350,8 @@ cd3d_status cd3d_kernel_bench(cd3d_problem *p, int which, int level, int reps, d
         break;
       }
       case 2:
-        if (!L.lat_exact64) throw ArgError("the FP64 lattice operator needs constant-coefficient assembly");
+        if (!L.lat_exact64 && !L.lat_aff64)
+          throw ArgError("the FP64 lattice operator needs assembly through class stencils");
         launch = [&] { kl_spmv64(p->stream, L, o, L.sol, L.aux); };
         b = 16 * N;
         break;

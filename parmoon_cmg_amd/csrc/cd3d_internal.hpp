@@ -129,6 +129,10 @@ struct DevLevel {
   // on a level too large for the one-workgroup program): lval / linv unused
   bool lat_affine = false;
   int lat_centre[8] = {0};  // stencil index of the offset (0, 0, 0) per class
+  // the CSR values of the interior rows are exactly the affine stencils
+  // (alpha (S0 + X SX + Y SY), or fma(alpha, that, M) with lat_alpha_m 1):
+  // the FP64 lattice operator can stand for the matrix
+  bool lat_aff64 = false;
   bool tcd_mass_uniform = false;  // TCD3D: the mass class stencils are in lst64 (mass slot)
 };
 

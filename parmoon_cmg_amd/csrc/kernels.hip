@@ -515,7 +515,7 @@ __global__ __launch_bounds__(kBlock) void csr_pair_active_k(int nact, const int 
 // C = D + alpha C (C and D of n entries)
 __global__ void matadd_into_k(int64_t n, double *__restrict__ C, const double *__restrict__ D, double alpha) {
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-    C[i] = D[i] + alpha * C[i];
+    C[i] = fma(alpha, C[i], D[i]);  // explicit: the FP64 lattice operator of the rotating field repeats it
 }
 
 inline int grid_for(int64_t n) {

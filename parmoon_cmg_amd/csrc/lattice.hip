@@ -953,8 +953,10 @@ __global__ __launch_bounds__(kCst) void class_stencil_affine_k(LatArgs a, const 
   }
 }
 
+// rounding spelled out (explicit fma): the FP64 lattice operator of the
+// rotating field evaluates the same expression and gets the CSR values' bits
 __host__ __device__ inline double affine_entry(const double *__restrict__ st, int id, int X, int Y) {
-  return st[id] + X * st[8 * kCst + 8 + id] + Y * st[2 * 8 * kCst + 8 + id];
+  return fma((double)Y, st[2 * 8 * kCst + 8 + id], fma((double)X, st[8 * kCst + 8 + id], st[id]));
 }
 
 
@@ -1104,6 +1106,80 @@ __global__ __launch_bounds__(kB) void lat64_pair_k(LatArgs a, const double *__re
   __syncthreads();
   const long long k = (long long)(blockIdx.x - a.pstart[pc]) * kB + threadIdx.x;
   if (k < a.c[2 * pc + 1].nodes) lat64_pair_node<MODE>(a, pc, k, cs, cs + kCst, x, f, out);
+}
+
+// FP64 operator of a rotating-field level: entry s of a node (X, Y) is
+// alpha (S0 + X SX + Y SY) (conv-diff alone, alpha 1) or
+// fma(alpha, S0 + X SX + Y SY, M) (the TCD3D system matrix M + tau th1 A),
+// the exact arithmetic of the CSR values (affine_entry in csr_fill_affine_k,
+// then matadd_into_k), so the lattice FGMRES applies the assembled matrix.
+// Class-blocked; blocks bstart[nclass].. take the Dirichlet rows (identity).
+template <int P, int PX, int PY, int PZ, int MODE>
+__host__ __device__ inline void lat64_aff_node(const LatArgs &a, const LatDev &C, long long k, const double *s0,
+                                               const double *sx, const double *sy, const double *sm, double alpha,
+                                               const double *__restrict__ x, const double *__restrict__ f,
+                                               double *__restrict__ out) {
+  const int kx = (int)(k % C.cnt[0]), ky = (int)((k / C.cnt[0]) % C.cnt[1]), kz = (int)(k / ((long long)C.cnt[0] * C.cnt[1]));
+  const int X = C.lo[0] + C.step * kx, Y = C.lo[1] + C.step * ky, Z = C.lo[2] + C.step * kz;
+  const long long nn = a.nn, node = X + nn * (Y + nn * Z);
+  double s = 0.0;
+  int si = 0;
+#pragma unroll
+  for (int dz = lo_off<P, PZ>(); dz <= hi_off<P, PZ>(); dz++)
+#pragma unroll
+    for (int dy = lo_off<P, PY>(); dy <= hi_off<P, PY>(); dy++)
+#pragma unroll
+      for (int dx = lo_off<P, PX>(); dx <= hi_off<P, PX>(); dx++, si++) {
+        const double e = fma((double)Y, sy[si], fma((double)X, sx[si], s0[si]));
+        const double v = sm ? fma(alpha, e, sm[si]) : alpha * e;
+        s += v * x[node + dx + nn * (dy + nn * dz)];
+      }
+  out[node] = MODE == kL64Defect ? f[node] - s : s;
+}
+
+template <int P, int MODE>
+__global__ __launch_bounds__(kB) void lat64_aff_k(LatArgs a, LatAff af, const double *__restrict__ x,
+                                                  const double *__restrict__ f, double *__restrict__ out, int nact,
+                                                  int ndof, const int *__restrict__ dof2node) {
+  __shared__ double st[4][kCst];
+  const int nb = a.bstart[a.nclass];
+  if ((int)blockIdx.x >= nb) {  // Dirichlet rows
+    const int d = nact + (blockIdx.x - nb) * kB + threadIdx.x;
+    if (d < ndof) {
+      const int node = dof2node[d];
+      out[node] = MODE == kL64Defect ? f[node] - x[node] : x[node];
+    }
+    return;
+  }
+  int c = 0;
+  while (c + 1 < a.nclass && (int)blockIdx.x >= a.bstart[c + 1]) c++;
+  for (int i = threadIdx.x; i < kCst; i += kB) {
+    st[0][i] = af.st[c * kCst + i];
+    st[1][i] = af.st[8 * kCst + 8 + c * kCst + i];
+    st[2][i] = af.st[2 * 8 * kCst + 8 + c * kCst + i];
+    st[3][i] = af.stm ? af.stm[c * kCst + i] : 0.0;
+  }
+  __syncthreads();
+  const long long k = (long long)(blockIdx.x - a.bstart[c]) * kB + threadIdx.x;
+  if (k >= a.c[c].nodes) return;
+  const double *sm = af.stm ? st[3] : nullptr;
+#define CD3D_A64(PX, PY, PZ) \
+  lat64_aff_node<P, PX, PY, PZ, MODE>(a, a.c[c], k, st[0], st[1], st[2], sm, af.alpha, x, f, out)
+  if (P == 1) {
+    CD3D_A64(0, 0, 0);
+  } else {
+    switch (c) {
+      case 0: CD3D_A64(0, 0, 0); break;
+      case 1: CD3D_A64(1, 0, 0); break;
+      case 2: CD3D_A64(0, 1, 0); break;
+      case 3: CD3D_A64(1, 1, 0); break;
+      case 4: CD3D_A64(0, 0, 1); break;
+      case 5: CD3D_A64(1, 0, 1); break;
+      case 6: CD3D_A64(0, 1, 1); break;
+      default: CD3D_A64(1, 1, 1); break;
+    }
+  }
+#undef CD3D_A64
 }
 
 // Dirichlet rows (DOFs nact..ndof-1): identity
@@ -1379,6 +1455,15 @@ void lat64_apply(hipStream_t s, const DevLevel &L, int order, const double *x, c
   const LatArgs a = lat_args(L);
   const int grid = a.bstart[L.nclass];
   const int nd = L.ndof - L.nact;
+  if (L.lat_aff64) {  // rotating field: interior and Dirichlet rows in one launch
+    LatAff af = lat_aff(L);
+    af.alpha = L.lat_alpha;
+    const int g = grid + (nd + kB - 1) / kB;
+    if (g <= 0) return;
+    if (order == 1) lat64_aff_k<1, MODE><<<g, kB, 0, s>>>(a, af, x, f, out, L.nact, L.ndof, L.ldof2node);
+    else lat64_aff_k<2, MODE><<<g, kB, 0, s>>>(a, af, x, f, out, L.nact, L.ndof, L.ldof2node);
+    return;
+  }
   if (order == 2 && grid > 0 && lat_pairs() && L.nn > kPairMinNN) {  // interior and Dirichlet rows in one launch
     lat64_pair_k<MODE><<<a.pstart[4] + (nd + kB - 1) / kB, kB, 0, s>>>(a, L.lst64, x, f, out, L.nact, L.ndof,
                                                                        L.ldof2node);

@@ -264,6 +264,37 @@ def test_small_level_program_matches_launch_per_step(order, m, k0, levels, conv_
 
 
 @pytest.mark.parametrize("order,m,k0,levels", [(1, 1, 2, 4), (2, 1, 1, 3)])
+def test_rotating_field_lattice_fgmres(order, m, k0, levels):
+    # Galerkin rotating field: the CSR values are the affine class stencils,
+    # so the mixed form runs FGMRES in lattice order with the FP64 lattice
+    # operator; the residual reported is recomputed with the CSR matrix
+    P = pm.Problem(order, m, k0, levels, box=(1.0, 1.0, 1.0))
+    P.assemble(pm.coeffs(eps=1e-2, conv_kind=1, rot_c=(0.45, 0.55), c=0.5, f_val=1.0, t=0.2))
+    sp = dict(REF_SOLVER, omega=0.5, pre=3, post=3, coarse_smoother=17, rel_tol=1e-10)
+    x, st = P.solve(pm.solver_params(**sp, precision=pm.MIXED))
+    x64, st64 = P.solve(pm.solver_params(**sp))
+    err = np.linalg.norm(x - x64) / np.linalg.norm(x64)
+    say(f"[gpu] rotating-field lattice FGMRES order={order}: its {st.iterations} / {st64.iterations}, "
+        f"rfinal/r0 {st.rfinal / st.r0:.2e}, err vs fp64 {err:.2e}")
+    assert st.status == 0 and st.rfinal <= 1.0001e-10 * st.r0
+    assert err < 1e-9
+    # TCD3D: mixed steps (lattice FGMRES on M + tau th1 A) against FP64 steps
+    co = pm.coeffs(eps=1e-3, conv_kind=1, rot_c=(0.45, 0.55), pulse_c=(0.3, 0.4, 0.5), f_val=0.0)
+    states = []
+    for prec in (pm.MIXED, pm.FP64):
+        Q = pm.Problem(order, m, k0, levels, box=(1.0, 1.0, 1.0))
+        Q.tcd_init(co, 0.0, Q.interp_pulse(levels - 1, co))
+        spt = pm.solver_params(**dict(sp, rel_tol=0.0, res_norm_min=1e-30, red_factor=1e-10), precision=prec)
+        for _ in range(3):
+            Q.tcd_step(co, spt, 1e-3)
+        states.append(Q.tcd_state())
+        Q.close()
+    terr = np.linalg.norm(states[0] - states[1]) / np.linalg.norm(states[1])
+    say(f"[gpu] rotating-field TCD3D mixed vs fp64 states: {terr:.2e}")
+    assert terr < 1e-8
+
+
+@pytest.mark.parametrize("order,m,k0,levels", [(1, 1, 2, 4), (2, 1, 1, 3)])
 def test_affine_sweeps_match_per_node_values(order, m, k0, levels, monkeypatch):
     # rotating field, mixed form: on levels of more than CD3D_SMALL_NODES
     # lattice nodes the FP32 sweeps evaluate the affine class stencils node by
