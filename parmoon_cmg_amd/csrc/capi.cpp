@@ -277,7 +277,9 @@ static void check_coeffs(const cd3d_coeffs &co) {
 // Returns which path ran: 1 constant coefficients (it leaves the FP32 compact
 // stencils of `val` behind), 2 rotating field (affine class stencils in
 // lst64), 0 cell by cell.
-static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int what, double *val) {
+// csr false: a level assembled through affine class stencils gets the
+// stencils (and rhs) only, not its CSR values
+static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int what, double *val, bool csr = true) {
   if (val == P.dl[l].val) P.dl[l].invdiag_ok = false;
   DevLevel &D = P.dl[l];
   AsmParams ap{};
@@ -308,7 +310,7 @@ static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int wha
     return 1;
   }
   if (co.conv_kind == 1 && co.f_kind == 0 && !co.supg && D.nclass > 0 && !uniform_assembly_off()) {
-    kl_assemble_affine(P.stream, D, P.mesh.order, P.tab, ap, val, D.rhs_asm);
+    kl_assemble_affine(P.stream, D, P.mesh.order, P.tab, ap, val, D.rhs_asm, csr);
     k_dirichlet_rows(P.stream, D, val, D.rhs_asm);
     HIPCHK(hipGetLastError());
     return 2;
@@ -1168,44 +1170,68 @@ static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_sol
   k_copy(P.stream, N, P.xs, P.tcd_oldsol);
   cd3d_coeffs co = co0;
   co.t = P.tcd_t + tau;
+  // The mixed form works on the lattice wherever the assembled matrices are
+  // affine class stencils with a uniform mass matrix (rotating field): there
+  // the system matrix fma(tau th1, A, M) and the rhs matrix
+  // fma(-tau th2, A, M) are evaluated entry by entry from the stencils by the
+  // FP32 sweeps and the FP64 lattice operator, with the arithmetic of the CSR
+  // path, so levels above the coarsest get no CSR values.  The coarsest level
+  // keeps its CSR system matrix for the exact coarse solve.
+  const bool affine_asm = co.conv_kind == 1 && co.f_kind == 0 && !co.supg && !uniform_assembly_off();
+  bool lattice = sp.precision == CD3D_MIXED && Lf > 0 && affine_asm;
+  for (int l = 1; l <= Lf; l++) lattice = lattice && P.dl[l].tcd_mass_uniform && P.dl[l].nclass > 0;
   // AssembleARhs at t_new (SystemTCD3D.C:204-226)
   std::vector<int> conv_path(Lf + 1);
-  for (int l = 0; l <= Lf; l++) conv_path[l] = assemble_level(P, l, co, 0, P.dl[l].aconv);
+  for (int l = 0; l <= Lf; l++) conv_path[l] = assemble_level(P, l, co, 0, P.dl[l].aconv, !(lattice && l > 0));
+  if (lattice)
+    for (int l = 1; l <= Lf; l++)
+      if (conv_path[l] != 2) throw std::logic_error("TCD3D lattice step: a level was not assembled affinely");
   k_copy_range(P.stream, NA, N, F.rhs_asm, P.xs);
   // AssembleSystMat (SystemTCD3D.C:228-294): B = tau th3 f_old + tau th4 f_new
   // + (M - tau th2 A) u_old on the active rows, system matrix M + tau th1 A.
   // The reference forms M - tau th2 A in place, adds tau (th1 + th2) A and
   // after the solve restores M (RestoreMassMat).  Here (M - tau th2 A) u_old
-  // is one pass over both matrices, and the system matrix is written over A
-  // (assembled again next step), so M is never changed and needs no restore:
-  // two passes over the matrices less per step, the same operators up to
-  // rounding.
+  // is one pass over both matrices (or over the stencils), and the system
+  // matrix is written over A (assembled again next step), so M is never
+  // changed and needs no restore: the same operators up to rounding.
   HIPCHK(hipMemsetAsync(P.bs, 0, sizeof(double) * N, P.stream));
   k_axpy(P.stream, NA, tau * th3, P.tcd_oldrhs, P.bs);
   k_axpy(P.stream, NA, tau * th4, F.rhs_asm, P.bs);
   HIPCHK(hipMemsetAsync(P.tcd_def, 0, sizeof(double) * N, P.stream));
-  k_matvec_pair_active(P.stream, F, F.val, F.aconv, 1.0, -tau * th2, P.tcd_oldsol, P.tcd_def);
+  if (lattice) {
+    if (!P.xl) {
+      P.xl = dalloc<double>(P, N);
+      P.bl = dalloc<double>(P, N);
+    }
+    kl_dof_to_lat64(P.stream, F, P.tcd_oldsol, P.xl);
+    kl_affine64_apply(P.stream, F, P.mesh.order, -tau * th2, true, P.xl, P.bl);
+    kl_lat64_to_dof(P.stream, F, P.bl, P.tcd_def);
+  } else {
+    k_matvec_pair_active(P.stream, F, F.val, F.aconv, 1.0, -tau * th2, P.tcd_oldsol, P.tcd_def);
+  }
   k_axpy(P.stream, NA, 1.0, P.tcd_def, P.bs);
   k_copy_range(P.stream, NA, N, F.rhs_asm, P.bs);
   k_copy_range(P.stream, NA, N, F.rhs_asm, P.xs);
   // the level matrices are the system matrices until the guard swaps back
   struct SystemSwap {
     cd3d_problem &P;
-    int n = 0;
+    std::vector<int> levels;
     explicit SystemSwap(cd3d_problem &p) : P(p) {}
     void swap_level(int l) {
       std::swap(P.dl[l].val, P.dl[l].aconv);
       P.dl[l].invdiag_ok = false;
     }
     ~SystemSwap() {
-      for (int l = 0; l < n; l++) swap_level(l);
+      for (int l : levels) swap_level(l);
     }
   } sys(P);
   for (int l = 0; l <= Lf; l++) {
     DevLevel &D = P.dl[l];
-    k_matadd_into(P.stream, P.hl[l].rowptr[D.nact], D.aconv, D.val, tau * th1);  // A <- M + tau th1 A
-    sys.swap_level(l);
-    sys.n = l + 1;
+    if (!(lattice && l > 0)) {
+      k_matadd_into(P.stream, P.hl[l].rowptr[D.nact], D.aconv, D.val, tau * th1);  // A <- M + tau th1 A
+      sys.swap_level(l);
+      sys.levels.push_back(l);
+    }
     if (l == 0) fork_coarse(P);
   }
   launch_coarse_inverse(P);  // beside the finer levels' updates

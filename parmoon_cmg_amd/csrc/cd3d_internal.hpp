@@ -268,8 +268,9 @@ void kl_assemble_uniform(hipStream_t s, DevLevel &L, int order, const double *ta
 void kl_fill_csr(hipStream_t s, const DevLevel &L, double *val);
 // rotating field (affine in space), Galerkin, constant f: affine class
 // stencils S0 + X SX + Y SY, CSR values of the active rows and the rhs
+// csr: also write the CSR values (else the class stencils only)
 void kl_assemble_affine(hipStream_t s, DevLevel &L, int order, const double *tab, const AsmParams &ap,
-                        double *val, double *rhs);
+                        double *val, double *rhs, bool csr = true);
 // FP32 per-node lattice values and 1/a_ii from the affine stencils
 // (alpha_m times the mass class stencils plus alpha times the affine ones)
 hipError_t kl_fill_lattice_affine(hipStream_t s, DevLevel &L, double alpha_m, double alpha);
@@ -280,6 +281,11 @@ constexpr int kLst64 = 4112;
 // FP64 operator in lattice order (class stencils lst64, identity rows on the
 // boundary) and the order / precision changes of the lattice FGMRES
 void kl_spmv64(hipStream_t s, const DevLevel &L, int order, const double *x, double *y);
+// y = B x in lattice order, B entries alpha (S0 + X SX + Y SY) or, with_mass,
+// fma(alpha, S0 + X SX + Y SY, M) from the level's affine and mass stencils
+// (identity rows on the boundary)
+void kl_affine64_apply(hipStream_t s, const DevLevel &L, int order, double alpha, bool with_mass, const double *x,
+                       double *y);
 void kl_defect64(hipStream_t s, const DevLevel &L, int order, const double *x, const double *f, double *d);
 void kl_dof_to_lat64(hipStream_t s, const DevLevel &L, const double *v, double *l);
 void kl_lat64_to_dof(hipStream_t s, const DevLevel &L, const double *l, double *v);

@@ -1380,7 +1380,7 @@ void kl_assemble_uniform(hipStream_t s, DevLevel &L, int order, const double *ta
 }
 
 void kl_assemble_affine(hipStream_t s, DevLevel &L, int order, const double *tab, const AsmParams &ap,
-                        double *val, double *rhs) {
+                        double *val, double *rhs, bool csr) {
   const LatArgs a = lat_args(L);
   if (order == 1) {
     elem_affine_k<1><<<1, 64, 0, s>>>(tab, ap, L.lelem);
@@ -1391,7 +1391,7 @@ void kl_assemble_affine(hipStream_t s, DevLevel &L, int order, const double *tab
     class_stencil_affine_k<2><<<L.nclass, kCst, 0, s>>>(a, L.lelem, ap.dn[0], ap.dn[1], ap.rot_c[0], ap.rot_c[1],
                                                         L.lst64);
   }
-  if (L.nblk > 0)
+  if (csr && L.nblk > 0)
     csr_fill_affine_k<<<L.nblk, kB, 0, s>>>(L.blk, L.nact, L.rowptr, L.ldof2node, L.nn, L.lsid, L.lst64, val);
   if (L.lat_nint > 0)
     rhs_fill_k<<<(int)((L.lat_nint + kB - 1) / kB), kB, 0, s>>>(a, L.lat_nint, L.lnode2dof, L.lst64, rhs);
@@ -1478,6 +1478,17 @@ void lat64_apply(hipStream_t s, const DevLevel &L, int order, const double *x, c
                                                       MODE == kL64Defect ? 1 : 0);
 }
 
+void kl_affine64_apply(hipStream_t s, const DevLevel &L, int order, double alpha, bool with_mass, const double *x,
+                       double *y) {
+  const LatArgs a = lat_args(L);
+  LatAff af = lat_aff(L);
+  af.alpha = alpha;
+  af.stm = with_mass ? L.lst64 + 3 * 8 * kCst + 8 : nullptr;
+  const int g = a.bstart[L.nclass] + (L.ndof - L.nact + kB - 1) / kB;
+  if (g <= 0) return;
+  if (order == 1) lat64_aff_k<1, kL64Spmv><<<g, kB, 0, s>>>(a, af, x, nullptr, y, L.nact, L.ndof, L.ldof2node);
+  else lat64_aff_k<2, kL64Spmv><<<g, kB, 0, s>>>(a, af, x, nullptr, y, L.nact, L.ndof, L.ldof2node);
+}
 void kl_spmv64(hipStream_t s, const DevLevel &L, int order, const double *x, double *y) {
   lat64_apply<kL64Spmv>(s, L, order, x, nullptr, y);
 }
