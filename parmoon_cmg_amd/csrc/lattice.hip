@@ -529,11 +529,63 @@ __host__ __device__ inline void prolong_cell(long long t, int ncc, int nnf, int 
   }
 }
 
+// The same per fine node, one thread per (coarse cell, local fine Z, local
+// fine Y): 4 P^2 times the threads of prolong_cell (coarse levels have few
+// cells), and consecutive lanes on consecutive cells in X write 2P adjacent
+// fine nodes each (coalesced).  Each node's value is formed in prolong_cell's
+// order (Z pass, Y pass, X pass); the host check finds the same bits.  On the
+// device the weights of prolong_cell are compile-time constants and its
+// products may be folded differently, so every device path (the
+// launch-per-step V-cycle and the one-workgroup program) uses this form.
+template <int P>
+__host__ __device__ inline void prolong_row(long long t, int ncc, int nnf, int nnc, const float *__restrict__ coarse,
+                                            float *__restrict__ x, float damp) {
+  constexpr int P1 = P + 1, T = 2 * P;
+  const int cx = (int)(t % ncc);
+  long long r = t / ncc;
+  const int ty = (int)(r % T);
+  r /= T;
+  const int tz = (int)(r % T);
+  r /= T;
+  const int cy = (int)(r % ncc), cz = (int)(r / ncc);
+  const int Y = T * cy + ty, Z = T * cz + tz;
+  if (Y == 0 || Z == 0) return;
+  const long long C = nnc, F = nnf;
+  float pz[P1][P1];
+#pragma unroll
+  for (int b = 0; b < P1; b++)
+#pragma unroll
+    for (int a = 0; a < P1; a++) {
+      float v = 0.f;
+#pragma unroll
+      for (int c = 0; c < P1; c++)
+        v += prol_w<P>(tz, c) * coarse[(P * cx + a) + C * ((P * cy + b) + C * (P * cz + c))];
+      pz[b][a] = v;
+    }
+  float py[P1];
+#pragma unroll
+  for (int a = 0; a < P1; a++) {
+    float v = 0.f;
+#pragma unroll
+    for (int b = 0; b < P1; b++) v += prol_w<P>(ty, b) * pz[b][a];
+    py[a] = v;
+  }
+#pragma unroll
+  for (int tx = 0; tx < T; tx++) {
+    const int X = T * cx + tx;
+    if (X == 0) continue;
+    float v = 0.f;
+#pragma unroll
+    for (int a = 0; a < P1; a++) v += prol_w<P>(tx, a) * py[a];
+    x[X + F * (Y + F * (long long)Z)] += damp * v;
+  }
+}
+
 template <int P>
 __global__ __launch_bounds__(kB) void lat_prolong_k(int ncc, int nnf, int nnc, const float *__restrict__ coarse,
                                                     float *__restrict__ x, float damp) {
   const long long t = (long long)blockIdx.x * kB + threadIdx.x;
-  if (t < (long long)ncc * ncc * ncc) prolong_cell<P>(t, ncc, nnf, nnc, coarse, x, damp);
+  if (t < (long long)ncc * ncc * ncc * 4 * P * P) prolong_row<P>(t, ncc, nnf, nnc, coarse, x, damp);
 }
 
 // ---------------------------------------------------------------- small levels
@@ -641,8 +693,8 @@ __global__ __launch_bounds__(small_threads<P>()) void lat_small_k(SmallArgs s) {
     const SmallLevel &L = s.L[i], &C = s.L[i - 1];
     float *x = xs[i], *t = x == L.lx ? L.lt : L.lx;
     const int ncc = (C.a.nn - 1) / P;
-    for (long long c = threadIdx.x; c < (long long)ncc * ncc * ncc; c += NT)
-      prolong_cell<P>(c, ncc, L.a.nn, C.a.nn, C.lx, x, s.damp);
+    for (long long c = threadIdx.x; c < (long long)ncc * ncc * ncc * 4 * P * P; c += NT)
+      prolong_row<P>(c, ncc, L.a.nn, C.a.nn, C.lx, x, s.damp);
     __syncthreads();
     for (int k = 0; k < s.post; k++) {
       small_sweep<P, kLJacobi, UNI>(L, x, L.lf, t, s.omega);
@@ -1326,7 +1378,7 @@ void kl_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, int order,
 }
 void kl_prolong_update(hipStream_t s, const DevLevel &F, const DevLevel &C, int order, const float *coarse, float *x,
                        float damp) {
-  const long long ncc = (C.nn - 1) / order, n = ncc * ncc * ncc;  // coarse cells
+  const long long ncc = (C.nn - 1) / order, n = ncc * ncc * ncc * 4 * order * order;  // (cell, fine Z, fine Y)
   if (n <= 0) return;
   if (order == 1) lat_prolong_k<1><<<(int)((n + kB - 1) / kB), kB, 0, s>>>((int)ncc, F.nn, C.nn, coarse, x, damp);
   else lat_prolong_k<2><<<(int)((n + kB - 1) / kB), kB, 0, s>>>((int)ncc, F.nn, C.nn, coarse, x, damp);

@@ -246,6 +246,20 @@ int check(int m, int k0) {
   }
   printf("   prolongation max err %.3g\n", emax);
   if (emax > 1e-4) bad++;
+  // the row form (one thread per cell, fine Z, fine Y) writes the same bits,
+  // onto an arbitrary fine vector, and nothing else
+  {
+    std::vector<float> x1(nn3), x2;
+    for (auto &v : x1) v = (float)rnd(g);
+    x2 = x1;
+    for (long long t = 0; t < ncc * ncc * ncc; t++) prolong_cell<P>(t, (int)ncc, nn, nc, cl.data(), x1.data(), 0.7f);
+    for (long long t = 0; t < ncc * ncc * ncc * 4 * P * P; t++)
+      prolong_row<P>(t, (int)ncc, nn, nc, cl.data(), x2.data(), 0.7f);
+    int64_t diff = 0;
+    for (size_t i = 0; i < nn3; i++) diff += memcmp(&x1[i], &x2[i], sizeof(float)) != 0;
+    printf("   prolongation rows vs cells: %lld differ\n", (long long)diff);
+    if (diff) bad++;
+  }
   return bad;
 }
 
