@@ -205,8 +205,8 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
   P.cinv = dalloc<double>(P, (size_t)na * na);
   // Krylov workspace (allocated for restart 20; grown on demand)
   P.red = dalloc<double>(P, 4 * kRedBlocks);
-  P.dscal = dalloc<double>(P, 256);
-  HIPCHK(hipHostMalloc((void **)&P.hscal, 256 * sizeof(double), hipHostMallocDefault));
+  P.dscal = dalloc<double>(P, P.nscal);
+  HIPCHK(hipHostMalloc((void **)&P.hscal, P.nscal * sizeof(double), hipHostMallocDefault));
   HIPCHK(hipHostMalloc((void **)&P.hflag, 64 * sizeof(int), hipHostMallocDefault));
 }
 
@@ -225,6 +225,16 @@ static void ensure_krylov(cd3d_problem &P, int restart) {
   P.krylov_ld = krylov_ld(N);
   P.V = dalloc<double>(P, (size_t)(restart + 1) * P.krylov_ld);
   P.Z = dalloc<double>(P, (size_t)(restart + 1) * P.krylov_ld);
+  // the H column (restart + 1 entries) and y (restart entries) share dscal
+  if (restart + 2 > P.yoff) {
+    dfree(P, P.dscal);
+    (void)hipHostFree(P.hscal);
+    P.hscal = nullptr;
+    P.yoff = restart + 2;
+    P.nscal = 2 * P.yoff;
+    P.dscal = dalloc<double>(P, P.nscal);
+    HIPCHK(hipHostMalloc((void **)&P.hscal, P.nscal * sizeof(double), hipHostMallocDefault));
+  }
   if (!P.w) {
     P.w = dalloc<double>(P, N);
     P.xs = dalloc<double>(P, N);
@@ -704,9 +714,10 @@ int fgmres(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S, double
       y[i] /= H[(size_t)i * R + i];
       for (int jj = i - 1; jj >= 0; jj--) y[jj] -= H[(size_t)jj * R + i] * y[i];
     }
-    memcpy(P.hscal + 64, y.data(), sizeof(double) * (k + 1));
-    HIPCHK(hipMemcpyAsync(P.dscal + 64, P.hscal + 64, sizeof(double) * (k + 1), hipMemcpyHostToDevice, P.stream));
-    k_multi_axpy(P.stream, N, k + 1, P.dscal + 64, Z, ld, S.x);
+    memcpy(P.hscal + P.yoff, y.data(), sizeof(double) * (k + 1));
+    HIPCHK(hipMemcpyAsync(P.dscal + P.yoff, P.hscal + P.yoff, sizeof(double) * (k + 1), hipMemcpyHostToDevice,
+                          P.stream));
+    k_multi_axpy(P.stream, N, k + 1, P.dscal + P.yoff, Z, ld, S.x);
     HIPCHK(hipStreamSynchronize(P.stream));  // pinned y is reused next time
   };
   while (j <= sp.maxit) {

@@ -164,8 +164,9 @@ struct Problem {
   int64_t krylov_ld = 0;  // column stride of V and Z
   double *V = nullptr, *Z = nullptr, *w = nullptr, *xs = nullptr, *bs = nullptr;
   double *red = nullptr;      // reduction partials
-  double *dscal = nullptr;    // device scalars (H column, norms)
+  double *dscal = nullptr;    // device scalars: H column at [0, restart], y at [yoff, yoff + restart)
   double *hscal = nullptr;    // pinned host mirror of dscal
+  int yoff = 64, nscal = 256;
   int *hflag = nullptr;       // pinned, lattice uniformity flags
   double *xl = nullptr, *bl = nullptr;  // lattice-ordered iterate and rhs of the lattice FGMRES
   int red_blocks = 0;

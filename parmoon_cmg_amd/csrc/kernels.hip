@@ -271,7 +271,10 @@ __global__ __launch_bounds__(kBlock) void jacobi_zero_k(int n, int nact, const i
 // ---------------------------------------------------------------- reductions
 // Deterministic two-stage sums: each block writes one partial; the consumer
 // sums the kRedBlocks partials in a fixed order (every block of the consumer
-// does the same sum, so all see the same value).
+// does the same sum, so all see the same value).  Products are accumulated
+// with explicit fma: under contraction a*b + c*d may fuse either product, so
+// written-out sums would leave the rounding to the compiler's choice per
+// kernel, and kernels that share a summation order would not agree bitwise.
 
 __device__ inline double block_sum(double v, double *sh) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -299,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void dot_k(int n, const double *__restrict_
                                                 double *__restrict__ partials) {
   __shared__ double sh[17];
   double v = 0;
-  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) v += a[i] * b[i];
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) v = fma(a[i], b[i], v);
   v = block_sum(v, sh);
   if (threadIdx.x == 0) partials[blockIdx.x] = v;
 }
@@ -316,17 +319,17 @@ __global__ __launch_bounds__(kBlock) void dot2_k(int n, const double *__restrict
   int i = blockIdx.x * kBlock + threadIdx.x;
   for (; i + gs < n2; i += 2 * gs) {
     const double2 p = a2[i], q = b2[i], r = a2[i + gs], t = b2[i + gs];
-    v += p.x * q.x;
-    v += p.y * q.y;
-    v += r.x * t.x;
-    v += r.y * t.y;
+    v = fma(p.x, q.x, v);
+    v = fma(p.y, q.y, v);
+    v = fma(r.x, t.x, v);
+    v = fma(r.y, t.y, v);
   }
   if (i < n2) {
     const double2 p = a2[i], q = b2[i];
-    v += p.x * q.x;
-    v += p.y * q.y;
+    v = fma(p.x, q.x, v);
+    v = fma(p.y, q.y, v);
   }
-  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) v += a[n - 1] * b[n - 1];
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) v = fma(a[n - 1], b[n - 1], v);
   v = block_sum(v, sh);
   if (threadIdx.x == 0) partials[blockIdx.x] = v;
 }
@@ -349,17 +352,20 @@ __global__ __launch_bounds__(kBlock) void axpy_dot_k(int n, const double *__rest
   const double a = sign * alpha;
   double v = 0;
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const double yi = y[i] + a * x[i];
+    const double yi = fma(a, x[i], y[i]);
     y[i] = yi;
-    v += yi * (z ? z[i] : yi);
+    v = fma(yi, z ? z[i] : yi, v);
   }
   v = block_sum(v, sh);
   if (threadIdx.x == 0) partials[blockIdx.x] = v;
 }
 
 // axpy_dot_k on 16-byte pairs (Z: a separate z vector, else the dot is <y, y>).
-// The loads of a thread's first two pairs do not depend on alpha: they are
-// issued before the reduction of the alpha partials, which hides its latency.
+// The loads of a thread's first kAxpyPre pairs do not depend on alpha: they
+// are all issued before the reduction of the alpha partials, which hides its
+// latency and keeps the whole stream in flight (kRedBlocks x kBlock threads
+// take the 2^20 pairs of a 2 M-unknown level in one round).
+constexpr int kAxpyPre = 4;
 template <bool Z>
 __global__ __launch_bounds__(kBlock) void axpy_dot2_k(int n, const double *__restrict__ alpha_partials,
                                                       double *__restrict__ alpha_out, double sign,
@@ -369,43 +375,46 @@ __global__ __launch_bounds__(kBlock) void axpy_dot2_k(int n, const double *__res
   const int n2 = n >> 1, gs = gridDim.x * kBlock;
   const double2 *x2 = (const double2 *)x, *z2 = (const double2 *)z;
   double2 *y2 = (double2 *)y;
-  int i = blockIdx.x * kBlock + threadIdx.x;
-  const bool ha = i < n2, hb = i + gs < n2;
+  const int i0 = blockIdx.x * kBlock + threadIdx.x;
   const double2 zero = {0, 0};
-  const double2 xa = ha ? x2[i] : zero, xb = hb ? x2[i + gs] : zero;
-  double2 ya = ha ? y2[i] : zero, yb = hb ? y2[i + gs] : zero;
-  const double2 za = (Z && ha) ? z2[i] : zero, zb = (Z && hb) ? z2[i + gs] : zero;
+  double2 xa[kAxpyPre], ya[kAxpyPre], za[kAxpyPre];
+#pragma unroll
+  for (int q = 0; q < kAxpyPre; q++) {
+    const int i = i0 + q * gs;
+    const bool h = i < n2;
+    xa[q] = h ? x2[i] : zero;
+    ya[q] = h ? y2[i] : zero;
+    za[q] = (Z && h) ? z2[i] : zero;
+  }
   const double alpha = sum_partials(alpha_partials, sh);
   if (blockIdx.x == 0 && threadIdx.x == 0 && alpha_out) *alpha_out = alpha;
   const double a = sign * alpha;
   double v = 0;
-  if (ha) {
-    ya.x += a * xa.x;
-    ya.y += a * xa.y;
-    y2[i] = ya;
-    v += ya.x * (Z ? za.x : ya.x);
-    v += ya.y * (Z ? za.y : ya.y);
+#pragma unroll
+  for (int q = 0; q < kAxpyPre; q++) {
+    const int i = i0 + q * gs;
+    if (i < n2) {
+      double2 yi = ya[q];
+      yi.x = fma(a, xa[q].x, yi.x);
+      yi.y = fma(a, xa[q].y, yi.y);
+      y2[i] = yi;
+      v = fma(yi.x, Z ? za[q].x : yi.x, v);
+      v = fma(yi.y, Z ? za[q].y : yi.y, v);
+    }
   }
-  if (hb) {
-    yb.x += a * xb.x;
-    yb.y += a * xb.y;
-    y2[i + gs] = yb;
-    v += yb.x * (Z ? zb.x : yb.x);
-    v += yb.y * (Z ? zb.y : yb.y);
-  }
-  for (i += 2 * gs; i < n2; i += gs) {
+  for (int i = i0 + kAxpyPre * gs; i < n2; i += gs) {
     double2 yi = y2[i];
     const double2 xi = x2[i], zi = Z ? z2[i] : zero;
-    yi.x += a * xi.x;
-    yi.y += a * xi.y;
+    yi.x = fma(a, xi.x, yi.x);
+    yi.y = fma(a, xi.y, yi.y);
     y2[i] = yi;
-    v += yi.x * (Z ? zi.x : yi.x);
-    v += yi.y * (Z ? zi.y : yi.y);
+    v = fma(yi.x, Z ? zi.x : yi.x, v);
+    v = fma(yi.y, Z ? zi.y : yi.y, v);
   }
   if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
-    const double yi = y[n - 1] + a * x[n - 1];
+    const double yi = fma(a, x[n - 1], y[n - 1]);
     y[n - 1] = yi;
-    v += yi * (Z ? z[n - 1] : yi);
+    v = fma(yi, Z ? z[n - 1] : yi, v);
   }
   v = block_sum(v, sh);
   if (threadIdx.x == 0) partials[blockIdx.x] = v;
