@@ -261,6 +261,9 @@ __host__ __device__ inline void pair_sums(const LatArgs &a, const PairPos &q, lo
   const long long nn = a.nn;
   const float *__restrict__ v0 = UNI ? nullptr : val + V.off + (q.hv ? q.kv : 0);
   const float *__restrict__ v1 = UNI ? nullptr : val + M.off + k;
+  // a boundary vertex (ix = 0) has no sum: its left segment is read from the
+  // centre instead (in bounds, unused), so the loads need no branch
+  const int lft_off = q.hv ? -2 : 0;
   s0 = s1 = xv = xm = T(0);
   constexpr int ny = hi_off<2, PY>() - lo_off<2, PY>() + 1;
   // planes in a rolled loop: one plane's loads (ny lines) in flight at a time
@@ -272,7 +275,7 @@ __host__ __device__ inline void pair_sums(const LatArgs &a, const PairPos &q, lo
     for (int dy = lo_off<2, PY>(); dy <= hi_off<2, PY>(); dy++) {
       const int li = l0 + dy - lo_off<2, PY>();
       const T *xl = x + q.nv + nn * (dy + nn * dz);
-      const V2 lft = q.hv ? *(const V2 *)(xl - 2) : V2{T(0), T(0)};
+      const V2 lft = *(const V2 *)(xl + lft_off);
       const V2 ctr = *(const V2 *)xl;
       const T rgt = xl[2];
       if (dy == 0 && dz == 0) {
@@ -282,7 +285,7 @@ __host__ __device__ inline void pair_sums(const LatArgs &a, const PairPos &q, lo
       const T w[5] = {lft.x, lft.y, ctr.x, ctr.y, rgt};
 #pragma unroll
       for (int j = 0; j < 5; j++)
-        s0 += (UNI ? cs0[li * 5 + j] : (q.hv ? T(v0[(long long)(li * 5 + j) * V.nodes]) : T(0))) * w[j];
+        s0 += (UNI ? cs0[li * 5 + j] : T(v0[(long long)(li * 5 + j) * V.nodes])) * w[j];
 #pragma unroll
       for (int j = 0; j < 3; j++)
         s1 += (UNI ? cs1[li * 3 + j] : T(v1[(long long)(li * 3 + j) * M.nodes])) * w[j + 2];
@@ -361,6 +364,151 @@ __global__ __launch_bounds__(kB) void lat_pair_k(LatArgs a, const float *__restr
   }
   const long long k = (long long)tile * kB + threadIdx.x;
   if (k < a.c[2 * pc + 1].nodes) lat_pair_node<MODE, UNI>(a, pc, k, val, inv, cs, cs + kCst, x, f, out, omega);
+}
+
+// 16-byte vector at a 4-byte aligned address
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+// 16 bytes of T (at the alignment of T)
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  typedef f4u type;
+  static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+  typedef Vec2<double>::type type;
+  static constexpr int n = 2;
+};
+
+// G vertex/midpoint pairs per thread (compact stencils): the thread's nodes
+// X0 .. X0 + 2G - 1 (X0 = 2 G g on its line) need the segment X0-2 .. X0+2G
+// of every neighbour line, read as one 2-element load, 16-byte loads and one
+// 1-element load (FP32: 2 + G/2 loads for 2G nodes, lat_pair_k needs 2 per
+// pair; FP64: 2 + G instead of 3 per pair).  Every node's sum keeps the
+// per-class order, so the results are bitwise those of one pair per thread.
+// The group's first vertex is a boundary node when g = 0: it is not written,
+// and its left segment is read from the centre.
+template <int G, int PY, int PZ>
+__host__ __device__ inline long long group_pos(const LatArgs &a, long long k, bool &hv) {
+  const LatDev &M = a.c[2 * PY + 4 * PZ + 1];
+  const int ng = M.cnt[0] / G;  // groups per line
+  const int g = (int)(k % ng);
+  const long long r = k / ng;
+  const int Y = M.lo[1] + 2 * (int)(r % M.cnt[1]), Z = M.lo[2] + 2 * (int)(r / M.cnt[1]);
+  hv = g > 0;
+  return 2LL * G * g + (long long)a.nn * (Y + (long long)a.nn * Z);
+}
+
+template <typename T, int G, int PY, int PZ>
+__host__ __device__ inline void group_sums(const LatArgs &a, long long nv, bool hv, const T *cs0, const T *cs1,
+                                           const T *__restrict__ x, T (&s0)[G], T (&s1)[G], T (&xs)[2 * G]) {
+  static_assert(G == 2 || G == 4, "two or four pairs per thread");
+  typedef typename Vec2<T>::type V2;
+  typedef typename Vec16<T>::type V16;
+  constexpr int vn = Vec16<T>::n;
+  const long long nn = a.nn;
+#pragma unroll
+  for (int q = 0; q < G; q++) s0[q] = s1[q] = T(0);
+  const int lft_off = hv ? -2 : 0;
+  constexpr int ny = hi_off<2, PY>() - lo_off<2, PY>() + 1;
+#pragma unroll 1
+  for (int dz = lo_off<2, PZ>(); dz <= hi_off<2, PZ>(); dz++) {
+    const int l0 = (dz - lo_off<2, PZ>()) * ny;
+#pragma unroll
+    for (int dy = lo_off<2, PY>(); dy <= hi_off<2, PY>(); dy++) {
+      const int li = l0 + dy - lo_off<2, PY>();
+      const T *xl = x + nv + nn * (dy + nn * dz);
+      T w[2 * G + 3];
+      const V2 lft = *(const V2 *)(xl + lft_off);
+      w[0] = lft.x;
+      w[1] = lft.y;
+#pragma unroll
+      for (int h = 0; h < 2 * G / vn; h++) {
+        const V16 m = *(const V16 *)(xl + vn * h);
+#pragma unroll
+        for (int e = 0; e < vn; e++) w[2 + vn * h + e] = m[e];
+      }
+      w[2 * G + 2] = xl[2 * G];
+      if (dy == 0 && dz == 0) {
+#pragma unroll
+        for (int j = 0; j < 2 * G; j++) xs[j] = w[2 + j];
+      }
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+#pragma unroll
+        for (int j = 0; j < 5; j++) s0[q] += cs0[li * 5 + j] * w[2 * q + j];
+#pragma unroll
+        for (int j = 0; j < 3; j++) s1[q] += cs1[li * 3 + j] * w[2 * q + 2 + j];
+      }
+    }
+  }
+}
+
+// FP32 sweep (modes of lat_node) of group k of pair class (PY, PZ)
+template <int G, int PY, int PZ, int MODE>
+__host__ __device__ inline void lat_pair_group(const LatArgs &a, long long k, const float *cs0, const float *cs1,
+                                      const float *__restrict__ x, const float *__restrict__ f,
+                                      float *__restrict__ out, float omega) {
+  bool hv;
+  const long long nv = group_pos<G, PY, PZ>(a, k, hv);
+  const float d0 = cs0[kCst - 1], d1 = cs1[kCst - 1];
+  float fv[2 * G];
+#pragma unroll
+  for (int h = 0; h < G / 2; h++) {
+    const f4u t = *(const f4u *)(f + nv + 4 * h);
+#pragma unroll
+    for (int e = 0; e < 4; e++) fv[4 * h + e] = t[e];
+  }
+  if (MODE == kLJacobiZero) {
+#pragma unroll
+    for (int q = 0; q < G; q++) {
+      if (q > 0 || hv) out[nv + 2 * q] = omega * fv[2 * q] * d0;
+      out[nv + 2 * q + 1] = omega * fv[2 * q + 1] * d1;
+    }
+    return;
+  }
+  float s0[G], s1[G], xs[2 * G];
+  group_sums<float, G, PY, PZ>(a, nv, hv, cs0, cs1, x, s0, s1, xs);
+#pragma unroll
+  for (int q = 0; q < G; q++) {
+    if (MODE == kLDefect) {
+      if (q > 0 || hv) out[nv + 2 * q] = fv[2 * q] - s0[q];
+      out[nv + 2 * q + 1] = fv[2 * q + 1] - s1[q];
+    } else {
+      if (q > 0 || hv) out[nv + 2 * q] = xs[2 * q] + omega * (fv[2 * q] - s0[q]) * d0;
+      out[nv + 2 * q + 1] = xs[2 * q + 1] + omega * (fv[2 * q + 1] - s1[q]) * d1;
+    }
+  }
+}
+
+template <int G, int MODE>
+__host__ __device__ inline void lat_pair_group_node(const LatArgs &a, int pc, long long k, const float *cs0,
+                                                    const float *cs1, const float *__restrict__ x,
+                                                    const float *__restrict__ f, float *__restrict__ out,
+                                                    float omega) {
+  switch (pc) {
+    case 0: lat_pair_group<G, 0, 0, MODE>(a, k, cs0, cs1, x, f, out, omega); break;
+    case 1: lat_pair_group<G, 1, 0, MODE>(a, k, cs0, cs1, x, f, out, omega); break;
+    case 2: lat_pair_group<G, 0, 1, MODE>(a, k, cs0, cs1, x, f, out, omega); break;
+    default: lat_pair_group<G, 1, 1, MODE>(a, k, cs0, cs1, x, f, out, omega); break;
+  }
+}
+
+// blocks in pair-class order, a.pstart counted in groups of G pairs
+template <int MODE, int G>
+__global__ __launch_bounds__(kB) void lat_pair_gk(LatArgs a, const float *__restrict__ cst,
+                                                  const float *__restrict__ x, const float *__restrict__ f,
+                                                  float *__restrict__ out, float omega) {
+  __shared__ float cs[2 * kCst];
+  int pc = 0;
+  while (pc < 3 && (int)blockIdx.x >= a.pstart[pc + 1]) pc++;
+  const int tile = blockIdx.x - a.pstart[pc];
+  cs[threadIdx.x] = cst[2 * pc * kCst + threadIdx.x];
+  __syncthreads();
+  const long long k = (long long)tile * kB + threadIdx.x;
+  if (k < a.c[2 * pc + 1].nodes / G) lat_pair_group_node<G, MODE>(a, pc, k, cs, cs + kCst, x, f, out, omega);
 }
 
 template <int P, int MODE>
@@ -1160,6 +1308,58 @@ __global__ __launch_bounds__(kB) void lat64_pair_k(LatArgs a, const double *__re
   if (k < a.c[2 * pc + 1].nodes) lat64_pair_node<MODE>(a, pc, k, cs, cs + kCst, x, f, out);
 }
 
+// FP64 operator by groups of G pairs (group_sums<double>); blocks
+// pstart[4].. (counted in groups) take the Dirichlet rows as in lat64_pair_k
+template <int G, int PY, int PZ, int MODE>
+__host__ __device__ inline void lat64_pair_group(const LatArgs &a, long long k, const double *cs0, const double *cs1,
+                                        const double *__restrict__ x, const double *__restrict__ f,
+                                        double *__restrict__ out) {
+  bool hv;
+  const long long nv = group_pos<G, PY, PZ>(a, k, hv);
+  double s0[G], s1[G], xs[2 * G];
+  group_sums<double, G, PY, PZ>(a, nv, hv, cs0, cs1, x, s0, s1, xs);
+#pragma unroll
+  for (int q = 0; q < G; q++) {
+    const long long v = nv + 2 * q;
+    if (q > 0 || hv) out[v] = MODE == kL64Defect ? f[v] - s0[q] : s0[q];
+    out[v + 1] = MODE == kL64Defect ? f[v + 1] - s1[q] : s1[q];
+  }
+}
+
+template <int G, int MODE>
+__host__ __device__ inline void lat64_pair_group_node(const LatArgs &a, int pc, long long k, const double *cs0,
+                                                      const double *cs1, const double *__restrict__ x,
+                                                      const double *__restrict__ f, double *__restrict__ out) {
+  switch (pc) {
+    case 0: lat64_pair_group<G, 0, 0, MODE>(a, k, cs0, cs1, x, f, out); break;
+    case 1: lat64_pair_group<G, 1, 0, MODE>(a, k, cs0, cs1, x, f, out); break;
+    case 2: lat64_pair_group<G, 0, 1, MODE>(a, k, cs0, cs1, x, f, out); break;
+    default: lat64_pair_group<G, 1, 1, MODE>(a, k, cs0, cs1, x, f, out); break;
+  }
+}
+
+template <int MODE, int G>
+__global__ __launch_bounds__(kB) void lat64_pair_gk(LatArgs a, const double *__restrict__ st64,
+                                                    const double *__restrict__ x, const double *__restrict__ f,
+                                                    double *__restrict__ out, int nact, int ndof,
+                                                    const int *__restrict__ dof2node) {
+  __shared__ double cs[2 * kCst];
+  if ((int)blockIdx.x >= a.pstart[4]) {
+    const int d = nact + (blockIdx.x - a.pstart[4]) * kB + threadIdx.x;
+    if (d < ndof) {
+      const int node = dof2node[d];
+      out[node] = MODE == kL64Defect ? f[node] - x[node] : x[node];
+    }
+    return;
+  }
+  int pc = 0;
+  while (pc < 3 && (int)blockIdx.x >= a.pstart[pc + 1]) pc++;
+  cs[threadIdx.x] = st64[2 * pc * kCst + threadIdx.x];
+  __syncthreads();
+  const long long k = (long long)(blockIdx.x - a.pstart[pc]) * kB + threadIdx.x;
+  if (k < a.c[2 * pc + 1].nodes / G) lat64_pair_group_node<G, MODE>(a, pc, k, cs, cs + kCst, x, f, out);
+}
+
 // FP64 operator of a rotating-field level: entry s of a node (X, Y) is
 // alpha (S0 + X SX + Y SY) (conv-diff alone, alpha 1) or
 // fma(alpha, S0 + X SX + Y SY, M) (the TCD3D system matrix M + tau th1 A),
@@ -1321,6 +1521,37 @@ static bool lat_pairs() {
   return v;
 }
 
+// Pairs per thread of the compact-stencil Q2 kernels: groups of gmax where
+// that still leaves kGroupMinThreads threads, else single pairs.  Measured on
+// the 129^3-node level of a 64^3-cell Q2 mesh: FP32 sweep 22.8 / 19.0 /
+// 18.5 us with 1 / 2 / 4 pairs per thread (gmax 4), FP64 operator 29.7 /
+// 26.4 / 37.0 us (gmax 2); the 65^3-node level is slower with groups (too
+// few threads).  CD3D_PAIR_GROUP = 1, 2 or 4 forces a size; one that does
+// not divide the pairs per line falls back to 1.
+constexpr long long kGroupMinThreads = 1 << 17;
+static int pair_group(const LatArgs &a, int gmax) {
+  long long pairs = 0;
+  for (int pc = 0; pc < 4; pc++) pairs += a.c[2 * pc + 1].nodes;
+  const char *e = getenv("CD3D_PAIR_GROUP");
+  const int G = e ? atoi(e) : pairs / gmax >= kGroupMinThreads ? gmax : 1;
+  if (G != 2 && G != 4) return 1;
+  for (int pc = 0; pc < 4; pc++)
+    if (a.c[2 * pc + 1].cnt[0] % G) return 1;
+  return G;
+}
+
+// pstart counted in groups of G pairs
+static LatArgs group_args(const LatArgs &a, int G) {
+  LatArgs ag = a;
+  int b = 0;
+  for (int pc = 0; pc < 4; pc++) {
+    ag.pstart[pc] = b;
+    b += (int)((a.c[2 * pc + 1].nodes / G + kB - 1) / kB);
+  }
+  ag.pstart[4] = b;
+  return ag;
+}
+
 static LatAff lat_aff(const DevLevel &L) {
   LatAff af{};
   af.st = L.lst64;
@@ -1349,8 +1580,13 @@ void lat_sweep(hipStream_t s, const DevLevel &L, int order, const float *x, cons
     else lat_aff_sweep_k<2, MODE><<<grid, kB, 0, s>>>(a, af, x, f, out, omega);
     return;
   }
+  const int G = order == 2 && lat_pairs() && L.nn > kPairMinNN && all_uni(a) ? pair_group(a, 4) : 1;
   if (order == 1) lat_sweep_k<1, MODE><<<grid, kB, 0, s>>>(a, L.lval, L.linv, L.lcst, x, f, out, omega);
-  else if (lat_pairs() && L.nn > kPairMinNN && all_uni(a))
+  else if (G > 1) {
+    const LatArgs ag = group_args(a, G);
+    if (G == 2) lat_pair_gk<MODE, 2><<<ag.pstart[4], kB, 0, s>>>(ag, L.lcst, x, f, out, omega);
+    else lat_pair_gk<MODE, 4><<<ag.pstart[4], kB, 0, s>>>(ag, L.lcst, x, f, out, omega);
+  } else if (lat_pairs() && L.nn > kPairMinNN && all_uni(a))
     lat_pair_k<MODE, true><<<a.pstart[4], kB, 0, s>>>(a, L.lval, L.linv, L.lcst, x, f, out, omega);
   else if (lat_pairs() && L.nn > kPairMinNN)
     lat_pair_k<MODE, false><<<a.pstart[4], kB, 0, s>>>(a, L.lval, L.linv, L.lcst, x, f, out, omega);
@@ -1517,6 +1753,14 @@ void lat64_apply(hipStream_t s, const DevLevel &L, int order, const double *x, c
     return;
   }
   if (order == 2 && grid > 0 && lat_pairs() && L.nn > kPairMinNN) {  // interior and Dirichlet rows in one launch
+    const int G = pair_group(a, 2);
+    if (G > 1) {
+      const LatArgs ag = group_args(a, G);
+      const int g = ag.pstart[4] + (nd + kB - 1) / kB;
+      if (G == 2) lat64_pair_gk<MODE, 2><<<g, kB, 0, s>>>(ag, L.lst64, x, f, out, L.nact, L.ndof, L.ldof2node);
+      else lat64_pair_gk<MODE, 4><<<g, kB, 0, s>>>(ag, L.lst64, x, f, out, L.nact, L.ndof, L.ldof2node);
+      return;
+    }
     lat64_pair_k<MODE><<<a.pstart[4] + (nd + kB - 1) / kB, kB, 0, s>>>(a, L.lst64, x, f, out, L.nact, L.ndof,
                                                                        L.ldof2node);
     return;

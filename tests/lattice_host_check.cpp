@@ -313,6 +313,27 @@ int check_pairs(int m, int k0) {
       printf("   pairs m%d k0=%d uni %d mode %d: %lld pairs, %lld nodes written, %lld differ\n", m, k0, uni, mode,
              (long long)npairs, (long long)written, (long long)diff);
       if (diff || written < F.nact - 2) bad++;
+      // groups of 2 and 4 pairs per thread (compact stencils) against single pairs
+      for (int G = 2; uni && G <= 4; G += 2) {
+        if (a.c[1].cnt[0] % G) continue;
+        std::vector<float> o3(nn3, 7.f);
+        for (int pc = 0; pc < 4; pc++)
+          for (int64_t k = 0; k < a.c[2 * pc + 1].nodes / G; k++) {
+            const float *c0 = cst.data() + 2 * pc * kCst, *c1 = cst.data() + (2 * pc + 1) * kCst;
+#define CD3D_GRP(GG, M) lat_pair_group_node<GG, M>(a, pc, k, c0, c1, x.data(), f.data(), o3.data(), 0.7f)
+            if (G == 2 && mode == kLJacobi) CD3D_GRP(2, kLJacobi);
+            if (G == 2 && mode == kLJacobiZero) CD3D_GRP(2, kLJacobiZero);
+            if (G == 2 && mode == kLDefect) CD3D_GRP(2, kLDefect);
+            if (G == 4 && mode == kLJacobi) CD3D_GRP(4, kLJacobi);
+            if (G == 4 && mode == kLJacobiZero) CD3D_GRP(4, kLJacobiZero);
+            if (G == 4 && mode == kLDefect) CD3D_GRP(4, kLDefect);
+#undef CD3D_GRP
+          }
+        int64_t gd = 0;
+        for (size_t i = 0; i < nn3; i++) gd += memcmp(&o2[i], &o3[i], sizeof(float)) != 0;
+        printf("   groups of %d m%d k0=%d mode %d: %lld differ\n", G, m, k0, mode, (long long)gd);
+        if (gd) bad++;
+      }
     }
   }
   // affine stencils evaluated node by node against per-node values filled
@@ -403,6 +424,20 @@ int check_pairs(int m, int k0) {
       for (size_t i = 0; i < nn3; i++) diff += memcmp(&o1[i], &o2[i], sizeof(double)) != 0;
       printf("   f64 pairs m%d k0=%d mode %d: %lld differ\n", m, k0, mode, (long long)diff);
       if (diff) bad++;
+      for (int G = 2; G <= 4; G += 2) {
+        if (a.c[1].cnt[0] % G) continue;
+        std::vector<double> o3(nn3, 7.0);
+        for (int pc = 0; pc < 4; pc++)
+          for (int64_t k = 0; k < a.c[2 * pc + 1].nodes / G; k++) {
+            const double *c0 = st.data() + 2 * pc * kCst, *c1 = st.data() + (2 * pc + 1) * kCst;
+            if (G == 2) (mode ? lat64_pair_group_node<2, kL64Defect> : lat64_pair_group_node<2, kL64Spmv>)(a, pc, k, c0, c1, xd.data(), fd.data(), o3.data());
+            else (mode ? lat64_pair_group_node<4, kL64Defect> : lat64_pair_group_node<4, kL64Spmv>)(a, pc, k, c0, c1, xd.data(), fd.data(), o3.data());
+          }
+        int64_t gd = 0;
+        for (size_t i = 0; i < nn3; i++) gd += memcmp(&o2[i], &o3[i], sizeof(double)) != 0;
+        printf("   f64 groups of %d m%d k0=%d mode %d: %lld differ\n", G, m, k0, mode, (long long)gd);
+        if (gd) bad++;
+      }
     }
   }
   return bad;

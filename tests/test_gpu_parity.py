@@ -263,6 +263,33 @@ def test_small_level_program_matches_launch_per_step(order, m, k0, levels, conv_
     assert out["0"][2] == out["100000"][2]
 
 
+@pytest.mark.parametrize("m,k0,levels", [(1, 1, 5), (3, 0, 4), (11, 0, 2)])
+def test_pair_groups_match_single_pairs(m, k0, levels, monkeypatch):
+    # the Q2 compact-stencil sweep with 2 or 4 vertex/midpoint pairs per
+    # thread against one pair per thread: the same per-node sums, so the
+    # V-cycle and the mixed solve are bitwise equal.  32, 24 and 22 cells per
+    # side: 22 pairs per line do not split into groups of 4 (that size falls
+    # back to single pairs)
+    P = pm.Problem(2, m, k0, levels, box=(1.0, 0.7, 0.5))
+    P.assemble(pm.coeffs(eps=1e-3, b=(1.0, 0.5, 0.25), supg=1, f_val=1.0))
+    i = P.info(levels - 1)
+    r = np.random.default_rng(5).standard_normal(i["ndof"])
+    r[i["nact"]:] = 0.0
+    sp = pm.solver_params(**dict(REF_SOLVER, omega=0.5, pre=3, post=3, coarse_smoother=17, rel_tol=1e-10),
+                          precision=pm.MIXED)
+    out = {}
+    for g in ("1", "2", "4"):
+        monkeypatch.setenv("CD3D_PAIR_GROUP", g)
+        z = P.vcycle(sp, r)
+        x, st = P.solve(sp)
+        out[g] = (z, x, st.iterations)
+    say(f"[gpu] pair groups m={m} k0={k0} levels={levels}: its {[out[g][2] for g in out]}")
+    for g in ("2", "4"):
+        assert np.array_equal(out[g][0], out["1"][0])
+        assert np.array_equal(out[g][1], out["1"][1])
+        assert out[g][2] == out["1"][2]
+
+
 @pytest.mark.parametrize("order,m,k0,levels", [(1, 1, 2, 4), (2, 1, 1, 3)])
 def test_rotating_field_lattice_fgmres(order, m, k0, levels):
     # Galerkin rotating field: the CSR values are the affine class stencils,
