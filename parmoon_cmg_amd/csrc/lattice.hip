@@ -610,6 +610,76 @@ __global__ __launch_bounds__(kB) void lat_restrict_k(LatArgs ca, int nnf, const 
   if (k < ca.c[c].nodes) restrict_class_node<P>(ca, c, k, nnf, fine, coarse);
 }
 
+// Q2 restriction by coarse vertex/midpoint pairs along X (the pair classes
+// of the coarse level, pair_pos): coarse nodes Xc (vertex) and Xc + 1
+// (midpoint) gather fine X offsets -3 -1 0 1 3 around 2 Xc and -1 0 1
+// around 2 Xc + 2, together the segment 2 Xc - 3 .. 2 Xc + 3 of each fine
+// line (one 1-, one 2- and one 4-element load, consecutive lanes on
+// consecutive pairs) instead of 5 + 3 loads at a stride of 4 floats.  Each
+// node's sum keeps restrict_node's order.  A boundary vertex (ix = 0) is not
+// written; its left values are read from the centre.
+template <int PY, int PZ>
+__host__ __device__ inline void restrict_pair(const LatArgs &ca, long long k, int nnf, const float *__restrict__ fine,
+                                              float *__restrict__ coarse) {
+  using RX0 = Restr1D<2, 0>;
+  using RX1 = Restr1D<2, 1>;
+  using RY = Restr1D<2, PY>;
+  using RZ = Restr1D<2, PZ>;
+  const PairPos q = pair_pos<PY, PZ>(ca, k);
+  const long long nnc = ca.nn, F = nnf;
+  const long long Xc = q.nv % nnc, Yc = (q.nv / nnc) % nnc, Zc = q.nv / (nnc * nnc);
+  const long long centre = 2 * Xc + F * (2 * Yc + F * 2 * Zc);
+  const int lo = q.hv ? 0 : 3;
+  float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+  for (int c = 0; c < RZ::n; c++) {
+    float sy0 = 0.f, sy1 = 0.f;
+#pragma unroll
+    for (int b = 0; b < RY::n; b++) {
+      const float *fl = fine + centre + F * (RY::off(b) + F * RZ::off(c));
+      float w[7];
+      w[0] = fl[lo - 3];
+      const f2u l2 = *(const f2u *)(fl + lo - 2);
+      w[1] = l2.x;
+      w[2] = l2.y;
+      const f4u m = *(const f4u *)fl;
+      w[3] = m.x;
+      w[4] = m.y;
+      w[5] = m.z;
+      w[6] = m.w;
+      float sx0 = 0.f, sx1 = 0.f;
+#pragma unroll
+      for (int a = 0; a < RX0::n; a++) sx0 += RX0::w(a) * w[3 + RX0::off(a)];
+#pragma unroll
+      for (int a = 0; a < RX1::n; a++) sx1 += RX1::w(a) * w[5 + RX1::off(a)];
+      sy0 += RY::w(b) * sx0;
+      sy1 += RY::w(b) * sx1;
+    }
+    s0 += RZ::w(c) * sy0;
+    s1 += RZ::w(c) * sy1;
+  }
+  if (q.hv) coarse[q.nv] = s0;
+  coarse[q.nv + 1] = s1;
+}
+
+__host__ __device__ inline void restrict_pair_node(const LatArgs &ca, int pc, long long k, int nnf,
+                                                   const float *__restrict__ fine, float *__restrict__ coarse) {
+  switch (pc) {
+    case 0: restrict_pair<0, 0>(ca, k, nnf, fine, coarse); break;
+    case 1: restrict_pair<1, 0>(ca, k, nnf, fine, coarse); break;
+    case 2: restrict_pair<0, 1>(ca, k, nnf, fine, coarse); break;
+    default: restrict_pair<1, 1>(ca, k, nnf, fine, coarse); break;
+  }
+}
+
+__global__ __launch_bounds__(kB) void lat_restrict_pair_k(LatArgs ca, int nnf, const float *__restrict__ fine,
+                                                          float *__restrict__ coarse) {
+  int pc = 0;
+  while (pc < 3 && (int)blockIdx.x >= ca.pstart[pc + 1]) pc++;
+  const long long k = (long long)(blockIdx.x - ca.pstart[pc]) * kB + threadIdx.x;
+  if (k < ca.c[2 * pc + 1].nodes) restrict_pair_node(ca, pc, k, nnf, fine, coarse);
+}
+
 // Prolongation + CorrectNodes + Update, one thread per coarse cell: the cell's
 // (P+1)^3 coarse values interpolated to its fine nodes with local index
 // 0..2P-1 per axis (the last plane belongs to the next cell) by three 1D
@@ -1610,6 +1680,7 @@ void kl_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, int order,
   const int grid = ca.bstart[C.nclass];
   if (grid <= 0) return;
   if (order == 1) lat_restrict_k<1><<<grid, kB, 0, s>>>(ca, F.nn, fine, coarse);
+  else if (lat_pairs() && C.nn > kPairMinNN) lat_restrict_pair_k<<<ca.pstart[4], kB, 0, s>>>(ca, F.nn, fine, coarse);
   else lat_restrict_k<2><<<grid, kB, 0, s>>>(ca, F.nn, fine, coarse);
 }
 void kl_prolong_update(hipStream_t s, const DevLevel &F, const DevLevel &C, int order, const float *coarse, float *x,

@@ -220,6 +220,15 @@ int check(int m, int k0) {
     LatArgs ca = lat_args(DC);
     for (int c = 0; c < DC.nclass; c++)
       for (int64_t k = 0; k < C.lat[c].nodes; k++) restrict_class_node<P>(ca, c, k, nn, fl.data(), cl.data());
+    if (P == 2) {  // by coarse vertex/midpoint pairs: the same bits
+      std::vector<float> cp((size_t)nc * nc * nc, 0.f);
+      for (int pc = 0; pc < 4; pc++)
+        for (int64_t k = 0; k < ca.c[2 * pc + 1].nodes; k++) restrict_pair_node(ca, pc, k, nn, fl.data(), cp.data());
+      int64_t rd = 0;
+      for (size_t i = 0; i < cp.size(); i++) rd += memcmp(&cp[i], &cl[i], sizeof(float)) != 0;
+      printf("   restriction by pairs: %lld differ\n", (long long)rd);
+      if (rd) bad++;
+    }
   }
   emax = 0;
   for (int d = 0; d < C.ndof; d++) {
