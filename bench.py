@@ -70,7 +70,7 @@ def kernel_table(P, reps=50):
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "cd3d_q2_64_fine_level_traffic.json")
 TRAFFIC_KERNEL = {0: "void csr_stream<0, double, double, true>", 1: "void lat_pair_gk<0, 4>",
                   2: "void lat64_pair_gk<0, 2>", 3: "lat_restrict_pair_k", 4: "void lat_prolong_k<2>",
-                  5: "csr_fill4_k", 6: "copy_k"}
+                  5: "csr_fill2_k", 6: "copy_k"}
 
 
 def measured_traffic(which):
