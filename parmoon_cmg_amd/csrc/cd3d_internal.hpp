@@ -189,7 +189,20 @@ void basis_tables(int order, std::vector<double> &tab);
 int quad_points_1d(int order);
 
 // ---------------------------------------------------------------- kernels (kernels.hip)
-constexpr int kRedBlocks = 1024;
+// Workgroup b of a launch runs on XCD b mod 8 (round-robin dispatch over the
+// 8 XCDs of an MI355X, each with its own L2).  xcd_block maps b to a logical
+// block so that XCD x takes the contiguous range of logical blocks
+// [x q + min(x, r), ...) of its q or q + 1 blocks (n = 8 q + r): a
+// bijection of 0..n-1 onto itself.
+__host__ __device__ inline int xcd_block(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+#ifndef CD3D_RED_BLOCKS
+#define CD3D_RED_BLOCKS 1024
+#endif
+constexpr int kRedBlocks = CD3D_RED_BLOCKS;
 // CSR-stream row blocks: at most kStreamNnz nonzeros and kStreamRows rows,
 // processed by kStreamBlock threads
 #ifndef CD3D_STREAM_NNZ
