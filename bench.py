@@ -200,7 +200,7 @@ def tcd_run(pm, precision, steps, n=128, levels=5):
     return dict(steps=steps, steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps, iterations=its), u
 
 
-def batched_uneven(pm, precision, count=256, seed=20261202, passes=2):
+def batched_uneven(pm, precision, count=256, seed=20261202, passes=2, inflight=16):
     """Configuration 4: `count` independent Q1 SUPG problems, n log-uniform in
     [8, 48] (snapped to m 2^k), eps log-uniform in [1e-6, 1], b a random unit
     direction times U(0, 1), f = 1; all submitted at once to the server.  The
@@ -212,7 +212,7 @@ def batched_uneven(pm, precision, count=256, seed=20261202, passes=2):
         co, _ = random_coeffs(pm, rng)
         reqs.append((pm.mesh_desc(1, m, 0, levels), co))
     sp = pm.solver_params(**dict(SOLVER, maxit=1000), precision=precision)
-    srv = pm.Server(max_inflight=16)
+    srv = pm.Server(max_inflight=inflight)
     out = None
     for _ in range(passes):
         t = time.perf_counter()
@@ -307,6 +307,7 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="only the headline configuration")
     ap.add_argument("--c3-steps", type=int, default=100)
     ap.add_argument("--c4-problems", type=int, default=256)
+    ap.add_argument("--c4-inflight", type=int, default=16, help="server slots for configuration 4")
     ap.add_argument("--c5-rates", type=str, default="10,50,400")
     ap.add_argument("--c5-seconds", type=float, default=4.0)
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both",
@@ -329,7 +330,7 @@ def main():
             if args.only == "tcd":
                 out[name] = tcd_run(pm, prec, args.c3_steps)[0]
             elif args.only == "batched":
-                out[name] = batched_uneven(pm, prec, args.c4_problems)
+                out[name] = batched_uneven(pm, prec, args.c4_problems, inflight=args.c4_inflight)
             else:
                 out[name] = serving_stream(pm, prec, [float(x) for x in args.c5_rates.split(",") if x],
                                            args.c5_seconds)
@@ -384,7 +385,8 @@ def main():
             np.linalg.norm(u["mixed"] - u["fp64"]) / np.linalg.norm(u["fp64"]))
         for name, prec in (("fp64", pm.FP64), ("mixed", pm.MIXED)):
             log(f"[bench] configuration 4 ({args.c4_problems} batched uneven Q1 problems), {name}")
-            extra.setdefault("batched_uneven_q1", {})[name] = batched_uneven(pm, prec, args.c4_problems)
+            extra.setdefault("batched_uneven_q1", {})[name] = batched_uneven(pm, prec, args.c4_problems,
+                                                                             inflight=args.c4_inflight)
         rates = [float(x) for x in args.c5_rates.split(",") if x]
         for name, prec in (("fp64", pm.FP64), ("mixed", pm.MIXED)):
             log(f"[bench] configuration 5 (Poisson serving stream), {name}")
