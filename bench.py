@@ -49,7 +49,8 @@ KERNEL_NAMES = {0: "csr_stream<kSpmv,f64>",
                 2: "lat64_pair_gk<kL64Spmv,2> f64 (2 vertex/midpoint pairs per thread)",
                 3: "lat_restrict_pair_k f32 (coarse vertex/midpoint pairs)",
                 4: "lat_prolong_k<f32>", 5: "csr_fill_k",
-                6: "copy_k (f64 vector copy, traffic calibration)"}
+                6: "copy_k (f64 vector copy, traffic calibration)",
+                7: "axpy_dot2_k f64 (fused Gram-Schmidt step, 16-byte pairs)"}
 
 
 def kernel_table(P, reps=50):
@@ -70,7 +71,7 @@ def kernel_table(P, reps=50):
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "cd3d_q2_64_fine_level_traffic.json")
 TRAFFIC_KERNEL = {0: "void csr_stream<0, double, double, true>", 1: "void lat_pair_gk<0, 4>",
                   2: "void lat64_pair_gk<0, 2>", 3: "lat_restrict_pair_k", 4: "void lat_prolong_k<2>",
-                  5: "csr_fill2_k", 6: "copy_k"}
+                  5: "csr_fill2_k", 6: "copy_k", 7: "void axpy_dot2_k<true>"}
 
 
 def measured_traffic(which):
@@ -365,8 +366,11 @@ def main():
     err = float(np.linalg.norm(sols["mixed"] - sols["fp64"]) / np.linalg.norm(sols["fp64"]))
     log("[bench] kernel table (HIP events on the library stream)")
     table = kernel_table(P)
-    roof = roofline_entry(table, 1, "dominant kernel of the mixed step: the fine-level FP32 Jacobi sweep "
-                                    "(algorithmic bytes: x, f and the result once, 12 bytes per node)")
+    roof = roofline_entry(table, 1, "dominant kernel of the mixed step (with the Gram-Schmidt step, mgs_step): "
+                                    "the fine-level FP32 Jacobi sweep (algorithmic bytes: x, f and the result "
+                                    "once, 12 bytes per node)")
+    mgs = roofline_entry(table, 7, "fused Gram-Schmidt step of the outer FGMRES on the fine level (32 bytes per "
+                         "unknown); it and the sweep take the largest shares of the mixed step's kernel time")
     spmv = roofline_entry(table, 0, "FP64 CSR SpMV of the fine level (12 nnz + 20 N bytes); the FP64 form's "
                                     "smoother, defect and FGMRES product")
     cpu = None
@@ -411,6 +415,7 @@ def main():
         "spmv_gbs": spmv["achieved"],
         "roofline": roof,
         "spmv_fp64_csr": spmv,
+        "mgs_step": mgs,
         "kernels_fine_level": table,
         "cpu_baseline": cpu,
     }

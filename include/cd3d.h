@@ -183,7 +183,9 @@ cd3d_status cd3d_interp_pulse(cd3d_problem *p, int level, const cd3d_coeffs *co,
  * 3 FP32 lattice restriction to level-1 (4 N_fine + 4 N_coarse);
  * 4 FP32 lattice prolongation + update from level-1 (8 N_fine + 4 N_coarse);
  * 5 CSR values from the class stencils (10 bytes per active-row nonzero); 6 copy of an
- * FP64 level vector (16 N: the traffic-counter calibration).
+ * FP64 level vector (16 N: the traffic-counter calibration); 7 one fused
+ * Gram-Schmidt step of FGMRES, w -= h v_k and the partial sums of <w, v_k+1>
+ * (32 N: v_k, w, v_k+1 read, w written).
  * Needs an assembled problem; the level vectors are overwritten. */
 cd3d_status cd3d_kernel_bench(cd3d_problem *p, int which, int level, int reps, double *ms_per_launch,
                               double *bytes_per_launch);

@@ -1363,7 +1363,7 @@ cd3d_status cd3d_device_ptr(cd3d_problem *p, int level, int which, double **dev)
 }
 
 cd3d_status cd3d_kernel_bench(cd3d_problem *p, int which, int level, int reps, double *ms, double *bytes) {
-  if (!p || !ms || !bytes || reps < 1 || level < 0 || level >= (int)p->dl.size() || which < 0 || which > 6)
+  if (!p || !ms || !bytes || reps < 1 || level < 0 || level >= (int)p->dl.size() || which < 0 || which > 7)
     return CD3D_ERR_ARG;
   std::lock_guard<std::mutex> g(p->mu);
   return run(p, [&]() {
@@ -1414,6 +1414,14 @@ cd3d_status cd3d_kernel_bench(cd3d_problem *p, int which, int level, int reps, d
       case 6:
         launch = [&] { k_copy(p->stream, L.ndof, L.sol, L.aux); };
         b = 16 * N;
+        break;
+      case 7:
+        // zero alpha partials: w stays finite however many launches run
+        HIPCHK(hipMemsetAsync(p->red, 0, sizeof(double) * 2 * kRedBlocks, p->stream));
+        launch = [&] {
+          k_axpy_dot(p->stream, L.ndof, p->red, -1.0, L.sol, L.aux, L.rhs, p->red + kRedBlocks, p->dscal);
+        };
+        b = 32 * N;
         break;
     }
     hipEvent_t e0, e1;
