@@ -454,6 +454,18 @@ int check_pairs(int m, int k0) {
 
 int main() {
   int bad = 0;
+  {  // the XCD-contiguous block order of the CSR kernels is a permutation
+    int nb = 0;
+    for (int n = 1; n <= 5000; n++) {
+      std::vector<char> seen(n, 0);
+      for (int b = 0; b < n; b++) {
+        const int l = xcd_block(b, n);
+        if (l < 0 || l >= n || seen[l]++) nb++;
+      }
+    }
+    printf("   xcd_block: %d collisions\n", nb);
+    if (nb) bad++;
+  }
   bad += check_pairs(1, 0);
   bad += check_pairs(1, 2);
   bad += check_pairs(3, 1);
