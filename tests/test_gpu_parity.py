@@ -380,3 +380,24 @@ def test_rotating_field_assembly_matches_oracle(order, m, k0):
         rerr = np.abs(P.rhs(l) - H.rhs(l)).max() / np.abs(H.rhs(l)).max()
         say(f"[gpu] rotating field order={order} m={m} level {l}: entry err {err:.2e}, rhs err {rerr:.2e}")
         assert err < ENTRY_TOL and rerr < ENTRY_TOL
+
+
+def test_kernel_bench_covers_every_kernel():
+    # every case of cd3d_kernel_bench runs on an assembled Q2 problem (the
+    # timings bench.py reports), with the stated algorithmic bytes per launch,
+    # and an out-of-range case is refused
+    P = pm.Problem(2, 1, 1, 3)
+    P.assemble(pm.coeffs(eps=1e-3, b=(1.0, 0.5, 0.25), supg=1, f_val=1.0))
+    l = 2
+    i = P.info(l)
+    N, nnz = i["ndof"], i["nnz"]
+    nbytes = {}
+    for which in range(8):
+        ms, b = P.kernel_bench(which, l, 3)
+        assert ms > 0 and b > 0
+        nbytes[which] = b
+    say(f"[gpu] kernel bench bytes {nbytes}")
+    assert nbytes[0] == 12 * nnz + 20 * N
+    assert nbytes[2] == 16 * N and nbytes[6] == 16 * N and nbytes[7] == 32 * N
+    with pytest.raises(pm.CD3DError):
+        P.kernel_bench(8, l, 1)
