@@ -64,6 +64,15 @@ __host__ __device__ inline double affine_value(const double *s0, const double *s
   return alpha_m != 0.0 ? fma(alpha_m, sm[s], alpha * e) : alpha * e;
 }
 
+// The same entry for the FP32 preconditioner: the stencils rounded to FP32
+// and the expression evaluated with FP32 fma (the FP64 operator keeps
+// affine_value and matches the CSR values)
+__host__ __device__ inline float affine_value32(const float *s0, const float *sx, const float *sy, const float *sm,
+                                                float alpha, float alpha_m, int s, int X, int Y) {
+  const float e = fmaf((float)Y, sy[s], fmaf((float)X, sx[s], s0[s]));
+  return alpha_m != 0.f ? fmaf(alpha_m, sm[s], alpha * e) : alpha * e;
+}
+
 // UNI: the class's stencil (and diagonal) are the same at every node and are
 // read from cst (uniform across the block); otherwise per-node values.
 template <int P, int PX, int PY, int PZ, int MODE, bool UNI>
@@ -126,9 +135,9 @@ __host__ __device__ inline void lat_class_node(const LatArgs &a, int c, long lon
 }
 
 // Sweep with affine stencils evaluated at each node (rotating convection):
-// the class's S0, SX, SY and mass stencils are staged in LDS and each value
-// is affine_value rounded to FP32, exactly what lat_fill_affine_k stores, so
-// the results are bitwise those of lat_node on the filled per-node values,
+// the class's S0, SX, SY and mass stencils are staged in LDS as FP32 and each
+// value is affine_value32, exactly what lat_fill_affine_k stores, so the
+// results are bitwise those of lat_node on the filled per-node values,
 // without their 4 bytes per nonzero of HBM traffic.
 struct LatAff {
   const double *st;   // lst64: S0 at 0, SX at 8 kCst + 8, SY at 2 (8 kCst) + 8
@@ -138,14 +147,14 @@ struct LatAff {
 };
 
 template <int P, int PX, int PY, int PZ, int MODE>
-__host__ __device__ inline void lat_node_aff(const LatArgs &a, const LatDev &C, long long k, const double *s0,
-                                             const double *sx, const double *sy, const double *sm, double alpha,
-                                             double alpha_m, int centre, const float *__restrict__ x,
+__host__ __device__ inline void lat_node_aff(const LatArgs &a, const LatDev &C, long long k, const float *s0,
+                                             const float *sx, const float *sy, const float *sm, float alpha,
+                                             float alpha_m, int centre, const float *__restrict__ x,
                                              const float *__restrict__ f, float *__restrict__ out, float omega) {
   const int kx = (int)(k % C.cnt[0]), ky = (int)((k / C.cnt[0]) % C.cnt[1]), kz = (int)(k / ((long long)C.cnt[0] * C.cnt[1]));
   const int X = C.lo[0] + C.step * kx, Y = C.lo[1] + C.step * ky, Z = C.lo[2] + C.step * kz;
   const long long nn = a.nn, node = X + nn * (Y + nn * Z);
-  const float di = MODE == kLDefect ? 0.f : (float)(1.0 / affine_value(s0, sx, sy, sm, alpha, alpha_m, centre, X, Y));
+  const float di = MODE == kLDefect ? 0.f : 1.0f / affine_value32(s0, sx, sy, sm, alpha, alpha_m, centre, X, Y);
   if (MODE == kLJacobiZero) {
     out[node] = omega * f[node] * di;
     return;
@@ -158,15 +167,15 @@ __host__ __device__ inline void lat_node_aff(const LatArgs &a, const LatDev &C, 
     for (int dy = lo_off<P, PY>(); dy <= hi_off<P, PY>(); dy++)
 #pragma unroll
       for (int dx = lo_off<P, PX>(); dx <= hi_off<P, PX>(); dx++, si++)
-        s += (float)affine_value(s0, sx, sy, sm, alpha, alpha_m, si, X, Y) * x[node + dx + nn * (dy + nn * dz)];
+        s += affine_value32(s0, sx, sy, sm, alpha, alpha_m, si, X, Y) * x[node + dx + nn * (dy + nn * dz)];
   if (MODE == kLDefect) out[node] = f[node] - s;
   else out[node] = x[node] + omega * (f[node] - s) * di;
 }
 
 template <int P, int MODE>
-__host__ __device__ inline void lat_class_node_aff(const LatArgs &a, int c, long long k, const double *s0,
-                                                   const double *sx, const double *sy, const double *sm,
-                                                   double alpha, double alpha_m, int centre, const float *x,
+__host__ __device__ inline void lat_class_node_aff(const LatArgs &a, int c, long long k, const float *s0,
+                                                   const float *sx, const float *sy, const float *sm,
+                                                   float alpha, float alpha_m, int centre, const float *x,
                                                    const float *f, float *out, float omega) {
 #define CD3D_AFF(PX, PY, PZ) \
   lat_node_aff<P, PX, PY, PZ, MODE>(a, a.c[c], k, s0, sx, sy, sm, alpha, alpha_m, centre, x, f, out, omega)
@@ -191,20 +200,20 @@ template <int P, int MODE>
 __global__ __launch_bounds__(kB) void lat_aff_sweep_k(LatArgs a, LatAff af, const float *__restrict__ x,
                                                       const float *__restrict__ f, float *__restrict__ out,
                                                       float omega) {
-  __shared__ double st[4][kCst];  // S0, SX, SY, mass of the block's class
+  __shared__ float st[4][kCst];  // S0, SX, SY, mass of the block's class, rounded to FP32
   int c = 0;
   while (c + 1 < a.nclass && (int)blockIdx.x >= a.bstart[c + 1]) c++;
   for (int i = threadIdx.x; i < kCst; i += kB) {
-    st[0][i] = af.st[c * kCst + i];
-    st[1][i] = af.st[8 * kCst + 8 + c * kCst + i];
-    st[2][i] = af.st[2 * 8 * kCst + 8 + c * kCst + i];
-    st[3][i] = af.stm ? af.stm[c * kCst + i] : 0.0;
+    st[0][i] = (float)af.st[c * kCst + i];
+    st[1][i] = (float)af.st[8 * kCst + 8 + c * kCst + i];
+    st[2][i] = (float)af.st[2 * 8 * kCst + 8 + c * kCst + i];
+    st[3][i] = af.stm ? (float)af.stm[c * kCst + i] : 0.f;
   }
   __syncthreads();
   const long long k = (long long)(blockIdx.x - a.bstart[c]) * kB + threadIdx.x;
   if (k < a.c[c].nodes)
-    lat_class_node_aff<P, MODE>(a, c, k, st[0], st[1], st[2], st[3], af.alpha, af.alpha_m, af.centre[c], x, f, out,
-                                omega);
+    lat_class_node_aff<P, MODE>(a, c, k, st[0], st[1], st[2], st[3], (float)af.alpha, (float)af.alpha_m,
+                                af.centre[c], x, f, out, omega);
 }
 
 // Q2 sweeps by vertex/midpoint pairs along X.  A thread takes the nodes
@@ -1271,7 +1280,7 @@ __global__ __launch_bounds__(kB) void csr_fill_affine_k(const int *__restrict__ 
 
 // FP32 per-node lattice values and 1/a_ii straight from the affine stencils:
 // alpha_m M + alpha (S0 + X SX + Y SY), M the mass class stencils in stm
-// (alpha_m = 0: the conv-diff matrix alone)
+// (alpha_m = 0: the conv-diff matrix alone), evaluated as affine_value32
 __global__ void lat_fill_affine_k(LatArgs a, long long n, const double *__restrict__ st,
                                   const double *__restrict__ stm, double alpha_m, double alpha,
                                   const int *__restrict__ centre, float *__restrict__ lval,
@@ -1282,11 +1291,12 @@ __global__ void lat_fill_affine_k(LatArgs a, long long n, const double *__restri
     const LatDev &C = a.c[c];
     const long long r = i - C.off, s = r / C.nodes, k = r % C.nodes;
     const int X = C.lo[0] + C.step * (int)(k % C.cnt[0]), Y = C.lo[1] + C.step * (int)((k / C.cnt[0]) % C.cnt[1]);
-    const int id = c * kCst;
-    const double v = affine_value(st + id, st + 8 * kCst + 8 + id, st + 2 * 8 * kCst + 8 + id, stm + id, alpha,
-                                  alpha_m, (int)s, X, Y);
-    lval[i] = (float)v;
-    if (s == centre[c]) linv[C.first + k] = (float)(1.0 / v);
+    const int id = c * kCst + (int)s;
+    const float s0 = (float)st[id], sx = (float)st[8 * kCst + 8 + id], sy = (float)st[2 * 8 * kCst + 8 + id],
+                sm = alpha_m != 0.0 ? (float)stm[id] : 0.f;
+    const float v = affine_value32(&s0, &sx, &sy, &sm, (float)alpha, (float)alpha_m, 0, X, Y);
+    lval[i] = v;
+    if (s == centre[c]) linv[C.first + k] = 1.0f / v;
   }
 }
 

@@ -346,11 +346,13 @@ int check_pairs(int m, int k0) {
     }
   }
   // affine stencils evaluated node by node against per-node values filled
-  // with the same formula (affine_value), every class and mode
+  // with the same formula (affine_value32 on the FP32-rounded stencils, as
+  // lat_fill_affine_k), every class and mode
   {
     std::vector<double> st(kLst64), stm(8 * kCst);
     for (auto &v : st) v = rnd(g);
     for (auto &v : stm) v = rnd(g);
+    std::vector<float> st32(st.begin(), st.end()), stm32(stm.begin(), stm.end());
     LatArgs a = lat_args(D);
     for (int c = 0; c < 8; c++) a.uni[c] = 0;
     int centre[8];
@@ -363,25 +365,25 @@ int check_pairs(int m, int k0) {
       }
       centre[c] = -lo[0] + len[0] * (-lo[1] + len[1] * -lo[2]);
     }
-    const double alpha = 0.37, alpha_m = 1.0;
+    const float alpha = 0.37f, alpha_m = 1.0f;
     std::vector<float> av(F.lat_nval), ai(F.nact);
     for (int c = 0; c < 8; c++) {
       const LatDev &C = a.c[c];
-      const double *s0 = st.data() + c * kCst, *sx = st.data() + 8 * kCst + 8 + c * kCst,
-                   *sy = st.data() + 2 * 8 * kCst + 8 + c * kCst, *sm = stm.data() + c * kCst;
+      const float *s0 = st32.data() + c * kCst, *sx = st32.data() + 8 * kCst + 8 + c * kCst,
+                  *sy = st32.data() + 2 * 8 * kCst + 8 + c * kCst, *sm = stm32.data() + c * kCst;
       for (int s2 = 0; s2 < C.nsten; s2++)
         for (int64_t k = 0; k < C.nodes; k++) {
           const int X = C.lo[0] + C.step * (int)(k % C.cnt[0]), Y = C.lo[1] + C.step * (int)((k / C.cnt[0]) % C.cnt[1]);
-          const double v = affine_value(s0, sx, sy, sm, alpha, alpha_m, s2, X, Y);
-          av[C.off + (int64_t)s2 * C.nodes + k] = (float)v;
-          if (s2 == centre[c]) ai[C.first + k] = (float)(1.0 / v);
+          const float v = affine_value32(s0, sx, sy, sm, alpha, alpha_m, s2, X, Y);
+          av[C.off + (int64_t)s2 * C.nodes + k] = v;
+          if (s2 == centre[c]) ai[C.first + k] = 1.0f / v;
         }
     }
     for (int mode = 0; mode < 3; mode++) {
       std::vector<float> o1(nn3, 7.f), o2(nn3, 7.f);
       for (int c = 0; c < 8; c++) {
-        const double *s0 = st.data() + c * kCst, *sx = st.data() + 8 * kCst + 8 + c * kCst,
-                     *sy = st.data() + 2 * 8 * kCst + 8 + c * kCst, *sm = stm.data() + c * kCst;
+        const float *s0 = st32.data() + c * kCst, *sx = st32.data() + 8 * kCst + 8 + c * kCst,
+                    *sy = st32.data() + 2 * 8 * kCst + 8 + c * kCst, *sm = stm32.data() + c * kCst;
         for (int64_t k = 0; k < a.c[c].nodes; k++) {
 #define CD3D_BOTH(M)                                                                                       \
   lat_class_node<2, M>(a, c, k, av.data(), ai.data(), nullptr, x.data(), f.data(), o1.data(), 0.7f);       \
