@@ -22,6 +22,11 @@ using namespace cd3d;
 
 struct cd3d_problem : public Problem {};
 
+// for the serving loop's handles (not part of the C ABI): no side stream
+void cd3d_internal_serving_handle(cd3d_problem *p) {
+  if (p) p->side_ok = false;
+}
+
 namespace {
 
 struct HipError : std::runtime_error {
@@ -345,8 +350,18 @@ static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int wha
 // marks the point of the main stream where the level-0 matrix is final;
 // launch_coarse_inverse enqueues the inversion after that point (called once
 // the main stream has other work queued, so that it runs beside that work).
+// The coarse inverse goes to the side stream unless the handle belongs to
+// the serving loop or CD3D_SIDE_STREAM=0.  Measured (one default bench run
+// each): one handle, configuration 2 13.29 vs 13.47 ms and configuration 3
+// 3.32 vs 3.50 ms per step with the side stream; the serving stream at 400
+// requests/s 54 vs 79 mixed solves/s.
+static bool side_stream_on(const cd3d_problem &P) {
+  const char *e = getenv("CD3D_SIDE_STREAM");
+  return P.side_ok && !(e && e[0] == '0');
+}
+
 static void fork_coarse(cd3d_problem &P) {
-  if (!k_invert_multi_launch(P.dl[0].nact)) return;  // inverted on the main stream
+  if (!k_invert_multi_launch(P.dl[0].nact) || !side_stream_on(P)) return;  // inverted on the main stream
   if (!P.side) {
     // created on first need only: HIP hands streams to its hardware queues
     // round-robin, and a side stream per handle would leave every other
@@ -364,7 +379,7 @@ static void launch_coarse_inverse(cd3d_problem &P) {
   const int na = C.nact;
   if (!P.fork_recorded) fork_coarse(P);
   P.fork_recorded = false;  // (fork_coarse records nothing when the inverse stays on the main stream)
-  if (na > 0 && !k_invert_multi_launch(na)) {
+  if (na > 0 && (!k_invert_multi_launch(na) || !side_stream_on(P))) {
     // one short launch: the fork and join would cost more than they hide
     if (!P.gj_scratch) P.gj_scratch = dalloc<double>(P, (size_t)gj_scratch_doubles(na));
     k_invert_active(P.stream, C, P.gj_scratch, P.cinv);

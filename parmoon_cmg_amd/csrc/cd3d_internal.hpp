@@ -148,6 +148,10 @@ struct Problem {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool coarse_pending = false, fork_recorded = false;
+  // the coarse inverse may run on `side` (off for handles of the serving
+  // loop: with many handles in flight, side streams share the hardware
+  // queues with other handles' main streams and hold them up)
+  bool side_ok = true;
   int device = 0;
   size_t dev_bytes = 0;
   std::vector<void *> allocs;

@@ -24,6 +24,9 @@
 
 #include "../../include/cd3d_server.h"
 
+// capi.cpp: the handle's coarse inverse stays on its main stream
+void cd3d_internal_serving_handle(cd3d_problem *p);
+
 namespace {
 
 double now_ms() {
@@ -179,7 +182,10 @@ struct cd3d_server {
     }
     cd3d_problem *&h = w.slot->h;
     cd3d_status st = CD3D_OK;
-    if (!h) st = cd3d_create(&r.mesh, &h);
+    if (!h) {
+      st = cd3d_create(&r.mesh, &h);
+      if (st == CD3D_OK) cd3d_internal_serving_handle(h);
+    }
     if (st != CD3D_OK) {
       h = nullptr;
       res.status = st;
