@@ -172,11 +172,55 @@ def mesh_for_n(n):
     return best[1]
 
 
-def random_coeffs(pm, rng, supg=1, conv_kind=0):
+def random_coeffs(rng, supg=1, conv_kind=0):
+    """eps log-uniform in [1e-6, 1], b a random unit direction times U(0, 1),
+    f = 1 (keyword arguments of coeffs())."""
     eps = 10 ** rng.uniform(-6, 0)
     d = rng.standard_normal(3)
     b = d / np.linalg.norm(d) * rng.uniform(0, 1)
-    return pm.coeffs(eps=eps, b=tuple(b), f_val=1.0, supg=supg, conv_kind=conv_kind), eps
+    return dict(eps=eps, b=tuple(float(x) for x in b), f_val=1.0, supg=supg, conv_kind=conv_kind)
+
+
+def c4_requests(count=256, seed=20261202, n_range=(8, 48)):
+    """Configuration 4: `count` Q1 SUPG problems, n log-uniform in n_range
+    (snapped to m 2^k, coarsest level m^3 cells), random eps and b.  Each
+    request: dict(order, m, k0, levels, n, co)."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(count):
+        n, m, levels = mesh_for_n(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
+        reqs.append(dict(order=1, m=m, k0=0, levels=levels, n=n, co=random_coeffs(rng)))
+    return reqs
+
+
+def c5_arrivals(rate, seed, span, n_range=(16, 96), tcd_share=0.2):
+    """Configuration 5: Poisson arrivals at `rate` per second over `span`
+    seconds; n log-uniform in n_range (snapped), Q1 or Q2 with equal
+    probability, a share of TCD3D requests (Galerkin, rotating field), the
+    others CD3D SUPG with random eps and b.  Each arrival: dict(t, order, m,
+    k0, levels, n, co, tcd)."""
+    rng = np.random.default_rng(seed)
+    arrivals, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > span:
+            break
+        n, m, levels = mesh_for_n(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
+        order = 1 if rng.uniform() < 0.5 else 2
+        tcd = bool(rng.uniform() < tcd_share)
+        co = random_coeffs(rng, supg=0 if tcd else 1, conv_kind=1 if tcd else 0)
+        arrivals.append(dict(t=t, order=order, m=m, k0=0, levels=levels, n=n, co=co, tcd=tcd))
+    return arrivals
+
+
+# TCD3D steps stop at a relative residual of 1e-10 against the step's own
+# initial residual (the reference's restart-relative SC_LIN_RED_FACTOR_SCALAR
+# would demand 1e-10 again after every restart)
+TCD_STEPS = 10
+
+
+def tcd_solver(pm, precision, maxit=1000):
+    return pm.solver_params(**dict(SOLVER, maxit=maxit), precision=precision)
 
 
 def tcd_run(pm, precision, steps, n=128, levels=5):
@@ -184,7 +228,7 @@ def tcd_run(pm, precision, steps, n=128, levels=5):
     dt 1e-3, convection matrix re-assembled every step on every level."""
     P = pm.Problem(1, 1, int(np.log2(n)) - levels + 1, levels, box=(1.0, 1.0, 1.0))
     co = pm.coeffs(eps=1e-4, conv_kind=1, rot_c=(0.5, 0.5), pulse_c=(0.3, 0.5, 0.5), f_val=0.0)
-    sp = pm.solver_params(**dict(SOLVER, rel_tol=0.0, res_norm_min=1e-30, red_factor=1e-10), precision=precision)
+    sp = tcd_solver(pm, precision)
     u0 = P.interp_pulse(levels - 1, co)
     P.tcd_init(co, 0.0, u0)
     P.tcd_step(co, sp, 1e-3, want_state=False)  # warm-up step (allocations)
@@ -201,30 +245,37 @@ def tcd_run(pm, precision, steps, n=128, levels=5):
     return dict(steps=steps, steps_per_s=steps / dt, ms_per_step=1e3 * dt / steps, iterations=its), u
 
 
+def failure_record(req, r):
+    """What names a request that did not converge."""
+    co = req["co"]
+    return dict(kind="tcd" if req.get("tcd") else "cd", order=req["order"], m=req["m"], levels=req["levels"],
+                n=req["n"], eps=co["eps"], b=list(co["b"]), status=int(r.status), iterations=int(r.iterations),
+                rel_residual=float(r.rel_residual), steps_done=int(r.steps_done))
+
+
 def batched_uneven(pm, precision, count=256, seed=20261202, passes=2, inflight=16):
     """Configuration 4: `count` independent Q1 SUPG problems, n log-uniform in
     [8, 48] (snapped to m 2^k), eps log-uniform in [1e-6, 1], b a random unit
     direction times U(0, 1), f = 1; all submitted at once to the server.  The
-    first pass builds the resident hierarchies, the last one is reported."""
-    rng = np.random.default_rng(seed)
-    reqs = []
-    for _ in range(count):
-        n, m, levels = mesh_for_n(np.exp(rng.uniform(np.log(8), np.log(48))))
-        co, _ = random_coeffs(pm, rng)
-        reqs.append((pm.mesh_desc(1, m, 0, levels), co))
+    first pass builds the resident hierarchies, the last one is reported;
+    solves/s counts converged requests only."""
+    reqs = c4_requests(count, seed)
     sp = pm.solver_params(**dict(SOLVER, maxit=1000), precision=precision)
     srv = pm.Server(max_inflight=inflight)
     out = None
     for _ in range(passes):
         t = time.perf_counter()
-        ids = [srv.submit(mesh, co, sp) for mesh, co in reqs]
+        ids = [srv.submit(pm.mesh_desc(q["order"], q["m"], q["k0"], q["levels"]), pm.coeffs(**q["co"]), sp)
+               for q in reqs]
         res = [srv.wait(i)[0] for i in ids]
         wall = time.perf_counter() - t
-        lat = np.array([r.latency_ms for r in res])
-        out = dict(problems=count, solves_per_s=count / wall, wall_s=wall, p50_ms=float(np.percentile(lat, 50)),
-                   p99_ms=float(np.percentile(lat, 99)), not_converged=sum(r.status != 0 for r in res),
-                   max_rel_residual=float(max(r.rel_residual for r in res)),
-                   median_iterations=int(np.median([r.iterations for r in res])),
+        ok = [r for r in res if r.status == 0]
+        lat = np.array([r.latency_ms for r in ok])
+        out = dict(problems=count, solves_per_s=len(ok) / wall, wall_s=wall, p50_ms=float(np.percentile(lat, 50)),
+                   p99_ms=float(np.percentile(lat, 99)), failed=count - len(ok),
+                   failures=[failure_record(q, r) for q, r in zip(reqs, res) if r.status != 0],
+                   max_rel_residual=float(max(r.rel_residual for r in ok)),
+                   median_iterations=int(np.median([r.iterations for r in ok])),
                    reused=sum(r.reused_hierarchy for r in res))
     st = srv.stats()
     out["peak_device_gib"] = st.peak_bytes / 2**30
@@ -239,36 +290,28 @@ def serving_stream(pm, precision, rates, duration_s=4.0, seed=7, tcd_share=0.2, 
     rotating field.  Per rate: offered load, completed solves/s, p50/p99.
     Arrivals span duration_s, or max_requests / rate seconds when shorter (a
     rate above the server's capacity then measures the capacity: solves/s
-    from the first arrival to the last completion)."""
+    from the first arrival to the last completion).  Requests that end
+    unconverged are listed under `failures` and not counted as solves."""
     import threading
     srv = pm.Server(max_inflight=16)
     out = []
     for ri, rate in enumerate(rates):
-        rng = np.random.default_rng(seed + ri)
-        arrivals, t = [], 0.0
         span = min(duration_s, max_requests / rate)
-        while True:
-            t += rng.exponential(1.0 / rate)
-            if t > span:
-                break
-            n, m, levels = mesh_for_n(np.exp(rng.uniform(np.log(16), np.log(96))))
-            order = 1 if rng.uniform() < 0.5 else 2
-            tcd = rng.uniform() < tcd_share
-            co, _ = random_coeffs(pm, rng, supg=0 if tcd else 1, conv_kind=1 if tcd else 0)
-            arrivals.append((t, pm.mesh_desc(order, m, 0, levels), co, tcd))
+        arrivals = c5_arrivals(rate, seed + ri, span, tcd_share=tcd_share)
         sp_cd = pm.solver_params(**dict(SOLVER, maxit=1000), precision=precision)
-        sp_tcd = pm.solver_params(**dict(SOLVER, rel_tol=0.0, res_norm_min=1e-30, red_factor=1e-10, maxit=1000),
-                                  precision=precision)
+        sp_tcd = tcd_solver(pm, precision)
 
         def submit_all(stream, ids, t0):
-            for (ta, mesh, co, tcd) in stream:
-                d = t0 + ta - time.perf_counter()
+            for a in stream:
+                d = t0 + a["t"] - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
-                if tcd:
-                    ids.append(srv.submit(mesh, co, sp_tcd, kind=pm.REQ_TCD, tau=1e-3, nsteps=10))
+                mesh = pm.mesh_desc(a["order"], a["m"], a["k0"], a["levels"])
+                if a["tcd"]:
+                    ids.append(srv.submit(mesh, pm.coeffs(**a["co"]), sp_tcd, kind=pm.REQ_TCD, tau=1e-3,
+                                          nsteps=TCD_STEPS))
                 else:
-                    ids.append(srv.submit(mesh, co, sp_cd))
+                    ids.append(srv.submit(mesh, pm.coeffs(**a["co"]), sp_cd))
 
         for phase in ("warm", "timed"):
             ids = []
@@ -278,12 +321,14 @@ def serving_stream(pm, precision, rates, duration_s=4.0, seed=7, tcd_share=0.2, 
             th.join()
             res = [srv.wait(i)[0] for i in ids]
             wall = time.perf_counter() - t0
-        lat = np.array([r.latency_ms for r in res])
+        ok = [r for r in res if r.status == 0]
+        lat = np.array([r.latency_ms for r in ok])
         out.append(dict(rate_per_s=rate, requests=len(res), duration_s=span, wall_s=wall,
-                        solves_per_s=len(res) / wall, p50_ms=float(np.percentile(lat, 50)),
-                        p99_ms=float(np.percentile(lat, 99)), failed=sum(r.status != 0 for r in res),
-                        tcd_requests=sum(a[3] for a in arrivals)))
-        log(f"[bench]   rate {rate}/s: {len(res)} requests, p50 {out[-1]['p50_ms']:.1f} ms, "
+                        solves_per_s=len(ok) / wall, p50_ms=float(np.percentile(lat, 50)),
+                        p99_ms=float(np.percentile(lat, 99)), failed=len(res) - len(ok),
+                        failures=[failure_record(a, r) for a, r in zip(arrivals, res) if r.status != 0],
+                        tcd_requests=sum(a["tcd"] for a in arrivals)))
+        log(f"[bench]   rate {rate}/s: {len(res)} requests ({out[-1]['failed']} failed), p50 {out[-1]['p50_ms']:.1f} ms, "
             f"p99 {out[-1]['p99_ms']:.1f} ms, {out[-1]['solves_per_s']:.1f} solves/s")
     srv.close()
     return out

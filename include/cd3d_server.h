@@ -50,20 +50,23 @@ typedef struct {
   cd3d_solver_params solver;
   double t0, tau, theta;       /* TCD3D: start time, step, theta (0.5 Crank-Nicolson) */
   int nsteps;                  /* TCD3D: time steps */
-  int want_solution;           /* keep the finest-level solution (CD) / final state (TCD) for cd3d_server_wait */
+  int want_solution;           /* keep the finest-level solution (CD) / final state (TCD) for cd3d_server_wait,
+                                  also when the solve ends CD3D_NOT_CONVERGED */
 } cd3d_request;
 
 typedef struct {
   uint64_t id;
   int status;                  /* cd3d_status of the request */
   int iterations;              /* FGMRES iterations (TCD: summed over the steps) */
-  double rel_residual;         /* CD: |b - A x| / |b - A x0| */
+  double rel_residual;         /* |b - A x| / |b - A x0| (TCD: the largest over the steps run) */
   double queue_ms;             /* submit -> admitted */
   double run_ms;               /* admitted -> done */
   double latency_ms;           /* submit -> done */
   size_t device_bytes;         /* bytes of the hierarchy it ran on */
   int reused_hierarchy;        /* 1: an idle resident hierarchy was reused */
   int n_dof;                   /* finest-level DOFs (length of the solution) */
+  int steps_done;              /* TCD: time steps completed; a step that does not converge ends the run
+                                  (status CD3D_NOT_CONVERGED) and the state after it is returned */
 } cd3d_result;
 
 typedef struct {

@@ -754,7 +754,9 @@ int orc_fgmres(orc_hier *h, const orc_solver *sv, const double *rhs, double *sol
   res = defect_(L, sol, rhs, v[0]);
   res = sqrt(ddot(N, v[0], v[0]));
   res0 = beta = res;
-  if (beta <= sv->res_norm_min) { ret = 0; goto done; }
+  double res_norm_min = sv->res_norm_min, red_factor = sv->red_factor;
+  if (sv->rel_tol > 0) { res_norm_min = sv->rel_tol * beta; red_factor = 0.0; }
+  if (beta <= res_norm_min) { ret = 0; goto done; }
   j = 1;
   while (j <= sv->maxit) {
     for (int k = 0; k < N; k++) v[0][k] *= 1.0 / beta;
@@ -778,7 +780,7 @@ int orc_fgmres(orc_hier *h, const orc_solver *sv, const double *rhs, double *sol
       res = fabs(s[i + 1]);
       if (hist) hist[nh] = res;
       nh++;
-      if (res < sv->res_norm_min || res < sv->red_factor * res0) {
+      if (res < res_norm_min || res < red_factor * res0) {
         memset(w, 0, sizeof(double) * N);
         update_iter(w, N, i, H, R, s, zv);
         for (int t = 0; t < N; t++) sol[t] += w[t];

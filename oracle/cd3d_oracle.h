@@ -43,6 +43,8 @@ typedef struct {
   double red_factor;     /* SC_LIN_RED_FACTOR_SCALAR (relative to the restart cycle start) */
   double div_factor;     /* SC_DIV_FACTOR */
   int cycle;             /* SC_MG_CYCLE_SCALAR: 1 V, 2 W, 0 F */
+  double rel_tol;        /* > 0: stop at |b - A x| <= rel_tol |b - A x0| of this solve (the reference's
+                            SC_LIN_RES_NORM_MIN_SCALAR set to rel_tol |r0| before the solve, red_factor 0) */
 } orc_solver;
 
 typedef struct orc_hier orc_hier;

@@ -34,7 +34,7 @@ class Solver(C.Structure):
                 ("coarse_smoother", C.c_int), ("coarse_maxit", C.c_int), ("coarse_red", C.c_double),
                 ("gmg_damp", C.c_double), ("restart", C.c_int), ("maxit", C.c_int),
                 ("res_norm_min", C.c_double), ("red_factor", C.c_double), ("div_factor", C.c_double),
-                ("cycle", C.c_int)]
+                ("cycle", C.c_int), ("rel_tol", C.c_double)]
 
 
 def coeffs(eps=1.0, b=(0, 0, 0), c=0.0, conv_kind=0, rot_c=(0.5, 0.5), pulse_c=(0.3, 0.5, 0.5),
@@ -44,9 +44,10 @@ def coeffs(eps=1.0, b=(0, 0, 0), c=0.0, conv_kind=0, rot_c=(0.5, 0.5), pulse_c=(
 
 
 def solver(smoother=1, pre=2, post=2, omega=0.8, coarse_smoother=3, coarse_maxit=10, coarse_red=0.1,
-           gmg_damp=1.0, restart=20, maxit=200, res_norm_min=1e-30, red_factor=1e-10, div_factor=1e10, cycle=1):
+           gmg_damp=1.0, restart=20, maxit=200, res_norm_min=1e-30, red_factor=1e-10, div_factor=1e10, cycle=1,
+           rel_tol=0.0):
     return Solver(smoother, pre, post, omega, coarse_smoother, coarse_maxit, coarse_red, gmg_damp,
-                  restart, maxit, res_norm_min, red_factor, div_factor, cycle)
+                  restart, maxit, res_norm_min, red_factor, div_factor, cycle, rel_tol)
 
 
 _lib = None

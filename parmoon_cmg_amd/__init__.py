@@ -77,7 +77,8 @@ class Request(C.Structure):
 class Result(C.Structure):
     _fields_ = [("id", C.c_uint64), ("status", C.c_int), ("iterations", C.c_int), ("rel_residual", C.c_double),
                 ("queue_ms", C.c_double), ("run_ms", C.c_double), ("latency_ms", C.c_double),
-                ("device_bytes", C.c_size_t), ("reused_hierarchy", C.c_int), ("n_dof", C.c_int)]
+                ("device_bytes", C.c_size_t), ("reused_hierarchy", C.c_int), ("n_dof", C.c_int),
+                ("steps_done", C.c_int)]
 
 
 class ServerStats(C.Structure):

@@ -251,13 +251,10 @@ static void ensure_krylov(cd3d_problem &P, int restart) {
 // ============================================================== assembly
 
 // CD3D_CELLWISE_ASSEMBLY=1 assembles constant coefficients cell by cell too
-// (the general kernel), for comparisons
+// (the general kernel), for comparisons; read at every assembly
 static bool uniform_assembly_off() {
-  static const bool v = [] {
-    const char *e = getenv("CD3D_CELLWISE_ASSEMBLY");
-    return e && atoi(e) != 0;
-  }();
-  return v;
+  const char *e = getenv("CD3D_CELLWISE_ASSEMBLY");
+  return e && atoi(e) != 0;
 }
 
 static double sdfem_delta_host(double hconv, double eps, const double *b, int order) {
@@ -699,7 +696,20 @@ Space lattice_space(cd3d_problem &P) {
   return S;
 }
 
-int fgmres(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S, double &r0, cd3d_status &status) {
+int fgmres_iterate(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S, double &r0, cd3d_status &status);
+
+// rfinal (optional): |b - A x| recomputed in the space after the iteration
+int fgmres(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S, double &r0, cd3d_status &status,
+           double *rfinal = nullptr) {
+  const int its = fgmres_iterate(P, sp, S, r0, status);
+  if (rfinal) {
+    S.defect(S.x, S.b, P.w);
+    *rfinal = dev_norm(P, S.n, P.w);
+  }
+  return its;
+}
+
+int fgmres_iterate(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S, double &r0, cd3d_status &status) {
   const int N = S.n;
   int R = std::min(sp.restart, sp.maxit);
   if (R <= 0) R = 10;
@@ -766,7 +776,12 @@ int fgmres(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S, double
       }
       if (res > sp.div_factor * res0) { status = CD3D_DIVERGED; return j; }
     }
-    update_x(R - 1);
+    // i columns were built in this cycle: i == R after a full cycle; i < R
+    // when maxit ends the loop inside a cycle.  The reference back-substitutes
+    // through restart - 1 columns in both cases (FgmresIte.C:520), which adds
+    // the previous cycle's leftover columns to the returned iterate; here
+    // only the columns of this cycle enter.
+    if (i > 0) update_x(i - 1);
     S.defect(S.x, S.b, Vk(0));
     beta = dev_norm(P, N, Vk(0));
     res0 = beta;
@@ -1183,8 +1198,10 @@ static void tcd_init_impl(cd3d_problem &P, const cd3d_coeffs &co0, double t0, co
 }
 
 // one step t -> t + tau; returns the FGMRES iterations
+// r0_out / rf_out: |b - A x| of the step's system before and after the solve
 static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_solver_params &sp, double tau,
-                         double theta, cd3d_status &solve_status) {
+                         double theta, cd3d_status &solve_status, double *r0_out = nullptr,
+                         double *rf_out = nullptr) {
   check_coeffs(co0);
   if (co0.supg) throw ArgError("TCD3D is Galerkin only, as TSystemTCD3D::Init (SystemTCD3D.C:114-131)");
   if (!P.tcd_ready) throw std::logic_error("cd3d_tcd_init before cd3d_tcd_step");
@@ -1283,11 +1300,12 @@ static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_sol
     }
     kl_dof_to_lat64(P.stream, F, P.bs, P.bl);
     kl_dof_to_lat64(P.stream, F, P.xs, P.xl);
-    its = fgmres(P, sp, lattice_space(P), r0, solve_status);
+    its = fgmres(P, sp, lattice_space(P), r0, solve_status, rf_out);
     kl_lat64_to_dof(P.stream, F, P.xl, P.xs);
   } else {
-    its = fgmres(P, sp, dof_space(P), r0, solve_status);
+    its = fgmres(P, sp, dof_space(P), r0, solve_status, rf_out);
   }
+  if (r0_out) *r0_out = r0;
   // the level matrices are the mass matrices again (RestoreMassMat,
   // SystemTCD3D.C:296-315, without arithmetic: M was not changed)
   coarse_join(P);
@@ -1312,10 +1330,11 @@ cd3d_status cd3d_tcd_step(cd3d_problem *p, const cd3d_coeffs *co, const cd3d_sol
   cd3d_status solve_status = CD3D_OK;
   const double t0 = now_ms();
   int its = 0;
+  double r0 = 0, rf = 0;
   cd3d_status rc = run(p, [&]() {
     check_params(*sp);
     check_theta(theta);
-    its = tcd_step_impl(*p, *co, *sp, tau, theta, solve_status);
+    its = tcd_step_impl(*p, *co, *sp, tau, theta, solve_status, &r0, st ? &rf : nullptr);
     const int N = p->dl.back().ndof;
     if (u_host) HIPCHK(hipMemcpyAsync(u_host, p->xs, sizeof(double) * N, hipMemcpyDeviceToHost, p->stream));
     HIPCHK(hipStreamSynchronize(p->stream));
@@ -1323,7 +1342,8 @@ cd3d_status cd3d_tcd_step(cd3d_problem *p, const cd3d_coeffs *co, const cd3d_sol
   if (st) {
     st->iterations = its;
     st->status = rc != CD3D_OK ? rc : solve_status;
-    st->r0 = st->rfinal = 0;
+    st->r0 = r0;
+    st->rfinal = rf;
     st->ms_assemble = 0;
     st->ms_solve = now_ms() - t0;
   }

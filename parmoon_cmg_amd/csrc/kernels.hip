@@ -1314,12 +1314,28 @@ void k_invert_active(hipStream_t s, const DevLevel &L, double *scratch, double *
   const int na = L.nact;
   if (na <= 0) return;
   if (na <= kGjLdsMax) {
-    static int lds_ok = -1;  // the larger-than-64 KiB LDS opt-in, once per process
+    // the larger-than-64 KiB LDS opt-in, once per device; the serving loop's
+    // worker threads reach this concurrently
+    static std::mutex lds_mu;
+    static int lds_ok[64];
+    static bool lds_init = false;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    bool ok;
+    {
+      std::lock_guard<std::mutex> g(lds_mu);
+      if (!lds_init) {
+        for (int &v : lds_ok) v = -1;
+        lds_init = true;
+      }
+      int &st = lds_ok[dev & 63];
+      if (st < 0)
+        st = hipFuncSetAttribute((const void *)gj_lds_k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(sizeof(double) * kGjLdsMax * kGjLdsMax)) == hipSuccess;
+      ok = st != 0;
+    }
     const size_t bytes = sizeof(double) * na * na;
-    if (lds_ok < 0)
-      lds_ok = hipFuncSetAttribute((const void *)gj_lds_k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)(sizeof(double) * kGjLdsMax * kGjLdsMax)) == hipSuccess;
-    if (lds_ok || bytes <= 65536) {
+    if (ok || bytes <= 65536) {
       gj_lds_k<<<1, 1024, bytes, s>>>(na, L.rowptr, L.col, L.val, inv);
       return;
     }

@@ -206,13 +206,19 @@ struct cd3d_server {
       std::vector<double> u(res.n_dof);
       st = cd3d_interp_pulse(h, r.mesh.levels - 1, &r.coeffs, u.data());
       if (st == CD3D_OK) st = cd3d_tcd_init(h, &r.coeffs, r.t0, u.data());
+      const bool initialised = st == CD3D_OK;
+      // a step that does not converge ends the run: later steps would start
+      // from an unconverged state
       for (int k = 0; k < r.nsteps && st == CD3D_OK; k++) {
         st = cd3d_tcd_step(h, &r.coeffs, &r.solver, r.tau, r.theta, nullptr, &s);
         res.iterations += s.iterations;
+        if (s.r0 > 0) res.rel_residual = std::max(res.rel_residual, s.rfinal / s.r0);
+        if (st == CD3D_OK) res.steps_done = k + 1;
       }
-      if (st == CD3D_OK && r.want_solution) {
+      if ((st == CD3D_OK || st == CD3D_NOT_CONVERGED) && initialised && r.want_solution) {
         d.sol.resize(res.n_dof);
-        st = cd3d_tcd_get_state(h, d.sol.data());
+        const cd3d_status g = cd3d_tcd_get_state(h, d.sol.data());
+        if (g != CD3D_OK) st = g;
       }
     }
     res.status = st;

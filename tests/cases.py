@@ -1,11 +1,15 @@
 """The golden cases written by tests/golden/make_ref_fixtures.py, described the
 way both the oracle and the HIP library take them.
 
-Each case: order, coarsest refinement k0 (n0 = 2^k0 cells per side), number of
-levels, and the problem coefficients (the .dat P-parameters of the reference
-run, decoded)."""
+Each case: order, macro bricks per side m (default 1: ParMooN's Wuerfel
+cube; m >= 2: the unit cube split into m^3 bricks by
+oracle/data/make_box_geo.py), coarsest refinement k0 (n0 = m 2^k0 cells per
+side), number of levels, the problem coefficients (the .dat P-parameters of
+the reference run, decoded) and, where the run changed them, the solver
+parameters that differ from REF_SOLVER."""
 
 REF_BOX = (1.0, 0.346, 0.346)
+UNIT_BOX = (1.0, 1.0, 1.0)
 
 CD_CASES = {
     "q1_l2_poisson": dict(order=1, k0=2, levels=2,
@@ -18,16 +22,56 @@ CD_CASES = {
                        co=dict(eps=1e-6, b=(-0.3, 0.7, 0.4), c=0.5, f_val=1.0, supg=1)),
     "q2_l2_galerkin_conv": dict(order=2, k0=1, levels=2,
                                 co=dict(eps=0.1, b=(1.0, 0.5, 0.25), c=1.0, f_kind=1)),
-    "unit_c1": dict(order=1, k0=2, levels=3, box=(1.0, 1.0, 1.0),
+    "unit_c1": dict(order=1, k0=2, levels=3, box=UNIT_BOX,
                     co=dict(eps=1.0, f_kind=1)),
-    "unit_c2_n8": dict(order=2, k0=1, levels=3, box=(1.0, 1.0, 1.0),
+    "unit_c2_n8": dict(order=2, k0=1, levels=3, box=UNIT_BOX,
                        co=dict(eps=1e-3, b=(1.0, 0.5, 0.25), f_val=1.0, supg=1)),
+    # macro-brick meshes (the batched and serving workloads' meshes)
+    "m2_q1_supg": dict(order=1, m=2, k0=1, levels=3, box=UNIT_BOX,
+                       co=dict(eps=1e-6, b=(0.48, -0.36, 0.64), f_val=1.0, supg=1)),
+    "m3_q1_supg": dict(order=1, m=3, k0=0, levels=3, box=UNIT_BOX,
+                       co=dict(eps=1e-2, b=(-0.5, 0.2, 0.6), f_val=1.0, supg=1)),
+    "m5_q1_supg": dict(order=1, m=5, k0=0, levels=2, box=UNIT_BOX,
+                       co=dict(eps=3e-4, b=(0.1, 0.7, -0.2), f_val=1.0, supg=1)),
+    "m2_q2_supg": dict(order=2, m=2, k0=0, levels=3, box=UNIT_BOX,
+                       co=dict(eps=1e-3, b=(0.6, 0.5, -0.4), f_val=1.0, supg=1)),
+    "m3_q2_supg": dict(order=2, m=3, k0=0, levels=2, box=UNIT_BOX,
+                       co=dict(eps=1e-5, b=(-0.2, -0.3, 0.5), f_val=1.0, supg=1)),
+    # SUPG with the rotating field (per-cell convection and SUPG delta)
+    "unit_q1_supg_rot": dict(order=1, k0=2, levels=3, box=UNIT_BOX, solver=dict(omega=0.5, pre=3, post=3),
+                             co=dict(eps=1e-3, conv_kind=1, rot_c=(0.45, 0.55), f_val=1.0, supg=1)),
+    "m2_q2_supg_rot": dict(order=2, m=2, k0=0, levels=3, box=UNIT_BOX, solver=dict(omega=0.5, pre=3, post=3),
+                           co=dict(eps=1e-4, conv_kind=1, rot_c=(0.45, 0.55), c=0.2, f_val=1.0, supg=1)),
+    # W- and F-cycle preconditioners
+    "m2_q1_wcycle": dict(order=1, m=2, k0=0, levels=4, box=UNIT_BOX, solver=dict(cycle=2),
+                         co=dict(eps=1e-2, b=(0.3, 0.4, -0.5), f_val=1.0, supg=1)),
+    "m2_q1_fcycle": dict(order=1, m=2, k0=0, levels=4, box=UNIT_BOX, solver=dict(cycle=0),
+                         co=dict(eps=1e-2, b=(0.3, 0.4, -0.5), f_val=1.0, supg=1)),
+    # the benchmark's smoother: damped Jacobi omega 0.5, 3 + 3 sweeps
+    "unit_c2_n8_bench": dict(order=2, k0=1, levels=3, box=UNIT_BOX, solver=dict(omega=0.5, pre=3, post=3),
+                             co=dict(eps=1e-3, b=(1.0, 0.5, 0.25), f_val=1.0, supg=1)),
 }
 
-TCD_CASE = dict(name="q1_tcd", order=1, k0=2, levels=2, tau=1e-3, nsteps=3,
-                co=dict(eps=1e-4, conv_kind=1, rot_c=(0.5, 0.173), pulse_c=(0.3, 0.173, 0.173), f_val=0.0))
+TCD_CASES = {
+    "q1_tcd": dict(order=1, k0=2, levels=2, tau=1e-3, nsteps=3, box=REF_BOX,
+                   co=dict(eps=1e-4, conv_kind=1, rot_c=(0.5, 0.173), pulse_c=(0.3, 0.173, 0.173), f_val=0.0)),
+    "m2_q1_tcd": dict(order=1, m=2, k0=1, levels=2, tau=1e-3, nsteps=3, box=UNIT_BOX,
+                      co=dict(eps=1e-4, conv_kind=1, rot_c=(0.5, 0.5), pulse_c=(0.3, 0.5, 0.5), f_val=0.0)),
+    "m2_q2_tcd": dict(order=2, m=2, k0=0, levels=2, tau=1e-3, nsteps=3, box=UNIT_BOX,
+                      co=dict(eps=1e-4, conv_kind=1, rot_c=(0.5, 0.5), pulse_c=(0.3, 0.5, 0.5), f_val=0.0)),
+}
+TCD_CASE = dict(TCD_CASES["q1_tcd"], name="q1_tcd")
 
 # solver of every reference run: Jacobi(0.8) 2+2, V-cycle, SSOR coarse smoother
 # (10 its / 0.1 reduction), FGMRES(20)
 REF_SOLVER = dict(smoother=1, pre=2, post=2, omega=0.8, coarse_smoother=3, coarse_maxit=10,
                   coarse_red=0.1, gmg_damp=1.0, restart=20, maxit=200)
+
+
+def case_solver(name):
+    """REF_SOLVER with the case's own solver changes."""
+    return dict(REF_SOLVER, **CD_CASES[name].get("solver", {}))
+
+
+def case_box(c):
+    return c.get("box", REF_BOX)

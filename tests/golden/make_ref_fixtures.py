@@ -76,14 +76,88 @@ CASES = {
                            P0=1e-4, P4=0, P5=0, P6=0.0, P7=1, P8=3,
                            P9=0.3, P10=0.173, P11=0.173, P12=0.5, P13=0.173,
                            TIME_DISC=2, STARTTIME=0.0, ENDTIME=1.0, TIMESTEPLENGTH=1e-3)),
+    # unit cube split into m^3 macro bricks (the meshes of the batched and
+    # serving workloads: coarsest level m^3 cells), SUPG with non-axis
+    # convection over several decades of eps
+    "m2_q1_supg": ("cd", dict(geo="m2", ANSATZ_ORDER=1, LEVELS=3, UNIFORM_STEPS=1, DISCTYPE=2,
+                              CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                              P0=1e-6, P1=0.48, P2=-0.36, P3=0.64, P4=0, P5=0, P6=1.0)),
+    "m3_q1_supg": ("cd", dict(geo="m3", ANSATZ_ORDER=1, LEVELS=3, UNIFORM_STEPS=0, DISCTYPE=2,
+                              CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                              P0=1e-2, P1=-0.5, P2=0.2, P3=0.6, P4=0, P5=0, P6=1.0)),
+    "m5_q1_supg": ("cd", dict(geo="m5", ANSATZ_ORDER=1, LEVELS=2, UNIFORM_STEPS=0, DISCTYPE=2,
+                              CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                              P0=3e-4, P1=0.1, P2=0.7, P3=-0.2, P4=0, P5=0, P6=1.0)),
+    "m2_q2_supg": ("cd", dict(geo="m2", ANSATZ_ORDER=2, LEVELS=3, UNIFORM_STEPS=0, DISCTYPE=2,
+                              CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                              P0=1e-3, P1=0.6, P2=0.5, P3=-0.4, P4=0, P5=0, P6=1.0)),
+    "m3_q2_supg": ("cd", dict(geo="m3", ANSATZ_ORDER=2, LEVELS=2, UNIFORM_STEPS=0, DISCTYPE=2,
+                              CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                              P0=1e-5, P1=-0.2, P2=-0.3, P3=0.5, P4=0, P5=0, P6=1.0)),
+    # W- and F-cycles (SC_MG_CYCLE_SCALAR 2 and 0, MultiGrid3D.C:690-703)
+    "m2_q1_wcycle": ("cd", dict(geo="m2", ANSATZ_ORDER=1, LEVELS=4, UNIFORM_STEPS=0, DISCTYPE=2,
+                                CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0, SC_MG_CYCLE_SCALAR=2,
+                                P0=1e-2, P1=0.3, P2=0.4, P3=-0.5, P4=0, P5=0, P6=1.0)),
+    "m2_q1_fcycle": ("cd", dict(geo="m2", ANSATZ_ORDER=1, LEVELS=4, UNIFORM_STEPS=0, DISCTYPE=2,
+                                CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0, SC_MG_CYCLE_SCALAR=0,
+                                P0=1e-2, P1=0.3, P2=0.4, P3=-0.5, P4=0, P5=0, P6=1.0)),
+    # the smoother of the GPU benchmark (damped Jacobi omega 0.5, 3 + 3 sweeps)
+    "unit_c2_n8_bench": ("cd", dict(geo="unit", ANSATZ_ORDER=2, LEVELS=3, UNIFORM_STEPS=1, DISCTYPE=2,
+                                    CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                                    SC_SMOOTH_DAMP_FACTOR_SCALAR=0.5, SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=0.5,
+                                    SC_PRE_SMOOTH_SCALAR=3, SC_POST_SMOOTH_SCALAR=3,
+                                    P0=1e-3, P1=1.0, P2=0.5, P3=0.25, P4=0, P5=0, P6=1.0)),
+    # SUPG with the rotating field (convection varying from cell to cell, so
+    # the element matrices and SUPG delta differ per cell)
+    "unit_q1_supg_rot": ("cd", dict(geo="unit", ANSATZ_ORDER=1, LEVELS=3, UNIFORM_STEPS=2, DISCTYPE=2,
+                                    CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                                    SC_SMOOTH_DAMP_FACTOR_SCALAR=0.5, SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=0.5,
+                                    SC_PRE_SMOOTH_SCALAR=3, SC_POST_SMOOTH_SCALAR=3,
+                                    P0=1e-3, P4=0, P5=0, P6=1.0, P7=1, P12=0.45, P13=0.55)),
+    "m2_q2_supg_rot": ("cd", dict(geo="m2", ANSATZ_ORDER=2, LEVELS=3, UNIFORM_STEPS=0, DISCTYPE=2,
+                                  CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                                  SC_SMOOTH_DAMP_FACTOR_SCALAR=0.5, SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=0.5,
+                                  SC_PRE_SMOOTH_SCALAR=3, SC_POST_SMOOTH_SCALAR=3,
+                                  P0=1e-4, P4=0.2, P5=0, P6=1.0, P7=1, P12=0.45, P13=0.55)),
+    # Crank-Nicolson with the rotating field on macro-brick meshes, Q1 and Q2
+    "m2_q1_tcd": ("tcd", dict(geo="m2", ANSATZ_ORDER=1, LEVELS=2, UNIFORM_STEPS=1, DISCTYPE=1,
+                              P0=1e-4, P4=0, P5=0, P6=0.0, P7=1, P8=3,
+                              P9=0.3, P10=0.5, P11=0.5, P12=0.5, P13=0.5,
+                              TIME_DISC=2, STARTTIME=0.0, ENDTIME=1.0, TIMESTEPLENGTH=1e-3)),
+    "m2_q2_tcd": ("tcd", dict(geo="m2", ANSATZ_ORDER=2, LEVELS=2, UNIFORM_STEPS=0, DISCTYPE=1,
+                              P0=1e-4, P4=0, P5=0, P6=0.0, P7=1, P8=3,
+                              P9=0.3, P10=0.5, P11=0.5, P12=0.5, P13=0.5,
+                              TIME_DISC=2, STARTTIME=0.0, ENDTIME=1.0, TIMESTEPLENGTH=1e-3)),
+}
+
+# Records only (iteration count and residual history, no level data): the
+# configuration-2 problem at 16^3 cells with the configuration-1 smoother
+# (omega 0.8, 2 + 2) and with the benchmark's (omega 0.5, 3 + 3)
+_C2_16 = dict(geo="unit", ANSATZ_ORDER=2, LEVELS=3, UNIFORM_STEPS=2, DISCTYPE=2,
+              CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+              P0=1e-3, P1=1.0, P2=0.5, P3=0.25, P4=0, P5=0, P6=1.0)
+RECORDS = {
+    "record_c2_n16_omega08": ("cd", dict(_C2_16)),
+    "record_c2_n16_omega05": ("cd", dict(_C2_16, SC_SMOOTH_DAMP_FACTOR_SCALAR=0.5,
+                                         SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=0.5,
+                                         SC_PRE_SMOOTH_SCALAR=3, SC_POST_SMOOTH_SCALAR=3)),
 }
 
 
 def write_dat(path, overrides):
     overrides = dict(overrides)
-    if overrides.pop("geo", None) == "unit":
-        data = os.path.join(REPO, "oracle", "data")
+    geo_kind = overrides.pop("geo", None)
+    data = os.path.join(REPO, "oracle", "data")
+    if geo_kind == "unit":
         geo, prm = os.path.join(data, "unitcube.GEO"), os.path.join(data, "unitcube.PRM")
+    elif geo_kind is not None and geo_kind.startswith("m"):
+        # unit cube split into m^3 macro bricks (oracle/data/make_box_geo.py)
+        sys.path.insert(0, data)
+        from make_box_geo import box_geo
+        geo = os.path.join(os.path.dirname(path), f"box_{geo_kind}.GEO")
+        with open(geo, "w") as f:
+            f.write(box_geo(int(geo_kind[1:])))
+        prm = os.path.join(data, "unitcube.PRM")
     else:
         geo, prm = f"{REF}/data/Wuerfel.GEO", f"{REF}/data/Wuerfel.PRM"
     kv = dict(GEOFILE=geo, BNDFILE=prm,
@@ -97,7 +171,8 @@ def write_dat(path, overrides):
 
 
 def run_case(name):
-    mode, ov = CASES[name]
+    record = name in RECORDS
+    mode, ov = RECORDS[name] if record else CASES[name]
     with tempfile.TemporaryDirectory() as td:
         dat = os.path.join(td, "case.dat")
         write_dat(dat, ov)
@@ -105,7 +180,7 @@ def run_case(name):
         if p.returncode:
             print(p.stdout[-3000:], p.stderr[-3000:])
             raise SystemExit(f"{name}: reference run failed ({p.returncode})")
-        out = {"params": np.array(repr(ov)), "box": np.array(UNIT if ov.get("geo") == "unit" else BOX)}
+        out = {"params": np.array(repr(ov)), "box": np.array(UNIT if ov.get("geo") else BOX)}
         levels = [tuple(map(int, re.findall(r"-?\d+", l)[:8]))
                   for l in p.stdout.splitlines() if l.startswith("LEVEL ")]
         for lev, nc, nbf, ndof, act, hb, ndir, nnz in levels:
@@ -133,6 +208,9 @@ def run_case(name):
             steps = int(ov["P8"])
             for k in range(steps + 1):
                 out[f"u{k}"] = np.fromfile(os.path.join(td, f"tcd_u{k}.f64"), dtype=np.float64)
+        if record:
+            keep = ("params", "box", "its", "r0", "rfinal", "res_hist", "ref_times")
+            out = {k: v for k, v in out.items() if k in keep}
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **out)
         print(f"{name}: {[l[:8] for l in levels]}", "its" in out and int(out["its"]) or "")
 
@@ -140,7 +218,7 @@ def run_case(name):
 def main():
     if not os.path.exists(EXE):
         raise SystemExit("build oracle/_ref first: python oracle/build_ref.py")
-    for name in (sys.argv[1:] or CASES):
+    for name in (sys.argv[1:] or list(CASES) + list(RECORDS)):
         run_case(name)
 
 

@@ -8,12 +8,16 @@ checks one oracle function against the reference's output:
   V-cycle probe z = V(r)                  1e-12 relative
   FGMRES                                  same iteration count, solution 1e-10 rel. L2
   TCD3D Crank-Nicolson states             1e-10 relative
+Cases cover ParMooN's Wuerfel cube and the unit cube split into m^3 macro
+bricks (m = 2, 3, 5; the meshes of the batched and serving workloads), Q1 and
+Q2, Galerkin and SUPG over eps 1e-6 ... 1, V-, W- and F-cycles, and the
+benchmark's smoother (omega 0.5, 3 + 3 sweeps).
 """
 import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from tests.cases import CD_CASES, REF_SOLVER, TCD_CASE
+from tests.cases import CD_CASES, TCD_CASES, REF_SOLVER, case_box, case_solver
 
 ENTRY_TOL = 1e-12
 SOL_TOL = 1e-10
@@ -32,7 +36,8 @@ def hier():
         if name not in cache:
             c = CD_CASES[name]
             d = golden(name)
-            H = po.Hierarchy(c["order"], 1, c["k0"], c["levels"], box=tuple(d["box"]))
+            assert tuple(d["box"]) == tuple(case_box(c))
+            H = po.Hierarchy(c["order"], c.get("m", 1), c["k0"], c["levels"], box=tuple(d["box"]))
             H.assemble(po.coeffs(**c["co"]))
             cache[name] = (H, H.rhs(c["levels"] - 1))
         return cache[name]
@@ -49,7 +54,8 @@ def test_structure_bit_exact(name, golden, hier):
         nc, nbf, ndof, act, hb, ndir, nnz = d[f"L{l}_meta"].tolist()
         assert (i["ncell"], i["ndof"], i["nact"], i["nnz"]) == (nc, ndof, act, nnz)
         assert ndir == ndof - act and hb == act
-        assert np.abs(H.cells(l) - d[f"L{l}_cells"]).max() <= 1e-15
+        # vertex coordinates: the GEO files of m = 3, 5 hold i/m rounded once
+        assert np.abs(H.cells(l) - d[f"L{l}_cells"]).max() <= 2.3e-16
         assert np.array_equal(H.gn(l), d[f"L{l}_gn"])
         rp, kc, _ = H.csr(l)
         assert np.array_equal(rp, d[f"L{l}_rowptr"])
@@ -73,10 +79,10 @@ def test_assembly_entries(name, golden, hier):
 def test_vcycle_and_fgmres(name, golden, hier):
     d = golden(name)
     H, b = hier(name, golden)
-    sv = po.solver(**REF_SOLVER)
+    sv = po.solver(**case_solver(name))
     z = H.vcycle(sv, d["vcycle_in"])
     assert np.abs(z - d["vcycle_out"]).max() <= 1e-12 * np.abs(d["vcycle_out"]).max()
-    sv = po.solver(**dict(REF_SOLVER, res_norm_min=1e-10 * float(d["r0"]), red_factor=0.0))
+    sv = po.solver(**dict(case_solver(name), res_norm_min=1e-10 * float(d["r0"]), red_factor=0.0))
     x, its, hist = H.fgmres(sv, b)
     assert its == int(d["its"])
     assert np.linalg.norm(x - d["sol"]) <= SOL_TOL * np.linalg.norm(d["sol"])
@@ -84,10 +90,20 @@ def test_vcycle_and_fgmres(name, golden, hier):
     assert abs(hist[0] / d["res_hist"][0] - 1) < 1e-12
 
 
-def test_tcd_crank_nicolson(golden):
-    c = TCD_CASE
-    d = golden(c["name"])
-    H = po.Hierarchy(c["order"], 1, c["k0"], c["levels"], box=tuple(d["box"]))
+def test_cycle_kinds_differ(golden):
+    # the W- and F-cycle fixtures (4 levels) are different preconditioners:
+    # a plausible mix-up of the recursion counts would make them equal
+    w, f = golden("m2_q1_wcycle"), golden("m2_q1_fcycle")
+    assert np.array_equal(w["vcycle_in"], f["vcycle_in"])
+    assert np.abs(w["vcycle_out"] - f["vcycle_out"]).max() > 1e-3 * np.abs(w["vcycle_out"]).max()
+
+
+@pytest.mark.parametrize("name", list(TCD_CASES))
+def test_tcd_crank_nicolson(name, golden):
+    c = TCD_CASES[name]
+    d = golden(name)
+    assert tuple(d["box"]) == tuple(c["box"])
+    H = po.Hierarchy(c["order"], c.get("m", 1), c["k0"], c["levels"], box=tuple(d["box"]))
     co = po.coeffs(**c["co"])
     u0 = H.interp_pulse(c["levels"] - 1, co)
     nact = H.info(c["levels"] - 1)["nact"]
@@ -97,3 +113,21 @@ def test_tcd_crank_nicolson(golden):
     for k in range(c["nsteps"]):
         ref = d[f"u{k + 1}"]
         assert np.abs(states[k] - ref).max() <= SOL_TOL * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("omega,sweeps,name", [(0.8, 2, "record_c2_n16_omega08"), (0.5, 3, "record_c2_n16_omega05")])
+def test_c2_smoother_records(omega, sweeps, name, golden):
+    # the configuration-2 problem (Q2 SUPG, eps 1e-3, b = (1, 0.5, 0.25)) at
+    # 16^3 cells: the reference needs 120 iterations with the
+    # configuration-1 smoother and 21 with the benchmark's; the oracle
+    # reproduces both counts
+    d = golden(name)
+    H = po.Hierarchy(2, 1, 2, 3, box=tuple(d["box"]))
+    H.assemble(po.coeffs(eps=1e-3, b=(1.0, 0.5, 0.25), f_val=1.0, supg=1))
+    b = H.rhs(2)
+    _, r0 = H.defect(2, np.zeros_like(b), b)
+    assert abs(r0 / float(d["r0"]) - 1) < 1e-12
+    sv = po.solver(**dict(REF_SOLVER, omega=omega, pre=sweeps, post=sweeps, res_norm_min=1e-10 * r0,
+                          red_factor=0.0))
+    _, its, _ = H.fgmres(sv, b)
+    assert its == int(d["its"]) == (120 if omega == 0.8 else 21)

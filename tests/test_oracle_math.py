@@ -193,3 +193,23 @@ def test_mass_matrix_integrates_volume():
         M1 = np.array([e[rp[i]:rp[i + 1]].sum() for i in range(na)])
         assert np.abs(M1 - r[:na]).max() < 1e-15
         assert r[:na].sum() < np.prod(box)
+
+
+def test_rel_tol_is_the_absolute_threshold_of_the_first_residual():
+    # rel_tol r stops exactly where res_norm_min = r |b - A x0| with
+    # red_factor 0 does (the reference's way of asking for a relative
+    # residual of r, SC_LIN_RES_NORM_MIN_SCALAR), also across restarts
+    H = po.Hierarchy(1, 2, 1, 3, box=po.UNIT_BOX)
+    H.assemble(po.coeffs(eps=1e-4, b=(0.3, -0.8, 0.5), f_val=1.0, supg=1))
+    b = H.rhs(2)
+    _, r0 = H.defect(2, np.zeros_like(b), b)
+    base = dict(omega=0.5, pre=1, post=1, coarse_smoother=17, restart=5, maxit=300)
+    xa, ia, _ = H.fgmres(po.solver(**base, rel_tol=1e-10), b)
+    xb, ib, _ = H.fgmres(po.solver(**base, res_norm_min=1e-10 * r0, red_factor=0.0), b)
+    assert ia == ib and ia > 10  # several restart cycles
+    assert np.array_equal(xa, xb)
+    _, rf = H.defect(2, xa, b)
+    assert rf <= 1.0001e-10 * r0
+    # the restart-relative test of the reference asks for 1e-10 again at every restart
+    _, ic, _ = H.fgmres(po.solver(**base, res_norm_min=0.0, red_factor=1e-10), b)
+    assert ic > ia

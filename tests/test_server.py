@@ -181,3 +181,77 @@ def test_server_result_equals_the_handle_api():
     srv.close()
     assert r.iterations == st.iterations
     assert np.array_equal(xs, x)
+
+
+# ---------------------------------------------------------------- workload draws against the oracle
+
+def _draws():
+    """16 requests drawn by the benchmark's own generators (configurations 4
+    and 5) at small n, with every kind the serving stream sends: Q1 and Q2
+    CD3D SUPG over eps 1e-6 ... 1, and TCD3D rotating-field runs."""
+    import bench
+    c4 = bench.c4_requests(count=6, seed=11, n_range=(8, 16))
+    c5 = [a for a in bench.c5_arrivals(rate=100.0, seed=12, span=1.0, n_range=(8, 12), tcd_share=0.3)]
+    cd5 = [a for a in c5 if not a["tcd"]][:6]
+    tcd5 = [a for a in c5 if a["tcd"]][:4]
+    return c4 + cd5 + tcd5
+
+
+TCD_TEST_STEPS = 3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_workload_draws_match_the_oracle(precision):
+    # every request through the server (all in flight together), each result
+    # checked against the FP64 oracle run of the same request: FP64 form the
+    # same iteration count and the solution within 1e-10; mixed form relative
+    # residual 1e-10 and the solution within 1e-9 (TCD3D states: 1e-8 after
+    # 3 steps)
+    import bench
+    from oracle import pyoracle as po
+    reqs = _draws()
+    prec = pm.FP64 if precision == "fp64" else pm.MIXED
+    sp_cd = pm.solver_params(**dict(bench.SOLVER, maxit=1000), precision=prec)
+    sp_tcd = bench.tcd_solver(pm, prec)
+    srv = pm.Server(max_inflight=8)
+    ids = []
+    for q in reqs:
+        mesh = pm.mesh_desc(q["order"], q["m"], q["k0"], q["levels"])
+        if q.get("tcd"):
+            ids.append(srv.submit(mesh, pm.coeffs(**q["co"]), sp_tcd, kind=pm.REQ_TCD, tau=1e-3,
+                                  nsteps=TCD_TEST_STEPS, want_solution=True))
+        else:
+            ids.append(srv.submit(mesh, pm.coeffs(**q["co"]), sp_cd, want_solution=True))
+    out = []
+    for q, i in zip(reqs, ids):
+        nd = (q["order"] * q["n"] + 1) ** 3
+        out.append(srv.wait(i, n_dof=nd))
+    srv.close()
+    osv = dict(smoother=1, pre=bench.SOLVER["pre"], post=bench.SOLVER["post"], omega=bench.SOLVER["omega"],
+               coarse_smoother=17, restart=20, maxit=1000, res_norm_min=0.0, red_factor=0.0, rel_tol=1e-10)
+    for q, (r, x) in zip(reqs, out):
+        H = po.Hierarchy(q["order"], q["m"], q["k0"], q["levels"], box=(1.0, 1.0, 1.0))
+        tag = (f"{'tcd' if q.get('tcd') else 'cd'} Q{q['order']} n={q['n']} m={q['m']} eps={q['co']['eps']:.1e} "
+               f"{precision}")
+        assert r.status == 0, tag
+        if q.get("tcd"):
+            co = po.coeffs(**q["co"])
+            u0 = H.interp_pulse(q["levels"] - 1, co)
+            xo, _, its_o = H.tcd_run(co, po.solver(**osv), 0.0, 1e-3, TCD_TEST_STEPS, u0)
+            tol = SOL_TOL_FP64 if prec == pm.FP64 else 1e-8
+            assert r.steps_done == TCD_TEST_STEPS
+        else:
+            H.assemble(po.coeffs(**q["co"]))
+            b = H.rhs(q["levels"] - 1)
+            xo, its_o, _ = H.fgmres(po.solver(**osv), b)
+            tol = SOL_TOL_FP64 if prec == pm.FP64 else 1e-9
+        err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+        say(f"[gpu] draw {tag}: its {r.iterations} (oracle {its_o}) rel res {r.rel_residual:.2e} err {err:.2e}")
+        assert r.rel_residual <= 1.0001e-10
+        if prec == pm.FP64:
+            assert r.iterations == its_o, tag
+        assert err < tol, tag
+
+
+SOL_TOL_FP64 = 1e-10
