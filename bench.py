@@ -14,8 +14,10 @@ solve, FGMRES(20).  (With ParMooN's configuration-1 smoother, omega 0.8 and
 
 value = solves/s of the mixed-precision form (FP32 V-cycle, FP64 FGMRES,
 residual, coarse solve and SUPG tau); the plain FP64 form, p50/p99 latency,
-the fine-level SpMV bandwidth (roofline of the dominant kernel) and the
-reference CPU path (timed on this host) are reported beside it.
+the fine-level kernels against their rooflines (HBM bandwidth; FP64 flops for
+the element kernel), the reference CPU path (its C restatement on the same
+problem, timed on this host's cores) and the error of both GPU forms against
+that FP64 solution are reported beside it.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -23,7 +25,6 @@ import argparse
 import ctypes as C
 import json
 import os
-import subprocess
 import sys
 import tempfile
 import time
@@ -50,7 +51,10 @@ KERNEL_NAMES = {0: "csr_stream<kSpmv,f64>",
                 3: "lat_restrict_pair_k f32 (coarse vertex/midpoint pairs)",
                 4: "lat_prolong_k<f32>", 5: "csr_fill_k",
                 6: "copy_k (f64 vector copy, traffic calibration)",
-                7: "axpy_dot2_k f64 (fused Gram-Schmidt step, 16-byte pairs)"}
+                7: "axpy_dot2_k f64 (fused Gram-Schmidt step, 16-byte pairs)",
+                8: "assemble_k<2> f64 (cell-by-cell SUPG element kernel, 8 colour launches)"}
+FP64_PEAK_TFLOPS = 78.6
+FLOP_KERNELS = {8}
 
 
 def kernel_table(P, reps=50):
@@ -60,9 +64,13 @@ def kernel_table(P, reps=50):
     l = P.levels - 1
     out = {}
     for which, name in KERNEL_NAMES.items():
-        ms, nbytes = P.kernel_bench(which, l, reps)
-        out[name] = dict(us_per_launch=round(ms * 1e3, 2), algorithmic_bytes=int(nbytes),
-                         achieved_gbs=round(nbytes / (ms * 1e-3) / 1e9, 1))
+        ms, amount = P.kernel_bench(which, l, reps if which not in FLOP_KERNELS else max(3, reps // 10))
+        if which in FLOP_KERNELS:  # FP64 flops per launch instead of bytes
+            out[name] = dict(us_per_launch=round(ms * 1e3, 2), algorithmic_flops=int(amount),
+                             achieved_tflops=round(amount / (ms * 1e-3) / 1e12, 3))
+        else:
+            out[name] = dict(us_per_launch=round(ms * 1e3, 2), algorithmic_bytes=int(amount),
+                             achieved_gbs=round(amount / (ms * 1e-3) / 1e9, 1))
     return out
 
 
@@ -71,7 +79,7 @@ def kernel_table(P, reps=50):
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "cd3d_q2_64_fine_level_traffic.json")
 TRAFFIC_KERNEL = {0: "void csr_stream<0, double, double, true>", 1: "void lat_pair_gk<0, 4>",
                   2: "void lat64_pair_gk<0, 2>", 3: "lat_restrict_pair_k", 4: "void lat_prolong_k<2>",
-                  5: "csr_fill2_k", 6: "copy_k", 7: "void axpy_dot2_k<true>"}
+                  5: "csr_fill2_k", 6: "copy_k", 7: "void axpy_dot2_k<true>", 8: "void assemble_k<2>"}
 
 
 def measured_traffic(which):
@@ -89,6 +97,14 @@ def roofline_entry(table, which, kernel_note):
                 frac=round(k["achieved_gbs"] / HBM_PEAK_GBS, 4), traffic=measured_traffic(which),
                 kernel=KERNEL_NAMES[which], note=kernel_note, us_per_launch=k["us_per_launch"],
                 algorithmic_bytes=k["algorithmic_bytes"])
+
+
+def flop_roofline_entry(table, which, kernel_note):
+    k = table[KERNEL_NAMES[which]]
+    return dict(bound="fp64", achieved=k["achieved_tflops"], peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+                frac=round(k["achieved_tflops"] / FP64_PEAK_TFLOPS, 4), traffic=measured_traffic(which),
+                kernel=KERNEL_NAMES[which], note=kernel_note, us_per_launch=k["us_per_launch"],
+                algorithmic_flops=k["algorithmic_flops"])
 
 
 def run_steps(pm, P, sp, co, warmup, steps):
@@ -109,52 +125,40 @@ def run_steps(pm, P, sp, co, warmup, steps):
     return np.array(lat), its, res
 
 
-def cpu_baseline(sample_levels=4, timeout=240):
-    """Reference CPU path (oracle/_ref, compiled from the reference's sources)
-    on a bounded sample: the same problem at 32^3 cells (4 levels, 274 625
-    DOFs, 1/8 of the workload), one thread; solves/s scaled to the workload
-    by the DOF ratio."""
-    exe = os.path.join(REPO, "oracle", "_ref", "cd3d_ref")
-    if not os.path.exists(exe):
-        return None
-    data = os.path.join(REPO, "oracle", "data")
-    kv = dict(GEOFILE=os.path.join(data, "unitcube.GEO"), BNDFILE=os.path.join(data, "unitcube.PRM"), MESH_TYPE=0,
-              OUTFILE="ref.out", LOGFILE="ref.log", MEASURE_ERRORS=0, WRITE_VTK=0, WRITE_PS=0,
-              ANSATZ_ORDER=2, LEVELS=sample_levels, UNIFORM_STEPS=2, DISCTYPE=2, CELL_MEASURE=4, SDFEM_TYPE=2,
-              DELTA0=1.0, P0=1e-3, P1=1.0, P2=0.5, P3=0.25, P4=0, P5=0, P6=1.0,
-              SOLVER_TYPE=1, SC_MG_TYPE_SCALAR=0, SC_SOLVER_SCALAR=16, SC_PRECONDITIONER_SCALAR=5,
-              SC_GMRES_RESTART=20, SC_LIN_MAXIT_SCALAR=400, SC_LIN_RED_FACTOR_SCALAR=1e-10,
-              SC_LIN_RES_NORM_MIN_SCALAR=1e-30, SC_MG_CYCLE_SCALAR=1, SC_SMOOTHER_SCALAR=1,
-              SC_PRE_SMOOTH_SCALAR=3, SC_POST_SMOOTH_SCALAR=3, SC_SMOOTH_DAMP_FACTOR_SCALAR=0.5,
-              SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=0.5, SC_COARSE_SMOOTHER_SCALAR=3, SC_COARSE_MAXIT_SCALAR=10,
-              SC_COARSE_RED_FACTOR_SCALAR=0.1, SC_GMG_DAMP_FACTOR_SCALAR=1.0, SC_STEP_LENGTH_CONTROL_FINE_SCALAR=0,
-              SC_STEP_LENGTH_CONTROL_ALL_SCALAR=0, SC_AMG_PREC_IT=1, SC_MINIT=0, SC_VERBOSE=0,
-              SC_FLEXIBLE_KRYLOV_SPACE_SOLVER=1)
-    with tempfile.TemporaryDirectory() as td:
-        dat = os.path.join(td, "c2.dat")
-        with open(dat, "w") as f:
-            for k, v in kv.items():
-                f.write(f"{k}: {v}\n")
-        t = time.perf_counter()
-        try:
-            p = subprocess.run([exe, dat, "-", "cd"], cwd=td, capture_output=True, text=True, timeout=timeout)
-        except subprocess.TimeoutExpired:
-            return None
-        wall = time.perf_counter() - t
-    if p.returncode != 0:
-        log("reference run failed", p.stderr[-500:])
-        return None
-    tline = [l for l in p.stdout.splitlines() if l.startswith("TIME ")][-1].split()
-    sline = [l for l in p.stdout.splitlines() if l.startswith("SOLVE ")][-1].split()
-    t_asm, t_solve = float(tline[6]), float(tline[8])
-    ndof_sample = (2 * 2 ** (2 + sample_levels - 1) + 1) ** 3
-    ndof_work = (2 * 64 + 1) ** 3
-    per_solve = (t_asm + t_solve) * ndof_work / ndof_sample
-    return dict(value=round(1.0 / per_solve, 6), unit="solves/s", cores=1, kind="reference",
-                sample=f"reference (oracle/_ref, -O2, _SEQ, 1 thread) on the same problem at 32^3 cells "
-                       f"(Q2, 4 levels, {ndof_sample} DOFs): assembly {t_asm:.2f} s + solve {t_solve:.2f} s "
-                       f"({sline[2]} FGMRES its, SSOR coarse smoother: no LAPACK in the build), wall {wall:.1f} s; "
-                       f"scaled by the DOF ratio {ndof_work / ndof_sample:.2f} to the 64^3 workload")
+CPU_RECORD = os.path.join(REPO, "profiles", "cpu_reference_vs_port_c2_64.json")
+
+
+def cpu_baseline(nproc=None, sol_out=None):
+    """The reference CPU path on this host: the C restatement of the
+    reference (oracle/cd3d_oracle.c, -O2, one thread per process) on the full
+    workload -- the same 64^3 Q2 problem, assembly of all 5 levels plus the
+    FGMRES solve to 1e-10, mesh and patterns built before the clock starts --
+    as nproc independent solves at once on nproc host cores (throughput mode).
+    Beside it: the restatement's and the reference's single-thread times on
+    one core of the build container (profiles/cpu_reference_vs_port_c2_64.json)."""
+    from oracle import cpu_baseline as cb
+
+    n = nproc or cb.host_cores()
+    r = cb.run_parallel(n, sol_out=sol_out, log=log)
+    model = cb.host_cpu_model()
+    rec = None
+    try:
+        with open(CPU_RECORD) as fh:
+            rec = json.load(fh)
+    except (OSError, ValueError):
+        pass
+    per = float(np.median(r["per_worker_timed_s"]))
+    out = dict(value=round(r["solves_per_s"], 6), unit="solves/s", cores=n, kind="port",
+               sample=f"the full workload (Q2 64^3, {r['ndof']} DOFs, assembly of 5 levels + FGMRES to 1e-10, "
+                      f"{r['iterations']} its): {n} independent single-thread solves of the C restatement at once "
+                      f"on {n} cores of {model}; median {per:.1f} s per solve (assembly "
+                      f"{np.median(r['assemble_s']):.1f} s + solve {np.median(r['solve_s']):.1f} s), "
+                      f"value = {n} / slowest ({r['max_timed_s']:.1f} s)",
+               cpu_model=model, per_solve_s_median=round(per, 3), iterations=r["iterations"],
+               rel_residual=r["rel_residual"])
+    if rec:
+        out["single_core_container"] = rec
+    return out
 
 
 # ---------------------------------------------------------------- configurations 3-5
@@ -350,6 +354,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-procs", type=int, default=0, help="CPU baseline processes (default: host cores, at most 16)")
     ap.add_argument("--no-extra", action="store_true", help="only the headline configuration")
     ap.add_argument("--c3-steps", type=int, default=100)
     ap.add_argument("--c4-problems", type=int, default=256)
@@ -416,12 +421,11 @@ def main():
                                     "once, 12 bytes per node)")
     mgs = roofline_entry(table, 7, "fused Gram-Schmidt step of the outer FGMRES on the fine level (32 bytes per "
                          "unknown); it and the sweep take the largest shares of the mixed step's kernel time")
+    asm = flop_roofline_entry(table, 8, "per-element quadrature (Gauss4, 64 points) of the Q2 SUPG element "
+                              "matrices on the fine level, colour-scattered into CSR; algorithmic flops "
+                              "n_cells n_q 8 n_b^2 per assembly (the path for variable coefficients)")
     spmv = roofline_entry(table, 0, "FP64 CSR SpMV of the fine level (12 nnz + 20 N bytes); the FP64 form's "
                                     "smoother, defect and FGMRES product")
-    cpu = None
-    if not args.no_cpu:
-        log("[bench] reference CPU baseline (bounded sample)")
-        cpu = cpu_baseline()
     extra = {}
     if not args.no_extra:
         for name, prec in (("fp64", pm.FP64), ("mixed", pm.MIXED)):
@@ -443,6 +447,17 @@ def main():
         ss = extra["serving_stream"]
         # at the highest offered rate both forms run at capacity
         ss["mixed_vs_fp64_at_max_rate"] = round(ss["mixed"][-1]["solves_per_s"] / ss["fp64"][-1]["solves_per_s"], 3)
+    cpu, vs_oracle = None, {}
+    if not args.no_cpu:
+        log("[bench] reference CPU path (C restatement) on the full workload, one solve per host core")
+        with tempfile.TemporaryDirectory() as td:
+            ofile = os.path.join(td, "oracle_sol.npy")
+            cpu = cpu_baseline(args.cpu_procs or None, sol_out=ofile)
+            xo = np.load(ofile)
+        # error of each GPU form against the FP64 oracle's solution of the same problem
+        for name in sols:
+            vs_oracle[name] = float(np.linalg.norm(sols[name] - xo) / np.linalg.norm(xo))
+        log(f"[bench] solution error vs the FP64 oracle: " + ", ".join(f"{k} {v:.2e}" for k, v in vs_oracle.items()))
     m = res["mixed"]
     out = {
         "metric": METRIC, "value": round(m["solves_per_s"], 4), "unit": "solves/s", "n_gpus": 1,
@@ -456,11 +471,13 @@ def main():
         "fp64": {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in res["fp64"].items()},
         "mixed_vs_fp64_speedup": round(m["solves_per_s"] / res["fp64"]["solves_per_s"], 3),
         "mixed_solution_rel_err_vs_fp64": err,
+        "solution_rel_err_vs_fp64_oracle": vs_oracle or None,
         "p50_ms": round(m["p50_ms"], 3), "p99_ms": round(m["p99_ms"], 3),
         "spmv_gbs": spmv["achieved"],
         "roofline": roof,
         "spmv_fp64_csr": spmv,
         "mgs_step": mgs,
+        "assembly_cellwise": asm,
         "kernels_fine_level": table,
         "cpu_baseline": cpu,
     }

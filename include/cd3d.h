@@ -185,7 +185,12 @@ cd3d_status cd3d_interp_pulse(cd3d_problem *p, int level, const cd3d_coeffs *co,
  * 5 CSR values from the class stencils (10 bytes per active-row nonzero); 6 copy of an
  * FP64 level vector (16 N: the traffic-counter calibration); 7 one fused
  * Gram-Schmidt step of FGMRES, w -= h v_k and the partial sums of <w, v_k+1>
- * (32 N: v_k, w, v_k+1 read, w written).
+ * (32 N: v_k, w, v_k+1 read, w written); 8 the cell-by-cell element kernel
+ * (BilinearAssemble / BilinearAssemble_SD per quadrature point, colour-scattered
+ * into CSR) over all cells of the level, with the coefficients of the last
+ * assembly, into a scratch value array: *bytes_per_launch then holds the
+ * algorithmic FP64 flops of one pass, n_cells n_q 8 n_b^2 (a "launch" is the 8
+ * colour launches).
  * Needs an assembled problem; the level vectors are overwritten. */
 cd3d_status cd3d_kernel_bench(cd3d_problem *p, int which, int level, int reps, double *ms_per_launch,
                               double *bytes_per_launch);

@@ -284,16 +284,9 @@ static void check_coeffs(const cd3d_coeffs &co) {
   if (co.f_kind != 0 && co.f_kind != 1) throw ArgError("f_kind must be 0 or 1");
 }
 
-// assemble level l: what 0 -> conv-diff into `val`, 1 -> mass into `val`;
-// rhs into D.rhs_asm (TAssembleMat3D::Reset + Assemble3D, AssembleMat3D.C:204,230).
-// Returns which path ran: 1 constant coefficients (it leaves the FP32 compact
-// stencils of `val` behind), 2 rotating field (affine class stencils in
-// lst64), 0 cell by cell.
-// csr false: a level assembled through affine class stencils gets the
-// stencils (and rhs) only, not its CSR values
-static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int what, double *val, bool csr = true) {
-  if (val == P.dl[l].val) P.dl[l].invdiag_ok = false;
-  DevLevel &D = P.dl[l];
+// kernel parameters of the element integration on level D (coefficients,
+// cell sizes, the SUPG delta of a constant convection field)
+static AsmParams asm_params(const cd3d_problem &P, const DevLevel &D, const cd3d_coeffs &co, int what) {
   AsmParams ap{};
   ap.what = what;
   ap.eps = co.eps;
@@ -314,6 +307,31 @@ static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int wha
   ap.delta_const = 0;
   if (co.supg && co.conv_kind == 0)
     ap.delta_const = sdfem_delta_host(hconv_host(D.h, co.b), co.eps, co.b, P.mesh.order);
+  return ap;
+}
+
+static void ensure_slots(cd3d_problem &P, int l) {
+  DevLevel &D = P.dl[l];
+  if (D.slots) return;  // slot maps only for the cell-by-cell path (nb^2 ints per cell)
+  std::vector<int> slots;
+  build_slots(P.hl[l], P.mesh.order, slots);
+  D.slots = dalloc<int>(P, slots.size());
+  upload(P, D.slots, slots);
+  HIPCHK(hipStreamSynchronize(P.stream));
+}
+
+// assemble level l: what 0 -> conv-diff into `val`, 1 -> mass into `val`;
+// rhs into D.rhs_asm (TAssembleMat3D::Reset + Assemble3D, AssembleMat3D.C:204,230).
+// Returns which path ran: 1 constant coefficients (it leaves the FP32 compact
+// stencils of `val` behind), 2 rotating field (affine class stencils in
+// lst64), 0 cell by cell.
+// csr false: a level assembled through affine class stencils gets the
+// stencils (and rhs) only, not its CSR values
+static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int what, double *val, bool csr = true) {
+  if (val == P.dl[l].val) P.dl[l].invdiag_ok = false;
+  DevLevel &D = P.dl[l];
+  const AsmParams ap = asm_params(P, D, co, what);
+  P.last_co = co;
   if ((co.conv_kind == 0 || what == 1) && co.f_kind == 0 && D.nclass > 0 && !uniform_assembly_off()) {
     // every active row is written: no clearing pass
     kl_assemble_uniform(P.stream, D, P.mesh.order, P.tab, ap, val, D.rhs_asm);
@@ -327,13 +345,7 @@ static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int wha
     HIPCHK(hipGetLastError());
     return 2;
   }
-  if (!D.slots) {  // slot maps only for the cell-by-cell path (nb^2 ints per cell)
-    std::vector<int> slots;
-    build_slots(P.hl[l], P.mesh.order, slots);
-    D.slots = dalloc<int>(P, slots.size());
-    upload(P, D.slots, slots);
-    HIPCHK(hipStreamSynchronize(P.stream));
-  }
+  ensure_slots(P, l);
   HIPCHK(hipMemsetAsync(val, 0, sizeof(double) * D.nnz, P.stream));
   HIPCHK(hipMemsetAsync(D.rhs_asm, 0, sizeof(double) * D.ndof, P.stream));
   for (int k = 0; k < 8; k++)
@@ -1398,7 +1410,7 @@ cd3d_status cd3d_device_ptr(cd3d_problem *p, int level, int which, double **dev)
 }
 
 cd3d_status cd3d_kernel_bench(cd3d_problem *p, int which, int level, int reps, double *ms, double *bytes) {
-  if (!p || !ms || !bytes || reps < 1 || level < 0 || level >= (int)p->dl.size() || which < 0 || which > 7)
+  if (!p || !ms || !bytes || reps < 1 || level < 0 || level >= (int)p->dl.size() || which < 0 || which > 8)
     return CD3D_ERR_ARG;
   std::lock_guard<std::mutex> g(p->mu);
   return run(p, [&]() {
@@ -1459,6 +1471,24 @@ cd3d_status cd3d_kernel_bench(cd3d_problem *p, int which, int level, int reps, d
         b = 32 * N;
         break;
     }
+    double *scratch_val = nullptr;
+    if (which == 8) {
+      // the cell-by-cell element kernel over all 8 colours of the level with
+      // the coefficients of the last assembly, into a scratch value array
+      // (the level's matrix stays as it is); FP64 flops per (cell, q, i, j): 8
+      ensure_slots(*p, level);
+      scratch_val = dalloc<double>(*p, L.nnz);
+      double *rhs_scratch = L.tmp;
+      const AsmParams ap = asm_params(*p, L, p->last_co, 0);
+      HIPCHK(hipMemsetAsync(scratch_val, 0, sizeof(double) * L.nnz, p->stream));
+      launch = [&, ap, rhs_scratch] {
+        for (int k = 0; k < 8; k++)
+          k_assemble(p->stream, L, p->tab, ap, scratch_val, rhs_scratch, L.colour_start[k],
+                     L.colour_start[k + 1] - L.colour_start[k]);
+      };
+      const double nq = std::pow((double)quad_points_1d(o), 3), nbd = p->nb;
+      b = (double)L.ncell * nq * 8.0 * nbd * nbd;
+    }
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -1472,6 +1502,7 @@ cd3d_status cd3d_kernel_bench(cd3d_problem *p, int which, int level, int reps, d
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     HIPCHK(hipGetLastError());
+    if (scratch_val) dfree(*p, scratch_val);
     *ms = t / reps;
     *bytes = b;
   });

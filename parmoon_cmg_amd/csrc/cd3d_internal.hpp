@@ -180,6 +180,8 @@ struct Problem {
   bool tcd_ready = false;
   // device tables for assembly (reference basis at quadrature points)
   double *tab = nullptr;
+  // coefficients of the last assembly (the kernel bench re-runs its element kernel)
+  cd3d_coeffs last_co{};
 };
 
 // ---------------------------------------------------------------- setup (setup.cpp)

@@ -476,20 +476,24 @@ def test_rotating_field_assembly_matches_oracle(order, m, k0):
 
 def test_kernel_bench_covers_every_kernel():
     # every case of cd3d_kernel_bench runs on an assembled Q2 problem (the
-    # timings bench.py reports), with the stated algorithmic bytes per launch,
-    # and an out-of-range case is refused
+    # timings bench.py reports), with the stated algorithmic bytes (flops for
+    # the element kernel) per launch, and an out-of-range case is refused
     P = pm.Problem(2, 1, 1, 3)
     P.assemble(pm.coeffs(eps=1e-3, b=(1.0, 0.5, 0.25), supg=1, f_val=1.0))
     l = 2
     i = P.info(l)
     N, nnz = i["ndof"], i["nnz"]
+    rp, kc, e0 = P.csr(l)
     nbytes = {}
-    for which in range(8):
+    for which in range(9):
         ms, b = P.kernel_bench(which, l, 3)
         assert ms > 0 and b > 0
         nbytes[which] = b
-    say(f"[gpu] kernel bench bytes {nbytes}")
+    say(f"[gpu] kernel bench bytes/flops {nbytes}")
     assert nbytes[0] == 12 * nnz + 20 * N
     assert nbytes[2] == 16 * N and nbytes[6] == 16 * N and nbytes[7] == 32 * N
+    assert nbytes[8] == i["ncell"] * 64 * 8 * 27 * 27
+    # the element kernel's timing leaves the level's matrix as it was
+    assert np.array_equal(P.csr(l)[2], e0)
     with pytest.raises(pm.CD3DError):
-        P.kernel_bench(8, l, 1)
+        P.kernel_bench(9, l, 1)
