@@ -133,6 +133,13 @@ cd3d_status cd3d_assemble(cd3d_problem *p, const cd3d_coeffs *co);
 cd3d_status cd3d_matvec(cd3d_problem *p, int level, const double *x_host, double *y_host);
 /* same on device buffers, on the handle's stream (asynchronous) */
 cd3d_status cd3d_matvec_dev(cd3d_problem *p, int level, const double *x_dev, double *y_dev);
+/* cd3d_matvec without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  x and y are staged through a 64 MiB array
+ * that the library holds on each device for the life of the process, so the
+ * level may have at most 4 194 304 unknowns (CD3D_ERR_ARG beyond that).
+ * Calls are serialised process-wide (they share the array). */
+cd3d_status cd3d_matvec_noalloc(cd3d_problem *p, int level, const double *x_host, double *y_host);
 /* Prolongate / DefectRestriction between level l-1 and l (MGComponents3D.C:182,706) */
 cd3d_status cd3d_prolongate(cd3d_problem *p, int level, const double *coarse_host, double *fine_host);
 cd3d_status cd3d_restrict(cd3d_problem *p, int level, const double *fine_host, double *coarse_host);

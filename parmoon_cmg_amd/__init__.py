@@ -23,7 +23,8 @@ FP64, MIXED = 0, 1
 # every entry point declared in include/cd3d.h
 EXPORTS = ["cd3d_default_solver", "cd3d_create", "cd3d_destroy", "cd3d_last_error", "cd3d_device_bytes",
            "cd3d_level_info", "cd3d_get_dofmap", "cd3d_get_cells", "cd3d_get_csr", "cd3d_get_rhs",
-           "cd3d_set_entries", "cd3d_assemble", "cd3d_matvec", "cd3d_matvec_dev", "cd3d_prolongate",
+           "cd3d_set_entries", "cd3d_assemble", "cd3d_matvec", "cd3d_matvec_dev", "cd3d_matvec_noalloc",
+           "cd3d_prolongate",
            "cd3d_restrict", "cd3d_vcycle", "cd3d_solve", "cd3d_get_solution", "cd3d_tcd_run",
            "cd3d_tcd_init", "cd3d_tcd_step", "cd3d_tcd_get_state",
            "cd3d_interp_pulse", "cd3d_kernel_bench", "cd3d_device_ptr", "cd3d_stream", "cd3d_synchronize",
@@ -119,6 +120,7 @@ def lib():
     L.cd3d_set_entries.argtypes = [P, I, dp]
     L.cd3d_assemble.argtypes = [P, C.POINTER(Coeffs)]
     L.cd3d_matvec.argtypes = [P, I, dp, dp]
+    L.cd3d_matvec_noalloc.argtypes = [P, I, dp, dp]
     L.cd3d_matvec_dev.argtypes = [P, I, C.c_void_p, C.c_void_p]
     L.cd3d_prolongate.argtypes = [P, I, dp, dp]
     L.cd3d_restrict.argtypes = [P, I, dp, dp]
@@ -246,6 +248,14 @@ class Problem:
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros_like(x)
         self._chk(self.L.cd3d_matvec(self.h, l, _dp(x), _dp(y)), "matvec")
+        return y
+
+    def matvec_noalloc(self, l, x):
+        """y = A x without device allocation (staged through the library's
+        per-device array; at most 4 194 304 unknowns)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros_like(x)
+        self._chk(self.L.cd3d_matvec_noalloc(self.h, l, _dp(x), _dp(y)), "matvec_noalloc")
         return y
 
     def prolongate(self, l, coarse):

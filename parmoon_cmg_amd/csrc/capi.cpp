@@ -1044,6 +1044,38 @@ cd3d_status cd3d_matvec(cd3d_problem *p, int level, const double *x, double *y) 
   });
 }
 
+}  // extern "C"
+
+int cd3d_internal_device(const cd3d_problem *p) { return p ? p->device : -1; }
+
+// cd3d_matvec staged through a device buffer the caller holds (noalloc.hip):
+// no allocation; nothing in the buffer needs clearing (x is copied in whole,
+// every row of y is written)
+cd3d_status cd3d_internal_matvec_staged(cd3d_problem *p, int level, const double *x, double *y, double *base,
+                                        size_t capacity) {
+  if (!p || !x || !y || !base || level < 0 || level >= (int)p->dl.size()) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  return run(p, [&]() {
+    const DevLevel &L = p->dl[level];
+    const size_t n = (size_t)L.ndof, ld = (n + 31) / 32 * 32;
+    if (2 * ld > capacity) {
+      char b[200];
+      snprintf(b, sizeof b,
+               "cd3d_matvec_noalloc: level %d has %zu unknowns; the staging array holds x and y of at most %zu",
+               level, n, capacity / 2);
+      throw ArgError(b);
+    }
+    double *xd = base, *yd = base + ld;
+    HIPCHK(hipMemcpyAsync(xd, x, sizeof(double) * n, hipMemcpyHostToDevice, p->stream));
+    k_spmv(p->stream, L, xd, yd);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(y, yd, sizeof(double) * n, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+  });
+}
+
+extern "C" {
+
 cd3d_status cd3d_matvec_dev(cd3d_problem *p, int level, const double *x, double *y) {
   if (!p || !x || !y || level < 0 || level >= (int)p->dl.size()) return CD3D_ERR_ARG;
   return run(p, [&]() {
