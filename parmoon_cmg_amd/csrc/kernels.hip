@@ -672,18 +672,28 @@ __device__ inline double hconv_brick(const double *h, double b0, double b1, doub
   return 2 * nb / den;
 }
 
-// One thread per (cell, local row i); NB rows of a cell are consecutive
-// threads.  Per quadrature point the thread forms, in the cell's reference
-// frame, u_i = kappa .* grad phi_i (diffusion), t_i = phi_i + delta beta.grad
-// phi_i (test function with SUPG term) and s_i = beta.grad phi_i + c phi_i
-// (convection-reaction applied to the ansatz function), shares s_i through
-// LDS and accumulates row i of the local matrix in registers:
-//   A_ij += w |J| (u_i . grad phi_j + t_i s_j)
-// (BilinearAssemble_SD / BilinearAssemble, ConvDiff3D.C:61-190, with the
-// physical gradients J^-T grad phi of THexaAffin folded into kappa and beta:
-// the cell frames are signed axis permutations).  Then the row is added into
-// the CSR values through the slot map; cells of one colour share no DOF.
-template <int P>
+// One thread per (cell, local row i); the NB rows of a cell are consecutive
+// threads of a flat index (cells may straddle wavefronts: threads share
+// nothing).  Per quadrature point the thread evaluates the coefficients at
+// the point and forms, in the cell's reference frame,
+//   t_i = phi_i + delta beta.grad phi_i     (test function with the SUPG term)
+//   a_i = w (kappa .* grad phi_i + t_i beta),  c_i = w t_i c
+// so that row i of the local matrix of BilinearAssemble_SD / BilinearAssemble
+// (ConvDiff3D.C:61-190),
+//   A_ij += w (eps grad phi_i . grad phi_j + t_i (b.grad phi_j + c phi_j)),
+// becomes A_ij += a_i . grad phi_j + c_i phi_j: four FMAs per (q, i, j) whose
+// j-operands are the same for every thread (wave-uniform table reads), with
+// no exchange between the threads of a cell.  kappa and beta fold in the
+// physical gradients J^-T grad phi of THexaAffin (the cell frames are signed
+// axis permutations of the brick).  The mass matrix (what 1) is
+// A_ij += (w phi_i) phi_j.  Then the row is added into the CSR values through
+// the slot map (cells of one colour share no DOF).
+// LDS_TAB: the per-thread operands (phi_i and its gradient at every q) are
+// read from an LDS copy of the basis table made at the start of the block;
+// otherwise from the table in global memory.  VAR: coefficients that vary
+// over the cell (rotating convection, manufactured source); without it the
+// kernel carries none of their code (and none of its registers).
+template <int P, bool LDS_TAB, bool VAR>
 __global__ __launch_bounds__(kBlock) void assemble_k(int ncells, const int *__restrict__ list,
                                                      const int4 *__restrict__ cells, const int *__restrict__ slots,
                                                      const int *__restrict__ gn, int nact,
@@ -693,22 +703,38 @@ __global__ __launch_bounds__(kBlock) void assemble_k(int ncells, const int *__re
   constexpr int N1 = P + 2;
   constexpr int NQ = N1 * N1 * N1;
   constexpr int STRIDE = 4 + 4 * NB;
-  constexpr int CPB = kBlock / NB;
-  __shared__ double s_sh[CPB][NB];
-  const int t = threadIdx.x, lc0 = t / NB, i = t % NB;
-  const int lc = lc0 < CPB ? lc0 : 0;  // tail threads of a Q2 block (t >= 243) work on cell slot 0, write nothing
-  const int ci = blockIdx.x * CPB + lc0;
-  const bool live = lc0 < CPB && ci < ncells;
-  const int c = live ? list[ci] : 0;
-  int4 cg = live ? cells[c] : make_int4(0, 0, 0, 0);
+  __shared__ double s_tab[LDS_TAB ? NQ * NB * 4 : 1];
+  if (LDS_TAB) {
+    for (int k = threadIdx.x; k < NQ * NB * 4; k += kBlock) s_tab[k] = tab[(k / (4 * NB)) * STRIDE + 4 + k % (4 * NB)];
+    __syncthreads();
+  }
+  const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (gid >= (int64_t)ncells * NB) return;
+  const int ci = (int)(gid / NB), i = (int)(gid % NB);
+  const int c = list[ci];
+  const int4 cg = cells[c];
   int dax[3], sg[3];
   decode_frame(cg.w, dax, sg);
   const int O[3] = {cg.x, cg.y, cg.z};
   const double detj = (ap.h[0] / 2) * (ap.h[1] / 2) * (ap.h[2] / 2);
-  double kap[3];
+  // per local axis a: 2 / h along the physical axis dax[a] (J^-T of the brick)
+  double ih[3], kap[3];
 #pragma unroll
-  for (int a = 0; a < 3; a++) kap[a] = ap.eps * 4.0 / (ap.h[dax[a]] * ap.h[dax[a]]);
+  for (int a = 0; a < 3; a++) {
+    const double h = dax[a] == 0 ? ap.h[0] : dax[a] == 1 ? ap.h[1] : ap.h[2];
+    ih[a] = 2.0 / h;
+    kap[a] = ap.eps * 4.0 / (h * h);
+  }
   const double nu = 1.0 / P;
+  // constant convection: beta and delta are the same at every point
+  double beta[3] = {0, 0, 0};
+  if (ap.conv_kind == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const double bd = dax[a] == 0 ? ap.b[0] : dax[a] == 1 ? ap.b[1] : ap.b[2];
+      beta[a] = sg[a] * bd * ih[a];
+    }
+  }
   double acc[NB];
 #pragma unroll
   for (int j = 0; j < NB; j++) acc[j] = 0;
@@ -716,64 +742,60 @@ __global__ __launch_bounds__(kBlock) void assemble_k(int ncells, const int *__re
   double hconv = -1;
   for (int q = 0; q < NQ; q++) {
     const double *T = tab + q * STRIDE;
+    const double *Ti = LDS_TAB ? s_tab + (q * NB + i) * 4 : T + 4 + 4 * i;
     const double w = T[0] * detj;
-    const double ref[3] = {T[1], T[2], T[3]};
+    const double phi_i = Ti[0], g0 = Ti[1], g1 = Ti[2], g2 = Ti[3];
     // physical quadrature point (only variable coefficients need it)
     double X[3] = {0, 0, 0};
-    if (ap.conv_kind == 1 || ap.f_kind == 1) {
+    if (VAR && (ap.conv_kind == 1 || ap.f_kind == 1)) {
 #pragma unroll
-      for (int a = 0; a < 3; a++) X[dax[a]] = ap.dn[dax[a]] * (O[dax[a]] + P * sg[a] * (ref[a] + 1) * 0.5);
+      for (int a = 0; a < 3; a++) {
+        const double xa = ap.dn[dax[a]] * (O[dax[a]] + P * sg[a] * (T[1 + a] + 1) * 0.5);
+        X[0] = dax[a] == 0 ? xa : X[0];
+        X[1] = dax[a] == 1 ? xa : X[1];
+        X[2] = dax[a] == 2 ? xa : X[2];
+      }
+    }
+    double f = ap.f_val;
+    if (ap.what == 1) {
+      if (VAR && ap.f_kind == 1)
+        f = ap.eps * 3 * M_PI * M_PI * sin(M_PI * X[0]) * sin(M_PI * X[1]) * sin(M_PI * X[2]);
+      const double bw = w * phi_i;
+#pragma unroll
+      for (int j = 0; j < NB; j++) acc[j] = fma(bw, T[4 + 4 * j], acc[j]);
+      racc += w * phi_i * f;
+      continue;
     }
     double b0 = ap.b[0], b1 = ap.b[1], b2 = ap.b[2];
-    if (ap.conv_kind == 1) {
+    double delta = ap.supg ? ap.delta_const : 0.0;
+    if (VAR && ap.conv_kind == 1) {
       b0 = -(X[1] - ap.rot_c[1]);
       b1 = X[0] - ap.rot_c[0];
       b2 = 0.1 * ap.sin2pit;
+#pragma unroll
+      for (int a = 0; a < 3; a++) beta[a] = sg[a] * (dax[a] == 0 ? b0 : dax[a] == 1 ? b1 : b2) * ih[a];
+      if (ap.supg) {
+        if (hconv < 0) hconv = hconv_brick(ap.h, b0, b1, b2);
+        delta = sdfem_delta(hconv, ap.eps, sqrt(b0 * b0 + b1 * b1 + b2 * b2), nu);
+      }
     }
-    double f = ap.f_val;
-    if (ap.f_kind == 1) {
+    if (VAR && ap.f_kind == 1) {
       const double sx = sin(M_PI * X[0]), sy = sin(M_PI * X[1]), sz = sin(M_PI * X[2]);
       const double cx = cos(M_PI * X[0]), cy = cos(M_PI * X[1]), cz = cos(M_PI * X[2]), u = sx * sy * sz;
       f = ap.eps * 3 * M_PI * M_PI * u + b0 * M_PI * cx * sy * sz + b1 * M_PI * sx * cy * sz +
           b2 * M_PI * sx * sy * cz + ap.c * u;
     }
-    double delta = 0;
-    if (ap.supg && ap.what == 0) {
-      if (ap.conv_kind == 0) {
-        delta = ap.delta_const;
-      } else {
-        if (hconv < 0) hconv = hconv_brick(ap.h, b0, b1, b2);
-        delta = sdfem_delta(hconv, ap.eps, sqrt(b0 * b0 + b1 * b1 + b2 * b2), nu);
-      }
-    }
-    const double bphys[3] = {b0, b1, b2};
-    double beta[3];
-#pragma unroll
-    for (int a = 0; a < 3; a++) beta[a] = sg[a] * bphys[dax[a]] * 2.0 / ap.h[dax[a]];
-    const double phi_i = T[4 + 4 * i], g0 = T[5 + 4 * i], g1 = T[6 + 4 * i], g2 = T[7 + 4 * i];
     const double bg = beta[0] * g0 + beta[1] * g1 + beta[2] * g2;
-    if (ap.what == 1) {
-      __syncthreads();
-      if (live) s_sh[lc][i] = phi_i;
-      __syncthreads();
-#pragma unroll
-      for (int j = 0; j < NB; j++) acc[j] += w * (phi_i * s_sh[lc][j]);
-      racc += w * phi_i * f;
-      continue;
-    }
-    const double u0 = kap[0] * g0, u1 = kap[1] * g1, u2 = kap[2] * g2;
     const double ti = phi_i + delta * bg;
-    __syncthreads();
-    if (live) s_sh[lc][i] = bg + ap.c * phi_i;
-    __syncthreads();
+    const double a0 = w * (kap[0] * g0 + ti * beta[0]);
+    const double a1 = w * (kap[1] * g1 + ti * beta[1]);
+    const double a2 = w * (kap[2] * g2 + ti * beta[2]);
+    const double cc = w * (ti * ap.c);
 #pragma unroll
-    for (int j = 0; j < NB; j++) {
-      const double v = u0 * T[5 + 4 * j] + u1 * T[6 + 4 * j] + u2 * T[7 + 4 * j] + ti * s_sh[lc][j];
-      acc[j] += w * v;
-    }
+    for (int j = 0; j < NB; j++)
+      acc[j] = fma(a0, T[5 + 4 * j], fma(a1, T[6 + 4 * j], fma(a2, T[7 + 4 * j], fma(cc, T[4 + 4 * j], acc[j]))));
     racc += w * f * ti;
   }
-  if (!live) return;
   const int r = gn[c * NB + i];
   if (r >= nact) return;
   const int *sl = slots + ((int64_t)c * NB + i) * NB;
@@ -1274,15 +1296,27 @@ void k_assemble(hipStream_t s, const DevLevel &L, const double *tab, const AsmPa
                 int colour_begin, int colour_count) {
   if (colour_count <= 0) return;
   const int *list = L.colour_cells + colour_begin;
+  // CD3D_ASM_LDS_TAB=1: per-thread table operands from an LDS copy of the
+  // table (measured on the 64^3 Q2 level, constant SUPG coefficients: 9.3 ms
+  // per assembly against 6.7 ms from global memory: the 55 KB copy limits a
+  // CU to two workgroups and costs a staging pass per workgroup)
+  static const int lds_tab = [] {
+    const char *e = getenv("CD3D_ASM_LDS_TAB");
+    return e ? atoi(e) : 0;
+  }();
+  const bool var = ap.conv_kind != 0 || ap.f_kind != 0;
+  const int64_t threads = (int64_t)colour_count * (ap.order == 1 ? 8 : 27);
+  const int grid = (int)((threads + kBlock - 1) / kBlock);
+#define CD3D_ASM(P_, LDS_, VAR_) \
+  assemble_k<P_, LDS_, VAR_><<<grid, kBlock, 0, s>>>(colour_count, list, L.cells, L.slots, L.gn, L.nact, tab, ap, val, rhs)
   if (ap.order == 1) {
-    constexpr int cpb = kBlock / 8;
-    assemble_k<1><<<(colour_count + cpb - 1) / cpb, kBlock, 0, s>>>(colour_count, list, L.cells, L.slots, L.gn,
-                                                                      L.nact, tab, ap, val, rhs);
+    if (lds_tab) { if (var) CD3D_ASM(1, true, true); else CD3D_ASM(1, true, false); }
+    else { if (var) CD3D_ASM(1, false, true); else CD3D_ASM(1, false, false); }
   } else {
-    constexpr int cpb = kBlock / 27;
-    assemble_k<2><<<(colour_count + cpb - 1) / cpb, kBlock, 0, s>>>(colour_count, list, L.cells, L.slots, L.gn,
-                                                                      L.nact, tab, ap, val, rhs);
+    if (lds_tab) { if (var) CD3D_ASM(2, true, true); else CD3D_ASM(2, true, false); }
+    else { if (var) CD3D_ASM(2, false, true); else CD3D_ASM(2, false, false); }
   }
+#undef CD3D_ASM
 }
 // blocked elimination (panels of NB pivots); returns the buffer holding the result
 template <int NB, int RPT>
