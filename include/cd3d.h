@@ -168,6 +168,9 @@ cd3d_status cd3d_tcd_run(cd3d_problem *p, const cd3d_coeffs *co, const cd3d_solv
  * (TimeDiscRout.C; theta 0.5 Crank-Nicolson, 1 backward Euler).
  * cd3d_tcd_init: TSystemTCD3D::AssembleMRhs at t0 (SystemTCD3D.C:176-202)
  * and the initial state u_host (finest level; Dirichlet values from the rhs).
+ * co->supg: the SUPG forms of TCD3D.C:48,102 -- the rhs with SUPG test
+ * functions and, per step, the time-derivative matrix K beside M (system
+ * M + K + tau theta A, rhs matrix M + K - tau (1 - theta) A).
  * cd3d_tcd_step: one step t -> t + tau: AssembleARhs at t + tau,
  * AssembleSystMat, Solve, RestoreMassMat (SystemTCD3D.C:204-376, driven as
  * in TCD3D_ParMooN.C:569-622); u_host (optional) receives the new state.

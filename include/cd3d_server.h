@@ -81,7 +81,8 @@ void cd3d_server_default_params(cd3d_server_params *p);
 /* device bytes one problem of this mesh needs, as the admission control
  * counts them: hierarchy, lattice form, Krylov space of the given restart;
  * cellwise_assembly: variable coefficients (slot maps, per-node FP32
- * stencils); tcd: the TCD3D matrices and vectors */
+ * stencils); tcd: the TCD3D matrices and vectors (1 Galerkin, 2 SUPG: also
+ * the time-derivative matrix K) */
 size_t cd3d_estimate_bytes(const cd3d_mesh_desc *mesh, int restart, int cellwise_assembly, int tcd);
 
 cd3d_status cd3d_server_create(const cd3d_server_params *p, cd3d_server **out);

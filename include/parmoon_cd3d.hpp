@@ -579,17 +579,25 @@ class TSystemCD3D {
 
 // TSystemTCD3D (SystemTCD3D.C:22-408): theta-scheme with the mass matrix in
 // the level matrices; the state lives on the device between steps.
+// DISCTYPE GALERKIN as in the reference; SDFEM/SUPG runs the SUPG forms of
+// TCD3D.C:48,102 with the time-derivative matrix K beside M (the SUPG case
+// that SystemTCD3D.C:42-51,114-131 leaves commented out).
 class TSystemTCD3D {
  public:
   TSystemTCD3D(int N_levels, TFESpace3D **fespaces, double **sol, double **rhs, int disctype, int solver)
       : n_levels_(N_levels), spaces_(fespaces, fespaces + N_levels), sol_(sol, sol + N_levels),
-        rhs_(rhs, rhs + N_levels) {
+        rhs_(rhs, rhs + N_levels), supg_(disctype == SDFEM || disctype == SUPG) {
     if (solver != GMG) Fail("Unknown Solver (the device path runs GMG)");
-    if (disctype != GALERKIN) Fail("Unknown DISCTYPE (TSystemTCD3D is Galerkin only, SystemTCD3D.C:114-131)");
+    if (disctype != GALERKIN && !supg_) Fail("Unknown or not yet implemented DISCTYPE");
     if (N_levels != spaces_[0]->Hierarchy()->levels())
       Fail("TSystemTCD3D: N_levels must equal the number of levels of the hierarchy");
   }
-  void Init(const cd3d_coeffs &coeffs) { co_ = coeffs; co_.supg = 0; }
+  void Init(const cd3d_coeffs &coeffs) { co_ = coeffs; co_.supg = supg_ ? 1 : 0; }
+  // stop every step's FGMRES at rel times that step's initial residual
+  // instead of the reference's restart-relative SC_LIN_RED_FACTOR_SCALAR
+  // test (which, once a step needs a second restart cycle, demands the
+  // reduction again from that cycle's start); 0: the reference's rule
+  void SetStepRelativeTolerance(double rel) { step_rel_tol_ = rel; }
   // AssembleMRhs (SystemTCD3D.C:176-202) at the current time with the
   // initial state in sol[N_levels-1]
   void AssembleMRhs(double t0) {
@@ -601,7 +609,8 @@ class TSystemTCD3D {
   // length tau (TCD3D_ParMooN.C:569-622); theta from TIME_DISC
   int Step(double tau) {
     cd3d_problem *h = handle();
-    const cd3d_solver_params sp = SolverParamsFromDB();
+    cd3d_solver_params sp = SolverParamsFromDB();
+    if (step_rel_tol_ > 0) sp.rel_tol = step_rel_tol_;
     const int td = TDatabase::ParamDB->TIME_DISC;
     if (td != 1 && td != 2) Fail("TIME_DISC: the device path runs backward Euler (1) or Crank-Nicolson (2)");
     cd3d_stats st{};
@@ -615,6 +624,8 @@ class TSystemTCD3D {
   int n_levels_;
   std::vector<TFESpace3D *> spaces_;
   std::vector<double *> sol_, rhs_;
+  bool supg_ = false;
+  double step_rel_tol_ = 0.0;
   cd3d_coeffs co_{};
 };
 

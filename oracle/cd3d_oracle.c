@@ -29,6 +29,7 @@ typedef struct {
   long nnz;
   double *A;         /* matrix used by the multigrid level (A, or the TCD system M) */
   double *Aconv;     /* TCD: conv-diff matrix */
+  double *Ksupg;     /* TCD SUPG: time-derivative SUPG matrix K */
   double *rhs, *sol; /* level vectors (TMGLevel3D Rhs / X) */
   double *aux0, *aux1;
   double *mult;      /* number of cells sharing each DOF (Prolongate aux) */
@@ -256,7 +257,7 @@ void orc_destroy(orc_hier *h) {
   for (int l = 0; l < h->L; l++) {
     level_t *L = h->lv + l;
     free(L->org); free(L->ax); free(L->gn); free(L->dof_node); free(L->rowptr); free(L->kcol);
-    free(L->A); free(L->Aconv); free(L->rhs); free(L->sol); free(L->aux0); free(L->aux1); free(L->mult);
+    free(L->A); free(L->Aconv); free(L->Ksupg); free(L->rhs); free(L->sol); free(L->aux0); free(L->aux1); free(L->mult);
   }
   free(h->lv); free(h->lu); free(h->piv);
   free(h);
@@ -355,9 +356,17 @@ double orc_sdfem_delta(double hconv, double eps, const double *b, int order) {
 /* Local matrices of one cell (TFEDatabase3D::GetOrig + THexaAffin,
  * TDiscreteForm3D::GetLocalForms DiscreteForm3D.C:187-270, kernels
  * BilinearAssemble / BilinearAssemble_SD ConvDiff3D.C:61-190,
- * MatrixMRhsAssemble TCD3D.C:11-46).  what: 0 conv-diff, 1 mass. */
+ * MatrixMRhsAssemble TCD3D.C:11-46, MatrixMRhsAssemble_SUPG TCD3D.C:48-100,
+ * MatricesAKRhsAssemble_SUPG TCD3D.C:102-240).
+ * what: 0 conv-diff A (SUPG test functions when co->supg), 1 mass M (rhs with
+ * the SUPG test functions when co->supg), 2 the SUPG time-derivative matrix
+ * K_ij = sum w|J| phi_j tau b.grad phi_i (no rhs).
+ * tscale = theta1 * tau > 0: the time-dependent SUPG parameter of
+ * MatricesAKRhsAssemble_SUPG -- Compute_SDFEM_delta of the scaled
+ * coefficients (tscale eps, tscale b), multiplied by tscale -- for what 0 and
+ * 2; tscale 0: the steady parameter of BilinearAssemble_SD. */
 static void local_forms(const orc_hier *h, const level_t *L, int c, const orc_coeffs *co, int what,
-                        double *Aloc, double *bloc) {
+                        double tscale, double *Aloc, double *bloc) {
   int p = h->order, nb = h->nb, n1 = p + 2; /* Gauss3 for Q1, Gauss4 for Q2 */
   const double *gx = p == 1 ? G3X : G4X, *gw = p == 1 ? G3W : G4W;
   double hd[3], dn[3];
@@ -386,17 +395,30 @@ static void local_forms(const orc_hier *h, const level_t *L, int c, const orc_co
         }
         double cf[7];
         coeff_at(co, X[0], X[1], X[2], cf);
+        double delta = 0;
+        if (co->supg && what != 1 && tscale > 0) {
+          /* TCD3D.C:141-156: tilde tau of (tscale eps, tscale b), then tau *= tscale */
+          double sb[3] = {tscale * cf[1], tscale * cf[2], tscale * cf[3]};
+          if (hconv < 0) hconv = orc_hconv_brick(hd, sb); /* cached per cell: AssembleMat3D.C:457 */
+          delta = orc_sdfem_delta(hconv, tscale * cf[0], sb, p) * tscale;
+        } else if (co->supg) {
+          if (hconv < 0) hconv = orc_hconv_brick(hd, cf + 1); /* cached per cell: AssembleMat3D.C:457 */
+          delta = orc_sdfem_delta(hconv, cf[0], cf + 1, p);
+        }
         if (what == 1) {
           for (int i = 0; i < nb; i++) {
-            bloc[i] += Mult * phi[i] * cf[5];
+            double bgradv = cf[1] * gx_[i] + cf[2] * gy_[i] + cf[3] * gz_[i];
+            bloc[i] += Mult * (co->supg ? phi[i] + delta * bgradv : phi[i]) * cf[5];
             for (int j = 0; j < nb; j++) Aloc[i * nb + j] += Mult * phi[j] * phi[i];
           }
           continue;
         }
-        double delta = 0;
-        if (co->supg) {
-          if (hconv < 0) hconv = orc_hconv_brick(hd, cf + 1); /* cached per cell: AssembleMat3D.C:457 */
-          delta = orc_sdfem_delta(hconv, cf[0], cf + 1, p);
+        if (what == 2) {
+          for (int i = 0; i < nb; i++) {
+            double bgradv = delta * (cf[1] * gx_[i] + cf[2] * gy_[i] + cf[3] * gz_[i]);
+            for (int j = 0; j < nb; j++) Aloc[i * nb + j] += Mult * phi[j] * bgradv;
+          }
+          continue;
         }
         for (int i = 0; i < nb; i++) {
           double t100 = gx_[i], t010 = gy_[i], t001 = gz_[i], t000 = phi[i];
@@ -421,14 +443,15 @@ static void local_forms(const orc_hier *h, const level_t *L, int c, const orc_co
  * AddLocalSqMatToGlobal (:721-827: active rows accumulate, Dirichlet rows get
  * 1 on the diagonal) and AddLocalRhsToGlobal (:883-1066: active rows
  * accumulate, Dirichlet rows get g = 0). */
-void orc_assemble_level(const orc_hier *h, int l, const orc_coeffs *co, int what, double *entries, double *rhs) {
+static void assemble_level_t(const orc_hier *h, int l, const orc_coeffs *co, int what, double tscale,
+                             double *entries, double *rhs) {
   const level_t *L = h->lv + l;
   int nb = h->nb;
   double Aloc[MAXNB * MAXNB], bloc[MAXNB];
   memset(entries, 0, sizeof(double) * L->nnz);
   memset(rhs, 0, sizeof(double) * L->ndof);
   for (int c = 0; c < L->ncell; c++) {
-    local_forms(h, L, c, co, what, Aloc, bloc);
+    local_forms(h, L, c, co, what, tscale, Aloc, bloc);
     const int *g = L->gn + c * nb;
     for (int m = 0; m < nb; m++) {
       int r = g[m];
@@ -445,6 +468,10 @@ void orc_assemble_level(const orc_hier *h, int l, const orc_coeffs *co, int what
       }
     }
   }
+}
+
+void orc_assemble_level(const orc_hier *h, int l, const orc_coeffs *co, int what, double *entries, double *rhs) {
+  assemble_level_t(h, l, co, what, 0.0, entries, rhs);
 }
 
 /* TSystemCD3D::Assemble (SystemCD3D.C:314-339): every level, then the
@@ -826,7 +853,13 @@ static void matadd(level_t *L, double *C, const double *D, double alpha) {
 
 /* Crank-Nicolson loop of 3DPrograms/TCD3D_ParMooN.C:539-637 over
  * TSystemTCD3D (SystemTCD3D.C:176-376): AssembleMRhs once, then per step
- * AssembleARhs at t_new, AssembleSystMat, Solve, RestoreMassMat. */
+ * AssembleARhs at t_new, AssembleSystMat, Solve, RestoreMassMat.
+ * SUPG (co->supg): M and the SUPG rhs of MatrixMRhsAssemble_SUPG at t0; per
+ * step A, K and rhs of MatricesAKRhsAssemble_SUPG (TCD3D.C:48,102), and K
+ * beside M in the system and rhs matrices, added and removed where the SUPG
+ * branch of a reference time loop does it (Main_Users/Sashi/3DPrograms/
+ * TCD_3D.C:1184-1188,1314-1317): system M + K + tau th1 A, rhs matrix
+ * M + K - tau th2 A. */
 int orc_tcd_run(orc_hier *h, const orc_coeffs *co0, const orc_solver *sv, double t0, double tau, int nsteps,
                 double *u, double *states) {
   int Lf = h->L - 1, tot = 0;
@@ -836,6 +869,7 @@ int orc_tcd_run(orc_hier *h, const orc_coeffs *co0, const orc_solver *sv, double
   for (int l = 0; l < h->L; l++) {
     level_t *L = h->lv + l;
     if (!L->Aconv) L->Aconv = calloc(L->nnz, sizeof(double));
+    if (co.supg && !L->Ksupg) L->Ksupg = calloc(L->nnz, sizeof(double));
     M[l] = L->A;
     rhsl[l] = L->rhs;
   }
@@ -854,7 +888,12 @@ int orc_tcd_run(orc_hier *h, const orc_coeffs *co0, const orc_solver *sv, double
     co.t = t;
     for (int l = 0; l < h->L; l++) {
       level_t *L = h->lv + l;
-      orc_assemble_level(h, l, &co, 0, L->Aconv, rhsl[l]);
+      assemble_level_t(h, l, &co, 0, th1 * tau, L->Aconv, rhsl[l]);
+      if (co.supg) {
+        double *scratch = calloc(L->ndof, sizeof(double));
+        assemble_level_t(h, l, &co, 2, th1 * tau, L->Ksupg, scratch);
+        free(scratch);
+      }
     }
     memcpy(u + NA, F->rhs + NA, sizeof(double) * (N - NA));
     /* AssembleSystMat (SystemTCD3D.C:228-294) */
@@ -863,6 +902,8 @@ int orc_tcd_run(orc_hier *h, const orc_coeffs *co0, const orc_solver *sv, double
     for (int k = 0; k < NA; k++) B[k] += tau * th4 * F->rhs[k];
     matadd(F, M[Lf], F->Aconv, -tau * th2);
     double gamma = -tau * th2;
+    if (co.supg)
+      for (int l = 0; l < h->L; l++) matadd(h->lv + l, M[l], h->lv[l].Ksupg, 1.0);
     memset(def, 0, sizeof(double) * N);
     for (int i = 0; i < NA; i++) { /* MatVectActive (Blas2.C:102-129) */
       double s = 0;
@@ -885,7 +926,10 @@ int orc_tcd_run(orc_hier *h, const orc_coeffs *co0, const orc_solver *sv, double
     tot += its;
     memcpy(u, x, sizeof(double) * N);
     /* RestoreMassMat (SystemTCD3D.C:296-315) */
-    for (int l = 0; l < h->L; l++) matadd(h->lv + l, M[l], h->lv[l].Aconv, -gamma);
+    for (int l = 0; l < h->L; l++) {
+      matadd(h->lv + l, M[l], h->lv[l].Aconv, -gamma);
+      if (co.supg) matadd(h->lv + l, M[l], h->lv[l].Ksupg, -1.0);
+    }
     if (states) memcpy(states + (size_t)m * N, u, sizeof(double) * N);
   }
   free(oldrhs); free(oldsol); free(B); free(def); free(x); free(M); free(rhsl);

@@ -10,6 +10,16 @@
 // usage: cd3d_ref <file.dat> <outdir|-> [mode]
 //   mode "cd"  (default): assemble all levels, V-cycle probe, FGMRES solve
 //   mode "tcd": Crank-Nicolson steps of the TCD3D system (TSystemTCD3D)
+//   mode "tcdsupg": Crank-Nicolson steps of the SUPG-stabilised TCD3D system:
+//     the mass matrix and rhs of MatrixMRhsAssemble_SUPG (TCD3D.C:48) at t0,
+//     then per step A, K and rhs of MatricesAKRhsAssemble_SUPG (TCD3D.C:102)
+//     at t_new and the steps of TSystemTCD3D::AssembleSystMat / Solve /
+//     RestoreMassMat (SystemTCD3D.C:228-376) with the SUPG matrix K added to
+//     M wherever the SUPG branch of a reference time loop adds it
+//     (Main_Users/Sashi/3DPrograms/TCD_3D.C:1184-1188,1314-1317): system
+//     M + K + tau theta1 A, rhs tau theta3 f_old + tau theta4 f_new +
+//     (M + K - tau theta2 A) u_old; GMG-preconditioned FGMRES as in "cd",
+//     stopping at SC_LIN_RED_FACTOR_SCALAR times each step's initial residual
 //
 // Coefficients come from the .dat P-parameters:
 //   P0 eps, P1..P3 convection b (P7>0: rotating field
@@ -38,6 +48,9 @@
 // ConvDiff3D.h defines data objects; declare the two kernels used here (ConvDiff3D.C:61,124)
 void BilinearAssemble(double, double *, double, double **, int *, double ***, double **);
 void BilinearAssemble_SD(double, double *, double, double **, int *, double ***, double **);
+// TCD3D SUPG kernels (TCD3D.C:48,102)
+void MatrixMRhsAssemble_SUPG(double, double *, double *, double, double **, int *, double ***, double **);
+void MatricesAKRhsAssemble_SUPG(double, double *, double *, double, double **, int *, double ***, double **);
 #include <SystemTCD3D.h>
 #include <LinAlg.h>
 #include <MainUtilities.h>
@@ -148,7 +161,7 @@ static void dump_level(int lev, TFESpace3D *sp, TSquareMatrix3D *A, double *rhs)
 
 // ---------------------------------------------------------------- main
 int main(int argc, char **argv) {
-  if (argc < 3) { fprintf(stderr, "usage: %s file.dat outdir|- [cd|tcd]\n", argv[0]); return 1; }
+  if (argc < 3) { fprintf(stderr, "usage: %s file.dat outdir|- [cd|tcd|tcdsupg]\n", argv[0]); return 1; }
   g_out = argv[2];
   std::string mode = argc > 3 ? argv[3] : "cd";
   TDatabase *Database = new TDatabase();
@@ -213,6 +226,109 @@ int main(int argc, char **argv) {
       printf("TCD step %d t %.17e |u| %.17e\n", m + 1, TDatabase::TimeDB->CURRENTTIME, norm2(Nf, sol[L - 1]));
     }
     printf("TIME tcd_steps %.6f\n", now_s() - ts);
+    return 0;
+  }
+
+  if (mode == "tcdsupg") {
+    TDatabase::TimeDB->TIME_DISC = 2;
+    int Nf = sp[L - 1]->GetN_DegreesOfFreedom(), NAf = sp[L - 1]->GetActiveBound();
+    TFEFunction3D *u0 = new TFEFunction3D(sp[L - 1], Name, Desc, sol[L - 1], Nf);
+    u0->Interpolate(DrvInitialCondition);
+    char cm[] = "M-SUPG", ca[] = "AK-SUPG", fm[] = "form";
+    static MultiIndex3D Der[4] = {D100, D010, D001, D000};
+    static int SpNum[4] = {0, 0, 0, 0}, RowSp[3] = {0, 0, 0}, ColSp[3] = {0, 0, 0}, RhsSp[1] = {0};
+    TDiscreteForm3D *formM = new TDiscreteForm3D(cm, fm, 4, Der, SpNum, 1, 1, RowSp, ColSp, RhsSp,
+                                                 MatrixMRhsAssemble_SUPG, DrvCoeffs, NULL);
+    TDiscreteForm3D *formAK = new TDiscreteForm3D(ca, fm, 4, Der, SpNum, 3, 1, RowSp, ColSp, RhsSp,
+                                                  MatricesAKRhsAssemble_SUPG, DrvCoeffs, NULL);
+    BoundCondFunct3D *BC[3] = {DrvBoundCondition, DrvBoundCondition, DrvBoundCondition};
+    BoundValueFunct3D *BV[3] = {DrvBoundValue, DrvBoundValue, DrvBoundValue};
+    std::vector<TSquareMatrix3D *> Am(L), Km(L), Sm(L);
+    std::vector<TAssembleMat3D *> asmM(L), asmAK(L);
+    for (int i = 0; i < L; i++) {
+      TSquareStructure3D *st = new TSquareStructure3D(sp[i]);
+      st->Sort();
+      M[i] = new TSquareMatrix3D(st);
+      Am[i] = new TSquareMatrix3D(st);
+      Km[i] = new TSquareMatrix3D(st);
+      Sm[i] = new TSquareMatrix3D(st);
+      TFESpace3D *fesp[1] = {sp[i]};
+      TSquareMatrix3D *SQM1[1] = {M[i]}, *SQM3[3] = {Am[i], Km[i], Sm[i]};
+      double *R[1] = {rhs[i]};
+      TAuxParam3D *aux = new TAuxParam3D(1, 0, 0, 0, fesp, NULL, NULL, NULL, NULL, 0, NULL);
+      asmM[i] = new TAssembleMat3D(1, fesp, 1, SQM1, 0, NULL, 1, R, fesp, formM, BC, BV, aux);
+      asmM[i]->Init();
+      asmAK[i] = new TAssembleMat3D(1, fesp, 3, SQM3, 0, NULL, 1, R, fesp, formAK, BC, BV, aux);
+      asmAK[i]->Init();
+    }
+    TDatabase::TimeDB->CURRENTTIME = TDatabase::TimeDB->STARTTIME;
+    for (int i = 0; i < L; i++) {  // TSystemTCD3D::AssembleMRhs (SystemTCD3D.C:176-202)
+      int N = sp[i]->GetN_DegreesOfFreedom(), NA = sp[i]->GetActiveBound();
+      asmM[i]->Reset();
+      asmM[i]->Assemble3D();
+      memcpy(sol[i] + NA, rhs[i] + NA, (N - NA) * 8);
+    }
+    dump("tcd_u0.f64", sol[L - 1], (size_t)Nf * 8);
+    double Parameters[2] = {TDatabase::ParamDB->SC_SMOOTH_DAMP_FACTOR_SCALAR,
+                            TDatabase::ParamDB->SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR};
+    TMultiGrid3D *MG = new TMultiGrid3D(1, 2, Parameters);
+    for (int i = 0; i < L; i++) MG->AddLevel(new TMGLevel3D(i, M[i], rhs[i], sol[i], 2, NULL));
+    TSquareMatrix *SQM[1] = {M[L - 1]};
+    TMultiGridScaIte *prec = new TMultiGridScaIte(MatVect_Scalar, Defect_Scalar, NULL, 0, Nf, MG, 1);
+    TFgmresIte *fg = new TFgmresIte(MatVect_Scalar, Defect_Scalar, prec, 0, Nf, 1);
+    std::vector<double> oldrhs(rhs[L - 1], rhs[L - 1] + Nf), oldsol(Nf), B(Nf), def(Nf), x(Nf), b(Nf);
+    int nsteps = (int)TDatabase::ParamDB->P8;
+    const double rel = TDatabase::ParamDB->SC_LIN_RED_FACTOR_SCALAR;
+    for (int m = 0; m < nsteps; m++) {
+      SetTimeDiscParameters(1);
+      memcpy(oldsol.data(), sol[L - 1], Nf * 8);
+      double tau = TDatabase::TimeDB->CURRENTTIMESTEPLENGTH;
+      double th1 = TDatabase::TimeDB->THETA1, th2 = TDatabase::TimeDB->THETA2;
+      double th3 = TDatabase::TimeDB->THETA3, th4 = TDatabase::TimeDB->THETA4;
+      TDatabase::TimeDB->CURRENTTIME += tau;
+      for (int i = 0; i < L; i++) {  // AssembleARhs with the SUPG form (A, K and rhs)
+        int N = sp[i]->GetN_DegreesOfFreedom(), NA = sp[i]->GetActiveBound();
+        asmAK[i]->Reset();
+        asmAK[i]->Assemble3D();
+        memcpy(sol[i] + NA, rhs[i] + NA, (N - NA) * 8);
+      }
+      // AssembleSystMat (SystemTCD3D.C:228-294) with K
+      std::fill(B.begin(), B.end(), 0.0);
+      Daxpy(NAf, tau * th3, oldrhs.data(), B.data());
+      Daxpy(NAf, tau * th4, rhs[L - 1], B.data());
+      MatAdd(M[L - 1], Am[L - 1], -tau * th2);
+      double gamma = -tau * th2;
+      for (int i = 0; i < L; i++) MatAdd(M[i], Km[i], 1.0);
+      std::fill(def.begin(), def.end(), 0.0);
+      MatVectActive(M[L - 1], oldsol.data(), def.data());
+      Daxpy(NAf, 1.0, def.data(), B.data());
+      memcpy(B.data() + NAf, rhs[L - 1] + NAf, (Nf - NAf) * 8);
+      memcpy(sol[L - 1] + NAf, rhs[L - 1] + NAf, (Nf - NAf) * 8);
+      for (int i = 0; i < L; i++) MatAdd(M[i], Am[i], i == L - 1 ? -gamma + tau * th1 : tau * th1);
+      gamma = tau * th1;
+      memcpy(oldrhs.data(), rhs[L - 1], Nf * 8);
+      // Solve (SystemTCD3D.C:317-376): the solver works on copies; the stop
+      // is relative to this step's initial residual (the reference's
+      // SC_LIN_RED_FACTOR_SCALAR test restarts from each restart cycle)
+      memcpy(x.data(), sol[L - 1], Nf * 8);
+      memcpy(b.data(), B.data(), Nf * 8);
+      double r0;
+      ScalarDefect(M[L - 1], x.data(), b.data(), def.data(), r0);
+      TDatabase::ParamDB->SC_LIN_RES_NORM_MIN_SCALAR = rel * r0;
+      TDatabase::ParamDB->SC_LIN_RED_FACTOR_SCALAR = 0.0;
+      int its = fg->Iterate(SQM, NULL, x.data(), b.data());
+      memcpy(sol[L - 1], x.data(), Nf * 8);
+      // RestoreMassMat (SystemTCD3D.C:296-315) and the SUPG matrix
+      for (int i = 0; i < L; i++) {
+        MatAdd(M[i], Am[i], -gamma);
+        MatAdd(M[i], Km[i], -1.0);
+      }
+      char bn[64];
+      snprintf(bn, sizeof bn, "tcd_u%d.f64", m + 1);
+      dump(bn, sol[L - 1], (size_t)Nf * 8);
+      printf("TCD step %d t %.17e its %d |u| %.17e\n", m + 1, TDatabase::TimeDB->CURRENTTIME, its,
+             norm2(Nf, sol[L - 1]));
+    }
     return 0;
   }
 

@@ -327,30 +327,37 @@ static void ensure_slots(cd3d_problem &P, int l) {
 // lst64), 0 cell by cell.
 // csr false: a level assembled through affine class stencils gets the
 // stencils (and rhs) only, not its CSR values
-static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int what, double *val, bool csr = true) {
+// what 2 (the SUPG time-derivative matrix K) writes no rhs: rhs names a
+// scratch vector then (default: the level's assembled rhs)
+static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int what, double *val, bool csr = true,
+                          double *rhs = nullptr) {
   if (val == P.dl[l].val) P.dl[l].invdiag_ok = false;
   DevLevel &D = P.dl[l];
+  if (!rhs) rhs = D.rhs_asm;
   const AsmParams ap = asm_params(P, D, co, what);
-  P.last_co = co;
-  if ((co.conv_kind == 0 || what == 1) && co.f_kind == 0 && D.nclass > 0 && !uniform_assembly_off()) {
+  if (what == 0) P.last_co = co;
+  // constant coefficients (and the plain mass matrix under any convection):
+  // one element matrix per level through the class stencils
+  const bool uniform = co.conv_kind == 0 || (what == 1 && !co.supg);
+  if (uniform && co.f_kind == 0 && D.nclass > 0 && !uniform_assembly_off()) {
     // every active row is written: no clearing pass
-    kl_assemble_uniform(P.stream, D, P.mesh.order, P.tab, ap, val, D.rhs_asm);
-    k_dirichlet_rows(P.stream, D, val, D.rhs_asm);
+    kl_assemble_uniform(P.stream, D, P.mesh.order, P.tab, ap, val, rhs);
+    k_dirichlet_rows(P.stream, D, val, rhs);
     HIPCHK(hipGetLastError());
     return 1;
   }
-  if (co.conv_kind == 1 && co.f_kind == 0 && !co.supg && D.nclass > 0 && !uniform_assembly_off()) {
-    kl_assemble_affine(P.stream, D, P.mesh.order, P.tab, ap, val, D.rhs_asm, csr);
-    k_dirichlet_rows(P.stream, D, val, D.rhs_asm);
+  if (what == 0 && co.conv_kind == 1 && co.f_kind == 0 && !co.supg && D.nclass > 0 && !uniform_assembly_off()) {
+    kl_assemble_affine(P.stream, D, P.mesh.order, P.tab, ap, val, rhs, csr);
+    k_dirichlet_rows(P.stream, D, val, rhs);
     HIPCHK(hipGetLastError());
     return 2;
   }
   ensure_slots(P, l);
   HIPCHK(hipMemsetAsync(val, 0, sizeof(double) * D.nnz, P.stream));
-  HIPCHK(hipMemsetAsync(D.rhs_asm, 0, sizeof(double) * D.ndof, P.stream));
+  HIPCHK(hipMemsetAsync(rhs, 0, sizeof(double) * D.ndof, P.stream));
   for (int k = 0; k < 8; k++)
-    k_assemble(P.stream, D, P.tab, ap, val, D.rhs_asm, D.colour_start[k], D.colour_start[k + 1] - D.colour_start[k]);
-  k_dirichlet_rows(P.stream, D, val, D.rhs_asm);
+    k_assemble(P.stream, D, P.tab, ap, val, rhs, D.colour_start[k], D.colour_start[k + 1] - D.colour_start[k]);
+  k_dirichlet_rows(P.stream, D, val, rhs);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -888,7 +895,7 @@ size_t cd3d_estimate_bytes(const cd3d_mesh_desc *mesh, int restart, int cellwise
       b += 4 * nb * nb * ncell;                       // slot maps
       if (l > 0) b += 8 * nact_nnz + 4 * na;          // per-node FP32 stencils and their CSR positions
     }
-    if (tcd) b += 8 * nnz;                          // conv-diff matrix beside the mass matrix
+    if (tcd) b += 8 * nnz * (tcd == 2 ? 2 : 1);      // conv-diff matrix (and K with SUPG) beside the mass matrix
     total += b + (2 << 20);                         // allocation granularity of ~20 buffers
     if (l == 0) total += 8.0 * na * na + 8.0 * (double)gj_scratch_doubles(na);  // coarse inverse + Gauss-Jordan scratch
     prevN = N;
@@ -1209,7 +1216,6 @@ cd3d_status cd3d_interp_pulse(cd3d_problem *p, int level, const cd3d_coeffs *co,
 
 static void tcd_init_impl(cd3d_problem &P, const cd3d_coeffs &co0, double t0, const double *u) {
   check_coeffs(co0);
-  if (co0.supg) throw ArgError("TCD3D is Galerkin only, as TSystemTCD3D::Init (SystemTCD3D.C:114-131)");
   const int Lf = (int)P.dl.size() - 1;
   DevLevel &F = P.dl[Lf];
   const int N = F.ndof, NA = F.nact;
@@ -1222,7 +1228,8 @@ static void tcd_init_impl(cd3d_problem &P, const cd3d_coeffs &co0, double t0, co
   }
   ensure_krylov(P, 1);
   // AssembleMRhs at t0 on every level (SystemTCD3D.C:176-202): the level
-  // matrices hold the mass matrices from here on
+  // matrices hold the mass matrices from here on; with SUPG the rhs carries
+  // the SUPG test functions (MatrixMRhsAssemble_SUPG, TCD3D.C:48)
   cd3d_coeffs co = co0;
   co.t = t0;
   P.assembled = false;
@@ -1247,7 +1254,6 @@ static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_sol
                          double theta, cd3d_status &solve_status, double *r0_out = nullptr,
                          double *rf_out = nullptr) {
   check_coeffs(co0);
-  if (co0.supg) throw ArgError("TCD3D is Galerkin only, as TSystemTCD3D::Init (SystemTCD3D.C:114-131)");
   if (!P.tcd_ready) throw std::logic_error("cd3d_tcd_init before cd3d_tcd_step");
   const double th1 = theta, th2 = 1 - theta, th3 = 1 - theta, th4 = theta;
   const int Lf = (int)P.dl.size() - 1;
@@ -1267,9 +1273,21 @@ static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_sol
   const bool affine_asm = co.conv_kind == 1 && co.f_kind == 0 && !co.supg && !uniform_assembly_off();
   bool lattice = sp.precision == CD3D_MIXED && Lf > 0 && affine_asm;
   for (int l = 1; l <= Lf; l++) lattice = lattice && P.dl[l].tcd_mass_uniform && P.dl[l].nclass > 0;
-  // AssembleARhs at t_new (SystemTCD3D.C:204-226)
+  // AssembleARhs at t_new (SystemTCD3D.C:204-226); with SUPG the A and rhs of
+  // MatricesAKRhsAssemble_SUPG (TCD3D.C:102) and its time-derivative matrix
+  // K, kept as M + K: the reference adds K to M for the rhs product and the
+  // system and removes it again after the solve (the SUPG branch of
+  // Main_Users/Sashi/3DPrograms/TCD_3D.C:1184-1188,1314-1317); here M itself
+  // stays untouched
   std::vector<int> conv_path(Lf + 1);
   for (int l = 0; l <= Lf; l++) conv_path[l] = assemble_level(P, l, co, 0, P.dl[l].aconv, !(lattice && l > 0));
+  if (co.supg)
+    for (int l = 0; l <= Lf; l++) {
+      DevLevel &D = P.dl[l];
+      if (!D.ksupg) D.ksupg = dalloc<double>(P, D.nnz);
+      assemble_level(P, l, co, 2, D.ksupg, true, D.tmp);
+      k_matadd(P.stream, P.hl[l].rowptr[D.nact], D.ksupg, D.val, 1.0);  // M + K on the active rows
+    }
   if (lattice)
     for (int l = 1; l <= Lf; l++)
       if (conv_path[l] != 2) throw std::logic_error("TCD3D lattice step: a level was not assembled affinely");
@@ -1294,7 +1312,9 @@ static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_sol
     kl_affine64_apply(P.stream, F, P.mesh.order, -tau * th2, true, P.xl, P.bl);
     kl_lat64_to_dof(P.stream, F, P.bl, P.tcd_def);
   } else {
-    k_matvec_pair_active(P.stream, F, F.val, F.aconv, 1.0, -tau * th2, P.tcd_oldsol, P.tcd_def);
+    // (M - tau th2 A) u_old, or (M + K - tau th2 A) u_old with SUPG
+    k_matvec_pair_active(P.stream, F, co.supg ? F.ksupg : F.val, F.aconv, 1.0, -tau * th2, P.tcd_oldsol,
+                         P.tcd_def);
   }
   k_axpy(P.stream, NA, 1.0, P.tcd_def, P.bs);
   k_copy_range(P.stream, NA, N, F.rhs_asm, P.bs);
@@ -1315,7 +1335,8 @@ static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_sol
   for (int l = 0; l <= Lf; l++) {
     DevLevel &D = P.dl[l];
     if (!(lattice && l > 0)) {
-      k_matadd_into(P.stream, P.hl[l].rowptr[D.nact], D.aconv, D.val, tau * th1);  // A <- M + tau th1 A
+      // A <- M + tau th1 A, or (M + K) + tau th1 A with SUPG
+      k_matadd_into(P.stream, P.hl[l].rowptr[D.nact], D.aconv, co.supg ? D.ksupg : D.val, tau * th1);
       sys.swap_level(l);
       sys.levels.push_back(l);
     }

@@ -81,8 +81,9 @@ struct DevLevel {
   bool invdiag_ok = false;    // computed on first use after each change of val
   double *sol = nullptr, *rhs = nullptr, *aux = nullptr, *tmp = nullptr;
   double *rhs_asm = nullptr;  // assembled rhs (the multigrid works in `rhs`)
-  // TCD3D: conv-diff matrix next to the system (mass) matrix in `val`
-  double *aconv = nullptr;
+  // TCD3D: conv-diff matrix next to the system (mass) matrix in `val`; with
+  // SUPG also the time-derivative matrix K (M + K once a step has formed it)
+  double *aconv = nullptr, *ksupg = nullptr;
   // transfer (level >= 1)
   int *p_rowptr = nullptr, *p_col = nullptr, *r_rowptr = nullptr, *r_col = nullptr;
   double *p_val = nullptr, *r_val = nullptr;
@@ -258,7 +259,7 @@ void k_matvec_pair_active(hipStream_t s, const DevLevel &L, const double *a, con
 void k_matvec_active(hipStream_t s, const DevLevel &L, const double *x, double *y);
 // assembly
 struct AsmParams {
-  int what;  // 0 conv-diff, 1 mass
+  int what;  // 0 conv-diff, 1 mass (SUPG rhs when supg), 2 SUPG time-derivative matrix K
   double eps, b[3], c, f_val;
   int conv_kind, f_kind, supg, order;
   double rot_c[2], sin2pit;

@@ -686,7 +686,8 @@ __device__ inline double hconv_brick(const double *h, double b0, double b1, doub
 // no exchange between the threads of a cell.  kappa and beta fold in the
 // physical gradients J^-T grad phi of THexaAffin (the cell frames are signed
 // axis permutations of the brick).  The mass matrix (what 1) is
-// A_ij += (w phi_i) phi_j.  Then the row is added into the CSR values through
+// A_ij += (w phi_i) phi_j with the rhs w f t_i, and the SUPG time-derivative
+// matrix (what 2) A_ij += (w delta beta.grad phi_i) phi_j.  Then the row is added into the CSR values through
 // the slot map (cells of one colour share no DOF).
 // LDS_TAB: the per-thread operands (phi_i and its gradient at every q) are
 // read from an LDS copy of the basis table made at the start of the block;
@@ -756,16 +757,6 @@ __global__ __launch_bounds__(kBlock) void assemble_k(int ncells, const int *__re
         X[2] = dax[a] == 2 ? xa : X[2];
       }
     }
-    double f = ap.f_val;
-    if (ap.what == 1) {
-      if (VAR && ap.f_kind == 1)
-        f = ap.eps * 3 * M_PI * M_PI * sin(M_PI * X[0]) * sin(M_PI * X[1]) * sin(M_PI * X[2]);
-      const double bw = w * phi_i;
-#pragma unroll
-      for (int j = 0; j < NB; j++) acc[j] = fma(bw, T[4 + 4 * j], acc[j]);
-      racc += w * phi_i * f;
-      continue;
-    }
     double b0 = ap.b[0], b1 = ap.b[1], b2 = ap.b[2];
     double delta = ap.supg ? ap.delta_const : 0.0;
     if (VAR && ap.conv_kind == 1) {
@@ -779,6 +770,7 @@ __global__ __launch_bounds__(kBlock) void assemble_k(int ncells, const int *__re
         delta = sdfem_delta(hconv, ap.eps, sqrt(b0 * b0 + b1 * b1 + b2 * b2), nu);
       }
     }
+    double f = ap.f_val;
     if (VAR && ap.f_kind == 1) {
       const double sx = sin(M_PI * X[0]), sy = sin(M_PI * X[1]), sz = sin(M_PI * X[2]);
       const double cx = cos(M_PI * X[0]), cy = cos(M_PI * X[1]), cz = cos(M_PI * X[2]), u = sx * sy * sz;
@@ -787,6 +779,19 @@ __global__ __launch_bounds__(kBlock) void assemble_k(int ncells, const int *__re
     }
     const double bg = beta[0] * g0 + beta[1] * g1 + beta[2] * g2;
     const double ti = phi_i + delta * bg;
+    if (ap.what == 1) {  // mass; rhs with the SUPG test function (MatrixMRhsAssemble_SUPG, TCD3D.C:48)
+      const double bw = w * phi_i;
+#pragma unroll
+      for (int j = 0; j < NB; j++) acc[j] = fma(bw, T[4 + 4 * j], acc[j]);
+      racc += w * f * ti;
+      continue;
+    }
+    if (ap.what == 2) {  // SUPG time-derivative matrix K (MatricesAKRhsAssemble_SUPG, TCD3D.C:102)
+      const double kw = w * (delta * bg);
+#pragma unroll
+      for (int j = 0; j < NB; j++) acc[j] = fma(kw, T[4 + 4 * j], acc[j]);
+      continue;
+    }
     const double a0 = w * (kap[0] * g0 + ti * beta[0]);
     const double a1 = w * (kap[1] * g1 + ti * beta[1]);
     const double a2 = w * (kap[2] * g2 + ti * beta[2]);

@@ -997,7 +997,8 @@ __global__ void lat_invdiag_k(long long n, const int *__restrict__ node2dof, con
 // the CSR values of every interior row; Dirichlet rows are set apart.
 
 // E[i][j] (NB x NB, cell-local lexicographic numbering) and the element rhs
-// E[NB*NB + i]; one thread per entry
+// E[NB*NB + i]; one thread per entry.  what: 0 conv-diff, 1 mass (rhs with
+// the SUPG test function when supg), 2 the SUPG time-derivative matrix K
 template <int P>
 __global__ __launch_bounds__(1024) void elem_k(const double *__restrict__ tab, AsmParams ap, double *__restrict__ E) {
   constexpr int NB = (P + 1) * (P + 1) * (P + 1), N1 = P + 2, NQ = N1 * N1 * N1, STRIDE = 4 + 4 * NB;
@@ -1016,16 +1017,18 @@ __global__ __launch_bounds__(1024) void elem_k(const double *__restrict__ tab, A
     kap[a] = ap.eps * 4.0 / (ap.h[a] * ap.h[a]);
     beta[a] = ap.b[a] * 2.0 / ap.h[a];
   }
-  const double delta = (ap.supg && ap.what == 0) ? ap.delta_const : 0.0;
+  const double delta = ap.supg ? ap.delta_const : 0.0;
   double acc = 0.0, racc = 0.0;
   for (int q = 0; q < NQ; q++) {
     const double *T = Ts + q * STRIDE;
     const double w = T[0] * detj;
     const double pi_ = T[4 + 4 * i], gi0 = T[5 + 4 * i], gi1 = T[6 + 4 * i], gi2 = T[7 + 4 * i];
     const double pj = T[4 + 4 * j], gj0 = T[5 + 4 * j], gj1 = T[6 + 4 * j], gj2 = T[7 + 4 * j];
-    if (ap.what == 1) {
+    if (ap.what == 1) {  // mass, rhs with the SUPG test function when supg (TCD3D.C:48)
       acc += w * (pi_ * pj);
-      racc += w * pi_ * ap.f_val;
+      racc += w * ap.f_val * (pi_ + delta * (beta[0] * gi0 + beta[1] * gi1 + beta[2] * gi2));
+    } else if (ap.what == 2) {  // SUPG time-derivative matrix K, no rhs (TCD3D.C:102)
+      acc += w * (delta * (beta[0] * gi0 + beta[1] * gi1 + beta[2] * gi2) * pj);
     } else {
       const double bgi = beta[0] * gi0 + beta[1] * gi1 + beta[2] * gi2;
       const double bgj = beta[0] * gj0 + beta[1] * gj1 + beta[2] * gj2;

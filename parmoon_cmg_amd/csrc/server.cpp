@@ -97,7 +97,8 @@ struct cd3d_server {
   size_t estimate(const cd3d_request &r) const {
     const bool cellwise = r.coeffs.conv_kind != 0 || r.coeffs.f_kind != 0;
     const int restart = std::max(1, std::min(r.solver.restart, r.solver.maxit));
-    return cd3d_estimate_bytes(&r.mesh, restart, cellwise ? 1 : 0, r.kind == CD3D_REQ_TCD ? 1 : 0);
+    const int tcd = r.kind == CD3D_REQ_TCD ? (r.coeffs.supg ? 2 : 1) : 0;
+    return cd3d_estimate_bytes(&r.mesh, restart, cellwise ? 1 : 0, tcd);
   }
 
   // evict idle hierarchies (least recently used first, never `keep`) until

@@ -7,6 +7,7 @@
 //                      SystemCD3D.C:138-203; tcd: TSystemTCD3D
 //     order k0 levels box=x,y,z eps b=x,y,z c f_kind f_val supg conv_kind
 //     rot=x,y pulse=x,y,z  (problem)     tau nsteps time_disc (tcd)
+//     tcd_stop=step  each TCD step stops at rel times its own initial residual
 //     rel=1e-10  stop at |r| <= rel |r0| (SC_LIN_RES_NORM_MIN = rel |r0|)
 //     mixed=0|1  out=<prefix>
 // Writes <prefix>.sol (finest-level solution or states, f64) and
@@ -112,14 +113,16 @@ int main(int argc, char **argv) {
   co.supg = (int)num("supg", 0);
   const double rel = num("rel", 1e-10);
 
+  const int disctype = co.supg ? SDFEM : GALERKIN;
   if (mode == "tcd") {
     // TCD3D_ParMooN.C:390-634: initial condition, AssembleMRhs, time loop
-    TSystemTCD3D sys(L, fes.data(), solp.data(), rhsp.data(), GALERKIN, GMG);
+    TSystemTCD3D sys(L, fes.data(), solp.data(), rhsp.data(), disctype, GMG);
     sys.Init(co);
     Check(cd3d_interp_pulse(spaces[0]->handle(), L - 1, &co, sol[L - 1].data()), "cd3d_interp_pulse",
           spaces[0]->handle());
     db.SC_LIN_RES_NORM_MIN_SCALAR = 1e-30;
     db.SC_LIN_RED_FACTOR_SCALAR = rel;
+    if (args.count("tcd_stop") && args["tcd_stop"] == "step") sys.SetStepRelativeTolerance(rel);
     sys.AssembleMRhs(0.0);
     const int nsteps = (int)num("nsteps", 3);
     const double tau = num("tau", 1e-3);
@@ -133,7 +136,6 @@ int main(int argc, char **argv) {
     return 0;
   }
 
-  const int disctype = co.supg ? SDFEM : GALERKIN;
   if (mode == "cd") {
     // CD3D_ParMooN.C:388-441
     TSystemCD3D sys(L, fes.data(), solp.data(), rhsp.data(), disctype, GMG);

@@ -59,6 +59,17 @@ TCD_CASES = {
                       co=dict(eps=1e-4, conv_kind=1, rot_c=(0.5, 0.5), pulse_c=(0.3, 0.5, 0.5), f_val=0.0)),
     "m2_q2_tcd": dict(order=2, m=2, k0=0, levels=2, tau=1e-3, nsteps=3, box=UNIT_BOX,
                       co=dict(eps=1e-4, conv_kind=1, rot_c=(0.5, 0.5), pulse_c=(0.3, 0.5, 0.5), f_val=0.0)),
+    # SUPG-stabilised Crank-Nicolson (TCD3D.C:48,102): rotating field with
+    # f = 1, Q1 and Q2, and a constant field with reaction
+    "m2_q1_tcd_supg": dict(solver=dict(omega=0.5, pre=3, post=3), order=1, m=2, k0=1, levels=2, tau=1e-3, nsteps=3, box=UNIT_BOX,
+                           co=dict(eps=1e-4, conv_kind=1, rot_c=(0.5, 0.5), pulse_c=(0.3, 0.5, 0.5), f_val=1.0,
+                                   supg=1)),
+    "m2_q2_tcd_supg": dict(solver=dict(omega=0.5, pre=3, post=3), order=2, m=2, k0=0, levels=2, tau=1e-3, nsteps=3, box=UNIT_BOX,
+                           co=dict(eps=1e-4, conv_kind=1, rot_c=(0.5, 0.5), pulse_c=(0.3, 0.5, 0.5), f_val=1.0,
+                                   supg=1)),
+    "unit_q1_tcd_supg_const": dict(solver=dict(omega=0.5, pre=3, post=3), order=1, k0=1, levels=3, tau=1e-2, nsteps=3, box=UNIT_BOX,
+                                   co=dict(eps=1e-3, b=(0.6, -0.3, 0.4), c=0.1, pulse_c=(0.3, 0.5, 0.5),
+                                           f_val=1.0, supg=1)),
 }
 TCD_CASE = dict(TCD_CASES["q1_tcd"], name="q1_tcd")
 
@@ -75,3 +86,14 @@ def case_solver(name):
 
 def case_box(c):
     return c.get("box", REF_BOX)
+
+
+def tcd_solver(c):
+    """Solver keywords of a TCD case's reference run: REF_SOLVER with the
+    case's changes; Galerkin runs stop on the reference's restart-relative
+    1e-10 (TSystemTCD3D::Solve), SUPG runs on 1e-10 of each step's initial
+    residual."""
+    kw = dict(REF_SOLVER, **c.get("solver", {}))
+    if c["co"].get("supg"):
+        return dict(kw, rel_tol=1e-10, res_norm_min=1e-30, red_factor=0.0)
+    return dict(kw, rel_tol=0.0, res_norm_min=1e-30, red_factor=1e-10)

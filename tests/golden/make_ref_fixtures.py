@@ -128,6 +128,30 @@ CASES = {
                               P0=1e-4, P4=0, P5=0, P6=0.0, P7=1, P8=3,
                               P9=0.3, P10=0.5, P11=0.5, P12=0.5, P13=0.5,
                               TIME_DISC=2, STARTTIME=0.0, ENDTIME=1.0, TIMESTEPLENGTH=1e-3)),
+    # SUPG-stabilised Crank-Nicolson (MatrixMRhsAssemble_SUPG at t0,
+    # MatricesAKRhsAssemble_SUPG per step, the SUPG matrix K beside M): the
+    # rotating field with f = 1 (per-cell delta, SUPG rhs), Q1 and Q2, and a
+    # constant field (the same element matrices in every cell); the
+    # benchmark's smoother (omega 0.5, 3 + 3): with omega 0.8, 2 + 2 the
+    # nonsymmetric M + K + tau theta1 A stalls near 1e-7 within 200 iterations
+    "m2_q1_tcd_supg": ("tcdsupg", dict(SC_SMOOTH_DAMP_FACTOR_SCALAR=0.5, SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=0.5,
+                                       SC_PRE_SMOOTH_SCALAR=3, SC_POST_SMOOTH_SCALAR=3, geo="m2", ANSATZ_ORDER=1, LEVELS=2, UNIFORM_STEPS=1, DISCTYPE=2,
+                                       CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                                       P0=1e-4, P4=0, P5=0, P6=1.0, P7=1, P8=3,
+                                       P9=0.3, P10=0.5, P11=0.5, P12=0.5, P13=0.5,
+                                       TIME_DISC=2, STARTTIME=0.0, ENDTIME=1.0, TIMESTEPLENGTH=1e-3)),
+    "m2_q2_tcd_supg": ("tcdsupg", dict(SC_SMOOTH_DAMP_FACTOR_SCALAR=0.5, SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=0.5,
+                                       SC_PRE_SMOOTH_SCALAR=3, SC_POST_SMOOTH_SCALAR=3, geo="m2", ANSATZ_ORDER=2, LEVELS=2, UNIFORM_STEPS=0, DISCTYPE=2,
+                                       CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                                       P0=1e-4, P4=0, P5=0, P6=1.0, P7=1, P8=3,
+                                       P9=0.3, P10=0.5, P11=0.5, P12=0.5, P13=0.5,
+                                       TIME_DISC=2, STARTTIME=0.0, ENDTIME=1.0, TIMESTEPLENGTH=1e-3)),
+    "unit_q1_tcd_supg_const": ("tcdsupg", dict(SC_SMOOTH_DAMP_FACTOR_SCALAR=0.5, SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=0.5,
+                                               SC_PRE_SMOOTH_SCALAR=3, SC_POST_SMOOTH_SCALAR=3, geo="unit", ANSATZ_ORDER=1, LEVELS=3, UNIFORM_STEPS=1, DISCTYPE=2,
+                                               CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                                               P0=1e-3, P1=0.6, P2=-0.3, P3=0.4, P4=0.1, P5=0, P6=1.0, P7=0,
+                                               P8=3, P9=0.3, P10=0.5, P11=0.5,
+                                               TIME_DISC=2, STARTTIME=0.0, ENDTIME=1.0, TIMESTEPLENGTH=1e-2)),
 }
 
 # Records only (iteration count and residual history, no level data): the
@@ -208,6 +232,9 @@ def run_case(name):
             steps = int(ov["P8"])
             for k in range(steps + 1):
                 out[f"u{k}"] = np.fromfile(os.path.join(td, f"tcd_u{k}.f64"), dtype=np.float64)
+            its = [int(x) for x in re.findall(r"TCD step \d+ t \S+ its (\d+)", p.stdout)]
+            if its:
+                out["its_steps"] = np.array(its)
         if record:
             keep = ("params", "box", "its", "r0", "rfinal", "res_hist", "ref_times")
             out = {k: v for k, v in out.items() if k in keep}

@@ -19,7 +19,7 @@ import pytest
 
 import parmoon_cmg_amd as pm
 from oracle import pyoracle as po
-from tests.cases import CD_CASES, REF_BOX, REF_SOLVER, TCD_CASE, TCD_CASES, case_box, case_solver
+from tests.cases import CD_CASES, REF_BOX, REF_SOLVER, TCD_CASE, TCD_CASES, case_box, case_solver, tcd_solver
 
 pytestmark = pytest.mark.gpu
 
@@ -229,13 +229,35 @@ def test_tcd_crank_nicolson_matches_reference(name, golden):
     P = make_problem(c)
     co = pm.coeffs(**c["co"])
     u0 = P.interp_pulse(c["levels"] - 1, co)
-    sp = pm.solver_params(**REF_SOLVER, rel_tol=0.0, res_norm_min=1e-30, red_factor=1e-10)
+    sp = pm.solver_params(**tcd_solver(c))
     _, states, st = P.tcd_run(co, sp, 0.0, c["tau"], c["nsteps"], u0)
     for k in range(c["nsteps"]):
         ref = d[f"u{k + 1}"]
         err = np.abs(states[k] - ref).max() / np.abs(ref).max()
         say(f"[gpu] tcd {name} step {k + 1} err {err:.2e}")
         assert err < SOL_TOL
+    if "its_steps" in d:
+        say(f"[gpu] tcd {name} its {st.iterations} (reference {int(d['its_steps'].sum())})")
+        assert st.iterations == int(d["its_steps"].sum())
+    P.close()
+
+
+@pytest.mark.parametrize("name", [n for n, c in TCD_CASES.items() if c["co"].get("supg")])
+def test_tcd_supg_mixed_matches_reference(name, golden):
+    # the SUPG Crank-Nicolson steps with the FP32 V-cycle: every state within
+    # the mixed form's bar of the reference's FP64 states
+    c = TCD_CASES[name]
+    d = golden(name)
+    P = make_problem(c)
+    co = pm.coeffs(**c["co"])
+    u0 = P.interp_pulse(c["levels"] - 1, co)
+    sp = pm.solver_params(**tcd_solver(c), precision=pm.MIXED)
+    _, states, st = P.tcd_run(co, sp, 0.0, c["tau"], c["nsteps"], u0)
+    for k in range(c["nsteps"]):
+        ref = d[f"u{k + 1}"]
+        err = np.abs(states[k] - ref).max() / np.abs(ref).max()
+        say(f"[gpu] tcd supg mixed {name} step {k + 1} err {err:.2e}")
+        assert err < MIXED_TOL
     P.close()
 
 

@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from tests.cases import CD_CASES, TCD_CASES, REF_SOLVER, case_box, case_solver
+from tests.cases import CD_CASES, TCD_CASES, REF_SOLVER, case_box, case_solver, tcd_solver
 
 ENTRY_TOL = 1e-12
 SOL_TOL = 1e-10
@@ -108,11 +108,13 @@ def test_tcd_crank_nicolson(name, golden):
     u0 = H.interp_pulse(c["levels"] - 1, co)
     nact = H.info(c["levels"] - 1)["nact"]
     assert np.abs(u0[:nact] - d["u0"][:nact]).max() < 1e-14
-    sv = po.solver(**dict(REF_SOLVER, res_norm_min=1e-30, red_factor=1e-10))
-    _, states, _ = H.tcd_run(co, sv, 0.0, c["tau"], c["nsteps"], u0)
+    sv = po.solver(**tcd_solver(c))
+    _, states, its = H.tcd_run(co, sv, 0.0, c["tau"], c["nsteps"], u0)
     for k in range(c["nsteps"]):
         ref = d[f"u{k + 1}"]
         assert np.abs(states[k] - ref).max() <= SOL_TOL * np.abs(ref).max()
+    if "its_steps" in d:  # the reference's FGMRES iterations, summed over the steps
+        assert its == int(d["its_steps"].sum())
 
 
 @pytest.mark.parametrize("omega,sweeps,name", [(0.8, 2, "record_c2_n16_omega08"), (0.5, 3, "record_c2_n16_omega05")])

@@ -10,7 +10,7 @@ import sys
 import numpy as np
 import pytest
 
-from tests.cases import CD_CASES, REF_BOX, TCD_CASE
+from tests.cases import CD_CASES, REF_BOX, TCD_CASES, tcd_solver
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(REPO, "parmoon_cmg_amd", "_bin", "cd3d_shim_main")
@@ -25,7 +25,8 @@ def say(*a):
 def case_args(c):
     co = c["co"]
     box = c.get("box", REF_BOX)
-    a = [f"order={c['order']}", f"k0={c['k0']}", f"levels={c['levels']}", "box=" + ",".join(map(repr, box))]
+    a = [f"order={c['order']}", f"m={c.get('m', 1)}", f"k0={c['k0']}", f"levels={c['levels']}",
+         "box=" + ",".join(map(repr, box))]
     a += [f"eps={co.get('eps', 1.0)!r}", "b=" + ",".join(map(repr, co.get("b", (0.0, 0.0, 0.0)))),
           f"c={co.get('c', 0.0)!r}", f"f_kind={co.get('f_kind', 0)}", f"f_val={co.get('f_val', 1.0)!r}",
           f"supg={co.get('supg', 0)}", f"conv_kind={co.get('conv_kind', 0)}"]
@@ -80,11 +81,17 @@ def test_shim_cd3d_matches_reference(name, mode, golden, tmp_path):
 
 
 @pytest.mark.gpu
-def test_shim_tcd3d_matches_reference(golden, tmp_path):
-    c = TCD_CASE
-    d = golden(c["name"])
-    say("[gpu] shim tcd")
-    x, _, _ = run_shim(["mode=tcd", f"tau={c['tau']!r}", f"nsteps={c['nsteps']}"] + case_args(c), tmp_path, "tcd")
+@pytest.mark.parametrize("name", ["q1_tcd", "unit_q1_tcd_supg_const"])
+def test_shim_tcd3d_matches_reference(name, golden, tmp_path):
+    c = TCD_CASES[name]
+    d = golden(name)
+    say(f"[gpu] shim tcd {name}")
+    sv = tcd_solver(c)
+    extra = [f"omega={sv['omega']!r}", f"pre={sv['pre']}", f"post={sv['post']}"]
+    if c["co"].get("supg"):
+        extra.append("tcd_stop=step")  # 1e-10 of each step's initial residual
+    x, _, _ = run_shim(["mode=tcd", f"tau={c['tau']!r}", f"nsteps={c['nsteps']}"] + extra + case_args(c), tmp_path,
+                       "tcd")
     states = x.reshape(c["nsteps"], -1)
     for k in range(c["nsteps"]):
         ref = d[f"u{k + 1}"]
