@@ -70,7 +70,9 @@ typedef enum { CD3D_FP64 = 0, CD3D_MIXED = 1 } cd3d_precision;
 /* Solver parameters: the ParamDB SC_* knobs of the GMG/FGMRES path
  * (Database.C:648-757, FgmresIte.C:71-85, MultiGrid3D.C). */
 typedef struct {
-  int smoother;         /* SC_SMOOTHER_SCALAR: 1 Jacobi */
+  int smoother;         /* SC_SMOOTHER_SCALAR: 1 damped Jacobi (MGLevel3D.C:443); 13 multicolour SSOR (no
+                           reference counterpart: relaxed Gauss-Seidel over node colours, forward then
+                           backward, for convection-dominated problems; FP64 in both precision forms) */
   int pre, post;        /* SC_PRE_SMOOTH_SCALAR, SC_POST_SMOOTH_SCALAR */
   double omega;         /* SC_SMOOTH_DAMP_FACTOR_SCALAR */
   int coarse_smoother;  /* SC_COARSE_SMOOTHER_SCALAR: 17 exact (default), 3 SSOR, 1 Jacobi */

@@ -39,6 +39,9 @@ typedef struct {
   int simulate;        /* 1: do not touch the device; a request takes sim_us_per_dof * DOFs microseconds
                           (admission and statistics only, for capacity planning) */
   double sim_us_per_dof;
+  int robust_after;    /* a CD solve with the damped Jacobi smoother that has not converged after this many
+                          FGMRES iterations continues from its iterate with the multicolour SSOR smoother
+                          (13, omega 1, 2 + 2) on the rest of its iteration budget; 0: never */
 } cd3d_server_params;
 
 typedef enum { CD3D_REQ_CD = 0, CD3D_REQ_TCD = 1 } cd3d_request_kind;
@@ -67,6 +70,7 @@ typedef struct {
   int n_dof;                   /* finest-level DOFs (length of the solution) */
   int steps_done;              /* TCD: time steps completed; a step that does not converge ends the run
                                   (status CD3D_NOT_CONVERGED) and the state after it is returned */
+  int escalated;               /* 1: the solve was continued with the multicolour SSOR smoother (robust_after) */
 } cd3d_result;
 
 typedef struct {

@@ -295,6 +295,10 @@ void orc_get_csr(const orc_hier *h, int l, int *rowptr, int *kcol, double *entri
   if (entries) memcpy(entries, L->A, sizeof(double) * L->nnz);
 }
 
+void orc_get_dof_nodes(const orc_hier *h, int l, int *node) {
+  memcpy(node, h->lv[l].dof_node, sizeof(int) * h->lv[l].ndof);
+}
+
 void orc_get_rhs(const orc_hier *h, int l, double *rhs) {
   memcpy(rhs, h->lv[l].rhs, sizeof(double) * h->lv[l].ndof);
 }
@@ -684,9 +688,53 @@ void orc_restrict(const orc_hier *h, int l, const double *fine, double *coarse) 
 
 /* ------------------------------------------------------------------ multigrid */
 
+/* Multicolour SSOR (smoother 13, no reference counterpart: a parallel
+ * smoother of this build for convection-dominated problems, where damped
+ * Jacobi stalls).  Interior nodes are coloured by their lattice coordinates
+ * modulo 2 (Q1) or 3 (Q2): c = X%k + k (Y%k + k Z%k), so two nodes of one
+ * colour never share a cell and their rows never couple.  One sweep: x_D = f_D,
+ * then relaxed Gauss-Seidel colour by colour, c = 0 .. k^3 - 1 (every row of a
+ * colour updated from the current x), then back, c = k^3 - 1 .. 0.
+ * Pinned by tests/test_oracle_math.py against a numpy statement of the same
+ * definition. */
+static int node_colour(const level_t *L, int dof, int p) {
+  int k = p == 1 ? 2 : 3, node = L->dof_node[dof], nn = L->nn;
+  int X = node % nn, Y = (node / nn) % nn, Z = node / (nn * nn);
+  return X % k + k * (Y % k + k * (Z % k));
+}
+static void colour_pass(level_t *L, double omega, int p, int colour) {
+  double *sol = L->sol, *f = L->rhs;
+  for (int i = 0; i < L->nact; i++) {
+    if (node_colour(L, i, p) != colour) continue;
+    double s = f[i], diag = 1;
+    for (int j = L->rowptr[i]; j < L->rowptr[i + 1]; j++) {
+      int c = L->kcol[j];
+      if (c == i) diag = L->A[j];
+      else s -= L->A[j] * sol[c];
+    }
+    sol[i] = (1 - omega) * sol[i] + omega * s / diag;
+  }
+}
+static void mcssor(level_t *L, double omega, int p) {
+  int k = p == 1 ? 2 : 3, ncol = k * k * k;
+  memcpy(L->sol + L->nact, L->rhs + L->nact, sizeof(double) * (L->ndof - L->nact));
+  for (int c = 0; c < ncol; c++) colour_pass(L, omega, p, c);
+  for (int c = ncol - 1; c >= 0; c--) colour_pass(L, omega, p, c);
+}
+void orc_mcssor_sweep(orc_hier *h, int l, double omega, const double *f, double *x) {
+  level_t *L = h->lv + l;
+  double *s0 = L->sol, *r0 = L->rhs;
+  L->sol = x;
+  L->rhs = (double *)f;
+  mcssor(L, omega, h->order);
+  L->sol = s0;
+  L->rhs = r0;
+}
+
 static void smooth(orc_hier *h, level_t *L, const orc_solver *s, int n) {
   for (int k = 0; k < n; k++) {
     if (s->smoother == 3) ssor(L, s->omega);
+    else if (s->smoother == 13) mcssor(L, s->omega, h->order);
     else jacobi(L, s->omega);
   }
 }

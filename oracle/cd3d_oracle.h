@@ -30,7 +30,7 @@ typedef struct {
 } orc_coeffs;
 
 typedef struct {
-  int smoother;          /* 1 Jacobi, 3 SSOR (SC_SMOOTHER_SCALAR) */
+  int smoother;          /* 1 Jacobi, 3 SSOR (SC_SMOOTHER_SCALAR); 13 multicolour SSOR (this build's own) */
   int pre, post;         /* SC_PRE/POST_SMOOTH_SCALAR */
   double omega;          /* SC_SMOOTH_DAMP_FACTOR_SCALAR */
   int coarse_smoother;   /* 1 Jacobi, 3 SSOR, 17 exact (SC_COARSE_SMOOTHER_SCALAR) */
@@ -63,6 +63,8 @@ void orc_get_cells(const orc_hier *h, int l, double *xyz /* n_cells*8*3 */);
 void orc_get_gn(const orc_hier *h, int l, int *gn /* n_cells*nb */);
 void orc_get_csr(const orc_hier *h, int l, int *rowptr, int *kcol, double *entries);
 void orc_get_rhs(const orc_hier *h, int l, double *rhs);
+/* lattice node of every DOF (node = X + nn (Y + nn Z), nn = order n + 1) */
+void orc_get_dof_nodes(const orc_hier *h, int l, int *node);
 void orc_set_entries(orc_hier *h, int l, const double *entries);
 
 /* assemble A and rhs on every level (SystemCD3D::Assemble) */
@@ -90,6 +92,9 @@ int orc_tcd_run(orc_hier *h, const orc_coeffs *co, const orc_solver *s, double t
                 int nsteps, double *u, double *states);
 /* nodal interpolation of the Gaussian pulse initial condition on level l */
 void orc_interp_pulse(const orc_hier *h, int l, const orc_coeffs *co, double *u);
+
+/* one multicolour SSOR sweep (smoother 13) on level l: x in/out, f the rhs */
+void orc_mcssor_sweep(orc_hier *h, int l, double omega, const double *f, double *x);
 
 /* SUPG parameter (Compute_SDFEM_delta, SDFEM_TYPE 2) for tests */
 double orc_sdfem_delta(double hconv, double eps, const double *b, int order);

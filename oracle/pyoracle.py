@@ -70,6 +70,7 @@ def lib():
         L.orc_get_gn.argtypes = [P, I, ip]
         L.orc_get_csr.argtypes = [P, I, ip, ip, dp]
         L.orc_get_rhs.argtypes = [P, I, dp]
+        L.orc_get_dof_nodes.argtypes = [P, I, ip]
         L.orc_set_entries.argtypes = [P, I, dp]
         L.orc_assemble.argtypes = [P, C.POINTER(Coeffs)]
         L.orc_assemble_level.argtypes = [P, I, C.POINTER(Coeffs), I, dp, dp]
@@ -84,6 +85,7 @@ def lib():
         L.orc_tcd_run.restype = I
         L.orc_tcd_run.argtypes = [P, C.POINTER(Coeffs), C.POINTER(Solver), D, D, I, dp, dp]
         L.orc_interp_pulse.argtypes = [P, I, C.POINTER(Coeffs), dp]
+        L.orc_mcssor_sweep.argtypes = [P, I, D, dp, dp]
         L.orc_sdfem_delta.restype = D
         L.orc_sdfem_delta.argtypes = [D, D, dp, I]
         L.orc_hconv_brick.restype = D
@@ -145,6 +147,11 @@ class Hierarchy:
         self.L.orc_get_rhs(self.h, l, _dp(a))
         return a
 
+    def dof_nodes(self, l):
+        a = np.zeros(self.info(l)["ndof"], dtype=np.int32)
+        self.L.orc_get_dof_nodes(self.h, l, _ip(a))
+        return a
+
     def set_entries(self, l, e):
         e = np.ascontiguousarray(e, dtype=np.float64)
         self.L.orc_set_entries(self.h, l, _dp(e))
@@ -200,6 +207,13 @@ class Hierarchy:
         u = np.zeros(self.info(l)["ndof"])
         self.L.orc_interp_pulse(self.h, l, C.byref(co), _dp(u))
         return u
+
+    def mcssor_sweep(self, l, omega, f, x):
+        """one multicolour SSOR sweep on level l (returns the new x)"""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        x = np.array(x, dtype=np.float64)
+        self.L.orc_mcssor_sweep(self.h, l, omega, _dp(f), _dp(x))
+        return x
 
     def tcd_run(self, co, sv, t0, tau, nsteps, u0):
         u = np.array(u0, dtype=np.float64)

@@ -66,7 +66,7 @@ class Stats(C.Structure):
 class ServerParams(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_inflight", C.c_int), ("workers", C.c_int),
                 ("keep_resident", C.c_int), ("max_bypass", C.c_int), ("simulate", C.c_int),
-                ("sim_us_per_dof", C.c_double)]
+                ("sim_us_per_dof", C.c_double), ("robust_after", C.c_int)]
 
 
 class Request(C.Structure):
@@ -79,7 +79,7 @@ class Result(C.Structure):
     _fields_ = [("id", C.c_uint64), ("status", C.c_int), ("iterations", C.c_int), ("rel_residual", C.c_double),
                 ("queue_ms", C.c_double), ("run_ms", C.c_double), ("latency_ms", C.c_double),
                 ("device_bytes", C.c_size_t), ("reused_hierarchy", C.c_int), ("n_dof", C.c_int),
-                ("steps_done", C.c_int)]
+                ("steps_done", C.c_int), ("escalated", C.c_int)]
 
 
 class ServerStats(C.Structure):
@@ -367,13 +367,15 @@ class Server:
     concurrently, each on its own resident hierarchy and stream."""
 
     def __init__(self, hbm_budget=0, max_inflight=8, workers=0, keep_resident=True, max_bypass=16,
-                 simulate=False, sim_us_per_dof=0.01):
+                 simulate=False, sim_us_per_dof=0.01, robust_after=None):
         self.L = lib()
         prm = ServerParams()
         self.L.cd3d_server_default_params(C.byref(prm))
         prm.hbm_budget, prm.max_inflight, prm.workers = hbm_budget, max_inflight, workers
         prm.keep_resident, prm.max_bypass = int(keep_resident), max_bypass
         prm.simulate, prm.sim_us_per_dof = int(simulate), sim_us_per_dof
+        if robust_after is not None:
+            prm.robust_after = robust_after
         h = C.c_void_p()
         st = self.L.cd3d_server_create(C.byref(prm), C.byref(h))
         if st != CD3D_OK:

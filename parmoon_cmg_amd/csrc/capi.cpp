@@ -532,9 +532,22 @@ void coarse64(VC &v, bool zero_start) {
   }
 }
 
+// the multicolour SSOR row lists of level l on the device (first use)
+static void ensure_mc(cd3d_problem &P, int l) {
+  DevLevel &D = P.dl[l];
+  if (D.mc_rows) return;
+  HostLevel &H = P.hl[l];
+  if (H.mc_start.empty()) build_mc_colours(H, P.mesh.order);
+  D.mc_rows = dalloc<int>(P, H.mc_rows.size());
+  upload(P, D.mc_rows, H.mc_rows);
+  D.mc_start = H.mc_start;
+  HIPCHK(hipStreamSynchronize(P.stream));
+}
+
 // TMultiGrid3D::Cycle (MultiGrid3D.C:423-669), V-cycle, FP64.  On entry the
 // level's rhs is set; zero_start: the iterate is zero and the Dirichlet part
 // of the rhs is zero, so the first Jacobi sweep needs only the diagonal.
+// Smoother 13 (multicolour SSOR) relaxes L.sol in place.
 void cycle64(VC &v, int i, bool zero_start) {
   cd3d_problem &P = v.P;
   if (i == 0) {
@@ -542,6 +555,19 @@ void cycle64(VC &v, int i, bool zero_start) {
     return;
   }
   DevLevel &L = P.dl[i], &C = P.dl[i - 1];
+  if (v.sp.smoother == 13) {
+    ensure_mc(P, i);
+    if (zero_start) HIPCHK(hipMemsetAsync(L.sol, 0, sizeof(double) * L.ndof, P.stream));
+    for (int k = 0; k < v.sp.pre; k++) k_mcssor(P.stream, L, L.sol, L.rhs, v.sp.omega);
+    if (v.sp.pre == 0 && zero_start) k_copy_range(P.stream, L.nact, L.ndof, L.rhs, L.sol);
+    k_defect(P.stream, L, L.sol, L.rhs, L.aux);
+    k_restrict(P.stream, L, C, L.aux, C.rhs);
+    for (int r = 0; r < v.rec[i]; r++) cycle64(v, i - 1, r == 0);
+    if (v.sp.cycle < 1) v.rec[i] = 1;  // F-cycle
+    k_prolong_update(P.stream, L, C, C.sol, L.sol, v.sp.gmg_damp);
+    for (int k = 0; k < v.sp.post; k++) k_mcssor(P.stream, L, L.sol, L.rhs, v.sp.omega);
+    return;
+  }
   double *x = L.sol, *t = L.tmp;
   if (!L.invdiag_ok) {
     k_invdiag(P.stream, L);
@@ -630,7 +656,7 @@ void precondition(VC &v, const double *r, double *z, bool rhs_dirichlet_zero) {
   v.reset();
   const int top = (int)P.dl.size() - 1;
   DevLevel &F = P.dl[top];
-  if (v.sp.precision == CD3D_MIXED && top > 0 && rhs_dirichlet_zero) {
+  if (v.sp.precision == CD3D_MIXED && top > 0 && rhs_dirichlet_zero && v.sp.smoother == 1) {
     kl_from_dof(P.stream, F, r, F.lf);
     cycleL(v, top, true);
     kl_to_dof(P.stream, F, F.lx, z);
@@ -811,7 +837,9 @@ int fgmres_iterate(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S
 }
 
 void check_params(const cd3d_solver_params &sp) {
-  if (sp.smoother != 1) throw ArgError("smoother: only 1 (Jacobi) is implemented");
+  if (sp.smoother != 1 && sp.smoother != 13)
+    throw ArgError("smoother: 1 (damped Jacobi) or 13 (multicolour SSOR); the reference's sequential SOR/SSOR/ILU "
+                   "(2, 3, 4) have no parallel form here");
   if (sp.coarse_smoother != 17 && sp.coarse_smoother != 3 && sp.coarse_smoother != 1)
     throw ArgError("coarse_smoother must be 17, 3 or 1");
   if (sp.pre < 0 || sp.post < 0 || sp.restart < 1 || sp.maxit < 1) throw ArgError("invalid solver counts");
@@ -1153,7 +1181,7 @@ cd3d_status cd3d_solve(cd3d_problem *p, const cd3d_solver_params *sp, const doub
     double r0 = 0;
     int its;
     const int top = (int)p->dl.size() - 1;
-    if (sp->precision == CD3D_MIXED && top > 0 && (F.lat_exact64 || F.lat_aff64)) {
+    if (sp->precision == CD3D_MIXED && top > 0 && sp->smoother == 1 && (F.lat_exact64 || F.lat_aff64)) {
       if (!p->xl) {
         p->xl = dalloc<double>(*p, N);
         p->bl = dalloc<double>(*p, N);
@@ -1271,7 +1299,7 @@ static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_sol
   // path, so levels above the coarsest get no CSR values.  The coarsest level
   // keeps its CSR system matrix for the exact coarse solve.
   const bool affine_asm = co.conv_kind == 1 && co.f_kind == 0 && !co.supg && !uniform_assembly_off();
-  bool lattice = sp.precision == CD3D_MIXED && Lf > 0 && affine_asm;
+  bool lattice = sp.precision == CD3D_MIXED && Lf > 0 && affine_asm && sp.smoother == 1;
   for (int l = 1; l <= Lf; l++) lattice = lattice && P.dl[l].tcd_mass_uniform && P.dl[l].nclass > 0;
   // AssembleARhs at t_new (SystemTCD3D.C:204-226); with SUPG the A and rhs of
   // MatricesAKRhsAssemble_SUPG (TCD3D.C:102) and its time-derivative matrix
@@ -1358,7 +1386,7 @@ static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_sol
   // order when the FP64 lattice operator is exactly the system matrix
   double r0;
   int its;
-  if (sp.precision == CD3D_MIXED && Lf > 0 && (F.lat_exact64 || F.lat_aff64)) {
+  if (sp.precision == CD3D_MIXED && Lf > 0 && sp.smoother == 1 && (F.lat_exact64 || F.lat_aff64)) {
     if (!P.xl) {
       P.xl = dalloc<double>(P, N);
       P.bl = dalloc<double>(P, N);

@@ -60,6 +60,9 @@ struct HostLevel {
   std::vector<double> r_val;
   std::vector<int> colour_start;  // 9 offsets into colour_cells
   std::vector<int> colour_cells;
+  // multicolour SSOR: active rows grouped by node colour X%k + k(Y%k + k Z%k),
+  // k = 2 (Q1) or 3 (Q2), ascending within a colour
+  std::vector<int> mc_start, mc_rows;
   // lattice-stencil form
   std::vector<LatClass> lat;
   int64_t lat_nval = 0;             // sum over classes of nsten * nodes
@@ -88,6 +91,9 @@ struct DevLevel {
   int *p_rowptr = nullptr, *p_col = nullptr, *r_rowptr = nullptr, *r_col = nullptr;
   double *p_val = nullptr, *r_val = nullptr;
   int64_t p_nnz = 0, r_nnz = 0;
+  // multicolour SSOR row lists (uploaded on first use)
+  int *mc_rows = nullptr;
+  std::vector<int> mc_start;
   // assembly
   int4 *cells = nullptr;
   int *slots = nullptr;
@@ -191,6 +197,7 @@ void build_transfer(HostLevel &fine, const HostLevel &coarse, int order);
 void build_slots(const HostLevel &L, int order, std::vector<int> &slots);
 void build_colours(HostLevel &L, int order);
 void build_row_blocks(const std::vector<int> &rowptr, int max_nnz, int max_rows, std::vector<int> &blk);
+void build_mc_colours(HostLevel &L, int order);
 void build_lattice(HostLevel &L, int order);
 void basis_tables(int order, std::vector<double> &tab);
 int quad_points_1d(int order);
@@ -228,6 +235,9 @@ void k_spmv(hipStream_t s, const DevLevel &L, const double *x, double *y);
 void k_defect(hipStream_t s, const DevLevel &L, const double *x, const double *f, double *d);
 void k_jacobi(hipStream_t s, const DevLevel &L, const double *x, const double *f, double *out, double omega);
 void k_jacobi_zero(hipStream_t s, const DevLevel &L, const double *f, double *out, double omega);
+// one multicolour SSOR sweep in place (smoother 13): x_D = f_D, relaxed
+// Gauss-Seidel colour by colour forward, then backward (needs mc_rows)
+void k_mcssor(hipStream_t s, const DevLevel &L, double *x, const double *f, double omega);
 void k_invdiag(hipStream_t s, const DevLevel &L);
 void k_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, const double *fine, double *coarse);
 void k_prolong_update(hipStream_t s, const DevLevel &F, const DevLevel &C, const double *coarse, double *x,

@@ -83,6 +83,41 @@ __global__ __launch_bounds__(kBlock) void csr_rows(int n, int nact, const int *_
   else if (MODE == kProlongUpd) out[row] += omega * s;
 }
 
+// Relaxed Gauss-Seidel over the rows of one colour of the multicolour SSOR
+// smoother (rows[0..nrows)): rows of one colour never couple, so every row
+// is updated in place from the current x of the other colours,
+//   x_i = (1 - w) x_i + w (f_i - sum_{j != i} a_ij x_j) / a_ii.
+template <int LANES, int UNROLL>
+__global__ __launch_bounds__(kBlock) void colour_sor_k(int nrows, const int *__restrict__ rows,
+                                                       const int *__restrict__ rowptr, const int *__restrict__ col,
+                                                       const double *__restrict__ val, double *x,
+                                                       const double *__restrict__ f, double omega) {
+  const int gid = blockIdx.x * kBlock + threadIdx.x;
+  const int k = gid / LANES, lane = gid % LANES;
+  if (k >= nrows) return;
+  const int row = rows[k];
+  const int b = rowptr[row], e = rowptr[row + 1];
+  double s = 0.0, diag = 0.0;
+  for (int j0 = b + lane; j0 < e; j0 += LANES * UNROLL) {
+    int c[UNROLL];
+    double a[UNROLL];
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++) {
+      const int j = j0 + u * LANES;
+      c[u] = j < e ? col[j] : row;
+      a[u] = j < e ? val[j] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < UNROLL; u++) {
+      if (c[u] == row) diag += a[u];
+      else s += a[u] * x[c[u]];
+    }
+  }
+  s = group_sum<LANES>(s);
+  diag = group_sum<LANES>(diag);
+  if (lane == 0) x[row] = (1.0 - omega) * x[row] + omega * (f[row] - s) / diag;
+}
+
 // lanes per row from the mean row length (overridable for tuning through
 // CD3D_CSR_LANES / CD3D_CSR_UNROLL)
 inline void row_shape(int64_t nnz, int n, int &lanes, int &unroll) {
@@ -1232,6 +1267,26 @@ void k_matadd_into(hipStream_t s, int64_t n, double *C, const double *D, double 
 void k_matvec_active(hipStream_t s, const DevLevel &L, const double *x, double *y) {
   launch_rows<kActiveSpmv, double, double>(s, L.nnz, L.ndof, L.nact, L.rowptr, L.col, L.val, x,
                                            nullptr, y, 0.0);
+}
+void k_mcssor(hipStream_t s, const DevLevel &L, double *x, const double *f, double omega) {
+  // x_D = f_D, then the colours forward and backward
+  k_copy_range(s, L.nact, L.ndof, f, x);
+  const int ncol = (int)L.mc_start.size() - 1;
+  int lanes, unroll;
+  row_shape(L.nnz, L.ndof, lanes, unroll);
+  for (int t = 0; t < 2 * ncol; t++) {
+    const int c = t < ncol ? t : 2 * ncol - 1 - t;
+    const int n = L.mc_start[c + 1] - L.mc_start[c];
+    if (n <= 0) continue;
+    const int *rows = L.mc_rows + L.mc_start[c];
+    const int grid = (int)(((int64_t)n * lanes + kBlock - 1) / kBlock);
+    switch (lanes) {
+      case 4: colour_sor_k<4, 1><<<grid, kBlock, 0, s>>>(n, rows, L.rowptr, L.col, L.val, x, f, omega); break;
+      case 8: colour_sor_k<8, 4><<<grid, kBlock, 0, s>>>(n, rows, L.rowptr, L.col, L.val, x, f, omega); break;
+      case 16: colour_sor_k<16, 4><<<grid, kBlock, 0, s>>>(n, rows, L.rowptr, L.col, L.val, x, f, omega); break;
+      default: colour_sor_k<32, 4><<<grid, kBlock, 0, s>>>(n, rows, L.rowptr, L.col, L.val, x, f, omega); break;
+    }
+  }
 }
 void k_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, const double *fine, double *coarse) {
   launch_rows<kRestrict, double, double>(s, F.r_nnz, C.ndof, C.nact, F.r_rowptr, F.r_col,

@@ -194,10 +194,34 @@ struct cd3d_server {
     }
     cd3d_stats s{};
     if (r.kind == CD3D_REQ_CD) {
+      // Jacobi-smoothed solves of convection-dominated problems can stall
+      // far from the tolerance (the reference's algorithm does too): after
+      // robust_after iterations such a solve continues from its iterate with
+      // the multicolour SSOR smoother, to the same tolerance relative to the
+      // request's initial residual and within its iteration budget
+      cd3d_solver_params sp = r.solver;
+      const bool robust = prm.robust_after > 0 && sp.smoother == 1 && sp.maxit > prm.robust_after;
+      if (robust) sp.maxit = prm.robust_after;
       st = cd3d_assemble(h, &r.coeffs);
-      if (st == CD3D_OK) st = cd3d_solve(h, &r.solver, nullptr, nullptr, 0, &s);
+      if (st == CD3D_OK) st = cd3d_solve(h, &sp, nullptr, nullptr, 0, &s);
       res.iterations = s.iterations;
       res.rel_residual = s.r0 > 0 ? s.rfinal / s.r0 : 0.0;
+      if (robust && st == CD3D_NOT_CONVERGED && s.rfinal > 0) {
+        std::vector<double> x(res.n_dof);
+        st = cd3d_get_solution(h, x.data());
+        cd3d_solver_params sp2 = r.solver;
+        sp2.smoother = 13;
+        sp2.omega = 1.0;
+        sp2.pre = sp2.post = 2;
+        sp2.maxit = r.solver.maxit - s.iterations;
+        if (r.solver.rel_tol > 0) sp2.rel_tol = r.solver.rel_tol * s.r0 / s.rfinal;
+        const double r0 = s.r0;
+        cd3d_stats s2{};
+        if (st == CD3D_OK) st = cd3d_solve(h, &sp2, nullptr, x.data(), 1, &s2);
+        res.iterations += s2.iterations;
+        res.rel_residual = r0 > 0 ? s2.rfinal / r0 : 0.0;
+        res.escalated = 1;
+      }
       if ((st == CD3D_OK || st == CD3D_NOT_CONVERGED) && r.want_solution) {
         d.sol.resize(res.n_dof);
         const cd3d_status g = cd3d_get_solution(h, d.sol.data());
@@ -295,6 +319,7 @@ void cd3d_server_default_params(cd3d_server_params *p) {
   p->max_bypass = 16;
   p->simulate = 0;
   p->sim_us_per_dof = 0.01;
+  p->robust_after = 200;
 }
 
 cd3d_status cd3d_server_create(const cd3d_server_params *prm, cd3d_server **out) {

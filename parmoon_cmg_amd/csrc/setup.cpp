@@ -364,6 +364,24 @@ void build_colours(HostLevel &L, int order) {
   for (int c = 0; c < L.ncell; c++) L.colour_cells[pos[col[c]]++] = c;
 }
 
+// Row lists of the multicolour SSOR smoother: every active row by the colour
+// of its lattice node, X%k + k (Y%k + k Z%k) with k = order + 1; rows of one
+// colour never share a cell, so their matrix rows never couple.
+void build_mc_colours(HostLevel &L, int order) {
+  const int k = order + 1, ncol = k * k * k, nn = L.nn;
+  std::vector<int> colour(L.nact);
+  L.mc_start.assign(ncol + 1, 0);
+  for (int i = 0; i < L.nact; i++) {
+    const int node = L.dof_node[i], X = node % nn, Y = (node / nn) % nn, Z = node / (nn * nn);
+    colour[i] = X % k + k * (Y % k + k * (Z % k));
+    L.mc_start[colour[i] + 1]++;
+  }
+  for (int c = 0; c < ncol; c++) L.mc_start[c + 1] += L.mc_start[c];
+  L.mc_rows.resize(L.nact);
+  std::vector<int> pos(L.mc_start.begin(), L.mc_start.end() - 1);
+  for (int i = 0; i < L.nact; i++) L.mc_rows[pos[colour[i]]++] = i;
+}
+
 // Reference basis at the quadrature points, per point q:
 //   [w_q, xi, eta, zeta, then per basis b: phi, dphi/dxi, dphi/deta, dphi/dzeta]
 // Gauss rules in the point order of QuadFormulaHexa.C:62-120; tensor-product

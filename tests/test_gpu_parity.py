@@ -519,3 +519,47 @@ def test_kernel_bench_covers_every_kernel():
     assert np.array_equal(P.csr(l)[2], e0)
     with pytest.raises(pm.CD3DError):
         P.kernel_bench(9, l, 1)
+
+
+# multicolour SSOR smoother (13): the HIP V-cycle and FGMRES against the
+# oracle's statement of it (itself pinned against its numpy definition), on a
+# moderate case and on a convection-dominated Q2 problem (eps 2.7e-6 on the
+# 5^3-brick mesh, 20^3 cells) where the damped Jacobi smoother stalls near 4e-3
+MC_CASES = {
+    "m2_q1": (1, 2, 1, 3, dict(eps=1e-2, b=(0.3, 0.4, -0.5), f_val=1.0, supg=1)),
+    "m2_q2": (2, 2, 0, 3, dict(eps=1e-3, b=(0.6, 0.5, -0.4), f_val=1.0, supg=1)),
+    "m5_q2_pe600": (2, 5, 0, 3, dict(eps=2.6659e-06, b=(0.039539, -0.036416, -0.032403), f_val=1.0, supg=1)),
+}
+MC_SOLVER = dict(smoother=13, omega=1.0, pre=2, post=2, coarse_smoother=17, restart=20, maxit=400)
+
+
+@pytest.mark.parametrize("name", list(MC_CASES))
+def test_multicolour_ssor_vcycle_and_fgmres_match_oracle(name):
+    order, m, k0, levels, co = MC_CASES[name]
+    say(f"[gpu] multicolour SSOR {name}")
+    P = pm.Problem(order, m, k0, levels, box=(1.0, 1.0, 1.0))
+    P.assemble(pm.coeffs(**co))
+    H = po.Hierarchy(order, m, k0, levels, box=(1.0, 1.0, 1.0))
+    H.assemble(po.coeffs(**co))
+    n = H.info(levels - 1)["ndof"]
+    na = H.info(levels - 1)["nact"]
+    # the oracle's solve first: its V-cycle works in the finest level's rhs array
+    b = H.rhs(levels - 1)
+    _, r0 = H.defect(levels - 1, np.zeros_like(b), b)
+    xo, its_o, _ = H.fgmres(po.solver(**dict(MC_SOLVER, res_norm_min=1e-10 * r0, red_factor=0.0)), b)
+    r = np.random.default_rng(11).standard_normal(n)
+    r[na:] = 0.0
+    z = P.vcycle(pm.solver_params(**MC_SOLVER), r)
+    zo = H.vcycle(po.solver(**MC_SOLVER), r)
+    verr = np.abs(z - zo).max() / np.abs(zo).max()
+    for prec, tol in ((pm.FP64, SOL_TOL), (pm.MIXED, MIXED_TOL)):
+        x, st = P.solve(pm.solver_params(**MC_SOLVER, rel_tol=1e-10, precision=prec))
+        err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+        say(f"[gpu] {name} precision {prec}: vcycle err {verr:.2e}, its {st.iterations} (oracle {its_o}), "
+            f"sol err {err:.2e}, rel res {st.rfinal / st.r0:.2e}")
+        assert st.status == 0 and st.rfinal <= 1.0001e-10 * st.r0
+        assert err < tol
+        if prec == pm.FP64:
+            assert st.iterations == its_o
+    assert verr < 1e-12
+    P.close()

@@ -213,3 +213,41 @@ def test_rel_tol_is_the_absolute_threshold_of_the_first_residual():
     # the restart-relative test of the reference asks for 1e-10 again at every restart
     _, ic, _ = H.fgmres(po.solver(**base, res_norm_min=0.0, red_factor=1e-10), b)
     assert ic > ia
+
+
+@pytest.mark.parametrize("order,m,k0,levels", [(1, 2, 1, 2), (2, 2, 0, 2), (2, 1, 1, 2)])
+def test_multicolour_ssor_matches_definition(order, m, k0, levels):
+    # smoother 13 (this build's own): the oracle's sweep against a numpy
+    # statement of the definition -- x_D = f_D; relaxed Gauss-Seidel colour by
+    # colour (colour = X%k + k (Y%k + k Z%k), k = 2 for Q1, 3 for Q2), forward
+    # then backward -- and the property the parallel sweep rests on: rows of
+    # one colour never couple
+    import scipy.sparse as sps
+    H = po.Hierarchy(order, m, k0, levels, box=(1.0, 1.0, 1.0))
+    H.assemble(po.coeffs(eps=1e-3, b=(0.7, -0.4, 0.5), c=0.3, f_val=1.0, supg=1))
+    l = levels - 1
+    rp, kc, e = H.csr(l)
+    n, na = len(rp) - 1, H.info(l)["nact"]
+    A = sps.csr_matrix((e, kc, rp), shape=(n, n))
+    nn = order * m * 2 ** (k0 + l) + 1
+    node = H.dof_nodes(l)
+    k = 2 if order == 1 else 3
+    X, Y, Z = node % nn, (node // nn) % nn, node // (nn * nn)
+    col = X % k + k * (Y % k + k * (Z % k))
+    ncol = k ** 3
+    rows_of = [np.nonzero((col == c) & (np.arange(n) < na))[0] for c in range(ncol)]
+    for rows in rows_of:  # rows of one colour never couple
+        sub = A[rows][:, rows]
+        assert sub.nnz == len(rows) and np.all(sub.diagonal() != 0)
+    rng = np.random.default_rng(3)
+    f, x0 = rng.standard_normal(n), rng.standard_normal(n)
+    omega = 0.9
+    d = A.diagonal()
+    x = x0.copy()
+    x[na:] = f[na:]
+    for c in list(range(ncol)) + list(range(ncol - 1, -1, -1)):
+        r = rows_of[c]
+        s = f[r] - A[r] @ x + d[r] * x[r]
+        x[r] = (1 - omega) * x[r] + omega * s / d[r]
+    xo = H.mcssor_sweep(l, omega, f, x0)
+    assert np.abs(xo - x).max() <= 1e-12 * np.abs(x).max()

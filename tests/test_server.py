@@ -255,3 +255,29 @@ def test_workload_draws_match_the_oracle(precision):
 
 
 SOL_TOL_FP64 = 1e-10
+
+
+@pytest.mark.gpu
+def test_stalled_jacobi_solve_is_continued_with_multicolour_ssor():
+    # a convection-dominated serving request (Q2 SUPG, eps 2.7e-6, 20^3 cells
+    # on 5^3 macro bricks) on which damped Jacobi stalls near 4e-3: after
+    # robust_after iterations the server continues with the multicolour SSOR
+    # smoother; the returned solution meets 1e-10 against the oracle's matrix
+    import numpy as np
+    from oracle import pyoracle as po
+    co = dict(eps=2.6659e-06, b=(0.039539, -0.036416, -0.032403), f_val=1.0, supg=1)
+    srv = pm.Server(max_inflight=2, robust_after=100)
+    sp = pm.solver_params(omega=0.5, pre=3, post=3, coarse_smoother=17, restart=20, maxit=1000, rel_tol=1e-10,
+                          precision=pm.MIXED)
+    i = srv.submit(pm.mesh_desc(2, 5, 0, 3), pm.coeffs(**co), sp, want_solution=True)
+    res, x = srv.wait(i, n_dof=41 ** 3)
+    srv.close()
+    H = po.Hierarchy(2, 5, 0, 3, box=(1.0, 1.0, 1.0))
+    H.assemble(po.coeffs(**co))
+    b = H.rhs(2)
+    _, r0 = H.defect(2, np.zeros_like(b), b)
+    _, rf = H.defect(2, x, b)
+    print(f"[gpu] escalated {res.escalated}, its {res.iterations}, rel res {res.rel_residual:.2e}, "
+          f"oracle-matrix rel res {rf / r0:.2e}", flush=True)
+    assert res.status == 0 and res.escalated == 1
+    assert rf <= 1.01e-10 * r0
