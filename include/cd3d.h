@@ -186,6 +186,31 @@ cd3d_status cd3d_tcd_get_state(cd3d_problem *p, double *u_host);
 /* nodal interpolation of the Gaussian pulse (TFEFunction3D::Interpolate, FEFunction3D.C:1446) */
 cd3d_status cd3d_interp_pulse(cd3d_problem *p, int level, const cd3d_coeffs *co, double *u_host);
 
+/* Diagnostics and output.  Known functions fn: 0 zero, 1 the sine
+ * sin(pi x) sin(pi y) sin(pi z) of Examples/CD_3D/Laplace.h:12-19 (value and
+ * gradient), 2 the Gaussian pulse centred at co->pulse_c (sigma 0.05; co may
+ * be NULL for fn 0 and 1). */
+/* TFEFunction3D::Interpolate (FEFunction3D.C:1446): nodal interpolation of fn
+ * on level l, evaluated on the device */
+cd3d_status cd3d_interpolate(cd3d_problem *p, int level, int fn, const cd3d_coeffs *co, double *u_host);
+/* TFEFunction3D::GetErrors with L2H1Errors (FEFunction3D.C:57,
+ * MainUtilities.C:1623) on the device, at the quadrature points of the
+ * assembly: errors[0] = |u - fn|_L2, errors[1] = |u - fn|_H1 (semi-norm),
+ * errors[2] = max |u - fn| at the last cell's points (what the reference
+ * leaves in its third entry), errors[3] = max |u - fn| at all points.
+ * u_host NULL: the level's current solution on the device (finest level: the
+ * last cd3d_solve, or the TCD3D state after cd3d_tcd_init). */
+cd3d_status cd3d_errors(cd3d_problem *p, int level, const double *u_host, int fn, const cd3d_coeffs *co,
+                        double *errors);
+/* TOutput3D::WriteVtk (Output3D.C:1747): legacy ASCII VTK of level l with the
+ * scalar u_host as point data `name` (the FE function at the cell vertices,
+ * averaged over the cells, as the reference writes it); vertices are numbered
+ * along the lattice (x fastest), not in the reference's pointer order */
+cd3d_status cd3d_write_vtk(cd3d_problem *p, int level, const double *u_host, const char *name, const char *path);
+/* TSquareMatrix::Write (SquareMatrix.C:156): binary file of int32 n_rows,
+ * n_cols, nnz, the row pointers, column indices and FP64 entries of level l */
+cd3d_status cd3d_write_matrix(cd3d_problem *p, int level, const char *path);
+
 /* Measurement: launch one kernel of the path `reps` times on the handle's
  * stream between two HIP events (after one untimed launch) and return the
  * mean duration per launch and the kernel's algorithmic bytes per launch.

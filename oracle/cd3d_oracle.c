@@ -893,6 +893,91 @@ void orc_interp_pulse(const orc_hier *h, int l, const orc_coeffs *co, double *u)
   }
 }
 
+/* ------------------------------------------------------------------ diagnostics */
+
+/* Known functions: 0 zero; 1 the sine of Examples/CD_3D/Laplace.h:12-19
+ * (value and gradient); 2 the Gaussian pulse (centre co->pulse_c, sigma 0.05). */
+static void known_fn(int fn, const orc_coeffs *co, double x, double y, double z, double *v) {
+  v[0] = v[1] = v[2] = v[3] = 0;
+  if (fn == 1) {
+    v[0] = sin(M_PI * x) * sin(M_PI * y) * sin(M_PI * z);
+    v[1] = M_PI * cos(M_PI * x) * sin(M_PI * y) * sin(M_PI * z);
+    v[2] = M_PI * sin(M_PI * x) * cos(M_PI * y) * sin(M_PI * z);
+    v[3] = M_PI * sin(M_PI * x) * sin(M_PI * y) * cos(M_PI * z);
+  } else if (fn == 2) {
+    const double *pc = co->pulse_c, s2 = 0.05 * 0.05;
+    double dx = x - pc[0], dy = y - pc[1], dz = z - pc[2];
+    v[0] = exp(-(dx * dx + dy * dy + dz * dz) / (2 * s2));
+    v[1] = -dx / s2 * v[0];
+    v[2] = -dy / s2 * v[0];
+    v[3] = -dz / s2 * v[0];
+  }
+}
+
+/* TFEFunction3D::Interpolate (FEFunction3D.C:1446-1560) for Lagrange Q1/Q2:
+ * the nodal functionals are point values at the nodes. */
+void orc_interpolate(const orc_hier *h, int l, int fn, const orc_coeffs *co, double *u) {
+  const level_t *L = h->lv + l;
+  double dn = 1.0 / (h->order * L->n);
+  for (int d = 0; d < L->ndof; d++) {
+    int node = L->dof_node[d];
+    double v[4];
+    known_fn(fn, co, (node % L->nn) * (h->box[0] * dn), ((node / L->nn) % L->nn) * (h->box[1] * dn),
+             (node / (L->nn * L->nn)) * (h->box[2] * dn), v);
+    u[d] = v[0];
+  }
+}
+
+/* TFEFunction3D::GetErrors with L2H1Errors (FEFunction3D.C:57-240,
+ * MainUtilities.C:1623-1653): per cell, at the quadrature points of the
+ * assembly, e = u_h - u and grad e; err[0] = sqrt(sum w|J| e^2), err[1] =
+ * sqrt(sum w|J| |grad e|^2), err[2] = max |e| over the points of the LAST
+ * cell (the reference resets that maximum for every cell, :211-219), err[3]
+ * = max |e| over all points. */
+void orc_errors(const orc_hier *h, int l, const double *u, int fn, const orc_coeffs *co, double *err) {
+  const level_t *L = h->lv + l;
+  int p = h->order, nb = h->nb, n1 = p + 2;
+  const double *gx = p == 1 ? G3X : G4X, *gw = p == 1 ? G3W : G4W;
+  double hd[3], dn[3];
+  for (int d = 0; d < 3; d++) { hd[d] = h->box[d] / L->n; dn[d] = h->box[d] / (p * L->n); }
+  double detj = (hd[0] / 2) * (hd[1] / 2) * (hd[2] / 2);
+  double e0 = 0, e1 = 0, emax = 0, elast = 0;
+  for (int c = 0; c < L->ncell; c++) {
+    const int *O = L->org + 3 * c, *E = L->ax + 9 * c, *g = L->gn + c * nb;
+    double cellmax = 0;
+    for (int qi = 0; qi < n1; qi++)
+      for (int qj = 0; qj < n1; qj++)
+        for (int qk = 0; qk < n1; qk++) {
+          double xi = gx[qk], eta = gx[qj], zeta = gx[qi];
+          double w = gw[qi] * gw[qj] * gw[qk] * detj;
+          double ref[3] = {xi, eta, zeta}, X[3];
+          for (int d = 0; d < 3; d++)
+            X[d] = O[d] * dn[d] + (ref[0] + 1) / 2 * hd[d] * E[d] + (ref[1] + 1) / 2 * hd[d] * E[3 + d] +
+                   (ref[2] + 1) / 2 * hd[d] * E[6 + d];
+          double phi[MAXNB], d0[MAXNB], d1[MAXNB], d2[MAXNB];
+          basis3d(p, xi, eta, zeta, phi, d0, d1, d2);
+          double uh = 0, gr[3] = {0, 0, 0};
+          for (int b = 0; b < nb; b++) {
+            double ub = u[g[b]];
+            uh += ub * phi[b];
+            for (int d = 0; d < 3; d++) gr[d] += ub * (2 / hd[d]) * (E[d] * d0[b] + E[3 + d] * d1[b] + E[6 + d] * d2[b]);
+          }
+          double v[4];
+          known_fn(fn, co, X[0], X[1], X[2], v);
+          double t = uh - v[0];
+          e0 += w * t * t;
+          for (int d = 0; d < 3; d++) e1 += w * (gr[d] - v[1 + d]) * (gr[d] - v[1 + d]);
+          if (fabs(t) > cellmax) cellmax = fabs(t);
+        }
+    if (cellmax > emax) emax = cellmax;
+    elast = cellmax;
+  }
+  err[0] = sqrt(e0);
+  err[1] = sqrt(e1);
+  err[2] = elast;
+  err[3] = emax;
+}
+
 /* MatAdd, active rows only (Blas2.C:132-150) */
 static void matadd(level_t *L, double *C, const double *D, double alpha) {
   long n = L->rowptr[L->nact];

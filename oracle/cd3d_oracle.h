@@ -93,6 +93,14 @@ int orc_tcd_run(orc_hier *h, const orc_coeffs *co, const orc_solver *s, double t
 /* nodal interpolation of the Gaussian pulse initial condition on level l */
 void orc_interp_pulse(const orc_hier *h, int l, const orc_coeffs *co, double *u);
 
+/* known functions: 0 zero, 1 sin(pi x) sin(pi y) sin(pi z), 2 the Gaussian pulse at co->pulse_c */
+/* nodal interpolation on level l (TFEFunction3D::Interpolate) */
+void orc_interpolate(const orc_hier *h, int l, int fn, const orc_coeffs *co, double *u);
+/* TFEFunction3D::GetErrors with L2H1Errors: err[0] L2, err[1] H1 semi-norm of
+ * u - fn, err[2] max |u - fn| at the last cell's quadrature points, err[3]
+ * the same over every cell */
+void orc_errors(const orc_hier *h, int l, const double *u, int fn, const orc_coeffs *co, double *err);
+
 /* one multicolour SSOR sweep (smoother 13) on level l: x in/out, f the rhs */
 void orc_mcssor_sweep(orc_hier *h, int l, double omega, const double *f, double *x);
 

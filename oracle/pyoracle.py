@@ -86,6 +86,8 @@ def lib():
         L.orc_tcd_run.argtypes = [P, C.POINTER(Coeffs), C.POINTER(Solver), D, D, I, dp, dp]
         L.orc_interp_pulse.argtypes = [P, I, C.POINTER(Coeffs), dp]
         L.orc_mcssor_sweep.argtypes = [P, I, D, dp, dp]
+        L.orc_interpolate.argtypes = [P, I, I, C.POINTER(Coeffs), dp]
+        L.orc_errors.argtypes = [P, I, dp, I, C.POINTER(Coeffs), dp]
         L.orc_sdfem_delta.restype = D
         L.orc_sdfem_delta.argtypes = [D, D, dp, I]
         L.orc_hconv_brick.restype = D
@@ -207,6 +209,17 @@ class Hierarchy:
         u = np.zeros(self.info(l)["ndof"])
         self.L.orc_interp_pulse(self.h, l, C.byref(co), _dp(u))
         return u
+
+    def interpolate(self, l, fn, co=None):
+        u = np.zeros(self.info(l)["ndof"])
+        self.L.orc_interpolate(self.h, l, fn, C.byref(co or coeffs()), _dp(u))
+        return u
+
+    def errors(self, l, u, fn, co=None):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        e = np.zeros(4)
+        self.L.orc_errors(self.h, l, _dp(u), fn, C.byref(co or coeffs()), _dp(e))
+        return e
 
     def mcssor_sweep(self, l, omega, f, x):
         """one multicolour SSOR sweep on level l (returns the new x)"""

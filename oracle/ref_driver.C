@@ -58,6 +58,7 @@ void MatricesAKRhsAssemble_SUPG(double, double *, double *, double, double **, i
 #include <It_Finest.h>
 #include <TimeDiscRout.h>
 #include <FEFunction3D.h>
+#include <Output3D.h>
 
 #include <chrono>
 #include <cmath>
@@ -120,6 +121,15 @@ void DrvCoeffs(int n_points, double *X, double *Y, double *Z, double **parameter
     }
     co[6] = sqrt(b1 * b1 + b2 * b2 + b3 * b3);
   }
+}
+
+// the exact solution of Examples/CD_3D/Laplace.h (value, gradient, Laplacian)
+void DrvExact(double x, double y, double z, double *values) {
+  values[0] = sin(Pi * x) * sin(Pi * y) * sin(Pi * z);
+  values[1] = Pi * cos(Pi * x) * sin(Pi * y) * sin(Pi * z);
+  values[2] = Pi * sin(Pi * x) * cos(Pi * y) * sin(Pi * z);
+  values[3] = Pi * sin(Pi * x) * sin(Pi * y) * cos(Pi * z);
+  values[4] = -3 * Pi * Pi * sin(Pi * x) * sin(Pi * y) * sin(Pi * z);
 }
 
 void DrvBoundCondition(int, double, double, double, BoundCond &cond) { cond = DIRICHLET; }
@@ -399,6 +409,37 @@ int main(int argc, char **argv) {
   ScalarDefect(A[L - 1], x.data(), b.data(), r.data(), rf);
   dump("sol.f64", x.data(), (size_t)N * 8);
   printf("SOLVE its %d r0 %.17e rfinal %.17e rel %.17e\n", its, r0, rf, rf / r0);
+
+  // ---- diagnostics and output of the solution (CD3D_ParMooN.C:496-520):
+  // L2/H1 errors against the sine of Laplace.h (TFEFunction3D::GetErrors with
+  // L2H1Errors), the nodal interpolant of the sine and its errors
+  // (Interpolate), the VTK file (TOutput3D::WriteVtk) and the binary matrix
+  // (TSquareMatrix::Write) of the finest level
+  {
+    char nm[] = "u", ds[] = "solution";
+    static MultiIndex3D AllDer[4] = {D000, D100, D010, D001};
+    TFESpace3D *fespL[1] = {sp[L - 1]};
+    TAuxParam3D *auxE = new TAuxParam3D(1, 0, 0, 0, fespL, NULL, NULL, NULL, NULL, 0, NULL);
+    TFEFunction3D *fsol = new TFEFunction3D(sp[L - 1], nm, ds, x.data(), N);
+    double err[4] = {0, 0, 0, 0};
+    fsol->GetErrors(DrvExact, 4, AllDer, 2, L2H1Errors, DrvCoeffs, auxE, 1, fespL, err);
+    dump("errors_sol.f64", err, 3 * 8);
+    std::vector<double> ui(N, 0.0);
+    TFEFunction3D *fint = new TFEFunction3D(sp[L - 1], nm, ds, ui.data(), N);
+    fint->Interpolate(DrvExact);
+    dump("interp_sine.f64", ui.data(), (size_t)N * 8);
+    double erri[4] = {0, 0, 0, 0};
+    fint->GetErrors(DrvExact, 4, AllDer, 2, L2H1Errors, DrvCoeffs, auxE, 1, fespL, erri);
+    dump("errors_interp.f64", erri, 3 * 8);
+    printf("ERRORS sol %.17e %.17e %.17e interp %.17e %.17e %.17e\n", err[0], err[1], err[2], erri[0], erri[1],
+           erri[2]);
+    if (g_out != "-") {
+      TOutput3D *out = new TOutput3D(1, 1, 0, 0, Domain, sp[L - 1]->GetCollection());
+      out->AddFEFunction(fsol);
+      out->WriteVtk((g_out + "/sol.vtk").c_str());
+      A[L - 1]->Write((g_out + "/matrix.bin").c_str());
+    }
+  }
   printf("TIME space %.6f struct %.6f assemble %.6f solve %.6f\n", t_space, t_struct, t_asm, t_solve);
   return 0;
 }

@@ -27,7 +27,8 @@ EXPORTS = ["cd3d_default_solver", "cd3d_create", "cd3d_destroy", "cd3d_last_erro
            "cd3d_prolongate",
            "cd3d_restrict", "cd3d_vcycle", "cd3d_solve", "cd3d_get_solution", "cd3d_tcd_run",
            "cd3d_tcd_init", "cd3d_tcd_step", "cd3d_tcd_get_state",
-           "cd3d_interp_pulse", "cd3d_kernel_bench", "cd3d_device_ptr", "cd3d_stream", "cd3d_synchronize",
+           "cd3d_interp_pulse", "cd3d_interpolate", "cd3d_errors", "cd3d_write_vtk", "cd3d_write_matrix",
+           "cd3d_kernel_bench", "cd3d_device_ptr", "cd3d_stream", "cd3d_synchronize",
            # cd3d_server.h
            "cd3d_server_default_params", "cd3d_estimate_bytes", "cd3d_server_create", "cd3d_server_submit",
            "cd3d_server_wait", "cd3d_server_drain", "cd3d_server_get_stats", "cd3d_server_destroy"]
@@ -134,6 +135,10 @@ def lib():
     L.cd3d_interp_pulse.argtypes = [P, I, C.POINTER(Coeffs), dp]
     L.cd3d_device_ptr.argtypes = [P, I, I, C.POINTER(C.c_void_p)]
     L.cd3d_kernel_bench.argtypes = [P, I, I, I, dp, dp]
+    L.cd3d_interpolate.argtypes = [P, I, I, C.POINTER(Coeffs), dp]
+    L.cd3d_errors.argtypes = [P, I, dp, I, C.POINTER(Coeffs), dp]
+    L.cd3d_write_vtk.argtypes = [P, I, dp, C.c_char_p, C.c_char_p]
+    L.cd3d_write_matrix.argtypes = [P, I, C.c_char_p]
     L.cd3d_stream.argtypes = [P]
     L.cd3d_stream.restype = C.c_void_p
     L.cd3d_synchronize.argtypes = [P]
@@ -334,6 +339,31 @@ class Problem:
 
     KERNELS = {0: "csr_stream<kSpmv> FP64", 1: "lat_sweep_k<kLJacobi> FP32", 2: "lat64_k<kL64Spmv> FP64",
                3: "lat_restrict_k FP32", 4: "lat_prolong_k FP32", 5: "csr_fill_k", 6: "copy_k FP64"}
+
+    # known functions of interpolate / errors: zero, the sine of Laplace.h, the Gaussian pulse
+    FN_ZERO, FN_SINE, FN_PULSE = 0, 1, 2
+
+    def interpolate(self, l, fn, co=None):
+        """Nodal interpolation of a known function on level l (on the device)."""
+        u = np.zeros(self.info(l)["ndof"])
+        self._chk(self.L.cd3d_interpolate(self.h, l, fn, None if co is None else C.byref(co), _dp(u)), "interpolate")
+        return u
+
+    def errors(self, l, fn, u=None, co=None):
+        """[L2, H1 semi-norm, last cell's max, max] of u - fn (u None: the
+        level's current solution on the device)."""
+        e = np.zeros(4)
+        uu = None if u is None else np.ascontiguousarray(u, dtype=np.float64)
+        self._chk(self.L.cd3d_errors(self.h, l, None if uu is None else _dp(uu), fn,
+                                     None if co is None else C.byref(co), _dp(e)), "errors")
+        return e
+
+    def write_vtk(self, l, u, path, name="u"):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        self._chk(self.L.cd3d_write_vtk(self.h, l, _dp(u), name.encode(), str(path).encode()), "write_vtk")
+
+    def write_matrix(self, l, path):
+        self._chk(self.L.cd3d_write_matrix(self.h, l, str(path).encode()), "write_matrix")
 
     def kernel_bench(self, which, level, reps=50):
         """(ms per launch, algorithmic bytes per launch) of one kernel of the path,

@@ -1482,6 +1482,144 @@ cd3d_status cd3d_tcd_run(cd3d_problem *p, const cd3d_coeffs *co, const cd3d_solv
   return rc != CD3D_OK ? rc : solve_status;
 }
 
+// ---------------------------------------------------------------- diagnostics and output
+
+static FnParams fn_params(int fn, const cd3d_coeffs *co) {
+  if (fn < 0 || fn > 2) throw ArgError("fn must be 0 (zero), 1 (sine) or 2 (pulse)");
+  FnParams f;
+  f.fn = fn;
+  if (co)
+    for (int d = 0; d < 3; d++) f.pc[d] = co->pulse_c[d];
+  return f;
+}
+
+cd3d_status cd3d_interpolate(cd3d_problem *p, int level, int fn, const cd3d_coeffs *co, double *u_host) {
+  if (!p || !u_host || level < 0 || level >= (int)p->dl.size()) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  return run(p, [&]() {
+    DevLevel &D = p->dl[level];
+    k_interpolate(p->stream, D, p->mesh.box, p->mesh.order, p->hl[level].n, fn_params(fn, co), D.tmp);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(u_host, D.tmp, sizeof(double) * D.ndof, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+  });
+}
+
+cd3d_status cd3d_errors(cd3d_problem *p, int level, const double *u_host, int fn, const cd3d_coeffs *co,
+                        double *errors) {
+  if (!p || !errors || level < 0 || level >= (int)p->dl.size()) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  return run(p, [&]() {
+    DevLevel &D = p->dl[level];
+    const FnParams f = fn_params(fn, co);
+    const double *u = D.sol;
+    if (u_host) {
+      HIPCHK(hipMemcpyAsync(D.tmp, u_host, sizeof(double) * D.ndof, hipMemcpyHostToDevice, p->stream));
+      u = D.tmp;
+    } else if (level == (int)p->dl.size() - 1 && p->tcd_ready) {
+      u = p->xs;  // the current TCD3D state
+    }
+    k_errors(p->stream, D, p->mesh.order, p->tab, u, f, p->red, p->dscal);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(p->hscal, p->dscal, sizeof(double) * 4, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    for (int k = 0; k < 4; k++) errors[k] = p->hscal[k];
+  });
+}
+
+// Local basis values at the 8 cell vertices in the order of the reference's
+// hexahedron (Output3D.C:1791-1800; the cell's vertex j sits at these
+// reference coordinates in its own frame): bv[j * nb + l]
+static void vertex_basis(int order, std::vector<double> &bv) {
+  static const int hv[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                               {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+  const int p1 = order + 1, nb = p1 * p1 * p1;
+  auto lag = [order](double x, int i) {
+    if (order == 1) return i == 0 ? 0.5 - 0.5 * x : 0.5 + 0.5 * x;
+    return i == 0 ? 0.5 * x * (x - 1) : i == 1 ? 1 - x * x : 0.5 * x * (x + 1);
+  };
+  bv.assign(8 * nb, 0.0);
+  for (int j = 0; j < 8; j++)
+    for (int k = 0; k < p1; k++)
+      for (int jj = 0; jj < p1; jj++)
+        for (int i = 0; i < p1; i++)
+          bv[j * nb + i + p1 * (jj + p1 * k)] = lag(hv[j][0], i) * lag(hv[j][1], jj) * lag(hv[j][2], k);
+}
+
+cd3d_status cd3d_write_vtk(cd3d_problem *p, int level, const double *u_host, const char *name, const char *path) {
+  if (!p || !u_host || !path || level < 0 || level >= (int)p->hl.size()) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  return run(p, [&]() {
+    const HostLevel &H = p->hl[level];
+    const int S = p->mesh.order, n = H.n, nv = n + 1, nb = p->nb;
+    static const int hexv[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                   {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    // vertices: the cell-corner lattice, x fastest; cells: their 8 vertices
+    // in the cell's own order; point data: the FE function at every cell's
+    // vertices, averaged over the cells (Output3D.C:1747-2100)
+    std::vector<int> cv((size_t)H.ncell * 8);
+    for (int c = 0; c < H.ncell; c++)
+      for (int v = 0; v < 8; v++) {
+        const int8_t *E = &H.ax[9 * c];
+        int X[3];
+        for (int d = 0; d < 3; d++)
+          X[d] = (H.org[3 * c + d] + S * (hexv[v][0] * E[d] + hexv[v][1] * E[3 + d] + hexv[v][2] * E[6 + d])) / S;
+        cv[(size_t)c * 8 + v] = X[0] + nv * (X[1] + nv * X[2]);
+      }
+    const size_t npt = (size_t)nv * nv * nv;
+    std::vector<double> val(npt, 0.0), bv;
+    std::vector<int> cnt(npt, 0);
+    vertex_basis(S, bv);
+    for (int c = 0; c < H.ncell; c++)
+      for (int v = 0; v < 8; v++) {
+        double s = 0.0;
+        for (int l = 0; l < nb; l++) s += bv[v * nb + l] * u_host[H.gn[(size_t)c * nb + l]];
+        val[cv[(size_t)c * 8 + v]] += s;
+        cnt[cv[(size_t)c * 8 + v]]++;
+      }
+    FILE *f = fopen(path, "w");
+    if (!f) throw ArgError(std::string("cannot open ") + path);
+    const double hx = p->mesh.box[0] / n, hy = p->mesh.box[1] / n, hz = p->mesh.box[2] / n;
+    fprintf(f, "# vtk DataFile Version 4.2\nfile created by ParMooN\nASCII\nDATASET UNSTRUCTURED_GRID\n\n");
+    fprintf(f, "POINTS %zu double\n", npt);
+    for (size_t k = 0; k < npt; k++)
+      fprintf(f, "%.6e %.6e %.6e\n", (k % nv) * hx, ((k / nv) % nv) * hy, (k / ((size_t)nv * nv)) * hz);
+    fprintf(f, "\nCELLS %d %d\n", H.ncell, 9 * H.ncell);
+    for (int c = 0; c < H.ncell; c++) {
+      fprintf(f, "8 ");
+      for (int v = 0; v < 8; v++) fprintf(f, "%d ", cv[(size_t)c * 8 + v]);
+      fprintf(f, "\n");
+    }
+    fprintf(f, "\nCELL_TYPES %d\n", H.ncell);
+    for (int c = 0; c < H.ncell; c++) fprintf(f, "12 ");
+    fprintf(f, "\n\nPOINT_DATA %zu\nSCALARS %s double\nLOOKUP_TABLE default\n", npt, name ? name : "u");
+    for (size_t k = 0; k < npt; k++) fprintf(f, "%.6e\n", cnt[k] ? val[k] / cnt[k] : 0.0);
+    fprintf(f, "\n\n\nCELL_DATA %d\nSCALARS RegionID int  1\nLOOKUP_TABLE default  \n", H.ncell);
+    for (int c = 0; c < H.ncell; c++) fprintf(f, "0\n");
+    const bool ok = !ferror(f);
+    if (fclose(f) != 0 || !ok) throw ArgError(std::string("write failed: ") + path);
+  });
+}
+
+cd3d_status cd3d_write_matrix(cd3d_problem *p, int level, const char *path) {
+  if (!p || !path || level < 0 || level >= (int)p->hl.size()) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  return run(p, [&]() {
+    const HostLevel &H = p->hl[level];
+    std::vector<double> e(H.nnz);
+    HIPCHK(hipMemcpyAsync(e.data(), p->dl[level].val, sizeof(double) * H.nnz, hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+    FILE *f = fopen(path, "wb");
+    if (!f) throw ArgError(std::string("cannot open ") + path);
+    const int hdr[3] = {H.ndof, H.ndof, (int)H.nnz};
+    bool ok = fwrite(hdr, sizeof(int), 3, f) == 3;
+    ok = ok && fwrite(H.rowptr.data(), sizeof(int), H.rowptr.size(), f) == H.rowptr.size();
+    ok = ok && fwrite(H.kcol.data(), sizeof(int), H.kcol.size(), f) == H.kcol.size();
+    ok = ok && fwrite(e.data(), sizeof(double), e.size(), f) == e.size();
+    if (fclose(f) != 0 || !ok) throw ArgError(std::string("write failed: ") + path);
+  });
+}
+
 cd3d_status cd3d_device_ptr(cd3d_problem *p, int level, int which, double **dev) {
   if (!p || !dev || level < 0 || level >= (int)p->dl.size() || which < 0 || which > 3) return CD3D_ERR_ARG;
   DevLevel &D = p->dl[level];

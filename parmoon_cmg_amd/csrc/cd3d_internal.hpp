@@ -279,6 +279,23 @@ struct AsmParams {
 void k_assemble(hipStream_t s, const DevLevel &L, const double *tab, const AsmParams &ap, double *val,
                 double *rhs, int colour_begin, int colour_count);
 void k_dirichlet_rows(hipStream_t s, const DevLevel &L, double *val, double *rhs);
+// ---------------------------------------------------------------- diagnostics (diag.hip)
+// known function of the interpolation and error norms: fn 0 zero, 1 the sine
+// of Examples/CD_3D/Laplace.h, 2 the Gaussian pulse centred at pc
+struct FnParams {
+  int fn = 0;
+  double pc[3] = {0, 0, 0};
+};
+struct ErrGeom {
+  double h[3], dn[3];
+};
+void k_interpolate(hipStream_t s, const DevLevel &L, const double *box, int order, int n, const FnParams &f,
+                   double *u);
+// workgroups of the error kernel (partials: 3 per workgroup + 1)
+int k_errors_blocks(int ncell);
+// out (device, 4 doubles): L2, H1 semi-norm, last cell's max |e|, max |e|
+void k_errors(hipStream_t s, const DevLevel &L, int order, const double *tab, const double *u, const FnParams &f,
+              double *partials, double *out);
 // ---------------------------------------------------------------- lattice form (lattice.hip)
 void kl_jacobi(hipStream_t s, const DevLevel &L, int order, const float *x, const float *f, float *out,
                float omega);

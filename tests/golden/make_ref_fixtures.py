@@ -168,6 +168,37 @@ RECORDS = {
 }
 
 
+def parse_vtk(path):
+    """Points, hexahedra, point data and the cell region ids of a legacy
+    ASCII VTK file as TOutput3D::WriteVtk writes it."""
+    lines = open(path).read().split("\n")
+    out, i = {}, 0
+    while i < len(lines):
+        w = lines[i].split()
+        if w and w[0] == "POINTS":
+            n = int(w[1])
+            out["vtk_points"] = np.array([[float(v) for v in lines[i + 1 + k].split()] for k in range(n)])
+            i += n
+        elif w and w[0] == "CELLS":
+            n = int(w[1])
+            out["vtk_cells"] = np.array([[int(v) for v in lines[i + 1 + k].split()[1:]] for k in range(n)], np.int32)
+            i += n
+        elif w and w[0] == "CELL_TYPES":
+            out["vtk_cell_types"] = np.array([int(v) for v in lines[i + 1].split()], np.int32)
+        elif w and w[0] == "SCALARS" and w[1] != "RegionID":
+            n = len(out["vtk_points"])
+            out["vtk_values"] = np.array([float(lines[i + 2 + k]) for k in range(n)])
+            out["vtk_name"] = np.array(w[1])
+            i += n
+        elif w and w[0] == "SCALARS" and w[1] == "RegionID":
+            n = len(out["vtk_cells"])
+            out["vtk_region"] = np.array([int(lines[i + 2 + k]) for k in range(n)], np.int32)
+            i += n
+        i += 1
+    out["vtk_text_head"] = np.array("\n".join(lines[:6]))
+    return out
+
+
 def write_dat(path, overrides):
     overrides = dict(overrides)
     geo_kind = overrides.pop("geo", None)
@@ -228,6 +259,23 @@ def run_case(name):
             out["res_hist"] = np.array(hist)
             t = re.search(r"TIME space (\S+) struct (\S+) assemble (\S+) solve (\S+)", p.stdout)
             out["ref_times"] = np.array([float(x) for x in t.groups()])
+            # diagnostics of the solution: L2/H1 errors against the sine of
+            # Examples/CD_3D/Laplace.h, its nodal interpolant and that one's
+            # errors, the VTK file (parsed) and the binary matrix's header
+            for f in ("errors_sol", "errors_interp", "interp_sine"):
+                out[f] = np.fromfile(os.path.join(td, f + ".f64"), dtype=np.float64)
+            out.update(parse_vtk(os.path.join(td, "sol.vtk")))
+            raw = np.fromfile(os.path.join(td, "matrix.bin"), dtype=np.uint8)
+            hdr = raw[:12].view(np.int32)
+            nrow, nnz = int(hdr[0]), int(hdr[2])
+            lf = len(levels) - 1
+            rp = raw[12:12 + 4 * (nrow + 1)].view(np.int32)
+            kc = raw[12 + 4 * (nrow + 1):12 + 4 * (nrow + 1) + 4 * nnz].view(np.int32)
+            ent = raw[12 + 4 * (nrow + 1) + 4 * nnz:].view(np.float64)
+            assert np.array_equal(rp, out[f"L{lf}_rowptr"]) and np.array_equal(kc, out[f"L{lf}_kcol"])
+            assert np.array_equal(ent, out[f"L{lf}_entries"])
+            out["matrix_header"] = hdr.copy()
+            out["matrix_bytes"] = np.array(len(raw))
         else:
             steps = int(ov["P8"])
             for k in range(steps + 1):

@@ -563,3 +563,50 @@ def test_multicolour_ssor_vcycle_and_fgmres_match_oracle(name):
             assert st.iterations == its_o
     assert verr < 1e-12
     P.close()
+
+
+@pytest.mark.parametrize("name", list(CD_CASES))
+def test_diagnostics_and_output_match_reference(name, golden, tmp_path):
+    # on the device: the nodal interpolant of the sine of Laplace.h and the
+    # L2/H1 errors (TFEFunction3D::Interpolate / GetErrors) of the
+    # reference's solution and interpolant; the errors of the library's own
+    # solution; the VTK file (TOutput3D::WriteVtk: same points, hexahedra and
+    # point data, vertices matched by coordinates) and the binary matrix
+    # (TSquareMatrix::Write: same header, size, pattern; entries 1e-12)
+    from tests.golden.make_ref_fixtures import parse_vtk
+    c = CD_CASES[name]
+    d = golden(name)
+    P = problem(name, golden)
+    lf = c["levels"] - 1
+    ui = P.interpolate(lf, P.FN_SINE)
+    ierr = np.abs(ui - d["interp_sine"]).max()
+    errs = []
+    for u, ref in ((d["sol"], d["errors_sol"]), (d["interp_sine"], d["errors_interp"])):
+        e = P.errors(lf, P.FN_SINE, u)
+        errs.append(np.abs(e[:3] - ref).max() / np.abs(ref).max())
+    x, st = P.solve(pm.solver_params(**case_solver(name), rel_tol=1e-10))
+    own = P.errors(lf, P.FN_SINE)  # the solution just computed, on the device
+    own_err = np.abs(own[:2] - d["errors_sol"][:2]).max() / np.abs(d["errors_sol"][:2]).max()
+    say(f"[gpu] {name} diagnostics: interp {ierr:.1e}, errors {errs[0]:.1e} / {errs[1]:.1e}, own solve {own_err:.1e}")
+    assert ierr <= 1e-15 and max(errs) <= 1e-12 and own_err <= 1e-6
+    P.write_vtk(lf, d["sol"], tmp_path / "u.vtk")
+    v = parse_vtk(str(tmp_path / "u.vtk"))
+    assert v["vtk_text_head"] == d["vtk_text_head"]
+    key = lambda pts: [tuple(np.round(q, 5)) for q in pts]
+    ref_idx = {k: i for i, k in enumerate(key(d["vtk_points"]))}
+    perm = np.array([ref_idx[k] for k in key(v["vtk_points"])])  # our point -> reference point
+    assert len(set(perm.tolist())) == len(perm) == len(d["vtk_points"])
+    assert np.array_equal(perm[v["vtk_cells"]], d["vtk_cells"])
+    assert np.array_equal(v["vtk_cell_types"], d["vtk_cell_types"])
+    assert np.array_equal(d["vtk_values"][perm], v["vtk_values"])
+    assert np.array_equal(v["vtk_region"], d["vtk_region"])
+    P.write_matrix(lf, tmp_path / "A.bin")
+    raw = np.fromfile(tmp_path / "A.bin", dtype=np.uint8)
+    hdr = raw[:12].view(np.int32)
+    assert np.array_equal(hdr, d["matrix_header"]) and len(raw) == int(d["matrix_bytes"])
+    n, nnz = int(hdr[0]), int(hdr[2])
+    rp = raw[12:12 + 4 * (n + 1)].view(np.int32)
+    kc = raw[12 + 4 * (n + 1):12 + 4 * (n + 1) + 4 * nnz].view(np.int32)
+    ent = raw[12 + 4 * (n + 1) + 4 * nnz:].view(np.float64)
+    assert np.array_equal(rp, d[f"L{lf}_rowptr"]) and np.array_equal(kc, d[f"L{lf}_kcol"])
+    assert row_scaled_err(ent, d[f"L{lf}_entries"], rp) < ENTRY_TOL

@@ -133,3 +133,20 @@ def test_c2_smoother_records(omega, sweeps, name, golden):
                           red_factor=0.0))
     _, its, _ = H.fgmres(sv, b)
     assert its == int(d["its"]) == (120 if omega == 0.8 else 21)
+
+
+@pytest.mark.parametrize("name", list(CD_CASES))
+def test_interpolation_and_error_norms(name, golden):
+    # TFEFunction3D::Interpolate of the sine of Examples/CD_3D/Laplace.h and
+    # TFEFunction3D::GetErrors with L2H1Errors, on the reference's solution
+    # and on the interpolant: the oracle reproduces the reference's values
+    # (the third entry is the maximum over the last cell's quadrature points)
+    c = CD_CASES[name]
+    d = golden(name)
+    H = po.Hierarchy(c["order"], c.get("m", 1), c["k0"], c["levels"], box=case_box(c))
+    lf = c["levels"] - 1
+    ui = H.interpolate(lf, 1)
+    assert np.abs(ui - d["interp_sine"]).max() <= 1e-15
+    for u, ref in ((d["sol"], d["errors_sol"]), (d["interp_sine"], d["errors_interp"])):
+        e = H.errors(lf, u, 1)
+        assert np.abs(e[:3] - ref).max() <= 1e-12 * np.abs(ref).max()

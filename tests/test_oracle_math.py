@@ -251,3 +251,25 @@ def test_multicolour_ssor_matches_definition(order, m, k0, levels):
         x[r] = (1 - omega) * x[r] + omega * s / d[r]
     xo = H.mcssor_sweep(l, omega, f, x0)
     assert np.abs(xo - x).max() <= 1e-12 * np.abs(x).max()
+
+
+def test_error_norms_closed_forms_and_rates():
+    # |sin(pi x) sin(pi y) sin(pi z)|_L2 = (1/2)^(3/2), |.|_H1 = pi sqrt(3/8) on
+    # the unit cube (errors of u_h = 0); the nodal interpolant's L2 error falls
+    # as h^(p+1) and its H1 error as h^p
+    for order in (1, 2):
+        H = po.Hierarchy(order, 1, 1, 3, box=(1.0, 1.0, 1.0))
+        e = H.errors(2, np.zeros(H.info(2)["ndof"]), 1)
+        assert abs(e[0] - 0.5 ** 1.5) < 2e-3 and abs(e[1] - np.pi * np.sqrt(3 / 8)) < 2e-3
+        l2, h1 = [], []
+        for l in range(3):
+            ei = H.errors(l, H.interpolate(l, 1), 1)
+            l2.append(ei[0])
+            h1.append(ei[1])
+        r2, r1 = np.log2(l2[1] / l2[2]), np.log2(h1[1] / h1[2])
+        assert abs(r2 - (order + 1)) < 0.25 and abs(r1 - order) < 0.25, (order, r2, r1)
+    # a function of the space is interpolated exactly: the pulse's interpolant
+    # compared with itself as u_h gives errors at the quadrature points only
+    H = po.Hierarchy(1, 1, 1, 1, box=(1.0, 1.0, 1.0))
+    u0 = np.zeros(H.info(0)["ndof"])
+    assert np.all(H.errors(0, u0, 0) == 0)
