@@ -126,6 +126,21 @@ cd3d_status cd3d_get_csr(cd3d_problem *p, int level, int *rowptr_host, int *kcol
 cd3d_status cd3d_get_rhs(cd3d_problem *p, int level, double *rhs_host);
 cd3d_status cd3d_set_entries(cd3d_problem *p, int level, const double *entries_host);
 
+/* Coefficients of any CoeffFct3D (Constants.h:123).  The callback cannot run
+ * on the device, so the caller evaluates it where the reference would: at
+ * the quadrature points of every cell (TFEDatabase3D::GetOrig,
+ * FEDatabase3D.C:1116; DiscreteForm3D.C:187-270).
+ * cd3d_quad_points: their physical coordinates, n_cells x n_q x 3 doubles in
+ * the reference's cell order and Gauss-point order (n_q = 27 for Q1, 64 for Q2).
+ * cd3d_set_coeffs_qp: the callback's values there, n_cells x n_q x 6 doubles
+ * (coeff[0..5] = eps, b1, b2, b3, c, f; |b| is recomputed), used by every
+ * later assembly of the level instead of the descriptor's functions (the
+ * descriptor still selects Galerkin or SUPG); NULL returns to the descriptor.
+ * The values are used as they are at every assembly: a time-dependent
+ * callback needs new values before each TCD3D step. */
+cd3d_status cd3d_quad_points(const cd3d_problem *p, int level, double *xyz_host);
+cd3d_status cd3d_set_coeffs_qp(cd3d_problem *p, int level, const double *coeffs_host);
+
 /* TSystemCD3D::Assemble (SystemCD3D.C:314): TAssembleMat3D::Assemble3D
  * (AssembleMat3D.C:230) on every level with BilinearAssemble /
  * BilinearAssemble_SD (ConvDiff3D.C:61,124); rhs on Dirichlet DOFs = 0. */

@@ -30,6 +30,7 @@ typedef struct {
   double *A;         /* matrix used by the multigrid level (A, or the TCD system M) */
   double *Aconv;     /* TCD: conv-diff matrix */
   double *Ksupg;     /* TCD SUPG: time-derivative SUPG matrix K */
+  double *qp;        /* coefficients per quadrature point (ncell*nq*6: eps, b1, b2, b3, c, f) or NULL */
   double *rhs, *sol; /* level vectors (TMGLevel3D Rhs / X) */
   double *aux0, *aux1;
   double *mult;      /* number of cells sharing each DOF (Prolongate aux) */
@@ -257,7 +258,7 @@ void orc_destroy(orc_hier *h) {
   for (int l = 0; l < h->L; l++) {
     level_t *L = h->lv + l;
     free(L->org); free(L->ax); free(L->gn); free(L->dof_node); free(L->rowptr); free(L->kcol);
-    free(L->A); free(L->Aconv); free(L->Ksupg); free(L->rhs); free(L->sol); free(L->aux0); free(L->aux1); free(L->mult);
+    free(L->A); free(L->Aconv); free(L->Ksupg); free(L->qp); free(L->rhs); free(L->sol); free(L->aux0); free(L->aux1); free(L->mult);
   }
   free(h->lv); free(h->lu); free(h->piv);
   free(h);
@@ -398,7 +399,13 @@ static void local_forms(const orc_hier *h, const level_t *L, int c, const orc_co
           gx_[b] = g[0]; gy_[b] = g[1]; gz_[b] = g[2];
         }
         double cf[7];
-        coeff_at(co, X[0], X[1], X[2], cf);
+        if (L->qp) { /* the CoeffFct3D values handed in for this point */
+          const double *v = L->qp + ((size_t)c * n1 * n1 * n1 + (qi * n1 + qj) * n1 + qk) * 6;
+          for (int k = 0; k < 6; k++) cf[k] = v[k];
+          cf[6] = sqrt(cf[1] * cf[1] + cf[2] * cf[2] + cf[3] * cf[3]);
+        } else {
+          coeff_at(co, X[0], X[1], X[2], cf);
+        }
         double delta = 0;
         if (co->supg && what != 1 && tscale > 0) {
           /* TCD3D.C:141-156: tilde tau of (tscale eps, tscale b), then tau *= tscale */
@@ -441,6 +448,41 @@ static void local_forms(const orc_hier *h, const level_t *L, int c, const orc_co
           }
         }
       }
+}
+
+/* The quadrature points of level l per cell (cell order) and point (Gauss
+ * rule order): the points TFEDatabase3D::GetOrig hands to the CoeffFct3D
+ * callback (FEDatabase3D.C:1116-1480, DiscreteForm3D.C:187-270). */
+void orc_quad_points(const orc_hier *h, int l, double *xyz) {
+  const level_t *L = h->lv + l;
+  int p = h->order, n1 = p + 2;
+  const double *gx = p == 1 ? G3X : G4X;
+  double hd[3], dn[3];
+  for (int d = 0; d < 3; d++) { hd[d] = h->box[d] / L->n; dn[d] = h->box[d] / (p * L->n); }
+  size_t k = 0;
+  for (int c = 0; c < L->ncell; c++) {
+    const int *O = L->org + 3 * c, *E = L->ax + 9 * c;
+    for (int qi = 0; qi < n1; qi++)
+      for (int qj = 0; qj < n1; qj++)
+        for (int qk = 0; qk < n1; qk++) {
+          double ref[3] = {gx[qk], gx[qj], gx[qi]};
+          for (int d = 0; d < 3; d++)
+            xyz[k++] = O[d] * dn[d] + (ref[0] + 1) / 2 * hd[d] * E[d] + (ref[1] + 1) / 2 * hd[d] * E[3 + d] +
+                       (ref[2] + 1) / 2 * hd[d] * E[6 + d];
+        }
+  }
+}
+
+/* coefficients per quadrature point for level l (NULL: back to the descriptor) */
+void orc_set_coeffs_qp(orc_hier *h, int l, const double *qp) {
+  level_t *L = h->lv + l;
+  free(L->qp);
+  L->qp = NULL;
+  if (!qp) return;
+  int n1 = h->order + 2;
+  size_t n = (size_t)L->ncell * n1 * n1 * n1 * 6;
+  L->qp = malloc(sizeof(double) * n);
+  memcpy(L->qp, qp, sizeof(double) * n);
 }
 
 /* TAssembleMat3D::Assemble3D cell loop (AssembleMat3D.C:386-491) with

@@ -67,6 +67,12 @@ void orc_get_rhs(const orc_hier *h, int l, double *rhs);
 void orc_get_dof_nodes(const orc_hier *h, int l, int *node);
 void orc_set_entries(orc_hier *h, int l, const double *entries);
 
+/* quadrature points of level l (n_cells * n_q * 3, cell order, Gauss rule
+ * order) and per-point coefficients for it (n_cells * n_q * 6: eps, b1, b2,
+ * b3, c, f -- what a CoeffFct3D returns there; NULL: the descriptor again) */
+void orc_quad_points(const orc_hier *h, int l, double *xyz);
+void orc_set_coeffs_qp(orc_hier *h, int l, const double *qp);
+
 /* assemble A and rhs on every level (SystemCD3D::Assemble) */
 void orc_assemble(orc_hier *h, const orc_coeffs *co);
 /* assemble only level l into caller arrays: what = 0 conv-diff (A), 1 mass (M) */

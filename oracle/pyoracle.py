@@ -87,6 +87,8 @@ def lib():
         L.orc_interp_pulse.argtypes = [P, I, C.POINTER(Coeffs), dp]
         L.orc_mcssor_sweep.argtypes = [P, I, D, dp, dp]
         L.orc_interpolate.argtypes = [P, I, I, C.POINTER(Coeffs), dp]
+        L.orc_quad_points.argtypes = [P, I, dp]
+        L.orc_set_coeffs_qp.argtypes = [P, I, dp]
         L.orc_errors.argtypes = [P, I, dp, I, C.POINTER(Coeffs), dp]
         L.orc_sdfem_delta.restype = D
         L.orc_sdfem_delta.argtypes = [D, D, dp, I]
@@ -209,6 +211,19 @@ class Hierarchy:
         u = np.zeros(self.info(l)["ndof"])
         self.L.orc_interp_pulse(self.h, l, C.byref(co), _dp(u))
         return u
+
+    def quad_points(self, l):
+        nq = (self.order + 2) ** 3
+        a = np.zeros((self.info(l)["ncell"], nq, 3))
+        self.L.orc_quad_points(self.h, l, _dp(a))
+        return a
+
+    def set_coeffs_qp(self, l, qp):
+        if qp is None:
+            self.L.orc_set_coeffs_qp(self.h, l, None)
+            return
+        qp = np.ascontiguousarray(qp, dtype=np.float64)
+        self.L.orc_set_coeffs_qp(self.h, l, _dp(qp))
 
     def interpolate(self, l, fn, co=None):
         u = np.zeros(self.info(l)["ndof"])

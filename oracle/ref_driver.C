@@ -22,11 +22,14 @@
 //     stopping at SC_LIN_RED_FACTOR_SCALAR times each step's initial residual
 //
 // Coefficients come from the .dat P-parameters:
-//   P0 eps, P1..P3 convection b (P7>0: rotating field
+//   P0 eps, P1..P3 convection b (P7 1: rotating field
 //   b = (-(y - P13), x - P12, 0.1 sin(2 pi t)) of the TCD3D config),
 //   P4 reaction c, P5 source kind (0: f = P6, 1: manufactured
 //   u = sin(pi x) sin(pi y) sin(pi z)), P8 number of CN steps,
 //   P9..P11 centre of the Gaussian pulse initial condition (sigma 0.05).
+//   P7 2: coefficients varying in space, evaluated by this CoeffFct3D at
+//   every quadrature point: eps = P0 (1 + xyz), b = (P1 sin(pi y),
+//   P2 cos(pi x), P3 z), c = P4 x, f = P6 (1 + x).
 // The mesh is what the reference builds from data/Wuerfel.GEO: ReadGeo.C:1753-1755
 // scales parametrised vertices by (1, 0.346, 0.346), so the domain is the box
 // [0,1] x [0,0.346] x [0,0.346].
@@ -90,7 +93,11 @@ static void dump(const std::string &name, const void *p, size_t bytes) {
 
 // ---------------------------------------------------------------- callbacks
 static void conv_field(double x, double y, double z, double t, double &b1, double &b2, double &b3) {
-  if (TDatabase::ParamDB->P7 > 0) {
+  if (TDatabase::ParamDB->P7 == 2) {  // variable field of the per-point coefficient cases
+    b1 = TDatabase::ParamDB->P1 * sin(Pi * y);
+    b2 = TDatabase::ParamDB->P2 * cos(Pi * x);
+    b3 = TDatabase::ParamDB->P3 * z;
+  } else if (TDatabase::ParamDB->P7 > 0) {
     b1 = -(y - TDatabase::ParamDB->P13);
     b2 = x - TDatabase::ParamDB->P12;
     b3 = 0.1 * sin(2 * Pi * t);
@@ -110,6 +117,13 @@ void DrvCoeffs(int n_points, double *X, double *Y, double *Z, double **parameter
     co[0] = eps;
     co[1] = b1; co[2] = b2; co[3] = b3;
     co[4] = c;
+    if ((int)TDatabase::ParamDB->P7 == 2) {  // eps, c and f vary too
+      co[0] = eps * (1 + X[i] * Y[i] * Z[i]);
+      co[4] = c * X[i];
+      co[5] = TDatabase::ParamDB->P6 * (1 + X[i]);
+      co[6] = sqrt(b1 * b1 + b2 * b2 + b3 * b3);
+      continue;
+    }
     if ((int)TDatabase::ParamDB->P5 == 1) {
       double sx = sin(Pi * X[i]), sy = sin(Pi * Y[i]), sz = sin(Pi * Z[i]);
       double cx = cos(Pi * X[i]), cy = cos(Pi * Y[i]), cz = cos(Pi * Z[i]);

@@ -23,7 +23,7 @@ FP64, MIXED = 0, 1
 # every entry point declared in include/cd3d.h
 EXPORTS = ["cd3d_default_solver", "cd3d_create", "cd3d_destroy", "cd3d_last_error", "cd3d_device_bytes",
            "cd3d_level_info", "cd3d_get_dofmap", "cd3d_get_cells", "cd3d_get_csr", "cd3d_get_rhs",
-           "cd3d_set_entries", "cd3d_assemble", "cd3d_matvec", "cd3d_matvec_dev", "cd3d_matvec_noalloc",
+           "cd3d_set_entries", "cd3d_quad_points", "cd3d_set_coeffs_qp", "cd3d_assemble", "cd3d_matvec", "cd3d_matvec_dev", "cd3d_matvec_noalloc",
            "cd3d_prolongate",
            "cd3d_restrict", "cd3d_vcycle", "cd3d_solve", "cd3d_get_solution", "cd3d_tcd_run",
            "cd3d_tcd_init", "cd3d_tcd_step", "cd3d_tcd_get_state",
@@ -136,6 +136,8 @@ def lib():
     L.cd3d_device_ptr.argtypes = [P, I, I, C.POINTER(C.c_void_p)]
     L.cd3d_kernel_bench.argtypes = [P, I, I, I, dp, dp]
     L.cd3d_interpolate.argtypes = [P, I, I, C.POINTER(Coeffs), dp]
+    L.cd3d_quad_points.argtypes = [P, I, dp]
+    L.cd3d_set_coeffs_qp.argtypes = [P, I, dp]
     L.cd3d_errors.argtypes = [P, I, dp, I, C.POINTER(Coeffs), dp]
     L.cd3d_write_vtk.argtypes = [P, I, dp, C.c_char_p, C.c_char_p]
     L.cd3d_write_matrix.argtypes = [P, I, C.c_char_p]
@@ -339,6 +341,22 @@ class Problem:
 
     KERNELS = {0: "csr_stream<kSpmv> FP64", 1: "lat_sweep_k<kLJacobi> FP32", 2: "lat64_k<kL64Spmv> FP64",
                3: "lat_restrict_k FP32", 4: "lat_prolong_k FP32", 5: "csr_fill_k", 6: "copy_k FP64"}
+
+    def quad_points(self, l):
+        """Physical quadrature points of level l: (n_cells, n_q, 3)."""
+        nq = (self.order + 2) ** 3
+        a = np.zeros((self.info(l)["ncell"], nq, 3))
+        self._chk(self.L.cd3d_quad_points(self.h, l, _dp(a)), "quad_points")
+        return a
+
+    def set_coeffs_qp(self, l, qp):
+        """Per-point coefficients (n_cells, n_q, 6: eps, b1, b2, b3, c, f) of
+        level l; None returns to the descriptor."""
+        if qp is None:
+            self._chk(self.L.cd3d_set_coeffs_qp(self.h, l, None), "set_coeffs_qp")
+            return
+        qp = np.ascontiguousarray(qp, dtype=np.float64)
+        self._chk(self.L.cd3d_set_coeffs_qp(self.h, l, _dp(qp)), "set_coeffs_qp")
 
     # known functions of interpolate / errors: zero, the sine of Laplace.h, the Gaussian pulse
     FN_ZERO, FN_SINE, FN_PULSE = 0, 1, 2

@@ -307,6 +307,7 @@ static AsmParams asm_params(const cd3d_problem &P, const DevLevel &D, const cd3d
   ap.delta_const = 0;
   if (co.supg && co.conv_kind == 0)
     ap.delta_const = sdfem_delta_host(hconv_host(D.h, co.b), co.eps, co.b, P.mesh.order);
+  ap.qp = D.qp;
   return ap;
 }
 
@@ -338,7 +339,7 @@ static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int wha
   if (what == 0) P.last_co = co;
   // constant coefficients (and the plain mass matrix under any convection):
   // one element matrix per level through the class stencils
-  const bool uniform = co.conv_kind == 0 || (what == 1 && !co.supg);
+  const bool uniform = (co.conv_kind == 0 || (what == 1 && !co.supg)) && !D.qp;
   if (uniform && co.f_kind == 0 && D.nclass > 0 && !uniform_assembly_off()) {
     // every active row is written: no clearing pass
     kl_assemble_uniform(P.stream, D, P.mesh.order, P.tab, ap, val, rhs);
@@ -346,7 +347,8 @@ static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int wha
     HIPCHK(hipGetLastError());
     return 1;
   }
-  if (what == 0 && co.conv_kind == 1 && co.f_kind == 0 && !co.supg && D.nclass > 0 && !uniform_assembly_off()) {
+  if (what == 0 && co.conv_kind == 1 && co.f_kind == 0 && !co.supg && !D.qp && D.nclass > 0 &&
+      !uniform_assembly_off()) {
     kl_assemble_affine(P.stream, D, P.mesh.order, P.tab, ap, val, rhs, csr);
     k_dirichlet_rows(P.stream, D, val, rhs);
     HIPCHK(hipGetLastError());
@@ -1480,6 +1482,50 @@ cd3d_status cd3d_tcd_run(cd3d_problem *p, const cd3d_coeffs *co, const cd3d_solv
     st->ms_solve = now_ms() - tstart;
   }
   return rc != CD3D_OK ? rc : solve_status;
+}
+
+// ---------------------------------------------------------------- per-point coefficients
+
+cd3d_status cd3d_quad_points(const cd3d_problem *p, int level, double *xyz) {
+  if (!p || !xyz || level < 0 || level >= (int)p->hl.size()) return CD3D_ERR_ARG;
+  const HostLevel &H = p->hl[level];
+  const int P = p->mesh.order, n1 = quad_points_1d(P), nq = n1 * n1 * n1;
+  std::vector<double> tab;
+  basis_tables(P, tab);
+  const int stride = 4 + 4 * p->nb;
+  double dn[3];
+  for (int d = 0; d < 3; d++) dn[d] = p->mesh.box[d] / ((double)P * H.n);
+  // the point of the element kernels: X[dax[a]] = dn (O + P sg (xi_a + 1) / 2)
+  for (int c = 0; c < H.ncell; c++)
+    for (int q = 0; q < nq; q++) {
+      double *X = xyz + ((size_t)c * nq + q) * 3;
+      for (int a = 0; a < 3; a++)
+        for (int d = 0; d < 3; d++) {
+          const int e = H.ax[9 * c + 3 * a + d];
+          if (e != 0) X[d] = dn[d] * (H.org[3 * c + d] + P * e * (tab[(size_t)q * stride + 1 + a] + 1) * 0.5);
+        }
+    }
+  return CD3D_OK;
+}
+
+cd3d_status cd3d_set_coeffs_qp(cd3d_problem *p, int level, const double *coeffs) {
+  if (!p || level < 0 || level >= (int)p->hl.size()) return CD3D_ERR_ARG;
+  std::lock_guard<std::mutex> g(p->mu);
+  return run(p, [&]() {
+    DevLevel &D = p->dl[level];
+    if (!coeffs) {
+      if (D.qp) dfree(*p, D.qp);
+      D.qp = nullptr;
+      return;
+    }
+    const int n1 = quad_points_1d(p->mesh.order);
+    const size_t n = (size_t)D.ncell * n1 * n1 * n1 * 6;
+    for (size_t k = 0; k < n; k++)
+      if (!std::isfinite(coeffs[k])) throw ArgError("non-finite coefficient");
+    if (!D.qp) D.qp = dalloc<double>(*p, n);
+    HIPCHK(hipMemcpyAsync(D.qp, coeffs, sizeof(double) * n, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipStreamSynchronize(p->stream));
+  });
 }
 
 // ---------------------------------------------------------------- diagnostics and output

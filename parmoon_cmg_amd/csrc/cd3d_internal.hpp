@@ -91,6 +91,8 @@ struct DevLevel {
   int *p_rowptr = nullptr, *p_col = nullptr, *r_rowptr = nullptr, *r_col = nullptr;
   double *p_val = nullptr, *r_val = nullptr;
   int64_t p_nnz = 0, r_nnz = 0;
+  // coefficients per quadrature point handed in by the caller (cd3d_set_coeffs_qp), or null
+  double *qp = nullptr;
   // multicolour SSOR row lists (uploaded on first use)
   int *mc_rows = nullptr;
   std::vector<int> mc_start;
@@ -275,6 +277,9 @@ struct AsmParams {
   double rot_c[2], sin2pit;
   double h[3], dn[3];
   double delta_const;  // SUPG delta for constant b (conv_kind 0)
+  // per quadrature point coefficients (ncell x nq x 6: eps, b1, b2, b3, c, f)
+  // replacing the descriptor's functions, or null
+  const double *qp;
 };
 void k_assemble(hipStream_t s, const DevLevel &L, const double *tab, const AsmParams &ap, double *val,
                 double *rhs, int colour_begin, int colour_count);

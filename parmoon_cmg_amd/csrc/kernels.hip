@@ -792,9 +792,28 @@ __global__ __launch_bounds__(kBlock) void assemble_k(int ncells, const int *__re
         X[2] = dax[a] == 2 ? xa : X[2];
       }
     }
-    double b0 = ap.b[0], b1 = ap.b[1], b2 = ap.b[2];
+    double b0 = ap.b[0], b1 = ap.b[1], b2 = ap.b[2], cr = ap.c;
     double delta = ap.supg ? ap.delta_const : 0.0;
-    if (VAR && ap.conv_kind == 1) {
+    double f = ap.f_val;
+    if (VAR && ap.qp) {
+      // the caller's CoeffFct3D values at this point
+      const double *cq = ap.qp + ((int64_t)c * NQ + q) * 6;
+      const double eps = cq[0];
+      b0 = cq[1];
+      b1 = cq[2];
+      b2 = cq[3];
+      cr = cq[4];
+      f = cq[5];
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        kap[a] = eps * ih[a] * ih[a];
+        beta[a] = sg[a] * (dax[a] == 0 ? b0 : dax[a] == 1 ? b1 : b2) * ih[a];
+      }
+      if (ap.supg) {
+        if (hconv < 0) hconv = hconv_brick(ap.h, b0, b1, b2);
+        delta = sdfem_delta(hconv, eps, sqrt(b0 * b0 + b1 * b1 + b2 * b2), nu);
+      }
+    } else if (VAR && ap.conv_kind == 1) {
       b0 = -(X[1] - ap.rot_c[1]);
       b1 = X[0] - ap.rot_c[0];
       b2 = 0.1 * ap.sin2pit;
@@ -805,8 +824,7 @@ __global__ __launch_bounds__(kBlock) void assemble_k(int ncells, const int *__re
         delta = sdfem_delta(hconv, ap.eps, sqrt(b0 * b0 + b1 * b1 + b2 * b2), nu);
       }
     }
-    double f = ap.f_val;
-    if (VAR && ap.f_kind == 1) {
+    if (VAR && !ap.qp && ap.f_kind == 1) {
       const double sx = sin(M_PI * X[0]), sy = sin(M_PI * X[1]), sz = sin(M_PI * X[2]);
       const double cx = cos(M_PI * X[0]), cy = cos(M_PI * X[1]), cz = cos(M_PI * X[2]), u = sx * sy * sz;
       f = ap.eps * 3 * M_PI * M_PI * u + b0 * M_PI * cx * sy * sz + b1 * M_PI * sx * cy * sz +
@@ -830,7 +848,7 @@ __global__ __launch_bounds__(kBlock) void assemble_k(int ncells, const int *__re
     const double a0 = w * (kap[0] * g0 + ti * beta[0]);
     const double a1 = w * (kap[1] * g1 + ti * beta[1]);
     const double a2 = w * (kap[2] * g2 + ti * beta[2]);
-    const double cc = w * (ti * ap.c);
+    const double cc = w * (ti * cr);
 #pragma unroll
     for (int j = 0; j < NB; j++)
       acc[j] = fma(a0, T[5 + 4 * j], fma(a1, T[6 + 4 * j], fma(a2, T[7 + 4 * j], fma(cc, T[4 + 4 * j], acc[j]))));
@@ -1364,7 +1382,7 @@ void k_assemble(hipStream_t s, const DevLevel &L, const double *tab, const AsmPa
     const char *e = getenv("CD3D_ASM_LDS_TAB");
     return e ? atoi(e) : 0;
   }();
-  const bool var = ap.conv_kind != 0 || ap.f_kind != 0;
+  const bool var = ap.conv_kind != 0 || ap.f_kind != 0 || ap.qp != nullptr;
   const int64_t threads = (int64_t)colour_count * (ap.order == 1 ? 8 : 27);
   const int grid = (int)((threads + kBlock - 1) / kBlock);
 #define CD3D_ASM(P_, LDS_, VAR_) \

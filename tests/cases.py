@@ -47,6 +47,14 @@ CD_CASES = {
                          co=dict(eps=1e-2, b=(0.3, 0.4, -0.5), f_val=1.0, supg=1)),
     "m2_q1_fcycle": dict(order=1, m=2, k0=0, levels=4, box=UNIT_BOX, solver=dict(cycle=0),
                          co=dict(eps=1e-2, b=(0.3, 0.4, -0.5), f_val=1.0, supg=1)),
+    # coefficients varying in space, handed in per quadrature point
+    # (qp_coeffs below evaluates the reference run's CoeffFct3D)
+    "unit_q1_var_gal": dict(order=1, k0=2, levels=3, box=UNIT_BOX, qp=dict(eps=0.05, b=(0.8, -0.6, 0.5), c=0.3, f=1.0),
+                            co=dict(eps=0.05, f_val=1.0)),
+    "unit_q1_var_supg": dict(order=1, k0=2, levels=3, box=UNIT_BOX, solver=dict(omega=0.5, pre=3, post=3),
+                             qp=dict(eps=1e-3, b=(0.8, -0.6, 0.5), c=0.3, f=1.0), co=dict(eps=1e-3, f_val=1.0, supg=1)),
+    "m2_q2_var_supg": dict(order=2, m=2, k0=0, levels=3, box=UNIT_BOX, solver=dict(omega=0.5, pre=3, post=3),
+                           qp=dict(eps=1e-3, b=(0.8, -0.6, 0.5), c=0.3, f=1.0), co=dict(eps=1e-3, f_val=1.0, supg=1)),
     # the benchmark's smoother: damped Jacobi omega 0.5, 3 + 3 sweeps
     "unit_c2_n8_bench": dict(order=2, k0=1, levels=3, box=UNIT_BOX, solver=dict(omega=0.5, pre=3, post=3),
                              co=dict(eps=1e-3, b=(1.0, 0.5, 0.25), f_val=1.0, supg=1)),
@@ -97,3 +105,20 @@ def tcd_solver(c):
     if c["co"].get("supg"):
         return dict(kw, rel_tol=1e-10, res_norm_min=1e-30, red_factor=0.0)
     return dict(kw, rel_tol=0.0, res_norm_min=1e-30, red_factor=1e-10)
+
+
+def qp_coeffs(c, xyz):
+    """The coefficients of a per-point case at the quadrature points xyz
+    (n_cells x n_q x 3): eps (1 + xyz), b = (b1 sin(pi y), b2 cos(pi x), b3 z),
+    c x, f (1 + x) -- the CoeffFct3D of the reference runs (ref_driver.C, P7 2)."""
+    import numpy as np
+    q = c["qp"]
+    x, y, z = xyz[..., 0], xyz[..., 1], xyz[..., 2]
+    out = np.empty(xyz.shape[:-1] + (6,))
+    out[..., 0] = q["eps"] * (1 + x * y * z)
+    out[..., 1] = q["b"][0] * np.sin(np.pi * y)
+    out[..., 2] = q["b"][1] * np.cos(np.pi * x)
+    out[..., 3] = q["b"][2] * z
+    out[..., 4] = q["c"] * x
+    out[..., 5] = q["f"] * (1 + x)
+    return out

@@ -119,6 +119,22 @@ CASES = {
                                   SC_SMOOTH_DAMP_FACTOR_SCALAR=0.5, SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=0.5,
                                   SC_PRE_SMOOTH_SCALAR=3, SC_POST_SMOOTH_SCALAR=3,
                                   P0=1e-4, P4=0.2, P5=0, P6=1.0, P7=1, P12=0.45, P13=0.55)),
+    # coefficients varying in space, evaluated by the reference's CoeffFct3D
+    # at every quadrature point (eps = P0 (1 + xyz), b = (P1 sin(pi y),
+    # P2 cos(pi x), P3 z), c = P4 x, f = P6 (1 + x)): the per-point
+    # coefficient path
+    "unit_q1_var_gal": ("cd", dict(geo="unit", ANSATZ_ORDER=1, LEVELS=3, UNIFORM_STEPS=2, DISCTYPE=1,
+                                   P0=0.05, P1=0.8, P2=-0.6, P3=0.5, P4=0.3, P5=0, P6=1.0, P7=2)),
+    "unit_q1_var_supg": ("cd", dict(geo="unit", ANSATZ_ORDER=1, LEVELS=3, UNIFORM_STEPS=2, DISCTYPE=2,
+                                    CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                                    SC_SMOOTH_DAMP_FACTOR_SCALAR=0.5, SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=0.5,
+                                    SC_PRE_SMOOTH_SCALAR=3, SC_POST_SMOOTH_SCALAR=3,
+                                    P0=1e-3, P1=0.8, P2=-0.6, P3=0.5, P4=0.3, P5=0, P6=1.0, P7=2)),
+    "m2_q2_var_supg": ("cd", dict(geo="m2", ANSATZ_ORDER=2, LEVELS=3, UNIFORM_STEPS=0, DISCTYPE=2,
+                                  CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                                  SC_SMOOTH_DAMP_FACTOR_SCALAR=0.5, SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=0.5,
+                                  SC_PRE_SMOOTH_SCALAR=3, SC_POST_SMOOTH_SCALAR=3,
+                                  P0=1e-3, P1=0.8, P2=-0.6, P3=0.5, P4=0.3, P5=0, P6=1.0, P7=2)),
     # Crank-Nicolson with the rotating field on macro-brick meshes, Q1 and Q2
     "m2_q1_tcd": ("tcd", dict(geo="m2", ANSATZ_ORDER=1, LEVELS=2, UNIFORM_STEPS=1, DISCTYPE=1,
                               P0=1e-4, P4=0, P5=0, P6=0.0, P7=1, P8=3,

@@ -19,7 +19,8 @@ import pytest
 
 import parmoon_cmg_amd as pm
 from oracle import pyoracle as po
-from tests.cases import CD_CASES, REF_BOX, REF_SOLVER, TCD_CASE, TCD_CASES, case_box, case_solver, tcd_solver
+from tests.cases import (CD_CASES, REF_BOX, REF_SOLVER, TCD_CASE, TCD_CASES, case_box, case_solver, qp_coeffs,
+                         tcd_solver)
 
 pytestmark = pytest.mark.gpu
 
@@ -39,7 +40,11 @@ def row_scaled_err(e, er, rp):
 
 
 def make_problem(c):
-    return pm.Problem(c["order"], c.get("m", 1), c["k0"], c["levels"], box=case_box(c))
+    P = pm.Problem(c["order"], c.get("m", 1), c["k0"], c["levels"], box=case_box(c))
+    if "qp" in c:  # the reference run's CoeffFct3D at every quadrature point
+        for l in range(c["levels"]):
+            P.set_coeffs_qp(l, qp_coeffs(c, P.quad_points(l)))
+    return P
 
 
 _problems = {}
@@ -63,6 +68,9 @@ def oracle_solution(name):
     if name not in _oracles:
         c = CD_CASES[name]
         H = po.Hierarchy(c["order"], c.get("m", 1), c["k0"], c["levels"], box=case_box(c))
+        if "qp" in c:
+            for l in range(c["levels"]):
+                H.set_coeffs_qp(l, qp_coeffs(c, H.quad_points(l)))
         H.assemble(po.coeffs(**c["co"]))
         b = H.rhs(c["levels"] - 1)
         _, r0 = H.defect(c["levels"] - 1, np.zeros_like(b), b)
@@ -160,7 +168,7 @@ def test_fgmres_exact_coarse_matches_oracle(name, golden):
 
 MIXED_CASES = ["q2_l2_supg", "q1_l3_supg", "q1_c1", "unit_c2_n8_bench", "m2_q1_supg", "m3_q1_supg",
                "m5_q1_supg", "m2_q2_supg", "m3_q2_supg", "unit_q1_supg_rot", "m2_q2_supg_rot",
-               "m2_q1_wcycle", "m2_q1_fcycle"]
+               "m2_q1_wcycle", "m2_q1_fcycle", "unit_q1_var_supg", "m2_q2_var_supg"]
 
 
 @pytest.mark.parametrize("name", MIXED_CASES)
@@ -610,3 +618,15 @@ def test_diagnostics_and_output_match_reference(name, golden, tmp_path):
     ent = raw[12 + 4 * (n + 1) + 4 * nnz:].view(np.float64)
     assert np.array_equal(rp, d[f"L{lf}_rowptr"]) and np.array_equal(kc, d[f"L{lf}_kcol"])
     assert row_scaled_err(ent, d[f"L{lf}_entries"], rp) < ENTRY_TOL
+
+
+@pytest.mark.parametrize("name", [n for n in CD_CASES if "qp" in CD_CASES[n]])
+def test_quad_points_match_oracle(name):
+    # the points handed to the caller's coefficient function are the
+    # oracle's (which restates GetOrig's affine map) to rounding
+    c = CD_CASES[name]
+    P = make_problem(c)
+    H = po.Hierarchy(c["order"], c.get("m", 1), c["k0"], c["levels"], box=case_box(c))
+    for l in range(c["levels"]):
+        assert np.abs(P.quad_points(l) - H.quad_points(l)).max() <= 1e-15
+    P.close()

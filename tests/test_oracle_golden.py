@@ -10,14 +10,15 @@ checks one oracle function against the reference's output:
   TCD3D Crank-Nicolson states             1e-10 relative
 Cases cover ParMooN's Wuerfel cube and the unit cube split into m^3 macro
 bricks (m = 2, 3, 5; the meshes of the batched and serving workloads), Q1 and
-Q2, Galerkin and SUPG over eps 1e-6 ... 1, V-, W- and F-cycles, and the
-benchmark's smoother (omega 0.5, 3 + 3 sweeps).
+Q2, Galerkin and SUPG over eps 1e-6 ... 1, V-, W- and F-cycles, the
+benchmark's smoother (omega 0.5, 3 + 3 sweeps), and coefficients varying in
+space handed in per quadrature point.
 """
 import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from tests.cases import CD_CASES, TCD_CASES, REF_SOLVER, case_box, case_solver, tcd_solver
+from tests.cases import CD_CASES, TCD_CASES, REF_SOLVER, case_box, case_solver, qp_coeffs, tcd_solver
 
 ENTRY_TOL = 1e-12
 SOL_TOL = 1e-10
@@ -38,6 +39,9 @@ def hier():
             d = golden(name)
             assert tuple(d["box"]) == tuple(case_box(c))
             H = po.Hierarchy(c["order"], c.get("m", 1), c["k0"], c["levels"], box=tuple(d["box"]))
+            if "qp" in c:  # the reference's CoeffFct3D at every quadrature point
+                for l in range(c["levels"]):
+                    H.set_coeffs_qp(l, qp_coeffs(c, H.quad_points(l)))
             H.assemble(po.coeffs(**c["co"]))
             cache[name] = (H, H.rhs(c["levels"] - 1))
         return cache[name]
