@@ -309,9 +309,27 @@ inline void Defect_Scalar(TSquareMatrix **A, TMatrix **B, double *x, double *b, 
 
 // ------------------------------------------------------------------ discrete forms and assembly
 
+// The reference's callback and kernel types (Constants.h:123,136,158) and
+// derivative names (MultiIndex3D).
+enum MultiIndex3D { D000, D100, D010, D001, D200, D110, D101, D020, D011, D002 };
+typedef void CoeffFct3D(int n_points, double *X, double *Y, double *Z, double **parameters, double **coeffs);
+typedef void AssembleFct3D(double, double *, double, double **, int *, double ***, double **);
+typedef void ManipulateFct3D(int, double **, double **, void *);
+typedef void BoundCondFunct3D(int, double, double, double, int &);
+typedef void BoundValueFunct3D(int, double, double, double, double &);
+
+// The local kernels the device runs (ConvDiff3D.C:61,124): a TDiscreteForm3D
+// built from these selects Galerkin or SDFEM; the bodies never run, the
+// device's element kernels integrate the same forms
+inline void BilinearAssemble(double, double *, double, double **, int *, double ***, double **) {}
+inline void BilinearAssemble_SD(double, double *, double, double **, int *, double ***, double **) {}
+
 // TDiscreteForm3D: the local form (BilinearAssemble for GALERKIN,
 // BilinearAssemble_SD for SDFEM, ConvDiff3D.C:61,124; MatrixMRhsAssemble
-// for the TCD3D mass, TCD3D.C:11) with its coefficients as a descriptor.
+// for the TCD3D mass, TCD3D.C:11) with its coefficients as a descriptor, or
+// -- through the reference's constructor -- as any CoeffFct3D, which the
+// host evaluates at the quadrature points of every cell before each assembly
+// (cd3d_quad_points / cd3d_set_coeffs_qp).
 class TDiscreteForm3D {
  public:
   enum Kind { STIFFNESS = 0, MASS = 1 };
@@ -319,12 +337,64 @@ class TDiscreteForm3D {
     if (disctype != GALERKIN && disctype != SDFEM) Fail("Unknown DISCTYPE");
     co_.supg = disctype == SDFEM ? 1 : 0;
   }
+  // the reference's constructor (DiscreteForm3D.h:81-86): derivatives
+  // {D100, D010, D001, D000}, one matrix and one rhs on one space
+  TDiscreteForm3D(char *name, char *description, int n_terms, MultiIndex3D *derivatives, int *fespacenumber,
+                  int n_matrices, int n_rhs, int *rowspace, int *columnspace, int *rhsspace,
+                  AssembleFct3D *assemble, CoeffFct3D *coeffs, ManipulateFct3D *manipulate)
+      : kind_(STIFFNESS), fct_(coeffs) {
+    (void)name;
+    (void)description;
+    (void)fespacenumber;
+    (void)rowspace;
+    (void)columnspace;
+    (void)rhsspace;
+    if (assemble == &BilinearAssemble) co_.supg = 0;
+    else if (assemble == &BilinearAssemble_SD) co_.supg = 1;
+    else Fail("TDiscreteForm3D: the device path integrates BilinearAssemble and BilinearAssemble_SD");
+    if (n_matrices != 1 || n_rhs != 1 || n_terms != 4 || manipulate != nullptr)
+      Fail("TDiscreteForm3D: one matrix, one rhs, derivatives D100 D010 D001 D000 and no manipulate function");
+    const MultiIndex3D want[4] = {D100, D010, D001, D000};
+    for (int k = 0; k < 4; k++)
+      if (derivatives[k] != want[k]) Fail("TDiscreteForm3D: derivatives must be D100 D010 D001 D000");
+    if (!coeffs) Fail("TDiscreteForm3D: no coefficient function");
+  }
   const cd3d_coeffs &coeffs() const { return co_; }
   Kind kind() const { return kind_; }
+  CoeffFct3D *callback() const { return fct_; }
+  // the callback at every quadrature point of every level of h (as
+  // TFEDatabase3D::GetOrig + GetLocalForms would call it, cell by cell);
+  // Gauss3 (27 points) for Q1, Gauss4 (64 points) for Q2
+  void HandCoefficientsTo(cd3d_problem *h, int levels, int order) const {
+    if (!fct_) return;
+    const int nq = (order + 2) * (order + 2) * (order + 2);
+    for (int l = 0; l < levels; l++) {
+      int ndof, nact, ncells;
+      int64_t nnz;
+      Check(cd3d_level_info(h, l, &ndof, &nact, &nnz, &ncells), "cd3d_level_info", h);
+      std::vector<double> xyz((size_t)ncells * nq * 3), qp((size_t)ncells * nq * 6);
+      Check(cd3d_quad_points(h, l, xyz.data()), "cd3d_quad_points", h);
+      std::vector<double> X(nq), Y(nq), Z(nq), aux((size_t)nq * 20, 0.0);
+      std::vector<double *> cp(nq), pp(nq, nullptr);
+      for (int q = 0; q < nq; q++) cp[q] = aux.data() + 20 * q;
+      for (int c = 0; c < ncells; c++) {
+        for (int q = 0; q < nq; q++) {
+          X[q] = xyz[((size_t)c * nq + q) * 3];
+          Y[q] = xyz[((size_t)c * nq + q) * 3 + 1];
+          Z[q] = xyz[((size_t)c * nq + q) * 3 + 2];
+        }
+        fct_(nq, X.data(), Y.data(), Z.data(), pp.data(), cp.data());
+        for (int q = 0; q < nq; q++)
+          for (int k = 0; k < 6; k++) qp[((size_t)c * nq + q) * 6 + k] = cp[q][k];
+      }
+      Check(cd3d_set_coeffs_qp(h, l, qp.data()), "cd3d_set_coeffs_qp", h);
+    }
+  }
 
  private:
-  cd3d_coeffs co_;
+  cd3d_coeffs co_{};
   Kind kind_;
+  CoeffFct3D *fct_ = nullptr;
 };
 
 // TAssembleMat3D (AssembleMat3D.h:43-65) for one square matrix and one rhs
@@ -347,8 +417,9 @@ class TAssembleMat3D {
   void DeAllocate() {}
   void Assemble3D() {
     THierarchy &H = *space_->Hierarchy();
-    const bool same = memcmp(&H.last, &form_->coeffs(), sizeof(cd3d_coeffs)) == 0;
+    const bool same = memcmp(&H.last, &form_->coeffs(), sizeof(cd3d_coeffs)) == 0 && !form_->callback();
     if (!same || seen_ == H.generation) {
+      form_->HandCoefficientsTo(H.handle(), H.levels(), H.mesh().order);
       Check(cd3d_assemble(H.handle(), &form_->coeffs()), "cd3d_assemble", H.handle());
       H.generation++;
       H.last = form_->coeffs();
@@ -526,10 +597,25 @@ class TSystemCD3D {
     prec_.reset(new TMultiGridScaIte(MatVect_Scalar, Defect_Scalar, nullptr, 0, n, mg_.get(), 1));
     it_.reset(new TFgmresIte(MatVect_Scalar, Defect_Scalar, prec_.get(), 0, n, 1));
   }
-  // Init(CoeffFct3D*, BoundCond, BoundValue, aux) with the coefficients as a
-  // descriptor; g = 0 on every face
-  void Init(const cd3d_coeffs &coeffs) {
-    form_.reset(new TDiscreteForm3D(coeffs, disctype_));
+  // Init(CoeffFct3D*, BoundCond, BoundValue, aux) (SystemCD3D.C:227-311):
+  // the reference's signature, any coefficient function (evaluated at the
+  // quadrature points on the host); g = 0 on every face
+  void Init(CoeffFct3D *BilinearCoeffs, BoundCondFunct3D *BoundCond, BoundValueFunct3D *BoundValue, void *aux) {
+    (void)BoundCond;
+    (void)BoundValue;
+    (void)aux;
+    static char nm[] = "Conv-Diff", ds[] = "form";
+    static MultiIndex3D der[4] = {D100, D010, D001, D000};
+    static int zero[4] = {0, 0, 0, 0};
+    if (disctype_ != GALERKIN && disctype_ != SDFEM) Fail("Unknown or not yet implemented DISCTYPE");
+    InitForm(new TDiscreteForm3D(nm, ds, 4, der, zero, 1, 1, zero, zero, zero,
+                                 disctype_ == SDFEM ? &BilinearAssemble_SD : &BilinearAssemble, BilinearCoeffs,
+                                 nullptr));
+  }
+  // the same with the coefficients as a descriptor (no host callback)
+  void Init(const cd3d_coeffs &coeffs) { InitForm(new TDiscreteForm3D(coeffs, disctype_)); }
+  void InitForm(TDiscreteForm3D *form) {
+    form_.reset(form);
     for (int l = 0; l < n_levels_; l++) {
       TFESpace3D *fe[1] = {spaces_[l]};
       TSquareMatrix3D *A[1] = {matrices_[l].get()};

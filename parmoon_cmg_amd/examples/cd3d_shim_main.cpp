@@ -2,7 +2,8 @@
 // TCD3D_ParMooN.C, written against the drop-in classes of parmoon_cd3d.hpp.
 //
 //   cd3d_shim_main key=value ...
-//     mode=cd|mg|tcd   cd: TSystemCD3D; mg: TMultiGrid3D + TMGLevel3D +
+//     mode=cd|cdfct|mg|tcd   cd: TSystemCD3D (cdfct: Init with a CoeffFct3D,
+//                      coefficients varying in space); mg: TMultiGrid3D + TMGLevel3D +
 //                      TMultiGridScaIte + TFgmresIte wired by hand as in
 //                      SystemCD3D.C:138-203; tcd: TSystemTCD3D
 //     order k0 levels box=x,y,z eps b=x,y,z c f_kind f_val supg conv_kind
@@ -46,6 +47,23 @@ void write_out(const std::string &prefix, const std::vector<double> &x, int its,
   if (!f) Fail("cannot write " + prefix + ".txt");
   fprintf(f, "its %d\nr0 %.17g\n", its, r0);
   fclose(f);
+}
+
+// A coefficient function in the shape of an Examples/CD_3D/*.h BilinearCoeffs
+// (coefficients varying in space: eps (1 + xyz), b = (b1 sin(pi y),
+// b2 cos(pi x), b3 z), c x, f (1 + x)); its constants come from the command line
+double g_eps = 1.0, g_b[3] = {0, 0, 0}, g_c = 0.0, g_f = 1.0;
+void VarCoeffs(int n_points, double *X, double *Y, double *Z, double **, double **coeffs) {
+  for (int i = 0; i < n_points; i++) {
+    double *co = coeffs[i];
+    co[0] = g_eps * (1 + X[i] * Y[i] * Z[i]);
+    co[1] = g_b[0] * sin(M_PI * Y[i]);
+    co[2] = g_b[1] * cos(M_PI * X[i]);
+    co[3] = g_b[2] * Z[i];
+    co[4] = g_c * X[i];
+    co[5] = g_f * (1 + X[i]);
+    co[6] = sqrt(co[1] * co[1] + co[2] * co[2] + co[3] * co[3]);
+  }
 }
 
 }  // namespace
@@ -136,10 +154,18 @@ int main(int argc, char **argv) {
     return 0;
   }
 
-  if (mode == "cd") {
-    // CD3D_ParMooN.C:388-441
+  if (mode == "cd" || mode == "cdfct") {
+    // CD3D_ParMooN.C:388-441; cdfct: Init with a CoeffFct3D (VarCoeffs)
     TSystemCD3D sys(L, fes.data(), solp.data(), rhsp.data(), disctype, GMG);
-    sys.Init(co);
+    if (mode == "cdfct") {
+      g_eps = co.eps;
+      for (int d = 0; d < 3; d++) g_b[d] = co.b[d];
+      g_c = co.c;
+      g_f = co.f_val;
+      sys.Init(VarCoeffs, nullptr, nullptr, nullptr);
+    } else {
+      sys.Init(co);
+    }
     sys.Assemble();
     // stopping rule of the fixture runs: |r| <= rel |r0|
     std::vector<double> r(N);

@@ -81,6 +81,26 @@ def test_shim_cd3d_matches_reference(name, mode, golden, tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name", ["unit_q1_var_gal", "unit_q1_var_supg"])
+def test_shim_coefficient_function_matches_reference(name, golden, tmp_path):
+    # TSystemCD3D::Init with a CoeffFct3D (the reference's signature): the
+    # callback runs on the host at every quadrature point, the device
+    # assembles with its values; the reference run used the same function
+    c = CD_CASES[name]
+    d = golden(name)
+    q = c["qp"]
+    say(f"[gpu] shim cdfct {name}")
+    args = [a for a in case_args(c) if not a.startswith(("eps=", "b=", "c=", "f_val="))]
+    args += [f"eps={q['eps']!r}", "b=" + ",".join(map(repr, q["b"])), f"c={q['c']!r}", f"f_val={q['f']!r}"]
+    sv = c.get("solver", {})
+    args += [f"{k}={v!r}" for k, v in sv.items()]
+    x, its, _ = run_shim(["mode=cdfct"] + args, tmp_path, "cdfct")
+    err = np.linalg.norm(x - d["sol"]) / np.linalg.norm(d["sol"])
+    say(f"[gpu] shim cdfct {name}: its {its} (reference {int(d['its'])}), err {err:.2e}")
+    assert its == int(d["its"]) and err < 1e-10
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("name", ["q1_tcd", "unit_q1_tcd_supg_const"])
 def test_shim_tcd3d_matches_reference(name, golden, tmp_path):
     c = TCD_CASES[name]
