@@ -204,7 +204,8 @@ cd3d_status cd3d_interp_pulse(cd3d_problem *p, int level, const cd3d_coeffs *co,
 /* Diagnostics and output.  Known functions fn: 0 zero, 1 the sine
  * sin(pi x) sin(pi y) sin(pi z) of Examples/CD_3D/Laplace.h:12-19 (value and
  * gradient), 2 the Gaussian pulse centred at co->pulse_c (sigma 0.05; co may
- * be NULL for fn 0 and 1). */
+ * be NULL for fn 0, 1 and 3), 3 the linear function x + 2y + 3z (it lies in
+ * every Q1/Q2 space, also on trilinear cells, so its errors vanish). */
 /* TFEFunction3D::Interpolate (FEFunction3D.C:1446): nodal interpolation of fn
  * on level l, evaluated on the device */
 cd3d_status cd3d_interpolate(cd3d_problem *p, int level, int fn, const cd3d_coeffs *co, double *u_host);

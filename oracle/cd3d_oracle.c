@@ -19,6 +19,7 @@
 typedef struct {
   int n;             /* cells per side */
   int ncell;
+  double *vx;        /* mapped hierarchy: (n+1)^3 x 3 vertex coordinates, x fastest; NULL: the box */
   int *org;          /* ncell*3: cell vertex 0, in node-lattice units (spacing 1/(p n)) */
   int *ax;           /* ncell*9: ax[9c+3a+d] = component d of local axis a (signed unit) */
   int nn;            /* nodes per side p n + 1 */
@@ -38,6 +39,7 @@ typedef struct {
 
 struct orc_hier {
   int order, nb, L, m, k0;
+  int mapped;  /* vertices given per macro vertex (general hexahedra, trilinear maps) */
   double box[3];
   level_t *lv;
   double *lu;  /* coarse dense LU (exact coarse solve) */
@@ -142,6 +144,8 @@ static void build_mesh(level_t *L, int m, int depth, int p) {
       }
 }
 
+static void cell_vertices(const orc_hier *h, const level_t *L, int c, double V[8][3]);
+
 static inline int cell_node(const level_t *L, int c, int p, int i, int j, int k) {
   const int *O = L->org + 3 * c, *E = L->ax + 9 * c;
   int X = O[0] + i * E[0] + j * E[3] + k * E[6];
@@ -230,7 +234,37 @@ static void build_structure(level_t *L, int p) {
   free(len);
 }
 
-orc_hier *orc_create(int order, int m, int k0, int levels, const double *box) {
+/* One regular refinement of the vertex lattice of an n^3 structured mesh
+ * (Refinement.C, RefHexaRegDesc.C: the new vertex of an edge at
+ * 0.5 v0 + 0.5 v1, of a face at the mean of its 4 vertices, inside a cell at
+ * the mean of its 8 vertices, DatOldFaceNewVertexPos / DatNewVertexPos).
+ * Sums run over the corners in lattice order (x fastest). */
+static double *refine_vertices(const double *src, int n) {
+  int ns = n + 1, nd = 2 * n + 1;
+  double *dst = malloc(sizeof(double) * 3 * nd * nd * nd);
+  for (int K = 0; K < nd; K++)
+    for (int J = 0; J < nd; J++)
+      for (int I = 0; I < nd; I++) {
+        int odd[3] = {I & 1, J & 1, K & 1}, base[3] = {I >> 1, J >> 1, K >> 1};
+        int nodd = odd[0] + odd[1] + odd[2];
+        double wt = nodd == 0 ? 1.0 : nodd == 1 ? 0.5 : nodd == 2 ? 0.25 : 0.125;
+        double *o = dst + 3 * (I + nd * (J + nd * K));
+        o[0] = o[1] = o[2] = 0;
+        for (int t = 0; t < 8; t++) {
+          int e[3] = {t & 1, (t >> 1) & 1, (t >> 2) & 1}, skip = 0, v[3];
+          for (int d = 0; d < 3; d++) {
+            if (e[d] && !odd[d]) skip = 1;
+            v[d] = base[d] + e[d];
+          }
+          if (skip) continue;
+          const double *x = src + 3 * (v[0] + ns * (v[1] + ns * v[2]));
+          for (int d = 0; d < 3; d++) o[d] += wt * x[d];
+        }
+      }
+  return dst;
+}
+
+static orc_hier *create_(int order, int m, int k0, int levels, const double *box, const double *macro_xyz) {
   orc_hier *h = calloc(1, sizeof(orc_hier));
   for (int d = 0; d < 3; d++) h->box[d] = box ? box[d] : 1.0;
   h->order = order;
@@ -238,9 +272,29 @@ orc_hier *orc_create(int order, int m, int k0, int levels, const double *box) {
   h->L = levels;
   h->m = m;
   h->k0 = k0;
+  h->mapped = macro_xyz != NULL;
   h->lv = calloc(levels, sizeof(level_t));
+  double *vx = NULL;
+  int nv = m;
+  if (macro_xyz) {
+    vx = malloc(sizeof(double) * 3 * (m + 1) * (m + 1) * (m + 1));
+    memcpy(vx, macro_xyz, sizeof(double) * 3 * (m + 1) * (m + 1) * (m + 1));
+    for (int r = 0; r < k0; r++, nv *= 2) {
+      double *t = refine_vertices(vx, nv);
+      free(vx);
+      vx = t;
+    }
+  }
   for (int l = 0; l < levels; l++) {
     level_t *L = h->lv + l;
+    if (macro_xyz) {
+      if (l > 0) {
+        double *t = refine_vertices(vx, nv);
+        vx = t;
+        nv *= 2;
+      }
+      L->vx = vx;
+    }
     build_mesh(L, m, k0 + l, order);
     build_dofs(L, order);
     build_structure(L, order);
@@ -250,14 +304,23 @@ orc_hier *orc_create(int order, int m, int k0, int levels, const double *box) {
     L->aux0 = calloc(L->ndof, sizeof(double));
     L->aux1 = calloc(L->ndof, sizeof(double));
   }
+  if (macro_xyz && levels == 0) free(vx);
   return h;
+}
+
+orc_hier *orc_create(int order, int m, int k0, int levels, const double *box) {
+  return create_(order, m, k0, levels, box, NULL);
+}
+
+orc_hier *orc_create_mapped(int order, int m, int k0, int levels, const double *macro_xyz) {
+  return create_(order, m, k0, levels, NULL, macro_xyz);
 }
 
 void orc_destroy(orc_hier *h) {
   if (!h) return;
   for (int l = 0; l < h->L; l++) {
     level_t *L = h->lv + l;
-    free(L->org); free(L->ax); free(L->gn); free(L->dof_node); free(L->rowptr); free(L->kcol);
+    free(L->vx); free(L->org); free(L->ax); free(L->gn); free(L->dof_node); free(L->rowptr); free(L->kcol);
     free(L->A); free(L->Aconv); free(L->Ksupg); free(L->qp); free(L->rhs); free(L->sol); free(L->aux0); free(L->aux1); free(L->mult);
   }
   free(h->lv); free(h->lu); free(h->piv);
@@ -275,6 +338,10 @@ int orc_level_info(const orc_hier *h, int l, int *n_active, long *nnz, int *n_ce
 void orc_get_cells(const orc_hier *h, int l, double *xyz) {
   const level_t *L = h->lv + l;
   int S = h->order;
+  if (L->vx) {
+    for (int c = 0; c < L->ncell; c++) cell_vertices(h, L, c, (double(*)[3])(xyz + 24 * (size_t)c));
+    return;
+  }
   for (int c = 0; c < L->ncell; c++) {
     const int *O = L->org + 3 * c, *E = L->ax + 9 * c;
     for (int v = 0; v < 8; v++)
@@ -345,6 +412,177 @@ double orc_hconv_brick(const double *h, const double *b) {
   return 2 * nb / den;
 }
 
+/* Mesh_size_in_convection_direction_without_storing, hexahedron branch
+ * (ConvDiff.C:411-469) for any hexahedron: the trilinear function of
+ * (x, y, z) -- coefficients of 1, x, y, z, xy, xz, yz, xyz -- that is 1 at
+ * vertex i and 0 at the others, by LU with partial pivoting of the 8 x 8
+ * vertex matrix for 8 right-hand sides at once (SolveMultipleSystems,
+ * LinAlg.C:466-578), and its gradient at the barycentre. */
+static double hconv_hexa(const double V[8][3], const double *bv) {
+  double a[64], r[64], s[3] = {0, 0, 0};
+  for (int i = 0; i < 8; i++)
+    for (int d = 0; d < 3; d++) s[d] += V[i][d];
+  for (int d = 0; d < 3; d++) s[d] /= 8;
+  for (int i = 0; i < 8; i++) {
+    double x = V[i][0], y = V[i][1], z = V[i][2];
+    double row[8] = {1, x, y, z, x * y, x * z, y * z, x * y * z};
+    memcpy(a + 8 * i, row, sizeof row);
+  }
+  memset(r, 0, sizeof r);
+  for (int i = 0; i < 8; i++) r[9 * i] = 1; /* right-hand side j is r[8 j ..] */
+  for (int k = 0; k < 7; k++) {
+    double best = 0;
+    int pr = k;
+    for (int i = k; i < 8; i++)
+      if (fabs(a[8 * i + k]) > best) { best = fabs(a[8 * i + k]); pr = i; }
+    if (pr > k) {
+      for (int j = 0; j < 8; j++) { double t = a[8 * k + j]; a[8 * k + j] = a[8 * pr + j]; a[8 * pr + j] = t; }
+      for (int j = 0; j < 8; j++) { double t = r[8 * j + k]; r[8 * j + k] = r[8 * j + pr]; r[8 * j + pr] = t; }
+    }
+    double piv = a[8 * k + k];
+    for (int i = k + 1; i < 8; i++) a[8 * i + k] /= piv;
+    for (int j = k + 1; j < 8; j++) {
+      double t = a[8 * k + j];
+      for (int i = k + 1; i < 8; i++) a[8 * i + j] -= a[8 * i + k] * t;
+    }
+  }
+  for (int k = 0; k < 8; k++)
+    for (int j = 0; j < 8; j++) {
+      double t = r[8 * j + k];
+      for (int i = k + 1; i < 8; i++) r[8 * j + i] -= a[8 * i + k] * t;
+    }
+  for (int k = 7; k >= 0; k--)
+    for (int j = 0; j < 8; j++) {
+      r[8 * j + k] /= a[8 * k + k];
+      double t = r[8 * j + k];
+      for (int i = 0; i < k; i++) r[8 * j + i] -= a[8 * i + k] * t;
+    }
+  double nb = sqrt(bv[0] * bv[0] + bv[1] * bv[1] + bv[2] * bv[2]), den = 0;
+  for (int i = 0; i < 8; i++) {
+    const double *q = r + 8 * i;
+    double v = bv[0] * (q[1] + q[4] * s[1] + q[5] * s[2] + q[7] * s[1] * s[2]);
+    v += bv[1] * (q[2] + q[4] * s[0] + q[6] * s[2] + q[7] * s[0] * s[2]);
+    v += bv[2] * (q[3] + q[5] * s[0] + q[6] * s[1] + q[7] * s[0] * s[1]);
+    den += fabs(v);
+  }
+  if (den < 1e-10) { /* hK: the longest space diagonal (THexahedron::GetDiameter, Hexahedron.C:78-104) */
+    static const int DG[4][2] = {{0, 6}, {1, 7}, {2, 4}, {3, 5}};
+    double dm = 0;
+    for (int k = 0; k < 4; k++) {
+      double l2 = 0;
+      for (int d = 0; d < 3; d++) l2 += (V[DG[k][0]][d] - V[DG[k][1]][d]) * (V[DG[k][0]][d] - V[DG[k][1]][d]);
+      if (l2 > dm) dm = l2;
+    }
+    return sqrt(dm);
+  }
+  return 2 * nb / den;
+}
+
+/* ------------------------------------------------------------------ cell geometry */
+
+/* The map of a cell from the reference cube.  The reference maps every Q1/Q2
+ * hexahedron with THexaTrilinear (the elements' RefTrans, FEDatabase3D.C:632-634).
+ * On the box hierarchy the cells are bricks whose frames are signed axis
+ * permutations, and the trilinear map reduces to the scaled frame; on a
+ * mapped hierarchy the map is the trilinear interpolation of the cell's 8
+ * vertices (THexaTrilinear::SetCell, HexaTrilinear.C:884-948): x(r) =
+ * sum_k xc_k m_k(r) over m = 1, xi, eta, zeta, xi eta, xi zeta, eta zeta,
+ * xi eta zeta, xc_k = 1/8 sum_v m_k(r_v) x_v. */
+typedef struct {
+  int mapped;
+  double V[8][3];  /* vertices in the cell's local order */
+  double xc[3][8];
+  const int *O, *E;
+  double hd[3], dn[3], detj;
+} cellgeo_t;
+
+typedef struct {
+  double X[3];     /* the point in physical coordinates */
+  double C[3][3];  /* mapped: cofactors, physical gradient = C grad_ref / det */
+  double rdet;     /* mapped: 1 / det */
+  double absdet;   /* |det J| */
+} geopt_t;
+
+/* vertices of cell c in its local order (mapped hierarchies) */
+static void cell_vertices(const orc_hier *h, const level_t *L, int c, double V[8][3]) {
+  const int *O = L->org + 3 * c, *E = L->ax + 9 * c;
+  int nv = L->n + 1, p = h->order;
+  for (int v = 0; v < 8; v++) {
+    int I[3];
+    for (int d = 0; d < 3; d++) I[d] = O[d] / p + HEX_V[v][0] * E[d] + HEX_V[v][1] * E[3 + d] + HEX_V[v][2] * E[6 + d];
+    const double *x = L->vx + 3 * (I[0] + nv * (I[1] + nv * I[2]));
+    for (int d = 0; d < 3; d++) V[v][d] = x[d];
+  }
+}
+
+static void geo_cell(const orc_hier *h, const level_t *L, int c, cellgeo_t *G) {
+  G->mapped = L->vx != NULL;
+  G->O = L->org + 3 * c;
+  G->E = L->ax + 9 * c;
+  for (int d = 0; d < 3; d++) { G->hd[d] = h->box[d] / L->n; G->dn[d] = h->box[d] / (h->order * L->n); }
+  G->detj = (G->hd[0] / 2) * (G->hd[1] / 2) * (G->hd[2] / 2);
+  if (!G->mapped) return;
+  cell_vertices(h, L, c, G->V);
+  /* vertices in tensor order (x fastest): 0 1 3 2 4 5 7 6 of the local order */
+  static const int TV[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+  for (int d = 0; d < 3; d++)
+    for (int k = 0; k < 8; k++) {
+      double acc = 0;
+      for (int t = 0; t < 8; t++) {
+        double sx = (t & 1) ? 1 : -1, sy = (t & 2) ? 1 : -1, sz = (t & 4) ? 1 : -1;
+        double mk[8] = {1, sx, sy, sz, sx * sy, sx * sz, sy * sz, sx * sy * sz};
+        acc += mk[k] * G->V[TV[t]][d];
+      }
+      G->xc[d][k] = acc * 0.125;
+    }
+}
+
+/* THexaTrilinear::GetOrigFromRef / GetOrigValues (HexaTrilinear.C): the point,
+ * the Jacobian's cofactors and |det J| at the reference point (xi, eta, zeta) */
+static void geo_point(const cellgeo_t *G, double xi, double eta, double zeta, geopt_t *P) {
+  if (!G->mapped) {
+    double ref[3] = {xi, eta, zeta};
+    const int *O = G->O, *E = G->E;
+    for (int d = 0; d < 3; d++)
+      P->X[d] = O[d] * G->dn[d] + (ref[0] + 1) / 2 * G->hd[d] * E[d] + (ref[1] + 1) / 2 * G->hd[d] * E[3 + d] +
+                (ref[2] + 1) / 2 * G->hd[d] * E[6 + d];
+    P->absdet = G->detj;
+    return;
+  }
+  double J[3][3];
+  for (int d = 0; d < 3; d++) {
+    const double *c = G->xc[d];
+    P->X[d] = c[0] + c[1] * xi + c[2] * eta + c[3] * zeta + c[4] * xi * eta + c[5] * xi * zeta + c[6] * eta * zeta +
+              c[7] * xi * eta * zeta;
+    J[d][0] = c[1] + c[4] * eta + c[5] * zeta + c[7] * eta * zeta;
+    J[d][1] = c[2] + c[4] * xi + c[6] * zeta + c[7] * xi * zeta;
+    J[d][2] = c[3] + c[5] * xi + c[6] * eta + c[7] * xi * eta;
+  }
+  /* rows x, y, z of J: (dx1 dx2 dx3), (dy1 dy2 dy3), (dz1 dz2 dz3) */
+  double dx1 = J[0][0], dx2 = J[0][1], dx3 = J[0][2], dy1 = J[1][0], dy2 = J[1][1], dy3 = J[1][2];
+  double dz1 = J[2][0], dz2 = J[2][1], dz3 = J[2][2];
+  double det = dx1 * dy2 * dz3 + dx2 * dy3 * dz1 + dx3 * dy1 * dz2 - dx3 * dy2 * dz1 - dx2 * dy1 * dz3 - dx1 * dy3 * dz2;
+  P->C[0][0] = dy2 * dz3 - dy3 * dz2; P->C[0][1] = dy3 * dz1 - dy1 * dz3; P->C[0][2] = dy1 * dz2 - dy2 * dz1;
+  P->C[1][0] = dx3 * dz2 - dx2 * dz3; P->C[1][1] = dx1 * dz3 - dx3 * dz1; P->C[1][2] = dx2 * dz1 - dx1 * dz2;
+  P->C[2][0] = dx2 * dy3 - dy2 * dx3; P->C[2][1] = dx3 * dy1 - dx1 * dy3; P->C[2][2] = dx1 * dy2 - dx2 * dy1;
+  P->rdet = 1 / det;
+  P->absdet = fabs(det);
+}
+
+/* physical gradient of a basis function from its reference derivatives */
+static void geo_grad(const cellgeo_t *G, const geopt_t *P, double d0, double d1, double d2, double g[3]) {
+  if (!G->mapped) {
+    const int *E = G->E;
+    for (int d = 0; d < 3; d++) g[d] = (2 / G->hd[d]) * (E[d] * d0 + E[3 + d] * d1 + E[6 + d] * d2);
+    return;
+  }
+  for (int d = 0; d < 3; d++) g[d] = (P->C[d][0] * d0 + P->C[d][1] * d1 + P->C[d][2] * d2) * P->rdet;
+}
+
+static double geo_hconv(const cellgeo_t *G, const double *b) {
+  return G->mapped ? hconv_hexa(G->V, b) : orc_hconv_brick(G->hd, b);
+}
+
 /* Compute_SDFEM_delta, SDFEM_TYPE 2 (ConvDiff.C:494-600) with DELTA0 = 1 and
  * INTERNAL_POLYNOMIAL_DEGREE = order (SetParametersCD, ConvDiff.C:160-177). */
 double orc_sdfem_delta(double hconv, double eps, const double *b, int order) {
@@ -374,10 +612,8 @@ static void local_forms(const orc_hier *h, const level_t *L, int c, const orc_co
                         double tscale, double *Aloc, double *bloc) {
   int p = h->order, nb = h->nb, n1 = p + 2; /* Gauss3 for Q1, Gauss4 for Q2 */
   const double *gx = p == 1 ? G3X : G4X, *gw = p == 1 ? G3W : G4W;
-  double hd[3], dn[3];
-  for (int d = 0; d < 3; d++) { hd[d] = h->box[d] / L->n; dn[d] = h->box[d] / (p * L->n); }
-  const int *O = L->org + 3 * c, *E = L->ax + 9 * c;
-  double detj = (hd[0] / 2) * (hd[1] / 2) * (hd[2] / 2);
+  cellgeo_t G;
+  geo_cell(h, L, c, &G);
   memset(Aloc, 0, sizeof(double) * nb * nb);
   memset(bloc, 0, sizeof(double) * nb);
   double hconv = -1;
@@ -386,16 +622,15 @@ static void local_forms(const orc_hier *h, const level_t *L, int c, const orc_co
       for (int qk = 0; qk < n1; qk++) {
         double xi = gx[qk], eta = gx[qj], zeta = gx[qi];
         double w = gw[qi] * gw[qj] * gw[qk];
-        double Mult = w * detj;
-        double ref[3] = {xi, eta, zeta}, X[3];
-        for (int d = 0; d < 3; d++)
-          X[d] = O[d] * dn[d] + (ref[0] + 1) / 2 * hd[d] * E[d] + (ref[1] + 1) / 2 * hd[d] * E[3 + d] +
-                 (ref[2] + 1) / 2 * hd[d] * E[6 + d];
+        geopt_t gp;
+        geo_point(&G, xi, eta, zeta, &gp);
+        double Mult = w * gp.absdet;
+        const double *X = gp.X;
         double phi[MAXNB], d0[MAXNB], d1[MAXNB], d2[MAXNB], gx_[MAXNB], gy_[MAXNB], gz_[MAXNB];
         basis3d(p, xi, eta, zeta, phi, d0, d1, d2);
         for (int b = 0; b < nb; b++) {
           double g[3];
-          for (int d = 0; d < 3; d++) g[d] = (2 / hd[d]) * (E[d] * d0[b] + E[3 + d] * d1[b] + E[6 + d] * d2[b]);
+          geo_grad(&G, &gp, d0[b], d1[b], d2[b], g);
           gx_[b] = g[0]; gy_[b] = g[1]; gz_[b] = g[2];
         }
         double cf[7];
@@ -410,10 +645,10 @@ static void local_forms(const orc_hier *h, const level_t *L, int c, const orc_co
         if (co->supg && what != 1 && tscale > 0) {
           /* TCD3D.C:141-156: tilde tau of (tscale eps, tscale b), then tau *= tscale */
           double sb[3] = {tscale * cf[1], tscale * cf[2], tscale * cf[3]};
-          if (hconv < 0) hconv = orc_hconv_brick(hd, sb); /* cached per cell: AssembleMat3D.C:457 */
+          if (hconv < 0) hconv = geo_hconv(&G, sb); /* cached per cell: AssembleMat3D.C:457 */
           delta = orc_sdfem_delta(hconv, tscale * cf[0], sb, p) * tscale;
         } else if (co->supg) {
-          if (hconv < 0) hconv = orc_hconv_brick(hd, cf + 1); /* cached per cell: AssembleMat3D.C:457 */
+          if (hconv < 0) hconv = geo_hconv(&G, cf + 1); /* cached per cell: AssembleMat3D.C:457 */
           delta = orc_sdfem_delta(hconv, cf[0], cf + 1, p);
         }
         if (what == 1) {
@@ -457,18 +692,16 @@ void orc_quad_points(const orc_hier *h, int l, double *xyz) {
   const level_t *L = h->lv + l;
   int p = h->order, n1 = p + 2;
   const double *gx = p == 1 ? G3X : G4X;
-  double hd[3], dn[3];
-  for (int d = 0; d < 3; d++) { hd[d] = h->box[d] / L->n; dn[d] = h->box[d] / (p * L->n); }
   size_t k = 0;
   for (int c = 0; c < L->ncell; c++) {
-    const int *O = L->org + 3 * c, *E = L->ax + 9 * c;
+    cellgeo_t G;
+    geo_cell(h, L, c, &G);
     for (int qi = 0; qi < n1; qi++)
       for (int qj = 0; qj < n1; qj++)
         for (int qk = 0; qk < n1; qk++) {
-          double ref[3] = {gx[qk], gx[qj], gx[qi]};
-          for (int d = 0; d < 3; d++)
-            xyz[k++] = O[d] * dn[d] + (ref[0] + 1) / 2 * hd[d] * E[d] + (ref[1] + 1) / 2 * hd[d] * E[3 + d] +
-                       (ref[2] + 1) / 2 * hd[d] * E[6 + d];
+          geopt_t gp;
+          geo_point(&G, gx[qk], gx[qj], gx[qi], &gp);
+          for (int d = 0; d < 3; d++) xyz[k++] = gp.X[d];
         }
   }
 }
@@ -922,8 +1155,14 @@ done:
 
 /* ------------------------------------------------------------------ TCD3D */
 
+static void interp_mapped(const orc_hier *h, int l, int fn, const orc_coeffs *co, double *u);
+
 void orc_interp_pulse(const orc_hier *h, int l, const orc_coeffs *co, double *u) {
   const level_t *L = h->lv + l;
+  if (L->vx) {
+    interp_mapped(h, l, 2, co, u);
+    return;
+  }
   const double *pc = co->pulse_c;
   for (int d = 0; d < L->ndof; d++) {
     int node = L->dof_node[d];
@@ -938,7 +1177,8 @@ void orc_interp_pulse(const orc_hier *h, int l, const orc_coeffs *co, double *u)
 /* ------------------------------------------------------------------ diagnostics */
 
 /* Known functions: 0 zero; 1 the sine of Examples/CD_3D/Laplace.h:12-19
- * (value and gradient); 2 the Gaussian pulse (centre co->pulse_c, sigma 0.05). */
+ * (value and gradient); 2 the Gaussian pulse (centre co->pulse_c, sigma 0.05);
+ * 3 the linear function x + 2y + 3z. */
 static void known_fn(int fn, const orc_coeffs *co, double x, double y, double z, double *v) {
   v[0] = v[1] = v[2] = v[3] = 0;
   if (fn == 1) {
@@ -953,13 +1193,41 @@ static void known_fn(int fn, const orc_coeffs *co, double x, double y, double z,
     v[1] = -dx / s2 * v[0];
     v[2] = -dy / s2 * v[0];
     v[3] = -dz / s2 * v[0];
+  } else if (fn == 3) { /* x + 2y + 3z: in every Q1/Q2 space on trilinear cells */
+    v[0] = x + 2 * y + 3 * z;
+    v[1] = 1; v[2] = 2; v[3] = 3;
   }
 }
 
 /* TFEFunction3D::Interpolate (FEFunction3D.C:1446-1560) for Lagrange Q1/Q2:
  * the nodal functionals are point values at the nodes. */
+/* on a mapped hierarchy: cell by cell, the nodes' reference points mapped by
+ * the cell's trilinear map (GetOrigFromRef); a DOF shared by several cells
+ * keeps the value of the last of them, as in the reference's loop */
+static void interp_mapped(const orc_hier *h, int l, int fn, const orc_coeffs *co, double *u) {
+  const level_t *L = h->lv + l;
+  int p = h->order, nb = h->nb, q = p + 1;
+  memset(u, 0, sizeof(double) * L->ndof);
+  for (int c = 0; c < L->ncell; c++) {
+    cellgeo_t G;
+    geo_cell(h, L, c, &G);
+    for (int b = 0; b < nb; b++) {
+      int i = b % q, j = (b / q) % q, k = b / (q * q);
+      geopt_t gp;
+      geo_point(&G, -1.0 + 2.0 * i / p, -1.0 + 2.0 * j / p, -1.0 + 2.0 * k / p, &gp);
+      double v[4];
+      known_fn(fn, co, gp.X[0], gp.X[1], gp.X[2], v);
+      u[L->gn[c * nb + b]] = v[0];
+    }
+  }
+}
+
 void orc_interpolate(const orc_hier *h, int l, int fn, const orc_coeffs *co, double *u) {
   const level_t *L = h->lv + l;
+  if (L->vx) {
+    interp_mapped(h, l, fn, co, u);
+    return;
+  }
   double dn = 1.0 / (h->order * L->n);
   for (int d = 0; d < L->ndof; d++) {
     int node = L->dof_node[d];
@@ -980,29 +1248,28 @@ void orc_errors(const orc_hier *h, int l, const double *u, int fn, const orc_coe
   const level_t *L = h->lv + l;
   int p = h->order, nb = h->nb, n1 = p + 2;
   const double *gx = p == 1 ? G3X : G4X, *gw = p == 1 ? G3W : G4W;
-  double hd[3], dn[3];
-  for (int d = 0; d < 3; d++) { hd[d] = h->box[d] / L->n; dn[d] = h->box[d] / (p * L->n); }
-  double detj = (hd[0] / 2) * (hd[1] / 2) * (hd[2] / 2);
   double e0 = 0, e1 = 0, emax = 0, elast = 0;
   for (int c = 0; c < L->ncell; c++) {
-    const int *O = L->org + 3 * c, *E = L->ax + 9 * c, *g = L->gn + c * nb;
+    const int *g = L->gn + c * nb;
+    cellgeo_t G;
+    geo_cell(h, L, c, &G);
     double cellmax = 0;
     for (int qi = 0; qi < n1; qi++)
       for (int qj = 0; qj < n1; qj++)
         for (int qk = 0; qk < n1; qk++) {
           double xi = gx[qk], eta = gx[qj], zeta = gx[qi];
-          double w = gw[qi] * gw[qj] * gw[qk] * detj;
-          double ref[3] = {xi, eta, zeta}, X[3];
-          for (int d = 0; d < 3; d++)
-            X[d] = O[d] * dn[d] + (ref[0] + 1) / 2 * hd[d] * E[d] + (ref[1] + 1) / 2 * hd[d] * E[3 + d] +
-                   (ref[2] + 1) / 2 * hd[d] * E[6 + d];
+          geopt_t gp;
+          geo_point(&G, xi, eta, zeta, &gp);
+          double w = gw[qi] * gw[qj] * gw[qk] * gp.absdet;
+          const double *X = gp.X;
           double phi[MAXNB], d0[MAXNB], d1[MAXNB], d2[MAXNB];
           basis3d(p, xi, eta, zeta, phi, d0, d1, d2);
           double uh = 0, gr[3] = {0, 0, 0};
           for (int b = 0; b < nb; b++) {
-            double ub = u[g[b]];
+            double ub = u[g[b]], gb[3];
             uh += ub * phi[b];
-            for (int d = 0; d < 3; d++) gr[d] += ub * (2 / hd[d]) * (E[d] * d0[b] + E[3 + d] * d1[b] + E[6 + d] * d2[b]);
+            geo_grad(&G, &gp, d0[b], d1[b], d2[b], gb);
+            for (int d = 0; d < 3; d++) gr[d] += ub * gb[d];
           }
           double v[4];
           known_fn(fn, co, X[0], X[1], X[2], v);

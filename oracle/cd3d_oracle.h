@@ -54,6 +54,11 @@ typedef struct orc_hier orc_hier;
  * cells refined k0 times (n0 = m * 2^k0 cells per side); m = 1 is ParMooN's
  * Wuerfel cube (which the reference reads as the box (1, 0.346, 0.346)). */
 orc_hier *orc_create(int order, int m, int k0, int levels, const double *box);
+/* The same hierarchy over general hexahedra: the m^3 macro cells (x fastest,
+ * identity local frames) have the given vertices ((m+1)^3 x 3, x fastest),
+ * refinement places new vertices at edge, face and cell means, and every cell
+ * is mapped trilinearly (THexaTrilinear). */
+orc_hier *orc_create_mapped(int order, int m, int k0, int levels, const double *macro_xyz);
 void orc_destroy(orc_hier *h);
 
 /* sizes of level l: returns n_dof; fills n_active, nnz, n_cells */
@@ -99,7 +104,8 @@ int orc_tcd_run(orc_hier *h, const orc_coeffs *co, const orc_solver *s, double t
 /* nodal interpolation of the Gaussian pulse initial condition on level l */
 void orc_interp_pulse(const orc_hier *h, int l, const orc_coeffs *co, double *u);
 
-/* known functions: 0 zero, 1 sin(pi x) sin(pi y) sin(pi z), 2 the Gaussian pulse at co->pulse_c */
+/* known functions: 0 zero, 1 sin(pi x) sin(pi y) sin(pi z), 2 the Gaussian pulse at co->pulse_c,
+ * 3 x + 2y + 3z */
 /* nodal interpolation on level l (TFEFunction3D::Interpolate) */
 void orc_interpolate(const orc_hier *h, int l, int fn, const orc_coeffs *co, double *u);
 /* TFEFunction3D::GetErrors with L2H1Errors: err[0] L2, err[1] H1 semi-norm of

@@ -14,9 +14,12 @@ plane components of unitcube.PRM (1 z=0, 2 x=0, 3 y=0, 4 z=1, 5 x=1, 6 y=1).
 A deformation can be given to move the interior vertices (general
 hexahedra, trilinear maps); boundary vertices stay on their planes.
 
-usage: python oracle/data/make_box_geo.py m [out.GEO]
+usage: python oracle/data/make_box_geo.py m|kind [out.GEO]   (kind: m3, m3w, ...)
 """
+import math
 import sys
+
+import numpy as np
 
 # local vertex offsets of a hexahedron in the reference's order
 HEXV = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
@@ -76,11 +79,40 @@ def box_geo(m, box=(1.0, 1.0, 1.0), deform=None):
     return "\n".join(lines) + "\n"
 
 
+def wavy(i, j, k):
+    """Displacement of interior macro vertex (i, j, k), in macro cell sizes:
+    up to 0.15 per axis, so that neighbouring vertices keep their order and
+    every cell stays a valid (non-parallelepiped) hexahedron."""
+    return (0.15 * math.sin(2.1 * i + 1.3 * j + 0.7 * k),
+            0.15 * math.sin(0.9 * i + 2.3 * j + 1.1 * k + 1.0),
+            0.15 * math.sin(1.7 * i + 0.5 * j + 2.9 * k + 2.0))
+
+
+DEFORMS = {"w": wavy}
+
+
+def geo_for_kind(kind):
+    """GEO text for a mesh kind "m<m>" (bricks) or "m<m><d>" (interior
+    vertices moved by DEFORMS[d])."""
+    body = kind[1:]
+    d = body[-1] if body[-1] in DEFORMS else None
+    m = int(body[:-1] if d else body)
+    return m, box_geo(m, deform=DEFORMS[d] if d else None)
+
+
+def macro_vertices(text):
+    """(m+1)^3 x 3 vertex coordinates of a GEO text as box_geo writes it."""
+    lines = text.split("\n")
+    nv = int(lines[2].split()[1])
+    return np.array([[float(x) for x in lines[4 + k].split()] for k in range(nv)])
+
+
 def main():
-    m = int(sys.argv[1])
-    out = sys.argv[2] if len(sys.argv) > 2 else f"box_m{m}.GEO"
+    kind = sys.argv[1] if sys.argv[1].startswith("m") else "m" + sys.argv[1]
+    m, text = geo_for_kind(kind)
+    out = sys.argv[2] if len(sys.argv) > 2 else f"box_{kind}.GEO"
     with open(out, "w") as f:
-        f.write(box_geo(m))
+        f.write(text)
 
 
 if __name__ == "__main__":

@@ -63,6 +63,8 @@ def lib():
         dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
         L.orc_create.restype = P
         L.orc_create.argtypes = [I, I, I, I, dp]
+        L.orc_create_mapped.restype = P
+        L.orc_create_mapped.argtypes = [I, I, I, I, dp]
         L.orc_destroy.argtypes = [P]
         L.orc_level_info.restype = I
         L.orc_level_info.argtypes = [P, I, ip, C.POINTER(C.c_long), ip]
@@ -109,12 +111,19 @@ def _ip(a):
 class Hierarchy:
     """Multilevel CD3D problem on the oracle side."""
 
-    def __init__(self, order, m, k0, levels, box=REF_BOX):
+    def __init__(self, order, m, k0, levels, box=REF_BOX, vertices=None):
+        """vertices: ((m+1)^3, 3) macro vertex coordinates (x fastest) of a
+        mapped hierarchy (general hexahedra); None: the box."""
         self.L = lib()
         self.order, self.levels = order, levels
         self.nb = (order + 1) ** 3
-        b = np.array(box, dtype=np.float64)
-        self.h = self.L.orc_create(order, m, k0, levels, _dp(b))
+        if vertices is not None:
+            v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1)
+            assert v.size == 3 * (m + 1) ** 3
+            self.h = self.L.orc_create_mapped(order, m, k0, levels, _dp(v))
+        else:
+            b = np.array(box, dtype=np.float64)
+            self.h = self.L.orc_create(order, m, k0, levels, _dp(b))
 
     def __del__(self):
         if getattr(self, "h", None):

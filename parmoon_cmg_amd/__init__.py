@@ -359,7 +359,7 @@ class Problem:
         self._chk(self.L.cd3d_set_coeffs_qp(self.h, l, _dp(qp)), "set_coeffs_qp")
 
     # known functions of interpolate / errors: zero, the sine of Laplace.h, the Gaussian pulse
-    FN_ZERO, FN_SINE, FN_PULSE = 0, 1, 2
+    FN_ZERO, FN_SINE, FN_PULSE, FN_LINEAR = 0, 1, 2, 3
 
     def interpolate(self, l, fn, co=None):
         """Nodal interpolation of a known function on level l (on the device)."""

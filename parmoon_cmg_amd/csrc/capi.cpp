@@ -1531,7 +1531,7 @@ cd3d_status cd3d_set_coeffs_qp(cd3d_problem *p, int level, const double *coeffs)
 // ---------------------------------------------------------------- diagnostics and output
 
 static FnParams fn_params(int fn, const cd3d_coeffs *co) {
-  if (fn < 0 || fn > 2) throw ArgError("fn must be 0 (zero), 1 (sine) or 2 (pulse)");
+  if (fn < 0 || fn > 3) throw ArgError("fn must be 0 (zero), 1 (sine), 2 (pulse) or 3 (linear)");
   FnParams f;
   f.fn = fn;
   if (co)

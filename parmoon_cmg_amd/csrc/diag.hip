@@ -15,7 +15,8 @@ namespace {
 constexpr int kDiagBlock = 256;
 
 // known functions: 0 zero; 1 sin(pi x) sin(pi y) sin(pi z) (Examples/CD_3D/Laplace.h:12-19);
-// 2 the Gaussian pulse exp(-|x - c|^2 / (2 0.05^2)); v = value, gradient
+// 2 the Gaussian pulse exp(-|x - c|^2 / (2 0.05^2)); 3 x + 2y + 3z (in every
+// Q1/Q2 space on trilinear cells: its errors vanish); v = value, gradient
 __device__ inline void known_fn(const FnParams &f, double x, double y, double z, double *v) {
   v[0] = v[1] = v[2] = v[3] = 0.0;
   if (f.fn == 1) {
@@ -30,6 +31,11 @@ __device__ inline void known_fn(const FnParams &f, double x, double y, double z,
     v[1] = -dx / s2 * v[0];
     v[2] = -dy / s2 * v[0];
     v[3] = -dz / s2 * v[0];
+  } else if (f.fn == 3) {
+    v[0] = x + 2 * y + 3 * z;
+    v[1] = 1.0;
+    v[2] = 2.0;
+    v[3] = 3.0;
   }
 }
 

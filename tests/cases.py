@@ -55,6 +55,16 @@ CD_CASES = {
                              qp=dict(eps=1e-3, b=(0.8, -0.6, 0.5), c=0.3, f=1.0), co=dict(eps=1e-3, f_val=1.0, supg=1)),
     "m2_q2_var_supg": dict(order=2, m=2, k0=0, levels=3, box=UNIT_BOX, solver=dict(omega=0.5, pre=3, post=3),
                            qp=dict(eps=1e-3, b=(0.8, -0.6, 0.5), c=0.3, f=1.0), co=dict(eps=1e-3, f_val=1.0, supg=1)),
+    # general hexahedra: the interior macro vertices moved (geo kind "m<m>w",
+    # oracle/data/make_box_geo.py), every cell mapped trilinearly
+    "m3w_q1_poisson": dict(order=1, m=3, k0=0, levels=3, box=UNIT_BOX, geo="m3w",
+                           co=dict(eps=1.0, f_kind=1)),
+    "m3w_q2_supg": dict(order=2, m=3, k0=0, levels=2, box=UNIT_BOX, geo="m3w", solver=dict(omega=0.5, pre=3, post=3),
+                        co=dict(eps=1e-3, b=(1.0, 0.5, 0.25), f_val=1.0, supg=1)),
+    "m2w_q1_var_supg": dict(order=1, m=2, k0=1, levels=3, box=UNIT_BOX, geo="m2w", solver=dict(omega=0.5, pre=3, post=3),
+                            qp=dict(eps=1e-3, b=(0.8, -0.6, 0.5), c=0.3, f=1.0), co=dict(eps=1e-3, f_val=1.0, supg=1)),
+    "m2w_q2_supg_rot": dict(order=2, m=2, k0=0, levels=3, box=UNIT_BOX, geo="m2w", solver=dict(omega=0.5, pre=3, post=3),
+                            co=dict(eps=1e-4, conv_kind=1, rot_c=(0.45, 0.55), c=0.2, f_val=1.0, supg=1)),
     # the benchmark's smoother: damped Jacobi omega 0.5, 3 + 3 sweeps
     "unit_c2_n8_bench": dict(order=2, k0=1, levels=3, box=UNIT_BOX, solver=dict(omega=0.5, pre=3, post=3),
                              co=dict(eps=1e-3, b=(1.0, 0.5, 0.25), f_val=1.0, supg=1)),
@@ -75,6 +85,10 @@ TCD_CASES = {
     "m2_q2_tcd_supg": dict(solver=dict(omega=0.5, pre=3, post=3), order=2, m=2, k0=0, levels=2, tau=1e-3, nsteps=3, box=UNIT_BOX,
                            co=dict(eps=1e-4, conv_kind=1, rot_c=(0.5, 0.5), pulse_c=(0.3, 0.5, 0.5), f_val=1.0,
                                    supg=1)),
+    "m2w_q2_tcd_supg": dict(solver=dict(omega=0.5, pre=3, post=3), order=2, m=2, k0=0, levels=2, tau=1e-3, nsteps=3,
+                            box=UNIT_BOX, geo="m2w",
+                            co=dict(eps=1e-4, conv_kind=1, rot_c=(0.5, 0.5), pulse_c=(0.3, 0.5, 0.5), f_val=1.0,
+                                    supg=1)),
     "unit_q1_tcd_supg_const": dict(solver=dict(omega=0.5, pre=3, post=3), order=1, k0=1, levels=3, tau=1e-2, nsteps=3, box=UNIT_BOX,
                                    co=dict(eps=1e-3, b=(0.6, -0.3, 0.4), c=0.1, pulse_c=(0.3, 0.5, 0.5),
                                            f_val=1.0, supg=1)),
@@ -94,6 +108,35 @@ def case_solver(name):
 
 def case_box(c):
     return c.get("box", REF_BOX)
+
+
+def _box_geo_module():
+    import os
+    import sys
+    d = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "data")
+    if d not in sys.path:
+        sys.path.insert(0, d)
+    import make_box_geo
+    return make_box_geo
+
+
+def geo_text(c):
+    """GEO file text of a deformed-mesh case (what the reference run read)."""
+    return _box_geo_module().geo_for_kind(c["geo"])[1]
+
+
+def case_vertices(c):
+    """(m+1)^3 x 3 macro vertex coordinates of a deformed-mesh case, None for a box."""
+    if "geo" not in c:
+        return None
+    return _box_geo_module().macro_vertices(geo_text(c))
+
+
+def oracle_hier(c, levels=None):
+    """The oracle hierarchy of a case (box or mapped)."""
+    from oracle import pyoracle as po
+    lv = c["levels"] if levels is None else levels
+    return po.Hierarchy(c["order"], c.get("m", 1), c["k0"], lv, box=case_box(c), vertices=case_vertices(c))
 
 
 def tcd_solver(c):

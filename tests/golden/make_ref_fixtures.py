@@ -168,6 +168,31 @@ CASES = {
                                                P0=1e-3, P1=0.6, P2=-0.3, P3=0.4, P4=0.1, P5=0, P6=1.0, P7=0,
                                                P8=3, P9=0.3, P10=0.5, P11=0.5,
                                                TIME_DISC=2, STARTTIME=0.0, ENDTIME=1.0, TIMESTEPLENGTH=1e-2)),
+    # general hexahedra: the interior macro vertices of the m^3 box moved
+    # (make_box_geo.py "w"), every cell mapped trilinearly (THexaTrilinear)
+    "m3w_q1_poisson": ("cd", dict(geo="m3w", ANSATZ_ORDER=1, LEVELS=3, UNIFORM_STEPS=0, DISCTYPE=1,
+                                  P0=1.0, P1=0, P2=0, P3=0, P4=0, P5=1)),
+    "m3w_q2_supg": ("cd", dict(geo="m3w", ANSATZ_ORDER=2, LEVELS=2, UNIFORM_STEPS=0, DISCTYPE=2,
+                               CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                               SC_SMOOTH_DAMP_FACTOR_SCALAR=0.5, SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=0.5,
+                               SC_PRE_SMOOTH_SCALAR=3, SC_POST_SMOOTH_SCALAR=3,
+                               P0=1e-3, P1=1.0, P2=0.5, P3=0.25, P4=0, P5=0, P6=1.0)),
+    "m2w_q1_var_supg": ("cd", dict(geo="m2w", ANSATZ_ORDER=1, LEVELS=3, UNIFORM_STEPS=1, DISCTYPE=2,
+                                   CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                                   SC_SMOOTH_DAMP_FACTOR_SCALAR=0.5, SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=0.5,
+                                   SC_PRE_SMOOTH_SCALAR=3, SC_POST_SMOOTH_SCALAR=3,
+                                   P0=1e-3, P1=0.8, P2=-0.6, P3=0.5, P4=0.3, P5=0, P6=1.0, P7=2)),
+    "m2w_q2_supg_rot": ("cd", dict(geo="m2w", ANSATZ_ORDER=2, LEVELS=3, UNIFORM_STEPS=0, DISCTYPE=2,
+                                   CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                                   SC_SMOOTH_DAMP_FACTOR_SCALAR=0.5, SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=0.5,
+                                   SC_PRE_SMOOTH_SCALAR=3, SC_POST_SMOOTH_SCALAR=3,
+                                   P0=1e-4, P4=0.2, P5=0, P6=1.0, P7=1, P12=0.45, P13=0.55)),
+    "m2w_q2_tcd_supg": ("tcdsupg", dict(SC_SMOOTH_DAMP_FACTOR_SCALAR=0.5, SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=0.5,
+                                        SC_PRE_SMOOTH_SCALAR=3, SC_POST_SMOOTH_SCALAR=3, geo="m2w", ANSATZ_ORDER=2, LEVELS=2, UNIFORM_STEPS=0, DISCTYPE=2,
+                                        CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,
+                                        P0=1e-4, P4=0, P5=0, P6=1.0, P7=1, P8=3,
+                                        P9=0.3, P10=0.5, P11=0.5, P12=0.5, P13=0.5,
+                                        TIME_DISC=2, STARTTIME=0.0, ENDTIME=1.0, TIMESTEPLENGTH=1e-3)),
 }
 
 # Records only (iteration count and residual history, no level data): the
@@ -224,10 +249,10 @@ def write_dat(path, overrides):
     elif geo_kind is not None and geo_kind.startswith("m"):
         # unit cube split into m^3 macro bricks (oracle/data/make_box_geo.py)
         sys.path.insert(0, data)
-        from make_box_geo import box_geo
+        from make_box_geo import geo_for_kind
         geo = os.path.join(os.path.dirname(path), f"box_{geo_kind}.GEO")
         with open(geo, "w") as f:
-            f.write(box_geo(int(geo_kind[1:])))
+            f.write(geo_for_kind(geo_kind)[1])
         prm = os.path.join(data, "unitcube.PRM")
     else:
         geo, prm = f"{REF}/data/Wuerfel.GEO", f"{REF}/data/Wuerfel.PRM"
@@ -252,6 +277,11 @@ def run_case(name):
             print(p.stdout[-3000:], p.stderr[-3000:])
             raise SystemExit(f"{name}: reference run failed ({p.returncode})")
         out = {"params": np.array(repr(ov)), "box": np.array(UNIT if ov.get("geo") else BOX)}
+        gk = ov.get("geo") or ""
+        if gk.startswith("m") and not gk[-1].isdigit():  # deformed: the macro vertices the reference read
+            sys.path.insert(0, os.path.join(REPO, "oracle", "data"))
+            from make_box_geo import geo_for_kind, macro_vertices
+            out["macro_xyz"] = macro_vertices(geo_for_kind(gk)[1])
         levels = [tuple(map(int, re.findall(r"-?\d+", l)[:8]))
                   for l in p.stdout.splitlines() if l.startswith("LEVEL ")]
         for lev, nc, nbf, ndof, act, hb, ndir, nnz in levels:

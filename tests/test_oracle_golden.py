@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as po
-from tests.cases import CD_CASES, TCD_CASES, REF_SOLVER, case_box, case_solver, qp_coeffs, tcd_solver
+from tests.cases import CD_CASES, TCD_CASES, REF_SOLVER, case_box, case_solver, oracle_hier, qp_coeffs, tcd_solver
 
 ENTRY_TOL = 1e-12
 SOL_TOL = 1e-10
@@ -38,7 +38,7 @@ def hier():
             c = CD_CASES[name]
             d = golden(name)
             assert tuple(d["box"]) == tuple(case_box(c))
-            H = po.Hierarchy(c["order"], c.get("m", 1), c["k0"], c["levels"], box=tuple(d["box"]))
+            H = oracle_hier(c)
             if "qp" in c:  # the reference's CoeffFct3D at every quadrature point
                 for l in range(c["levels"]):
                     H.set_coeffs_qp(l, qp_coeffs(c, H.quad_points(l)))
@@ -107,7 +107,7 @@ def test_tcd_crank_nicolson(name, golden):
     c = TCD_CASES[name]
     d = golden(name)
     assert tuple(d["box"]) == tuple(c["box"])
-    H = po.Hierarchy(c["order"], c.get("m", 1), c["k0"], c["levels"], box=tuple(d["box"]))
+    H = oracle_hier(c)
     co = po.coeffs(**c["co"])
     u0 = H.interp_pulse(c["levels"] - 1, co)
     nact = H.info(c["levels"] - 1)["nact"]
@@ -147,10 +147,12 @@ def test_interpolation_and_error_norms(name, golden):
     # (the third entry is the maximum over the last cell's quadrature points)
     c = CD_CASES[name]
     d = golden(name)
-    H = po.Hierarchy(c["order"], c.get("m", 1), c["k0"], c["levels"], box=case_box(c))
+    H = oracle_hier(c)
     lf = c["levels"] - 1
     ui = H.interpolate(lf, 1)
-    assert np.abs(ui - d["interp_sine"]).max() <= 1e-15
+    # on deformed meshes the refined vertices agree to an ulp or two (the
+    # reference sums a face's corners in the first neighbour's local order)
+    assert np.abs(ui - d["interp_sine"]).max() <= (4e-15 if "geo" in c else 1e-15)
     for u, ref in ((d["sol"], d["errors_sol"]), (d["interp_sine"], d["errors_interp"])):
         e = H.errors(lf, u, 1)
         assert np.abs(e[:3] - ref).max() <= 1e-12 * np.abs(ref).max()

@@ -273,3 +273,48 @@ def test_error_norms_closed_forms_and_rates():
     H = po.Hierarchy(1, 1, 1, 1, box=(1.0, 1.0, 1.0))
     u0 = np.zeros(H.info(0)["ndof"])
     assert np.all(H.errors(0, u0, 0) == 0)
+
+
+def _wavy_vertices(m):
+    from tests.cases import _box_geo_module
+    g = _box_geo_module()
+    return g.macro_vertices(g.geo_for_kind(f"m{m}w")[1])
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_mapped_hexahedra_patch_test_and_volume(order):
+    # On trilinear cells x + 2y + 3z lies in the Q1 and Q2 spaces: its
+    # interpolant has zero L2 and H1 error (a wrong cofactor, a transposed
+    # Jacobian or a map that disagrees with the vertices fails this), and the
+    # Galerkin Laplacian annihilates it on the active rows.  The |det J|
+    # weights of the deformed cells add up to the volume of the unit cube.
+    H = po.Hierarchy(order, 3, 0, 2, vertices=_wavy_vertices(3))
+    H.assemble(po.coeffs(eps=1.0, f_val=0.0))
+    for l in range(2):
+        u = H.interpolate(l, 3)
+        e = H.errors(l, u, 3)
+        assert e[0] < 1e-13 and e[1] < 1e-12
+        na = H.info(l)["nact"]
+        assert np.abs(H.matvec(l, u)[:na]).max() < 1e-12
+        one = np.ones(H.info(l)["ndof"])
+        assert abs(H.errors(l, one, 0)[0] - 1.0) < 1e-14  # sqrt(volume)
+    # the deformation is real: the vertices left the brick lattice
+    assert np.abs(H.cells(1) - po.Hierarchy(order, 3, 0, 2, box=po.UNIT_BOX).cells(1)).max() > 0.02
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_mapped_hierarchy_on_brick_vertices_is_the_box(order):
+    # the trilinear map of a brick is its affine map, and the general
+    # hexahedron h_conv (8 x 8 fit) is the brick's closed form
+    g = np.arange(4) / 3.0
+    V = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).transpose(2, 1, 0, 3).reshape(-1, 3)
+    A = po.Hierarchy(order, 3, 0, 2, box=po.UNIT_BOX)
+    B = po.Hierarchy(order, 3, 0, 2, vertices=V)
+    co = po.coeffs(eps=1e-3, b=(1.0, 0.5, 0.25), c=0.3, f_val=1.0, supg=1)
+    A.assemble(co)
+    B.assemble(co)
+    for l in range(2):
+        ea, eb = A.csr(l)[2], B.csr(l)[2]
+        assert np.abs(ea - eb).max() <= 1e-13 * np.abs(ea).max()
+        assert np.abs(A.cells(l) - B.cells(l)).max() <= 2.3e-16
+    assert np.abs(A.rhs(1) - B.rhs(1)).max() <= 1e-13 * np.abs(A.rhs(1)).max()
