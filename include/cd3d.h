@@ -108,6 +108,23 @@ void cd3d_default_solver(cd3d_solver_params *p);
  * TSquareMatrix3D (SquareMatrix3D.C), TMGLevel3D/TMultiGrid3D::AddLevel
  * (MultiGrid3D.C:61-68), TFgmresIte ctor (FgmresIte.C:44). */
 cd3d_status cd3d_create(const cd3d_mesh_desc *mesh, cd3d_problem **out);
+/* The same hierarchy over general hexahedra (TDomain::Init with a GEO file of
+ * explicit vertex coordinates, Domain.C:818 / ReadGeo.C:1790-1812; every Q1/Q2
+ * cell mapped by THexaTrilinear, HexaTrilinear.C): the m^3 macro cells have
+ * the vertices macro_xyz ((m+1)^3 x 3 doubles, x fastest) and the identity
+ * local frame; refinement puts new vertices at edge, face and cell means
+ * (Refinement.C).  mesh->box is not used.  Every macro cell must be oriented
+ * like the reference cube (positive Jacobian at its 8 corners), else
+ * CD3D_ERR_ARG.  Assembly runs cell by cell on every level (the class-stencil
+ * paths need bricks); both precision forms and TCD3D work as on the box. */
+cd3d_status cd3d_create_mapped(const cd3d_mesh_desc *mesh, const double *macro_xyz, cd3d_problem **out);
+/* TDomain::ReadGeo (ReadGeo.C:46-200) for structured macro meshes: reads a
+ * GEO file whose NEL = m^3 hexahedra and NVT = (m+1)^3 vertices are numbered
+ * along the lattice (x fastest, cell vertices in the reference's hexahedron
+ * order, every KNPR 0); writes m and, when xyz is non-NULL, the vertex
+ * coordinates (cap: doubles available, at least 3 (m+1)^3).  Other meshes:
+ * CD3D_ERR_ARG with the reason in msg (msg_len bytes, may be NULL). */
+cd3d_status cd3d_read_geo(const char *path, int *m, double *xyz, size_t cap, char *msg, size_t msg_len);
 cd3d_status cd3d_destroy(cd3d_problem *p);
 const char *cd3d_last_error(const cd3d_problem *p);
 /* bytes of device memory held by the handle */

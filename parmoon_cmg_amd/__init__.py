@@ -21,7 +21,7 @@ STATUS_NAMES = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_NOMEM", 4: "NOT_CON
 FP64, MIXED = 0, 1
 
 # every entry point declared in include/cd3d.h
-EXPORTS = ["cd3d_default_solver", "cd3d_create", "cd3d_destroy", "cd3d_last_error", "cd3d_device_bytes",
+EXPORTS = ["cd3d_default_solver", "cd3d_create", "cd3d_create_mapped", "cd3d_read_geo", "cd3d_destroy", "cd3d_last_error", "cd3d_device_bytes",
            "cd3d_level_info", "cd3d_get_dofmap", "cd3d_get_cells", "cd3d_get_csr", "cd3d_get_rhs",
            "cd3d_set_entries", "cd3d_quad_points", "cd3d_set_coeffs_qp", "cd3d_assemble", "cd3d_matvec", "cd3d_matvec_dev", "cd3d_matvec_noalloc",
            "cd3d_prolongate",
@@ -108,6 +108,8 @@ def lib():
     L.cd3d_default_solver.argtypes = [C.POINTER(SolverParams)]
     L.cd3d_default_solver.restype = None
     L.cd3d_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(P)]
+    L.cd3d_create_mapped.argtypes = [C.POINTER(MeshDesc), dp, C.POINTER(P)]
+    L.cd3d_read_geo.argtypes = [C.c_char_p, ip, dp, C.c_size_t, C.c_char_p, C.c_size_t]
     L.cd3d_destroy.argtypes = [P]
     L.cd3d_last_error.argtypes = [P]
     L.cd3d_last_error.restype = C.c_char_p
@@ -182,22 +184,52 @@ def solver_params(**kw):
     return sp
 
 
+def read_geo(path):
+    """(m, vertices) of a GEO file of a structured m^3 macro mesh (cd3d_read_geo)."""
+    L = lib()
+    m = C.c_int()
+    msg = C.create_string_buffer(512)
+    st = L.cd3d_read_geo(str(path).encode(), C.byref(m), None, 0, msg, 512)
+    if st != CD3D_OK:
+        raise CD3DError(st, msg.value.decode())
+    xyz = np.zeros(((m.value + 1) ** 3, 3))
+    st = L.cd3d_read_geo(str(path).encode(), C.byref(m), _dp(xyz), xyz.size, msg, 512)
+    if st != CD3D_OK:
+        raise CD3DError(st, msg.value.decode())
+    return m.value, xyz
+
+
 class Problem:
     """One multilevel CD3D problem resident on the GPU (a cd3d_problem handle).
 
     Mirrors the reference's TSystemCD3D flow: construct (spaces, structures,
     multigrid levels), assemble(coeffs), solve(params)."""
 
-    def __init__(self, order, m=1, k0=1, levels=2, box=(1.0, 1.0, 1.0)):
+    def __init__(self, order, m=1, k0=1, levels=2, box=(1.0, 1.0, 1.0), vertices=None):
+        """vertices: ((m+1)^3, 3) macro vertex coordinates, x fastest (general
+        hexahedra, cd3d_create_mapped); None: the box."""
         self.L = lib()
         self.order, self.levels = order, levels
         self.nb = (order + 1) ** 3
         md = MeshDesc(order, m, k0, levels, (C.c_double * 3)(*box))
         h = C.c_void_p()
-        st = self.L.cd3d_create(C.byref(md), C.byref(h))
+        if vertices is not None:
+            v = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1)
+            if v.size != 3 * (m + 1) ** 3:
+                raise ValueError(f"vertices: {3 * (m + 1) ** 3} coordinates expected, got {v.size}")
+            st = self.L.cd3d_create_mapped(C.byref(md), _dp(v), C.byref(h))
+        else:
+            st = self.L.cd3d_create(C.byref(md), C.byref(h))
         if st != CD3D_OK:
             raise CD3DError(st, "cd3d_create failed")
         self.h = h
+
+    @classmethod
+    def from_geo(cls, path, order, k0=0, levels=2):
+        """The hierarchy of a GEO file (structured m^3 macro mesh, explicit
+        vertex coordinates; TDomain::ReadGeo)."""
+        m, xyz = read_geo(path)
+        return cls(order, m, k0, levels, vertices=xyz)
 
     def close(self):
         if getattr(self, "h", None):

@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcd3d_hip.so")
-SOURCES = ["setup.cpp", "kernels.hip", "lattice.hip", "diag.hip", "capi.cpp", "server.cpp", "noalloc.hip"]
+SOURCES = ["setup.cpp", "kernels.hip", "lattice.hip", "diag.hip", "capi.cpp", "server.cpp", "noalloc.hip", "geo.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
          # packed FP32 instructions stay off (v_pk_* FP32 forms are avoided on this platform)

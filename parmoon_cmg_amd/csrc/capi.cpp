@@ -99,11 +99,41 @@ cd3d_status run(cd3d_problem *p, const std::function<void()> &f);
 
 // ============================================================== setup
 
-static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
+// the macro vertices of a mapped hierarchy: finite, and every macro cell's
+// trilinear map with a positive Jacobian determinant at its 8 corners (the
+// hexahedra are oriented like the reference cube)
+static void check_macro_vertices(int m, const double *xyz) {
+  const int64_t nv = m + 1;
+  for (int64_t k = 0; k < 3 * nv * nv * nv; k++)
+    if (!std::isfinite(xyz[k])) throw ArgError("non-finite macro vertex coordinate");
+  auto X = [&](int64_t i, int64_t j, int64_t k, int d) { return xyz[3 * (i + nv * (j + nv * k)) + d]; };
+  for (int k = 0; k < m; k++)
+    for (int j = 0; j < m; j++)
+      for (int i = 0; i < m; i++)
+        for (int t = 0; t < 8; t++) {
+          const int a = t & 1, b = (t >> 1) & 1, c = (t >> 2) & 1;
+          double J[3][3];
+          for (int d = 0; d < 3; d++) {
+            // one-sided differences along the three edges leaving corner t
+            J[d][0] = (X(i + 1, j + b, k + c, d) - X(i, j + b, k + c, d));
+            J[d][1] = (X(i + a, j + 1, k + c, d) - X(i + a, j, k + c, d));
+            J[d][2] = (X(i + a, j + b, k + 1, d) - X(i + a, j + b, k, d));
+          }
+          const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                             J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                             J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+          if (!(det > 0))
+            throw ArgError("macro cell (" + std::to_string(i) + "," + std::to_string(j) + "," + std::to_string(k) +
+                           ") is degenerate or inverted");
+        }
+}
+
+static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m, const double *macro_xyz = nullptr) {
   if (m.order != 1 && m.order != 2) throw ArgError("order must be 1 or 2");
   if (m.m < 1 || m.k0 < 0 || m.levels < 1 || m.levels > 12) throw ArgError("invalid mesh hierarchy");
   for (int d = 0; d < 3; d++)
-    if (!(m.box[d] > 0)) throw ArgError("box sides must be positive");
+    if (!(m.box[d] > 0) && !macro_xyz) throw ArgError("box sides must be positive");
+  if (macro_xyz) check_macro_vertices(m.m, macro_xyz);
   int64_t nfine = (int64_t)m.m << (m.k0 + m.levels - 1);
   if (nfine * m.order + 1 > 1290) throw ArgError("mesh too large (nodes per side > 1290)");
   {
@@ -114,6 +144,9 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
       throw ArgError("mesh too large (finest level exceeds 2^31 matrix entries)");
   }
   P.mesh = m;
+  P.mapped = macro_xyz != nullptr;
+  if (P.mapped)
+    for (int d = 0; d < 3; d++) P.mesh.box[d] = 1.0;  // unused: the geometry is the vertex lattice
   P.nb = (m.order + 1) * (m.order + 1) * (m.order + 1);
   HIPCHK(hipGetDevice(&P.device));
   HIPCHK(hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking));
@@ -121,6 +154,21 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
   P.hl.resize(L);
   P.dl.resize(L);
   for (int l = 0; l < L; l++) build_host_level(P.hl[l], m.order, m.m, m.k0 + l);
+  if (P.mapped) {
+    std::vector<double> vx(macro_xyz, macro_xyz + 3 * (size_t)(m.m + 1) * (m.m + 1) * (m.m + 1)), t;
+    int nv = m.m;
+    for (int r = 0; r < m.k0; r++, nv *= 2) {
+      refine_vertices(vx, nv, t);
+      vx.swap(t);
+    }
+    for (int l = 0; l < L; l++) {
+      if (l > 0) {
+        refine_vertices(P.hl[l - 1].vx, P.hl[l - 1].n, t);
+        vx.swap(t);
+      }
+      P.hl[l].vx = vx;
+    }
+  }
   for (int l = 1; l < L; l++) build_transfer(P.hl[l], P.hl[l - 1], m.order);
   std::vector<double> tab;
   basis_tables(m.order, tab);
@@ -134,8 +182,13 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m) {
     D.nnz = H.nnz;
     D.ncell = H.ncell;
     for (int d = 0; d < 3; d++) {
-      D.h[d] = m.box[d] / H.n;
-      D.dn[d] = m.box[d] / ((double)m.order * H.n);
+      D.h[d] = P.mesh.box[d] / H.n;
+      D.dn[d] = P.mesh.box[d] / ((double)m.order * H.n);
+    }
+    if (P.mapped) {
+      D.nv = H.n + 1;
+      D.vx = dalloc<double>(P, H.vx.size());
+      upload(P, D.vx, H.vx);
     }
     D.rowptr = dalloc<int>(P, H.ndof + 1);
     D.col = dalloc<int>(P, H.nnz);
@@ -252,9 +305,11 @@ static void ensure_krylov(cd3d_problem &P, int restart) {
 
 // CD3D_CELLWISE_ASSEMBLY=1 assembles constant coefficients cell by cell too
 // (the general kernel), for comparisons; read at every assembly
-static bool uniform_assembly_off() {
+// the class-stencil assembly paths need the uniform lattice of bricks: off on
+// mapped hierarchies and under CD3D_CELLWISE_ASSEMBLY=1
+static bool uniform_assembly_off(const cd3d_problem &P) {
   const char *e = getenv("CD3D_CELLWISE_ASSEMBLY");
-  return e && atoi(e) != 0;
+  return P.mapped || (e && atoi(e) != 0);
 }
 
 static double sdfem_delta_host(double hconv, double eps, const double *b, int order) {
@@ -308,6 +363,9 @@ static AsmParams asm_params(const cd3d_problem &P, const DevLevel &D, const cd3d
   if (co.supg && co.conv_kind == 0)
     ap.delta_const = sdfem_delta_host(hconv_host(D.h, co.b), co.eps, co.b, P.mesh.order);
   ap.qp = D.qp;
+  ap.vx = P.mapped ? D.vx : nullptr;
+  ap.nv = D.nv;
+  ap.hconv = D.hconv;
   return ap;
 }
 
@@ -340,7 +398,7 @@ static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int wha
   // constant coefficients (and the plain mass matrix under any convection):
   // one element matrix per level through the class stencils
   const bool uniform = (co.conv_kind == 0 || (what == 1 && !co.supg)) && !D.qp;
-  if (uniform && co.f_kind == 0 && D.nclass > 0 && !uniform_assembly_off()) {
+  if (uniform && co.f_kind == 0 && D.nclass > 0 && !uniform_assembly_off(P)) {
     // every active row is written: no clearing pass
     kl_assemble_uniform(P.stream, D, P.mesh.order, P.tab, ap, val, rhs);
     k_dirichlet_rows(P.stream, D, val, rhs);
@@ -348,7 +406,7 @@ static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int wha
     return 1;
   }
   if (what == 0 && co.conv_kind == 1 && co.f_kind == 0 && !co.supg && !D.qp && D.nclass > 0 &&
-      !uniform_assembly_off()) {
+      !uniform_assembly_off(P)) {
     kl_assemble_affine(P.stream, D, P.mesh.order, P.tab, ap, val, rhs, csr);
     k_dirichlet_rows(P.stream, D, val, rhs);
     HIPCHK(hipGetLastError());
@@ -357,8 +415,14 @@ static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int wha
   ensure_slots(P, l);
   HIPCHK(hipMemsetAsync(val, 0, sizeof(double) * D.nnz, P.stream));
   HIPCHK(hipMemsetAsync(rhs, 0, sizeof(double) * D.ndof, P.stream));
+  AsmParams apc = ap;
+  if (P.mapped && co.supg) {  // per-cell h_conv of the general hexahedra
+    if (!D.hconv) D.hconv = dalloc<double>(P, (size_t)D.ncell);
+    apc.hconv = D.hconv;
+    k_hconv_geo(P.stream, D, P.tab, apc, D.hconv);
+  }
   for (int k = 0; k < 8; k++)
-    k_assemble(P.stream, D, P.tab, ap, val, rhs, D.colour_start[k], D.colour_start[k + 1] - D.colour_start[k]);
+    k_assemble(P.stream, D, P.tab, apc, val, rhs, D.colour_start[k], D.colour_start[k + 1] - D.colour_start[k]);
   k_dirichlet_rows(P.stream, D, val, rhs);
   HIPCHK(hipGetLastError());
   return 0;
@@ -952,6 +1016,21 @@ cd3d_status cd3d_create(const cd3d_mesh_desc *mesh, cd3d_problem **out) {
   return CD3D_OK;
 }
 
+cd3d_status cd3d_create_mapped(const cd3d_mesh_desc *mesh, const double *macro_xyz, cd3d_problem **out) {
+  if (!mesh || !macro_xyz || !out) return CD3D_ERR_ARG;
+  *out = nullptr;
+  cd3d_problem *p = new (std::nothrow) cd3d_problem();
+  if (!p) return CD3D_ERR_NOMEM;
+  cd3d_status st = run(p, [&]() { setup_problem(*p, *mesh, macro_xyz); });
+  if (st != CD3D_OK) {
+    fprintf(stderr, "cd3d_create_mapped: %s\n", p->err.c_str());
+    cd3d_destroy(p);
+    return st;
+  }
+  *out = p;
+  return CD3D_OK;
+}
+
 cd3d_status cd3d_destroy(cd3d_problem *p) {
   if (!p) return CD3D_ERR_ARG;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
@@ -986,11 +1065,43 @@ cd3d_status cd3d_get_dofmap(const cd3d_problem *p, int level, int *gn) {
   return CD3D_OK;
 }
 
+// ---------------------------------------------------------------- host geometry of mapped levels
+
+static const int kHexV[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+
+// vertex-lattice index of local vertex v of cell c
+static int64_t cell_vertex_index(const cd3d_problem &P, const HostLevel &H, int c, int v) {
+  const int8_t *E = &H.ax[9 * c];
+  int64_t I[3];
+  for (int d = 0; d < 3; d++)
+    I[d] = H.org[3 * c + d] / P.mesh.order + kHexV[v][0] * E[d] + kHexV[v][1] * E[3 + d] + kHexV[v][2] * E[6 + d];
+  const int64_t nv = H.n + 1;
+  return I[0] + nv * (I[1] + nv * I[2]);
+}
+
+// the trilinear map of cell c of a mapped level at the reference point r
+// (THexaTrilinear::GetOrigFromRef, HexaTrilinear.C)
+static void cell_map_point(const cd3d_problem &P, const HostLevel &H, int c, const double *r, double *X) {
+  for (int d = 0; d < 3; d++) X[d] = 0.0;
+  for (int v = 0; v < 8; v++) {
+    double w = 1.0;
+    for (int a = 0; a < 3; a++) w *= kHexV[v][a] ? 0.5 * (1 + r[a]) : 0.5 * (1 - r[a]);
+    const double *x = &H.vx[3 * cell_vertex_index(P, H, c, v)];
+    for (int d = 0; d < 3; d++) X[d] += w * x[d];
+  }
+}
+
 cd3d_status cd3d_get_cells(const cd3d_problem *p, int level, double *xyz) {
   if (!p || !xyz || level < 0 || level >= (int)p->hl.size()) return CD3D_ERR_ARG;
   static const int hexv[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
   const HostLevel &H = p->hl[level];
   const int S = p->mesh.order;
+  if (p->mapped) {
+    for (int c = 0; c < H.ncell; c++)
+      for (int v = 0; v < 8; v++)
+        for (int d = 0; d < 3; d++) xyz[24 * (size_t)c + 3 * v + d] = H.vx[3 * cell_vertex_index(*p, H, c, v) + d];
+    return CD3D_OK;
+  }
   for (int c = 0; c < H.ncell; c++)
     for (int v = 0; v < 8; v++)
       for (int d = 0; d < 3; d++) {
@@ -1228,6 +1339,19 @@ cd3d_status cd3d_interp_pulse(cd3d_problem *p, int level, const cd3d_coeffs *co,
   if (!p || !co || !u || level < 0 || level >= (int)p->hl.size()) return CD3D_ERR_ARG;
   const HostLevel &H = p->hl[level];
   const double *pc = co->pulse_c;
+  if (p->mapped) {  // cell by cell, the nodes' reference points mapped (the last cell wins)
+    const int q = p->mesh.order + 1, nb = p->nb;
+    for (int c = 0; c < H.ncell; c++)
+      for (int b = 0; b < nb; b++) {
+        const double r[3] = {-1.0 + 2.0 * (b % q) / p->mesh.order, -1.0 + 2.0 * ((b / q) % q) / p->mesh.order,
+                             -1.0 + 2.0 * (b / (q * q)) / p->mesh.order};
+        double X[3];
+        cell_map_point(*p, H, c, r, X);
+        const double r2 = (X[0] - pc[0]) * (X[0] - pc[0]) + (X[1] - pc[1]) * (X[1] - pc[1]) + (X[2] - pc[2]) * (X[2] - pc[2]);
+        u[H.gn[(size_t)c * nb + b]] = exp(-r2 / (2 * 0.05 * 0.05));
+      }
+    return CD3D_OK;
+  }
   for (int d = 0; d < H.ndof; d++) {
     const int node = H.dof_node[d];
     const double den = (double)p->mesh.order * H.n;
@@ -1300,7 +1424,7 @@ static int tcd_step_impl(cd3d_problem &P, const cd3d_coeffs &co0, const cd3d_sol
   // FP32 sweeps and the FP64 lattice operator, with the arithmetic of the CSR
   // path, so levels above the coarsest get no CSR values.  The coarsest level
   // keeps its CSR system matrix for the exact coarse solve.
-  const bool affine_asm = co.conv_kind == 1 && co.f_kind == 0 && !co.supg && !uniform_assembly_off();
+  const bool affine_asm = co.conv_kind == 1 && co.f_kind == 0 && !co.supg && !uniform_assembly_off(P);
   bool lattice = sp.precision == CD3D_MIXED && Lf > 0 && affine_asm && sp.smoother == 1;
   for (int l = 1; l <= Lf; l++) lattice = lattice && P.dl[l].tcd_mass_uniform && P.dl[l].nclass > 0;
   // AssembleARhs at t_new (SystemTCD3D.C:204-226); with SUPG the A and rhs of
@@ -1495,6 +1619,12 @@ cd3d_status cd3d_quad_points(const cd3d_problem *p, int level, double *xyz) {
   const int stride = 4 + 4 * p->nb;
   double dn[3];
   for (int d = 0; d < 3; d++) dn[d] = p->mesh.box[d] / ((double)P * H.n);
+  if (p->mapped) {
+    for (int c = 0; c < H.ncell; c++)
+      for (int q = 0; q < nq; q++)
+        cell_map_point(*p, H, c, &tab[(size_t)q * stride + 1], xyz + ((size_t)c * nq + q) * 3);
+    return CD3D_OK;
+  }
   // the point of the element kernels: X[dax[a]] = dn (O + P sg (xi_a + 1) / 2)
   for (int c = 0; c < H.ncell; c++)
     for (int q = 0; q < nq; q++) {
@@ -1628,8 +1758,12 @@ cd3d_status cd3d_write_vtk(cd3d_problem *p, int level, const double *u_host, con
     const double hx = p->mesh.box[0] / n, hy = p->mesh.box[1] / n, hz = p->mesh.box[2] / n;
     fprintf(f, "# vtk DataFile Version 4.2\nfile created by ParMooN\nASCII\nDATASET UNSTRUCTURED_GRID\n\n");
     fprintf(f, "POINTS %zu double\n", npt);
-    for (size_t k = 0; k < npt; k++)
-      fprintf(f, "%.6e %.6e %.6e\n", (k % nv) * hx, ((k / nv) % nv) * hy, (k / ((size_t)nv * nv)) * hz);
+    for (size_t k = 0; k < npt; k++) {
+      if (p->mapped)
+        fprintf(f, "%.6e %.6e %.6e\n", H.vx[3 * k], H.vx[3 * k + 1], H.vx[3 * k + 2]);
+      else
+        fprintf(f, "%.6e %.6e %.6e\n", (k % nv) * hx, ((k / nv) % nv) * hy, (k / ((size_t)nv * nv)) * hz);
+    }
     fprintf(f, "\nCELLS %d %d\n", H.ncell, 9 * H.ncell);
     for (int c = 0; c < H.ncell; c++) {
       fprintf(f, "8 ");

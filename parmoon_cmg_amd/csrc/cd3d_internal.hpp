@@ -48,6 +48,7 @@ struct HostLevel {
   int ndof = 0, nact = 0;
   int64_t nnz = 0;
   std::vector<int> org;         // 3 per cell, node-lattice units
+  std::vector<double> vx;       // mapped hierarchy: (n+1)^3 x 3 vertex coordinates, x fastest
   std::vector<int8_t> ax;       // 9 per cell: ax[9c + 3a + d]
   std::vector<int> gn;          // nb per cell
   std::vector<int> dof_node;    // DOF -> lattice node
@@ -103,6 +104,11 @@ struct DevLevel {
   int *colour_cells = nullptr;
   std::vector<int> colour_start;
   int ncell = 0;
+  // mapped hierarchy: the vertex lattice ((nv)^3 x 3, nv = n + 1) and the
+  // per-cell SUPG mesh size in convection direction (allocated on first use)
+  double *vx = nullptr;
+  int nv = 0;
+  double *hconv = nullptr;
   double h[3] = {0, 0, 0};   // cell sides
   double dn[3] = {0, 0, 0};  // node spacing
   // lattice-stencil form (mixed-precision V-cycle)
@@ -147,6 +153,9 @@ struct DevLevel {
 
 struct Problem {
   cd3d_mesh_desc mesh{};
+  // general hexahedra: the macro vertices were given (cd3d_create_mapped);
+  // every cell is mapped trilinearly and assembled cell by cell
+  bool mapped = false;
   int nb = 0;
   std::vector<HostLevel> hl;
   std::vector<DevLevel> dl;
@@ -195,6 +204,8 @@ struct Problem {
 
 // ---------------------------------------------------------------- setup (setup.cpp)
 void build_host_level(HostLevel &L, int order, int m, int depth);
+// one regular refinement of an n^3 vertex lattice (edge, face and cell means)
+void refine_vertices(const std::vector<double> &src, int n, std::vector<double> &dst);
 void build_transfer(HostLevel &fine, const HostLevel &coarse, int order);
 void build_slots(const HostLevel &L, int order, std::vector<int> &slots);
 void build_colours(HostLevel &L, int order);
@@ -280,10 +291,16 @@ struct AsmParams {
   // per quadrature point coefficients (ncell x nq x 6: eps, b1, b2, b3, c, f)
   // replacing the descriptor's functions, or null
   const double *qp;
+  // mapped hierarchy: vertex lattice (nv^3 x 3) and per-cell h_conv, or null
+  const double *vx;
+  int nv;
+  const double *hconv;
 };
 void k_assemble(hipStream_t s, const DevLevel &L, const double *tab, const AsmParams &ap, double *val,
                 double *rhs, int colour_begin, int colour_count);
 void k_dirichlet_rows(hipStream_t s, const DevLevel &L, double *val, double *rhs);
+// mapped levels: per-cell SUPG mesh size in convection direction into hconv (ncell)
+void k_hconv_geo(hipStream_t s, const DevLevel &L, const double *tab, const AsmParams &ap, double *hconv);
 // ---------------------------------------------------------------- diagnostics (diag.hip)
 // known function of the interpolation and error norms: fn 0 zero, 1 the sine
 // of Examples/CD_3D/Laplace.h, 2 the Gaussian pulse centred at pc
@@ -293,6 +310,8 @@ struct FnParams {
 };
 struct ErrGeom {
   double h[3], dn[3];
+  const double *vx = nullptr;  // mapped level: vertex lattice (trilinear cells)
+  int nv = 0;
 };
 void k_interpolate(hipStream_t s, const DevLevel &L, const double *box, int order, int n, const FnParams &f,
                    double *u);

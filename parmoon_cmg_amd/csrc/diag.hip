@@ -7,6 +7,7 @@
 #include <cmath>
 
 #include "cd3d_internal.hpp"
+#include "trilinear.hpp"
 
 namespace cd3d {
 
@@ -39,14 +40,37 @@ __device__ inline void known_fn(const FnParams &f, double x, double y, double z,
   }
 }
 
-// one thread per DOF: the Lagrange nodal functional is the value at the node
+// one thread per DOF: the Lagrange nodal functional is the value at the
+// node.  On a mapped level (vx non-null) the node is the trilinear map of a
+// lattice cell containing it at the node's reference point (cells sharing
+// the node agree on it: their maps coincide on shared faces).
 __global__ __launch_bounds__(kDiagBlock) void interp_k(int ndof, const int *__restrict__ dof_node, int nn, double dx,
-                                                      double dy, double dz, FnParams f, double *__restrict__ u) {
+                                                      double dy, double dz, const double *__restrict__ vx, int P,
+                                                      FnParams f, double *__restrict__ u) {
   const int d = blockIdx.x * kDiagBlock + threadIdx.x;
   if (d >= ndof) return;
   const int node = dof_node[d];
-  double v[4];
-  known_fn(f, (node % nn) * dx, ((node / nn) % nn) * dy, (node / (nn * nn)) * dz, v);
+  const int I[3] = {node % nn, (node / nn) % nn, node / (nn * nn)};
+  double X[3], v[4];
+  if (vx) {
+    const int n = (nn - 1) / P;
+    int cc[3];
+    double r[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      cc[a] = min(I[a] / P, n - 1);
+      r[a] = 2.0 * (I[a] - P * cc[a]) / P - 1.0;
+    }
+    TriMap M;
+    // the identity frame: local axis a along lattice axis a
+    tri_load(vx, n + 1, make_int4(P * cc[0], P * cc[1], P * cc[2], 8 | (2 << 6)), P, M);
+    tri_point(M, r[0], r[1], r[2], X);
+  } else {
+    X[0] = I[0] * dx;
+    X[1] = I[1] * dy;
+    X[2] = I[2] * dz;
+  }
+  known_fn(f, X[0], X[1], X[2], v);
   u[d] = v[0];
 }
 
@@ -66,6 +90,8 @@ __global__ __launch_bounds__(kDiagBlock) void errors_k(int ncell, const int4 *__
   const double detj = (g.h[0] / 2) * (g.h[1] / 2) * (g.h[2] / 2);
   for (int c = blockIdx.x * kDiagBlock + threadIdx.x; c < ncell; c += gridDim.x * kDiagBlock) {
     const int4 cg = cells[c];
+    TriMap M;
+    if (g.vx) tri_load(g.vx, g.nv, cg, P, M);
     const int O[3] = {cg.x, cg.y, cg.z};
     int dax[3], sg[3];
 #pragma unroll
@@ -79,7 +105,7 @@ __global__ __launch_bounds__(kDiagBlock) void errors_k(int ncell, const int4 *__
     double cmax = 0.0;
     for (int q = 0; q < NQ; q++) {
       const double *T = tab + q * STRIDE;
-      const double w = T[0] * detj;
+      double w = T[0] * detj;
       double uh = 0.0, gr[3] = {0.0, 0.0, 0.0};
 #pragma unroll
       for (int b = 0; b < NB; b++) {
@@ -89,17 +115,26 @@ __global__ __launch_bounds__(kDiagBlock) void errors_k(int ncell, const int4 *__
         gr[2] = fma(uc[b], T[7 + 4 * b], gr[2]);
       }
       double X[3], G[3];
+      if (g.vx) {  // trilinear cell: grad = J^-T grad_ref = C grad_ref / det
+        double C[3][3];
+        const double det = tri_jac(M, T[1], T[2], T[3], C);
+        w = T[0] * fabs(det);
+        tri_point(M, T[1], T[2], T[3], X);
 #pragma unroll
-      for (int a = 0; a < 3; a++) {
-        const int d = dax[a];
-        const double xa = g.dn[d] * (O[d] + P * sg[a] * (T[1 + a] + 1) * 0.5);
-        const double ga = sg[a] * (2.0 / g.h[d]) * gr[a];
-        X[0] = d == 0 ? xa : X[0];
-        X[1] = d == 1 ? xa : X[1];
-        X[2] = d == 2 ? xa : X[2];
-        G[0] = d == 0 ? ga : G[0];
-        G[1] = d == 1 ? ga : G[1];
-        G[2] = d == 2 ? ga : G[2];
+        for (int d = 0; d < 3; d++) G[d] = (C[d][0] * gr[0] + C[d][1] * gr[1] + C[d][2] * gr[2]) / det;
+      } else {
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          const int d = dax[a];
+          const double xa = g.dn[d] * (O[d] + P * sg[a] * (T[1 + a] + 1) * 0.5);
+          const double ga = sg[a] * (2.0 / g.h[d]) * gr[a];
+          X[0] = d == 0 ? xa : X[0];
+          X[1] = d == 1 ? xa : X[1];
+          X[2] = d == 2 ? xa : X[2];
+          G[0] = d == 0 ? ga : G[0];
+          G[1] = d == 1 ? ga : G[1];
+          G[2] = d == 2 ? ga : G[2];
+        }
       }
       double v[4];
       known_fn(f, X[0], X[1], X[2], v);
@@ -148,8 +183,8 @@ void k_interpolate(hipStream_t s, const DevLevel &L, const double *box, int orde
                    double *u) {
   if (L.ndof <= 0) return;
   const double den = (double)order * n;
-  interp_k<<<(L.ndof + kDiagBlock - 1) / kDiagBlock, kDiagBlock, 0, s>>>(L.ndof, L.ldof2node, L.nn, box[0] / den,
-                                                                        box[1] / den, box[2] / den, f, u);
+  interp_k<<<(L.ndof + kDiagBlock - 1) / kDiagBlock, kDiagBlock, 0, s>>>(
+      L.ndof, L.ldof2node, L.nn, box[0] / den, box[1] / den, box[2] / den, L.vx, order, f, u);
 }
 
 int k_errors_blocks(int ncell) {
@@ -164,6 +199,8 @@ void k_errors(hipStream_t s, const DevLevel &L, int order, const double *tab, co
     g.h[d] = L.h[d];
     g.dn[d] = L.dn[d];
   }
+  g.vx = L.vx;
+  g.nv = L.nv;
   const int nblk = k_errors_blocks(L.ncell);
   double *last = partials + 3 * nblk;
   if (order == 1) errors_k<1><<<nblk, kDiagBlock, 0, s>>>(L.ncell, L.cells, L.gn, tab, g, u, f, partials, last);

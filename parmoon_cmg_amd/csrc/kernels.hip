@@ -14,6 +14,7 @@
 #include <cstdlib>
 
 #include "cd3d_internal.hpp"
+#include "trilinear.hpp"
 
 namespace cd3d {
 
@@ -862,6 +863,145 @@ __global__ __launch_bounds__(kBlock) void assemble_k(int ncells, const int *__re
   rhs[r] += racc;
 }
 
+// The element kernel on general hexahedra (mapped hierarchies): the same
+// thread layout and j-loop as assemble_k, with the geometry of THexaTrilinear
+// at every quadrature point.  With K = J^-T = C / det (C the cofactors) and
+// the reference gradients g_i, the physical gradient is K g_i, and
+//   eps (K g_i) . (K g_j) + t_i b . K g_j = (eps K^T K g_i + t_i K^T b) . g_j,
+// so a_i = w |det| (eps K^T (K g_i) + t_i beta), beta = K^T b,
+// t_i = phi_i + delta beta . g_i keep the brick kernel's form
+// A_ij += a_i . g_j + c_i phi_j.  The SUPG delta uses the cell's mesh size in
+// convection direction from hconv[c] (hconv_geo_k, run before).
+template <int P, bool VAR>
+__global__ __launch_bounds__(kBlock) void assemble_geo_k(int ncells, const int *__restrict__ list,
+                                                         const int4 *__restrict__ cells, const int *__restrict__ slots,
+                                                         const int *__restrict__ gn, int nact,
+                                                         const double *__restrict__ tab, AsmParams ap,
+                                                         double *__restrict__ val, double *__restrict__ rhs) {
+  constexpr int NB = (P + 1) * (P + 1) * (P + 1);
+  constexpr int N1 = P + 2;
+  constexpr int NQ = N1 * N1 * N1;
+  constexpr int STRIDE = 4 + 4 * NB;
+  const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (gid >= (int64_t)ncells * NB) return;
+  const int ci = (int)(gid / NB), i = (int)(gid % NB);
+  const int c = list[ci];
+  TriMap M;
+  tri_load(ap.vx, ap.nv, cells[c], P, M);
+  const double nu = 1.0 / P;
+  const double hc = ap.supg ? ap.hconv[c] : 0.0;
+  // constant coefficients: one delta per cell
+  double delta_c = 0.0;
+  if (!VAR && ap.supg)
+    delta_c = sdfem_delta(hc, ap.eps, sqrt(ap.b[0] * ap.b[0] + ap.b[1] * ap.b[1] + ap.b[2] * ap.b[2]), nu);
+  double acc[NB];
+#pragma unroll
+  for (int j = 0; j < NB; j++) acc[j] = 0;
+  double racc = 0;
+  for (int q = 0; q < NQ; q++) {
+    const double *T = tab + q * STRIDE;
+    const double *Ti = T + 4 + 4 * i;
+    const double phi_i = Ti[0], g0 = Ti[1], g1 = Ti[2], g2 = Ti[3];
+    double eps = ap.eps, b0 = ap.b[0], b1 = ap.b[1], b2 = ap.b[2], cr = ap.c, f = ap.f_val;
+    double delta = delta_c;
+    if (VAR) {
+      double X[3] = {0, 0, 0};
+      if (ap.conv_kind == 1 || ap.f_kind == 1) tri_point(M, T[1], T[2], T[3], X);
+      if (ap.qp) {
+        const double *cq = ap.qp + ((int64_t)c * NQ + q) * 6;
+        eps = cq[0];
+        b0 = cq[1];
+        b1 = cq[2];
+        b2 = cq[3];
+        cr = cq[4];
+        f = cq[5];
+      } else if (ap.conv_kind == 1) {
+        b0 = -(X[1] - ap.rot_c[1]);
+        b1 = X[0] - ap.rot_c[0];
+        b2 = 0.1 * ap.sin2pit;
+      }
+      if (!ap.qp && ap.f_kind == 1) {
+        const double sx = sin(M_PI * X[0]), sy = sin(M_PI * X[1]), sz = sin(M_PI * X[2]);
+        const double cx = cos(M_PI * X[0]), cy = cos(M_PI * X[1]), cz = cos(M_PI * X[2]), u = sx * sy * sz;
+        f = ap.eps * 3 * M_PI * M_PI * u + b0 * M_PI * cx * sy * sz + b1 * M_PI * sx * cy * sz +
+            b2 * M_PI * sx * sy * cz + ap.c * u;
+      }
+      if (ap.supg) delta = sdfem_delta(hc, eps, sqrt(b0 * b0 + b1 * b1 + b2 * b2), nu);
+    }
+    double K[3][3];
+    const double det = tri_jac(M, T[1], T[2], T[3], K);
+    const double rd = 1.0 / det;
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+#pragma unroll
+      for (int a = 0; a < 3; a++) K[d][a] *= rd;
+    const double w = T[0] * fabs(det);
+    double beta[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) beta[a] = b0 * K[0][a] + b1 * K[1][a] + b2 * K[2][a];
+    const double bg = beta[0] * g0 + beta[1] * g1 + beta[2] * g2;
+    const double ti = phi_i + delta * bg;
+    if (ap.what == 1) {  // mass; rhs with the SUPG test function
+      const double bw = w * phi_i;
+#pragma unroll
+      for (int j = 0; j < NB; j++) acc[j] = fma(bw, T[4 + 4 * j], acc[j]);
+      racc += w * f * ti;
+      continue;
+    }
+    if (ap.what == 2) {  // SUPG time-derivative matrix K
+      const double kw = w * (delta * bg);
+#pragma unroll
+      for (int j = 0; j < NB; j++) acc[j] = fma(kw, T[4 + 4 * j], acc[j]);
+      continue;
+    }
+    double G[3];  // physical gradient of phi_i
+#pragma unroll
+    for (int d = 0; d < 3; d++) G[d] = K[d][0] * g0 + K[d][1] * g1 + K[d][2] * g2;
+    double a[3];
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) a[ax] = w * (eps * (K[0][ax] * G[0] + K[1][ax] * G[1] + K[2][ax] * G[2]) + ti * beta[ax]);
+    const double cc = w * (ti * cr);
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+      acc[j] = fma(a[0], T[5 + 4 * j], fma(a[1], T[6 + 4 * j], fma(a[2], T[7 + 4 * j], fma(cc, T[4 + 4 * j], acc[j]))));
+    racc += w * f * ti;
+  }
+  const int r = gn[c * NB + i];
+  if (r >= nact) return;
+  const int *sl = slots + ((int64_t)c * NB + i) * NB;
+#pragma unroll
+  for (int j = 0; j < NB; j++) val[sl[j]] += acc[j];
+  rhs[r] += racc;
+}
+
+// per cell of a mapped level: the SUPG mesh size in convection direction,
+// with b at the cell's first quadrature point (the reference evaluates it
+// once per cell, at the first point, and caches it: AssembleMat3D.C:457)
+template <int P>
+__global__ __launch_bounds__(kBlock) void hconv_geo_k(int ncell, const int4 *__restrict__ cells,
+                                                      const double *__restrict__ tab, AsmParams ap,
+                                                      double *__restrict__ hconv) {
+  constexpr int NQ = (P + 2) * (P + 2) * (P + 2);
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= ncell) return;
+  TriMap M;
+  tri_load(ap.vx, ap.nv, cells[c], P, M);
+  double b0 = ap.b[0], b1 = ap.b[1], b2 = ap.b[2];
+  if (ap.qp) {
+    const double *cq = ap.qp + (int64_t)c * NQ * 6;
+    b0 = cq[1];
+    b1 = cq[2];
+    b2 = cq[3];
+  } else if (ap.conv_kind == 1) {
+    double X[3];
+    tri_point(M, tab[1], tab[2], tab[3], X);
+    b0 = -(X[1] - ap.rot_c[1]);
+    b1 = X[0] - ap.rot_c[0];
+    b2 = 0.1 * ap.sin2pit;
+  }
+  hconv[c] = tri_hconv(M, b0, b1, b2);
+}
+
 // Explicit inverse of the active block of the coarsest matrix by Gauss-Jordan
 // elimination with partial pivoting, spread over many workgroups: one launch
 // per pivot column.  Rows are never swapped or scaled during the sweep: the
@@ -1385,6 +1525,16 @@ void k_assemble(hipStream_t s, const DevLevel &L, const double *tab, const AsmPa
   const bool var = ap.conv_kind != 0 || ap.f_kind != 0 || ap.qp != nullptr;
   const int64_t threads = (int64_t)colour_count * (ap.order == 1 ? 8 : 27);
   const int grid = (int)((threads + kBlock - 1) / kBlock);
+  if (ap.vx) {  // general hexahedra
+    if (ap.order == 1) {
+      if (var) assemble_geo_k<1, true><<<grid, kBlock, 0, s>>>(colour_count, list, L.cells, L.slots, L.gn, L.nact, tab, ap, val, rhs);
+      else assemble_geo_k<1, false><<<grid, kBlock, 0, s>>>(colour_count, list, L.cells, L.slots, L.gn, L.nact, tab, ap, val, rhs);
+    } else {
+      if (var) assemble_geo_k<2, true><<<grid, kBlock, 0, s>>>(colour_count, list, L.cells, L.slots, L.gn, L.nact, tab, ap, val, rhs);
+      else assemble_geo_k<2, false><<<grid, kBlock, 0, s>>>(colour_count, list, L.cells, L.slots, L.gn, L.nact, tab, ap, val, rhs);
+    }
+    return;
+  }
 #define CD3D_ASM(P_, LDS_, VAR_) \
   assemble_k<P_, LDS_, VAR_><<<grid, kBlock, 0, s>>>(colour_count, list, L.cells, L.slots, L.gn, L.nact, tab, ap, val, rhs)
   if (ap.order == 1) {
@@ -1396,6 +1546,13 @@ void k_assemble(hipStream_t s, const DevLevel &L, const double *tab, const AsmPa
   }
 #undef CD3D_ASM
 }
+void k_hconv_geo(hipStream_t s, const DevLevel &L, const double *tab, const AsmParams &ap, double *hconv) {
+  if (L.ncell <= 0) return;
+  const int grid = (L.ncell + kBlock - 1) / kBlock;
+  if (ap.order == 1) hconv_geo_k<1><<<grid, kBlock, 0, s>>>(L.ncell, L.cells, tab, ap, hconv);
+  else hconv_geo_k<2><<<grid, kBlock, 0, s>>>(L.ncell, L.cells, tab, ap, hconv);
+}
+
 // blocked elimination (panels of NB pivots); returns the buffer holding the result
 template <int NB, int RPT>
 const double *gj_blocked(hipStream_t s, int na, double *M, double *M1, int *used, int *perm, double *dval,

@@ -82,6 +82,31 @@ inline int cell_node(const HostLevel &L, int c, int i, int j, int k) {
 
 int quad_points_1d(int order) { return order + 2; }  // Gauss3 (Q1), Gauss4 (Q2): FEDatabase3D.C:1009-1016
 
+// New vertices of the regular hexahedron refinement (Refinement.C with
+// RefHexaRegDesc.C): an edge's at 0.5 v0 + 0.5 v1, a face's at the mean of
+// its 4 vertices, a cell's at the mean of its 8; the sums run over the
+// corners in lattice order.  The vertex lattice of an n^3 structured mesh
+// becomes that of its 2n^3 refinement.
+void refine_vertices(const std::vector<double> &src, int n, std::vector<double> &dst) {
+  const int64_t ns = n + 1, nd = 2 * (int64_t)n + 1;
+  dst.assign(3 * nd * nd * nd, 0.0);
+  for (int64_t K = 0; K < nd; K++)
+    for (int64_t J = 0; J < nd; J++)
+      for (int64_t I = 0; I < nd; I++) {
+        const int odd[3] = {(int)(I & 1), (int)(J & 1), (int)(K & 1)};
+        const int64_t base[3] = {I >> 1, J >> 1, K >> 1};
+        const int nodd = odd[0] + odd[1] + odd[2];
+        const double wt = nodd == 0 ? 1.0 : nodd == 1 ? 0.5 : nodd == 2 ? 0.25 : 0.125;
+        double *o = &dst[3 * (I + nd * (J + nd * K))];
+        for (int t = 0; t < 8; t++) {
+          const int e[3] = {t & 1, (t >> 1) & 1, (t >> 2) & 1};
+          if ((e[0] && !odd[0]) || (e[1] && !odd[1]) || (e[2] && !odd[2])) continue;
+          const double *x = &src[3 * ((base[0] + e[0]) + ns * ((base[1] + e[1]) + ns * (base[2] + e[2])))];
+          for (int d = 0; d < 3; d++) o[d] += wt * x[d];
+        }
+      }
+}
+
 void build_host_level(HostLevel &L, int order, int m, int depth) {
   const int p = order, nb = (p + 1) * (p + 1) * (p + 1);
   L.n = m << depth;
