@@ -125,6 +125,39 @@ def run_steps(pm, P, sp, co, warmup, steps):
     return np.array(lat), its, res
 
 
+def wavy_vertices(m):
+    """Macro vertices of the unit cube split into m^3 hexahedra with every
+    interior vertex moved by up to 0.15 macro cells per axis (general,
+    non-parallelepiped hexahedra), x fastest."""
+    g = np.arange(m + 1, dtype=np.float64)
+    k, j, i = np.meshgrid(g, g, g, indexing="ij")
+    inner = (i > 0) & (i < m) & (j > 0) & (j < m) & (k > 0) & (k < m)
+    d = 0.15 * np.stack([np.sin(2.1 * i + 1.3 * j + 0.7 * k), np.sin(0.9 * i + 2.3 * j + 1.1 * k + 1.0),
+                         np.sin(1.7 * i + 0.5 * j + 2.9 * k + 2.0)], -1) * inner[..., None]
+    return ((np.stack([i, j, k], -1) + d) / m).reshape(-1, 3)
+
+
+def general_hexahedra(pm, steps, warmup):
+    """The headline problem on 64^3 general hexahedra (4^3 deformed macro
+    cells refined 4 times, 5 levels): the trilinear element kernel's FP64
+    rate on the finest level and the assemble + solve rate of both forms."""
+    P = pm.Problem(2, 4, 0, 5, vertices=wavy_vertices(4))
+    co = pm.coeffs(**COEFFS)
+    out = {}
+    for name, prec in (("fp64", pm.FP64), ("mixed", pm.MIXED)):
+        lat, its, rr = run_steps(pm, P, pm.solver_params(**SOLVER, precision=prec), co, warmup, steps)
+        out[name] = dict(solves_per_s=steps / lat.sum(), ms_per_step=1e3 * lat.mean(),
+                         iterations=int(np.median(its)), max_rel_residual=float(max(rr)))
+    ms, flops = P.kernel_bench(8, 4, 5)
+    out["assembly_trilinear"] = dict(
+        bound="fp64", achieved=round(flops / (ms * 1e-3) / 1e12, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
+        frac=round(flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4), us_per_launch=round(ms * 1e3, 2),
+        algorithmic_flops=int(flops),
+        kernel="assemble_geo_k<2,false> f64 (trilinear map at every quadrature point, 8 colour launches)")
+    P.close()
+    return out
+
+
 CPU_RECORD = os.path.join(REPO, "profiles", "cpu_reference_vs_port_c2_64.json")
 
 
@@ -428,6 +461,8 @@ def main():
                                     "smoother, defect and FGMRES product")
     extra = {}
     if not args.no_extra:
+        log("[bench] 64^3 Q2 SUPG on general hexahedra (deformed 4^3 macro mesh)")
+        extra["general_hexahedra_q2_64"] = general_hexahedra(pm, 3, 1)
         for name, prec in (("fp64", pm.FP64), ("mixed", pm.MIXED)):
             log(f"[bench] configuration 3 (TCD3D Q1 128^3, {args.c3_steps} CN steps), {name}")
             r, u = tcd_run(pm, prec, args.c3_steps)

@@ -7,8 +7,11 @@
 // TMultiGrid3D + TMGLevel3D + TMultiGridScaIte + TFgmresIte by hand as
 // SystemCD3D.C:138-203 does, runs on the GPU unchanged apart from two
 // points the device cannot take over:
-//   * the mesh hierarchy comes from a cd3d_mesh_desc (CreateHierarchy)
-//     instead of TDomain + GEO/PRM files; the FE spaces it hands out have the
+//   * the mesh hierarchy comes from TDomain::Init with a GEO file of a
+//     structured m^3 macro mesh with explicit vertex coordinates (general
+//     hexahedra; the PRM file is not read: every outer face is Dirichlet),
+//     or from a cd3d_mesh_desc (CreateHierarchy) for boxes, ParMooN's
+//     parametrised Wuerfel.GEO among them; the FE spaces have the
 //     reference's DOF numbering (FESpace3D.C), so GlobalNumbers, RowPtr and
 //     KCol are bit-identical;
 //   * the CoeffFct3D callback (Constants.h:123) is replaced by a
@@ -24,6 +27,7 @@
 #ifndef PARMOON_CD3D_HPP
 #define PARMOON_CD3D_HPP
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -146,6 +150,10 @@ class THierarchy {
   explicit THierarchy(const cd3d_mesh_desc &m) : mesh_(m) {
     Check(cd3d_create(&m, &h_), "cd3d_create", nullptr);
   }
+  // general hexahedra: the (m+1)^3 macro vertices of a GEO file
+  THierarchy(const cd3d_mesh_desc &m, const std::vector<double> &macro_xyz) : mesh_(m) {
+    Check(cd3d_create_mapped(&m, macro_xyz.data(), &h_), "cd3d_create_mapped", nullptr);
+  }
   ~THierarchy() {
     if (h_) cd3d_destroy(h_);
   }
@@ -163,16 +171,115 @@ class THierarchy {
   cd3d_problem *h_ = nullptr;
 };
 
+typedef void BoundCondFunct3D(int, double, double, double, int &);
+enum Iterators { It_EQ, It_LE, It_Finest, It_EQLevel, It_LELevel, It_Between, It_OCAF, It_Mortar1, It_Mortar2 };
+
+class TDomain;
+
+// TCollection (Collection.C): the finest cells of a domain after a number of
+// refinements (TDomain::GetCollection(It_Finest, 0))
+class TCollection {
+ public:
+  TCollection(TDomain *d, int refinements) : domain_(d), refinements_(refinements) {}
+  TDomain *GetDomain() const { return domain_; }
+  int Refinements() const { return refinements_; }
+  int GetN_Cells() const;
+
+ private:
+  TDomain *domain_;
+  int refinements_;
+};
+
+// TDomain (Domain.C) over a GEO file: Init reads the macro mesh
+// (TDomain::Init, Domain.C:818 / ReadGeo.C:46-200, through cd3d_read_geo),
+// RegRefineAll (Domain.C:1037) counts refinements, GetCollection hands out
+// the finest cells.  The device hierarchy of an element order spans the
+// refinements of the spaces built on the domain; it is (re)built when a space
+// on a finer collection appears, so a main program that creates its spaces
+// level by level, as CD3D_ParMooN.C:262-330 does, sees every space on one
+// hierarchy once the last one exists.
+class TDomain {
+ public:
+  explicit TDomain(const char * /*ParamFile*/ = nullptr) {}
+  TDomain(const TDomain &) = delete;
+  TDomain &operator=(const TDomain &) = delete;
+  void Init(const char * /*PRM*/, const char *GEO) {
+    char msg[512] = {0};
+    int m = 0;
+    cd3d_status st = cd3d_read_geo(GEO, &m, nullptr, 0, msg, sizeof msg);
+    if (st == CD3D_OK) {
+      xyz_.assign(3 * (size_t)(m + 1) * (m + 1) * (m + 1), 0.0);
+      st = cd3d_read_geo(GEO, &m, xyz_.data(), xyz_.size(), msg, sizeof msg);
+    }
+    if (st != CD3D_OK) Fail(std::string("TDomain::Init: ") + msg);
+    m_ = m;
+  }
+  int RegRefineAll() {
+    ++refinements_;
+    return 0;
+  }
+  void GenerateEdgeInfo() {}
+  TCollection *GetCollection(Iterators /*it*/, int /*level*/) {
+    colls_.emplace_back(new TCollection(this, refinements_));
+    return colls_.back().get();
+  }
+  int MacroCells() const { return m_; }
+  // the hierarchy of element order `order` covering refinements
+  // [coarsest registered, finest registered]
+  void Register(int order, int refinements) {
+    Span &s = spans_[order];
+    if (s.lo < 0 || refinements < s.lo || refinements > s.hi) {
+      s.lo = s.lo < 0 ? refinements : std::min(s.lo, refinements);
+      s.hi = std::max(s.hi, refinements);
+      s.hier.reset();
+    }
+  }
+  const std::shared_ptr<THierarchy> &Hierarchy(int order) {
+    Span &s = spans_[order];
+    if (!s.hier) {
+      if (m_ < 1) Fail("TDomain: Init with a GEO file first");
+      cd3d_mesh_desc md{};
+      md.order = order;
+      md.m = m_;
+      md.k0 = s.lo;
+      md.levels = s.hi - s.lo + 1;
+      md.box[0] = md.box[1] = md.box[2] = 1.0;
+      s.hier = std::make_shared<THierarchy>(md, xyz_);
+    }
+    return s.hier;
+  }
+  int Level(int order, int refinements) { return refinements - spans_[order].lo; }
+
+ private:
+  struct Span {
+    int lo = -1, hi = -1;
+    std::shared_ptr<THierarchy> hier;
+  };
+  int m_ = 0, refinements_ = 0;
+  std::vector<double> xyz_;
+  std::map<int, Span> spans_;
+  std::vector<std::unique_ptr<TCollection>> colls_;
+};
+
+inline int TCollection::GetN_Cells() const {
+  const long n = (long)domain_->MacroCells() << refinements_;
+  return (int)(n * n * n);
+}
+
 // TFESpace3D of one level (FESpace3D.C:55): conforming Q1/Q2, Dirichlet on
 // every face, DOFs [inner | Dirichlet].
 class TFESpace3D {
  public:
-  TFESpace3D(std::shared_ptr<THierarchy> hier, int level) : hier_(std::move(hier)), level_(level) {
-    int64_t nnz;
-    Check(cd3d_level_info(hier_->handle(), level_, &ndof_, &nact_, &nnz, &ncells_), "cd3d_level_info",
-          hier_->handle());
-    const int o = hier_->mesh().order;
-    nbf_ = (o + 1) * (o + 1) * (o + 1);
+  TFESpace3D(std::shared_ptr<THierarchy> hier, int level) : hier_(std::move(hier)), level_(level) { Query(); }
+  // the reference's constructor (FESpace3D.h:52-53, FESpace3D.C:55): space
+  // of order k on a collection of a TDomain; name, description and the
+  // boundary condition are not used (every outer face is Dirichlet)
+  TFESpace3D(TCollection *coll, const char * /*name*/, const char * /*description*/,
+             BoundCondFunct3D * /*BoundaryCondition*/, int k)
+      : dom_(coll->GetDomain()), order_(k), refinements_(coll->Refinements()) {
+    if (k != 1 && k != 2) Fail("TFESpace3D: order must be 1 or 2");
+    dom_->Register(k, refinements_);
+    Query();
   }
   int GetN_DegreesOfFreedom() const { return ndof_; }
   int GetActiveBound() const { return nact_; }
@@ -184,7 +291,7 @@ class TFESpace3D {
   int *GetGlobalNumbers() {
     if (gn_.empty()) {
       gn_.resize((size_t)ncells_ * nbf_);
-      Check(cd3d_get_dofmap(hier_->handle(), level_, gn_.data()), "cd3d_get_dofmap", hier_->handle());
+      Check(cd3d_get_dofmap(handle(), GetLevel(), gn_.data()), "cd3d_get_dofmap", handle());
     }
     return gn_.data();
   }
@@ -195,13 +302,21 @@ class TFESpace3D {
     }
     return bi_.data();
   }
-  int GetLevel() const { return level_; }
-  const std::shared_ptr<THierarchy> &Hierarchy() const { return hier_; }
-  cd3d_problem *handle() const { return hier_->handle(); }
+  int GetLevel() const { return dom_ ? dom_->Level(order_, refinements_) : level_; }
+  const std::shared_ptr<THierarchy> &Hierarchy() const { return dom_ ? dom_->Hierarchy(order_) : hier_; }
+  cd3d_problem *handle() const { return Hierarchy()->handle(); }
 
  private:
+  void Query() {
+    int64_t nnz;
+    Check(cd3d_level_info(handle(), GetLevel(), &ndof_, &nact_, &nnz, &ncells_), "cd3d_level_info", handle());
+    const int o = Hierarchy()->mesh().order;
+    nbf_ = (o + 1) * (o + 1) * (o + 1);
+  }
   std::shared_ptr<THierarchy> hier_;
-  int level_, ndof_ = 0, nact_ = 0, ncells_ = 0, nbf_ = 0;
+  TDomain *dom_ = nullptr;
+  int order_ = 0, refinements_ = 0;
+  int level_ = 0, ndof_ = 0, nact_ = 0, ncells_ = 0, nbf_ = 0;
   std::vector<int> gn_, bi_;
 };
 

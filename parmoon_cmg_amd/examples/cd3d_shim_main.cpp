@@ -6,6 +6,9 @@
 //                      coefficients varying in space); mg: TMultiGrid3D + TMGLevel3D +
 //                      TMultiGridScaIte + TFgmresIte wired by hand as in
 //                      SystemCD3D.C:138-203; tcd: TSystemTCD3D
+//     geo=<file.GEO>   mesh from a GEO file through TDomain::Init, k0 + levels - 1
+//                      RegRefineAll and TFESpace3D(coll, ...) per level, as
+//                      CD3D_ParMooN.C:137-330 (otherwise the box m, box=)
 //     order k0 levels box=x,y,z eps b=x,y,z c f_kind f_val supg conv_kind
 //     rot=x,y pulse=x,y,z  (problem)     tau nsteps time_disc (tcd)
 //     tcd_stop=step  each TCD step stops at rel times its own initial residual
@@ -48,6 +51,9 @@ void write_out(const std::string &prefix, const std::vector<double> &x, int its,
   fprintf(f, "its %d\nr0 %.17g\n", its, r0);
   fclose(f);
 }
+
+// BoundCondFunct3D of the examples: Dirichlet everywhere
+void BoundCondition(int, double, double, double, int &cond) { cond = 0; }
 
 // A coefficient function in the shape of an Examples/CD_3D/*.h BilinearCoeffs
 // (coefficients varying in space: eps (1 + xyz), b = (b1 sin(pi y),
@@ -101,8 +107,24 @@ int main(int argc, char **argv) {
   mesh.levels = (int)num("levels", 3);
   mesh.box[0] = mesh.box[1] = mesh.box[2] = 1.0;
   vec("box", mesh.box, 3);
-  std::vector<std::unique_ptr<TFESpace3D>> spaces = CreateHierarchy(mesh);
   const int L = mesh.levels;
+  std::unique_ptr<TDomain> Domain;
+  std::vector<std::unique_ptr<TFESpace3D>> spaces;
+  if (args.count("geo")) {
+    // CD3D_ParMooN.C:137-330: Init(BNDFILE, GEOFILE), UNIFORM_STEPS
+    // refinements, then one refinement and one space per level
+    Domain.reset(new TDomain());
+    Domain->Init(nullptr, args["geo"].c_str());
+    for (int i = 0; i < mesh.k0; i++) Domain->RegRefineAll();
+    for (int i = 0; i < L; i++) {
+      if (i) Domain->RegRefineAll();
+      TCollection *coll = Domain->GetCollection(It_Finest, 0);
+      spaces.emplace_back(new TFESpace3D(coll, "c", "c", BoundCondition, mesh.order));
+      printf("level %d: %d dofs\n", i, spaces.back()->GetN_DegreesOfFreedom());
+    }
+  } else {
+    spaces = CreateHierarchy(mesh);
+  }
   std::vector<TFESpace3D *> fes;
   std::vector<std::vector<double>> sol(L), rhs(L);
   std::vector<double *> solp, rhsp;

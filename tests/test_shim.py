@@ -116,3 +116,38 @@ def test_shim_tcd3d_matches_reference(name, golden, tmp_path):
     for k in range(c["nsteps"]):
         ref = d[f"u{k + 1}"]
         assert np.abs(states[k] - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["m3w_q1_poisson", "m3w_q2_supg"])
+def test_shim_geo_domain_matches_reference(name, golden, tmp_path):
+    # the reference main's own mesh set-up: TDomain::Init with the GEO file
+    # the reference run read, RegRefineAll, GetCollection(It_Finest, 0) and
+    # TFESpace3D(coll, name, description, BoundCondition, order) per level
+    from tests.cases import case_solver, geo_text
+    c = CD_CASES[name]
+    d = golden(name)
+    geo = tmp_path / "mesh.GEO"
+    geo.write_text(geo_text(c))
+    sv = case_solver(name)
+    args = [a for a in case_args(c) if not a.startswith(("box=", "m="))] + [f"geo={geo}"]
+    args += [f"omega={sv['omega']!r}", f"pre={sv['pre']}", f"post={sv['post']}"]
+    say(f"[gpu] shim geo {name}")
+    x, its, _ = run_shim(["mode=cd"] + args, tmp_path, "geo")
+    err = np.linalg.norm(x - d["sol"]) / np.linalg.norm(d["sol"])
+    say(f"[gpu] shim geo {name}: its {its} (reference {int(d['its'])}), err {err:.2e}")
+    assert its == int(d["its"]) and err < 1e-10
+
+
+def test_shim_geo_domain_refuses_parametrised_vertices(tmp_path):
+    # a GEO file whose vertices are boundary parameters (KNPR != 0, as in
+    # ParMooN's Wuerfel.GEO) needs the PRM file: refused with the reader's reason
+    from tests.cases import _box_geo_module
+    lines = _box_geo_module().box_geo(1).split("\n")
+    k = lines.index("KNPR")
+    lines[k + 1] = " " + " ".join(["  1"] * 8)
+    geo = tmp_path / "param.GEO"
+    geo.write_text("\n".join(lines))
+    p = subprocess.run([EXE, f"geo={geo}", f"out={tmp_path / 'x'}"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 4711 & 255
+    assert "TDomain::Init" in p.stderr and "KNPR" in p.stderr
