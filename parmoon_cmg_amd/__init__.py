@@ -471,13 +471,17 @@ class Server:
             raise CD3DError(st, "cd3d_server_submit")
         return rid.value
 
-    def wait(self, rid, timeout_ms=-1.0, n_dof=0):
-        """Returns (Result, solution or None)."""
+    def wait(self, rid, timeout_ms=-1.0, n_dof=0, check=True):
+        """Returns (Result, solution or None).  A request that ended in an
+        error raises CD3DError unless check is False (then its Result, with
+        the status, is returned); a wait that times out always raises
+        (status CD3D_ERR_STATE) and leaves the request outstanding."""
         res = Result()
         sol = np.zeros(n_dof) if n_dof else None
         st = self.L.cd3d_server_wait(self.h, rid, timeout_ms, C.byref(res), None if sol is None else _dp(sol),
                                      n_dof)
-        if st not in (CD3D_OK, 4):
+        timed_out = st == CD3D_ERR_STATE and res.id == 0  # no result was written (ids start at 1)
+        if timed_out or (check and st not in (CD3D_OK, 4)):
             raise CD3DError(st, f"request {rid}")
         return res, sol
 
