@@ -290,7 +290,7 @@ def failure_record(req, r):
                 rel_residual=float(r.rel_residual), steps_done=int(r.steps_done))
 
 
-def batched_uneven(pm, precision, count=256, seed=20261202, passes=2, inflight=16):
+def batched_uneven(pm, precision, count=256, seed=20261202, passes=2, inflight=16, max_batch=None):
     """Configuration 4: `count` independent Q1 SUPG problems, n log-uniform in
     [8, 48] (snapped to m 2^k), eps log-uniform in [1e-6, 1], b a random unit
     direction times U(0, 1), f = 1; all submitted at once to the server.  The
@@ -298,7 +298,7 @@ def batched_uneven(pm, precision, count=256, seed=20261202, passes=2, inflight=1
     solves/s counts converged requests only."""
     reqs = c4_requests(count, seed)
     sp = pm.solver_params(**dict(SOLVER, maxit=1000), precision=precision)
-    srv = pm.Server(max_inflight=inflight)
+    srv = pm.Server(max_inflight=inflight, max_batch=max_batch)
     out = None
     for _ in range(passes):
         t = time.perf_counter()
@@ -313,7 +313,8 @@ def batched_uneven(pm, precision, count=256, seed=20261202, passes=2, inflight=1
                    failures=[failure_record(q, r) for q, r in zip(reqs, res) if r.status != 0],
                    max_rel_residual=float(max(r.rel_residual for r in ok)),
                    median_iterations=int(np.median([r.iterations for r in ok])),
-                   reused=sum(r.reused_hierarchy for r in res))
+                   reused=sum(r.reused_hierarchy for r in res),
+                   mean_batch=float(np.mean([r.batch_size for r in res])))
     st = srv.stats()
     out["peak_device_gib"] = st.peak_bytes / 2**30
     srv.close()

@@ -189,6 +189,22 @@ cd3d_status cd3d_vcycle(cd3d_problem *p, const cd3d_solver_params *sp, const dou
  * use_guess != 0. */
 cd3d_status cd3d_solve(cd3d_problem *p, const cd3d_solver_params *sp, const double *rhs_host, double *sol_host,
                        int use_guess, cd3d_stats *st);
+/* Same-level batching (no reference counterpart: the reference solves one
+ * problem per process): `count` assembled problems with the same number of
+ * levels on one device -- of any sizes, element orders and coefficients --
+ * solved together, each as cd3d_solve(p, sp, NULL, NULL, 0, &stats[b])
+ * would: the FP64 V-cycle (damped Jacobi, exact coarse solve) inside
+ * FGMRES, in lockstep, one launch per step for all problems still iterating
+ * (kernels index each problem's arrays from device tables) and one host
+ * transfer of all H columns per iteration.  Each problem keeps its own
+ * stopping test; iteration counts equal a lone solve's, solutions agree to
+ * rounding (the reduction trees differ).  Other parameters (mixed precision,
+ * smoother 13, W/F cycles, coarse smoothers other than 17, no pre-smoothing,
+ * restart > 200) or count 1: the problems are solved one after another.
+ * Runs on the first handle's stream; every handle is locked meanwhile.
+ * stats (count entries, may be NULL) receives each problem's result; the
+ * return value is CD3D_OK or the first non-OK status. */
+cd3d_status cd3d_solve_batch(cd3d_problem **problems, int count, const cd3d_solver_params *sp, cd3d_stats *stats);
 cd3d_status cd3d_get_solution(cd3d_problem *p, double *sol_host);
 
 /* TCD3D Crank-Nicolson (3DPrograms/TCD3D_ParMooN.C:539-637 over

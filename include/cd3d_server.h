@@ -42,6 +42,9 @@ typedef struct {
   int robust_after;    /* a CD solve with the damped Jacobi smoother that has not converged after this many
                           FGMRES iterations continues from its iterate with the multicolour SSOR smoother
                           (13, omega 1, 2 + 2) on the rest of its iteration budget; 0: never */
+  int max_batch;       /* a worker takes up to this many admitted CD requests with the same number of levels
+                          and identical FP64 solver parameters (damped Jacobi, V-cycle, exact coarse solve)
+                          and solves them together (cd3d_solve_batch); 1: every request alone */
 } cd3d_server_params;
 
 typedef enum { CD3D_REQ_CD = 0, CD3D_REQ_TCD = 1 } cd3d_request_kind;
@@ -71,6 +74,7 @@ typedef struct {
   int steps_done;              /* TCD: time steps completed; a step that does not converge ends the run
                                   (status CD3D_NOT_CONVERGED) and the state after it is returned */
   int escalated;               /* 1: the solve was continued with the multicolour SSOR smoother (robust_after) */
+  int batch_size;              /* requests solved together with this one (max_batch), itself included */
 } cd3d_result;
 
 typedef struct {

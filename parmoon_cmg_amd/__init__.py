@@ -21,7 +21,8 @@ STATUS_NAMES = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_NOMEM", 4: "NOT_CON
 FP64, MIXED = 0, 1
 
 # every entry point declared in include/cd3d.h
-EXPORTS = ["cd3d_default_solver", "cd3d_create", "cd3d_create_mapped", "cd3d_read_geo", "cd3d_destroy", "cd3d_last_error", "cd3d_device_bytes",
+EXPORTS = ["cd3d_default_solver", "cd3d_create", "cd3d_create_mapped", "cd3d_read_geo", "cd3d_solve_batch",
+           "cd3d_destroy", "cd3d_last_error", "cd3d_device_bytes",
            "cd3d_level_info", "cd3d_get_dofmap", "cd3d_get_cells", "cd3d_get_csr", "cd3d_get_rhs",
            "cd3d_set_entries", "cd3d_quad_points", "cd3d_set_coeffs_qp", "cd3d_assemble", "cd3d_matvec", "cd3d_matvec_dev", "cd3d_matvec_noalloc",
            "cd3d_prolongate",
@@ -67,7 +68,7 @@ class Stats(C.Structure):
 class ServerParams(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_inflight", C.c_int), ("workers", C.c_int),
                 ("keep_resident", C.c_int), ("max_bypass", C.c_int), ("simulate", C.c_int),
-                ("sim_us_per_dof", C.c_double), ("robust_after", C.c_int)]
+                ("sim_us_per_dof", C.c_double), ("robust_after", C.c_int), ("max_batch", C.c_int)]
 
 
 class Request(C.Structure):
@@ -80,7 +81,7 @@ class Result(C.Structure):
     _fields_ = [("id", C.c_uint64), ("status", C.c_int), ("iterations", C.c_int), ("rel_residual", C.c_double),
                 ("queue_ms", C.c_double), ("run_ms", C.c_double), ("latency_ms", C.c_double),
                 ("device_bytes", C.c_size_t), ("reused_hierarchy", C.c_int), ("n_dof", C.c_int),
-                ("steps_done", C.c_int), ("escalated", C.c_int)]
+                ("steps_done", C.c_int), ("escalated", C.c_int), ("batch_size", C.c_int)]
 
 
 class ServerStats(C.Structure):
@@ -109,6 +110,7 @@ def lib():
     L.cd3d_default_solver.restype = None
     L.cd3d_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(P)]
     L.cd3d_create_mapped.argtypes = [C.POINTER(MeshDesc), dp, C.POINTER(P)]
+    L.cd3d_solve_batch.argtypes = [C.POINTER(P), C.c_int, C.POINTER(SolverParams), C.POINTER(Stats)]
     L.cd3d_read_geo.argtypes = [C.c_char_p, ip, dp, C.c_size_t, C.c_char_p, C.c_size_t]
     L.cd3d_destroy.argtypes = [P]
     L.cd3d_last_error.argtypes = [P]
@@ -182,6 +184,19 @@ def solver_params(**kw):
             raise KeyError(k)
         setattr(sp, k, v)
     return sp
+
+
+def solve_batch(problems, sp):
+    """Solve assembled problems together (cd3d_solve_batch); returns the
+    list of their Stats (each with its own status) -- the solutions are in
+    each problem's solution()."""
+    n = len(problems)
+    hs = (C.c_void_p * n)(*[p.h for p in problems])
+    st = (Stats * n)()
+    rc = lib().cd3d_solve_batch(hs, n, C.byref(sp), st)
+    if rc not in (CD3D_OK, 4, 5, 6):
+        raise CD3DError(rc, "cd3d_solve_batch: " + problems[0].L.cd3d_last_error(problems[0].h).decode())
+    return list(st)
 
 
 def read_geo(path):
@@ -447,7 +462,7 @@ class Server:
     concurrently, each on its own resident hierarchy and stream."""
 
     def __init__(self, hbm_budget=0, max_inflight=8, workers=0, keep_resident=True, max_bypass=16,
-                 simulate=False, sim_us_per_dof=0.01, robust_after=None):
+                 simulate=False, sim_us_per_dof=0.01, robust_after=None, max_batch=None):
         self.L = lib()
         prm = ServerParams()
         self.L.cd3d_server_default_params(C.byref(prm))
@@ -456,6 +471,8 @@ class Server:
         prm.simulate, prm.sim_us_per_dof = int(simulate), sim_us_per_dof
         if robust_after is not None:
             prm.robust_after = robust_after
+        if max_batch is not None:
+            prm.max_batch = max_batch
         h = C.c_void_p()
         st = self.L.cd3d_server_create(C.byref(prm), C.byref(h))
         if st != CD3D_OK:

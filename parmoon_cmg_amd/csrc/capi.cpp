@@ -902,6 +902,333 @@ int fgmres_iterate(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S
   return sp.maxit;
 }
 
+// ============================================================== same-level batches
+
+// A batch: B assembled problems with the same number of levels, solved by
+// the FP64 V-cycle (damped Jacobi, exact coarse solve) inside FGMRES in
+// lockstep.  Every launch serves every problem still iterating
+// (blockIdx.y = problem, arrays through the tables); the host runs the
+// Givens rotations and stopping tests of all problems after one transfer of
+// their H columns per iteration.  Per problem the iteration follows
+// fgmres_iterate step by step (same stopping rules, same iteration count
+// semantics); only the reduction trees differ, so results agree with a
+// problem solved alone to rounding.
+struct BatchRun {
+  std::vector<cd3d_problem *> ps;
+  const cd3d_solver_params &sp;
+  cd3d_problem &lead;
+  hipStream_t s;
+  int B, NL, R;
+  std::vector<int> maxblk, maxn, maxn_c;
+  int maxna = 0;
+  BatchRun(std::vector<cd3d_problem *> p, const cd3d_solver_params &sp_)
+      : ps(std::move(p)), sp(sp_), lead(*ps[0]), s(ps[0]->stream), B((int)ps.size()),
+        NL((int)ps[0]->dl.size()) {
+    R = std::min(sp.restart, sp.maxit);
+    if (R <= 0) R = 10;
+  }
+  Problem::BatchWs &ws() { return lead.bws; }
+  const BatchLevel *lev(int l) { return ws().dlev + (size_t)l * ws().cap_b; }
+  const BatchProblem *prob() { return ws().dprob; }
+  const int *act() { return ws().dactive; }
+};
+
+static void batch_workspace(cd3d_problem &P, int B, int NL) {
+  Problem::BatchWs &w = P.bws;
+  if (w.cap_b >= B && w.cap_l >= NL) return;
+  const int cb = std::max(B, w.cap_b), cl = std::max(NL, w.cap_l);
+  for (void *d : {(void *)w.dlev, (void *)w.dprob, (void *)w.dactive, (void *)w.dred, (void *)w.dscal,
+                  (void *)w.dcoef})
+    if (d) dfree(P, d);
+  for (void *h : {(void *)w.hscal, (void *)w.hcoef, (void *)w.hactive})
+    if (h) (void)hipHostFree(h);
+  w = Problem::BatchWs{};
+  w.dlev = dalloc<BatchLevel>(P, (size_t)cb * cl);
+  w.dprob = dalloc<BatchProblem>(P, cb);
+  w.dactive = dalloc<int>(P, cb);
+  w.dred = dalloc<double>(P, (size_t)cb * 2 * kBatchRed);
+  w.dscal = dalloc<double>(P, (size_t)cb * kBatchScal);
+  w.dcoef = dalloc<double>(P, (size_t)cb * kBatchCoef);
+  HIPCHK(hipHostMalloc((void **)&w.hscal, sizeof(double) * cb * kBatchScal, hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void **)&w.hcoef, sizeof(double) * cb * kBatchCoef, hipHostMallocDefault));
+  HIPCHK(hipHostMalloc((void **)&w.hactive, sizeof(int) * cb, hipHostMallocDefault));
+  w.cap_b = cb;
+  w.cap_l = cl;
+}
+
+// one V-cycle of all active problems, z = V(r) with the finest level's rhs r
+// (zero on Dirichlet rows), as precondition / cycle64 / coarse64
+static void batch_cycle(BatchRun &br, int i) {
+  const cd3d_solver_params &sp = br.sp;
+  const int B = br.B;
+  if (i == 0) {
+    kb_fill_zero(br.s, B, br.maxn[0], br.lev(0), br.prob(), br.act(), kBSol);
+    kb_copy(br.s, B, br.maxn[0], br.lev(0), br.prob(), br.act(), kBRhs, kBSol, true);
+    kb_stream(br.s, kDefectMode, B, br.maxblk[0], br.lev(0), br.prob(), br.act(), kBSol, kBRhs, kBAux, 0.0);
+    kb_gemv_coarse(br.s, B, br.maxna, br.lev(0), br.prob(), br.act(), kBAux, kBSol);
+    return;
+  }
+  int x = kBSol, t = kBTmp;
+  for (int k = 0; k < sp.pre; k++) {
+    if (k == 0) kb_jacobi_zero(br.s, B, br.maxn[i], br.lev(i), br.prob(), br.act(), kBRhs, t, sp.omega);
+    else kb_stream(br.s, kJacobiMode, B, br.maxblk[i], br.lev(i), br.prob(), br.act(), x, kBRhs, t, sp.omega);
+    std::swap(x, t);
+  }
+  kb_stream(br.s, kDefectMode, B, br.maxblk[i], br.lev(i), br.prob(), br.act(), x, kBRhs, kBAux, 0.0);
+  kb_restrict(br.s, B, br.maxn_c[i], br.lev(i), br.prob(), br.act(), kBAux, kBRhs, br.lev(i - 1));
+  batch_cycle(br, i - 1);
+  kb_prolong_update(br.s, B, br.maxn[i], br.lev(i), br.prob(), br.act(), br.lev(i - 1), kBSol, x, sp.gmg_damp);
+  for (int k = 0; k < sp.post; k++) {
+    kb_stream(br.s, kJacobiMode, B, br.maxblk[i], br.lev(i), br.prob(), br.act(), x, kBRhs, t, sp.omega);
+    std::swap(x, t);
+  }
+  if (x != kBSol) kb_copy(br.s, B, br.maxn[i], br.lev(i), br.prob(), br.act(), x, kBSol, false);
+}
+
+// the stopping state of one problem of a batch
+struct BatchState {
+  std::vector<double> H, s, cs, sn;
+  double beta = 0, res0 = 0, r0 = 0, res_norm_min = 0, red_factor = 0;
+  int its = 0;
+  cd3d_status status = CD3D_OK;
+  bool active = false;
+};
+
+static void batch_solve_impl(BatchRun &br, std::vector<BatchState> &st) {
+  const cd3d_solver_params &sp = br.sp;
+  const int B = br.B, NL = br.NL, R = br.R, top = NL - 1;
+  if (R > kBatchMaxRestart) throw ArgError("batched solves take restart <= 200");
+  batch_workspace(br.lead, B, NL);
+  Problem::BatchWs &w = br.ws();
+  // per problem: Krylov space, 1/a_ii, initial guess (0 with the Dirichlet
+  // values of the rhs) and rhs on its own stream, then the tables
+  std::vector<BatchLevel> hl((size_t)w.cap_b * NL);
+  std::vector<BatchProblem> hp(B);
+  br.maxblk.assign(NL, 0);
+  br.maxn.assign(NL, 0);
+  br.maxn_c.assign(NL, 0);
+  for (int b = 0; b < B; b++) {
+    cd3d_problem &P = *br.ps[b];
+    coarse_join(P);
+    if (!P.cinv_valid) {
+      launch_coarse_inverse(P);
+      coarse_join(P);
+    }
+    ensure_krylov(P, R);
+    DevLevel &F = P.dl.back();
+    for (int l = 0; l < NL; l++) {
+      DevLevel &D = P.dl[l];
+      if (!D.invdiag_ok) {
+        k_invdiag(P.stream, D);
+        D.invdiag_ok = true;
+      }
+      BatchLevel &L = hl[(size_t)l * w.cap_b + b];
+      L.ndof = D.ndof;
+      L.nact = D.nact;
+      L.nblk = D.nblk;
+      L.blk = D.blk;
+      L.rowptr = D.rowptr;
+      L.col = D.col;
+      L.diag = D.diag;
+      L.val = D.val;
+      L.invdiag = D.invdiag;
+      L.vec[0] = D.sol;
+      L.vec[1] = D.rhs;
+      L.vec[2] = D.aux;
+      L.vec[3] = D.tmp;
+      if (l > 0) {
+        L.p_rowptr = D.p_rowptr;
+        L.p_col = D.p_col;
+        L.p_val = D.p_val;
+        L.r_rowptr = D.r_rowptr;
+        L.r_col = D.r_col;
+        L.r_val = D.r_val;
+        L.ndof_c = P.dl[l - 1].ndof;
+        L.nact_c = P.dl[l - 1].nact;
+        br.maxn_c[l] = std::max(br.maxn_c[l], L.ndof_c);
+      }
+      br.maxblk[l] = std::max(br.maxblk[l], D.nblk);
+      br.maxn[l] = std::max(br.maxn[l], D.ndof);
+    }
+    BatchProblem &Q = hp[b];
+    Q.n = F.ndof;
+    Q.na0 = P.dl[0].nact;
+    Q.cinv = P.cinv;
+    Q.V = P.V;
+    Q.Z = P.Z;
+    Q.ld = P.krylov_ld;
+    Q.xs = P.xs;
+    Q.bs = P.bs;
+    Q.w = P.w;
+    Q.red = w.dred + (size_t)b * 2 * kBatchRed;
+    Q.scal = w.dscal + (size_t)b * kBatchScal;
+    Q.coef = w.dcoef + (size_t)b * kBatchCoef;
+    br.maxna = std::max(br.maxna, Q.na0);
+    k_copy(P.stream, F.ndof, F.rhs_asm, P.bs);
+    HIPCHK(hipMemsetAsync(P.xs, 0, sizeof(double) * F.ndof, P.stream));
+    k_copy_range(P.stream, F.nact, F.ndof, P.bs, P.xs);
+    HIPCHK(hipStreamSynchronize(P.stream));
+  }
+  HIPCHK(hipMemcpyAsync(w.dlev, hl.data(), sizeof(BatchLevel) * hl.size(), hipMemcpyHostToDevice, br.s));
+  HIPCHK(hipMemcpyAsync(w.dprob, hp.data(), sizeof(BatchProblem) * B, hipMemcpyHostToDevice, br.s));
+  const BatchLevel *FL = br.lev(top);
+  auto upload_active = [&] {
+    for (int b = 0; b < B; b++) w.hactive[b] = st[b].active ? 1 : 0;
+    HIPCHK(hipMemcpyAsync(w.dactive, w.hactive, sizeof(int) * B, hipMemcpyHostToDevice, br.s));
+  };
+  auto upload_coef = [&] {
+    HIPCHK(hipMemcpyAsync(w.dcoef, w.hcoef, sizeof(double) * kBatchCoef * B, hipMemcpyHostToDevice, br.s));
+  };
+  // betas of the active problems: |b - A x| into scal[0]
+  auto residual_norms = [&](int out_vec) {
+    kb_stream(br.s, kDefectMode, B, br.maxblk[top], FL, br.prob(), br.act(), kBXs, kBBs, out_vec, 0.0);
+    kb_dot(br.s, B, FL, br.prob(), br.act(), out_vec, out_vec, 0);
+    kb_finish(br.s, B, br.prob(), br.act(), 0, 0, true);
+    HIPCHK(hipMemcpyAsync(w.hscal, w.dscal, sizeof(double) * kBatchScal * B, hipMemcpyDeviceToHost, br.s));
+    HIPCHK(hipStreamSynchronize(br.s));
+  };
+  auto Hm = [&](BatchState &q, int r, int c) -> double & { return q.H[(size_t)r * R + c]; };
+  // x += Z y for the problems listed (back substitution through k + 1 columns)
+  auto update = [&](const std::vector<std::pair<int, int>> &which) {
+    if (which.empty()) return;
+    for (int b = 0; b < B; b++) w.hcoef[(size_t)b * kBatchCoef] = 0.0;
+    for (auto [b, k] : which) {
+      BatchState &q = st[b];
+      std::vector<double> y(q.s.begin(), q.s.begin() + k + 1);
+      for (int i = k; i >= 0; i--) {
+        y[i] /= Hm(q, i, i);
+        for (int jj = i - 1; jj >= 0; jj--) y[jj] -= Hm(q, jj, i) * y[i];
+      }
+      double *c = w.hcoef + (size_t)b * kBatchCoef;
+      c[0] = k + 1;
+      for (int i = 0; i <= k; i++) c[1 + i] = y[i];
+    }
+    upload_coef();
+    kb_update(br.s, B, FL, br.prob());
+    HIPCHK(hipStreamSynchronize(br.s));  // the pinned coefficients are rewritten next
+  };
+  for (int b = 0; b < B; b++) {
+    BatchState &q = st[b];
+    q.H.assign((size_t)(R + 1) * R, 0.0);
+    q.s.assign(R + 1, 0.0);
+    q.cs.assign(R + 1, 0.0);
+    q.sn.assign(R + 1, 0.0);
+    q.active = true;
+  }
+  upload_active();
+  residual_norms(kBV0);
+  for (int b = 0; b < B; b++) {
+    BatchState &q = st[b];
+    q.beta = q.r0 = q.res0 = w.hscal[(size_t)b * kBatchScal];
+    q.res_norm_min = sp.res_norm_min;
+    q.red_factor = sp.red_factor;
+    if (sp.rel_tol > 0) {
+      q.res_norm_min = sp.rel_tol * q.beta;
+      q.red_factor = 0.0;
+    }
+    if (!std::isfinite(q.beta)) {
+      q.status = CD3D_NONFINITE;
+      q.active = false;
+    } else if (q.beta <= q.res_norm_min) {
+      q.active = false;
+    }
+  }
+  upload_active();
+  int j = 1;
+  auto any_active = [&] {
+    for (auto &q : st)
+      if (q.active) return true;
+    return false;
+  };
+  while (j <= sp.maxit && any_active()) {
+    // v_0 /= beta (problems that finished at the restart boundary keep their coef 0)
+    for (int b = 0; b < B; b++) {
+      double *c = w.hcoef + (size_t)b * kBatchCoef;
+      c[0] = st[b].active ? 1.0 : 0.0;
+      c[1] = st[b].active ? 1.0 / st[b].beta : 0.0;
+      if (st[b].active) {
+        std::fill(st[b].s.begin(), st[b].s.end(), 0.0);
+        st[b].s[0] = st[b].beta;
+      }
+    }
+    upload_coef();
+    kb_scale(br.s, B, FL, br.prob(), br.act(), kBV0);
+    int i;
+    for (i = 0; i < R && j <= sp.maxit && any_active(); i++, j++) {
+      // z_i = V(v_i), w = A z_i, modified Gram-Schmidt, v_{i+1} = w / |w|
+      kb_copy(br.s, B, br.maxn[top], FL, br.prob(), br.act(), kBV0 + i, kBRhs, false);
+      batch_cycle(br, top);
+      kb_copy(br.s, B, br.maxn[top], FL, br.prob(), br.act(), kBSol, kBZ0 + i, false);
+      kb_stream(br.s, kSpmvMode, B, br.maxblk[top], FL, br.prob(), br.act(), kBZ0 + i, -1, kBW, 0.0);
+      kb_dot(br.s, B, FL, br.prob(), br.act(), kBW, kBV0, 0);
+      for (int k = 0; k <= i; k++)
+        kb_axpy_dot(br.s, B, FL, br.prob(), br.act(), k % 2, k, -1.0, kBV0 + k, kBW, k < i ? kBV0 + k + 1 : -1,
+                    1 - k % 2);
+      kb_scale_copy(br.s, B, FL, br.prob(), br.act(), (i + 1) % 2, i + 1, kBW, kBV0 + i + 1);
+      HIPCHK(hipMemcpyAsync(w.hscal, w.dscal, sizeof(double) * kBatchScal * B, hipMemcpyDeviceToHost, br.s));
+      HIPCHK(hipStreamSynchronize(br.s));
+      std::vector<std::pair<int, int>> done;
+      bool changed = false;
+      for (int b = 0; b < B; b++) {
+        BatchState &q = st[b];
+        if (!q.active) continue;
+        const double *hs = w.hscal + (size_t)b * kBatchScal;
+        for (int k = 0; k <= i + 1; k++) Hm(q, k, i) = hs[k];
+        if (!std::isfinite(Hm(q, i + 1, i))) {
+          q.status = CD3D_NONFINITE;
+          q.its = j;
+          q.active = false;
+          changed = true;
+          continue;
+        }
+        for (int k = 0; k < i; k++) app_rot(Hm(q, k, i), Hm(q, k + 1, i), q.cs[k], q.sn[k]);
+        gen_rot(Hm(q, i, i), Hm(q, i + 1, i), q.cs[i], q.sn[i]);
+        app_rot(Hm(q, i, i), Hm(q, i + 1, i), q.cs[i], q.sn[i]);
+        app_rot(q.s[i], q.s[i + 1], q.cs[i], q.sn[i]);
+        const double res = fabs(q.s[i + 1]);
+        if (res < q.res_norm_min || res < q.red_factor * q.res0) {
+          done.push_back({b, i});
+          q.its = j;
+          q.active = false;
+          changed = true;
+        } else if (res > sp.div_factor * q.res0) {
+          q.status = CD3D_DIVERGED;
+          q.its = j;
+          q.active = false;
+          changed = true;
+        }
+      }
+      update(done);
+      if (changed) upload_active();
+    }
+    // end of a cycle (or maxit inside one): the active problems take this
+    // cycle's columns and restart from their new residual
+    std::vector<std::pair<int, int>> cyc;
+    for (int b = 0; b < B; b++)
+      if (st[b].active && i > 0) cyc.push_back({b, i - 1});
+    update(cyc);
+    if (!any_active()) break;
+    residual_norms(kBV0);
+    for (int b = 0; b < B; b++) {
+      BatchState &q = st[b];
+      if (!q.active) continue;
+      q.beta = q.res0 = w.hscal[(size_t)b * kBatchScal];
+      if (!std::isfinite(q.beta)) {
+        q.status = CD3D_NONFINITE;
+        q.its = j;
+        q.active = false;
+      }
+    }
+    upload_active();
+  }
+  for (auto &q : st)
+    if (q.active) {
+      q.status = CD3D_NOT_CONVERGED;
+      q.its = sp.maxit;
+      q.active = false;
+    }
+}
+
 void check_params(const cd3d_solver_params &sp) {
   if (sp.smoother != 1 && sp.smoother != 13)
     throw ArgError("smoother: 1 (damped Jacobi) or 13 (multicolour SSOR); the reference's sequential SOR/SSOR/ILU "
@@ -1037,6 +1364,8 @@ cd3d_status cd3d_destroy(cd3d_problem *p) {
   if (p->side) (void)hipStreamSynchronize(p->side);
   for (void *a : p->allocs) (void)hipFree(a);
   if (p->hscal) (void)hipHostFree(p->hscal);
+  for (void *h : {(void *)p->bws.hscal, (void *)p->bws.hcoef, (void *)p->bws.hactive})
+    if (h) (void)hipHostFree(h);
   if (p->hflag) (void)hipHostFree(p->hflag);
   if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
   if (p->ev_join) (void)hipEventDestroy(p->ev_join);
@@ -1323,6 +1652,66 @@ cd3d_status cd3d_solve(cd3d_problem *p, const cd3d_solver_params *sp, const doub
     }
   });
   return rc != CD3D_OK ? rc : solve_status;
+}
+
+cd3d_status cd3d_solve_batch(cd3d_problem **ps, int count, const cd3d_solver_params *sp, cd3d_stats *stats) {
+  if (!ps || count < 1 || !sp) return CD3D_ERR_ARG;
+  std::vector<cd3d_problem *> v(ps, ps + count);
+  for (auto *p : v)
+    if (!p || p->device != v[0]->device || p->dl.size() != v[0]->dl.size()) return CD3D_ERR_ARG;
+  std::vector<cd3d_problem *> order(v);
+  std::sort(order.begin(), order.end());
+  if (std::adjacent_find(order.begin(), order.end()) != order.end()) return CD3D_ERR_ARG;  // a handle twice
+  const int R = std::min(sp->restart, sp->maxit);
+  const bool batchable = count > 1 && sp->precision == CD3D_FP64 && sp->smoother == 1 && sp->coarse_smoother == 17 &&
+                         sp->cycle == 1 && sp->pre >= 1 && R >= 1 && R <= kBatchMaxRestart;
+  if (!batchable) {  // one after another, each with its own stream
+    cd3d_status worst = CD3D_OK;
+    for (int b = 0; b < count; b++) {
+      const cd3d_status st = cd3d_solve(v[b], sp, nullptr, nullptr, 0, stats ? &stats[b] : nullptr);
+      if (worst == CD3D_OK) worst = st;
+    }
+    return worst;
+  }
+  std::vector<std::unique_lock<std::mutex>> locks;
+  for (auto *p : order) locks.emplace_back(p->mu);
+  std::vector<BatchState> st(count);
+  double ms = 0;
+  const cd3d_status rc = run(v[0], [&]() {
+    check_params(*sp);
+    for (auto *p : v)
+      if (!p->assembled) throw std::logic_error("assemble every problem before cd3d_solve_batch");
+    const double t0 = now_ms();
+    BatchRun br(v, *sp);
+    batch_solve_impl(br, st);
+    // |b - A x| of every problem, and x into the finest level's solution
+    Problem::BatchWs &w = br.ws();
+    for (int b = 0; b < count; b++) w.hactive[b] = 1;
+    HIPCHK(hipMemcpyAsync(w.dactive, w.hactive, sizeof(int) * count, hipMemcpyHostToDevice, br.s));
+    const BatchLevel *FL = br.lev(br.NL - 1);
+    kb_stream(br.s, kDefectMode, count, br.maxblk[br.NL - 1], FL, br.prob(), br.act(), kBXs, kBBs, kBW, 0.0);
+    kb_dot(br.s, count, FL, br.prob(), br.act(), kBW, kBW, 0);
+    kb_finish(br.s, count, br.prob(), br.act(), 0, 0, true);
+    kb_copy(br.s, count, br.maxn[br.NL - 1], FL, br.prob(), br.act(), kBXs, kBSol, false);
+    HIPCHK(hipMemcpyAsync(w.hscal, w.dscal, sizeof(double) * kBatchScal * count, hipMemcpyDeviceToHost, br.s));
+    HIPCHK(hipStreamSynchronize(br.s));
+    ms = now_ms() - t0;
+    if (stats)
+      for (int b = 0; b < count; b++) {
+        stats[b].iterations = st[b].its;
+        stats[b].status = st[b].status;
+        stats[b].r0 = st[b].r0;
+        stats[b].rfinal = w.hscal[(size_t)b * kBatchScal];
+        stats[b].ms_solve = ms;
+      }
+  });
+  if (rc != CD3D_OK) {
+    for (int b = 1; b < count; b++) v[b]->err = v[0]->err;
+    return rc;
+  }
+  for (auto &q : st)
+    if (q.status != CD3D_OK) return q.status;
+  return CD3D_OK;
 }
 
 cd3d_status cd3d_get_solution(cd3d_problem *p, double *sol) {

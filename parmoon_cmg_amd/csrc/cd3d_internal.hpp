@@ -151,6 +151,9 @@ struct DevLevel {
   bool tcd_mass_uniform = false;  // TCD3D: the mass class stencils are in lst64 (mass slot)
 };
 
+struct BatchLevel;
+struct BatchProblem;
+
 struct Problem {
   cd3d_mesh_desc mesh{};
   // general hexahedra: the macro vertices were given (cd3d_create_mapped);
@@ -200,6 +203,16 @@ struct Problem {
   double *tab = nullptr;
   // coefficients of the last assembly (the kernel bench re-runs its element kernel)
   cd3d_coeffs last_co{};
+  // workspace of the same-level batches this handle leads (grown on demand)
+  struct BatchWs {
+    int cap_b = 0, cap_l = 0;
+    BatchLevel *dlev = nullptr;
+    BatchProblem *dprob = nullptr;
+    int *dactive = nullptr;
+    double *dred = nullptr, *dscal = nullptr, *dcoef = nullptr;
+    double *hscal = nullptr, *hcoef = nullptr;  // pinned mirrors
+    int *hactive = nullptr;                     // pinned
+  } bws;
 };
 
 // ---------------------------------------------------------------- setup (setup.cpp)
@@ -301,6 +314,68 @@ void k_assemble(hipStream_t s, const DevLevel &L, const double *tab, const AsmPa
 void k_dirichlet_rows(hipStream_t s, const DevLevel &L, double *val, double *rhs);
 // mapped levels: per-cell SUPG mesh size in convection direction into hconv (ncell)
 void k_hconv_geo(hipStream_t s, const DevLevel &L, const double *tab, const AsmParams &ap, double *hconv);
+// ---------------------------------------------------------------- same-level batches (kernels.hip)
+// B problems with the same number of levels run one FP64 V-cycle/FGMRES in
+// lockstep: every launch serves all of them (blockIdx.y = problem), each
+// problem reading its own arrays through these tables in device memory.
+// Problems that have finished are masked (active[b] = 0).
+constexpr int kBatchRed = 32;         // reduction workgroups per problem
+constexpr int kBatchScal = 256;       // device scalars per problem (H column)
+constexpr int kBatchCoef = 256;       // host-written coefficients per problem: [count, c_0, c_1, ...]
+constexpr int kBatchMaxRestart = 200;
+// row operations of the batched stream kernel (the values of kernels.hip's RowMode)
+constexpr int kSpmvMode = 0, kDefectMode = 1, kJacobiMode = 2;
+// vector ids: the level's own vectors, then the finest level's Krylov space
+enum BatchVec { kBSol = 0, kBRhs = 1, kBAux = 2, kBTmp = 3, kBXs = 4, kBBs = 5, kBW = 6, kBV0 = 8,
+                kBZ0 = 8 + kBatchMaxRestart + 2 };
+struct BatchLevel {
+  int ndof = 0, nact = 0, nblk = 0;
+  const int *blk = nullptr, *rowptr = nullptr, *col = nullptr, *diag = nullptr;
+  const double *val = nullptr, *invdiag = nullptr;
+  double *vec[4] = {nullptr, nullptr, nullptr, nullptr};  // sol, rhs, aux, tmp
+  // prolongation (rows of this level) and restriction (rows of the coarser level)
+  const int *p_rowptr = nullptr, *p_col = nullptr, *r_rowptr = nullptr, *r_col = nullptr;
+  const double *p_val = nullptr, *r_val = nullptr;
+  int ndof_c = 0, nact_c = 0;
+};
+struct BatchProblem {
+  int n = 0, na0 = 0;             // finest unknowns, coarsest active unknowns
+  const double *cinv = nullptr;   // explicit inverse of the coarsest active block
+  double *V = nullptr, *Z = nullptr, *xs = nullptr, *bs = nullptr, *w = nullptr;
+  int64_t ld = 0;                 // column stride of V and Z
+  double *red = nullptr;          // 2 x kBatchRed reduction partials
+  double *scal = nullptr;         // kBatchScal device scalars
+  const double *coef = nullptr;   // kBatchCoef host-written coefficients
+};
+// level kernels: lev = this level's table (B entries); grid sizes from the largest problem
+void kb_stream(hipStream_t s, int mode, int B, int maxblk, const BatchLevel *lev, const BatchProblem *prob,
+               const int *active, int x, int f, int out, double omega);
+void kb_jacobi_zero(hipStream_t s, int B, int maxn, const BatchLevel *lev, const BatchProblem *prob,
+                    const int *active, int f, int out, double omega);
+void kb_restrict(hipStream_t s, int B, int maxn_c, const BatchLevel *fine, const BatchProblem *prob,
+                 const int *active, int x, int out_coarse, const BatchLevel *coarse);
+void kb_prolong_update(hipStream_t s, int B, int maxn, const BatchLevel *fine, const BatchProblem *prob,
+                       const int *active, const BatchLevel *coarse, int x_coarse, int out, double damp);
+// out = x on [from, ndof) (from 0: the whole vector) or, with x < 0, out = 0
+void kb_copy(hipStream_t s, int B, int maxn, const BatchLevel *lev, const BatchProblem *prob, const int *active,
+             int x, int out, bool dirichlet_only);
+void kb_fill_zero(hipStream_t s, int B, int maxn, const BatchLevel *lev, const BatchProblem *prob, const int *active,
+                  int out);
+// coarsest level: out_A += Ainv d_A
+void kb_gemv_coarse(hipStream_t s, int B, int maxna, const BatchLevel *lev, const BatchProblem *prob,
+                    const int *active, int d, int out);
+// finest level reductions: partials slot 0/1; scalars at scal[idx]
+void kb_dot(hipStream_t s, int B, const BatchLevel *fine, const BatchProblem *prob, const int *active, int x, int y,
+            int slot);
+void kb_finish(hipStream_t s, int B, const BatchProblem *prob, const int *active, int slot, int idx, bool root);
+void kb_axpy_dot(hipStream_t s, int B, const BatchLevel *fine, const BatchProblem *prob, const int *active,
+                 int slot_in, int alpha_idx, double sign, int x, int y, int z, int slot_out);
+void kb_scale_copy(hipStream_t s, int B, const BatchLevel *fine, const BatchProblem *prob, const int *active,
+                   int slot_in, int norm_idx, int x, int y);
+// x *= coef[1] (problems whose coef[0] != 0)
+void kb_scale(hipStream_t s, int B, const BatchLevel *fine, const BatchProblem *prob, const int *active, int x);
+// xs += sum_k coef[1 + k] Z_k over k < coef[0] (problems whose coef[0] != 0; ignores the mask)
+void kb_update(hipStream_t s, int B, const BatchLevel *fine, const BatchProblem *prob);
 // ---------------------------------------------------------------- diagnostics (diag.hip)
 // known function of the interpolation and error norms: fn 0 zero, 1 the sine
 // of Examples/CD_3D/Laplace.h, 2 the Gaussian pulse centred at pc

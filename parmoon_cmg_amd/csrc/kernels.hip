@@ -42,11 +42,9 @@ __device__ inline T group_sum(T v) {
 // UNROLL independent (column, value, x) load chains per lane and iteration
 // keep more HBM requests in flight.
 template <int LANES, int UNROLL, int MODE, typename T, typename V>
-__global__ __launch_bounds__(kBlock) void csr_rows(int n, int nact, const int *__restrict__ rowptr,
-                                                   const int *__restrict__ col, const V *__restrict__ val,
-                                                   const T *__restrict__ x, const T *__restrict__ f,
-                                                   T *__restrict__ out, T omega) {
-  const int gid = blockIdx.x * kBlock + threadIdx.x;
+__device__ inline void csr_rows_body(int gid, int n, int nact, const int *__restrict__ rowptr,
+                                     const int *__restrict__ col, const V *__restrict__ val, const T *__restrict__ x,
+                                     const T *__restrict__ f, T *__restrict__ out, T omega) {
   const int row = gid / LANES, lane = gid % LANES;
   if (row >= n) return;  // whole groups leave together
   if ((MODE == kRestrict || MODE == kProlongUpd || MODE == kActiveSpmv) && row >= nact) {
@@ -82,6 +80,15 @@ __global__ __launch_bounds__(kBlock) void csr_rows(int n, int nact, const int *_
   else if (MODE == kDefect) out[row] = f[row] - s;
   else if (MODE == kJacobi) out[row] = row < nact ? (T(1) - omega) * x[row] + omega * (f[row] - s) / diag : f[row];
   else if (MODE == kProlongUpd) out[row] += omega * s;
+}
+
+template <int LANES, int UNROLL, int MODE, typename T, typename V>
+__global__ __launch_bounds__(kBlock) void csr_rows(int n, int nact, const int *__restrict__ rowptr,
+                                                   const int *__restrict__ col, const V *__restrict__ val,
+                                                   const T *__restrict__ x, const T *__restrict__ f,
+                                                   T *__restrict__ out, T omega) {
+  csr_rows_body<LANES, UNROLL, MODE, T, V>(blockIdx.x * kBlock + threadIdx.x, n, nact, rowptr, col, val, x, f, out,
+                                           omega);
 }
 
 // Relaxed Gauss-Seidel over the rows of one colour of the multicolour SSOR
@@ -196,18 +203,15 @@ __device__ inline P stream_load(const P *p) {
   return *p;
 }
 
+// One row block bi of the stream kernel (the whole workgroup takes part).
 // NT: non-temporal matrix loads (matrices larger than the caches)
 template <int MODE, typename T, typename V, bool NT>
-__global__ __launch_bounds__(kStreamBlock) void csr_stream(const int *__restrict__ blk, int nact,
-                                                     const int *__restrict__ rowptr, const int *__restrict__ col,
-                                                     const V *__restrict__ val, const T *__restrict__ x,
-                                                     const T *__restrict__ f, const T *__restrict__ invdiag,
-                                                     T *__restrict__ out, T omega) {
+__device__ inline void csr_stream_block(int bi, const int *__restrict__ blk, int nact,
+                                        const int *__restrict__ rowptr, const int *__restrict__ col,
+                                        const V *__restrict__ val, const T *__restrict__ x,
+                                        const T *__restrict__ f, const T *__restrict__ invdiag, T *__restrict__ out,
+                                        T omega) {
   __shared__ T prod[kStreamNnz];
-  // workgroups go to the 8 XCDs round-robin: XCD x takes a contiguous range
-  // of row blocks, so that its L2 sees about 1/8 of the gathered x (fine
-  // level of the 64^3 Q2 mesh: 370.7 -> 352.8 us)
-  const int bi = xcd_block(blockIdx.x, gridDim.x);
   const int r0 = blk[bi], r1 = blk[bi + 1];
   const int j0 = rowptr[r0], nz = rowptr[r1] - j0;
   const int nr = r1 - r0;
@@ -271,6 +275,19 @@ __global__ __launch_bounds__(kStreamBlock) void csr_stream(const int *__restrict
     case 2: stream_rows<2, MODE, T>(nr, r0, j0, nact, rowptr, prod, x, f, invdiag, out, omega); break;
     default: stream_rows<1, MODE, T>(nr, r0, j0, nact, rowptr, prod, x, f, invdiag, out, omega); break;
   }
+}
+
+template <int MODE, typename T, typename V, bool NT>
+__global__ __launch_bounds__(kStreamBlock) void csr_stream(const int *__restrict__ blk, int nact,
+                                                     const int *__restrict__ rowptr, const int *__restrict__ col,
+                                                     const V *__restrict__ val, const T *__restrict__ x,
+                                                     const T *__restrict__ f, const T *__restrict__ invdiag,
+                                                     T *__restrict__ out, T omega) {
+  // workgroups go to the 8 XCDs round-robin: XCD x takes a contiguous range
+  // of row blocks, so that its L2 sees about 1/8 of the gathered x (fine
+  // level of the 64^3 Q2 mesh: 370.7 -> 352.8 us)
+  csr_stream_block<MODE, T, V, NT>(xcd_block(blockIdx.x, gridDim.x), blk, nact, rowptr, col, val, x, f, invdiag, out,
+                                   omega);
 }
 
 template <typename T>
@@ -1647,6 +1664,261 @@ void k_dirichlet_rows(hipStream_t s, const DevLevel &L, double *val, double *rhs
   const int nd = L.ndof - L.nact;
   if (nd <= 0) return;
   dirichlet_rows_k<<<(nd + kBlock - 1) / kBlock, kBlock, 0, s>>>(L.ndof, L.nact, L.rowptr, val, rhs);
+}
+
+// ---------------------------------------------------------------- same-level batches
+// blockIdx.y is the problem; each kernel takes its arrays from the tables and
+// returns at once for a finished problem (the test is uniform per workgroup).
+
+__device__ inline double *bvec(const BatchLevel &L, const BatchProblem &P, int id) {
+  if (id < 4) return L.vec[id];
+  if (id == kBXs) return P.xs;
+  if (id == kBBs) return P.bs;
+  if (id == kBW) return P.w;
+  if (id >= kBZ0) return P.Z + (int64_t)(id - kBZ0) * P.ld;
+  return P.V + (int64_t)(id - kBV0) * P.ld;
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kStreamBlock) void bstream_k(const BatchLevel *__restrict__ lev,
+                                                    const BatchProblem *__restrict__ prob,
+                                                    const int *__restrict__ active, int xid, int fid, int oid,
+                                                    double omega) {
+  const int b = blockIdx.y;
+  if (!active[b]) return;
+  const BatchLevel &L = lev[b];
+  if ((int)blockIdx.x >= L.nblk) return;
+  const BatchProblem &P = prob[b];
+  csr_stream_block<MODE, double, double, false>(blockIdx.x, L.blk, L.nact, L.rowptr, L.col, L.val, bvec(L, P, xid),
+                                                fid >= 0 ? bvec(L, P, fid) : nullptr, L.invdiag, bvec(L, P, oid),
+                                                omega);
+}
+
+__global__ __launch_bounds__(kBlock) void bjacobi_zero_k(const BatchLevel *__restrict__ lev,
+                                                         const BatchProblem *__restrict__ prob,
+                                                         const int *__restrict__ active, int fid, int oid,
+                                                         double omega) {
+  const int b = blockIdx.y;
+  if (!active[b]) return;
+  const BatchLevel &L = lev[b];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= L.ndof) return;
+  const double *f = bvec(L, prob[b], fid);
+  double *out = bvec(L, prob[b], oid);
+  out[i] = i < L.nact ? omega * f[i] / L.val[L.diag[i]] : f[i];
+}
+
+// restriction onto the coarser level's rows: 8 lanes per row
+__global__ __launch_bounds__(kBlock) void brestrict_k(const BatchLevel *__restrict__ fine,
+                                                      const BatchProblem *__restrict__ prob,
+                                                      const int *__restrict__ active,
+                                                      const BatchLevel *__restrict__ coarse, int xid, int oid) {
+  const int b = blockIdx.y;
+  if (!active[b]) return;
+  const BatchLevel &F = fine[b], &Cl = coarse[b];
+  csr_rows_body<8, 1, kRestrict, double, double>(blockIdx.x * kBlock + threadIdx.x, F.ndof_c, F.nact_c, F.r_rowptr,
+                                                 F.r_col, F.r_val, bvec(F, prob[b], xid), nullptr,
+                                                 bvec(Cl, prob[b], oid), 0.0);
+}
+
+__global__ __launch_bounds__(kBlock) void bprolong_k(const BatchLevel *__restrict__ fine,
+                                                     const BatchProblem *__restrict__ prob,
+                                                     const int *__restrict__ active,
+                                                     const BatchLevel *__restrict__ coarse, int xid, int oid,
+                                                     double damp) {
+  const int b = blockIdx.y;
+  if (!active[b]) return;
+  const BatchLevel &F = fine[b], &Cl = coarse[b];
+  csr_rows_body<8, 1, kProlongUpd, double, double>(blockIdx.x * kBlock + threadIdx.x, F.ndof, F.nact, F.p_rowptr,
+                                                   F.p_col, F.p_val, bvec(Cl, prob[b], xid), nullptr,
+                                                   bvec(F, prob[b], oid), damp);
+}
+
+__global__ __launch_bounds__(kBlock) void bcopy_k(const BatchLevel *__restrict__ lev,
+                                                  const BatchProblem *__restrict__ prob,
+                                                  const int *__restrict__ active, int xid, int oid,
+                                                  int dirichlet_only) {
+  const int b = blockIdx.y;
+  if (!active[b]) return;
+  const BatchLevel &L = lev[b];
+  const int i = (dirichlet_only ? L.nact : 0) + blockIdx.x * kBlock + threadIdx.x;
+  if (i >= L.ndof) return;
+  double *out = bvec(L, prob[b], oid);
+  out[i] = xid >= 0 ? bvec(L, prob[b], xid)[i] : 0.0;
+}
+
+__global__ __launch_bounds__(kBlock) void bgemv_k(const BatchLevel *__restrict__ lev,
+                                                  const BatchProblem *__restrict__ prob,
+                                                  const int *__restrict__ active, int did, int oid) {
+  const int b = blockIdx.y;
+  if (!active[b]) return;
+  const BatchProblem &P = prob[b];
+  const int na = P.na0, row = (blockIdx.x * kBlock + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  if (row >= na) return;
+  const double *d = bvec(lev[b], P, did);
+  double s = 0;
+  for (int j = lane; j < na; j += 64) s += P.cinv[(int64_t)row * na + j] * d[j];
+  s = group_sum<64>(s);
+  if (lane == 0) bvec(lev[b], P, oid)[row] += s;
+}
+
+__device__ inline double bsum_partials(const double *__restrict__ p, double *sh) {
+  double v = threadIdx.x < kBatchRed ? p[threadIdx.x] : 0.0;
+  v = block_sum(v, sh);
+  __syncthreads();
+  if (threadIdx.x == 0) sh[16] = v;
+  __syncthreads();
+  return sh[16];
+}
+
+__global__ __launch_bounds__(kBlock) void bdot_k(const BatchLevel *__restrict__ fine,
+                                                 const BatchProblem *__restrict__ prob,
+                                                 const int *__restrict__ active, int xid, int yid, int slot) {
+  __shared__ double sh[17];
+  const int b = blockIdx.y;
+  if (!active[b]) return;
+  const BatchProblem &P = prob[b];
+  const double *x = bvec(fine[b], P, xid), *y = bvec(fine[b], P, yid);
+  double v = 0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < P.n; i += kBatchRed * kBlock) v = fma(x[i], y[i], v);
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) P.red[slot * kBatchRed + blockIdx.x] = v;
+}
+
+// scal[idx] = sum of the partials (root: its square root)
+__global__ __launch_bounds__(kBlock) void bfinish_k(const BatchProblem *__restrict__ prob,
+                                                    const int *__restrict__ active, int slot, int idx, int root) {
+  __shared__ double sh[17];
+  const int b = blockIdx.y;
+  if (!active[b]) return;
+  const BatchProblem &P = prob[b];
+  const double v = bsum_partials(P.red + slot * kBatchRed, sh);
+  if (threadIdx.x == 0) P.scal[idx] = root ? sqrt(v) : v;
+}
+
+// y += sign alpha x with alpha the sum of the partials in slot_in (stored at
+// scal[alpha_idx]), then partials of <y, z> (z < 0: <y, y>) into slot_out
+__global__ __launch_bounds__(kBlock) void baxpy_dot_k(const BatchLevel *__restrict__ fine,
+                                                      const BatchProblem *__restrict__ prob,
+                                                      const int *__restrict__ active, int slot_in, int alpha_idx,
+                                                      double sign, int xid, int yid, int zid, int slot_out) {
+  __shared__ double sh[17];
+  const int b = blockIdx.y;
+  if (!active[b]) return;
+  const BatchProblem &P = prob[b];
+  const double alpha = bsum_partials(P.red + slot_in * kBatchRed, sh);
+  if (blockIdx.x == 0 && threadIdx.x == 0) P.scal[alpha_idx] = alpha;
+  const double a = sign * alpha;
+  const double *x = bvec(fine[b], P, xid), *z = zid >= 0 ? bvec(fine[b], P, zid) : nullptr;
+  double *y = bvec(fine[b], P, yid);
+  double v = 0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < P.n; i += kBatchRed * kBlock) {
+    const double yi = fma(a, x[i], y[i]);
+    y[i] = yi;
+    v = fma(yi, z ? z[i] : yi, v);
+  }
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) P.red[slot_out * kBatchRed + blockIdx.x] = v;
+}
+
+// y = x / sqrt(sum of the partials in slot_in); the norm to scal[norm_idx]
+__global__ __launch_bounds__(kBlock) void bscale_copy_k(const BatchLevel *__restrict__ fine,
+                                                        const BatchProblem *__restrict__ prob,
+                                                        const int *__restrict__ active, int slot_in, int norm_idx,
+                                                        int xid, int yid) {
+  __shared__ double sh[17];
+  const int b = blockIdx.y;
+  if (!active[b]) return;
+  const BatchProblem &P = prob[b];
+  const double nrm = sqrt(bsum_partials(P.red + slot_in * kBatchRed, sh));
+  if (blockIdx.x == 0 && threadIdx.x == 0) P.scal[norm_idx] = nrm;
+  const double sc = nrm != 0.0 ? 1.0 / nrm : 1.0;
+  const double *x = bvec(fine[b], P, xid);
+  double *y = bvec(fine[b], P, yid);
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < P.n; i += kBatchRed * kBlock) y[i] = x[i] * sc;
+}
+
+__global__ __launch_bounds__(kBlock) void bscale_k(const BatchLevel *__restrict__ fine,
+                                                   const BatchProblem *__restrict__ prob,
+                                                   const int *__restrict__ active, int xid) {
+  const int b = blockIdx.y;
+  const BatchProblem &P = prob[b];
+  if (!active[b] || P.coef[0] == 0.0) return;
+  const double a = P.coef[1];
+  double *x = bvec(fine[b], P, xid);
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < P.n; i += kBatchRed * kBlock) x[i] *= a;
+}
+
+__global__ __launch_bounds__(kBlock) void bupdate_k(const BatchProblem *__restrict__ prob) {
+  const int b = blockIdx.y;
+  const BatchProblem &P = prob[b];
+  const int m = (int)P.coef[0];
+  if (m <= 0) return;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < P.n; i += gridDim.x * kBlock) {
+    double t = 0;
+    for (int k = 0; k < m; k++) t += P.coef[1 + k] * P.Z[(int64_t)k * P.ld + i];
+    P.xs[i] += t;
+  }
+}
+
+void kb_stream(hipStream_t s, int mode, int B, int maxblk, const BatchLevel *lev, const BatchProblem *prob,
+               const int *active, int x, int f, int out, double omega) {
+  if (maxblk <= 0 || B <= 0) return;
+  const dim3 grid(maxblk, B);
+  if (mode == kSpmv) bstream_k<kSpmv><<<grid, kStreamBlock, 0, s>>>(lev, prob, active, x, f, out, omega);
+  else if (mode == kDefect) bstream_k<kDefect><<<grid, kStreamBlock, 0, s>>>(lev, prob, active, x, f, out, omega);
+  else bstream_k<kJacobi><<<grid, kStreamBlock, 0, s>>>(lev, prob, active, x, f, out, omega);
+}
+void kb_jacobi_zero(hipStream_t s, int B, int maxn, const BatchLevel *lev, const BatchProblem *prob,
+                    const int *active, int f, int out, double omega) {
+  if (maxn > 0) bjacobi_zero_k<<<dim3((maxn + kBlock - 1) / kBlock, B), kBlock, 0, s>>>(lev, prob, active, f, out, omega);
+}
+void kb_restrict(hipStream_t s, int B, int maxn_c, const BatchLevel *fine, const BatchProblem *prob,
+                 const int *active, int x, int out_coarse, const BatchLevel *coarse) {
+  if (maxn_c > 0)
+    brestrict_k<<<dim3((maxn_c * 8 + kBlock - 1) / kBlock, B), kBlock, 0, s>>>(fine, prob, active, coarse, x,
+                                                                                out_coarse);
+}
+void kb_prolong_update(hipStream_t s, int B, int maxn, const BatchLevel *fine, const BatchProblem *prob,
+                       const int *active, const BatchLevel *coarse, int x_coarse, int out, double damp) {
+  if (maxn > 0)
+    bprolong_k<<<dim3((maxn * 8 + kBlock - 1) / kBlock, B), kBlock, 0, s>>>(fine, prob, active, coarse, x_coarse, out,
+                                                                             damp);
+}
+void kb_copy(hipStream_t s, int B, int maxn, const BatchLevel *lev, const BatchProblem *prob, const int *active,
+             int x, int out, bool dirichlet_only) {
+  if (maxn > 0)
+    bcopy_k<<<dim3((maxn + kBlock - 1) / kBlock, B), kBlock, 0, s>>>(lev, prob, active, x, out, dirichlet_only ? 1 : 0);
+}
+void kb_fill_zero(hipStream_t s, int B, int maxn, const BatchLevel *lev, const BatchProblem *prob, const int *active,
+                  int out) {
+  kb_copy(s, B, maxn, lev, prob, active, -1, out, false);
+}
+void kb_gemv_coarse(hipStream_t s, int B, int maxna, const BatchLevel *lev, const BatchProblem *prob,
+                    const int *active, int d, int out) {
+  if (maxna > 0) bgemv_k<<<dim3((maxna * 64 + kBlock - 1) / kBlock, B), kBlock, 0, s>>>(lev, prob, active, d, out);
+}
+void kb_dot(hipStream_t s, int B, const BatchLevel *fine, const BatchProblem *prob, const int *active, int x, int y,
+            int slot) {
+  bdot_k<<<dim3(kBatchRed, B), kBlock, 0, s>>>(fine, prob, active, x, y, slot);
+}
+void kb_finish(hipStream_t s, int B, const BatchProblem *prob, const int *active, int slot, int idx, bool root) {
+  bfinish_k<<<dim3(1, B), kBlock, 0, s>>>(prob, active, slot, idx, root ? 1 : 0);
+}
+void kb_axpy_dot(hipStream_t s, int B, const BatchLevel *fine, const BatchProblem *prob, const int *active,
+                 int slot_in, int alpha_idx, double sign, int x, int y, int z, int slot_out) {
+  baxpy_dot_k<<<dim3(kBatchRed, B), kBlock, 0, s>>>(fine, prob, active, slot_in, alpha_idx, sign, x, y, z, slot_out);
+}
+void kb_scale_copy(hipStream_t s, int B, const BatchLevel *fine, const BatchProblem *prob, const int *active,
+                   int slot_in, int norm_idx, int x, int y) {
+  bscale_copy_k<<<dim3(kBatchRed, B), kBlock, 0, s>>>(fine, prob, active, slot_in, norm_idx, x, y);
+}
+void kb_scale(hipStream_t s, int B, const BatchLevel *fine, const BatchProblem *prob, const int *active, int x) {
+  bscale_k<<<dim3(kBatchRed, B), kBlock, 0, s>>>(fine, prob, active, x);
+}
+void kb_update(hipStream_t s, int B, const BatchLevel *fine, const BatchProblem *prob) {
+  (void)fine;
+  bupdate_k<<<dim3(kBatchRed, B), kBlock, 0, s>>>(prob);
 }
 
 }  // namespace cd3d

@@ -249,20 +249,117 @@ struct cd3d_server {
     res.status = st;
   }
 
+  // CD requests that cd3d_solve_batch runs in lockstep: FP64, damped Jacobi,
+  // V-cycle, exact coarse solve, at least one pre-smoothing sweep
+  bool batchable(const cd3d_request &r) const {
+    const cd3d_solver_params &s = r.solver;
+    return !prm.simulate && prm.max_batch > 1 && r.kind == CD3D_REQ_CD && s.precision == CD3D_FP64 &&
+           s.smoother == 1 && s.coarse_smoother == 17 && s.cycle == 1 && s.pre >= 1 &&
+           std::min(s.restart, s.maxit) <= 200;
+  }
+  static bool same_batch(const cd3d_request &a, const cd3d_request &b) {
+    return a.mesh.levels == b.mesh.levels && memcmp(&a.solver, &b.solver, sizeof(a.solver)) == 0;
+  }
+
+  // admitted CD requests of one batch: each assembles on its own hierarchy,
+  // all solve together; a member that has not converged after robust_after
+  // iterations continues alone, as in run_request
+  void run_batch(std::vector<Work> &ws, std::vector<Done> &ds) {
+    const int n = (int)ws.size();
+    std::vector<cd3d_problem *> hs;
+    std::vector<int> idx;
+    cd3d_solver_params sp = ws[0].p.req.solver;
+    const bool robust = prm.robust_after > 0 && sp.maxit > prm.robust_after;
+    if (robust) sp.maxit = prm.robust_after;
+    for (int k = 0; k < n; k++) {
+      const cd3d_request &r = ws[k].p.req;
+      ds[k].res.n_dof = (int)finest_dofs(r.mesh);
+      cd3d_problem *&h = ws[k].slot->h;
+      cd3d_status st = CD3D_OK;
+      if (!h) {
+        st = cd3d_create(&r.mesh, &h);
+        if (st == CD3D_OK) cd3d_internal_serving_handle(h);
+        else h = nullptr;
+      }
+      if (st == CD3D_OK) st = cd3d_assemble(h, &r.coeffs);
+      ds[k].res.status = st;
+      if (st == CD3D_OK) {
+        hs.push_back(h);
+        idx.push_back(k);
+      }
+    }
+    if (hs.empty()) return;
+    std::vector<cd3d_stats> st(hs.size());
+    const cd3d_status bst = cd3d_solve_batch(hs.data(), (int)hs.size(), &sp, st.data());
+    for (size_t q = 0; q < hs.size(); q++) {
+      const int k = idx[q];
+      const cd3d_request &r = ws[k].p.req;
+      cd3d_result &res = ds[k].res;
+      cd3d_problem *h = hs[q];
+      cd3d_status s1 = (bst == CD3D_OK || bst == CD3D_NOT_CONVERGED || bst == CD3D_DIVERGED ||
+                        bst == CD3D_NONFINITE) ? (cd3d_status)st[q].status : bst;
+      res.iterations = st[q].iterations;
+      res.rel_residual = st[q].r0 > 0 ? st[q].rfinal / st[q].r0 : 0.0;
+      if (robust && s1 == CD3D_NOT_CONVERGED && st[q].rfinal > 0) {
+        std::vector<double> x(res.n_dof);
+        s1 = cd3d_get_solution(h, x.data());
+        cd3d_solver_params sp2 = r.solver;
+        sp2.smoother = 13;
+        sp2.omega = 1.0;
+        sp2.pre = sp2.post = 2;
+        sp2.maxit = r.solver.maxit - st[q].iterations;
+        if (r.solver.rel_tol > 0) sp2.rel_tol = r.solver.rel_tol * st[q].r0 / st[q].rfinal;
+        cd3d_stats s2{};
+        if (s1 == CD3D_OK) s1 = cd3d_solve(h, &sp2, nullptr, x.data(), 1, &s2);
+        res.iterations += s2.iterations;
+        res.rel_residual = st[q].r0 > 0 ? s2.rfinal / st[q].r0 : 0.0;
+        res.escalated = 1;
+      }
+      if ((s1 == CD3D_OK || s1 == CD3D_NOT_CONVERGED) && r.want_solution) {
+        ds[k].sol.resize(res.n_dof);
+        const cd3d_status g = cd3d_get_solution(h, ds[k].sol.data());
+        if (g != CD3D_OK) s1 = g;
+      }
+      res.status = s1;
+    }
+    for (auto &d : ds) d.res.batch_size = n;
+  }
+
   void work_loop() {
     if (!prm.simulate) (void)hipSetDevice(device);
     std::unique_lock<std::mutex> lk(mu);
     while (true) {
       cv_work.wait(lk, [&] { return stop || !ready.empty(); });
       if (ready.empty()) return;  // stop, and nothing admitted is left
-      Work w = ready.front();
+      std::vector<Work> ws{ready.front()};
       ready.pop_front();
+      if (batchable(ws[0].p.req))
+        for (auto it = ready.begin(); it != ready.end() && (int)ws.size() < prm.max_batch;) {
+          if (batchable(it->p.req) && same_batch(ws[0].p.req, it->p.req)) {
+            ws.push_back(*it);
+            it = ready.erase(it);
+          } else {
+            ++it;
+          }
+        }
       lk.unlock();
-      Done d;
-      run_request(w, d);
+      std::vector<Done> ds(ws.size());
+      if (ws.size() == 1) {
+        run_request(ws[0], ds[0]);
+        ds[0].res.batch_size = 1;
+      }
+      else run_batch(ws, ds);
       const double t_end = now_ms();
-      cd3d_problem *destroy = nullptr;
       lk.lock();
+      for (size_t k = 0; k < ws.size(); k++) finish(ws[k], ds[k], t_end, lk);
+    }
+  }
+
+  // bookkeeping of one finished request (called with mu held; may drop it
+  // to destroy a hierarchy)
+  void finish(Work &w, Done &d, double t_end, std::unique_lock<std::mutex> &lk) {
+    {
+      cd3d_problem *destroy = nullptr;
       Resident *slot = w.slot;
       size_t actual = slot->h ? cd3d_device_bytes(slot->h) : (prm.simulate ? slot->bytes : 0);
       bytes = bytes - slot->bytes + actual;
@@ -320,10 +417,12 @@ void cd3d_server_default_params(cd3d_server_params *p) {
   p->simulate = 0;
   p->sim_us_per_dof = 0.01;
   p->robust_after = 200;
+  p->max_batch = 8;
 }
 
 cd3d_status cd3d_server_create(const cd3d_server_params *prm, cd3d_server **out) {
-  if (!prm || !out || prm->max_inflight < 1 || prm->workers < 0 || prm->max_bypass < 0) return CD3D_ERR_ARG;
+  if (!prm || !out || prm->max_inflight < 1 || prm->workers < 0 || prm->max_bypass < 0 || prm->max_batch < 1)
+    return CD3D_ERR_ARG;
   *out = nullptr;
   cd3d_server *s = new (std::nothrow) cd3d_server();
   if (!s) return CD3D_ERR_NOMEM;

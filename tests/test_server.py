@@ -140,7 +140,7 @@ def test_results_are_the_same_alone_and_in_the_mix():
         alone.append(srv.wait(i, n_dof=n_dof(c)))
         srv.close()
     say("[gpu] server mix")
-    srv = pm.Server(max_inflight=4)
+    srv = pm.Server(max_inflight=4, max_batch=1)  # batches agree with lone solves to rounding only
     ids = submit_mix(srv) + submit_mix(srv)  # the second round reuses resident hierarchies
     mix = [srv.wait(i, n_dof=n_dof(MIX[k % len(MIX)])) for k, i in enumerate(ids)]
     st = srv.stats()
@@ -153,6 +153,30 @@ def test_results_are_the_same_alone_and_in_the_mix():
         assert r.iterations == ra.iterations
         assert np.array_equal(x, xa), k
     assert sum(r.reused_hierarchy for r, _ in mix) >= 1
+
+
+@pytest.mark.gpu
+def test_batched_requests_agree_with_lone_solves():
+    # CD requests with the same number of levels and FP64 solver parameters,
+    # admitted together, run as one batch (cd3d_solve_batch): the same
+    # iteration counts as alone and solutions equal to rounding
+    cases = [c for c in MIX if not c.get("tcd") and not c.get("mixed") and c["mesh"][3] == 3]
+    alone = []
+    for c in cases:
+        P = pm.Problem(*c["mesh"])
+        P.assemble(pm.coeffs(**c["co"]))
+        alone.append(P.solve(solver()))
+        P.close()
+    srv = pm.Server(max_inflight=8, workers=1, max_batch=8)
+    ids = [srv.submit(pm.mesh_desc(*c["mesh"]), pm.coeffs(**c["co"]), solver(), want_solution=True) for c in cases]
+    out = [srv.wait(i, n_dof=n_dof(c)) for c, i in zip(cases, ids)]
+    srv.close()
+    say(f"[gpu] batched server requests: sizes {[r.batch_size for r, _ in out]}, "
+        f"its {[r.iterations for r, _ in out]} (alone {[s.iterations for _, s in alone]})")
+    assert max(r.batch_size for r, _ in out) >= 2
+    for (r, x), (xa, sa) in zip(out, alone):
+        assert r.status == 0 and r.iterations == sa.iterations
+        assert np.linalg.norm(x - xa) <= 1e-12 * np.linalg.norm(xa)
 
 
 @pytest.mark.gpu
