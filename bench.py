@@ -290,7 +290,7 @@ def failure_record(req, r):
                 rel_residual=float(r.rel_residual), steps_done=int(r.steps_done))
 
 
-def batched_uneven(pm, precision, count=256, seed=20261202, passes=2, inflight=16, max_batch=None):
+def batched_uneven(pm, precision, count=256, seed=20261202, passes=2, inflight=16, max_batch=None, workers=0):
     """Configuration 4: `count` independent Q1 SUPG problems, n log-uniform in
     [8, 48] (snapped to m 2^k), eps log-uniform in [1e-6, 1], b a random unit
     direction times U(0, 1), f = 1; all submitted at once to the server.  The
@@ -298,7 +298,7 @@ def batched_uneven(pm, precision, count=256, seed=20261202, passes=2, inflight=1
     solves/s counts converged requests only."""
     reqs = c4_requests(count, seed)
     sp = pm.solver_params(**dict(SOLVER, maxit=1000), precision=precision)
-    srv = pm.Server(max_inflight=inflight, max_batch=max_batch)
+    srv = pm.Server(max_inflight=inflight, max_batch=max_batch, workers=workers)
     out = None
     for _ in range(passes):
         t = time.perf_counter()
@@ -319,6 +319,17 @@ def batched_uneven(pm, precision, count=256, seed=20261202, passes=2, inflight=1
     out["peak_device_gib"] = st.peak_bytes / 2**30
     srv.close()
     return out
+
+
+def c4_form(pm, precision, args):
+    """Configuration 4 in one precision form: the FP64 form runs same-level
+    batches (all problems with the same level count and solver in lockstep,
+    cd3d_solve_batch: 256 slots, 4 worker threads, up to --c4-batch per
+    batch); the mixed form, which batches do not cover, one request per
+    slot."""
+    if precision == pm.FP64 and args.c4_batch > 1:
+        return batched_uneven(pm, precision, args.c4_problems, inflight=256, max_batch=args.c4_batch, workers=4)
+    return batched_uneven(pm, precision, args.c4_problems, inflight=args.c4_inflight, max_batch=1)
 
 
 def serving_stream(pm, precision, rates, duration_s=4.0, seed=7, tcd_share=0.2, max_requests=200):
@@ -392,7 +403,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="only the headline configuration")
     ap.add_argument("--c3-steps", type=int, default=100)
     ap.add_argument("--c4-problems", type=int, default=256)
-    ap.add_argument("--c4-inflight", type=int, default=16, help="server slots for configuration 4")
+    ap.add_argument("--c4-inflight", type=int, default=16, help="server slots for configuration 4 (unbatched)")
+    ap.add_argument("--c4-batch", type=int, default=64,
+                    help="configuration 4, FP64 form: requests per same-level batch (with 256 slots, 4 workers)")
     ap.add_argument("--c5-rates", type=str, default="10,50,400")
     ap.add_argument("--c5-seconds", type=float, default=4.0)
     ap.add_argument("--precision", choices=["both", "mixed", "fp64"], default="both",
@@ -415,7 +428,7 @@ def main():
             if args.only == "tcd":
                 out[name] = tcd_run(pm, prec, args.c3_steps)[0]
             elif args.only == "batched":
-                out[name] = batched_uneven(pm, prec, args.c4_problems, inflight=args.c4_inflight)
+                out[name] = c4_form(pm, prec, args)
             else:
                 out[name] = serving_stream(pm, prec, [float(x) for x in args.c5_rates.split(",") if x],
                                            args.c5_seconds)
@@ -474,8 +487,10 @@ def main():
             np.linalg.norm(u["mixed"] - u["fp64"]) / np.linalg.norm(u["fp64"]))
         for name, prec in (("fp64", pm.FP64), ("mixed", pm.MIXED)):
             log(f"[bench] configuration 4 ({args.c4_problems} batched uneven Q1 problems), {name}")
-            extra.setdefault("batched_uneven_q1", {})[name] = batched_uneven(pm, prec, args.c4_problems,
-                                                                             inflight=args.c4_inflight)
+            extra.setdefault("batched_uneven_q1", {})[name] = c4_form(pm, prec, args)
+        log("[bench] configuration 4, FP64 one request at a time (no batching)")
+        extra["batched_uneven_q1"]["fp64_unbatched"] = batched_uneven(pm, pm.FP64, args.c4_problems,
+                                                                      inflight=args.c4_inflight, max_batch=1)
         rates = [float(x) for x in args.c5_rates.split(",") if x]
         for name, prec in (("fp64", pm.FP64), ("mixed", pm.MIXED)):
             log(f"[bench] configuration 5 (Poisson serving stream), {name}")

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -271,7 +272,9 @@ struct cd3d_server {
     cd3d_solver_params sp = ws[0].p.req.solver;
     const bool robust = prm.robust_after > 0 && sp.maxit > prm.robust_after;
     if (robust) sp.maxit = prm.robust_after;
-    for (int k = 0; k < n; k++) {
+    // hierarchies and assembly of the members, on up to 8 host threads (each
+    // member has its own stream; assembly ends with a synchronisation)
+    auto prepare = [&](int k) {
       const cd3d_request &r = ws[k].p.req;
       ds[k].res.n_dof = (int)finest_dofs(r.mesh);
       cd3d_problem *&h = ws[k].slot->h;
@@ -283,11 +286,27 @@ struct cd3d_server {
       }
       if (st == CD3D_OK) st = cd3d_assemble(h, &r.coeffs);
       ds[k].res.status = st;
-      if (st == CD3D_OK) {
-        hs.push_back(h);
+    };
+    {
+      std::atomic<int> next{0};
+      std::vector<std::thread> th;
+      const int nt = std::min(n, 8) - 1;  // the worker itself takes part
+      auto drain = [&] {
+        for (int k; (k = next++) < n;) prepare(k);
+      };
+      for (int t = 0; t < nt; t++)
+        th.emplace_back([&] {
+          (void)hipSetDevice(device);
+          drain();
+        });
+      drain();
+      for (auto &x : th) x.join();
+    }
+    for (int k = 0; k < n; k++)
+      if (ds[k].res.status == CD3D_OK) {
+        hs.push_back(ws[k].slot->h);
         idx.push_back(k);
       }
-    }
     if (hs.empty()) return;
     std::vector<cd3d_stats> st(hs.size());
     const cd3d_status bst = cd3d_solve_batch(hs.data(), (int)hs.size(), &sp, st.data());
