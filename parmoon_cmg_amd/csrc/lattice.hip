@@ -521,6 +521,116 @@ __global__ __launch_bounds__(kB) void lat_pair_gk(LatArgs a, const float *__rest
   if (k < a.c[2 * pc + 1].nodes / G) lat_pair_group_node<G, MODE>(a, pc, k, cs, cs + kCst, x, f, out, omega);
 }
 
+// Q2 compact-stencil sweep by quads: one thread takes the same X segment (G
+// vertex/midpoint pairs) of the four lines (Y0 + oy, Z0 + oz), oy, oz in
+// {0, 1}, Y0 = 2 by, Z0 = 2 bz -- one line of each pair class.  The
+// neighbour lines of the four together are the 5 x 5 lines Y0 - 2 .. Y0 + 2,
+// Z0 - 2 .. Z0 + 2 (a midpoint line's stencil reaches one line either side):
+// each is loaded once and feeds every class whose stencil covers it, 4x
+// fewer gathers than a thread per pair-class line.  Each node's sum runs
+// over its stencil in the order of lat_pair_group, so the results are
+// bitwise the same.  The even lines Y0 = 0 / Z0 = 0 are boundary lines: not
+// written, and the lines below them are read clamped to the boundary.
+template <int G, int MODE, int TB>
+__global__ __launch_bounds__(TB) void lat_quad_gk(LatArgs a, const float *__restrict__ cst,
+                                                  const float *__restrict__ x, const float *__restrict__ f,
+                                                  float *__restrict__ out, float omega, int xcd) {
+  __shared__ float cs[8 * kCst];
+  for (int i = threadIdx.x; i < 8 * kCst; i += TB) cs[i] = cst[i];
+  __syncthreads();
+  const int n = a.c[1].cnt[0];  // pairs per line (and blocks per axis)
+  const int ng = n / G;
+  const int bid = xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const long long k = (long long)bid * TB + threadIdx.x;
+  if (k >= (long long)ng * n * n) return;
+  const int g = (int)(k % ng);
+  const long long r = k / ng;
+  const int by = (int)(r % n), bz = (int)(r / n);
+  const long long nn = a.nn;
+  const int Y0 = 2 * by, Z0 = 2 * bz;
+  const long long base = 2LL * G * g + nn * (Y0 + nn * Z0);
+  const bool hv = g > 0;
+  float s0[4][G], s1[4][G], xs[4][2 * G];
+  if (MODE != kLJacobiZero) {
+#pragma unroll
+    for (int pc = 0; pc < 4; pc++)
+#pragma unroll
+      for (int q = 0; q < G; q++) s0[pc][q] = s1[pc][q] = 0.0f;
+    const int lft_off = hv ? -2 : 0;
+#pragma unroll 1
+    for (int dz = -2; dz <= 2; dz++) {
+      const int zz = Z0 + dz < 0 ? 0 : Z0 + dz;
+#pragma unroll
+      for (int dy = -2; dy <= 2; dy++) {
+        const int yy = Y0 + dy < 0 ? 0 : Y0 + dy;
+        const float *xl = x + 2LL * G * g + nn * (yy + nn * zz);
+        float w[2 * G + 3];
+        const f2u lft = *(const f2u *)(xl + lft_off);
+        w[0] = lft.x;
+        w[1] = lft.y;
+#pragma unroll
+        for (int h = 0; h < G / 2; h++) {
+          const f4u m = *(const f4u *)(xl + 4 * h);
+#pragma unroll
+          for (int e = 0; e < 4; e++) w[2 + 4 * h + e] = m[e];
+        }
+        w[2 * G + 2] = xl[2 * G];
+#pragma unroll
+        for (int oz = 0; oz < 2; oz++) {
+          const int dzr = dz - oz, loz = oz ? -1 : -2, hiz = oz ? 1 : 2;
+          if (dzr < loz || dzr > hiz) continue;
+#pragma unroll
+          for (int oy = 0; oy < 2; oy++) {
+            const int dyr = dy - oy, loy = oy ? -1 : -2, hiy = oy ? 1 : 2;
+            if (dyr < loy || dyr > hiy) continue;
+            const int pc = oy + 2 * oz, ny = hiy - loy + 1;
+            const int li = (dzr - loz) * ny + (dyr - loy);
+            const float *cs0 = cs + 2 * pc * kCst, *cs1 = cs0 + kCst;
+            if (dyr == 0 && dzr == 0) {
+#pragma unroll
+              for (int j = 0; j < 2 * G; j++) xs[pc][j] = w[2 + j];
+            }
+#pragma unroll
+            for (int q = 0; q < G; q++) {
+#pragma unroll
+              for (int j = 0; j < 5; j++) s0[pc][q] += cs0[li * 5 + j] * w[2 * q + j];
+#pragma unroll
+              for (int j = 0; j < 3; j++) s1[pc][q] += cs1[li * 3 + j] * w[2 * q + 2 + j];
+            }
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int pc = 0; pc < 4; pc++) {
+    const int oy = pc & 1, oz = pc >> 1;
+    if ((oy == 0 && by == 0) || (oz == 0 && bz == 0)) continue;  // a boundary line
+    const long long nv = base + nn * (oy + nn * oz);
+    const float d0 = cs[2 * pc * kCst + kCst - 1], d1 = cs[(2 * pc + 1) * kCst + kCst - 1];
+    float fv[2 * G];
+#pragma unroll
+    for (int h = 0; h < G / 2; h++) {
+      const f4u t = *(const f4u *)(f + nv + 4 * h);
+#pragma unroll
+      for (int e = 0; e < 4; e++) fv[4 * h + e] = t[e];
+    }
+#pragma unroll
+    for (int q = 0; q < G; q++) {
+      if (MODE == kLJacobiZero) {
+        if (q > 0 || hv) out[nv + 2 * q] = omega * fv[2 * q] * d0;
+        out[nv + 2 * q + 1] = omega * fv[2 * q + 1] * d1;
+      } else if (MODE == kLDefect) {
+        if (q > 0 || hv) out[nv + 2 * q] = fv[2 * q] - s0[pc][q];
+        out[nv + 2 * q + 1] = fv[2 * q + 1] - s1[pc][q];
+      } else {
+        if (q > 0 || hv) out[nv + 2 * q] = xs[pc][2 * q] + omega * (fv[2 * q] - s0[pc][q]) * d0;
+        out[nv + 2 * q + 1] = xs[pc][2 * q + 1] + omega * (fv[2 * q + 1] - s1[pc][q]) * d1;
+      }
+    }
+  }
+}
+
 template <int P, int MODE>
 __global__ __launch_bounds__(kB) void lat_sweep_k(LatArgs a, const float *__restrict__ val,
                                                   const float *__restrict__ inv, const float *__restrict__ cst,
@@ -1616,6 +1726,13 @@ static bool lat_pairs() {
   return v;
 }
 
+// CD3D_LAT_QUADS=0: the grouped Q2 FP32 sweeps by pair class (lat_pair_gk)
+// instead of by quads of lines (lat_quad_gk); read at every launch
+static bool lat_quads() {
+  const char *e = getenv("CD3D_LAT_QUADS");
+  return !(e && atoi(e) == 0);
+}
+
 // Pairs per thread of the compact-stencil Q2 kernels: groups of gmax where
 // that still leaves kGroupMinThreads threads, else single pairs.  Measured on
 // the 129^3-node level of a 64^3-cell Q2 mesh: FP32 sweep 22.8 / 19.0 /
@@ -1677,7 +1794,25 @@ void lat_sweep(hipStream_t s, const DevLevel &L, int order, const float *x, cons
   }
   const int G = order == 2 && lat_pairs() && L.nn > kPairMinNN && all_uni(a) ? pair_group(a, 4) : 1;
   if (order == 1) lat_sweep_k<1, MODE><<<grid, kB, 0, s>>>(a, L.lval, L.linv, L.lcst, x, f, out, omega);
-  else if (G > 1) {
+  else if (G > 1 && lat_quads()) {
+    // measured on the 129^3-node level of a 64^3-cell Q2 mesh (FP32 Jacobi
+    // sweep): 2 pairs per thread, 256 threads, XCD-contiguous blocks 12.4 us;
+    // without the XCD mapping 13.6; 4 pairs 13.4 / 15.1; 128 threads
+    // 12.6 / 15.0; the pair-class kernel it replaces 18.7 (CD3D_QUAD_G,
+    // CD3D_QUAD_TB, CD3D_QUAD_XCD select the variants)
+    const char *eg = getenv("CD3D_QUAD_G"), *et = getenv("CD3D_QUAD_TB"), *ex = getenv("CD3D_QUAD_XCD");
+    const int gq_g = eg ? atoi(eg) : 2, tb = et ? atoi(et) : 256, xcd = ex ? atoi(ex) : 1;
+    const int Gq = (gq_g == 4 && G == 4) ? 4 : 2;  // G == 4 implies 2 divides the pairs per line too
+    const long long n = a.c[1].cnt[0], threads = n / Gq * n * n;
+    const int gq = (int)((threads + tb - 1) / tb);
+    if (tb == 128) {
+      if (Gq == 2) lat_quad_gk<2, MODE, 128><<<gq, 128, 0, s>>>(a, L.lcst, x, f, out, omega, xcd);
+      else lat_quad_gk<4, MODE, 128><<<gq, 128, 0, s>>>(a, L.lcst, x, f, out, omega, xcd);
+    } else {
+      if (Gq == 2) lat_quad_gk<2, MODE, 256><<<(int)((threads + 255) / 256), 256, 0, s>>>(a, L.lcst, x, f, out, omega, xcd);
+      else lat_quad_gk<4, MODE, 256><<<(int)((threads + 255) / 256), 256, 0, s>>>(a, L.lcst, x, f, out, omega, xcd);
+    }
+  } else if (G > 1) {
     const LatArgs ag = group_args(a, G);
     if (G == 2) lat_pair_gk<MODE, 2><<<ag.pstart[4], kB, 0, s>>>(ag, L.lcst, x, f, out, omega);
     else lat_pair_gk<MODE, 4><<<ag.pstart[4], kB, 0, s>>>(ag, L.lcst, x, f, out, omega);
