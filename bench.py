@@ -47,7 +47,7 @@ def log(*a):
 
 KERNEL_NAMES = {0: "csr_stream<kSpmv,f64>",
                 1: "lat_quad_gk<2,kLJacobi> f32 (compact stencils, 2 vertex/midpoint pairs on 4 lines per thread)",
-                2: "lat64_pair_gk<kL64Spmv,2> f64 (2 vertex/midpoint pairs per thread)",
+                2: "lat64_quad_gk<1,kL64Spmv> f64 (1 vertex/midpoint pair on 4 lines per thread)",
                 3: "lat_restrict_pair_k f32 (coarse vertex/midpoint pairs)",
                 4: "lat_prolong_k<f32>", 5: "csr_fill_k",
                 6: "copy_k (f64 vector copy, traffic calibration)",
@@ -78,7 +78,7 @@ def kernel_table(P, reps=50):
 # runs (FETCH_SIZE x 2.0 + WRITE_SIZE, calibrated on a copy kernel)
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "cd3d_q2_64_fine_level_traffic.json")
 TRAFFIC_KERNEL = {0: "void csr_stream<0, double, double, true>", 1: "void lat_quad_gk<2, 0, 256>",
-                  2: "void lat64_pair_gk<0, 2>", 3: "lat_restrict_pair_k", 4: "void lat_prolong_k<2>",
+                  2: "void lat64_quad_gk<1, 0>", 3: "lat_restrict_pair_k", 4: "void lat_prolong_k<2>",
                   5: "csr_fill2_k", 6: "copy_k", 7: "void axpy_dot2_k<true>", 8: "void assemble_k<2>"}
 
 

@@ -531,6 +531,88 @@ __global__ __launch_bounds__(kB) void lat_pair_gk(LatArgs a, const float *__rest
 // over its stencil in the order of lat_pair_group, so the results are
 // bitwise the same.  The even lines Y0 = 0 / Z0 = 0 are boundary lines: not
 // written, and the lines below them are read clamped to the boundary.
+// position of quad thread k: segment g (G pairs) of the lines of block (by, bz)
+struct QuadPos {
+  int g, by, bz;
+  long long base;  // node (2 G g, 2 by, 2 bz)
+  bool hv;         // the segment's first vertex is interior (g > 0)
+};
+template <int G>
+__host__ __device__ inline bool quad_pos(const LatArgs &a, long long k, QuadPos &p) {
+  const int n = a.c[1].cnt[0];  // pairs per line (and blocks per axis)
+  const int ng = n / G;
+  if (k >= (long long)ng * n * n) return false;
+  p.g = (int)(k % ng);
+  const long long r = k / ng;
+  p.by = (int)(r % n);
+  p.bz = (int)(r / n);
+  p.base = 2LL * G * p.g + (long long)a.nn * (2 * p.by + (long long)a.nn * 2 * p.bz);
+  p.hv = p.g > 0;
+  return true;
+}
+
+// the stencil sums of the 4 lines of a quad (cs: the 8 classes' compact
+// stencils, class c at cs + c kCst; xs: the lines' own x values)
+template <typename T, int G>
+__host__ __device__ inline void quad_sums(const LatArgs &a, const QuadPos &p, const T *cs,
+                                          const T *__restrict__ x, T (&s0)[4][G], T (&s1)[4][G],
+                                          T (&xs)[4][2 * G]) {
+  typedef typename Vec2<T>::type V2;
+  typedef typename Vec16<T>::type V16;
+  constexpr int vn = Vec16<T>::n;
+  const long long nn = a.nn;
+  const int Y0 = 2 * p.by, Z0 = 2 * p.bz;
+#pragma unroll
+  for (int pc = 0; pc < 4; pc++)
+#pragma unroll
+    for (int q = 0; q < G; q++) s0[pc][q] = s1[pc][q] = T(0);
+  const int lft_off = p.hv ? -2 : 0;
+#pragma unroll 1
+  for (int dz = -2; dz <= 2; dz++) {
+    const int zz = Z0 + dz < 0 ? 0 : Z0 + dz;
+#pragma unroll
+    for (int dy = -2; dy <= 2; dy++) {
+      const int yy = Y0 + dy < 0 ? 0 : Y0 + dy;
+      const T *xl = x + 2LL * G * p.g + nn * (yy + nn * zz);
+      T w[2 * G + 3];
+      const V2 lft = *(const V2 *)(xl + lft_off);
+      w[0] = lft.x;
+      w[1] = lft.y;
+#pragma unroll
+      for (int h = 0; h < 2 * G / vn; h++) {
+        const V16 m = *(const V16 *)(xl + vn * h);
+#pragma unroll
+        for (int e = 0; e < vn; e++) w[2 + vn * h + e] = m[e];
+      }
+      w[2 * G + 2] = xl[2 * G];
+#pragma unroll
+      for (int oz = 0; oz < 2; oz++) {
+        const int dzr = dz - oz, loz = oz ? -1 : -2, hiz = oz ? 1 : 2;
+        if (dzr < loz || dzr > hiz) continue;
+#pragma unroll
+        for (int oy = 0; oy < 2; oy++) {
+          const int dyr = dy - oy, loy = oy ? -1 : -2, hiy = oy ? 1 : 2;
+          if (dyr < loy || dyr > hiy) continue;
+          const int pc = oy + 2 * oz, ny = hiy - loy + 1;
+          const int li = (dzr - loz) * ny + (dyr - loy);
+          const T *cs0 = cs + 2 * pc * kCst, *cs1 = cs0 + kCst;
+          if (dyr == 0 && dzr == 0) {
+#pragma unroll
+            for (int j = 0; j < 2 * G; j++) xs[pc][j] = w[2 + j];
+          }
+#pragma unroll
+          for (int q = 0; q < G; q++) {
+#pragma unroll
+            for (int j = 0; j < 5; j++) s0[pc][q] += cs0[li * 5 + j] * w[2 * q + j];
+#pragma unroll
+            for (int j = 0; j < 3; j++) s1[pc][q] += cs1[li * 3 + j] * w[2 * q + 2 + j];
+          }
+        }
+      }
+    }
+  }
+}
+
 template <int G, int MODE, int TB>
 __global__ __launch_bounds__(TB) void lat_quad_gk(LatArgs a, const float *__restrict__ cst,
                                                   const float *__restrict__ x, const float *__restrict__ f,
@@ -538,75 +620,17 @@ __global__ __launch_bounds__(TB) void lat_quad_gk(LatArgs a, const float *__rest
   __shared__ float cs[8 * kCst];
   for (int i = threadIdx.x; i < 8 * kCst; i += TB) cs[i] = cst[i];
   __syncthreads();
-  const int n = a.c[1].cnt[0];  // pairs per line (and blocks per axis)
-  const int ng = n / G;
   const int bid = xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
-  const long long k = (long long)bid * TB + threadIdx.x;
-  if (k >= (long long)ng * n * n) return;
-  const int g = (int)(k % ng);
-  const long long r = k / ng;
-  const int by = (int)(r % n), bz = (int)(r / n);
+  QuadPos p;
+  if (!quad_pos<G>(a, (long long)bid * TB + threadIdx.x, p)) return;
   const long long nn = a.nn;
-  const int Y0 = 2 * by, Z0 = 2 * bz;
-  const long long base = 2LL * G * g + nn * (Y0 + nn * Z0);
-  const bool hv = g > 0;
   float s0[4][G], s1[4][G], xs[4][2 * G];
-  if (MODE != kLJacobiZero) {
-#pragma unroll
-    for (int pc = 0; pc < 4; pc++)
-#pragma unroll
-      for (int q = 0; q < G; q++) s0[pc][q] = s1[pc][q] = 0.0f;
-    const int lft_off = hv ? -2 : 0;
-#pragma unroll 1
-    for (int dz = -2; dz <= 2; dz++) {
-      const int zz = Z0 + dz < 0 ? 0 : Z0 + dz;
-#pragma unroll
-      for (int dy = -2; dy <= 2; dy++) {
-        const int yy = Y0 + dy < 0 ? 0 : Y0 + dy;
-        const float *xl = x + 2LL * G * g + nn * (yy + nn * zz);
-        float w[2 * G + 3];
-        const f2u lft = *(const f2u *)(xl + lft_off);
-        w[0] = lft.x;
-        w[1] = lft.y;
-#pragma unroll
-        for (int h = 0; h < G / 2; h++) {
-          const f4u m = *(const f4u *)(xl + 4 * h);
-#pragma unroll
-          for (int e = 0; e < 4; e++) w[2 + 4 * h + e] = m[e];
-        }
-        w[2 * G + 2] = xl[2 * G];
-#pragma unroll
-        for (int oz = 0; oz < 2; oz++) {
-          const int dzr = dz - oz, loz = oz ? -1 : -2, hiz = oz ? 1 : 2;
-          if (dzr < loz || dzr > hiz) continue;
-#pragma unroll
-          for (int oy = 0; oy < 2; oy++) {
-            const int dyr = dy - oy, loy = oy ? -1 : -2, hiy = oy ? 1 : 2;
-            if (dyr < loy || dyr > hiy) continue;
-            const int pc = oy + 2 * oz, ny = hiy - loy + 1;
-            const int li = (dzr - loz) * ny + (dyr - loy);
-            const float *cs0 = cs + 2 * pc * kCst, *cs1 = cs0 + kCst;
-            if (dyr == 0 && dzr == 0) {
-#pragma unroll
-              for (int j = 0; j < 2 * G; j++) xs[pc][j] = w[2 + j];
-            }
-#pragma unroll
-            for (int q = 0; q < G; q++) {
-#pragma unroll
-              for (int j = 0; j < 5; j++) s0[pc][q] += cs0[li * 5 + j] * w[2 * q + j];
-#pragma unroll
-              for (int j = 0; j < 3; j++) s1[pc][q] += cs1[li * 3 + j] * w[2 * q + 2 + j];
-            }
-          }
-        }
-      }
-    }
-  }
+  if (MODE != kLJacobiZero) quad_sums<float, G>(a, p, cs, x, s0, s1, xs);
 #pragma unroll
   for (int pc = 0; pc < 4; pc++) {
     const int oy = pc & 1, oz = pc >> 1;
-    if ((oy == 0 && by == 0) || (oz == 0 && bz == 0)) continue;  // a boundary line
-    const long long nv = base + nn * (oy + nn * oz);
+    if ((oy == 0 && p.by == 0) || (oz == 0 && p.bz == 0)) continue;  // a boundary line
+    const long long nv = p.base + nn * (oy + nn * oz);
     const float d0 = cs[2 * pc * kCst + kCst - 1], d1 = cs[(2 * pc + 1) * kCst + kCst - 1];
     float fv[2 * G];
 #pragma unroll
@@ -618,13 +642,13 @@ __global__ __launch_bounds__(TB) void lat_quad_gk(LatArgs a, const float *__rest
 #pragma unroll
     for (int q = 0; q < G; q++) {
       if (MODE == kLJacobiZero) {
-        if (q > 0 || hv) out[nv + 2 * q] = omega * fv[2 * q] * d0;
+        if (q > 0 || p.hv) out[nv + 2 * q] = omega * fv[2 * q] * d0;
         out[nv + 2 * q + 1] = omega * fv[2 * q + 1] * d1;
       } else if (MODE == kLDefect) {
-        if (q > 0 || hv) out[nv + 2 * q] = fv[2 * q] - s0[pc][q];
+        if (q > 0 || p.hv) out[nv + 2 * q] = fv[2 * q] - s0[pc][q];
         out[nv + 2 * q + 1] = fv[2 * q + 1] - s1[pc][q];
       } else {
-        if (q > 0 || hv) out[nv + 2 * q] = xs[pc][2 * q] + omega * (fv[2 * q] - s0[pc][q]) * d0;
+        if (q > 0 || p.hv) out[nv + 2 * q] = xs[pc][2 * q] + omega * (fv[2 * q] - s0[pc][q]) * d0;
         out[nv + 2 * q + 1] = xs[pc][2 * q + 1] + omega * (fv[2 * q + 1] - s1[pc][q]) * d1;
       }
     }
@@ -1565,6 +1589,46 @@ __global__ __launch_bounds__(kB) void lat64_pair_gk(LatArgs a, const double *__r
   if (k < a.c[2 * pc + 1].nodes / G) lat64_pair_group_node<G, MODE>(a, pc, k, cs, cs + kCst, x, f, out);
 }
 
+// The FP64 lattice operator by quads of lines (quad_sums over the FP64
+// compact stencils); blocks from nquad on take the Dirichlet rows (identity).
+// Per node the same sums as lat64_pair_group.
+template <int G, int MODE>
+__global__ __launch_bounds__(kB) void lat64_quad_gk(LatArgs a, const double *__restrict__ st64,
+                                                    const double *__restrict__ x, const double *__restrict__ f,
+                                                    double *__restrict__ out, int nquad, int nact, int ndof,
+                                                    const int *__restrict__ dof2node, int xcd) {
+  __shared__ double cs[8 * kCst];
+  const int bid0 = (int)blockIdx.x;
+  if (bid0 >= nquad) {
+    const int d = nact + (bid0 - nquad) * kB + threadIdx.x;
+    if (d < ndof) {
+      const int node = dof2node[d];
+      out[node] = MODE == kL64Defect ? f[node] - x[node] : x[node];
+    }
+    return;
+  }
+  for (int i = threadIdx.x; i < 8 * kCst; i += kB) cs[i] = st64[i];
+  __syncthreads();
+  const int bid = xcd ? xcd_block(bid0, nquad) : bid0;
+  QuadPos p;
+  if (!quad_pos<G>(a, (long long)bid * kB + threadIdx.x, p)) return;
+  const long long nn = a.nn;
+  double s0[4][G], s1[4][G], xs[4][2 * G];
+  quad_sums<double, G>(a, p, cs, x, s0, s1, xs);
+#pragma unroll
+  for (int pc = 0; pc < 4; pc++) {
+    const int oy = pc & 1, oz = pc >> 1;
+    if ((oy == 0 && p.by == 0) || (oz == 0 && p.bz == 0)) continue;
+    const long long nv = p.base + nn * (oy + nn * oz);
+#pragma unroll
+    for (int q = 0; q < G; q++) {
+      const long long v = nv + 2 * q;
+      if (q > 0 || p.hv) out[v] = MODE == kL64Defect ? f[v] - s0[pc][q] : s0[pc][q];
+      out[v + 1] = MODE == kL64Defect ? f[v + 1] - s1[pc][q] : s1[pc][q];
+    }
+  }
+}
+
 // FP64 operator of a rotating-field level: entry s of a node (X, Y) is
 // alpha (S0 + X SX + Y SY) (conv-diff alone, alpha 1) or
 // fma(alpha, S0 + X SX + Y SY, M) (the TCD3D system matrix M + tau th1 A),
@@ -1985,6 +2049,20 @@ void lat64_apply(hipStream_t s, const DevLevel &L, int order, const double *x, c
   }
   if (order == 2 && grid > 0 && lat_pairs() && L.nn > kPairMinNN) {  // interior and Dirichlet rows in one launch
     const int G = pair_group(a, 2);
+    if (G > 1 && lat_quads()) {
+      // measured on the 129^3-node level (FP64 product): quads of one pair
+      // 21.1 us (XCD mapping or not), of two pairs 22.5 (256 VGPRs, one wave
+      // per SIMD), the pair-class kernel 27.0
+      const char *ex = getenv("CD3D_QUAD_XCD"), *eg = getenv("CD3D_QUAD64_G");
+      const int xcd = ex ? atoi(ex) : 1, gq = eg ? atoi(eg) : 1;
+      const long long n = a.c[1].cnt[0];
+      const int nq = (int)((n / gq * n * n + kB - 1) / kB), ng = nq + (nd + kB - 1) / kB;
+      if (gq == 2)
+        lat64_quad_gk<2, MODE><<<ng, kB, 0, s>>>(a, L.lst64, x, f, out, nq, L.nact, L.ndof, L.ldof2node, xcd);
+      else
+        lat64_quad_gk<1, MODE><<<ng, kB, 0, s>>>(a, L.lst64, x, f, out, nq, L.nact, L.ndof, L.ldof2node, xcd);
+      return;
+    }
     if (G > 1) {
       const LatArgs ag = group_args(a, G);
       const int g = ag.pstart[4] + (nd + kB - 1) / kB;
