@@ -52,7 +52,7 @@ KERNEL_NAMES = {0: "csr_stream<kSpmv,f64>",
                 4: "lat_prolong_k<f32>", 5: "csr_fill_k",
                 6: "copy_k (f64 vector copy, traffic calibration)",
                 7: "axpy_dot2_k f64 (fused Gram-Schmidt step, 16-byte pairs)",
-                8: "assemble_k<2> f64 (cell-by-cell SUPG element kernel, 8 colour launches)"}
+                8: "assemble_k<2> + gather_entries_k f64 (cell-by-cell SUPG element rows, then CSR entries gathered)"}
 FP64_PEAK_TFLOPS = 78.6
 FLOP_KERNELS = {8}
 
@@ -153,7 +153,7 @@ def general_hexahedra(pm, steps, warmup):
         bound="fp64", achieved=round(flops / (ms * 1e-3) / 1e12, 3), peak=FP64_PEAK_TFLOPS, unit="TFLOP/s",
         frac=round(flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4), us_per_launch=round(ms * 1e3, 2),
         algorithmic_flops=int(flops),
-        kernel="assemble_geo_k<2,false> f64 (trilinear map at every quadrature point, 8 colour launches)")
+        kernel="assemble_geo_k<2,false> + gather_entries_k f64 (trilinear map at every quadrature point)")
     P.close()
     return out
 
@@ -469,8 +469,9 @@ def main():
     mgs = roofline_entry(table, 7, "fused Gram-Schmidt step of the outer FGMRES on the fine level (32 bytes per "
                          "unknown); it and the sweep take the largest shares of the mixed step's kernel time")
     asm = flop_roofline_entry(table, 8, "per-element quadrature (Gauss4, 64 points) of the Q2 SUPG element "
-                              "matrices on the fine level, colour-scattered into CSR; algorithmic flops "
-                              "n_cells n_q 8 n_b^2 per assembly (the path for variable coefficients)")
+                              "matrices on the fine level: element rows written out, then every CSR entry "
+                              "gathered from its contributions; algorithmic flops n_cells n_q 8 n_b^2 per "
+                              "assembly (the path for variable coefficients and general hexahedra)")
     spmv = roofline_entry(table, 0, "FP64 CSR SpMV of the fine level (12 nnz + 20 N bytes); the FP64 form's "
                                     "smoother, defect and FGMRES product")
     extra = {}

@@ -369,6 +369,38 @@ static AsmParams asm_params(const cd3d_problem &P, const DevLevel &D, const cd3d
   return ap;
 }
 
+// CD3D_ASM_SCATTER=1: the cell-by-cell assembly scatters element rows into
+// CSR colour by colour (the one-pass form) instead of writing them out and
+// gathering (read at every assembly; the two give the same bits)
+static bool scatter_assembly(const cd3d_problem &P, int l) {
+  const char *e = getenv("CD3D_ASM_SCATTER");
+  if (e && atoi(e) != 0) return true;
+  // the gather lists index the element rows with int32
+  return (int64_t)P.dl[l].ncell * P.nb * P.nb >= ((int64_t)1 << 31);
+}
+
+// element-row buffers and gather lists of level l (first use)
+static void ensure_gather(cd3d_problem &P, int l) {
+  DevLevel &D = P.dl[l];
+  if (D.E) return;
+  std::vector<int> slots, gptr, gidx, rptr, ridx;
+  build_slots(P.hl[l], P.mesh.order, slots);
+  build_gather(P.hl[l], P.mesh.order, slots, gptr, gidx, rptr, ridx);
+  slots.clear();
+  slots.shrink_to_fit();
+  D.gptr = dalloc<int>(P, gptr.size());
+  D.gidx = dalloc<int>(P, gidx.size());
+  D.rptr = dalloc<int>(P, rptr.size());
+  D.ridx = dalloc<int>(P, ridx.size());
+  upload(P, D.gptr, gptr);
+  upload(P, D.gidx, gidx);
+  upload(P, D.rptr, rptr);
+  upload(P, D.ridx, ridx);
+  D.E = dalloc<double>(P, (size_t)D.ncell * P.nb * P.nb);
+  D.Er = dalloc<double>(P, (size_t)D.ncell * P.nb);
+  HIPCHK(hipStreamSynchronize(P.stream));
+}
+
 static void ensure_slots(cd3d_problem &P, int l) {
   DevLevel &D = P.dl[l];
   if (D.slots) return;  // slot maps only for the cell-by-cell path (nb^2 ints per cell)
@@ -412,15 +444,30 @@ static int assemble_level(cd3d_problem &P, int l, const cd3d_coeffs &co, int wha
     HIPCHK(hipGetLastError());
     return 2;
   }
-  ensure_slots(P, l);
-  HIPCHK(hipMemsetAsync(val, 0, sizeof(double) * D.nnz, P.stream));
-  HIPCHK(hipMemsetAsync(rhs, 0, sizeof(double) * D.ndof, P.stream));
   AsmParams apc = ap;
   if (P.mapped && co.supg) {  // per-cell h_conv of the general hexahedra
     if (!D.hconv) D.hconv = dalloc<double>(P, (size_t)D.ncell);
     apc.hconv = D.hconv;
     k_hconv_geo(P.stream, D, P.tab, apc, D.hconv);
   }
+  if (!scatter_assembly(P, l)) {
+    // two passes: every element row written out in one launch, then each CSR
+    // entry and rhs entry of the active rows summed from its contributions
+    // (colour order); Dirichlet rows: zero, then the identity row
+    ensure_gather(P, l);
+    apc.E = D.E;
+    apc.Er = D.Er;
+    k_assemble_rows(P.stream, D, P.tab, apc);
+    const int64_t nnz_act = P.hl[l].rowptr[D.nact];
+    k_gather_assembly(P.stream, D, nnz_act, val, rhs);
+    if (D.nnz > nnz_act) HIPCHK(hipMemsetAsync(val + nnz_act, 0, sizeof(double) * (D.nnz - nnz_act), P.stream));
+    k_dirichlet_rows(P.stream, D, val, rhs);
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
+  ensure_slots(P, l);
+  HIPCHK(hipMemsetAsync(val, 0, sizeof(double) * D.nnz, P.stream));
+  HIPCHK(hipMemsetAsync(rhs, 0, sizeof(double) * D.ndof, P.stream));
   for (int k = 0; k < 8; k++)
     k_assemble(P.stream, D, P.tab, apc, val, rhs, D.colour_start[k], D.colour_start[k + 1] - D.colour_start[k]);
   k_dirichlet_rows(P.stream, D, val, rhs);
@@ -2261,19 +2308,38 @@ cd3d_status cd3d_kernel_bench(cd3d_problem *p, int which, int level, int reps, d
     }
     double *scratch_val = nullptr;
     if (which == 8) {
-      // the cell-by-cell element kernel over all 8 colours of the level with
+      // the cell-by-cell assembly of the level as assemble_level runs it
+      // (element rows of every cell, then the gather of the active rows'
+      // entries; CD3D_ASM_SCATTER=1: the 8 colour-by-colour scatters) with
       // the coefficients of the last assembly, into a scratch value array
       // (the level's matrix stays as it is); FP64 flops per (cell, q, i, j): 8
-      ensure_slots(*p, level);
       scratch_val = dalloc<double>(*p, L.nnz);
       double *rhs_scratch = L.tmp;
-      const AsmParams ap = asm_params(*p, L, p->last_co, 0);
+      AsmParams ap = asm_params(*p, L, p->last_co, 0);
+      if (p->mapped && p->last_co.supg) {
+        if (!L.hconv) L.hconv = dalloc<double>(*p, (size_t)L.ncell);
+        ap.hconv = L.hconv;
+      }
       HIPCHK(hipMemsetAsync(scratch_val, 0, sizeof(double) * L.nnz, p->stream));
-      launch = [&, ap, rhs_scratch] {
-        for (int k = 0; k < 8; k++)
-          k_assemble(p->stream, L, p->tab, ap, scratch_val, rhs_scratch, L.colour_start[k],
-                     L.colour_start[k + 1] - L.colour_start[k]);
-      };
+      if (scatter_assembly(*p, level)) {
+        ensure_slots(*p, level);
+        launch = [&, ap, rhs_scratch] {
+          if (ap.hconv && ap.supg) k_hconv_geo(p->stream, L, p->tab, ap, L.hconv);
+          for (int k = 0; k < 8; k++)
+            k_assemble(p->stream, L, p->tab, ap, scratch_val, rhs_scratch, L.colour_start[k],
+                       L.colour_start[k + 1] - L.colour_start[k]);
+        };
+      } else {
+        ensure_gather(*p, level);
+        ap.E = L.E;
+        ap.Er = L.Er;
+        const int64_t nnz_act = p->hl[level].rowptr[L.nact];
+        launch = [&, ap, rhs_scratch, nnz_act] {
+          if (ap.hconv && ap.supg) k_hconv_geo(p->stream, L, p->tab, ap, L.hconv);
+          k_assemble_rows(p->stream, L, p->tab, ap);
+          k_gather_assembly(p->stream, L, nnz_act, scratch_val, rhs_scratch);
+        };
+      }
       const double nq = std::pow((double)quad_points_1d(o), 3), nbd = p->nb;
       b = (double)L.ncell * nq * 8.0 * nbd * nbd;
     }

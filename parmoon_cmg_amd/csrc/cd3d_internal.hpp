@@ -100,6 +100,10 @@ struct DevLevel {
   // assembly
   int4 *cells = nullptr;
   int *slots = nullptr;
+  // two-pass cell-by-cell assembly: element rows E (ncell nb x nb), their rhs
+  // entries Er (ncell nb) and the gather lists (allocated on first use)
+  double *E = nullptr, *Er = nullptr;
+  int *gptr = nullptr, *gidx = nullptr, *rptr = nullptr, *ridx = nullptr;
   int *gn = nullptr;
   int *colour_cells = nullptr;
   std::vector<int> colour_start;
@@ -221,6 +225,8 @@ void build_host_level(HostLevel &L, int order, int m, int depth);
 void refine_vertices(const std::vector<double> &src, int n, std::vector<double> &dst);
 void build_transfer(HostLevel &fine, const HostLevel &coarse, int order);
 void build_slots(const HostLevel &L, int order, std::vector<int> &slots);
+void build_gather(const HostLevel &L, int order, const std::vector<int> &slots, std::vector<int> &gptr,
+                  std::vector<int> &gidx, std::vector<int> &rptr, std::vector<int> &ridx);
 void build_colours(HostLevel &L, int order);
 void build_row_blocks(const std::vector<int> &rowptr, int max_nnz, int max_rows, std::vector<int> &blk);
 void build_mc_colours(HostLevel &L, int order);
@@ -308,10 +314,17 @@ struct AsmParams {
   const double *vx;
   int nv;
   const double *hconv;
+  // two-pass assembly: the element rows and rhs entries go here (cells in
+  // natural order, no colours) instead of being scattered into CSR
+  double *E, *Er;
 };
 void k_assemble(hipStream_t s, const DevLevel &L, const double *tab, const AsmParams &ap, double *val,
                 double *rhs, int colour_begin, int colour_count);
 void k_dirichlet_rows(hipStream_t s, const DevLevel &L, double *val, double *rhs);
+// the element kernel over every cell at once into ap.E / ap.Er, then the
+// CSR entries and rhs entries of the active rows gathered from them
+void k_assemble_rows(hipStream_t s, const DevLevel &L, const double *tab, const AsmParams &ap);
+void k_gather_assembly(hipStream_t s, const DevLevel &L, int64_t nnz_act, double *val, double *rhs);
 // mapped levels: per-cell SUPG mesh size in convection direction into hconv (ncell)
 void k_hconv_geo(hipStream_t s, const DevLevel &L, const double *tab, const AsmParams &ap, double *hconv);
 // ---------------------------------------------------------------- same-level batches (kernels.hip)

@@ -765,7 +765,7 @@ __global__ __launch_bounds__(kBlock) void assemble_k(int ncells, const int *__re
   const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (gid >= (int64_t)ncells * NB) return;
   const int ci = (int)(gid / NB), i = (int)(gid % NB);
-  const int c = list[ci];
+  const int c = list ? list[ci] : ci;
   const int4 cg = cells[c];
   int dax[3], sg[3];
   decode_frame(cg.w, dax, sg);
@@ -872,6 +872,13 @@ __global__ __launch_bounds__(kBlock) void assemble_k(int ncells, const int *__re
       acc[j] = fma(a0, T[5 + 4 * j], fma(a1, T[6 + 4 * j], fma(a2, T[7 + 4 * j], fma(cc, T[4 + 4 * j], acc[j]))));
     racc += w * f * ti;
   }
+  if (ap.E) {  // two-pass assembly: the element row, contiguously
+    const int64_t row = (int64_t)c * NB + i;
+#pragma unroll
+    for (int j = 0; j < NB; j++) ap.E[row * NB + j] = acc[j];
+    ap.Er[row] = racc;
+    return;
+  }
   const int r = gn[c * NB + i];
   if (r >= nact) return;
   const int *sl = slots + ((int64_t)c * NB + i) * NB;
@@ -902,7 +909,7 @@ __global__ __launch_bounds__(kBlock) void assemble_geo_k(int ncells, const int *
   const int64_t gid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (gid >= (int64_t)ncells * NB) return;
   const int ci = (int)(gid / NB), i = (int)(gid % NB);
-  const int c = list[ci];
+  const int c = list ? list[ci] : ci;
   TriMap M;
   tri_load(ap.vx, ap.nv, cells[c], P, M);
   const double nu = 1.0 / P;
@@ -982,6 +989,13 @@ __global__ __launch_bounds__(kBlock) void assemble_geo_k(int ncells, const int *
     for (int j = 0; j < NB; j++)
       acc[j] = fma(a[0], T[5 + 4 * j], fma(a[1], T[6 + 4 * j], fma(a[2], T[7 + 4 * j], fma(cc, T[4 + 4 * j], acc[j]))));
     racc += w * f * ti;
+  }
+  if (ap.E) {  // two-pass assembly: the element row, contiguously
+    const int64_t row = (int64_t)c * NB + i;
+#pragma unroll
+    for (int j = 0; j < NB; j++) ap.E[row * NB + j] = acc[j];
+    ap.Er[row] = racc;
+    return;
   }
   const int r = gn[c * NB + i];
   if (r >= nact) return;
@@ -1563,6 +1577,54 @@ void k_assemble(hipStream_t s, const DevLevel &L, const double *tab, const AsmPa
   }
 #undef CD3D_ASM
 }
+// CSR entry e of an active row: the sum of its element-row contributions in
+// colour order (gather lists of build_gather); active rhs entries likewise
+__global__ __launch_bounds__(kBlock) void gather_entries_k(int64_t n, const int *__restrict__ gptr,
+                                                           const int *__restrict__ gidx,
+                                                           const double *__restrict__ E, double *__restrict__ val) {
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBlock) {
+    double s = 0.0;
+    for (int k = gptr[e]; k < gptr[e + 1]; k++) s += E[gidx[k]];
+    val[e] = s;
+  }
+}
+
+void k_assemble_rows(hipStream_t s, const DevLevel &L, const double *tab, const AsmParams &ap) {
+  const int nb = ap.order == 1 ? 8 : 27;
+  const int64_t threads = (int64_t)L.ncell * nb;
+  const int grid = (int)((threads + kBlock - 1) / kBlock);
+  if (ap.vx) {
+    if (ap.order == 1) {
+      if (ap.conv_kind != 0 || ap.f_kind != 0 || ap.qp)
+        assemble_geo_k<1, true><<<grid, kBlock, 0, s>>>(L.ncell, nullptr, L.cells, nullptr, L.gn, L.nact, tab, ap, nullptr, nullptr);
+      else
+        assemble_geo_k<1, false><<<grid, kBlock, 0, s>>>(L.ncell, nullptr, L.cells, nullptr, L.gn, L.nact, tab, ap, nullptr, nullptr);
+    } else {
+      if (ap.conv_kind != 0 || ap.f_kind != 0 || ap.qp)
+        assemble_geo_k<2, true><<<grid, kBlock, 0, s>>>(L.ncell, nullptr, L.cells, nullptr, L.gn, L.nact, tab, ap, nullptr, nullptr);
+      else
+        assemble_geo_k<2, false><<<grid, kBlock, 0, s>>>(L.ncell, nullptr, L.cells, nullptr, L.gn, L.nact, tab, ap, nullptr, nullptr);
+    }
+    return;
+  }
+  const bool var = ap.conv_kind != 0 || ap.f_kind != 0 || ap.qp != nullptr;
+  if (ap.order == 1) {
+    if (var) assemble_k<1, false, true><<<grid, kBlock, 0, s>>>(L.ncell, nullptr, L.cells, nullptr, L.gn, L.nact, tab, ap, nullptr, nullptr);
+    else assemble_k<1, false, false><<<grid, kBlock, 0, s>>>(L.ncell, nullptr, L.cells, nullptr, L.gn, L.nact, tab, ap, nullptr, nullptr);
+  } else {
+    if (var) assemble_k<2, false, true><<<grid, kBlock, 0, s>>>(L.ncell, nullptr, L.cells, nullptr, L.gn, L.nact, tab, ap, nullptr, nullptr);
+    else assemble_k<2, false, false><<<grid, kBlock, 0, s>>>(L.ncell, nullptr, L.cells, nullptr, L.gn, L.nact, tab, ap, nullptr, nullptr);
+  }
+}
+
+void k_gather_assembly(hipStream_t s, const DevLevel &L, int64_t nnz_act, double *val, double *rhs) {
+  if (nnz_act > 0)
+    gather_entries_k<<<(int)std::min<int64_t>((nnz_act + kBlock - 1) / kBlock, 1 << 16), kBlock, 0, s>>>(
+        nnz_act, L.gptr, L.gidx, L.E, val);
+  if (L.nact > 0)
+    gather_entries_k<<<(L.nact + kBlock - 1) / kBlock, kBlock, 0, s>>>(L.nact, L.rptr, L.ridx, L.Er, rhs);
+}
+
 void k_hconv_geo(hipStream_t s, const DevLevel &L, const double *tab, const AsmParams &ap, double *hconv) {
   if (L.ncell <= 0) return;
   const int grid = (L.ncell + kBlock - 1) / kBlock;

@@ -366,6 +366,50 @@ void build_slots(const HostLevel &L, int order, std::vector<int> &slots) {
   });
 }
 
+// Gather maps of the two-pass cell-by-cell assembly.  The element kernel
+// writes row i of cell c's element matrix to E[(c nb + i) nb ..] and its rhs
+// entry to Er[c nb + i]; CSR entry e of an active row then sums
+// E[gidx[gptr[e] .. gptr[e+1])] and active row r sums Er[ridx[rptr[r] ..
+// rptr[r+1])].  The lists run over the cells by colour, so every entry adds
+// its contributions in the order of the colour-by-colour scatter.
+void build_gather(const HostLevel &L, int order, const std::vector<int> &slots, std::vector<int> &gptr,
+                  std::vector<int> &gidx, std::vector<int> &rptr, std::vector<int> &ridx) {
+  const int nb = (order + 1) * (order + 1) * (order + 1);
+  const int64_t nnz_act = L.rowptr[L.nact];
+  gptr.assign(nnz_act + 1, 0);
+  rptr.assign(L.nact + 1, 0);
+  // cells of one colour share no DOF, so within a colour every row and entry
+  // is touched by one cell at most: the colour's cells run in parallel
+  auto each_colour = [&](auto &&cell) {
+    for (int k = 0; k < 8; k++) {
+      const int b = L.colour_start[k];
+      parallel_for(L.colour_start[k + 1] - b, [&](int q) { cell(L.colour_cells[b + q]); });
+    }
+  };
+  each_colour([&](int c) {
+    for (int i = 0; i < nb; i++) {
+      const int r = L.gn[(size_t)c * nb + i];
+      if (r >= L.nact) continue;
+      rptr[r + 1]++;
+      for (int j = 0; j < nb; j++) gptr[slots[((size_t)c * nb + i) * nb + j] + 1]++;
+    }
+  });
+  for (int64_t e = 0; e < nnz_act; e++) gptr[e + 1] += gptr[e];
+  for (int r = 0; r < L.nact; r++) rptr[r + 1] += rptr[r];
+  gidx.resize(gptr[nnz_act]);
+  ridx.resize(rptr[L.nact]);
+  std::vector<int> gp(gptr.begin(), gptr.end() - 1), rp(rptr.begin(), rptr.end() - 1);
+  each_colour([&](int c) {
+    for (int i = 0; i < nb; i++) {
+      const int r = L.gn[(size_t)c * nb + i];
+      if (r >= L.nact) continue;
+      const int row = c * nb + i;
+      ridx[rp[r]++] = row;
+      for (int j = 0; j < nb; j++) gidx[gp[slots[(size_t)row * nb + j]]++] = row * nb + j;
+    }
+  });
+}
+
 // 8 colours by the parity of the cell's integer position: two cells of one
 // colour never share a node, so one colour can be scattered without atomics.
 void build_colours(HostLevel &L, int order) {

@@ -49,6 +49,49 @@ static void check_level(const HostLevel &L, int order) {
   for (int s : slots) EXPECT(s >= -1 && s < L.nnz);
 }
 
+// the gather lists of the two-pass assembly give, entry by entry, the sums of
+// the colour-by-colour scatter bit for bit
+static void check_gather(const HostLevel &L, int order) {
+  const int nb = (order + 1) * (order + 1) * (order + 1);
+  std::vector<int> slots, gptr, gidx, rptr, ridx;
+  build_slots(L, order, slots);
+  build_gather(L, order, slots, gptr, gidx, rptr, ridx);
+  std::vector<double> E((size_t)L.ncell * nb * nb), Er((size_t)L.ncell * nb);
+  unsigned s = 12345;
+  auto rnd = [&] {
+    s = s * 1103515245u + 12345u;
+    return (double)(s >> 8) / (1 << 24) - 0.5;
+  };
+  for (auto &e : E) e = rnd();
+  for (auto &e : Er) e = rnd();
+  std::vector<double> val(L.nnz, 0.0), rhs(L.ndof, 0.0);
+  for (int k = 0; k < 8; k++)
+    for (int q = L.colour_start[k]; q < L.colour_start[k + 1]; q++) {
+      const int c = L.colour_cells[q];
+      for (int i = 0; i < nb; i++) {
+        const int r = L.gn[(size_t)c * nb + i];
+        if (r >= L.nact) continue;
+        for (int j = 0; j < nb; j++) val[slots[((size_t)c * nb + i) * nb + j]] += E[((size_t)c * nb + i) * nb + j];
+        rhs[r] += Er[(size_t)c * nb + i];
+      }
+    }
+  const int nnz_act = L.rowptr[L.nact];
+  EXPECT((int)gptr.size() == nnz_act + 1 && (int)rptr.size() == L.nact + 1);
+  for (int e = 0; e < nnz_act; e++) {
+    double t = 0.0;
+    for (int k = gptr[e]; k < gptr[e + 1]; k++) {
+      EXPECT(gidx[k] >= 0 && (size_t)gidx[k] < E.size());
+      t += E[gidx[k]];
+    }
+    EXPECT(t == val[e]);
+  }
+  for (int r = 0; r < L.nact; r++) {
+    double t = 0.0;
+    for (int k = rptr[r]; k < rptr[r + 1]; k++) t += Er[ridx[k]];
+    EXPECT(t == rhs[r]);
+  }
+}
+
 static int geo_status(const std::string &path, const std::string &text) {
   std::ofstream(path) << text;
   int m = 0;
@@ -71,6 +114,7 @@ int main(int argc, char **argv) {
         build_mc_colours(F, order);
         check_level(C, order);
         check_level(F, order);
+        if (depth <= 1) check_gather(F, order);
         for (int c : F.p_col) EXPECT(c >= 0 && c < C.ndof);
         for (int c : F.r_col) EXPECT(c >= 0 && c < F.ndof);
         EXPECT((int)F.mc_rows.size() == F.nact);

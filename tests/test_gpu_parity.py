@@ -647,3 +647,22 @@ def test_quad_points_match_oracle(name):
     for l in range(c["levels"]):
         assert np.abs(P.quad_points(l) - H.quad_points(l)).max() <= 1e-15
     P.close()
+
+
+@pytest.mark.parametrize("name", ["unit_q1_var_supg", "m2_q2_supg_rot", "m3w_q2_supg", "m2w_q1_var_supg"])
+def test_two_pass_assembly_equals_the_scatter(name, golden, monkeypatch):
+    # the cell-by-cell assembly in two passes (element rows written out, CSR
+    # entries gathered in colour order) against the colour-by-colour scatter:
+    # the same additions in the same order, so the same bits
+    c = CD_CASES[name]
+    monkeypatch.setenv("CD3D_CELLWISE_ASSEMBLY", "1")
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("CD3D_ASM_SCATTER", mode)
+        P = make_problem(c)
+        P.assemble(pm.coeffs(**c["co"]))
+        out[mode] = [(P.csr(l)[2], P.rhs(l)) for l in range(c["levels"])]
+        P.close()
+    say(f"[gpu] two-pass assembly {name}: {len(out['0'])} levels compared")
+    for (e0, r0), (e1, r1) in zip(out["0"], out["1"]):
+        assert np.array_equal(e0, e1) and np.array_equal(r0, r1)
