@@ -1360,7 +1360,8 @@ size_t cd3d_estimate_bytes(const cd3d_mesh_desc *mesh, int restart, int cellwise
     b += 4 * N + 2 * nact_nnz + 4 * na + 4 * 4 * N;  // lattice: maps, stencil ids, vectors
     b += 8 * (nb * nb + nb) + 8 * 8 * 129 + 4 * 8 * 128 * 2;
     if (cellwise_assembly) {
-      b += 4 * nb * nb * ncell;                       // slot maps
+      // two-pass assembly: element rows, their gather list and pointers, rhs lists
+      b += 12.0 * nb * nb * ncell + 4 * nact_nnz + 12 * nb * ncell + 4 * N;
       if (l > 0) b += 8 * nact_nnz + 4 * na;          // per-node FP32 stencils and their CSR positions
     }
     if (tcd) b += 8 * nnz * (tcd == 2 ? 2 : 1);      // conv-diff matrix (and K with SUPG) beside the mass matrix
