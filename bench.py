@@ -79,7 +79,7 @@ def kernel_table(P, reps=50):
 TRAFFIC_FILE = os.path.join(REPO, "profiles", "cd3d_q2_64_fine_level_traffic.json")
 TRAFFIC_KERNEL = {0: "void csr_stream<0, double, double, true>", 1: "void lat_quad_gk<2, 0, 256>",
                   2: "void lat64_quad_gk<1, 0>", 3: "lat_restrict_pair_k", 4: "void lat_prolong_k<2>",
-                  5: "csr_fill2_k", 6: "copy_k", 7: "void axpy_dot2_k<true>", 8: "void assemble_k<2>"}
+                  5: "csr_fill2_k", 6: "copy_k", 7: "void axpy_dot2_k<true>", 8: "assemble_k + gather_entries_k"}
 
 
 def measured_traffic(which):
