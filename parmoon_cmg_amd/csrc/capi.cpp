@@ -200,6 +200,18 @@ static void setup_problem(cd3d_problem &P, const cd3d_mesh_desc &m, const double
     D.nblk = (int)H.blk.size() - 1;
     D.blk = dalloc<int>(P, H.blk.size());
     upload(P, D.blk, H.blk);
+    {
+      std::vector<int> bt(4 * (size_t)D.nblk);
+      for (int b = 0; b < D.nblk; b++) {
+        const int r0 = H.blk[b], r1 = H.blk[b + 1];
+        bt[4 * b] = r0;
+        bt[4 * b + 1] = r1;
+        bt[4 * b + 2] = H.rowptr[r0];
+        bt[4 * b + 3] = H.rowptr[r1] - H.rowptr[r0];
+      }
+      D.bt = (int4 *)dalloc<int>(P, bt.size());
+      upload(P, (int *)D.bt, bt);
+    }
     D.invdiag = dalloc<double>(P, H.ndof);
     for (double **v : {&D.sol, &D.rhs, &D.aux, &D.tmp, &D.rhs_asm}) {
       *v = dalloc<double>(P, H.ndof);
@@ -1073,7 +1085,7 @@ static void batch_solve_impl(BatchRun &br, std::vector<BatchState> &st) {
       L.ndof = D.ndof;
       L.nact = D.nact;
       L.nblk = D.nblk;
-      L.blk = D.blk;
+      L.bt = D.bt;
       L.rowptr = D.rowptr;
       L.col = D.col;
       L.diag = D.diag;
@@ -1350,7 +1362,7 @@ size_t cd3d_estimate_bytes(const cd3d_mesh_desc *mesh, int restart, int cellwise
     const double nact_nnz = row * row * row, nnz = nact_nnz + (N - na);
     double b = 0;
     b += 4 * (N + 1) + 4 * nnz + 4 * N + 8 * nnz;  // rowptr, col, diag, val
-    b += 4 * (nnz / 2048 + N / 256 + 2);           // row blocks
+    b += (4 + 16) * (nnz / 2048 + N / 256 + 2);    // row blocks and their extents
     b += 8 * N + 5 * 8 * N;                        // 1/a_ii, sol rhs aux tmp rhs_asm
     if (l > 0) {
       const double pnnz = (p == 1 ? 3.375 : 8.0) * N;

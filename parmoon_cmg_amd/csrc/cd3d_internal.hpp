@@ -81,6 +81,8 @@ struct DevLevel {
   // row blocks of the stream kernels (at most 2048 nonzeros and 256 rows)
   int *blk = nullptr;
   int nblk = 0;
+  // per row block: first row, end row, first nonzero, nonzero count
+  int4 *bt = nullptr;
   double *invdiag = nullptr;  // 1 / a_ii (valid when invdiag_ok)
   bool invdiag_ok = false;    // computed on first use after each change of val
   double *sol = nullptr, *rhs = nullptr, *aux = nullptr, *tmp = nullptr;
@@ -343,7 +345,8 @@ enum BatchVec { kBSol = 0, kBRhs = 1, kBAux = 2, kBTmp = 3, kBXs = 4, kBBs = 5, 
                 kBZ0 = 8 + kBatchMaxRestart + 2 };
 struct BatchLevel {
   int ndof = 0, nact = 0, nblk = 0;
-  const int *blk = nullptr, *rowptr = nullptr, *col = nullptr, *diag = nullptr;
+  const int4 *bt = nullptr;
+  const int *rowptr = nullptr, *col = nullptr, *diag = nullptr;
   const double *val = nullptr, *invdiag = nullptr;
   double *vec[4] = {nullptr, nullptr, nullptr, nullptr};  // sol, rhs, aux, tmp
   // prolongation (rows of this level) and restriction (rows of the coarser level)

@@ -178,25 +178,6 @@ constexpr int kStreamPer = kStreamNnz / kStreamBlock;
 // matrices above the 256 MB last-level (Infinity) cache are streamed non-temporally
 constexpr double kStreamNtBytes = 256.0 * (1 << 20);
 
-template <int G, int MODE, typename T>
-__device__ inline void stream_rows(int nr, int r0, int j0, int nact, const int *__restrict__ rowptr,
-                                   const T *__restrict__ prod, const T *__restrict__ x, const T *__restrict__ f,
-                                   const T *__restrict__ invdiag, T *__restrict__ out, T omega) {
-  const int g = threadIdx.x / G, l = threadIdx.x % G;
-  for (int r = g; r < nr; r += kStreamBlock / G) {
-    const int row = r0 + r;
-    const int a = rowptr[row] - j0, e = rowptr[row + 1] - j0;
-    T s = T(0);
-    for (int t = a + l; t < e; t += G) s += prod[t];
-    s = group_sum<G>(s);
-    if (l == 0) {
-      if (MODE == kSpmv) out[row] = s;
-      else if (MODE == kDefect) out[row] = f[row] - s;
-      else out[row] = row < nact ? x[row] + omega * (f[row] - s) * invdiag[row] : f[row];
-    }
-  }
-}
-
 template <bool NT, typename P>
 __device__ inline P stream_load(const P *p) {
   if (NT) return __builtin_nontemporal_load(p);
@@ -204,17 +185,26 @@ __device__ inline P stream_load(const P *p) {
 }
 
 // One row block bi of the stream kernel (the whole workgroup takes part).
-// NT: non-temporal matrix loads (matrices larger than the caches)
+// bt[bi] = (first row, end row, first nonzero, nonzero count): the block's
+// extent in one scalar load.  Phase 1 streams the nonzeros with coalesced
+// loads and stores a_ij x_j into LDS; phase 2 sums each row from LDS with G
+// lanes per row (G = the largest power of two with G * rows <= 256, so every
+// lane group owns at most one row).  A lane group's row extent and the
+// row's operands of the defect and Jacobi modes do not depend on phase 1:
+// they are loaded beside the matrix stream, ahead of the barrier, so the
+// block's chain of dependent memory trips is the matrix load, the gather of
+// x and the store (fine level of the 64^3 Q2 mesh: 346 -> 332 us).  NT:
+// non-temporal matrix loads (matrices larger than the caches).
 template <int MODE, typename T, typename V, bool NT>
-__device__ inline void csr_stream_block(int bi, const int *__restrict__ blk, int nact,
+__device__ inline void csr_stream_block(int bi, const int4 *__restrict__ bt, int nact,
                                         const int *__restrict__ rowptr, const int *__restrict__ col,
                                         const V *__restrict__ val, const T *__restrict__ x,
                                         const T *__restrict__ f, const T *__restrict__ invdiag, T *__restrict__ out,
                                         T omega) {
+  static_assert(kStreamRows <= kStreamBlock, "one row per lane group");
   __shared__ T prod[kStreamNnz];
-  const int r0 = blk[bi], r1 = blk[bi + 1];
-  const int j0 = rowptr[r0], nz = rowptr[r1] - j0;
-  const int nr = r1 - r0;
+  const int4 ext = bt[bi];
+  const int r0 = ext.x, nr = ext.y - ext.x, j0 = ext.z, nz = ext.w;
   if (nz > kStreamNnz) {
     // a single row longer than a block: strided partial sums and a block reduction
     __shared__ T red[kStreamBlock / 64];
@@ -258,6 +248,20 @@ __device__ inline void csr_stream_block(int bi, const int *__restrict__ blk, int
       a[2 * k + 1] = T(0);
     }
   }
+  // this lane group's row: extent and operands, loaded ahead of the barrier
+  const int lg = nr <= 4 ? 6 : nr <= 8 ? 5 : nr <= 16 ? 4 : nr <= 32 ? 3 : nr <= 64 ? 2 : nr <= 128 ? 1 : 0;
+  const int g = threadIdx.x >> lg, l = threadIdx.x & ((1 << lg) - 1), row = r0 + g;
+  int ra = 0, re = 0;
+  T fr = T(0), xr = T(0), dr = T(0);
+  if (g < nr) {
+    ra = rowptr[row] - j0;
+    re = rowptr[row + 1] - j0;
+    if (MODE != kSpmv) fr = f[row];
+    if (MODE == kJacobi && row < nact) {
+      xr = x[row];
+      dr = invdiag[row];
+    }
+  }
 #pragma unroll
   for (int k = 0; k < kStreamPer / 2; k++) {
     const int t = 2 * threadIdx.x + k * 2 * kStreamBlock;
@@ -265,20 +269,26 @@ __device__ inline void csr_stream_block(int bi, const int *__restrict__ blk, int
     if (t + 1 < nz) prod[t + 1] = a[2 * k + 1] * x[c[2 * k + 1]];
   }
   __syncthreads();
-  const int G = nr <= 4 ? 64 : nr <= 8 ? 32 : nr <= 16 ? 16 : nr <= 32 ? 8 : nr <= 64 ? 4 : nr <= 128 ? 2 : 1;
-  switch (G) {
-    case 64: stream_rows<64, MODE, T>(nr, r0, j0, nact, rowptr, prod, x, f, invdiag, out, omega); break;
-    case 32: stream_rows<32, MODE, T>(nr, r0, j0, nact, rowptr, prod, x, f, invdiag, out, omega); break;
-    case 16: stream_rows<16, MODE, T>(nr, r0, j0, nact, rowptr, prod, x, f, invdiag, out, omega); break;
-    case 8: stream_rows<8, MODE, T>(nr, r0, j0, nact, rowptr, prod, x, f, invdiag, out, omega); break;
-    case 4: stream_rows<4, MODE, T>(nr, r0, j0, nact, rowptr, prod, x, f, invdiag, out, omega); break;
-    case 2: stream_rows<2, MODE, T>(nr, r0, j0, nact, rowptr, prod, x, f, invdiag, out, omega); break;
-    default: stream_rows<1, MODE, T>(nr, r0, j0, nact, rowptr, prod, x, f, invdiag, out, omega); break;
+  T s = T(0);
+  for (int t = ra + l; t < re; t += 1 << lg) s += prod[t];
+  switch (lg) {
+    case 6: s = group_sum<64>(s); break;
+    case 5: s = group_sum<32>(s); break;
+    case 4: s = group_sum<16>(s); break;
+    case 3: s = group_sum<8>(s); break;
+    case 2: s = group_sum<4>(s); break;
+    case 1: s = group_sum<2>(s); break;
+    default: break;
+  }
+  if (g < nr && l == 0) {
+    if (MODE == kSpmv) out[row] = s;
+    else if (MODE == kDefect) out[row] = fr - s;
+    else out[row] = row < nact ? xr + omega * (fr - s) * dr : fr;
   }
 }
 
 template <int MODE, typename T, typename V, bool NT>
-__global__ __launch_bounds__(kStreamBlock) void csr_stream(const int *__restrict__ blk, int nact,
+__global__ __launch_bounds__(kStreamBlock) void csr_stream(const int4 *__restrict__ bt, int nact,
                                                      const int *__restrict__ rowptr, const int *__restrict__ col,
                                                      const V *__restrict__ val, const T *__restrict__ x,
                                                      const T *__restrict__ f, const T *__restrict__ invdiag,
@@ -286,7 +296,7 @@ __global__ __launch_bounds__(kStreamBlock) void csr_stream(const int *__restrict
   // workgroups go to the 8 XCDs round-robin: XCD x takes a contiguous range
   // of row blocks, so that its L2 sees about 1/8 of the gathered x (fine
   // level of the 64^3 Q2 mesh: 370.7 -> 352.8 us)
-  csr_stream_block<MODE, T, V, NT>(xcd_block(blockIdx.x, gridDim.x), blk, nact, rowptr, col, val, x, f, invdiag, out,
+  csr_stream_block<MODE, T, V, NT>(xcd_block(blockIdx.x, gridDim.x), bt, nact, rowptr, col, val, x, f, invdiag, out,
                                    omega);
 }
 
@@ -300,11 +310,12 @@ template <int MODE, typename T, typename V>
 void launch_stream(hipStream_t st, const DevLevel &L, const V *val, const T *x, const T *f, const T *invdiag, T *out,
                    T omega) {
   if (L.nblk <= 0) return;
-  if ((double)L.nnz * (sizeof(V) + sizeof(int)) > kStreamNtBytes)
-    csr_stream<MODE, T, V, true><<<L.nblk, kStreamBlock, 0, st>>>(L.blk, L.nact, L.rowptr, L.col, val, x, f, invdiag,
+  const bool nt = (double)L.nnz * (sizeof(V) + sizeof(int)) > kStreamNtBytes;
+  if (nt)
+    csr_stream<MODE, T, V, true><<<L.nblk, kStreamBlock, 0, st>>>(L.bt, L.nact, L.rowptr, L.col, val, x, f, invdiag,
                                                                   out, omega);
   else
-    csr_stream<MODE, T, V, false><<<L.nblk, kStreamBlock, 0, st>>>(L.blk, L.nact, L.rowptr, L.col, val, x, f, invdiag,
+    csr_stream<MODE, T, V, false><<<L.nblk, kStreamBlock, 0, st>>>(L.bt, L.nact, L.rowptr, L.col, val, x, f, invdiag,
                                                                    out, omega);
 }
 
@@ -1751,7 +1762,7 @@ __global__ __launch_bounds__(kStreamBlock) void bstream_k(const BatchLevel *__re
   const BatchLevel &L = lev[b];
   if ((int)blockIdx.x >= L.nblk) return;
   const BatchProblem &P = prob[b];
-  csr_stream_block<MODE, double, double, false>(blockIdx.x, L.blk, L.nact, L.rowptr, L.col, L.val, bvec(L, P, xid),
+  csr_stream_block<MODE, double, double, false>(blockIdx.x, L.bt, L.nact, L.rowptr, L.col, L.val, bvec(L, P, xid),
                                                 fid >= 0 ? bvec(L, P, fid) : nullptr, L.invdiag, bvec(L, P, oid),
                                                 omega);
 }
