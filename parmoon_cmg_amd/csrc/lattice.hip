@@ -584,6 +584,12 @@ __host__ __device__ inline void quad_sums(const LatArgs &a, const QuadPos &p, co
 #pragma unroll
         for (int e = 0; e < vn; e++) w[2 + vn * h + e] = m[e];
       }
+      if constexpr (2 * G < vn) {
+        // one pair per line in FP32: the pair in one 8-byte load
+        const V2 m = *(const V2 *)xl;
+        w[2] = m.x;
+        w[3] = m.y;
+      }
       w[2 * G + 2] = xl[2 * G];
 #pragma unroll
       for (int oz = 0; oz < 2; oz++) {
@@ -613,16 +619,15 @@ __host__ __device__ inline void quad_sums(const LatArgs &a, const QuadPos &p, co
   }
 }
 
-template <int G, int MODE, int TB>
-__global__ __launch_bounds__(TB) void lat_quad_gk(LatArgs a, const float *__restrict__ cst,
-                                                  const float *__restrict__ x, const float *__restrict__ f,
-                                                  float *__restrict__ out, float omega, int xcd) {
-  __shared__ float cs[8 * kCst];
-  for (int i = threadIdx.x; i < 8 * kCst; i += TB) cs[i] = cst[i];
-  __syncthreads();
-  const int bid = xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+// quad k of the FP32 sweep: G vertex/midpoint pairs on each of the 4 lines
+// (Y0 + oy, Z0 + oz) (cs: the 8 classes' compact stencils)
+template <int G, int MODE>
+__host__ __device__ __attribute__((always_inline)) inline void lat_quad_node(const LatArgs &a, long long k,
+                                                                             const float *cs,
+                                              const float *__restrict__ x, const float *__restrict__ f,
+                                              float *__restrict__ out, float omega) {
   QuadPos p;
-  if (!quad_pos<G>(a, (long long)bid * TB + threadIdx.x, p)) return;
+  if (!quad_pos<G>(a, k, p)) return;
   const long long nn = a.nn;
   float s0[4][G], s1[4][G], xs[4][2 * G];
   if (MODE != kLJacobiZero) quad_sums<float, G>(a, p, cs, x, s0, s1, xs);
@@ -633,11 +638,17 @@ __global__ __launch_bounds__(TB) void lat_quad_gk(LatArgs a, const float *__rest
     const long long nv = p.base + nn * (oy + nn * oz);
     const float d0 = cs[2 * pc * kCst + kCst - 1], d1 = cs[(2 * pc + 1) * kCst + kCst - 1];
     float fv[2 * G];
+    if constexpr (G == 1) {
+      const f2u t = *(const f2u *)(f + nv);
+      fv[0] = t.x;
+      fv[1] = t.y;
+    } else {
 #pragma unroll
-    for (int h = 0; h < G / 2; h++) {
-      const f4u t = *(const f4u *)(f + nv + 4 * h);
+      for (int h = 0; h < G / 2; h++) {
+        const f4u t = *(const f4u *)(f + nv + 4 * h);
 #pragma unroll
-      for (int e = 0; e < 4; e++) fv[4 * h + e] = t[e];
+        for (int e = 0; e < 4; e++) fv[4 * h + e] = t[e];
+      }
     }
 #pragma unroll
     for (int q = 0; q < G; q++) {
@@ -653,6 +664,17 @@ __global__ __launch_bounds__(TB) void lat_quad_gk(LatArgs a, const float *__rest
       }
     }
   }
+}
+
+template <int G, int MODE, int TB>
+__global__ __launch_bounds__(TB) void lat_quad_gk(LatArgs a, const float *__restrict__ cst,
+                                                  const float *__restrict__ x, const float *__restrict__ f,
+                                                  float *__restrict__ out, float omega, int xcd) {
+  __shared__ float cs[8 * kCst];
+  for (int i = threadIdx.x; i < 8 * kCst; i += TB) cs[i] = cst[i];
+  __syncthreads();
+  const int bid = xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  lat_quad_node<G, MODE>(a, (long long)bid * TB + threadIdx.x, cs, x, f, out, omega);
 }
 
 template <int P, int MODE>
@@ -1861,20 +1883,24 @@ void lat_sweep(hipStream_t s, const DevLevel &L, int order, const float *x, cons
   else if (G > 1 && lat_quads()) {
     // measured on the 129^3-node level of a 64^3-cell Q2 mesh (FP32 Jacobi
     // sweep): 2 pairs per thread, 256 threads, XCD-contiguous blocks 12.4 us;
-    // without the XCD mapping 13.6; 4 pairs 13.4 / 15.1; 128 threads
-    // 12.6 / 15.0; the pair-class kernel it replaces 18.7 (CD3D_QUAD_G,
-    // CD3D_QUAD_TB, CD3D_QUAD_XCD select the variants)
+    // without the XCD mapping 13.6; 4 pairs 13.4 / 15.1; 1 pair 14.0 (61
+    // instead of 159 VGPRs, but 2.5 instead of 1.75 neighbour-line loads per
+    // node); 128 threads 12.6 / 15.0; the pair-class kernel it replaces 18.7
+    // (CD3D_QUAD_G, CD3D_QUAD_TB, CD3D_QUAD_XCD select the variants)
     const char *eg = getenv("CD3D_QUAD_G"), *et = getenv("CD3D_QUAD_TB"), *ex = getenv("CD3D_QUAD_XCD");
     const int gq_g = eg ? atoi(eg) : 2, tb = et ? atoi(et) : 256, xcd = ex ? atoi(ex) : 1;
-    const int Gq = (gq_g == 4 && G == 4) ? 4 : 2;  // G == 4 implies 2 divides the pairs per line too
+    // G == 4 implies 2 divides the pairs per line too
+    const int Gq = gq_g == 1 ? 1 : (gq_g == 4 && G == 4) ? 4 : 2;
     const long long n = a.c[1].cnt[0], threads = n / Gq * n * n;
     const int gq = (int)((threads + tb - 1) / tb);
     if (tb == 128) {
-      if (Gq == 2) lat_quad_gk<2, MODE, 128><<<gq, 128, 0, s>>>(a, L.lcst, x, f, out, omega, xcd);
+      if (Gq == 1) lat_quad_gk<1, MODE, 128><<<gq, 128, 0, s>>>(a, L.lcst, x, f, out, omega, xcd);
+      else if (Gq == 2) lat_quad_gk<2, MODE, 128><<<gq, 128, 0, s>>>(a, L.lcst, x, f, out, omega, xcd);
       else lat_quad_gk<4, MODE, 128><<<gq, 128, 0, s>>>(a, L.lcst, x, f, out, omega, xcd);
     } else {
-      if (Gq == 2) lat_quad_gk<2, MODE, 256><<<(int)((threads + 255) / 256), 256, 0, s>>>(a, L.lcst, x, f, out, omega, xcd);
-      else lat_quad_gk<4, MODE, 256><<<(int)((threads + 255) / 256), 256, 0, s>>>(a, L.lcst, x, f, out, omega, xcd);
+      if (Gq == 1) lat_quad_gk<1, MODE, 256><<<gq, 256, 0, s>>>(a, L.lcst, x, f, out, omega, xcd);
+      else if (Gq == 2) lat_quad_gk<2, MODE, 256><<<gq, 256, 0, s>>>(a, L.lcst, x, f, out, omega, xcd);
+      else lat_quad_gk<4, MODE, 256><<<gq, 256, 0, s>>>(a, L.lcst, x, f, out, omega, xcd);
     }
   } else if (G > 1) {
     const LatArgs ag = group_args(a, G);

@@ -343,6 +343,29 @@ int check_pairs(int m, int k0) {
         printf("   groups of %d m%d k0=%d mode %d: %lld differ\n", G, m, k0, mode, (long long)gd);
         if (gd) bad++;
       }
+      // quads of lines (G pairs on each of 4 neighbouring lines per thread) against single pairs
+      for (int G = 1; uni && G <= 4; G *= 2) {
+        const long long n = a.c[1].cnt[0];
+        if (n % G) continue;
+        std::vector<float> o4(nn3, 7.f);
+        for (long long k = 0; k < n / G * n * n; k++) {
+#define CD3D_QD(GG, M) lat_quad_node<GG, M>(a, k, cst.data(), x.data(), f.data(), o4.data(), 0.7f)
+          if (G == 1 && mode == kLJacobi) CD3D_QD(1, kLJacobi);
+          if (G == 1 && mode == kLJacobiZero) CD3D_QD(1, kLJacobiZero);
+          if (G == 1 && mode == kLDefect) CD3D_QD(1, kLDefect);
+          if (G == 2 && mode == kLJacobi) CD3D_QD(2, kLJacobi);
+          if (G == 2 && mode == kLJacobiZero) CD3D_QD(2, kLJacobiZero);
+          if (G == 2 && mode == kLDefect) CD3D_QD(2, kLDefect);
+          if (G == 4 && mode == kLJacobi) CD3D_QD(4, kLJacobi);
+          if (G == 4 && mode == kLJacobiZero) CD3D_QD(4, kLJacobiZero);
+          if (G == 4 && mode == kLDefect) CD3D_QD(4, kLDefect);
+#undef CD3D_QD
+        }
+        int64_t qd = 0;
+        for (size_t i = 0; i < nn3; i++) qd += memcmp(&o2[i], &o4[i], sizeof(float)) != 0;
+        printf("   quads of %d m%d k0=%d mode %d: %lld differ\n", G, m, k0, mode, (long long)qd);
+        if (qd) bad++;
+      }
     }
   }
   // affine stencils evaluated node by node against per-node values filled
