@@ -70,12 +70,15 @@ typedef enum { CD3D_FP64 = 0, CD3D_MIXED = 1 } cd3d_precision;
 /* Solver parameters: the ParamDB SC_* knobs of the GMG/FGMRES path
  * (Database.C:648-757, FgmresIte.C:71-85, MultiGrid3D.C). */
 typedef struct {
-  int smoother;         /* SC_SMOOTHER_SCALAR: 1 damped Jacobi (MGLevel3D.C:443); 13 multicolour SSOR (no
-                           reference counterpart: relaxed Gauss-Seidel over node colours, forward then
-                           backward, for convection-dominated problems; FP64 in both precision forms) */
+  int smoother;         /* SC_SMOOTHER_SCALAR: 1 damped Jacobi (MGLevel3D.C:443); 2 SOR and 3 SSOR in the
+                           reference's row order (MGLevel3D.C:254-441: sequential by nature, run as
+                           wavefronts of independent rows by one workgroup; FP64 in both precision
+                           forms); 13 multicolour SSOR (no reference counterpart: relaxed Gauss-Seidel
+                           over node colours, forward then backward, for convection-dominated problems;
+                           FP64 in both precision forms) */
   int pre, post;        /* SC_PRE_SMOOTH_SCALAR, SC_POST_SMOOTH_SCALAR */
   double omega;         /* SC_SMOOTH_DAMP_FACTOR_SCALAR */
-  int coarse_smoother;  /* SC_COARSE_SMOOTHER_SCALAR: 17 exact (default), 3 SSOR, 1 Jacobi */
+  int coarse_smoother;  /* SC_COARSE_SMOOTHER_SCALAR: 17 exact (default), 3 SSOR, 2 SOR, 1 Jacobi */
   int coarse_maxit;     /* SC_COARSE_MAXIT_SCALAR */
   double coarse_red;    /* SC_COARSE_RED_FACTOR_SCALAR */
   double gmg_damp;      /* SC_GMG_DAMP_FACTOR_SCALAR */
@@ -199,7 +202,7 @@ cd3d_status cd3d_solve(cd3d_problem *p, const cd3d_solver_params *sp, const doub
  * transfer of all H columns per iteration.  Each problem keeps its own
  * stopping test; iteration counts equal a lone solve's, solutions agree to
  * rounding (the reduction trees differ).  Other parameters (mixed precision,
- * smoother 13, W/F cycles, coarse smoothers other than 17, no pre-smoothing,
+ * smoothers other than 1, W/F cycles, coarse smoothers other than 17, no pre-smoothing,
  * restart > 200) or count 1: the problems are solved one after another.
  * Runs on the first handle's stream; every handle is locked meanwhile.
  * stats (count entries, may be NULL) receives each problem's result; the

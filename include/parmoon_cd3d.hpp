@@ -82,12 +82,12 @@ struct TParamDB {
   int SC_PRECONDITIONER_SCALAR = 5;   // 5 multigrid
   int SC_MG_TYPE_SCALAR = 0;          // 0 geometric
   int SC_MG_CYCLE_SCALAR = 1;         // 1 V, 2 W, 0 F
-  int SC_SMOOTHER_SCALAR = 3;         // the device smoother is Jacobi (1)
+  int SC_SMOOTHER_SCALAR = 3;         // 1 Jacobi, 2 SOR, 3 SSOR (13: multicolour SSOR, an extension)
   int SC_PRE_SMOOTH_SCALAR = 2;
   int SC_POST_SMOOTH_SCALAR = 2;
   double SC_SMOOTH_DAMP_FACTOR_SCALAR = 1.0;
   double SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR = 1.0;
-  int SC_COARSE_SMOOTHER_SCALAR = 3;  // 17 exact, 3 SSOR, 1 Jacobi
+  int SC_COARSE_SMOOTHER_SCALAR = 3;  // 17 exact, 3 SSOR, 2 SOR, 1 Jacobi
   int SC_COARSE_MAXIT_SCALAR = 10;
   double SC_COARSE_RED_FACTOR_SCALAR = 0.1;
   double SC_GMG_DAMP_FACTOR_SCALAR = 1.0;
@@ -122,7 +122,10 @@ inline cd3d_solver_params SolverParamsFromDB() {
     Fail("Unknown preconditioner !!! (the device path runs multigrid, SC_PRECONDITIONER_SCALAR 5)");
   if (db.SC_STEP_LENGTH_CONTROL_ALL_SCALAR || db.SC_STEP_LENGTH_CONTROL_FINE_SCALAR)
     Fail("step length control is not available on the device path");
-  sp.smoother = 1;
+  if (db.SC_SMOOTHER_SCALAR != 1 && db.SC_SMOOTHER_SCALAR != 2 && db.SC_SMOOTHER_SCALAR != 3 &&
+      db.SC_SMOOTHER_SCALAR != 13)
+    Fail("smoother not implemented !!! (the device path has SC_SMOOTHER_SCALAR 1 Jacobi, 2 SOR, 3 SSOR, 13)");
+  sp.smoother = db.SC_SMOOTHER_SCALAR;
   sp.pre = db.SC_PRE_SMOOTH_SCALAR;
   sp.post = db.SC_POST_SMOOTH_SCALAR;
   sp.omega = db.SC_SMOOTH_DAMP_FACTOR_SCALAR;

@@ -816,6 +816,23 @@ static void jacobi(level_t *L, double omega) {
   }
 }
 
+/* TMGLevel3D::SOR (MGLevel3D.C:254-335): Dirichlet values, then one
+ * relaxed Gauss-Seidel pass over the active rows in ascending order */
+static void sor(level_t *L, double omega) {
+  double *sol = L->sol, *f = L->rhs;
+  memcpy(sol + L->nact, f + L->nact, sizeof(double) * (L->ndof - L->nact));
+  for (int i = 0; i < L->nact; i++) {
+    double s = f[i], diag = 1;
+    for (int j = L->rowptr[i]; j < L->rowptr[i + 1]; j++) {
+      int c = L->kcol[j];
+      if (c == i) diag = L->A[j];
+      else s -= L->A[j] * sol[c];
+    }
+    double t = sol[i];
+    sol[i] = omega * (s / diag - t) + t;
+  }
+}
+
 /* TMGLevel3D::SSOR (MGLevel3D.C:337-441) */
 static void ssor(level_t *L, double omega) {
   double *sol = L->sol, *f = L->rhs;
@@ -996,6 +1013,18 @@ static void mcssor(level_t *L, double omega, int p) {
   for (int c = 0; c < ncol; c++) colour_pass(L, omega, p, c);
   for (int c = ncol - 1; c >= 0; c--) colour_pass(L, omega, p, c);
 }
+/* one SOR (kind 2) or SSOR (kind 3) sweep of level l on x with rhs f */
+void orc_gs_sweep(orc_hier *h, int l, int kind, double omega, const double *f, double *x) {
+  level_t *L = h->lv + l;
+  double *s0 = L->sol, *r0 = L->rhs;
+  L->sol = x;
+  L->rhs = (double *)f;
+  if (kind == 2) sor(L, omega);
+  else ssor(L, omega);
+  L->sol = s0;
+  L->rhs = r0;
+}
+
 void orc_mcssor_sweep(orc_hier *h, int l, double omega, const double *f, double *x) {
   level_t *L = h->lv + l;
   double *s0 = L->sol, *r0 = L->rhs;
@@ -1009,6 +1038,7 @@ void orc_mcssor_sweep(orc_hier *h, int l, double omega, const double *f, double 
 static void smooth(orc_hier *h, level_t *L, const orc_solver *s, int n) {
   for (int k = 0; k < n; k++) {
     if (s->smoother == 3) ssor(L, s->omega);
+    else if (s->smoother == 2) sor(L, s->omega);
     else if (s->smoother == 13) mcssor(L, s->omega, h->order);
     else jacobi(L, s->omega);
   }
@@ -1030,6 +1060,7 @@ static void cycle(orc_hier *h, int i, const orc_solver *s, int *rec) {
     while (res > reduction && it < s->coarse_maxit) {
       if (s->coarse_smoother == 17) coarse_solve(h);
       else if (s->coarse_smoother == 1) jacobi(L, s->omega);
+      else if (s->coarse_smoother == 2) sor(L, s->omega);
       else ssor(L, s->omega);
       res = defect_(L, L->sol, L->rhs, L->aux0);
       it++;

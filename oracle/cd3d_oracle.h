@@ -30,10 +30,10 @@ typedef struct {
 } orc_coeffs;
 
 typedef struct {
-  int smoother;          /* 1 Jacobi, 3 SSOR (SC_SMOOTHER_SCALAR); 13 multicolour SSOR (this build's own) */
+  int smoother;          /* 1 Jacobi, 2 SOR, 3 SSOR (SC_SMOOTHER_SCALAR); 13 multicolour SSOR (this build's own) */
   int pre, post;         /* SC_PRE/POST_SMOOTH_SCALAR */
   double omega;          /* SC_SMOOTH_DAMP_FACTOR_SCALAR */
-  int coarse_smoother;   /* 1 Jacobi, 3 SSOR, 17 exact (SC_COARSE_SMOOTHER_SCALAR) */
+  int coarse_smoother;   /* 1 Jacobi, 2 SOR, 3 SSOR, 17 exact (SC_COARSE_SMOOTHER_SCALAR) */
   int coarse_maxit;      /* SC_COARSE_MAXIT_SCALAR */
   double coarse_red;     /* SC_COARSE_RED_FACTOR_SCALAR */
   double gmg_damp;       /* SC_GMG_DAMP_FACTOR_SCALAR */
@@ -113,6 +113,8 @@ void orc_interpolate(const orc_hier *h, int l, int fn, const orc_coeffs *co, dou
  * the same over every cell */
 void orc_errors(const orc_hier *h, int l, const double *u, int fn, const orc_coeffs *co, double *err);
 
+/* one SOR (kind 2) or SSOR (kind 3) sweep in row order on level l: x in/out, f the rhs */
+void orc_gs_sweep(orc_hier *h, int l, int kind, double omega, const double *f, double *x);
 /* one multicolour SSOR sweep (smoother 13) on level l: x in/out, f the rhs */
 void orc_mcssor_sweep(orc_hier *h, int l, double omega, const double *f, double *x);
 

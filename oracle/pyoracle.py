@@ -88,6 +88,7 @@ def lib():
         L.orc_tcd_run.argtypes = [P, C.POINTER(Coeffs), C.POINTER(Solver), D, D, I, dp, dp]
         L.orc_interp_pulse.argtypes = [P, I, C.POINTER(Coeffs), dp]
         L.orc_mcssor_sweep.argtypes = [P, I, D, dp, dp]
+        L.orc_gs_sweep.argtypes = [P, I, I, D, dp, dp]
         L.orc_interpolate.argtypes = [P, I, I, C.POINTER(Coeffs), dp]
         L.orc_quad_points.argtypes = [P, I, dp]
         L.orc_set_coeffs_qp.argtypes = [P, I, dp]
@@ -244,6 +245,13 @@ class Hierarchy:
         e = np.zeros(4)
         self.L.orc_errors(self.h, l, _dp(u), fn, C.byref(co or coeffs()), _dp(e))
         return e
+
+    def gs_sweep(self, l, kind, omega, f, x):
+        """one SOR (kind 2) or SSOR (kind 3) sweep on level l (returns the new x)"""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        x = np.array(x, dtype=np.float64)
+        self.L.orc_gs_sweep(self.h, l, kind, omega, _dp(f), _dp(x))
+        return x
 
     def mcssor_sweep(self, l, omega, f, x):
         """one multicolour SSOR sweep on level l (returns the new x)"""

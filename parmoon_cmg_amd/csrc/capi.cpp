@@ -652,8 +652,8 @@ void coarse64(VC &v, bool zero_start) {
     k_defect(v.P.stream, C, C.sol, C.rhs, C.aux);
     k_gemv_dense(v.P.stream, C.nact, v.P.cinv, C.aux, C.sol);
   } else {
-    k_coarse_smooth(v.P.stream, C, C.sol, C.rhs, C.aux, v.sp.coarse_smoother == 1 ? 1 : 3, v.sp.omega,
-                    v.sp.coarse_maxit, v.sp.coarse_red);
+    k_coarse_smooth(v.P.stream, C, C.sol, C.rhs, C.aux, v.sp.coarse_smoother, v.sp.omega, v.sp.coarse_maxit,
+                    v.sp.coarse_red);
   }
 }
 
@@ -669,10 +669,27 @@ static void ensure_mc(cd3d_problem &P, int l) {
   HIPCHK(hipStreamSynchronize(P.stream));
 }
 
+// the wavefront schedules of the row-order SOR (and, for SSOR, the backward
+// sweep) of level l on the device (first use)
+static void ensure_gs(cd3d_problem &P, int l, bool backward) {
+  DevLevel &D = P.dl[l];
+  for (int b = 0; b <= (backward ? 1 : 0); b++) {
+    if (D.gs_rows[b]) continue;
+    std::vector<int> rows, start;
+    build_gs_schedule(P.hl[l], b == 1, rows, start);
+    D.gs_rows[b] = dalloc<int>(P, rows.size());
+    if (!rows.empty()) upload(P, D.gs_rows[b], rows);
+    D.gs_start[b] = dalloc<int>(P, start.size());
+    upload(P, D.gs_start[b], start);
+    D.gs_nlev[b] = (int)start.size() - 1;
+    HIPCHK(hipStreamSynchronize(P.stream));  // the host vectors end here
+  }
+}
+
 // TMultiGrid3D::Cycle (MultiGrid3D.C:423-669), V-cycle, FP64.  On entry the
 // level's rhs is set; zero_start: the iterate is zero and the Dirichlet part
 // of the rhs is zero, so the first Jacobi sweep needs only the diagonal.
-// Smoother 13 (multicolour SSOR) relaxes L.sol in place.
+// Smoothers 2 (SOR), 3 (SSOR) and 13 (multicolour SSOR) relax L.sol in place.
 void cycle64(VC &v, int i, bool zero_start) {
   cd3d_problem &P = v.P;
   if (i == 0) {
@@ -680,17 +697,23 @@ void cycle64(VC &v, int i, bool zero_start) {
     return;
   }
   DevLevel &L = P.dl[i], &C = P.dl[i - 1];
-  if (v.sp.smoother == 13) {
-    ensure_mc(P, i);
+  const int sm = v.sp.smoother;
+  if (sm == 13 || sm == 2 || sm == 3) {
+    if (sm == 13) ensure_mc(P, i);
+    else ensure_gs(P, i, sm == 3);
+    auto relax = [&]() {
+      if (sm == 13) k_mcssor(P.stream, L, L.sol, L.rhs, v.sp.omega);
+      else k_gs_sweep(P.stream, L, L.sol, L.rhs, v.sp.omega, sm);
+    };
     if (zero_start) HIPCHK(hipMemsetAsync(L.sol, 0, sizeof(double) * L.ndof, P.stream));
-    for (int k = 0; k < v.sp.pre; k++) k_mcssor(P.stream, L, L.sol, L.rhs, v.sp.omega);
+    for (int k = 0; k < v.sp.pre; k++) relax();
     if (v.sp.pre == 0 && zero_start) k_copy_range(P.stream, L.nact, L.ndof, L.rhs, L.sol);
     k_defect(P.stream, L, L.sol, L.rhs, L.aux);
     k_restrict(P.stream, L, C, L.aux, C.rhs);
     for (int r = 0; r < v.rec[i]; r++) cycle64(v, i - 1, r == 0);
     if (v.sp.cycle < 1) v.rec[i] = 1;  // F-cycle
     k_prolong_update(P.stream, L, C, C.sol, L.sol, v.sp.gmg_damp);
-    for (int k = 0; k < v.sp.post; k++) k_mcssor(P.stream, L, L.sol, L.rhs, v.sp.omega);
+    for (int k = 0; k < v.sp.post; k++) relax();
     return;
   }
   double *x = L.sol, *t = L.tmp;
@@ -1289,11 +1312,11 @@ static void batch_solve_impl(BatchRun &br, std::vector<BatchState> &st) {
 }
 
 void check_params(const cd3d_solver_params &sp) {
-  if (sp.smoother != 1 && sp.smoother != 13)
-    throw ArgError("smoother: 1 (damped Jacobi) or 13 (multicolour SSOR); the reference's sequential SOR/SSOR/ILU "
-                   "(2, 3, 4) have no parallel form here");
-  if (sp.coarse_smoother != 17 && sp.coarse_smoother != 3 && sp.coarse_smoother != 1)
-    throw ArgError("coarse_smoother must be 17, 3 or 1");
+  if (sp.smoother != 1 && sp.smoother != 2 && sp.smoother != 3 && sp.smoother != 13)
+    throw ArgError("smoother: 1 (damped Jacobi), 2 (SOR), 3 (SSOR) or 13 (multicolour SSOR); the reference's ILU "
+                   "and reordered SOR variants (4-7) are not provided");
+  if (sp.coarse_smoother != 17 && sp.coarse_smoother != 3 && sp.coarse_smoother != 2 && sp.coarse_smoother != 1)
+    throw ArgError("coarse_smoother must be 17, 3, 2 or 1");
   if (sp.pre < 0 || sp.post < 0 || sp.restart < 1 || sp.maxit < 1) throw ArgError("invalid solver counts");
   if (sp.precision != CD3D_FP64 && sp.precision != CD3D_MIXED) throw ArgError("precision must be 0 or 1");
   if (sp.cycle < 0 || sp.cycle > 2) throw ArgError("cycle must be 0 (F), 1 (V) or 2 (W)");

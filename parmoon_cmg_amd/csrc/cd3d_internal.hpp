@@ -99,6 +99,10 @@ struct DevLevel {
   // multicolour SSOR row lists (uploaded on first use)
   int *mc_rows = nullptr;
   std::vector<int> mc_start;
+  // SOR / SSOR in row order: forward [0] and backward [1] wavefront schedules
+  // (build_gs_schedule; uploaded on first use)
+  int *gs_rows[2] = {nullptr, nullptr}, *gs_start[2] = {nullptr, nullptr};
+  int gs_nlev[2] = {0, 0};
   // assembly
   int4 *cells = nullptr;
   int *slots = nullptr;
@@ -232,6 +236,7 @@ void build_gather(const HostLevel &L, int order, const std::vector<int> &slots, 
 void build_colours(HostLevel &L, int order);
 void build_row_blocks(const std::vector<int> &rowptr, int max_nnz, int max_rows, std::vector<int> &blk);
 void build_mc_colours(HostLevel &L, int order);
+void build_gs_schedule(const HostLevel &L, bool backward, std::vector<int> &rows, std::vector<int> &start);
 void build_lattice(HostLevel &L, int order);
 void basis_tables(int order, std::vector<double> &tab);
 int quad_points_1d(int order);
@@ -272,6 +277,9 @@ void k_jacobi_zero(hipStream_t s, const DevLevel &L, const double *f, double *ou
 // one multicolour SSOR sweep in place (smoother 13): x_D = f_D, relaxed
 // Gauss-Seidel colour by colour forward, then backward (needs mc_rows)
 void k_mcssor(hipStream_t s, const DevLevel &L, double *x, const double *f, double omega);
+// one SOR (kind 2) or SSOR (kind 3) sweep in the reference's row order, in
+// place: x_D = f_D, then the wavefront schedule(s) of the level
+void k_gs_sweep(hipStream_t s, const DevLevel &L, double *x, const double *f, double omega, int kind);
 void k_invdiag(hipStream_t s, const DevLevel &L);
 void k_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, const double *fine, double *coarse);
 void k_prolong_update(hipStream_t s, const DevLevel &F, const DevLevel &C, const double *coarse, double *x,

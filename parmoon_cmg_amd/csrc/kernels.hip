@@ -632,7 +632,7 @@ __global__ __launch_bounds__(kBlock) void coarse_lat_k(int na, const double *__r
 }
 
 // Coarse-level smoothing loop of TMultiGrid3D::Smooth(CoarseSmooth)
-// (MultiGrid3D.C:197-260) with SSOR (MGLevel3D.C:337-441) or Jacobi sweeps,
+// (MultiGrid3D.C:197-260) with SOR, SSOR (MGLevel3D.C:254-441) or Jacobi sweeps,
 // run by a single wavefront: rows are updated one after another, each row's
 // dot product spread over the 64 lanes; the iterate lives in LDS.
 __global__ __launch_bounds__(64) void coarse_smooth_k(int n, int nact, const int *__restrict__ rp,
@@ -681,7 +681,7 @@ __global__ __launch_bounds__(64) void coarse_smooth_k(int n, int nact, const int
         if (lane == 0) x[i] = (1 - omega) * aux[i] + omega * q;
       }
       __syncthreads();
-    } else {  // SSOR: Dirichlet rows = f, forward then backward relaxed Gauss-Seidel
+    } else {  // SOR / SSOR: Dirichlet rows = f, forward (then, SSOR, backward) relaxed Gauss-Seidel
       for (int i = nact + lane; i < n; i += 64) x[i] = f[i];
       __syncthreads();
       for (int i = 0; i < nact; i++) {
@@ -690,7 +690,7 @@ __global__ __launch_bounds__(64) void coarse_smooth_k(int n, int nact, const int
         if (lane == 0) x[i] = omega * (q - x[i]) + x[i];
         __syncthreads();
       }
-      for (int i = nact - 1; i >= 0; i--) {
+      for (int i = smoother == 3 ? nact - 1 : -1; i >= 0; i--) {
         const double q = row_update(i, x);
         __syncthreads();
         if (lane == 0) x[i] = omega * (q - x[i]) + x[i];
@@ -1431,6 +1431,86 @@ __global__ void dirichlet_rows_k(int n, int nact, const int *__restrict__ rp, do
   rhs[r] = 0.0;      // AddLocalRhsToGlobal: Dirichlet value g = 0
 }
 
+// Smoothers 2 and 3 in the reference's row order (TMGLevel3D::SOR / SSOR,
+// MGLevel3D.C:254-441).  The rows come grouped by wavefront level
+// (build_gs_schedule): every row of a level reads only rows of lower levels
+// (new values) or of higher levels (old values), exactly what the sequential
+// sweep reads, so a level's rows update at once.  One workgroup walks the
+// levels with a barrier between them; each wave takes one row at a time,
+// its lanes over the row's nonzeros.  Every wave reaches every barrier (the
+// level loop is uniform).  The sweep is as sequential as the reference's:
+// levels hold a handful of rows (about 8 on the 32^3-cell Q2 level).  Only
+// the gather of x depends on the previous level, so each wave fetches its
+// row of the next level (index, extent, the first 128 entries, f_i, x_i)
+// while it works on the current one.
+constexpr int kGsBlock = 1024;
+struct GsRow {
+  int i, b, e, c0, c1;
+  double v0, v1, fi, xi;
+};
+__device__ inline void gs_fetch(int r, int end, const int *__restrict__ rows, const int *__restrict__ rp,
+                                const int *__restrict__ col, const double *__restrict__ val,
+                                const double *x, const double *__restrict__ f, int lane, GsRow &q) {
+  q.i = -1;
+  if (r >= end) return;
+  q.i = rows[r];
+  q.b = rp[q.i];
+  q.e = rp[q.i + 1];
+  const int j0 = q.b + lane, j1 = j0 + 64;
+  q.c0 = j0 < q.e ? col[j0] : -1;
+  q.v0 = j0 < q.e ? val[j0] : 0.0;
+  q.c1 = j1 < q.e ? col[j1] : -1;
+  q.v1 = j1 < q.e ? val[j1] : 0.0;
+  q.fi = f[q.i];
+  q.xi = x[q.i];  // written only by this row's own update
+}
+__device__ inline void gs_update(const GsRow &q, const int *__restrict__ col, const double *__restrict__ val,
+                                 double *x, double omega, int lane) {
+  double s = 0.0, diag = 0.0;
+  if (q.c0 >= 0) {
+    if (q.c0 == q.i) diag = q.v0;
+    else s += q.v0 * x[q.c0];
+  }
+  if (q.c1 >= 0) {
+    if (q.c1 == q.i) diag = q.v1;
+    else s += q.v1 * x[q.c1];
+  }
+  for (int j = q.b + 128 + lane; j < q.e; j += 64) {  // rows longer than 128 entries
+    const int c = col[j];
+    if (c == q.i) diag = val[j];
+    else s += val[j] * x[c];
+  }
+  s = group_sum<64>(s);
+  diag = group_sum<64>(diag);
+  if (lane == 0) x[q.i] = omega * ((q.fi - s) / diag - q.xi) + q.xi;
+}
+__global__ __launch_bounds__(kGsBlock) void gs_sweep_k(int nlev, const int *__restrict__ start,
+                                                       const int *__restrict__ rows, const int *__restrict__ rp,
+                                                       const int *__restrict__ col, const double *__restrict__ val,
+                                                       double *x, const double *__restrict__ f, double omega) {
+  constexpr int W = kGsBlock / 64;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (nlev <= 0) return;
+  int sb = start[0], se = start[1];
+  GsRow cur, nxt;
+  gs_fetch(sb + w, se, rows, rp, col, val, x, f, lane, cur);
+  for (int k = 0; k < nlev; k++) {
+    const int ne = k + 2 <= nlev ? start[k + 2] : se;
+    // the next level's row: x of it is old until its own update
+    gs_fetch(se + w, ne, rows, rp, col, val, x, f, lane, nxt);
+    if (cur.i >= 0) gs_update(cur, col, val, x, omega, lane);
+    for (int r = sb + w + W; r < se; r += W) {  // levels with more rows than waves
+      GsRow q;
+      gs_fetch(r, se, rows, rp, col, val, x, f, lane, q);
+      gs_update(q, col, val, x, omega, lane);
+    }
+    __syncthreads();
+    cur = nxt;
+    sb = se;
+    se = ne;
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
@@ -1487,6 +1567,13 @@ void k_mcssor(hipStream_t s, const DevLevel &L, double *x, const double *f, doub
       default: colour_sor_k<32, 4><<<grid, kBlock, 0, s>>>(n, rows, L.rowptr, L.col, L.val, x, f, omega); break;
     }
   }
+}
+void k_gs_sweep(hipStream_t s, const DevLevel &L, double *x, const double *f, double omega, int kind) {
+  k_copy_range(s, L.nact, L.ndof, f, x);
+  gs_sweep_k<<<1, kGsBlock, 0, s>>>(L.gs_nlev[0], L.gs_start[0], L.gs_rows[0], L.rowptr, L.col, L.val, x, f, omega);
+  if (kind == 3)
+    gs_sweep_k<<<1, kGsBlock, 0, s>>>(L.gs_nlev[1], L.gs_start[1], L.gs_rows[1], L.rowptr, L.col, L.val, x, f,
+                                      omega);
 }
 void k_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, const double *fine, double *coarse) {
   launch_rows<kRestrict, double, double>(s, F.r_nnz, C.ndof, C.nact, F.r_rowptr, F.r_col,

@@ -288,6 +288,38 @@ void build_row_blocks(const std::vector<int> &rowptr, int max_nnz, int max_rows,
   }
 }
 
+// Wavefront schedule of the reference's sequential SOR sweep over the active
+// rows (TMGLevel3D::SOR / SSOR, MGLevel3D.C:254-441): forward, row i reads
+// the new values of its lower-numbered neighbours and the old values of the
+// higher-numbered ones; backward (the second half of SSOR), the other way
+// round.  level(i) = 1 + the largest level of i's earlier-visited active
+// neighbours, so that with the symmetric pattern of the level matrices every
+// row of a level depends only on rows of lower levels and the rows of one
+// level can update at once with the values the sequential sweep would read.
+// rows: the active rows by level, ascending within a level; start: nlev + 1
+// offsets.
+void build_gs_schedule(const HostLevel &L, bool backward, std::vector<int> &rows, std::vector<int> &start) {
+  const int na = L.nact;
+  std::vector<int> lev(na, 0);
+  int nlev = 0;
+  for (int t = 0; t < na; t++) {
+    const int i = backward ? na - 1 - t : t;
+    int m = -1;
+    for (int j = L.rowptr[i]; j < L.rowptr[i + 1]; j++) {
+      const int c = L.kcol[j];
+      if (c < na && (backward ? c > i : c < i)) m = std::max(m, lev[c]);
+    }
+    lev[i] = m + 1;
+    nlev = std::max(nlev, m + 2);
+  }
+  start.assign(nlev + 1, 0);
+  for (int i = 0; i < na; i++) start[lev[i] + 1]++;
+  for (int k = 0; k < nlev; k++) start[k + 1] += start[k];
+  rows.assign(na, 0);
+  std::vector<int> pos(start.begin(), start.end() - 1);
+  for (int i = 0; i < na; i++) rows[pos[lev[i]]++] = i;
+}
+
 // Grid transfer between coarse level and the next finer one: P(i,k) is the
 // value of coarse basis function k at fine node i -- what Prolongate
 // computes cell by cell with the prolongation matrices and the multiplicity

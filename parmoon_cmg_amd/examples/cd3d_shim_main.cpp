@@ -86,7 +86,7 @@ int main(int argc, char **argv) {
   TParamDB &db = *TDatabase::ParamDB;
   db.SC_SOLVER_SCALAR = 16;
   db.SC_PRECONDITIONER_SCALAR = 5;
-  db.SC_SMOOTHER_SCALAR = 1;
+  db.SC_SMOOTHER_SCALAR = (int)num("smoother", 1);
   db.SC_PRE_SMOOTH_SCALAR = (int)num("pre", 2);
   db.SC_POST_SMOOTH_SCALAR = (int)num("post", 2);
   db.SC_SMOOTH_DAMP_FACTOR_SCALAR = num("omega", 0.8);

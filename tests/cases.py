@@ -42,6 +42,13 @@ CD_CASES = {
                              co=dict(eps=1e-3, conv_kind=1, rot_c=(0.45, 0.55), f_val=1.0, supg=1)),
     "m2_q2_supg_rot": dict(order=2, m=2, k0=0, levels=3, box=UNIT_BOX, solver=dict(omega=0.5, pre=3, post=3),
                            co=dict(eps=1e-4, conv_kind=1, rot_c=(0.45, 0.55), c=0.2, f_val=1.0, supg=1)),
+    # SOR / SSOR smoothers in the reference's row order (SC_SMOOTHER_SCALAR 2, 3)
+    "m2_q1_ssor": dict(order=1, m=2, k0=0, levels=3, box=UNIT_BOX, solver=dict(smoother=3, pre=3, post=3, omega=1.0),
+                       co=dict(eps=1e-2, b=(0.3, 0.4, -0.5), f_val=1.0, supg=1)),
+    "unit_q2_sor": dict(order=2, k0=1, levels=3, box=UNIT_BOX, solver=dict(smoother=2, coarse_smoother=2, omega=1.0),
+                        co=dict(eps=1e-3, b=(1.0, 0.5, 0.25), f_val=1.0, supg=1)),
+    "m2_q2_ssor_w12": dict(order=2, m=2, k0=0, levels=2, box=UNIT_BOX, solver=dict(smoother=3, omega=1.2),
+                           co=dict(eps=1e-2, b=(-0.4, 0.6, 0.3), c=0.5, f_val=1.0, supg=1)),
     # W- and F-cycle preconditioners
     "m2_q1_wcycle": dict(order=1, m=2, k0=0, levels=4, box=UNIT_BOX, solver=dict(cycle=2),
                          co=dict(eps=1e-2, b=(0.3, 0.4, -0.5), f_val=1.0, supg=1)),

@@ -101,6 +101,24 @@ CASES = {
     "m2_q1_fcycle": ("cd", dict(geo="m2", ANSATZ_ORDER=1, LEVELS=4, UNIFORM_STEPS=0, DISCTYPE=2,
                                 CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0, SC_MG_CYCLE_SCALAR=0,
                                 P0=1e-2, P1=0.3, P2=0.4, P3=-0.5, P4=0, P5=0, P6=1.0)),
+    # SOR and SSOR smoothers in the reference's row order (SC_SMOOTHER_SCALAR
+    # 2 and 3, MGLevel3D.C:254-441), as the reference's ReadIn/cd3d.dat (SSOR
+    # 3 + 3, omega 1) and ReadIn/tcd3d.dat (SOR 2 + 2) set them; SOR also as
+    # the coarse smoother
+    "m2_q1_ssor": ("cd", dict(geo="m2", ANSATZ_ORDER=1, LEVELS=3, UNIFORM_STEPS=0, DISCTYPE=2,
+                              CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0, SC_SMOOTHER_SCALAR=3,
+                              SC_PRE_SMOOTH_SCALAR=3, SC_POST_SMOOTH_SCALAR=3,
+                              SC_SMOOTH_DAMP_FACTOR_SCALAR=1.0, SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=1.0,
+                              P0=1e-2, P1=0.3, P2=0.4, P3=-0.5, P4=0, P5=0, P6=1.0)),
+    "unit_q2_sor": ("cd", dict(geo="unit", ANSATZ_ORDER=2, LEVELS=3, UNIFORM_STEPS=1, DISCTYPE=2,
+                               CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0, SC_SMOOTHER_SCALAR=2,
+                               SC_COARSE_SMOOTHER_SCALAR=2,
+                               SC_SMOOTH_DAMP_FACTOR_SCALAR=1.0, SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=1.0,
+                               P0=1e-3, P1=1.0, P2=0.5, P3=0.25, P4=0, P5=0, P6=1.0)),
+    "m2_q2_ssor_w12": ("cd", dict(geo="m2", ANSATZ_ORDER=2, LEVELS=2, UNIFORM_STEPS=0, DISCTYPE=2,
+                                  CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0, SC_SMOOTHER_SCALAR=3,
+                                  SC_SMOOTH_DAMP_FACTOR_SCALAR=1.2, SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=1.2,
+                                  P0=1e-2, P1=-0.4, P2=0.6, P3=0.3, P4=0.5, P5=0, P6=1.0)),
     # the smoother of the GPU benchmark (damped Jacobi omega 0.5, 3 + 3 sweeps)
     "unit_c2_n8_bench": ("cd", dict(geo="unit", ANSATZ_ORDER=2, LEVELS=3, UNIFORM_STEPS=1, DISCTYPE=2,
                                     CELL_MEASURE=4, SDFEM_TYPE=2, DELTA0=1.0,

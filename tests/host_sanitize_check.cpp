@@ -49,6 +49,34 @@ static void check_level(const HostLevel &L, int order) {
   for (int s : slots) EXPECT(s >= -1 && s < L.nnz);
 }
 
+// the wavefront schedules of the row-order SOR: each active row once; every
+// neighbour visited before it in the sequential sweep sits on a lower level
+// and every one visited after it on a higher level (so a level's rows read
+// exactly the values the sequential sweep reads)
+static void check_gs(const HostLevel &L) {
+  for (int b = 0; b < 2; b++) {
+    std::vector<int> rows, start;
+    build_gs_schedule(L, b == 1, rows, start);
+    EXPECT((int)rows.size() == L.nact && !start.empty() && start.back() == L.nact);
+    std::vector<int> lev(L.nact, -1);
+    for (int k = 0; k + 1 < (int)start.size(); k++)
+      for (int r = start[k]; r < start[k + 1]; r++) {
+        EXPECT(rows[r] >= 0 && rows[r] < L.nact);
+        if (rows[r] >= 0 && rows[r] < L.nact) {
+          EXPECT(lev[rows[r]] == -1);
+          lev[rows[r]] = k;
+        }
+      }
+    for (int i = 0; i < L.nact; i++)
+      for (int j = L.rowptr[i]; j < L.rowptr[i + 1]; j++) {
+        const int c = L.kcol[j];
+        if (c >= L.nact || c == i) continue;
+        const bool before = b == 0 ? c < i : c > i;
+        EXPECT(before ? lev[c] < lev[i] : lev[c] > lev[i]);
+      }
+  }
+}
+
 // the gather lists of the two-pass assembly give, entry by entry, the sums of
 // the colour-by-colour scatter bit for bit
 static void check_gather(const HostLevel &L, int order) {
@@ -115,6 +143,8 @@ int main(int argc, char **argv) {
         check_level(C, order);
         check_level(F, order);
         if (depth <= 1) check_gather(F, order);
+        check_gs(C);
+        check_gs(F);
         for (int c : F.p_col) EXPECT(c >= 0 && c < C.ndof);
         for (int c : F.r_col) EXPECT(c >= 0 && c < F.ndof);
         EXPECT((int)F.mc_rows.size() == F.nact);

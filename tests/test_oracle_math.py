@@ -253,6 +253,41 @@ def test_multicolour_ssor_matches_definition(order, m, k0, levels):
     assert np.abs(xo - x).max() <= 1e-12 * np.abs(x).max()
 
 
+@pytest.mark.parametrize("order,m,k0,levels", [(1, 2, 1, 2), (2, 1, 1, 2)])
+@pytest.mark.parametrize("kind,omega", [(2, 1.0), (2, 1.3), (3, 0.9)])
+def test_sor_and_ssor_match_the_matrix_splitting(order, m, k0, levels, kind, omega):
+    # smoothers 2 and 3 (TMGLevel3D::SOR / SSOR) against their matrix form:
+    # with x_D = f_D and b = f_A - A_AD x_D, one forward SOR sweep solves
+    # (D + w L) x' = w b - (w U + (w - 1) D) x, L / U the strictly lower /
+    # upper parts of A_AA in DOF order; SSOR follows it by the backward sweep
+    # (D + w U) x'' = w b - (w L + (w - 1) D) x'.  Dense triangular solves,
+    # not the row loop, so a wrong sweep direction or a stale neighbour fails.
+    # Also the symmetric pattern the device's wavefront order rests on.
+    import scipy.linalg as sla
+    H = po.Hierarchy(order, m, k0, levels, box=(1.0, 1.0, 1.0))
+    H.assemble(po.coeffs(eps=1e-3, b=(0.7, -0.4, 0.5), c=0.3, f_val=1.0, supg=1))
+    l = levels - 1
+    rp, kc, e = H.csr(l)
+    n, na = len(rp) - 1, H.info(l)["nact"]
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, kc[rp[i]:rp[i + 1]]] = e[rp[i]:rp[i + 1]]
+    P = A[:na, :na] != 0
+    assert np.array_equal(P, P.T)
+    rng = np.random.default_rng(11)
+    f, x0 = rng.standard_normal(n), rng.standard_normal(n)
+    xD = f[na:]
+    b = f[:na] - A[:na, na:] @ xD
+    AA = A[:na, :na]
+    D, Lo, Up = np.diag(np.diag(AA)), np.tril(AA, -1), np.triu(AA, 1)
+    x = sla.solve_triangular(D + omega * Lo, omega * b - (omega * Up + (omega - 1) * D) @ x0[:na], lower=True)
+    if kind == 3:
+        x = sla.solve_triangular(D + omega * Up, omega * b - (omega * Lo + (omega - 1) * D) @ x, lower=False)
+    xo = H.gs_sweep(l, kind, omega, f, x0)
+    assert np.array_equal(xo[na:], xD)
+    assert np.abs(xo[:na] - x).max() <= 1e-11 * np.abs(x).max()
+
+
 def test_error_norms_closed_forms_and_rates():
     # |sin(pi x) sin(pi y) sin(pi z)|_L2 = (1/2)^(3/2), |.|_H1 = pi sqrt(3/8) on
     # the unit cube (errors of u_h = 0); the nodal interpolant's L2 error falls
