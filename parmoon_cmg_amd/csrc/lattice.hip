@@ -838,11 +838,28 @@ __host__ __device__ inline void restrict_pair_node(const LatArgs &ca, int pc, lo
   }
 }
 
+// inter: the four pair classes' blocks interleaved (logical block 4 j + pc
+// is block j of class pc) and dealt to the XCDs in contiguous ranges, so the
+// fine lines that coarse lines of all four parities share are read on one
+// XCD at about the same time; otherwise class after class.  Measured on the
+// 129^3 -> 65^3-node restriction: interleaved, the L2 fetches 10.3 instead
+// of 22.2 MB (algorithmic 9.7 MB) but the kernel takes 6.5 instead of
+// 6.0 us (the class-order re-reads come from the Infinity Cache), and the
+// mixed solve is unchanged; class order is the default
+// (CD3D_RESTRICT_INTER=1 selects the other, bitwise the same)
 __global__ __launch_bounds__(kB) void lat_restrict_pair_k(LatArgs ca, int nnf, const float *__restrict__ fine,
-                                                          float *__restrict__ coarse) {
-  int pc = 0;
-  while (pc < 3 && (int)blockIdx.x >= ca.pstart[pc + 1]) pc++;
-  const long long k = (long long)(blockIdx.x - ca.pstart[pc]) * kB + threadIdx.x;
+                                                          float *__restrict__ coarse, int inter) {
+  int pc = 0, j;
+  if (inter) {
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    pc = lb & 3;
+    j = lb >> 2;
+    if (j >= ca.pstart[pc + 1] - ca.pstart[pc]) return;
+  } else {
+    while (pc < 3 && (int)blockIdx.x >= ca.pstart[pc + 1]) pc++;
+    j = (int)blockIdx.x - ca.pstart[pc];
+  }
+  const long long k = (long long)j * kB + threadIdx.x;
   if (k < ca.c[2 * pc + 1].nodes) restrict_pair_node(ca, pc, k, nnf, fine, coarse);
 }
 
@@ -1930,7 +1947,13 @@ void kl_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, int order,
   const int grid = ca.bstart[C.nclass];
   if (grid <= 0) return;
   if (order == 1) lat_restrict_k<1><<<grid, kB, 0, s>>>(ca, F.nn, fine, coarse);
-  else if (lat_pairs() && C.nn > kPairMinNN) lat_restrict_pair_k<<<ca.pstart[4], kB, 0, s>>>(ca, F.nn, fine, coarse);
+  else if (lat_pairs() && C.nn > kPairMinNN) {
+    const char *e = getenv("CD3D_RESTRICT_INTER");
+    const int inter = e ? atoi(e) : 0;
+    int nbmax = 0;
+    for (int pc = 0; pc < 4; pc++) nbmax = std::max(nbmax, ca.pstart[pc + 1] - ca.pstart[pc]);
+    lat_restrict_pair_k<<<inter ? 4 * nbmax : ca.pstart[4], kB, 0, s>>>(ca, F.nn, fine, coarse, inter);
+  }
   else lat_restrict_k<2><<<grid, kB, 0, s>>>(ca, F.nn, fine, coarse);
 }
 void kl_prolong_update(hipStream_t s, const DevLevel &F, const DevLevel &C, int order, const float *coarse, float *x,

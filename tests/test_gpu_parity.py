@@ -416,15 +416,17 @@ def test_pair_groups_match_single_pairs(m, k0, levels, monkeypatch):
     # the 4 lines: CD3D_QUAD_G) or, with CD3D_LAT_QUADS=0, by pair class
     # (lat_pair_gk)
     out = {}
-    for g in ("1", "2", "4", "2c", "4c", "2q1", "4q4"):
+    # ("2r": the restriction's pair classes interleaved, CD3D_RESTRICT_INTER)
+    for g in ("1", "2", "4", "2c", "4c", "2q1", "4q4", "2r"):
         monkeypatch.setenv("CD3D_PAIR_GROUP", g[0])
         monkeypatch.setenv("CD3D_LAT_QUADS", "0" if g.endswith("c") else "1")
         monkeypatch.setenv("CD3D_QUAD_G", g[2] if "q" in g else "2")
+        monkeypatch.setenv("CD3D_RESTRICT_INTER", "1" if g.endswith("r") else "0")
         z = P.vcycle(sp, r)
         x, st = P.solve(sp)
         out[g] = (z, x, st.iterations)
     say(f"[gpu] pair groups m={m} k0={k0} levels={levels}: its {[out[g][2] for g in out]}")
-    for g in ("2", "4", "2c", "4c", "2q1", "4q4"):
+    for g in ("2", "4", "2c", "4c", "2q1", "4q4", "2r"):
         assert np.array_equal(out[g][0], out["1"][0])
         assert np.array_equal(out[g][1], out["1"][1])
         assert out[g][2] == out["1"][2]
