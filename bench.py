@@ -322,12 +322,12 @@ def batched_uneven(pm, precision, count=256, seed=20261202, passes=2, inflight=1
 
 
 def c4_form(pm, precision, args):
-    """Configuration 4 in one precision form: the FP64 form runs same-level
-    batches (all problems with the same level count and solver in lockstep,
+    """Configuration 4 in one precision form as same-level batches (all
+    problems with the same level count and solver in lockstep,
     cd3d_solve_batch: 256 slots, 4 worker threads, up to --c4-batch per
-    batch); the mixed form, which batches do not cover, one request per
-    slot."""
-    if precision == pm.FP64 and args.c4_batch > 1:
+    batch; the mixed form's batches run an FP32 V-cycle over the CSR
+    levels), or with --c4-batch 1 one request per slot."""
+    if args.c4_batch > 1:
         return batched_uneven(pm, precision, args.c4_problems, inflight=256, max_batch=args.c4_batch, workers=4)
     return batched_uneven(pm, precision, args.c4_problems, inflight=args.c4_inflight, max_batch=1)
 
@@ -489,9 +489,10 @@ def main():
         for name, prec in (("fp64", pm.FP64), ("mixed", pm.MIXED)):
             log(f"[bench] configuration 4 ({args.c4_problems} batched uneven Q1 problems), {name}")
             extra.setdefault("batched_uneven_q1", {})[name] = c4_form(pm, prec, args)
-        log("[bench] configuration 4, FP64 one request at a time (no batching)")
-        extra["batched_uneven_q1"]["fp64_unbatched"] = batched_uneven(pm, pm.FP64, args.c4_problems,
-                                                                      inflight=args.c4_inflight, max_batch=1)
+        for name, prec in (("fp64", pm.FP64), ("mixed", pm.MIXED)):
+            log(f"[bench] configuration 4, {name} one request per slot (no batching)")
+            extra["batched_uneven_q1"][f"{name}_unbatched"] = batched_uneven(pm, prec, args.c4_problems,
+                                                                             inflight=args.c4_inflight, max_batch=1)
         rates = [float(x) for x in args.c5_rates.split(",") if x]
         for name, prec in (("fp64", pm.FP64), ("mixed", pm.MIXED)):
             log(f"[bench] configuration 5 (Poisson serving stream), {name}")

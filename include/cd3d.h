@@ -196,14 +196,19 @@ cd3d_status cd3d_solve(cd3d_problem *p, const cd3d_solver_params *sp, const doub
  * problem per process): `count` assembled problems with the same number of
  * levels on one device -- of any sizes, element orders and coefficients --
  * solved together, each as cd3d_solve(p, sp, NULL, NULL, 0, &stats[b])
- * would: the FP64 V-cycle (damped Jacobi, exact coarse solve) inside
+ * would: the V-cycle (damped Jacobi, exact coarse solve) inside FP64
  * FGMRES, in lockstep, one launch per step for all problems still iterating
  * (kernels index each problem's arrays from device tables) and one host
  * transfer of all H columns per iteration.  Each problem keeps its own
- * stopping test; iteration counts equal a lone solve's, solutions agree to
- * rounding (the reduction trees differ).  Other parameters (mixed precision,
- * smoothers other than 1, W/F cycles, coarse smoothers other than 17, no pre-smoothing,
- * restart > 200) or count 1: the problems are solved one after another.
+ * stopping test.  FP64: iteration counts equal a lone solve's, solutions
+ * agree to rounding (the reduction trees differ).  Mixed precision: the
+ * V-cycle runs on FP32 level vectors over the CSR matrices rounded to FP32
+ * as they are read (the lone mixed solve uses the lattice form instead), to
+ * the same FP64 residual test; iteration counts may differ from a lone
+ * mixed solve's by the rounding of the preconditioner.  Other parameters
+ * (smoothers other than 1, W/F cycles, coarse smoothers other than 17, no
+ * pre-smoothing, restart > 200) or count 1: the problems are solved one
+ * after another.
  * Runs on the first handle's stream; every handle is locked meanwhile.
  * stats (count entries, may be NULL) receives each problem's result; the
  * return value is CD3D_OK or the first non-OK status. */

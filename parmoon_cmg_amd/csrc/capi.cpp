@@ -1039,32 +1039,35 @@ static void batch_workspace(cd3d_problem &P, int B, int NL) {
 }
 
 // one V-cycle of all active problems, z = V(r) with the finest level's rhs r
-// (zero on Dirichlet rows), as precondition / cycle64 / coarse64
+// (zero on Dirichlet rows), as precondition / cycle64 / coarse64; in the
+// mixed form on the FP32 level vectors (the matrices rounded to FP32 as they
+// are read, the coarse solve accumulated in FP64)
 static void batch_cycle(BatchRun &br, int i) {
   const cd3d_solver_params &sp = br.sp;
   const int B = br.B;
+  const bool f32 = sp.precision == CD3D_MIXED;
   if (i == 0) {
-    kb_fill_zero(br.s, B, br.maxn[0], br.lev(0), br.prob(), br.act(), kBSol);
-    kb_copy(br.s, B, br.maxn[0], br.lev(0), br.prob(), br.act(), kBRhs, kBSol, true);
-    kb_stream(br.s, kDefectMode, B, br.maxblk[0], br.lev(0), br.prob(), br.act(), kBSol, kBRhs, kBAux, 0.0);
-    kb_gemv_coarse(br.s, B, br.maxna, br.lev(0), br.prob(), br.act(), kBAux, kBSol);
+    kb_fill_zero(br.s, B, br.maxn[0], br.lev(0), br.prob(), br.act(), kBSol, f32);
+    kb_copy(br.s, B, br.maxn[0], br.lev(0), br.prob(), br.act(), kBRhs, kBSol, true, f32);
+    kb_stream(br.s, kDefectMode, B, br.maxblk[0], br.lev(0), br.prob(), br.act(), kBSol, kBRhs, kBAux, 0.0, f32);
+    kb_gemv_coarse(br.s, B, br.maxna, br.lev(0), br.prob(), br.act(), kBAux, kBSol, f32);
     return;
   }
   int x = kBSol, t = kBTmp;
   for (int k = 0; k < sp.pre; k++) {
-    if (k == 0) kb_jacobi_zero(br.s, B, br.maxn[i], br.lev(i), br.prob(), br.act(), kBRhs, t, sp.omega);
-    else kb_stream(br.s, kJacobiMode, B, br.maxblk[i], br.lev(i), br.prob(), br.act(), x, kBRhs, t, sp.omega);
+    if (k == 0) kb_jacobi_zero(br.s, B, br.maxn[i], br.lev(i), br.prob(), br.act(), kBRhs, t, sp.omega, f32);
+    else kb_stream(br.s, kJacobiMode, B, br.maxblk[i], br.lev(i), br.prob(), br.act(), x, kBRhs, t, sp.omega, f32);
     std::swap(x, t);
   }
-  kb_stream(br.s, kDefectMode, B, br.maxblk[i], br.lev(i), br.prob(), br.act(), x, kBRhs, kBAux, 0.0);
-  kb_restrict(br.s, B, br.maxn_c[i], br.lev(i), br.prob(), br.act(), kBAux, kBRhs, br.lev(i - 1));
+  kb_stream(br.s, kDefectMode, B, br.maxblk[i], br.lev(i), br.prob(), br.act(), x, kBRhs, kBAux, 0.0, f32);
+  kb_restrict(br.s, B, br.maxn_c[i], br.lev(i), br.prob(), br.act(), kBAux, kBRhs, br.lev(i - 1), f32);
   batch_cycle(br, i - 1);
-  kb_prolong_update(br.s, B, br.maxn[i], br.lev(i), br.prob(), br.act(), br.lev(i - 1), kBSol, x, sp.gmg_damp);
+  kb_prolong_update(br.s, B, br.maxn[i], br.lev(i), br.prob(), br.act(), br.lev(i - 1), kBSol, x, sp.gmg_damp, f32);
   for (int k = 0; k < sp.post; k++) {
-    kb_stream(br.s, kJacobiMode, B, br.maxblk[i], br.lev(i), br.prob(), br.act(), x, kBRhs, t, sp.omega);
+    kb_stream(br.s, kJacobiMode, B, br.maxblk[i], br.lev(i), br.prob(), br.act(), x, kBRhs, t, sp.omega, f32);
     std::swap(x, t);
   }
-  if (x != kBSol) kb_copy(br.s, B, br.maxn[i], br.lev(i), br.prob(), br.act(), x, kBSol, false);
+  if (x != kBSol) kb_copy(br.s, B, br.maxn[i], br.lev(i), br.prob(), br.act(), x, kBSol, false, f32);
 }
 
 // the stopping state of one problem of a batch
@@ -1118,6 +1121,11 @@ static void batch_solve_impl(BatchRun &br, std::vector<BatchState> &st) {
       L.vec[1] = D.rhs;
       L.vec[2] = D.aux;
       L.vec[3] = D.tmp;
+      if (sp.precision == CD3D_MIXED) {
+        if (!D.b32) D.b32 = dalloc<float>(P, 5 * (size_t)D.ndof);
+        for (int k = 0; k < 4; k++) L.vec32[k] = D.b32 + (size_t)k * D.ndof;
+        L.invdiag32 = D.b32 + 4 * (size_t)D.ndof;
+      }
       if (l > 0) {
         L.p_rowptr = D.p_rowptr;
         L.p_col = D.p_col;
@@ -1153,6 +1161,8 @@ static void batch_solve_impl(BatchRun &br, std::vector<BatchState> &st) {
   }
   HIPCHK(hipMemcpyAsync(w.dlev, hl.data(), sizeof(BatchLevel) * hl.size(), hipMemcpyHostToDevice, br.s));
   HIPCHK(hipMemcpyAsync(w.dprob, hp.data(), sizeof(BatchProblem) * B, hipMemcpyHostToDevice, br.s));
+  if (sp.precision == CD3D_MIXED)
+    for (int l = 0; l < NL; l++) kb_invdiag32(br.s, B, br.maxn[l], br.lev(l));
   const BatchLevel *FL = br.lev(top);
   auto upload_active = [&] {
     for (int b = 0; b < B; b++) w.hactive[b] = st[b].active ? 1 : 0;
@@ -1238,9 +1248,15 @@ static void batch_solve_impl(BatchRun &br, std::vector<BatchState> &st) {
     int i;
     for (i = 0; i < R && j <= sp.maxit && any_active(); i++, j++) {
       // z_i = V(v_i), w = A z_i, modified Gram-Schmidt, v_{i+1} = w / |w|
-      kb_copy(br.s, B, br.maxn[top], FL, br.prob(), br.act(), kBV0 + i, kBRhs, false);
-      batch_cycle(br, top);
-      kb_copy(br.s, B, br.maxn[top], FL, br.prob(), br.act(), kBSol, kBZ0 + i, false);
+      if (sp.precision == CD3D_MIXED) {
+        kb_to32(br.s, B, br.maxn[top], FL, br.prob(), br.act(), kBV0 + i, kBRhs);
+        batch_cycle(br, top);
+        kb_from32(br.s, B, br.maxn[top], FL, br.prob(), br.act(), kBSol, kBZ0 + i);
+      } else {
+        kb_copy(br.s, B, br.maxn[top], FL, br.prob(), br.act(), kBV0 + i, kBRhs, false);
+        batch_cycle(br, top);
+        kb_copy(br.s, B, br.maxn[top], FL, br.prob(), br.act(), kBSol, kBZ0 + i, false);
+      }
       kb_stream(br.s, kSpmvMode, B, br.maxblk[top], FL, br.prob(), br.act(), kBZ0 + i, -1, kBW, 0.0);
       kb_dot(br.s, B, FL, br.prob(), br.act(), kBW, kBV0, 0);
       for (int k = 0; k <= i; k++)
@@ -1746,7 +1762,8 @@ cd3d_status cd3d_solve_batch(cd3d_problem **ps, int count, const cd3d_solver_par
   std::sort(order.begin(), order.end());
   if (std::adjacent_find(order.begin(), order.end()) != order.end()) return CD3D_ERR_ARG;  // a handle twice
   const int R = std::min(sp->restart, sp->maxit);
-  const bool batchable = count > 1 && sp->precision == CD3D_FP64 && sp->smoother == 1 && sp->coarse_smoother == 17 &&
+  const bool batchable = count > 1 && (sp->precision == CD3D_FP64 || sp->precision == CD3D_MIXED) &&
+                         sp->smoother == 1 && sp->coarse_smoother == 17 &&
                          sp->cycle == 1 && sp->pre >= 1 && R >= 1 && R <= kBatchMaxRestart;
   if (!batchable) {  // one after another, each with its own stream
     cd3d_status worst = CD3D_OK;

@@ -99,6 +99,9 @@ struct DevLevel {
   // multicolour SSOR row lists (uploaded on first use)
   int *mc_rows = nullptr;
   std::vector<int> mc_start;
+  // batched mixed V-cycle: sol, rhs, aux, tmp and 1/a_ii in FP32 (5 ndof
+  // floats, allocated on first use)
+  float *b32 = nullptr;
   // SOR / SSOR in row order: forward [0] and backward [1] wavefront schedules
   // (build_gs_schedule; uploaded on first use)
   int *gs_rows[2] = {nullptr, nullptr}, *gs_start[2] = {nullptr, nullptr};
@@ -357,6 +360,9 @@ struct BatchLevel {
   const int *rowptr = nullptr, *col = nullptr, *diag = nullptr;
   const double *val = nullptr, *invdiag = nullptr;
   double *vec[4] = {nullptr, nullptr, nullptr, nullptr};  // sol, rhs, aux, tmp
+  // the mixed form's FP32 V-cycle: the same four vectors and 1/a_ii in FP32
+  float *vec32[4] = {nullptr, nullptr, nullptr, nullptr};
+  float *invdiag32 = nullptr;
   // prolongation (rows of this level) and restriction (rows of the coarser level)
   const int *p_rowptr = nullptr, *p_col = nullptr, *r_rowptr = nullptr, *r_col = nullptr;
   const double *p_val = nullptr, *r_val = nullptr;
@@ -371,23 +377,32 @@ struct BatchProblem {
   double *scal = nullptr;         // kBatchScal device scalars
   const double *coef = nullptr;   // kBatchCoef host-written coefficients
 };
-// level kernels: lev = this level's table (B entries); grid sizes from the largest problem
+// level kernels: lev = this level's table (B entries); grid sizes from the
+// largest problem.  f32: the level vectors are the FP32 ones (vec32,
+// invdiag32) of the mixed form's V-cycle; the matrices are read in FP64
 void kb_stream(hipStream_t s, int mode, int B, int maxblk, const BatchLevel *lev, const BatchProblem *prob,
-               const int *active, int x, int f, int out, double omega);
+               const int *active, int x, int f, int out, double omega, bool f32 = false);
 void kb_jacobi_zero(hipStream_t s, int B, int maxn, const BatchLevel *lev, const BatchProblem *prob,
-                    const int *active, int f, int out, double omega);
+                    const int *active, int f, int out, double omega, bool f32 = false);
 void kb_restrict(hipStream_t s, int B, int maxn_c, const BatchLevel *fine, const BatchProblem *prob,
-                 const int *active, int x, int out_coarse, const BatchLevel *coarse);
+                 const int *active, int x, int out_coarse, const BatchLevel *coarse, bool f32 = false);
 void kb_prolong_update(hipStream_t s, int B, int maxn, const BatchLevel *fine, const BatchProblem *prob,
-                       const int *active, const BatchLevel *coarse, int x_coarse, int out, double damp);
+                       const int *active, const BatchLevel *coarse, int x_coarse, int out, double damp,
+                       bool f32 = false);
 // out = x on [from, ndof) (from 0: the whole vector) or, with x < 0, out = 0
 void kb_copy(hipStream_t s, int B, int maxn, const BatchLevel *lev, const BatchProblem *prob, const int *active,
-             int x, int out, bool dirichlet_only);
+             int x, int out, bool dirichlet_only, bool f32 = false);
 void kb_fill_zero(hipStream_t s, int B, int maxn, const BatchLevel *lev, const BatchProblem *prob, const int *active,
-                  int out);
-// coarsest level: out_A += Ainv d_A
+                  int out, bool f32 = false);
+// coarsest level: out_A += Ainv d_A (accumulated in FP64)
 void kb_gemv_coarse(hipStream_t s, int B, int maxna, const BatchLevel *lev, const BatchProblem *prob,
-                    const int *active, int d, int out);
+                    const int *active, int d, int out, bool f32 = false);
+// FP64 level vector x -> FP32 level vector out32, and back; invdiag32 = (float) invdiag
+void kb_to32(hipStream_t s, int B, int maxn, const BatchLevel *lev, const BatchProblem *prob, const int *active,
+             int x, int out32);
+void kb_from32(hipStream_t s, int B, int maxn, const BatchLevel *lev, const BatchProblem *prob, const int *active,
+               int x32, int out);
+void kb_invdiag32(hipStream_t s, int B, int maxn, const BatchLevel *lev);
 // finest level reductions: partials slot 0/1; scalars at scal[idx]
 void kb_dot(hipStream_t s, int B, const BatchLevel *fine, const BatchProblem *prob, const int *active, int x, int y,
             int slot);

@@ -1839,7 +1839,31 @@ __device__ inline double *bvec(const BatchLevel &L, const BatchProblem &P, int i
   return P.V + (int64_t)(id - kBV0) * P.ld;
 }
 
-template <int MODE>
+// the level vectors of the V-cycle in the batch's working precision: FP64,
+// or FP32 for the mixed form (vec32, 1/a_ii in invdiag32); the matrices
+// stay FP64 and are rounded to FP32 as they are read
+template <typename T>
+__device__ inline T *bvec_t(const BatchLevel &L, const BatchProblem &P, int id);
+template <>
+__device__ inline double *bvec_t<double>(const BatchLevel &L, const BatchProblem &P, int id) {
+  return bvec(L, P, id);
+}
+template <>
+__device__ inline float *bvec_t<float>(const BatchLevel &L, const BatchProblem &, int id) {
+  return L.vec32[id];
+}
+template <typename T>
+__device__ inline const T *binv_t(const BatchLevel &L);
+template <>
+__device__ inline const double *binv_t<double>(const BatchLevel &L) {
+  return L.invdiag;
+}
+template <>
+__device__ inline const float *binv_t<float>(const BatchLevel &L) {
+  return L.invdiag32;
+}
+
+template <int MODE, typename T>
 __global__ __launch_bounds__(kStreamBlock) void bstream_k(const BatchLevel *__restrict__ lev,
                                                     const BatchProblem *__restrict__ prob,
                                                     const int *__restrict__ active, int xid, int fid, int oid,
@@ -1849,11 +1873,12 @@ __global__ __launch_bounds__(kStreamBlock) void bstream_k(const BatchLevel *__re
   const BatchLevel &L = lev[b];
   if ((int)blockIdx.x >= L.nblk) return;
   const BatchProblem &P = prob[b];
-  csr_stream_block<MODE, double, double, false>(blockIdx.x, L.bt, L.nact, L.rowptr, L.col, L.val, bvec(L, P, xid),
-                                                fid >= 0 ? bvec(L, P, fid) : nullptr, L.invdiag, bvec(L, P, oid),
-                                                omega);
+  csr_stream_block<MODE, T, double, false>(blockIdx.x, L.bt, L.nact, L.rowptr, L.col, L.val, bvec_t<T>(L, P, xid),
+                                           fid >= 0 ? bvec_t<T>(L, P, fid) : nullptr, binv_t<T>(L),
+                                           bvec_t<T>(L, P, oid), (T)omega);
 }
 
+template <typename T>
 __global__ __launch_bounds__(kBlock) void bjacobi_zero_k(const BatchLevel *__restrict__ lev,
                                                          const BatchProblem *__restrict__ prob,
                                                          const int *__restrict__ active, int fid, int oid,
@@ -1863,12 +1888,13 @@ __global__ __launch_bounds__(kBlock) void bjacobi_zero_k(const BatchLevel *__res
   const BatchLevel &L = lev[b];
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= L.ndof) return;
-  const double *f = bvec(L, prob[b], fid);
-  double *out = bvec(L, prob[b], oid);
-  out[i] = i < L.nact ? omega * f[i] / L.val[L.diag[i]] : f[i];
+  const T *f = bvec_t<T>(L, prob[b], fid);
+  T *out = bvec_t<T>(L, prob[b], oid);
+  out[i] = i < L.nact ? (T)omega * f[i] / (T)L.val[L.diag[i]] : f[i];
 }
 
 // restriction onto the coarser level's rows: 8 lanes per row
+template <typename T>
 __global__ __launch_bounds__(kBlock) void brestrict_k(const BatchLevel *__restrict__ fine,
                                                       const BatchProblem *__restrict__ prob,
                                                       const int *__restrict__ active,
@@ -1876,11 +1902,12 @@ __global__ __launch_bounds__(kBlock) void brestrict_k(const BatchLevel *__restri
   const int b = blockIdx.y;
   if (!active[b]) return;
   const BatchLevel &F = fine[b], &Cl = coarse[b];
-  csr_rows_body<8, 1, kRestrict, double, double>(blockIdx.x * kBlock + threadIdx.x, F.ndof_c, F.nact_c, F.r_rowptr,
-                                                 F.r_col, F.r_val, bvec(F, prob[b], xid), nullptr,
-                                                 bvec(Cl, prob[b], oid), 0.0);
+  csr_rows_body<8, 1, kRestrict, T, double>(blockIdx.x * kBlock + threadIdx.x, F.ndof_c, F.nact_c, F.r_rowptr,
+                                            F.r_col, F.r_val, bvec_t<T>(F, prob[b], xid), nullptr,
+                                            bvec_t<T>(Cl, prob[b], oid), T(0));
 }
 
+template <typename T>
 __global__ __launch_bounds__(kBlock) void bprolong_k(const BatchLevel *__restrict__ fine,
                                                      const BatchProblem *__restrict__ prob,
                                                      const int *__restrict__ active,
@@ -1889,11 +1916,12 @@ __global__ __launch_bounds__(kBlock) void bprolong_k(const BatchLevel *__restric
   const int b = blockIdx.y;
   if (!active[b]) return;
   const BatchLevel &F = fine[b], &Cl = coarse[b];
-  csr_rows_body<8, 1, kProlongUpd, double, double>(blockIdx.x * kBlock + threadIdx.x, F.ndof, F.nact, F.p_rowptr,
-                                                   F.p_col, F.p_val, bvec(Cl, prob[b], xid), nullptr,
-                                                   bvec(F, prob[b], oid), damp);
+  csr_rows_body<8, 1, kProlongUpd, T, double>(blockIdx.x * kBlock + threadIdx.x, F.ndof, F.nact, F.p_rowptr,
+                                              F.p_col, F.p_val, bvec_t<T>(Cl, prob[b], xid), nullptr,
+                                              bvec_t<T>(F, prob[b], oid), (T)damp);
 }
 
+template <typename T>
 __global__ __launch_bounds__(kBlock) void bcopy_k(const BatchLevel *__restrict__ lev,
                                                   const BatchProblem *__restrict__ prob,
                                                   const int *__restrict__ active, int xid, int oid,
@@ -1903,10 +1931,12 @@ __global__ __launch_bounds__(kBlock) void bcopy_k(const BatchLevel *__restrict__
   const BatchLevel &L = lev[b];
   const int i = (dirichlet_only ? L.nact : 0) + blockIdx.x * kBlock + threadIdx.x;
   if (i >= L.ndof) return;
-  double *out = bvec(L, prob[b], oid);
-  out[i] = xid >= 0 ? bvec(L, prob[b], xid)[i] : 0.0;
+  T *out = bvec_t<T>(L, prob[b], oid);
+  out[i] = xid >= 0 ? bvec_t<T>(L, prob[b], xid)[i] : T(0);
 }
 
+// coarsest level, accumulated in FP64 in both precisions
+template <typename T>
 __global__ __launch_bounds__(kBlock) void bgemv_k(const BatchLevel *__restrict__ lev,
                                                   const BatchProblem *__restrict__ prob,
                                                   const int *__restrict__ active, int did, int oid) {
@@ -1915,11 +1945,39 @@ __global__ __launch_bounds__(kBlock) void bgemv_k(const BatchLevel *__restrict__
   const BatchProblem &P = prob[b];
   const int na = P.na0, row = (blockIdx.x * kBlock + threadIdx.x) >> 6, lane = threadIdx.x & 63;
   if (row >= na) return;
-  const double *d = bvec(lev[b], P, did);
+  const T *d = bvec_t<T>(lev[b], P, did);
   double s = 0;
-  for (int j = lane; j < na; j += 64) s += P.cinv[(int64_t)row * na + j] * d[j];
+  for (int j = lane; j < na; j += 64) s += P.cinv[(int64_t)row * na + j] * (double)d[j];
   s = group_sum<64>(s);
-  if (lane == 0) bvec(lev[b], P, oid)[row] += s;
+  if (lane == 0) bvec_t<T>(lev[b], P, oid)[row] += (T)s;
+}
+
+// the mixed form's precision boundary: FP64 vector sid -> FP32 level vector
+// did (to32) or back (from32), and 1/a_ii rounded to FP32
+__global__ __launch_bounds__(kBlock) void bto32_k(const BatchLevel *__restrict__ lev,
+                                                  const BatchProblem *__restrict__ prob,
+                                                  const int *__restrict__ active, int sid, int did) {
+  const int b = blockIdx.y;
+  if (!active[b]) return;
+  const BatchLevel &L = lev[b];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < L.ndof) L.vec32[did][i] = (float)bvec(L, prob[b], sid)[i];
+}
+__global__ __launch_bounds__(kBlock) void bfrom32_k(const BatchLevel *__restrict__ lev,
+                                                    const BatchProblem *__restrict__ prob,
+                                                    const int *__restrict__ active, int sid, int did) {
+  const int b = blockIdx.y;
+  if (!active[b]) return;
+  const BatchLevel &L = lev[b];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < L.ndof) bvec(L, prob[b], did)[i] = (double)L.vec32[sid][i];
+}
+__global__ __launch_bounds__(kBlock) void binv32_k(const BatchLevel *__restrict__ lev, int B) {
+  const int b = blockIdx.y;
+  if (b >= B) return;
+  const BatchLevel &L = lev[b];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < L.ndof) L.invdiag32[i] = (float)L.invdiag[i];
 }
 
 __device__ inline double bsum_partials(const double *__restrict__ p, double *sh) {
@@ -2021,42 +2079,69 @@ __global__ __launch_bounds__(kBlock) void bupdate_k(const BatchProblem *__restri
   }
 }
 
-void kb_stream(hipStream_t s, int mode, int B, int maxblk, const BatchLevel *lev, const BatchProblem *prob,
-               const int *active, int x, int f, int out, double omega) {
-  if (maxblk <= 0 || B <= 0) return;
+template <typename T>
+static void kb_stream_t(hipStream_t s, int mode, int B, int maxblk, const BatchLevel *lev, const BatchProblem *prob,
+                        const int *active, int x, int f, int out, double omega) {
   const dim3 grid(maxblk, B);
-  if (mode == kSpmv) bstream_k<kSpmv><<<grid, kStreamBlock, 0, s>>>(lev, prob, active, x, f, out, omega);
-  else if (mode == kDefect) bstream_k<kDefect><<<grid, kStreamBlock, 0, s>>>(lev, prob, active, x, f, out, omega);
-  else bstream_k<kJacobi><<<grid, kStreamBlock, 0, s>>>(lev, prob, active, x, f, out, omega);
+  if (mode == kSpmv) bstream_k<kSpmv, T><<<grid, kStreamBlock, 0, s>>>(lev, prob, active, x, f, out, omega);
+  else if (mode == kDefect) bstream_k<kDefect, T><<<grid, kStreamBlock, 0, s>>>(lev, prob, active, x, f, out, omega);
+  else bstream_k<kJacobi, T><<<grid, kStreamBlock, 0, s>>>(lev, prob, active, x, f, out, omega);
+}
+void kb_stream(hipStream_t s, int mode, int B, int maxblk, const BatchLevel *lev, const BatchProblem *prob,
+               const int *active, int x, int f, int out, double omega, bool f32) {
+  if (maxblk <= 0 || B <= 0) return;
+  if (f32) kb_stream_t<float>(s, mode, B, maxblk, lev, prob, active, x, f, out, omega);
+  else kb_stream_t<double>(s, mode, B, maxblk, lev, prob, active, x, f, out, omega);
 }
 void kb_jacobi_zero(hipStream_t s, int B, int maxn, const BatchLevel *lev, const BatchProblem *prob,
-                    const int *active, int f, int out, double omega) {
-  if (maxn > 0) bjacobi_zero_k<<<dim3((maxn + kBlock - 1) / kBlock, B), kBlock, 0, s>>>(lev, prob, active, f, out, omega);
+                    const int *active, int f, int out, double omega, bool f32) {
+  if (maxn <= 0) return;
+  const dim3 grid((maxn + kBlock - 1) / kBlock, B);
+  if (f32) bjacobi_zero_k<float><<<grid, kBlock, 0, s>>>(lev, prob, active, f, out, omega);
+  else bjacobi_zero_k<double><<<grid, kBlock, 0, s>>>(lev, prob, active, f, out, omega);
 }
 void kb_restrict(hipStream_t s, int B, int maxn_c, const BatchLevel *fine, const BatchProblem *prob,
-                 const int *active, int x, int out_coarse, const BatchLevel *coarse) {
-  if (maxn_c > 0)
-    brestrict_k<<<dim3((maxn_c * 8 + kBlock - 1) / kBlock, B), kBlock, 0, s>>>(fine, prob, active, coarse, x,
-                                                                                out_coarse);
+                 const int *active, int x, int out_coarse, const BatchLevel *coarse, bool f32) {
+  if (maxn_c <= 0) return;
+  const dim3 grid((maxn_c * 8 + kBlock - 1) / kBlock, B);
+  if (f32) brestrict_k<float><<<grid, kBlock, 0, s>>>(fine, prob, active, coarse, x, out_coarse);
+  else brestrict_k<double><<<grid, kBlock, 0, s>>>(fine, prob, active, coarse, x, out_coarse);
 }
 void kb_prolong_update(hipStream_t s, int B, int maxn, const BatchLevel *fine, const BatchProblem *prob,
-                       const int *active, const BatchLevel *coarse, int x_coarse, int out, double damp) {
-  if (maxn > 0)
-    bprolong_k<<<dim3((maxn * 8 + kBlock - 1) / kBlock, B), kBlock, 0, s>>>(fine, prob, active, coarse, x_coarse, out,
-                                                                             damp);
+                       const int *active, const BatchLevel *coarse, int x_coarse, int out, double damp, bool f32) {
+  if (maxn <= 0) return;
+  const dim3 grid((maxn * 8 + kBlock - 1) / kBlock, B);
+  if (f32) bprolong_k<float><<<grid, kBlock, 0, s>>>(fine, prob, active, coarse, x_coarse, out, damp);
+  else bprolong_k<double><<<grid, kBlock, 0, s>>>(fine, prob, active, coarse, x_coarse, out, damp);
 }
 void kb_copy(hipStream_t s, int B, int maxn, const BatchLevel *lev, const BatchProblem *prob, const int *active,
-             int x, int out, bool dirichlet_only) {
-  if (maxn > 0)
-    bcopy_k<<<dim3((maxn + kBlock - 1) / kBlock, B), kBlock, 0, s>>>(lev, prob, active, x, out, dirichlet_only ? 1 : 0);
+             int x, int out, bool dirichlet_only, bool f32) {
+  if (maxn <= 0) return;
+  const dim3 grid((maxn + kBlock - 1) / kBlock, B);
+  if (f32) bcopy_k<float><<<grid, kBlock, 0, s>>>(lev, prob, active, x, out, dirichlet_only ? 1 : 0);
+  else bcopy_k<double><<<grid, kBlock, 0, s>>>(lev, prob, active, x, out, dirichlet_only ? 1 : 0);
 }
 void kb_fill_zero(hipStream_t s, int B, int maxn, const BatchLevel *lev, const BatchProblem *prob, const int *active,
-                  int out) {
-  kb_copy(s, B, maxn, lev, prob, active, -1, out, false);
+                  int out, bool f32) {
+  kb_copy(s, B, maxn, lev, prob, active, -1, out, false, f32);
 }
 void kb_gemv_coarse(hipStream_t s, int B, int maxna, const BatchLevel *lev, const BatchProblem *prob,
-                    const int *active, int d, int out) {
-  if (maxna > 0) bgemv_k<<<dim3((maxna * 64 + kBlock - 1) / kBlock, B), kBlock, 0, s>>>(lev, prob, active, d, out);
+                    const int *active, int d, int out, bool f32) {
+  if (maxna <= 0) return;
+  const dim3 grid((maxna * 64 + kBlock - 1) / kBlock, B);
+  if (f32) bgemv_k<float><<<grid, kBlock, 0, s>>>(lev, prob, active, d, out);
+  else bgemv_k<double><<<grid, kBlock, 0, s>>>(lev, prob, active, d, out);
+}
+void kb_to32(hipStream_t s, int B, int maxn, const BatchLevel *lev, const BatchProblem *prob, const int *active,
+             int x, int out32) {
+  if (maxn > 0) bto32_k<<<dim3((maxn + kBlock - 1) / kBlock, B), kBlock, 0, s>>>(lev, prob, active, x, out32);
+}
+void kb_from32(hipStream_t s, int B, int maxn, const BatchLevel *lev, const BatchProblem *prob, const int *active,
+               int x32, int out) {
+  if (maxn > 0) bfrom32_k<<<dim3((maxn + kBlock - 1) / kBlock, B), kBlock, 0, s>>>(lev, prob, active, x32, out);
+}
+void kb_invdiag32(hipStream_t s, int B, int maxn, const BatchLevel *lev) {
+  if (maxn > 0 && B > 0) binv32_k<<<dim3((maxn + kBlock - 1) / kBlock, B), kBlock, 0, s>>>(lev, B);
 }
 void kb_dot(hipStream_t s, int B, const BatchLevel *fine, const BatchProblem *prob, const int *active, int x, int y,
             int slot) {

@@ -254,7 +254,8 @@ struct cd3d_server {
   // V-cycle, exact coarse solve, at least one pre-smoothing sweep
   bool batchable(const cd3d_request &r) const {
     const cd3d_solver_params &s = r.solver;
-    return !prm.simulate && prm.max_batch > 1 && r.kind == CD3D_REQ_CD && s.precision == CD3D_FP64 &&
+    return !prm.simulate && prm.max_batch > 1 && r.kind == CD3D_REQ_CD &&
+           (s.precision == CD3D_FP64 || s.precision == CD3D_MIXED) &&
            s.smoother == 1 && s.coarse_smoother == 17 && s.cycle == 1 && s.pre >= 1 &&
            std::min(s.restart, s.maxit) <= 200;
   }

@@ -3,7 +3,9 @@ orders and coefficients with the same number of levels, solved together in
 lockstep.  Each problem must end as it does alone: the same iteration count
 as the FP64 oracle (exact coarse solve, relative residual 1e-10) and as a
 lone cd3d_solve, solution within 1e-10 of the oracle's; its own status when
-it starts converged, stops at maxit, or diverges."""
+it starts converged, stops at maxit, or diverges.  In the mixed form (FP32
+V-cycle): relative residual 1e-10 and the solution within 1e-9 of the FP64
+oracle's."""
 import sys
 
 import numpy as np
@@ -62,6 +64,34 @@ def test_batch_matches_oracle_and_lone_solves():
             f"rel res {st.rfinal / st.r0:.1e}")
         assert st.status == 0 and st.iterations == its_o == lone.iterations
         assert err < 1e-10 and st.rfinal <= 1.0001e-10 * st.r0
+    for P in probs:
+        P.close()
+
+
+def test_mixed_batch_matches_fp64_oracle():
+    # the mixed form in a batch: FP32 V-cycle over the CSR levels inside FP64
+    # FGMRES.  Each member reaches relative residual 1e-10 (recomputed here
+    # in FP64 from the CSR matrix), its solution is within 1e-9 of the FP64
+    # oracle's, and it needs at most 2 iterations more than the oracle
+    import scipy.sparse as sps
+    sp = pm.solver_params(**SOLVER, precision=pm.MIXED)
+    probs = [make(*c) for c in MIX]
+    stats = pm.solve_batch(probs, sp)
+    for c, P, st in zip(MIX, probs, stats):
+        top = c[3] - 1
+        x = P.solution()
+        xo, its_o = oracle_solve(*c)
+        rp, kc, e = P.csr(top)
+        n, na = len(rp) - 1, P.info(top)["nact"]
+        A = sps.csr_matrix((e, kc, rp), shape=(n, n))
+        b = P.rhs(top)
+        x0 = np.zeros(n)
+        x0[na:] = b[na:]
+        rel = np.linalg.norm(b - A @ x) / np.linalg.norm(b - A @ x0)
+        err = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+        say(f"[gpu] mixed batch {c[:4]}: its {st.iterations} (fp64 oracle {its_o}), rel res {rel:.1e}, err {err:.1e}")
+        assert st.status == 0 and st.iterations <= its_o + 2
+        assert rel <= 1.001e-10 and err < 1e-9
     for P in probs:
         P.close()
 

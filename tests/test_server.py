@@ -180,6 +180,30 @@ def test_batched_requests_agree_with_lone_solves():
 
 
 @pytest.mark.gpu
+def test_mixed_requests_run_as_one_batch():
+    # CD requests in the mixed form with the same number of levels and solver
+    # parameters, admitted together, run as one batch (FP32 V-cycle over the
+    # CSR levels inside FP64 FGMRES): each reaches relative residual 1e-10
+    # and agrees with its lone FP64 solve to 1e-9
+    cases = [c for c in MIX if not c.get("tcd") and c["mesh"][3] == 3]
+    srv = pm.Server(max_inflight=8, workers=1, max_batch=8)
+    ids = [srv.submit(pm.mesh_desc(*c["mesh"]), pm.coeffs(**c["co"]), solver(precision=pm.MIXED),
+                      want_solution=True) for c in cases]
+    out = [srv.wait(i, n_dof=n_dof(c)) for c, i in zip(cases, ids)]
+    srv.close()
+    say(f"[gpu] mixed batched requests: sizes {[r.batch_size for r, _ in out]}, "
+        f"its {[r.iterations for r, _ in out]}, rel res {[f'{r.rel_residual:.1e}' for r, _ in out]}")
+    assert max(r.batch_size for r, _ in out) >= 2
+    for c, (r, x) in zip(cases, out):
+        P = pm.Problem(*c["mesh"])
+        P.assemble(pm.coeffs(**c["co"]))
+        x64, _ = P.solve(solver())
+        P.close()
+        assert r.status == 0 and r.rel_residual <= 1.0001e-10
+        assert np.linalg.norm(x - x64) <= 1e-9 * np.linalg.norm(x64)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("mesh", [(2, 1, 2, 4), (1, 1, 2, 5)])
 def test_device_footprint_stays_within_the_estimate(mesh):
     P = pm.Problem(*mesh)
