@@ -158,6 +158,29 @@ def general_hexahedra(pm, steps, warmup):
     return out
 
 
+def large_q2_128(pm, steps, warmup):
+    """The headline problem at 128^3 cells (Q2, 16 974 593 DOFs, 1.06e9
+    nonzeros, 5 levels 8^3 .. 128^3): assemble + solve rate of both forms,
+    device memory, and the fine-level sweep and SpMV rates at that size."""
+    t = time.perf_counter()
+    P = pm.Problem(2, 1, 3, 5, box=(1.0, 1.0, 1.0))
+    setup_s = time.perf_counter() - t
+    co = pm.coeffs(**COEFFS)
+    out = dict(ndof=P.info(4)["ndof"], nnz=P.info(4)["nnz"], setup_s=round(setup_s, 2))
+    for name, prec in (("mixed", pm.MIXED), ("fp64", pm.FP64)):
+        lat, its, rr = run_steps(pm, P, pm.solver_params(**SOLVER, precision=prec), co, warmup, steps)
+        out[name] = dict(solves_per_s=steps / lat.sum(), ms_per_step=1e3 * lat.mean(),
+                         iterations=int(np.median(its)), max_rel_residual=float(max(rr)))
+    out["device_gib"] = round(P.L.cd3d_device_bytes(P.h) / 2**30, 2)
+    out["mixed_vs_fp64_speedup"] = out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"]
+    for which, key, reps in ((1, "sweep", 50), (0, "spmv_fp64_csr", 10)):
+        ms, b = P.kernel_bench(which, 4, reps)
+        out[key] = dict(kernel=KERNEL_NAMES[which], us_per_launch=round(ms * 1e3, 2), algorithmic_bytes=int(b),
+                        achieved_gbs=round(b / ms / 1e6, 1), frac=round(b / ms / 1e6 / HBM_PEAK_GBS, 4))
+    P.close()
+    return out
+
+
 CPU_RECORD = os.path.join(REPO, "profiles", "cpu_reference_vs_port_c2_64.json")
 
 
@@ -401,6 +424,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-procs", type=int, default=0, help="CPU baseline processes (default: host cores, at most 16)")
     ap.add_argument("--no-extra", action="store_true", help="only the headline configuration")
+    ap.add_argument("--no-large", action="store_true", help="skip the 128^3-cell Q2 record")
     ap.add_argument("--c3-steps", type=int, default=100)
     ap.add_argument("--c4-problems", type=int, default=256)
     ap.add_argument("--c4-inflight", type=int, default=16, help="server slots for configuration 4 (unbatched)")
@@ -478,6 +502,9 @@ def main():
     if not args.no_extra:
         log("[bench] 64^3 Q2 SUPG on general hexahedra (deformed 4^3 macro mesh)")
         extra["general_hexahedra_q2_64"] = general_hexahedra(pm, 3, 1)
+        if not args.no_large:
+            log("[bench] the headline problem at 128^3 cells (16 974 593 DOFs)")
+            extra["large_q2_128"] = large_q2_128(pm, 3, 1)
         for name, prec in (("fp64", pm.FP64), ("mixed", pm.MIXED)):
             log(f"[bench] configuration 3 (TCD3D Q1 128^3, {args.c3_steps} CN steps), {name}")
             r, u = tcd_run(pm, prec, args.c3_steps)
