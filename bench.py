@@ -355,7 +355,7 @@ def c4_form(pm, precision, args):
     return batched_uneven(pm, precision, args.c4_problems, inflight=args.c4_inflight, max_batch=1)
 
 
-def serving_stream(pm, precision, rates, duration_s=4.0, seed=7, tcd_share=0.2, max_requests=200):
+def serving_stream(pm, precision, rates, duration_s=4.0, seed=7, tcd_share=0.2, max_requests=400):
     """Configuration 5: Poisson arrivals of CD3D / TCD3D requests, n in
     [16, 96] (snapped), Q1 or Q2 with equal probability, eps and b as in
     configuration 4; TCD3D requests run 10 Crank-Nicolson steps of the
@@ -398,6 +398,9 @@ def serving_stream(pm, precision, rates, duration_s=4.0, seed=7, tcd_share=0.2, 
         out.append(dict(rate_per_s=rate, requests=len(res), duration_s=span, wall_s=wall,
                         solves_per_s=len(ok) / wall, p50_ms=float(np.percentile(lat, 50)),
                         p99_ms=float(np.percentile(lat, 99)), failed=len(res) - len(ok),
+                        escalated=sum(int(r.escalated) for r in res),
+                        median_iterations=int(np.median([r.iterations for r in res])),
+                        max_iterations=int(max(r.iterations for r in res)),
                         failures=[failure_record(a, r) for a, r in zip(arrivals, res) if r.status != 0],
                         tcd_requests=sum(a["tcd"] for a in arrivals)))
         log(f"[bench]   rate {rate}/s: {len(res)} requests ({out[-1]['failed']} failed), p50 {out[-1]['p50_ms']:.1f} ms, "

@@ -41,7 +41,7 @@ typedef struct {
   double sim_us_per_dof;
   int robust_after;    /* a CD solve with the damped Jacobi smoother that has not converged after this many
                           FGMRES iterations continues from its iterate with the multicolour SSOR smoother
-                          (13, omega 1, 2 + 2) on the rest of its iteration budget; 0: never */
+                          (13, omega 1, 4 + 4) on the rest of its iteration budget; 0: never */
   int max_batch;       /* a worker takes up to this many admitted CD requests with the same number of levels
                           and identical FP64 solver parameters (damped Jacobi, V-cycle, exact coarse solve)
                           and solves them together (cd3d_solve_batch); 1: every request alone */
