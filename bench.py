@@ -52,7 +52,8 @@ KERNEL_NAMES = {0: "csr_stream<kSpmv,f64>",
                 4: "lat_prolong_k<f32>", 5: "csr_fill_k",
                 6: "copy_k (f64 vector copy, traffic calibration)",
                 7: "axpy_dot2_k f64 (fused Gram-Schmidt step, 16-byte pairs)",
-                8: "assemble_k<2> + gather_entries_k f64 (cell-by-cell SUPG element rows, then CSR entries gathered)"}
+                8: "assemble_k<2> + gather_entries_k f64 (cell-by-cell SUPG element rows, then CSR entries gathered)",
+                9: "ls_dots_k + ls_coeffs_k + ls_update2_k f64 (low-synchronisation Gram-Schmidt step, i = 10)"}
 FP64_PEAK_TFLOPS = 78.6
 FLOP_KERNELS = {8}
 
@@ -490,11 +491,14 @@ def main():
     err = float(np.linalg.norm(sols["mixed"] - sols["fp64"]) / np.linalg.norm(sols["fp64"]))
     log("[bench] kernel table (HIP events on the library stream)")
     table = kernel_table(P)
-    roof = roofline_entry(table, 1, "dominant kernel of the mixed step (with the Gram-Schmidt step, mgs_step): "
-                                    "the fine-level FP32 Jacobi sweep (algorithmic bytes: x, f and the result "
-                                    "once, 12 bytes per node)")
-    mgs = roofline_entry(table, 7, "fused Gram-Schmidt step of the outer FGMRES on the fine level (32 bytes per "
-                         "unknown); it and the sweep take the largest shares of the mixed step's kernel time")
+    roof = roofline_entry(table, 1, "dominant kernel of the mixed step (with the Gram-Schmidt passes, "
+                                    "lowsync_gs_step): the fine-level FP32 Jacobi sweep (algorithmic bytes: x, f "
+                                    "and the result once, 12 bytes per node)")
+    mgs = roofline_entry(table, 7, "fused modified Gram-Schmidt step of the FP64 form's FGMRES on the fine level "
+                         "(32 bytes per unknown; the reference's Gram-Schmidt, one launch per basis vector)")
+    lowsync = roofline_entry(table, 9, "the mixed form's low-synchronisation Gram-Schmidt step at i = 10 on the "
+                             "fine level: one pass for the dots, one for w -= V h and |w|^2 ((2 i + 5) 8 bytes "
+                             "per unknown)")
     asm = flop_roofline_entry(table, 8, "per-element quadrature (Gauss4, 64 points) of the Q2 SUPG element "
                               "matrices on the fine level: element rows written out, then every CSR entry "
                               "gathered from its contributions; algorithmic flops n_cells n_q 8 n_b^2 per "
@@ -560,6 +564,7 @@ def main():
         "roofline": roof,
         "spmv_fp64_csr": spmv,
         "mgs_step": mgs,
+        "lowsync_gs_step": lowsync,
         "assembly_cellwise": asm,
         "kernels_fine_level": table,
         "cpu_baseline": cpu,

@@ -280,11 +280,13 @@ cd3d_status cd3d_write_matrix(cd3d_problem *p, int level, const char *path);
  * FP64 level vector (16 N: the traffic-counter calibration); 7 one fused
  * Gram-Schmidt step of FGMRES, w -= h v_k and the partial sums of <w, v_k+1>
  * (32 N: v_k, w, v_k+1 read, w written); 8 the cell-by-cell element kernel
- * (BilinearAssemble / BilinearAssemble_SD per quadrature point, colour-scattered
- * into CSR) over all cells of the level, with the coefficients of the last
- * assembly, into a scratch value array: *bytes_per_launch then holds the
- * algorithmic FP64 flops of one pass, n_cells n_q 8 n_b^2 (a "launch" is the 8
- * colour launches).
+ * (BilinearAssemble / BilinearAssemble_SD per quadrature point) over all cells
+ * of the level with the coefficients of the last assembly, element rows
+ * written out and gathered into a scratch CSR value array (CD3D_ASM_SCATTER=1:
+ * the 8 colour-by-colour scatters): *bytes_per_launch then holds the
+ * algorithmic FP64 flops of one pass, n_cells n_q 8 n_b^2; 9 the mixed form's
+ * low-synchronisation Gram-Schmidt step at i = 10 on the finest level (dot
+ * pass, coefficients, update pass: (2 i + 5) 8 N bytes).
  * Needs an assembled problem; the level vectors are overwritten. */
 cd3d_status cd3d_kernel_bench(cd3d_problem *p, int which, int level, int reps, double *ms_per_launch,
                               double *bytes_per_launch);

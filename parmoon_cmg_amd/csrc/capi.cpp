@@ -855,6 +855,9 @@ struct Space {
   // (float) r, written by the kernel that produced r
   std::function<void(VC &, const double *, double *, bool)> precond;
   float *r32 = nullptr;  // FP32 copy of the preconditioner input (lattice space)
+  // orthogonalise by the low-synchronisation form of MGS (k_ls_step)
+  // instead of MGS step by step (the FP64 form keeps the reference's MGS)
+  bool lowsync = false;
 };
 
 Space dof_space(cd3d_problem &P) {
@@ -880,6 +883,9 @@ Space lattice_space(cd3d_problem &P) {
   S.matvec = [&P, &F, p](const double *x, double *y) { kl_spmv64(P.stream, F, p, x, y); };
   // the Krylov vectors vanish on the boundary, as the V-cycle's rhs must
   S.r32 = F.lf;
+  // CD3D_FGMRES_MGS=1: MGS step by step as in the FP64 form
+  const char *em = getenv("CD3D_FGMRES_MGS");
+  S.lowsync = !(em && em[0] == '1');
   S.precond = [&P, &F, top](VC &v, const double *r, double *z, bool r32_ready) {
     v.reset();
     if (!r32_ready) kl_cast_d2f(P.stream, F.ndof, r, F.lf);
@@ -890,6 +896,13 @@ Space lattice_space(cd3d_problem &P) {
 }
 
 int fgmres_iterate(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S, double &r0, cd3d_status &status);
+
+// the buffers of the low-synchronisation Gram-Schmidt step (first use)
+void ensure_ls(cd3d_problem &P, int n) {
+  if (P.ls_part) return;
+  P.ls_part = dalloc<double>(P, (size_t)(2 * k_ls_max() + 1) * k_ls_blocks(n));
+  P.ls_L = dalloc<double>(P, (size_t)k_ls_max() * k_ls_max());
+}
 
 // rfinal (optional): |b - A x| recomputed in the space after the iteration
 int fgmres(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S, double &r0, cd3d_status &status,
@@ -915,6 +928,9 @@ int fgmres_iterate(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S
   auto Zk = [&](int k) { return Z + (int64_t)k * ld; };
   std::vector<double> H((size_t)(R + 1) * R, 0.0), s(R + 1), cs(R + 1), sn(R + 1);
   double *pa = P.red, *pb = P.red + kRedBlocks;
+  const bool lowsync = S.lowsync && R <= k_ls_max() && ld % 2 == 0 && ((uintptr_t)V & 15) == 0 &&
+                       ((uintptr_t)w & 15) == 0;
+  if (lowsync) ensure_ls(P, N);
   S.defect(S.x, S.b, Vk(0));
   double beta = dev_norm(P, N, Vk(0));
   r0 = beta;
@@ -946,13 +962,20 @@ int fgmres_iterate(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S
     for (i = 0; i < R && j <= sp.maxit; i++, j++) {
       S.precond(v, Vk(i), Zk(i), S.r32 != nullptr);
       S.matvec(Zk(i), w);
-      k_dot(P.stream, N, w, Vk(0), pa, nullptr);
-      // modified Gram-Schmidt: H[k][i] = <w, v_k>; w -= H[k][i] v_k, fused with the next dot
-      for (int k = 0; k <= i; k++) {
-        double *in = (k % 2 == 0) ? pa : pb, *out = (k % 2 == 0) ? pb : pa;
-        k_axpy_dot(P.stream, N, in, -1.0, Vk(k), w, k < i ? Vk(k + 1) : nullptr, out, P.dscal + k);
+      double *last;
+      if (lowsync) {
+        // the MGS column in two passes over the basis (k_ls_step)
+        k_ls_step(P.stream, N, i, V, ld, w, P.ls_part, P.ls_L, P.dscal, pa);
+        last = pa;
+      } else {
+        k_dot(P.stream, N, w, Vk(0), pa, nullptr);
+        // modified Gram-Schmidt: H[k][i] = <w, v_k>; w -= H[k][i] v_k, fused with the next dot
+        for (int k = 0; k <= i; k++) {
+          double *in = (k % 2 == 0) ? pa : pb, *out = (k % 2 == 0) ? pb : pa;
+          k_axpy_dot(P.stream, N, in, -1.0, Vk(k), w, k < i ? Vk(k + 1) : nullptr, out, P.dscal + k);
+        }
+        last = (i % 2 == 0) ? pb : pa;
       }
-      double *last = (i % 2 == 0) ? pb : pa;
       k_scale_copy(P.stream, N, last, P.dscal + i + 1, w, Vk(i + 1), S.r32);
       HIPCHK(hipMemcpyAsync(P.hscal, P.dscal, sizeof(double) * (i + 2), hipMemcpyDeviceToHost, P.stream));
       HIPCHK(hipStreamSynchronize(P.stream));
@@ -2298,7 +2321,7 @@ cd3d_status cd3d_device_ptr(cd3d_problem *p, int level, int which, double **dev)
 }
 
 cd3d_status cd3d_kernel_bench(cd3d_problem *p, int which, int level, int reps, double *ms, double *bytes) {
-  if (!p || !ms || !bytes || reps < 1 || level < 0 || level >= (int)p->dl.size() || which < 0 || which > 8)
+  if (!p || !ms || !bytes || reps < 1 || level < 0 || level >= (int)p->dl.size() || which < 0 || which > 9)
     return CD3D_ERR_ARG;
   std::lock_guard<std::mutex> g(p->mu);
   return run(p, [&]() {
@@ -2358,6 +2381,23 @@ cd3d_status cd3d_kernel_bench(cd3d_problem *p, int which, int level, int reps, d
         };
         b = 32 * N;
         break;
+      case 9: {
+        // the mixed form's low-synchronisation Gram-Schmidt step at i = 10
+        // (dot pass, coefficients, update pass) on the finest level, over a
+        // zero basis: (i + 2) + (i + 3) vectors of 8 N bytes
+        if (level != (int)p->dl.size() - 1) throw ArgError("the Gram-Schmidt step runs on the finest level");
+        constexpr int ii = 10;
+        ensure_krylov(*p, 2 * ii);
+        ensure_ls(*p, L.ndof);
+        if (p->krylov_ld % 2) throw std::logic_error("Krylov basis stride not a multiple of 2");
+        HIPCHK(hipMemsetAsync(p->V, 0, sizeof(double) * (ii + 1) * p->krylov_ld, p->stream));
+        HIPCHK(hipMemsetAsync(p->ls_L, 0, sizeof(double) * k_ls_max() * k_ls_max(), p->stream));
+        launch = [&] {
+          k_ls_step(p->stream, L.ndof, ii, p->V, p->krylov_ld, p->w, p->ls_part, p->ls_L, p->dscal, p->red);
+        };
+        b = (2 * ii + 5) * 8 * N;
+        break;
+      }
     }
     double *scratch_val = nullptr;
     if (which == 8) {

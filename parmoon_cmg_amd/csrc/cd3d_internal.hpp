@@ -203,6 +203,9 @@ struct Problem {
   double *V = nullptr, *Z = nullptr, *w = nullptr, *xs = nullptr, *bs = nullptr;
   double *red = nullptr;      // reduction partials
   double *dscal = nullptr;    // device scalars: H column at [0, restart], y at [yoff, yoff + restart)
+  // low-synchronisation Gram-Schmidt of the mixed form: dot partials and the
+  // strictly lower part of V^T V (allocated on first use)
+  double *ls_part = nullptr, *ls_L = nullptr;
   double *hscal = nullptr;    // pinned host mirror of dscal
   int yoff = 64, nscal = 256;
   int *hflag = nullptr;       // pinned, lattice uniformity flags
@@ -280,6 +283,14 @@ void k_jacobi_zero(hipStream_t s, const DevLevel &L, const double *f, double *ou
 // one multicolour SSOR sweep in place (smoother 13): x_D = f_D, relaxed
 // Gauss-Seidel colour by colour forward, then backward (needs mc_rows)
 void k_mcssor(hipStream_t s, const DevLevel &L, double *x, const double *f, double omega);
+// low-synchronisation Gram-Schmidt step i (restart length <= k_ls_max()):
+// h = the MGS column of w against V_0 .. V_i into h[0 .. i], w -= V h, and
+// the partials of |w|^2 (kRedBlocks); part holds (2 k_ls_max() + 1)
+// k_ls_blocks(n) doubles, Lm k_ls_max()^2
+int k_ls_blocks(int n);
+int k_ls_max();
+void k_ls_step(hipStream_t s, int n, int i, const double *V, int64_t ld, double *w, double *part, double *Lm,
+               double *h, double *partials);
 // one SOR (kind 2) or SSOR (kind 3) sweep in the reference's row order, in
 // place: x_D = f_D, then the wavefront schedule(s) of the level
 void k_gs_sweep(hipStream_t s, const DevLevel &L, double *x, const double *f, double omega, int kind);

@@ -529,6 +529,149 @@ __global__ __launch_bounds__(kBlock) void scale_copy2_k(int n, const double *__r
   }
 }
 
+// Low-synchronisation Gram-Schmidt of the mixed form's FGMRES (the
+// inverse compact WY form of modified Gram-Schmidt: Swirydowicz, Langou,
+// Ananthan, Yang, Thomas, Numer. Linear Algebra Appl. 28 (2021) e2343).
+// At Arnoldi step i, with w = A z_i and the basis V_0 .. V_i, one pass
+// forms r_k = <w, V_k> (k <= i) and the newest vector's Gram row
+// g_k = <V_i, V_k> (k < i); then h = (I + L)^-1 r with L the strictly lower
+// part of V^T V (rows kept from the earlier steps) is the MGS column:
+// h_k = <w - sum_{m<k} h_m V_m, V_k> = r_k - sum_{m<k} h_m L_km.  A second
+// pass forms w - V h and the partial sums of its squared norm.  Two passes
+// over the basis per step instead of one pass over w and two basis vectors
+// per MGS step.
+constexpr int kLsChunk = 2048;  // elements per workgroup of the dot pass
+constexpr int kLsMax = 32;      // restart length up to which the form is used
+
+// per-workgroup partials: r_k at part[k nb + b], g_k at part[(i + 1 + k) nb + b]
+__global__ __launch_bounds__(kBlock) void ls_dots_k(int n, int i, const double *__restrict__ V, int64_t ld,
+                                                    const double *__restrict__ w, double *__restrict__ part) {
+  __shared__ double ws[kLsChunk], vs[kLsChunk];
+  const int b = blockIdx.x, nb = gridDim.x;
+  const int64_t e0 = (int64_t)b * kLsChunk;
+  const int ne = (int)min((int64_t)kLsChunk, (int64_t)n - e0);
+  const double *Vi = V + (int64_t)i * ld + e0;
+  for (int t = threadIdx.x; t < kLsChunk; t += kBlock) {
+    ws[t] = t < ne ? w[e0 + t] : 0.0;
+    vs[t] = t < ne ? Vi[t] : 0.0;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  constexpr int kPer = kLsChunk / 64;
+  for (int k = wave; k <= i; k += kBlock / 64) {
+    double r = 0.0, g = 0.0;
+    if (k == i) {
+      for (int t = lane; t < ne; t += 64) r = fma(vs[t], ws[t], r);
+    } else {
+      // all of the lane's loads of V_k first, then the products
+      const double *Vk = V + (int64_t)k * ld + e0;
+      double v[kPer];
+#pragma unroll
+      for (int u = 0; u < kPer; u++) {
+        const int t = lane + 64 * u;
+        v[u] = t < ne ? Vk[t] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kPer; u++) {
+        const int t = lane + 64 * u;
+        r = fma(v[u], ws[t], r);
+        g = fma(v[u], vs[t], g);
+      }
+    }
+    r = group_sum<64>(r);
+    g = group_sum<64>(g);
+    if (lane == 0) {
+      part[(int64_t)k * nb + b] = r;
+      if (k < i) part[(int64_t)(i + 1 + k) * nb + b] = g;
+    }
+  }
+}
+
+// one workgroup of kLsCoef threads: the sums of the partials (a wave per
+// dot, four interleaved accumulators, a fixed order), row i of L (Lm, ld
+// kLsMax; rows 0 .. i-1 staged in LDS), and h = (I + L)^-1 r into h[0 .. i]
+// by one wave whose lane m holds h_m (one butterfly sum per row)
+constexpr int kLsCoef = 1024;
+__global__ __launch_bounds__(kLsCoef) void ls_coeffs_k(int i, int nb, const double *__restrict__ part,
+                                                       double *__restrict__ Lm, double *__restrict__ h) {
+  __shared__ double d[2 * kLsMax + 1], Ls[kLsMax * kLsMax];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int t = threadIdx.x; t < i * kLsMax; t += kLsCoef) Ls[t] = Lm[t];
+  for (int q = wave; q < 2 * i + 1; q += kLsCoef / 64) {
+    const double *pq = part + (int64_t)q * nb;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int t = lane;
+    for (; t + 192 < nb; t += 256) {
+      a0 += pq[t];
+      a1 += pq[t + 64];
+      a2 += pq[t + 128];
+      a3 += pq[t + 192];
+    }
+    for (; t < nb; t += 64) a0 += pq[t];
+    const double v = group_sum<64>((a0 + a1) + (a2 + a3));
+    if (lane == 0) d[q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < i) {
+    const double g = d[i + 1 + threadIdx.x];
+    Ls[i * kLsMax + threadIdx.x] = g;
+    Lm[i * kLsMax + threadIdx.x] = g;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    double hm = 0.0;  // lane m: h_m once row m is done
+    for (int k = 0; k <= i; k++) {
+      const double s = group_sum<64>(lane < k ? Ls[k * kLsMax + lane] * hm : 0.0);
+      if (lane == k) hm = d[k] - s;
+    }
+    if (lane <= i) h[lane] = hm;
+  }
+}
+
+// w -= sum_{k <= i} h_k V_k on 16-byte pairs, and the partial sums of |w|^2
+// (one per workgroup, kRedBlocks of them, as the other reductions)
+__global__ __launch_bounds__(kBlock) void ls_update2_k(int n, int i, const double *__restrict__ h,
+                                                       const double *__restrict__ V, int64_t ld,
+                                                       double *__restrict__ w, double *__restrict__ partials) {
+  __shared__ double sh[17], hs[kLsMax];
+  if (threadIdx.x <= i) hs[threadIdx.x] = h[threadIdx.x];
+  __syncthreads();
+  const int n2 = n >> 1;
+  double2 *w2 = (double2 *)w;
+  double v = 0.0;
+  for (int e = blockIdx.x * kBlock + threadIdx.x; e < n2; e += gridDim.x * kBlock) {
+    double2 x = w2[e];
+    // the basis vectors four at a time: loads first, then the products
+    int k = 0;
+    for (; k + 4 <= i + 1; k += 4) {
+      double2 z[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) z[u] = ((const double2 *)(V + (int64_t)(k + u) * ld))[e];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        x.x = fma(-hs[k + u], z[u].x, x.x);
+        x.y = fma(-hs[k + u], z[u].y, x.y);
+      }
+    }
+    for (; k <= i; k++) {
+      const double2 z = ((const double2 *)(V + (int64_t)k * ld))[e];
+      x.x = fma(-hs[k], z.x, x.x);
+      x.y = fma(-hs[k], z.y, x.y);
+    }
+    w2[e] = x;
+    v = fma(x.x, x.x, v);
+    v = fma(x.y, x.y, v);
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    double x = w[n - 1];
+    for (int k = 0; k <= i; k++) x = fma(-hs[k], V[(int64_t)k * ld + n - 1], x);
+    w[n - 1] = x;
+    v = fma(x, x, v);
+  }
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) partials[blockIdx.x] = v;
+}
+
 // x *= alpha and, when y32 is given, y32 = (float) x (scalar loop)
 __global__ void scale_cast_k(int n, double alpha, double *__restrict__ x, float *__restrict__ y32) {
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
@@ -1612,6 +1755,15 @@ void k_scale_copy(hipStream_t s, int n, const double *partials, double *norm_out
     if (y32) scale_cast_k<<<grid_for(n), kBlock, 0, s>>>(n, 1.0, y, y32);
   }
 }
+int k_ls_blocks(int n) { return (n + kLsChunk - 1) / kLsChunk; }
+int k_ls_max() { return kLsMax; }
+void k_ls_step(hipStream_t s, int n, int i, const double *V, int64_t ld, double *w, double *part, double *Lm,
+               double *h, double *partials) {
+  const int nb = k_ls_blocks(n);
+  ls_dots_k<<<nb, kBlock, 0, s>>>(n, i, V, ld, w, part);
+  ls_coeffs_k<<<1, kLsCoef, 0, s>>>(i, nb, part, Lm, h);
+  ls_update2_k<<<kRedBlocks, kBlock, 0, s>>>(n, i, h, V, ld, w, partials);
+}
 void k_scale_cast(hipStream_t s, int n, double alpha, double *x, float *y32) {
   scale_cast_k<<<grid_for(n), kBlock, 0, s>>>(n, alpha, x, y32);
 }
@@ -1813,7 +1965,8 @@ bool k_invert_multi_launch(int na) { return na > kGjLdsMax; }
 int64_t gj_scratch_doubles(int na) {
   // M and M' (na x 2na each), W (na x kGjPanel), dval, partials, used + perm (ints)
   return (int64_t)4 * na * na + (int64_t)(kGjPanel + 1) * na + 512 + (512 + 2 * (int64_t)na + 2) / 2 + 1;
-}void k_multi_axpy(hipStream_t s, int n, int m, const double *coef, const double *Z, int64_t ldz, double *x) {
+}
+void k_multi_axpy(hipStream_t s, int n, int m, const double *coef, const double *Z, int64_t ldz, double *x) {
   if (al16(Z) && al16(x) && ldz % 2 == 0) multi_axpy2_k<<<grid_for((n + 1) / 2), kBlock, 0, s>>>(n, m, coef, Z, ldz, x);
   else multi_axpy_k<<<grid_for(n), kBlock, 0, s>>>(n, m, coef, Z, ldz, x);
 }
