@@ -535,7 +535,7 @@ def test_kernel_bench_covers_every_kernel():
     N, nnz = i["ndof"], i["nnz"]
     rp, kc, e0 = P.csr(l)
     nbytes = {}
-    for which in range(9):
+    for which in range(10):
         ms, b = P.kernel_bench(which, l, 3)
         assert ms > 0 and b > 0
         nbytes[which] = b
@@ -543,10 +543,13 @@ def test_kernel_bench_covers_every_kernel():
     assert nbytes[0] == 12 * nnz + 20 * N
     assert nbytes[2] == 16 * N and nbytes[6] == 16 * N and nbytes[7] == 32 * N
     assert nbytes[8] == i["ncell"] * 64 * 8 * 27 * 27
+    assert nbytes[9] == (2 * 10 + 5) * 8 * N
     # the element kernel's timing leaves the level's matrix as it was
     assert np.array_equal(P.csr(l)[2], e0)
+    with pytest.raises(pm.CD3DError):  # the Gram-Schmidt step runs on the finest level only
+        P.kernel_bench(9, 1, 1)
     with pytest.raises(pm.CD3DError):
-        P.kernel_bench(9, l, 1)
+        P.kernel_bench(10, l, 1)
 
 
 # multicolour SSOR smoother (13): the HIP V-cycle and FGMRES against the
