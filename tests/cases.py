@@ -82,6 +82,10 @@ TCD_CASES = {
                    co=dict(eps=1e-4, conv_kind=1, rot_c=(0.5, 0.173), pulse_c=(0.3, 0.173, 0.173), f_val=0.0)),
     "m2_q1_tcd": dict(order=1, m=2, k0=1, levels=2, tau=1e-3, nsteps=3, box=UNIT_BOX,
                       co=dict(eps=1e-4, conv_kind=1, rot_c=(0.5, 0.5), pulse_c=(0.3, 0.5, 0.5), f_val=0.0)),
+    # SOR 2 + 2 (omega 1), as the reference's ReadIn/tcd3d.dat
+    "m2_q1_tcd_sor": dict(solver=dict(smoother=2, omega=1.0), order=1, m=2, k0=1, levels=2, tau=1e-3, nsteps=3,
+                          box=UNIT_BOX,
+                          co=dict(eps=1e-4, conv_kind=1, rot_c=(0.5, 0.5), pulse_c=(0.3, 0.5, 0.5), f_val=0.0)),
     "m2_q2_tcd": dict(order=2, m=2, k0=0, levels=2, tau=1e-3, nsteps=3, box=UNIT_BOX,
                       co=dict(eps=1e-4, conv_kind=1, rot_c=(0.5, 0.5), pulse_c=(0.3, 0.5, 0.5), f_val=0.0)),
     # SUPG-stabilised Crank-Nicolson (TCD3D.C:48,102): rotating field with

@@ -158,6 +158,14 @@ CASES = {
                               P0=1e-4, P4=0, P5=0, P6=0.0, P7=1, P8=3,
                               P9=0.3, P10=0.5, P11=0.5, P12=0.5, P13=0.5,
                               TIME_DISC=2, STARTTIME=0.0, ENDTIME=1.0, TIMESTEPLENGTH=1e-3)),
+    # Crank-Nicolson with SOR 2 + 2 (omega 1) and the SSOR coarse smoother, as
+    # the reference's ReadIn/tcd3d.dat sets them
+    "m2_q1_tcd_sor": ("tcd", dict(geo="m2", ANSATZ_ORDER=1, LEVELS=2, UNIFORM_STEPS=1, DISCTYPE=1,
+                                  SC_SMOOTHER_SCALAR=2, SC_SMOOTH_DAMP_FACTOR_SCALAR=1.0,
+                                  SC_SMOOTH_DAMP_FACTOR_FINE_SCALAR=1.0,
+                                  P0=1e-4, P4=0, P5=0, P6=0.0, P7=1, P8=3,
+                                  P9=0.3, P10=0.5, P11=0.5, P12=0.5, P13=0.5,
+                                  TIME_DISC=2, STARTTIME=0.0, ENDTIME=1.0, TIMESTEPLENGTH=1e-3)),
     "m2_q2_tcd": ("tcd", dict(geo="m2", ANSATZ_ORDER=2, LEVELS=2, UNIFORM_STEPS=0, DISCTYPE=1,
                               P0=1e-4, P4=0, P5=0, P6=0.0, P7=1, P8=3,
                               P9=0.3, P10=0.5, P11=0.5, P12=0.5, P13=0.5,
