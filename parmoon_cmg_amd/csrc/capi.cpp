@@ -899,9 +899,11 @@ int fgmres_iterate(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S
 
 // the buffers of the low-synchronisation Gram-Schmidt step (first use)
 void ensure_ls(cd3d_problem &P, int n) {
-  if (P.ls_part) return;
-  P.ls_part = dalloc<double>(P, (size_t)(2 * k_ls_max() + 1) * k_ls_blocks(n));
-  P.ls_L = dalloc<double>(P, (size_t)k_ls_max() * k_ls_max());
+  if (!P.ls_L) P.ls_L = dalloc<double>(P, (size_t)k_ls_max() * k_ls_max());
+  if (P.ls_blocks >= k_ls_blocks(n)) return;
+  if (P.ls_part) dfree(P, P.ls_part);
+  P.ls_blocks = k_ls_blocks(n);
+  P.ls_part = dalloc<double>(P, (size_t)(2 * k_ls_max() + 1) * P.ls_blocks);
 }
 
 // rfinal (optional): |b - A x| recomputed in the space after the iteration
@@ -1115,6 +1117,11 @@ static void batch_solve_impl(BatchRun &br, std::vector<BatchState> &st) {
   br.maxblk.assign(NL, 0);
   br.maxn.assign(NL, 0);
   br.maxn_c.assign(NL, 0);
+  // the mixed form orthogonalises in two passes over the basis per Arnoldi
+  // step (kb_ls_step) unless CD3D_FGMRES_MGS=1; the FP64 form keeps MGS
+  // vector by vector
+  const char *em = getenv("CD3D_FGMRES_MGS");
+  bool lowsync = sp.precision == CD3D_MIXED && R <= k_ls_max() && !(em && em[0] == '1');
   for (int b = 0; b < B; b++) {
     cd3d_problem &P = *br.ps[b];
     coarse_join(P);
@@ -1177,11 +1184,20 @@ static void batch_solve_impl(BatchRun &br, std::vector<BatchState> &st) {
     Q.scal = w.dscal + (size_t)b * kBatchScal;
     Q.coef = w.dcoef + (size_t)b * kBatchCoef;
     br.maxna = std::max(br.maxna, Q.na0);
+    // 16-byte pairs in the update pass
+    if (Q.ld % 2 || ((uintptr_t)Q.V & 15) || ((uintptr_t)Q.w & 15)) lowsync = false;
     k_copy(P.stream, F.ndof, F.rhs_asm, P.bs);
     HIPCHK(hipMemsetAsync(P.xs, 0, sizeof(double) * F.ndof, P.stream));
     k_copy_range(P.stream, F.nact, F.ndof, P.bs, P.xs);
     HIPCHK(hipStreamSynchronize(P.stream));
   }
+  if (lowsync)
+    for (int b = 0; b < B; b++) {
+      cd3d_problem &P = *br.ps[b];
+      ensure_ls(P, hp[b].n);
+      hp[b].ls_part = P.ls_part;
+      hp[b].ls_L = P.ls_L;
+    }
   HIPCHK(hipMemcpyAsync(w.dlev, hl.data(), sizeof(BatchLevel) * hl.size(), hipMemcpyHostToDevice, br.s));
   HIPCHK(hipMemcpyAsync(w.dprob, hp.data(), sizeof(BatchProblem) * B, hipMemcpyHostToDevice, br.s));
   if (sp.precision == CD3D_MIXED)
@@ -1281,11 +1297,17 @@ static void batch_solve_impl(BatchRun &br, std::vector<BatchState> &st) {
         kb_copy(br.s, B, br.maxn[top], FL, br.prob(), br.act(), kBSol, kBZ0 + i, false);
       }
       kb_stream(br.s, kSpmvMode, B, br.maxblk[top], FL, br.prob(), br.act(), kBZ0 + i, -1, kBW, 0.0);
-      kb_dot(br.s, B, FL, br.prob(), br.act(), kBW, kBV0, 0);
-      for (int k = 0; k <= i; k++)
-        kb_axpy_dot(br.s, B, FL, br.prob(), br.act(), k % 2, k, -1.0, kBV0 + k, kBW, k < i ? kBV0 + k + 1 : -1,
-                    1 - k % 2);
-      kb_scale_copy(br.s, B, FL, br.prob(), br.act(), (i + 1) % 2, i + 1, kBW, kBV0 + i + 1);
+      int last = 0;  // reduction slot holding the partials of |w|^2
+      if (lowsync) {
+        kb_ls_step(br.s, B, br.maxn[top], br.prob(), br.act(), i, 0);
+      } else {
+        kb_dot(br.s, B, FL, br.prob(), br.act(), kBW, kBV0, 0);
+        for (int k = 0; k <= i; k++)
+          kb_axpy_dot(br.s, B, FL, br.prob(), br.act(), k % 2, k, -1.0, kBV0 + k, kBW, k < i ? kBV0 + k + 1 : -1,
+                      1 - k % 2);
+        last = (i + 1) % 2;
+      }
+      kb_scale_copy(br.s, B, FL, br.prob(), br.act(), last, i + 1, kBW, kBV0 + i + 1);
       HIPCHK(hipMemcpyAsync(w.hscal, w.dscal, sizeof(double) * kBatchScal * B, hipMemcpyDeviceToHost, br.s));
       HIPCHK(hipStreamSynchronize(br.s));
       std::vector<std::pair<int, int>> done;

@@ -206,6 +206,7 @@ struct Problem {
   // low-synchronisation Gram-Schmidt of the mixed form: dot partials and the
   // strictly lower part of V^T V (allocated on first use)
   double *ls_part = nullptr, *ls_L = nullptr;
+  int ls_blocks = 0;          // chunks ls_part holds partials for
   double *hscal = nullptr;    // pinned host mirror of dscal
   int yoff = 64, nscal = 256;
   int *hflag = nullptr;       // pinned, lattice uniformity flags
@@ -387,6 +388,8 @@ struct BatchProblem {
   double *red = nullptr;          // 2 x kBatchRed reduction partials
   double *scal = nullptr;         // kBatchScal device scalars
   const double *coef = nullptr;   // kBatchCoef host-written coefficients
+  // the mixed form's low-synchronisation Gram-Schmidt: dot partials and L
+  double *ls_part = nullptr, *ls_L = nullptr;
 };
 // level kernels: lev = this level's table (B entries); grid sizes from the
 // largest problem.  f32: the level vectors are the FP32 ones (vec32,
@@ -422,6 +425,11 @@ void kb_axpy_dot(hipStream_t s, int B, const BatchLevel *fine, const BatchProble
                  int slot_in, int alpha_idx, double sign, int x, int y, int z, int slot_out);
 void kb_scale_copy(hipStream_t s, int B, const BatchLevel *fine, const BatchProblem *prob, const int *active,
                    int slot_in, int norm_idx, int x, int y);
+// the low-synchronisation Gram-Schmidt step i (i < k_ls_max()) of every
+// active problem (w = kBW against kBV0 .. kBV0 + i; maxn its largest n): the
+// column h to scal[0 .. i], w -= V h, the partials of |w|^2 in slot `slot`
+// (kb_scale_copy then normalises)
+void kb_ls_step(hipStream_t s, int B, int maxn, const BatchProblem *prob, const int *active, int i, int slot);
 // x *= coef[1] (problems whose coef[0] != 0)
 void kb_scale(hipStream_t s, int B, const BatchLevel *fine, const BatchProblem *prob, const int *active, int x);
 // xs += sum_k coef[1 + k] Z_k over k < coef[0] (problems whose coef[0] != 0; ignores the mask)

@@ -543,11 +543,11 @@ __global__ __launch_bounds__(kBlock) void scale_copy2_k(int n, const double *__r
 constexpr int kLsChunk = 2048;  // elements per workgroup of the dot pass
 constexpr int kLsMax = 32;      // restart length up to which the form is used
 
-// per-workgroup partials: r_k at part[k nb + b], g_k at part[(i + 1 + k) nb + b]
-__global__ __launch_bounds__(kBlock) void ls_dots_k(int n, int i, const double *__restrict__ V, int64_t ld,
-                                                    const double *__restrict__ w, double *__restrict__ part) {
-  __shared__ double ws[kLsChunk], vs[kLsChunk];
-  const int b = blockIdx.x, nb = gridDim.x;
+// chunk b of nb: partials r_k at part[k nb + b], g_k at part[(i + 1 + k) nb + b]
+// (ws, vs: kLsChunk doubles of LDS each)
+__device__ __forceinline__ void ls_dots_chunk(int n, int i, const double *__restrict__ V, int64_t ld,
+                                              const double *__restrict__ w, double *__restrict__ part, int b,
+                                              int nb, double *ws, double *vs) {
   const int64_t e0 = (int64_t)b * kLsChunk;
   const int ne = (int)min((int64_t)kLsChunk, (int64_t)n - e0);
   const double *Vi = V + (int64_t)i * ld + e0;
@@ -586,15 +586,21 @@ __global__ __launch_bounds__(kBlock) void ls_dots_k(int n, int i, const double *
     }
   }
 }
+__global__ __launch_bounds__(kBlock) void ls_dots_k(int n, int i, const double *__restrict__ V, int64_t ld,
+                                                    const double *__restrict__ w, double *__restrict__ part) {
+  __shared__ double ws[kLsChunk], vs[kLsChunk];
+  ls_dots_chunk(n, i, V, ld, w, part, blockIdx.x, gridDim.x, ws, vs);
+}
 
 // one workgroup of kLsCoef threads: the sums of the partials (a wave per
 // dot, four interleaved accumulators, a fixed order), row i of L (Lm, ld
 // kLsMax; rows 0 .. i-1 staged in LDS), and h = (I + L)^-1 r into h[0 .. i]
 // by one wave whose lane m holds h_m (one butterfly sum per row)
+// (d: 2 kLsMax + 1 doubles of LDS, Ls: kLsMax^2)
 constexpr int kLsCoef = 1024;
-__global__ __launch_bounds__(kLsCoef) void ls_coeffs_k(int i, int nb, const double *__restrict__ part,
-                                                       double *__restrict__ Lm, double *__restrict__ h) {
-  __shared__ double d[2 * kLsMax + 1], Ls[kLsMax * kLsMax];
+__device__ __forceinline__ void ls_coeffs_body(int i, int nb, const double *__restrict__ part,
+                                               double *__restrict__ Lm, double *__restrict__ h, double *d,
+                                               double *Ls) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int t = threadIdx.x; t < i * kLsMax; t += kLsCoef) Ls[t] = Lm[t];
   for (int q = wave; q < 2 * i + 1; q += kLsCoef / 64) {
@@ -627,19 +633,25 @@ __global__ __launch_bounds__(kLsCoef) void ls_coeffs_k(int i, int nb, const doub
     if (lane <= i) h[lane] = hm;
   }
 }
+__global__ __launch_bounds__(kLsCoef) void ls_coeffs_k(int i, int nb, const double *__restrict__ part,
+                                                       double *__restrict__ Lm, double *__restrict__ h) {
+  __shared__ double d[2 * kLsMax + 1], Ls[kLsMax * kLsMax];
+  ls_coeffs_body(i, nb, part, Lm, h, d, Ls);
+}
 
 // w -= sum_{k <= i} h_k V_k on 16-byte pairs, and the partial sums of |w|^2
-// (one per workgroup, kRedBlocks of them, as the other reductions)
-__global__ __launch_bounds__(kBlock) void ls_update2_k(int n, int i, const double *__restrict__ h,
-                                                       const double *__restrict__ V, int64_t ld,
-                                                       double *__restrict__ w, double *__restrict__ partials) {
-  __shared__ double sh[17], hs[kLsMax];
+// (workgroup blk of nblk writes partials[blk]; sh: 17 doubles of LDS, hs:
+// kLsMax)
+__device__ __forceinline__ void ls_update_part(int n, int i, const double *__restrict__ h,
+                                               const double *__restrict__ V, int64_t ld, double *__restrict__ w,
+                                               double *__restrict__ partials, int blk, int nblk, double *sh,
+                                               double *hs) {
   if (threadIdx.x <= i) hs[threadIdx.x] = h[threadIdx.x];
   __syncthreads();
   const int n2 = n >> 1;
   double2 *w2 = (double2 *)w;
   double v = 0.0;
-  for (int e = blockIdx.x * kBlock + threadIdx.x; e < n2; e += gridDim.x * kBlock) {
+  for (int e = blk * kBlock + threadIdx.x; e < n2; e += nblk * kBlock) {
     double2 x = w2[e];
     // the basis vectors four at a time: loads first, then the products
     int k = 0;
@@ -662,14 +674,20 @@ __global__ __launch_bounds__(kBlock) void ls_update2_k(int n, int i, const doubl
     v = fma(x.x, x.x, v);
     v = fma(x.y, x.y, v);
   }
-  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+  if ((n & 1) && blk == 0 && threadIdx.x == 0) {
     double x = w[n - 1];
     for (int k = 0; k <= i; k++) x = fma(-hs[k], V[(int64_t)k * ld + n - 1], x);
     w[n - 1] = x;
     v = fma(x, x, v);
   }
   v = block_sum(v, sh);
-  if (threadIdx.x == 0) partials[blockIdx.x] = v;
+  if (threadIdx.x == 0) partials[blk] = v;
+}
+__global__ __launch_bounds__(kBlock) void ls_update2_k(int n, int i, const double *__restrict__ h,
+                                                       const double *__restrict__ V, int64_t ld,
+                                                       double *__restrict__ w, double *__restrict__ partials) {
+  __shared__ double sh[17], hs[kLsMax];
+  ls_update_part(n, i, h, V, ld, w, partials, blockIdx.x, gridDim.x, sh, hs);
 }
 
 // x *= alpha and, when y32 is given, y32 = (float) x (scalar loop)
@@ -2209,6 +2227,38 @@ __global__ __launch_bounds__(kBlock) void bscale_copy_k(const BatchLevel *__rest
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < P.n; i += kBatchRed * kBlock) y[i] = x[i] * sc;
 }
 
+// the low-synchronisation Gram-Schmidt step of the single-problem kernels
+// above for every active problem of a batch (blockIdx.y = problem): dots over
+// the problem's own ceil(n / kLsChunk) chunks into P.ls_part, the MGS column
+// h into P.scal[0 .. i] (row i of L into P.ls_L), then w -= V h with the
+// partial sums of |w|^2 in reduction slot `slot`
+__global__ __launch_bounds__(kBlock) void bls_dots_k(const BatchProblem *__restrict__ prob,
+                                                     const int *__restrict__ active, int i) {
+  __shared__ double ws[kLsChunk], vs[kLsChunk];
+  const int b = blockIdx.y;
+  if (!active[b]) return;
+  const BatchProblem &P = prob[b];
+  const int nb = (P.n + kLsChunk - 1) / kLsChunk;
+  if ((int)blockIdx.x >= nb) return;
+  ls_dots_chunk(P.n, i, P.V, P.ld, P.w, P.ls_part, blockIdx.x, nb, ws, vs);
+}
+__global__ __launch_bounds__(kLsCoef) void bls_coeffs_k(const BatchProblem *__restrict__ prob,
+                                                        const int *__restrict__ active, int i) {
+  __shared__ double d[2 * kLsMax + 1], Ls[kLsMax * kLsMax];
+  const int b = blockIdx.y;
+  if (!active[b]) return;
+  const BatchProblem &P = prob[b];
+  ls_coeffs_body(i, (P.n + kLsChunk - 1) / kLsChunk, P.ls_part, P.ls_L, P.scal, d, Ls);
+}
+__global__ __launch_bounds__(kBlock) void bls_update_k(const BatchProblem *__restrict__ prob,
+                                                       const int *__restrict__ active, int i, int slot) {
+  __shared__ double sh[17], hs[kLsMax];
+  const int b = blockIdx.y;
+  if (!active[b]) return;
+  const BatchProblem &P = prob[b];
+  ls_update_part(P.n, i, P.scal, P.V, P.ld, P.w, P.red + slot * kBatchRed, blockIdx.x, kBatchRed, sh, hs);
+}
+
 __global__ __launch_bounds__(kBlock) void bscale_k(const BatchLevel *__restrict__ fine,
                                                    const BatchProblem *__restrict__ prob,
                                                    const int *__restrict__ active, int xid) {
@@ -2306,6 +2356,11 @@ void kb_finish(hipStream_t s, int B, const BatchProblem *prob, const int *active
 void kb_axpy_dot(hipStream_t s, int B, const BatchLevel *fine, const BatchProblem *prob, const int *active,
                  int slot_in, int alpha_idx, double sign, int x, int y, int z, int slot_out) {
   baxpy_dot_k<<<dim3(kBatchRed, B), kBlock, 0, s>>>(fine, prob, active, slot_in, alpha_idx, sign, x, y, z, slot_out);
+}
+void kb_ls_step(hipStream_t s, int B, int maxn, const BatchProblem *prob, const int *active, int i, int slot) {
+  bls_dots_k<<<dim3(k_ls_blocks(maxn), B), kBlock, 0, s>>>(prob, active, i);
+  bls_coeffs_k<<<dim3(1, B), kLsCoef, 0, s>>>(prob, active, i);
+  bls_update_k<<<dim3(kBatchRed, B), kBlock, 0, s>>>(prob, active, i, slot);
 }
 void kb_scale_copy(hipStream_t s, int B, const BatchLevel *fine, const BatchProblem *prob, const int *active,
                    int slot_in, int norm_idx, int x, int y) {

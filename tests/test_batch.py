@@ -96,6 +96,33 @@ def test_mixed_batch_matches_fp64_oracle():
         P.close()
 
 
+
+def test_mixed_batch_low_sync_and_mgs_agree(monkeypatch):
+    # the mixed batch orthogonalises with the low-synchronisation
+    # Gram-Schmidt (two passes over the basis per Arnoldi step, each member
+    # over its own number of 2048-element chunks: 1 .. 34 here);
+    # CD3D_FGMRES_MGS=1 takes MGS vector by vector.  The two are the same
+    # algorithm in exact arithmetic: iteration counts within one of each
+    # other, solutions within 2e-9.  Bitwise equality over all members would
+    # mean the two settings ran the same code
+    sp = pm.solver_params(**SOLVER, precision=pm.MIXED)
+    probs = [make(*c) for c in MIX]
+    monkeypatch.delenv("CD3D_FGMRES_MGS", raising=False)
+    st_ls = pm.solve_batch(probs, sp)
+    x_ls = [P.solution() for P in probs]
+    monkeypatch.setenv("CD3D_FGMRES_MGS", "1")
+    st_mgs = pm.solve_batch(probs, sp)
+    x_mgs = [P.solution() for P in probs]
+    monkeypatch.delenv("CD3D_FGMRES_MGS")
+    for c, a, b, xa, xb in zip(MIX, st_ls, st_mgs, x_ls, x_mgs):
+        d = np.linalg.norm(xa - xb) / np.linalg.norm(xb)
+        say(f"[gpu] mixed batch {c[:4]}: its low-sync {a.iterations}, MGS {b.iterations}, rel diff {d:.1e}")
+        assert a.status == 0 and b.status == 0
+        assert abs(a.iterations - b.iterations) <= 1 and d < 2e-9
+    assert not all(np.array_equal(xa, xb) for xa, xb in zip(x_ls, x_mgs))
+    for P in probs:
+        P.close()
+
 def test_batch_statuses_of_its_members():
     # a zero rhs (converged before the first iteration), a stop at maxit, a
     # divergence test that trips, and a normal member, all in one batch
