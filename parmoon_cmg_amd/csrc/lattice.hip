@@ -44,6 +44,25 @@ typedef float f2u __attribute__((ext_vector_type(2), aligned(4)));
 
 enum LatMode { kLJacobi = 0, kLJacobiZero = 1, kLDefect = 2 };
 
+// (kx, ky, kz) of node k of a class.  A class holds fewer than 2^31 nodes
+// (every level's CSR has 32-bit column indices), so 32-bit unsigned division
+// is exact and replaces three 64-bit division sequences per thread
+__host__ __device__ inline void class_coords(const LatDev &C, long long k, int &kx, int &ky, int &kz) {
+  const unsigned u = (unsigned)k, c0 = (unsigned)C.cnt[0], c1 = (unsigned)C.cnt[1], t = u / c0;
+  kx = (int)(u - t * c0);
+  ky = (int)(t % c1);
+  kz = (int)(t / c1);
+}
+
+// class of block blk from the classes' first blocks (nondecreasing): seven
+// independent comparisons instead of a scan that waits on each entry
+__host__ __device__ inline int block_class(const int *bstart, int nclass, int blk) {
+  int c = 0;
+#pragma unroll
+  for (int q = 1; q < 8; q++) c += (q < nclass && blk >= bstart[q]) ? 1 : 0;
+  return c;
+}
+
 template <int P, int PXS>
 __host__ __device__ inline constexpr int lo_off() { return (P == 1 || PXS == 0) ? -P : -1; }
 template <int P, int PXS>
@@ -80,7 +99,8 @@ __host__ __device__ inline void lat_node(const LatArgs &a, const LatDev &C, long
                                          const float *__restrict__ inv, const float *__restrict__ cst,
                                          const float *__restrict__ x, const float *__restrict__ f,
                                          float *__restrict__ out, float omega) {
-  const int kx = (int)(k % C.cnt[0]), ky = (int)((k / C.cnt[0]) % C.cnt[1]), kz = (int)(k / ((long long)C.cnt[0] * C.cnt[1]));
+  int kx, ky, kz;
+  class_coords(C, k, kx, ky, kz);
   const int X = C.lo[0] + C.step * kx, Y = C.lo[1] + C.step * ky, Z = C.lo[2] + C.step * kz;
   const long long nn = a.nn, node = X + nn * (Y + nn * Z);
   const float di = UNI ? cst[kCst - 1] : (MODE == kLDefect ? 0.f : inv[C.first + k]);
@@ -151,7 +171,8 @@ __host__ __device__ inline void lat_node_aff(const LatArgs &a, const LatDev &C, 
                                              const float *sx, const float *sy, const float *sm, float alpha,
                                              float alpha_m, int centre, const float *__restrict__ x,
                                              const float *__restrict__ f, float *__restrict__ out, float omega) {
-  const int kx = (int)(k % C.cnt[0]), ky = (int)((k / C.cnt[0]) % C.cnt[1]), kz = (int)(k / ((long long)C.cnt[0] * C.cnt[1]));
+  int kx, ky, kz;
+  class_coords(C, k, kx, ky, kz);
   const int X = C.lo[0] + C.step * kx, Y = C.lo[1] + C.step * ky, Z = C.lo[2] + C.step * kz;
   const long long nn = a.nn, node = X + nn * (Y + nn * Z);
   const float di = MODE == kLDefect ? 0.f : 1.0f / affine_value32(s0, sx, sy, sm, alpha, alpha_m, centre, X, Y);
@@ -201,8 +222,7 @@ __global__ __launch_bounds__(kB) void lat_aff_sweep_k(LatArgs a, LatAff af, cons
                                                       const float *__restrict__ f, float *__restrict__ out,
                                                       float omega) {
   __shared__ float st[4][kCst];  // S0, SX, SY, mass of the block's class, rounded to FP32
-  int c = 0;
-  while (c + 1 < a.nclass && (int)blockIdx.x >= a.bstart[c + 1]) c++;
+  const int c = block_class(a.bstart, a.nclass, (int)blockIdx.x);
   for (int i = threadIdx.x; i < kCst; i += kB) {
     st[0][i] = (float)af.st[c * kCst + i];
     st[1][i] = (float)af.st[8 * kCst + 8 + c * kCst + i];
@@ -247,12 +267,14 @@ template <int PY, int PZ>
 __host__ __device__ inline PairPos pair_pos(const LatArgs &a, long long k) {
   const LatDev &M = a.c[2 * PY + 4 * PZ + 1];
   const int n = M.cnt[0];  // pairs per line
-  const int ix = (int)(k % n);
-  const long long r = k / n;  // ky + cnt[1] kz, the same for both classes
-  const int Y = M.lo[1] + 2 * (int)(r % M.cnt[1]), Z = M.lo[2] + 2 * (int)(r / M.cnt[1]);
+  // 32-bit unsigned division: lattice indices stay below 2^31 (class_coords)
+  const unsigned u = (unsigned)k, r = u / (unsigned)n;  // ky + cnt[1] kz, the same for both classes
+  const unsigned c1 = (unsigned)M.cnt[1], rz = r / c1;
+  const int ix = (int)(u - r * (unsigned)n);
+  const int Y = M.lo[1] + 2 * (int)(r - rz * c1), Z = M.lo[2] + 2 * (int)rz;
   PairPos q;
   q.nv = 2 * ix + (long long)a.nn * (Y + (long long)a.nn * Z);
-  q.kv = r * (n - 1) + ix - 1;
+  q.kv = (long long)r * (n - 1) + ix - 1;
   q.hv = ix > 0;
   return q;
 }
@@ -403,9 +425,9 @@ template <int G, int PY, int PZ>
 __host__ __device__ inline long long group_pos(const LatArgs &a, long long k, bool &hv) {
   const LatDev &M = a.c[2 * PY + 4 * PZ + 1];
   const int ng = M.cnt[0] / G;  // groups per line
-  const int g = (int)(k % ng);
-  const long long r = k / ng;
-  const int Y = M.lo[1] + 2 * (int)(r % M.cnt[1]), Z = M.lo[2] + 2 * (int)(r / M.cnt[1]);
+  const unsigned u = (unsigned)k, r = u / (unsigned)ng, c1 = (unsigned)M.cnt[1], rz = r / c1;
+  const int g = (int)(u - r * (unsigned)ng);
+  const int Y = M.lo[1] + 2 * (int)(r - rz * c1), Z = M.lo[2] + 2 * (int)rz;
   hv = g > 0;
   return 2LL * G * g + (long long)a.nn * (Y + (long long)a.nn * Z);
 }
@@ -542,10 +564,10 @@ __host__ __device__ inline bool quad_pos(const LatArgs &a, long long k, QuadPos 
   const int n = a.c[1].cnt[0];  // pairs per line (and blocks per axis)
   const int ng = n / G;
   if (k >= (long long)ng * n * n) return false;
-  p.g = (int)(k % ng);
-  const long long r = k / ng;
-  p.by = (int)(r % n);
-  p.bz = (int)(r / n);
+  const unsigned u = (unsigned)k, r = u / (unsigned)ng, rz = r / (unsigned)n;
+  p.g = (int)(u - r * (unsigned)ng);
+  p.by = (int)(r - rz * (unsigned)n);
+  p.bz = (int)rz;
   p.base = 2LL * G * p.g + (long long)a.nn * (2 * p.by + (long long)a.nn * 2 * p.bz);
   p.hv = p.g > 0;
   return true;
@@ -682,8 +704,7 @@ __global__ __launch_bounds__(kB) void lat_sweep_k(LatArgs a, const float *__rest
                                                   const float *__restrict__ inv, const float *__restrict__ cst,
                                                   const float *__restrict__ x, const float *__restrict__ f,
                                                   float *__restrict__ out, float omega) {
-  int c = 0;
-  while (c + 1 < a.nclass && (int)blockIdx.x >= a.bstart[c + 1]) c++;
+  const int c = block_class(a.bstart, a.nclass, (int)blockIdx.x);
   const long long k = (long long)(blockIdx.x - a.bstart[c]) * kB + threadIdx.x;
   if (k < a.c[c].nodes) lat_class_node<P, MODE>(a, c, k, val, inv, cst, x, f, out, omega);
 }
@@ -726,7 +747,8 @@ __host__ __device__ inline void restrict_node(const LatArgs &ca, const LatDev &C
   using RX = Restr1D<P, PX>;
   using RY = Restr1D<P, PY>;
   using RZ = Restr1D<P, PZ>;
-  const int kx = (int)(k % C.cnt[0]), ky = (int)((k / C.cnt[0]) % C.cnt[1]), kz = (int)(k / ((long long)C.cnt[0] * C.cnt[1]));
+  int kx, ky, kz;
+  class_coords(C, k, kx, ky, kz);
   const int Xc = C.lo[0] + C.step * kx, Yc = C.lo[1] + C.step * ky, Zc = C.lo[2] + C.step * kz;
   const long long F = nnf, centre = 2 * Xc + F * (2 * Yc + F * 2 * Zc);
   float s = 0.f;
@@ -770,8 +792,7 @@ __host__ __device__ inline void restrict_class_node(const LatArgs &ca, int c, lo
 template <int P>
 __global__ __launch_bounds__(kB) void lat_restrict_k(LatArgs ca, int nnf, const float *__restrict__ fine,
                                                      float *__restrict__ coarse) {
-  int c = 0;
-  while (c + 1 < ca.nclass && (int)blockIdx.x >= ca.bstart[c + 1]) c++;
+  const int c = block_class(ca.bstart, ca.nclass, (int)blockIdx.x);
   const long long k = (long long)(blockIdx.x - ca.bstart[c]) * kB + threadIdx.x;
   if (k < ca.c[c].nodes) restrict_class_node<P>(ca, c, k, nnf, fine, coarse);
 }
@@ -792,8 +813,9 @@ __host__ __device__ inline void restrict_pair(const LatArgs &ca, long long k, in
   using RY = Restr1D<2, PY>;
   using RZ = Restr1D<2, PZ>;
   const PairPos q = pair_pos<PY, PZ>(ca, k);
-  const long long nnc = ca.nn, F = nnf;
-  const long long Xc = q.nv % nnc, Yc = (q.nv / nnc) % nnc, Zc = q.nv / (nnc * nnc);
+  const long long F = nnf;
+  const unsigned nnc = (unsigned)ca.nn, u = (unsigned)q.nv, t = u / nnc, tz = t / nnc;
+  const long long Xc = u - t * nnc, Yc = t - tz * nnc, Zc = tz;
   const long long centre = 2 * Xc + F * (2 * Yc + F * 2 * Zc);
   const int lo = q.hv ? 0 : 3;
   float s0 = 0.f, s1 = 0.f;
@@ -882,7 +904,8 @@ template <int P>
 __host__ __device__ inline void prolong_cell(long long t, int ncc, int nnf, int nnc, const float *__restrict__ coarse,
                                              float *__restrict__ x, float damp) {
   constexpr int P1 = P + 1, T = 2 * P;
-  const int cx = (int)(t % ncc), cy = (int)((t / ncc) % ncc), cz = (int)(t / ((long long)ncc * ncc));
+  const unsigned u = (unsigned)t, n = (unsigned)ncc, r = u / n, rz = r / n;
+  const int cx = (int)(u - r * n), cy = (int)(r - rz * n), cz = (int)rz;
   const long long C = nnc, F = nnf;
   float cv[P1][P1][P1];
 #pragma unroll
@@ -942,13 +965,15 @@ template <int P>
 __host__ __device__ inline void prolong_row(long long t, int ncc, int nnf, int nnc, const float *__restrict__ coarse,
                                             float *__restrict__ x, float damp) {
   constexpr int P1 = P + 1, T = 2 * P;
-  const int cx = (int)(t % ncc);
-  long long r = t / ncc;
+  // 32-bit unsigned division: t counts fine-level rows, below 2^31
+  const unsigned n = (unsigned)ncc;
+  unsigned r = (unsigned)t / n;
+  const int cx = (int)((unsigned)t - r * n);
   const int ty = (int)(r % T);
   r /= T;
   const int tz = (int)(r % T);
   r /= T;
-  const int cy = (int)(r % ncc), cz = (int)(r / ncc);
+  const int cy = (int)(r % n), cz = (int)(r / n);
   const int Y = T * cy + ty, Z = T * cz + tz;
   if (Y == 0 || Z == 0) return;
   const long long C = nnc, F = nnf;
@@ -1487,7 +1512,8 @@ template <int P, int PX, int PY, int PZ, int MODE>
 __host__ __device__ inline void lat64_node(const LatArgs &a, const LatDev &C, long long k, const double *__restrict__ st,
                                   const double *__restrict__ x, const double *__restrict__ f,
                                   double *__restrict__ out) {
-  const int kx = (int)(k % C.cnt[0]), ky = (int)((k / C.cnt[0]) % C.cnt[1]), kz = (int)(k / ((long long)C.cnt[0] * C.cnt[1]));
+  int kx, ky, kz;
+  class_coords(C, k, kx, ky, kz);
   const int X = C.lo[0] + C.step * kx, Y = C.lo[1] + C.step * ky, Z = C.lo[2] + C.step * kz;
   const long long nn = a.nn, node = X + nn * (Y + nn * Z);
   double s = 0.0;
@@ -1505,8 +1531,7 @@ __host__ __device__ inline void lat64_node(const LatArgs &a, const LatDev &C, lo
 template <int P, int MODE>
 __global__ __launch_bounds__(kB) void lat64_k(LatArgs a, const double *__restrict__ st64, const double *__restrict__ x,
                                               const double *__restrict__ f, double *__restrict__ out) {
-  int c = 0;
-  while (c + 1 < a.nclass && (int)blockIdx.x >= a.bstart[c + 1]) c++;
+  const int c = block_class(a.bstart, a.nclass, (int)blockIdx.x);
   const long long k = (long long)(blockIdx.x - a.bstart[c]) * kB + threadIdx.x;
   if (k >= a.c[c].nodes) return;
   const double *st = st64 + c * kCst;
@@ -1679,7 +1704,8 @@ __host__ __device__ inline void lat64_aff_node(const LatArgs &a, const LatDev &C
                                                const double *sx, const double *sy, const double *sm, double alpha,
                                                const double *__restrict__ x, const double *__restrict__ f,
                                                double *__restrict__ out) {
-  const int kx = (int)(k % C.cnt[0]), ky = (int)((k / C.cnt[0]) % C.cnt[1]), kz = (int)(k / ((long long)C.cnt[0] * C.cnt[1]));
+  int kx, ky, kz;
+  class_coords(C, k, kx, ky, kz);
   const int X = C.lo[0] + C.step * kx, Y = C.lo[1] + C.step * ky, Z = C.lo[2] + C.step * kz;
   const long long nn = a.nn, node = X + nn * (Y + nn * Z);
   double s = 0.0;
@@ -1711,8 +1737,7 @@ __global__ __launch_bounds__(kB) void lat64_aff_k(LatArgs a, LatAff af, const do
     }
     return;
   }
-  int c = 0;
-  while (c + 1 < a.nclass && (int)blockIdx.x >= a.bstart[c + 1]) c++;
+  const int c = block_class(a.bstart, a.nclass, (int)blockIdx.x);
   for (int i = threadIdx.x; i < kCst; i += kB) {
     st[0][i] = af.st[c * kCst + i];
     st[1][i] = af.st[8 * kCst + 8 + c * kCst + i];
