@@ -628,8 +628,18 @@ struct VC {
   const cd3d_solver_params &sp;
   int rec[32];     // mg_recursions (TMultiGrid3D::SetRecursion, MultiGrid3D.C:690-703)
   int small = -1;  // the level whose zero-start visit runs as one launch (mixed V-cycle; 0: none)
-  VC(cd3d_problem &p, const cd3d_solver_params &s) : P(p), sp(s) { reset(); }
+  // mixed V-cycle: bit i set when the restriction into level i also wrote
+  // its first pre-smoothing sweep from zero (into lt); CD3D_FUSE_ZERO_SWEEP=0
+  // launches that sweep instead
+  unsigned zero_ready = 0;
+  bool fuse_zero = true;
+  VC(cd3d_problem &p, const cd3d_solver_params &s) : P(p), sp(s) {
+    const char *e = getenv("CD3D_FUSE_ZERO_SWEEP");
+    fuse_zero = !(e && e[0] == '0');
+    reset();
+  }
   void reset() {
+    zero_ready = 0;
     rec[1] = 1;
     for (int k = 2; k < 32; k++) rec[k] = sp.cycle > 0 ? sp.cycle : 2;
   }
@@ -777,15 +787,26 @@ void cycleL(VC &v, int i, bool zero_start) {
   const int p = P.mesh.order;
   float *x = L.lx, *t = L.lt;
   const float om = (float)v.sp.omega;
+  const bool ready = zero_start && (v.zero_ready >> i & 1u);
+  v.zero_ready &= ~(1u << i);
   for (int k = 0; k < v.sp.pre; k++) {
-    if (k == 0 && zero_start) kl_jacobi_zero(P.stream, L, p, L.lf, t, om);
-    else kl_jacobi(P.stream, L, p, x, L.lf, t, om);
+    if (k == 0 && zero_start) {
+      if (!ready) kl_jacobi_zero(P.stream, L, p, L.lf, t, om);
+    } else {
+      kl_jacobi(P.stream, L, p, x, L.lf, t, om);
+    }
     std::swap(x, t);
   }
   const size_t nn3 = (size_t)L.nn * L.nn * L.nn;
   if (v.sp.pre == 0 && zero_start) HIPCHK(hipMemsetAsync(x, 0, sizeof(float) * nn3, P.stream));
   kl_defect(P.stream, L, p, x, L.lf, L.la);
-  kl_restrict(P.stream, L, C, p, L.la, C.lf);
+  // the coarse level's zero-start sweep rides on the restriction when that
+  // level runs the launch-per-sweep cycle (not the direct solve of level 0 or
+  // the one-launch bottom) with uniform stencils
+  const bool fuse = v.fuse_zero && v.sp.pre > 0 && i - 1 > 0 && i - 1 != v.small && i - 1 < 32 &&
+                    kl_zero_sweep_fusable(C);
+  kl_restrict(P.stream, L, C, p, L.la, C.lf, fuse ? C.lt : nullptr, om);
+  if (fuse) v.zero_ready |= 1u << (i - 1);
   for (int r = 0; r < v.rec[i]; r++) cycleL(v, i - 1, r == 0);
   if (v.sp.cycle < 1) v.rec[i] = 1;  // F-cycle
   kl_prolong_update(P.stream, L, C, p, C.lx, x, (float)v.sp.gmg_damp);

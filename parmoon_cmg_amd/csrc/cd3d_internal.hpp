@@ -458,7 +458,12 @@ void kl_jacobi(hipStream_t s, const DevLevel &L, int order, const float *x, cons
                float omega);
 void kl_jacobi_zero(hipStream_t s, const DevLevel &L, int order, const float *f, float *out, float omega);
 void kl_defect(hipStream_t s, const DevLevel &L, int order, const float *x, const float *f, float *d);
-void kl_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, int order, const float *fine, float *coarse);
+// zero_sweep: also write the coarse level's first pre-smoothing sweep from a
+// zero start, omega D^-1 coarse (needs kl_zero_sweep_fusable(C))
+void kl_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, int order, const float *fine, float *coarse,
+                 float *zero_sweep = nullptr, float omega = 0.f);
+// every class of C has a uniform stencil (and no affine field)
+bool kl_zero_sweep_fusable(const DevLevel &C);
 void kl_prolong_update(hipStream_t s, const DevLevel &F, const DevLevel &C, int order, const float *coarse, float *x,
                        float damp);
 // classes whose stencil is the same at every node (within 1e-12 of the

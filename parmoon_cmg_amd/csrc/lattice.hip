@@ -741,9 +741,22 @@ struct Restr1D<2, 1> {
   __host__ __device__ static constexpr float w(int i) { return i == 1 ? 1.0f : 0.75f; }
 };
 
+// The coarse level's first pre-smoothing sweep from a zero start,
+// x = omega D^-1 f (lat_node's kLJacobiZero), is pointwise: when every class
+// of the coarse level has a uniform stencil the restriction writes it too,
+// omega * f * d_c at the nodes it writes (d_c = cst[c kCst + kCst - 1], the
+// value and expression the sweep uses, so the bits are the same), and the
+// sweep's launch is saved.  x == nullptr: f only.
+struct ZeroSweep {
+  float *x;
+  const float *cst;
+  float omega;
+};
+
 template <int P, int PX, int PY, int PZ>
 __host__ __device__ inline void restrict_node(const LatArgs &ca, const LatDev &C, long long k, int nnf,
-                                              const float *__restrict__ fine, float *__restrict__ coarse) {
+                                              const float *__restrict__ fine, float *__restrict__ coarse,
+                                              int cls, const ZeroSweep &zs) {
   using RX = Restr1D<P, PX>;
   using RY = Restr1D<P, PY>;
   using RZ = Restr1D<P, PZ>;
@@ -764,25 +777,28 @@ __host__ __device__ inline void restrict_node(const LatArgs &ca, const LatDev &C
     }
     s += RZ::w(c) * sy;
   }
-  coarse[Xc + (long long)ca.nn * (Yc + (long long)ca.nn * Zc)] = s;
+  const long long node = Xc + (long long)ca.nn * (Yc + (long long)ca.nn * Zc);
+  coarse[node] = s;
+  if (zs.x) zs.x[node] = zs.omega * s * zs.cst[cls * kCst + kCst - 1];
 }
 
 template <int P>
 __host__ __device__ inline void restrict_class_node(const LatArgs &ca, int c, long long k, int nnf,
-                                                    const float *__restrict__ fine, float *__restrict__ coarse) {
+                                                    const float *__restrict__ fine, float *__restrict__ coarse,
+                                                    const ZeroSweep &zs) {
   const LatDev &C = ca.c[c];
   if (P == 1) {
-    restrict_node<1, 0, 0, 0>(ca, C, k, nnf, fine, coarse);
+    restrict_node<1, 0, 0, 0>(ca, C, k, nnf, fine, coarse, c, zs);
   } else {
     switch (c) {
-      case 0: restrict_node<2, 0, 0, 0>(ca, C, k, nnf, fine, coarse); break;
-      case 1: restrict_node<2, 1, 0, 0>(ca, C, k, nnf, fine, coarse); break;
-      case 2: restrict_node<2, 0, 1, 0>(ca, C, k, nnf, fine, coarse); break;
-      case 3: restrict_node<2, 1, 1, 0>(ca, C, k, nnf, fine, coarse); break;
-      case 4: restrict_node<2, 0, 0, 1>(ca, C, k, nnf, fine, coarse); break;
-      case 5: restrict_node<2, 1, 0, 1>(ca, C, k, nnf, fine, coarse); break;
-      case 6: restrict_node<2, 0, 1, 1>(ca, C, k, nnf, fine, coarse); break;
-      default: restrict_node<2, 1, 1, 1>(ca, C, k, nnf, fine, coarse); break;
+      case 0: restrict_node<2, 0, 0, 0>(ca, C, k, nnf, fine, coarse, c, zs); break;
+      case 1: restrict_node<2, 1, 0, 0>(ca, C, k, nnf, fine, coarse, c, zs); break;
+      case 2: restrict_node<2, 0, 1, 0>(ca, C, k, nnf, fine, coarse, c, zs); break;
+      case 3: restrict_node<2, 1, 1, 0>(ca, C, k, nnf, fine, coarse, c, zs); break;
+      case 4: restrict_node<2, 0, 0, 1>(ca, C, k, nnf, fine, coarse, c, zs); break;
+      case 5: restrict_node<2, 1, 0, 1>(ca, C, k, nnf, fine, coarse, c, zs); break;
+      case 6: restrict_node<2, 0, 1, 1>(ca, C, k, nnf, fine, coarse, c, zs); break;
+      default: restrict_node<2, 1, 1, 1>(ca, C, k, nnf, fine, coarse, c, zs); break;
     }
   }
 }
@@ -791,10 +807,10 @@ __host__ __device__ inline void restrict_class_node(const LatArgs &ca, int c, lo
 // boundary stays zero); ca: the coarse level's classes
 template <int P>
 __global__ __launch_bounds__(kB) void lat_restrict_k(LatArgs ca, int nnf, const float *__restrict__ fine,
-                                                     float *__restrict__ coarse) {
+                                                     float *__restrict__ coarse, ZeroSweep zs) {
   const int c = block_class(ca.bstart, ca.nclass, (int)blockIdx.x);
   const long long k = (long long)(blockIdx.x - ca.bstart[c]) * kB + threadIdx.x;
-  if (k < ca.c[c].nodes) restrict_class_node<P>(ca, c, k, nnf, fine, coarse);
+  if (k < ca.c[c].nodes) restrict_class_node<P>(ca, c, k, nnf, fine, coarse, zs);
 }
 
 // Q2 restriction by coarse vertex/midpoint pairs along X (the pair classes
@@ -807,7 +823,7 @@ __global__ __launch_bounds__(kB) void lat_restrict_k(LatArgs ca, int nnf, const 
 // written; its left values are read from the centre.
 template <int PY, int PZ>
 __host__ __device__ inline void restrict_pair(const LatArgs &ca, long long k, int nnf, const float *__restrict__ fine,
-                                              float *__restrict__ coarse) {
+                                              float *__restrict__ coarse, const ZeroSweep &zs) {
   using RX0 = Restr1D<2, 0>;
   using RX1 = Restr1D<2, 1>;
   using RY = Restr1D<2, PY>;
@@ -848,15 +864,22 @@ __host__ __device__ inline void restrict_pair(const LatArgs &ca, long long k, in
   }
   if (q.hv) coarse[q.nv] = s0;
   coarse[q.nv + 1] = s1;
+  if (zs.x) {
+    // vertex class 2 PY + 4 PZ, midpoint class one above
+    constexpr int cv = 2 * PY + 4 * PZ;
+    if (q.hv) zs.x[q.nv] = zs.omega * s0 * zs.cst[cv * kCst + kCst - 1];
+    zs.x[q.nv + 1] = zs.omega * s1 * zs.cst[(cv + 1) * kCst + kCst - 1];
+  }
 }
 
 __host__ __device__ inline void restrict_pair_node(const LatArgs &ca, int pc, long long k, int nnf,
-                                                   const float *__restrict__ fine, float *__restrict__ coarse) {
+                                                   const float *__restrict__ fine, float *__restrict__ coarse,
+                                                   const ZeroSweep &zs) {
   switch (pc) {
-    case 0: restrict_pair<0, 0>(ca, k, nnf, fine, coarse); break;
-    case 1: restrict_pair<1, 0>(ca, k, nnf, fine, coarse); break;
-    case 2: restrict_pair<0, 1>(ca, k, nnf, fine, coarse); break;
-    default: restrict_pair<1, 1>(ca, k, nnf, fine, coarse); break;
+    case 0: restrict_pair<0, 0>(ca, k, nnf, fine, coarse, zs); break;
+    case 1: restrict_pair<1, 0>(ca, k, nnf, fine, coarse, zs); break;
+    case 2: restrict_pair<0, 1>(ca, k, nnf, fine, coarse, zs); break;
+    default: restrict_pair<1, 1>(ca, k, nnf, fine, coarse, zs); break;
   }
 }
 
@@ -870,7 +893,7 @@ __host__ __device__ inline void restrict_pair_node(const LatArgs &ca, int pc, lo
 // mixed solve is unchanged; class order is the default
 // (CD3D_RESTRICT_INTER=1 selects the other, bitwise the same)
 __global__ __launch_bounds__(kB) void lat_restrict_pair_k(LatArgs ca, int nnf, const float *__restrict__ fine,
-                                                          float *__restrict__ coarse, int inter) {
+                                                          float *__restrict__ coarse, int inter, ZeroSweep zs) {
   int pc = 0, j;
   if (inter) {
     const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -882,7 +905,7 @@ __global__ __launch_bounds__(kB) void lat_restrict_pair_k(LatArgs ca, int nnf, c
     j = (int)blockIdx.x - ca.pstart[pc];
   }
   const long long k = (long long)j * kB + threadIdx.x;
-  if (k < ca.c[2 * pc + 1].nodes) restrict_pair_node(ca, pc, k, nnf, fine, coarse);
+  if (k < ca.c[2 * pc + 1].nodes) restrict_pair_node(ca, pc, k, nnf, fine, coarse, zs);
 }
 
 // Prolongation + CorrectNodes + Update, one thread per coarse cell: the cell's
@@ -1098,7 +1121,7 @@ __global__ __launch_bounds__(small_threads<P>()) void lat_small_k(SmallArgs s) {
     __syncthreads();
     for (int c = 0; c < C.a.nclass; c++)
       for (long long k = threadIdx.x; k < C.a.c[c].nodes; k += NT)
-        restrict_class_node<P>(C.a, c, k, L.a.nn, L.la, C.lf);
+        restrict_class_node<P>(C.a, c, k, L.a.nn, L.la, C.lf, ZeroSweep{nullptr, nullptr, 0.f});
     xs[i] = x;
     __syncthreads();
   }
@@ -1967,19 +1990,29 @@ void kl_jacobi_zero(hipStream_t s, const DevLevel &L, int order, const float *f,
 void kl_defect(hipStream_t s, const DevLevel &L, int order, const float *x, const float *f, float *d) {
   lat_sweep<kLDefect>(s, L, order, x, f, d, 0.f);
 }
-void kl_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, int order, const float *fine, float *coarse) {
+bool kl_zero_sweep_fusable(const DevLevel &C) {
+  if (C.lat_affine || C.nclass <= 0) return false;
+  for (int c = 0; c < C.nclass; c++)
+    if (!C.uni[c]) return false;
+  return true;
+}
+void kl_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, int order, const float *fine, float *coarse,
+                 float *zero_sweep, float omega) {
   const LatArgs ca = lat_args(C);
   const int grid = ca.bstart[C.nclass];
   if (grid <= 0) return;
-  if (order == 1) lat_restrict_k<1><<<grid, kB, 0, s>>>(ca, F.nn, fine, coarse);
+  if (zero_sweep && !kl_zero_sweep_fusable(C))
+    throw std::logic_error("the restriction writes the zero-start sweep only for uniform class stencils");
+  const ZeroSweep zs{zero_sweep, C.lcst, omega};
+  if (order == 1) lat_restrict_k<1><<<grid, kB, 0, s>>>(ca, F.nn, fine, coarse, zs);
   else if (lat_pairs() && C.nn > kPairMinNN) {
     const char *e = getenv("CD3D_RESTRICT_INTER");
     const int inter = e ? atoi(e) : 0;
     int nbmax = 0;
     for (int pc = 0; pc < 4; pc++) nbmax = std::max(nbmax, ca.pstart[pc + 1] - ca.pstart[pc]);
-    lat_restrict_pair_k<<<inter ? 4 * nbmax : ca.pstart[4], kB, 0, s>>>(ca, F.nn, fine, coarse, inter);
+    lat_restrict_pair_k<<<inter ? 4 * nbmax : ca.pstart[4], kB, 0, s>>>(ca, F.nn, fine, coarse, inter, zs);
   }
-  else lat_restrict_k<2><<<grid, kB, 0, s>>>(ca, F.nn, fine, coarse);
+  else lat_restrict_k<2><<<grid, kB, 0, s>>>(ca, F.nn, fine, coarse, zs);
 }
 void kl_prolong_update(hipStream_t s, const DevLevel &F, const DevLevel &C, int order, const float *coarse, float *x,
                        float damp) {

@@ -218,15 +218,33 @@ int check(int m, int k0) {
     DC.nclass = (int)C.lat.size();
     for (int c = 0; c < DC.nclass; c++) DC.cls[c] = C.lat[c];
     LatArgs ca = lat_args(DC);
+    // uniform class stencils: the restriction also writes the zero-start
+    // sweep omega d_c f, which must be the bits of the kLJacobiZero sweep
+    std::vector<float> cst(8 * kCst, 0.f), zc((size_t)nc * nc * nc, 0.f), zref(zc.size(), 0.f);
+    for (int c = 0; c < 8; c++) cst[c * kCst + kCst - 1] = (float)(0.5 + 0.5 * std::fabs(rnd(g)));
+    const float om = 0.7f;
+    const ZeroSweep zs{zc.data(), cst.data(), om};
     for (int c = 0; c < DC.nclass; c++)
-      for (int64_t k = 0; k < C.lat[c].nodes; k++) restrict_class_node<P>(ca, c, k, nn, fl.data(), cl.data());
+      for (int64_t k = 0; k < C.lat[c].nodes; k++) restrict_class_node<P>(ca, c, k, nn, fl.data(), cl.data(), zs);
+    LatArgs cu = ca;
+    for (int c = 0; c < DC.nclass; c++) cu.uni[c] = 1;
+    for (int c = 0; c < DC.nclass; c++)
+      for (int64_t k = 0; k < C.lat[c].nodes; k++)
+        lat_class_node<P, kLJacobiZero>(cu, c, k, nullptr, nullptr, cst.data(), nullptr, cl.data(), zref.data(), om);
+    int64_t zd = 0;
+    for (size_t i = 0; i < zc.size(); i++) zd += memcmp(&zc[i], &zref[i], sizeof(float)) != 0;
+    printf("   restriction's zero-start sweep vs the sweep: %lld differ\n", (long long)zd);
+    if (zd) bad++;
     if (P == 2) {  // by coarse vertex/midpoint pairs: the same bits
-      std::vector<float> cp((size_t)nc * nc * nc, 0.f);
+      std::vector<float> cp((size_t)nc * nc * nc, 0.f), zp(zc.size(), 0.f);
+      const ZeroSweep zsp{zp.data(), cst.data(), om};
       for (int pc = 0; pc < 4; pc++)
-        for (int64_t k = 0; k < ca.c[2 * pc + 1].nodes; k++) restrict_pair_node(ca, pc, k, nn, fl.data(), cp.data());
+        for (int64_t k = 0; k < ca.c[2 * pc + 1].nodes; k++)
+          restrict_pair_node(ca, pc, k, nn, fl.data(), cp.data(), zsp);
       int64_t rd = 0;
-      for (size_t i = 0; i < cp.size(); i++) rd += memcmp(&cp[i], &cl[i], sizeof(float)) != 0;
-      printf("   restriction by pairs: %lld differ\n", (long long)rd);
+      for (size_t i = 0; i < cp.size(); i++)
+        rd += (memcmp(&cp[i], &cl[i], sizeof(float)) != 0) + (memcmp(&zp[i], &zref[i], sizeof(float)) != 0);
+      printf("   restriction by pairs (and its zero-start sweep): %lld differ\n", (long long)rd);
       if (rd) bad++;
     }
   }

@@ -432,6 +432,38 @@ def test_pair_groups_match_single_pairs(m, k0, levels, monkeypatch):
         assert out[g][2] == out["1"][2]
 
 
+@pytest.mark.parametrize("order,m,k0,levels,pre,cycle", [
+    (2, 1, 2, 5, 3, 1), (1, 1, 2, 5, 3, 1), (2, 3, 0, 4, 1, 2), (2, 1, 1, 4, 2, 0), (1, 3, 0, 3, 0, 1)])
+def test_restriction_zero_sweep_matches_the_sweep(order, m, k0, levels, pre, cycle, monkeypatch):
+    # the mixed V-cycle's restriction also writes the coarse level's first
+    # pre-smoothing sweep from zero (omega D^-1 f, pointwise) when that
+    # level's stencils are uniform, and the sweep's launch is skipped: the
+    # V-cycle and the solve are bitwise those with the sweep launched
+    # (CD3D_FUSE_ZERO_SWEEP=0).  Q2 64^3 cells puts a pair-form level
+    # (65^3 nodes) under the fine one; V (1), W (2) and F (0) cycles; pre = 0
+    # runs no such sweep
+    P = pm.Problem(order, m, k0, levels, box=(1.0, 1.0, 1.0))
+    P.assemble(pm.coeffs(eps=1e-3, b=(1.0, 0.5, 0.25), supg=1, f_val=1.0))
+    i = P.info(levels - 1)
+    r = np.random.default_rng(9).standard_normal(i["ndof"])
+    r[i["nact"]:] = 0.0
+    sp = pm.solver_params(**dict(REF_SOLVER, omega=0.5, pre=pre, post=3, coarse_smoother=17, rel_tol=1e-10,
+                                 cycle=cycle), precision=pm.MIXED)
+    out = {}
+    for fz in ("1", "0"):
+        monkeypatch.setenv("CD3D_FUSE_ZERO_SWEEP", fz)
+        z = P.vcycle(sp, r)
+        x, st = P.solve(sp, check=False)
+        out[fz] = (z, x, st.iterations, st.status)
+    say(f"[gpu] zero-start sweep in the restriction order={order} cells={m * 2 ** (k0 + levels - 1)} pre={pre} "
+        f"cycle={cycle}: its {out['1'][2]} / {out['0'][2]}, status {out['1'][3]}")
+    assert out["1"][3] == out["0"][3]
+    assert np.array_equal(out["1"][0], out["0"][0])
+    assert np.array_equal(out["1"][1], out["0"][1])
+    assert out["1"][2] == out["0"][2]
+    P.close()
+
+
 @pytest.mark.parametrize("order,m,k0,levels", [(1, 1, 2, 4), (2, 1, 1, 3)])
 def test_rotating_field_lattice_fgmres(order, m, k0, levels):
     # Galerkin rotating field: the CSR values are the affine class stencils,
