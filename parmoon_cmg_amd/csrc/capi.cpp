@@ -872,10 +872,14 @@ struct Space {
   double *x, *b;
   std::function<void(const double *, const double *, double *)> defect;
   std::function<void(const double *, double *)> matvec;
-  // precond(v, r, z, r32_ready): z = M^-1 r; r32_ready: r32 already holds
-  // (float) r, written by the kernel that produced r
-  std::function<void(VC &, const double *, double *, bool)> precond;
+  // precond(v, r, z, r32_state): z = M^-1 r; r32_state 1: r32 already holds
+  // (float) r, written by the kernel that produced r; 2: that kernel also
+  // wrote the finest level's first pre-smoothing sweep from zero (into lt);
+  // 0: neither
+  std::function<void(VC &, const double *, double *, int)> precond;
   float *r32 = nullptr;  // FP32 copy of the preconditioner input (lattice space)
+  DevLevel *lat_top = nullptr;  // the finest level of the lattice V-cycle
+  int lat_order = 0;
   // orthogonalise by the low-synchronisation form of MGS (k_ls_step)
   // instead of MGS step by step (the FP64 form keeps the reference's MGS)
   bool lowsync = false;
@@ -889,7 +893,7 @@ Space dof_space(cd3d_problem &P) {
   S.b = P.bs;
   S.defect = [&P, &F](const double *x, const double *f, double *d) { k_defect(P.stream, F, x, f, d); };
   S.matvec = [&P, &F](const double *x, double *y) { k_spmv(P.stream, F, x, y); };
-  S.precond = [](VC &v, const double *r, double *z, bool) { precondition(v, r, z, true); };
+  S.precond = [](VC &v, const double *r, double *z, int) { precondition(v, r, z, true); };
   return S;
 }
 
@@ -907,9 +911,12 @@ Space lattice_space(cd3d_problem &P) {
   // CD3D_FGMRES_MGS=1: MGS step by step as in the FP64 form
   const char *em = getenv("CD3D_FGMRES_MGS");
   S.lowsync = !(em && em[0] == '1');
-  S.precond = [&P, &F, top](VC &v, const double *r, double *z, bool r32_ready) {
+  S.lat_top = &F;
+  S.lat_order = p;
+  S.precond = [&P, &F, top](VC &v, const double *r, double *z, int r32_state) {
     v.reset();
-    if (!r32_ready) kl_cast_d2f(P.stream, F.ndof, r, F.lf);
+    if (r32_state == 0) kl_cast_d2f(P.stream, F.ndof, r, F.lf);
+    if (r32_state == 2 && top < 32) v.zero_ready |= 1u << top;
     cycleL(v, top, true);
     kl_cast_f2d(P.stream, F.ndof, F.lx, z);
   };
@@ -954,6 +961,17 @@ int fgmres_iterate(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S
   const bool lowsync = S.lowsync && R <= k_ls_max() && ld % 2 == 0 && ((uintptr_t)V & 15) == 0 &&
                        ((uintptr_t)w & 15) == 0;
   if (lowsync) ensure_ls(P, N);
+  // the finest lattice level's zero-start sweep written beside each Krylov
+  // vector's FP32 copy (TopZeroSweep); CD3D_FUSE_ZERO_SWEEP=0: launched
+  TopZeroSweep zt{};
+  bool topz = false;
+  if (S.lat_top && S.r32 && sp.pre > 0) {
+    const char *ez = getenv("CD3D_FUSE_ZERO_SWEEP");
+    if (!(ez && ez[0] == '0'))
+      topz = kl_top_zero_sweep(*S.lat_top, S.lat_order, (float)sp.omega, S.lat_top->lt, zt);
+  }
+  const TopZeroSweep *ztp = topz ? &zt : nullptr;
+  const int r32_state = S.r32 ? (topz ? 2 : 1) : 0;
   S.defect(S.x, S.b, Vk(0));
   double beta = dev_norm(P, N, Vk(0));
   r0 = beta;
@@ -978,12 +996,12 @@ int fgmres_iterate(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S
     HIPCHK(hipStreamSynchronize(P.stream));  // pinned y is reused next time
   };
   while (j <= sp.maxit) {
-    k_scale_cast(P.stream, N, 1.0 / beta, Vk(0), S.r32);  // Dscal(N, 1/beta, v[0]), and its FP32 copy
+    k_scale_cast(P.stream, N, 1.0 / beta, Vk(0), S.r32, ztp);  // Dscal(N, 1/beta, v[0]), and its FP32 copy
     std::fill(s.begin(), s.end(), 0.0);
     s[0] = beta;
     int i;
     for (i = 0; i < R && j <= sp.maxit; i++, j++) {
-      S.precond(v, Vk(i), Zk(i), S.r32 != nullptr);
+      S.precond(v, Vk(i), Zk(i), r32_state);
       S.matvec(Zk(i), w);
       double *last;
       if (lowsync) {
@@ -999,7 +1017,7 @@ int fgmres_iterate(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S
         }
         last = (i % 2 == 0) ? pb : pa;
       }
-      k_scale_copy(P.stream, N, last, P.dscal + i + 1, w, Vk(i + 1), S.r32);
+      k_scale_copy(P.stream, N, last, P.dscal + i + 1, w, Vk(i + 1), S.r32, ztp);
       HIPCHK(hipMemcpyAsync(P.hscal, P.dscal, sizeof(double) * (i + 2), hipMemcpyDeviceToHost, P.stream));
       HIPCHK(hipStreamSynchronize(P.stream));
       for (int k = 0; k <= i + 1; k++) H[(size_t)k * R + i] = P.hscal[k];

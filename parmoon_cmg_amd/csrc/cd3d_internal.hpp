@@ -308,10 +308,23 @@ void k_dot(hipStream_t s, int n, const double *a, const double *b, double *parti
 void k_axpy_dot(hipStream_t s, int n, const double *alpha_partials, double sign, const double *x, double *y,
                 const double *z, double *partials, double *alpha_out);
 // y32 (optional): also (float) y
+// The finest lattice level's first pre-smoothing sweep from zero, written
+// beside the FP32 copy of a Krylov vector (which is that V-cycle's rhs f):
+// x[e] = omega f[e] d[c dstride] at the interior lattice nodes e (1 <= X,
+// Y, Z <= nn - 2), c the node's parity class (X & 1) + 2 (Y & 1) + 4 (Z & 1)
+// for Q2, 0 for Q1; other entries untouched.  x == nullptr: none.
+struct TopZeroSweep {
+  float *x = nullptr;
+  const float *d = nullptr;
+  int dstride = 0;
+  float omega = 0.f;
+  int nn = 0, order = 0;
+};
+// zt: the finest level's zero-start sweep beside y32 (y32 in lattice order)
 void k_scale_copy(hipStream_t s, int n, const double *partials, double *norm_out, const double *x, double *y,
-                  float *y32 = nullptr);
+                  float *y32 = nullptr, const TopZeroSweep *zt = nullptr);
 // x *= alpha, and y32 = (float) x when y32 is given
-void k_scale_cast(hipStream_t s, int n, double alpha, double *x, float *y32);
+void k_scale_cast(hipStream_t s, int n, double alpha, double *x, float *y32, const TopZeroSweep *zt = nullptr);
 void k_axpy(hipStream_t s, int n, double alpha, const double *x, double *y);
 void k_scale(hipStream_t s, int n, double alpha, double *x);
 void k_axpy_dev(hipStream_t s, int n, const double *alpha_dev, int idx, const double *x, double *y);
@@ -464,6 +477,8 @@ void kl_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, int order,
                  float *zero_sweep = nullptr, float omega = 0.f);
 // every class of C has a uniform stencil (and no affine field)
 bool kl_zero_sweep_fusable(const DevLevel &C);
+// zt for the finest level F (x: its lt) when its classes allow it
+bool kl_top_zero_sweep(const DevLevel &F, int order, float omega, float *x, TopZeroSweep &zt);
 void kl_prolong_update(hipStream_t s, const DevLevel &F, const DevLevel &C, int order, const float *coarse, float *x,
                        float damp);
 // classes whose stencil is the same at every node (within 1e-12 of the

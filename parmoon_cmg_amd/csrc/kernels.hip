@@ -499,9 +499,22 @@ __global__ __launch_bounds__(kBlock) void scale_copy_k(int n, const double *__re
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) y[i] = x[i] * s;
 }
 
+// the finest lattice level's zero-start sweep at lattice node e from the
+// FP32 value f just stored there (TopZeroSweep)
+__device__ inline void top_zero_sweep(const TopZeroSweep &t, int64_t e, float f) {
+  // 32-bit unsigned division: a level has fewer than 2^31 lattice nodes
+  const unsigned u = (unsigned)e, nn = (unsigned)t.nn, q = u / nn, Z = q / nn;
+  const unsigned X = u - q * nn, Y = q - Z * nn;
+  if (X - 1u < nn - 2u && Y - 1u < nn - 2u && Z - 1u < nn - 2u) {
+    const int c = t.order == 2 ? (int)((X & 1u) + 2u * (Y & 1u) + 4u * (Z & 1u)) : 0;
+    t.x[e] = t.omega * f * t.d[c * t.dstride];
+  }
+}
+
 __global__ __launch_bounds__(kBlock) void scale_copy2_k(int n, const double *__restrict__ partials,
                                                         double *__restrict__ norm_out, const double *__restrict__ x,
-                                                        double *__restrict__ y, float *__restrict__ y32) {
+                                                        double *__restrict__ y, float *__restrict__ y32,
+                                                        TopZeroSweep zt) {
   __shared__ double sh[17];
   const int n2 = n >> 1, gs = gridDim.x * kBlock;
   const double2 *x2 = (const double2 *)x;
@@ -515,17 +528,35 @@ __global__ __launch_bounds__(kBlock) void scale_copy2_k(int n, const double *__r
   if (i < n2) {
     const double2 r = {v0.x * s, v0.y * s};
     y2[i] = r;
-    if (y32) ((f2 *)y32)[i] = f2{(float)r.x, (float)r.y};
+    if (y32) {
+      const f2 r32 = f2{(float)r.x, (float)r.y};
+      ((f2 *)y32)[i] = r32;
+      if (zt.x) {
+        top_zero_sweep(zt, 2 * (int64_t)i, r32.x);
+        top_zero_sweep(zt, 2 * (int64_t)i + 1, r32.y);
+      }
+    }
   }
   for (i += gs; i < n2; i += gs) {
     const double2 v = x2[i];
     const double2 r = {v.x * s, v.y * s};
     y2[i] = r;
-    if (y32) ((f2 *)y32)[i] = f2{(float)r.x, (float)r.y};
+    if (y32) {
+      const f2 r32 = f2{(float)r.x, (float)r.y};
+      ((f2 *)y32)[i] = r32;
+      if (zt.x) {
+        top_zero_sweep(zt, 2 * (int64_t)i, r32.x);
+        top_zero_sweep(zt, 2 * (int64_t)i + 1, r32.y);
+      }
+    }
   }
   if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
     y[n - 1] = x[n - 1] * s;
-    if (y32) y32[n - 1] = (float)(x[n - 1] * s);
+    if (y32) {
+      const float v32 = (float)(x[n - 1] * s);
+      y32[n - 1] = v32;
+      if (zt.x) top_zero_sweep(zt, n - 1, v32);
+    }
   }
 }
 
@@ -691,11 +722,15 @@ __global__ __launch_bounds__(kBlock) void ls_update2_k(int n, int i, const doubl
 }
 
 // x *= alpha and, when y32 is given, y32 = (float) x (scalar loop)
-__global__ void scale_cast_k(int n, double alpha, double *__restrict__ x, float *__restrict__ y32) {
+__global__ void scale_cast_k(int n, double alpha, double *__restrict__ x, float *__restrict__ y32, TopZeroSweep zt) {
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const double v = x[i] * alpha;
     x[i] = v;
-    if (y32) y32[i] = (float)v;
+    if (y32) {
+      const float v32 = (float)v;
+      y32[i] = v32;
+      if (zt.x) top_zero_sweep(zt, i, v32);
+    }
   }
 }
 
@@ -1765,12 +1800,13 @@ void k_axpy_dot(hipStream_t s, int n, const double *alpha_partials, double sign,
   }
 }
 void k_scale_copy(hipStream_t s, int n, const double *partials, double *norm_out, const double *x, double *y,
-                  float *y32) {
+                  float *y32, const TopZeroSweep *zt) {
+  const TopZeroSweep z = zt && y32 ? *zt : TopZeroSweep{};
   if (al16(x) && al16(y) && ((uintptr_t)y32 & 7) == 0) {
-    scale_copy2_k<<<kRedBlocks, kBlock, 0, s>>>(n, partials, norm_out, x, y, y32);
+    scale_copy2_k<<<kRedBlocks, kBlock, 0, s>>>(n, partials, norm_out, x, y, y32, z);
   } else {
     scale_copy_k<<<kRedBlocks, kBlock, 0, s>>>(n, partials, norm_out, x, y);
-    if (y32) scale_cast_k<<<grid_for(n), kBlock, 0, s>>>(n, 1.0, y, y32);
+    if (y32) scale_cast_k<<<grid_for(n), kBlock, 0, s>>>(n, 1.0, y, y32, z);
   }
 }
 int k_ls_blocks(int n) { return (n + kLsChunk - 1) / kLsChunk; }
@@ -1782,8 +1818,8 @@ void k_ls_step(hipStream_t s, int n, int i, const double *V, int64_t ld, double 
   ls_coeffs_k<<<1, kLsCoef, 0, s>>>(i, nb, part, Lm, h);
   ls_update2_k<<<kRedBlocks, kBlock, 0, s>>>(n, i, h, V, ld, w, partials);
 }
-void k_scale_cast(hipStream_t s, int n, double alpha, double *x, float *y32) {
-  scale_cast_k<<<grid_for(n), kBlock, 0, s>>>(n, alpha, x, y32);
+void k_scale_cast(hipStream_t s, int n, double alpha, double *x, float *y32, const TopZeroSweep *zt) {
+  scale_cast_k<<<grid_for(n), kBlock, 0, s>>>(n, alpha, x, y32, zt && y32 ? *zt : TopZeroSweep{});
 }
 void k_scale(hipStream_t s, int n, double alpha, double *x) { scale_k<<<grid_for(n), kBlock, 0, s>>>(n, alpha, x); }
 void k_axpy(hipStream_t s, int n, double alpha, const double *x, double *y) {

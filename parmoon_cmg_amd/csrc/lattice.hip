@@ -1996,6 +1996,30 @@ bool kl_zero_sweep_fusable(const DevLevel &C) {
     if (!C.uni[c]) return false;
   return true;
 }
+bool kl_top_zero_sweep(const DevLevel &F, int order, float omega, float *x, TopZeroSweep &zt) {
+  // uniform stencils, and classes that are exactly the parity sub-lattices of
+  // the interior (class c holds the nodes of parity c, all of them)
+  if (!kl_zero_sweep_fusable(F) || F.nclass != (order == 2 ? 8 : 1)) return false;
+  const int nn = F.nn;
+  long long tot = 0;
+  for (int c = 0; c < F.nclass; c++) {
+    const LatClass &L = F.cls[c];
+    if (L.step != order) return false;
+    for (int ax = 0; ax < 3; ax++) {
+      if (L.cnt[ax] <= 0 || L.lo[ax] < 1 || L.lo[ax] + L.step * (L.cnt[ax] - 1) > nn - 2) return false;
+      if (order == 2 && (L.lo[ax] & 1) != ((c >> ax) & 1)) return false;
+    }
+    tot += L.nodes;
+  }
+  if (tot != (long long)(nn - 2) * (nn - 2) * (nn - 2)) return false;
+  zt.x = x;
+  zt.d = F.lcst + kCst - 1;
+  zt.dstride = kCst;
+  zt.omega = omega;
+  zt.nn = nn;
+  zt.order = order;
+  return true;
+}
 void kl_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, int order, const float *fine, float *coarse,
                  float *zero_sweep, float omega) {
   const LatArgs ca = lat_args(C);

@@ -437,8 +437,10 @@ def test_pair_groups_match_single_pairs(m, k0, levels, monkeypatch):
 def test_restriction_zero_sweep_matches_the_sweep(order, m, k0, levels, pre, cycle, monkeypatch):
     # the mixed V-cycle's restriction also writes the coarse level's first
     # pre-smoothing sweep from zero (omega D^-1 f, pointwise) when that
-    # level's stencils are uniform, and the sweep's launch is skipped: the
-    # V-cycle and the solve are bitwise those with the sweep launched
+    # level's stencils are uniform, and in the lattice FGMRES the kernel that
+    # writes each Krylov vector's FP32 copy writes the finest level's; the
+    # sweeps' launches are skipped.  The V-cycle (coarse levels) and the
+    # solve (all levels) are bitwise those with the sweeps launched
     # (CD3D_FUSE_ZERO_SWEEP=0).  Q2 64^3 cells puts a pair-form level
     # (65^3 nodes) under the fine one; V (1), W (2) and F (0) cycles; pre = 0
     # runs no such sweep
