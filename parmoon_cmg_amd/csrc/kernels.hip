@@ -500,14 +500,18 @@ __global__ __launch_bounds__(kBlock) void scale_copy_k(int n, const double *__re
 }
 
 // the finest lattice level's zero-start sweep at lattice node e from the
-// FP32 value f just stored there (TopZeroSweep)
-__device__ inline void top_zero_sweep(const TopZeroSweep &t, int64_t e, float f) {
+// FP32 value f just stored there (TopZeroSweep); d: the 8 class values of
+// t.d staged in LDS by top_zero_stage
+__device__ inline void top_zero_stage(const TopZeroSweep &t, float *d) {
+  if (t.x && threadIdx.x < 8) d[threadIdx.x] = threadIdx.x < (t.order == 2 ? 8u : 1u) ? t.d[threadIdx.x * t.dstride] : 0.f;
+}
+__device__ inline void top_zero_sweep(const TopZeroSweep &t, const float *d, int64_t e, float f) {
   // 32-bit unsigned division: a level has fewer than 2^31 lattice nodes
   const unsigned u = (unsigned)e, nn = (unsigned)t.nn, q = u / nn, Z = q / nn;
   const unsigned X = u - q * nn, Y = q - Z * nn;
   if (X - 1u < nn - 2u && Y - 1u < nn - 2u && Z - 1u < nn - 2u) {
     const int c = t.order == 2 ? (int)((X & 1u) + 2u * (Y & 1u) + 4u * (Z & 1u)) : 0;
-    t.x[e] = t.omega * f * t.d[c * t.dstride];
+    t.x[e] = t.omega * f * d[c];
   }
 }
 
@@ -516,11 +520,13 @@ __global__ __launch_bounds__(kBlock) void scale_copy2_k(int n, const double *__r
                                                         double *__restrict__ y, float *__restrict__ y32,
                                                         TopZeroSweep zt) {
   __shared__ double sh[17];
+  __shared__ float zd[8];
   const int n2 = n >> 1, gs = gridDim.x * kBlock;
   const double2 *x2 = (const double2 *)x;
   double2 *y2 = (double2 *)y;
   int i = blockIdx.x * kBlock + threadIdx.x;
   const double2 v0 = i < n2 ? x2[i] : double2{0, 0};  // issued before the reduction
+  top_zero_stage(zt, zd);                              // visible after its barriers
   const double nrm = sqrt(sum_partials(partials, sh));
   if (blockIdx.x == 0 && threadIdx.x == 0) *norm_out = nrm;
   const double s = nrm != 0.0 ? 1.0 / nrm : 1.0;
@@ -532,8 +538,8 @@ __global__ __launch_bounds__(kBlock) void scale_copy2_k(int n, const double *__r
       const f2 r32 = f2{(float)r.x, (float)r.y};
       ((f2 *)y32)[i] = r32;
       if (zt.x) {
-        top_zero_sweep(zt, 2 * (int64_t)i, r32.x);
-        top_zero_sweep(zt, 2 * (int64_t)i + 1, r32.y);
+        top_zero_sweep(zt, zd, 2 * (int64_t)i, r32.x);
+        top_zero_sweep(zt, zd, 2 * (int64_t)i + 1, r32.y);
       }
     }
   }
@@ -545,8 +551,8 @@ __global__ __launch_bounds__(kBlock) void scale_copy2_k(int n, const double *__r
       const f2 r32 = f2{(float)r.x, (float)r.y};
       ((f2 *)y32)[i] = r32;
       if (zt.x) {
-        top_zero_sweep(zt, 2 * (int64_t)i, r32.x);
-        top_zero_sweep(zt, 2 * (int64_t)i + 1, r32.y);
+        top_zero_sweep(zt, zd, 2 * (int64_t)i, r32.x);
+        top_zero_sweep(zt, zd, 2 * (int64_t)i + 1, r32.y);
       }
     }
   }
@@ -555,7 +561,7 @@ __global__ __launch_bounds__(kBlock) void scale_copy2_k(int n, const double *__r
     if (y32) {
       const float v32 = (float)(x[n - 1] * s);
       y32[n - 1] = v32;
-      if (zt.x) top_zero_sweep(zt, n - 1, v32);
+      if (zt.x) top_zero_sweep(zt, zd, n - 1, v32);
     }
   }
 }
@@ -723,13 +729,16 @@ __global__ __launch_bounds__(kBlock) void ls_update2_k(int n, int i, const doubl
 
 // x *= alpha and, when y32 is given, y32 = (float) x (scalar loop)
 __global__ void scale_cast_k(int n, double alpha, double *__restrict__ x, float *__restrict__ y32, TopZeroSweep zt) {
+  __shared__ float zd[8];
+  top_zero_stage(zt, zd);
+  __syncthreads();
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const double v = x[i] * alpha;
     x[i] = v;
     if (y32) {
       const float v32 = (float)v;
       y32[i] = v32;
-      if (zt.x) top_zero_sweep(zt, i, v32);
+      if (zt.x) top_zero_sweep(zt, zd, i, v32);
     }
   }
 }

@@ -757,6 +757,8 @@ template <int P, int PX, int PY, int PZ>
 __host__ __device__ inline void restrict_node(const LatArgs &ca, const LatDev &C, long long k, int nnf,
                                               const float *__restrict__ fine, float *__restrict__ coarse,
                                               int cls, const ZeroSweep &zs) {
+  // the class value first: its load overlaps the gathers
+  const float dz = zs.x ? zs.cst[cls * kCst + kCst - 1] : 0.f;
   using RX = Restr1D<P, PX>;
   using RY = Restr1D<P, PY>;
   using RZ = Restr1D<P, PZ>;
@@ -779,7 +781,7 @@ __host__ __device__ inline void restrict_node(const LatArgs &ca, const LatDev &C
   }
   const long long node = Xc + (long long)ca.nn * (Yc + (long long)ca.nn * Zc);
   coarse[node] = s;
-  if (zs.x) zs.x[node] = zs.omega * s * zs.cst[cls * kCst + kCst - 1];
+  if (zs.x) zs.x[node] = zs.omega * s * dz;
 }
 
 template <int P>
@@ -824,6 +826,10 @@ __global__ __launch_bounds__(kB) void lat_restrict_k(LatArgs ca, int nnf, const 
 template <int PY, int PZ>
 __host__ __device__ inline void restrict_pair(const LatArgs &ca, long long k, int nnf, const float *__restrict__ fine,
                                               float *__restrict__ coarse, const ZeroSweep &zs) {
+  // vertex class 2 PY + 4 PZ, midpoint class one above; loaded first so the
+  // loads overlap the gathers
+  constexpr int cv = 2 * PY + 4 * PZ;
+  const float dz0 = zs.x ? zs.cst[cv * kCst + kCst - 1] : 0.f, dz1 = zs.x ? zs.cst[(cv + 1) * kCst + kCst - 1] : 0.f;
   using RX0 = Restr1D<2, 0>;
   using RX1 = Restr1D<2, 1>;
   using RY = Restr1D<2, PY>;
@@ -865,10 +871,8 @@ __host__ __device__ inline void restrict_pair(const LatArgs &ca, long long k, in
   if (q.hv) coarse[q.nv] = s0;
   coarse[q.nv + 1] = s1;
   if (zs.x) {
-    // vertex class 2 PY + 4 PZ, midpoint class one above
-    constexpr int cv = 2 * PY + 4 * PZ;
-    if (q.hv) zs.x[q.nv] = zs.omega * s0 * zs.cst[cv * kCst + kCst - 1];
-    zs.x[q.nv + 1] = zs.omega * s1 * zs.cst[(cv + 1) * kCst + kCst - 1];
+    if (q.hv) zs.x[q.nv] = zs.omega * s0 * dz0;
+    zs.x[q.nv + 1] = zs.omega * s1 * dz1;
   }
 }
 
