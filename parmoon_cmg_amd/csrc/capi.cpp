@@ -880,6 +880,10 @@ struct Space {
   float *r32 = nullptr;  // FP32 copy of the preconditioner input (lattice space)
   DevLevel *lat_top = nullptr;  // the finest level of the lattice V-cycle
   int lat_order = 0;
+  // the Krylov columns z_i kept in FP32 (the V-cycle's FP32 output, exact):
+  // precond leaves z in lat_top->lx (z == nullptr) and the product
+  // kl_spmv64_f32 reads it there and writes the column
+  bool z32 = false;
   // orthogonalise by the low-synchronisation form of MGS (k_ls_step)
   // instead of MGS step by step (the FP64 form keeps the reference's MGS)
   bool lowsync = false;
@@ -913,12 +917,15 @@ Space lattice_space(cd3d_problem &P) {
   S.lowsync = !(em && em[0] == '1');
   S.lat_top = &F;
   S.lat_order = p;
+  // CD3D_FGMRES_Z32=0: FP64 columns, written by a cast after each V-cycle
+  const char *ez = getenv("CD3D_FGMRES_Z32");
+  S.z32 = !(ez && ez[0] == '0') && kl_spmv64_f32_ok(F, p);
   S.precond = [&P, &F, top](VC &v, const double *r, double *z, int r32_state) {
     v.reset();
     if (r32_state == 0) kl_cast_d2f(P.stream, F.ndof, r, F.lf);
     if (r32_state == 2 && top < 32) v.zero_ready |= 1u << top;
     cycleL(v, top, true);
-    kl_cast_f2d(P.stream, F.ndof, F.lx, z);
+    if (z) kl_cast_f2d(P.stream, F.ndof, F.lx, z);
   };
   return S;
 }
@@ -956,6 +963,8 @@ int fgmres_iterate(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S
   if (ld < N) throw std::logic_error("Krylov basis smaller than the space");
   auto Vk = [&](int k) { return V + (int64_t)k * ld; };
   auto Zk = [&](int k) { return Z + (int64_t)k * ld; };
+  // FP32 columns at the same stride, in the same buffer
+  float *Z32 = S.z32 ? (float *)Z : nullptr;
   std::vector<double> H((size_t)(R + 1) * R, 0.0), s(R + 1), cs(R + 1), sn(R + 1);
   double *pa = P.red, *pb = P.red + kRedBlocks;
   const bool lowsync = S.lowsync && R <= k_ls_max() && ld % 2 == 0 && ((uintptr_t)V & 15) == 0 &&
@@ -992,7 +1001,8 @@ int fgmres_iterate(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S
     memcpy(P.hscal + P.yoff, y.data(), sizeof(double) * (k + 1));
     HIPCHK(hipMemcpyAsync(P.dscal + P.yoff, P.hscal + P.yoff, sizeof(double) * (k + 1), hipMemcpyHostToDevice,
                           P.stream));
-    k_multi_axpy(P.stream, N, k + 1, P.dscal + P.yoff, Z, ld, S.x);
+    if (Z32) k_multi_axpy32(P.stream, N, k + 1, P.dscal + P.yoff, Z32, ld, S.x);
+    else k_multi_axpy(P.stream, N, k + 1, P.dscal + P.yoff, Z, ld, S.x);
     HIPCHK(hipStreamSynchronize(P.stream));  // pinned y is reused next time
   };
   while (j <= sp.maxit) {
@@ -1001,8 +1011,13 @@ int fgmres_iterate(cd3d_problem &P, const cd3d_solver_params &sp, const Space &S
     s[0] = beta;
     int i;
     for (i = 0; i < R && j <= sp.maxit; i++, j++) {
-      S.precond(v, Vk(i), Zk(i), r32_state);
-      S.matvec(Zk(i), w);
+      if (Z32) {
+        S.precond(v, Vk(i), nullptr, r32_state);
+        kl_spmv64_f32(P.stream, *S.lat_top, S.lat_order, S.lat_top->lx, w, Z32 + (int64_t)i * ld);
+      } else {
+        S.precond(v, Vk(i), Zk(i), r32_state);
+        S.matvec(Zk(i), w);
+      }
       double *last;
       if (lowsync) {
         // the MGS column in two passes over the basis (k_ls_step)

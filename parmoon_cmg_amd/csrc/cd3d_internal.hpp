@@ -479,6 +479,12 @@ void kl_restrict(hipStream_t s, const DevLevel &F, const DevLevel &C, int order,
 bool kl_zero_sweep_fusable(const DevLevel &C);
 // zt for the finest level F (x: its lt) when its classes allow it
 bool kl_top_zero_sweep(const DevLevel &F, int order, float omega, float *x, TopZeroSweep &zt);
+// y = A x on the lattice with FP32 x (converted exactly: the products of
+// kl_spmv64 on the FP64 copy of x); xcopy (optional) receives x at every
+// lattice node.  Only where kl_spmv64 takes its quad form
+// (kl_spmv64_f32_ok)
+bool kl_spmv64_f32_ok(const DevLevel &L, int order);
+void kl_spmv64_f32(hipStream_t s, const DevLevel &L, int order, const float *x, double *y, float *xcopy);
 void kl_prolong_update(hipStream_t s, const DevLevel &F, const DevLevel &C, int order, const float *coarse, float *x,
                        float damp);
 // classes whose stencil is the same at every node (within 1e-12 of the
@@ -534,6 +540,8 @@ void k_coarse_from_lattice(hipStream_t s, const DevLevel &C, const double *Ainv,
 // the inversion takes more than one launch (larger than the one-workgroup LDS form)
 bool k_invert_multi_launch(int na);
 void k_multi_axpy(hipStream_t s, int n, int m, const double *coef, const double *Z, int64_t ldz, double *x);
+// the same with FP32 columns (converted exactly)
+void k_multi_axpy32(hipStream_t s, int n, int m, const double *coef, const float *Z, int64_t ldz, double *x);
 void k_copy_range(hipStream_t s, int b, int e, const double *src, double *dst);
 
 }  // namespace cd3d

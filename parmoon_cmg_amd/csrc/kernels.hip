@@ -1613,6 +1613,18 @@ __global__ void multi_axpy2_k(int n, int m, const double *__restrict__ coef, con
   }
 }
 
+// x += sum_k coef[k] Z_k with FP32 columns (the mixed form's Krylov columns
+// z_k, FP32 V-cycle outputs): each value converted exactly, so the sums of
+// multi_axpy_k over their FP64 copies
+__global__ void multi_axpy32_k(int n, int m, const double *__restrict__ coef, const float *__restrict__ Z,
+                               int64_t ldz, double *__restrict__ x) {
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    double s = 0;
+    for (int k = 0; k < m; k++) s += coef[k] * (double)Z[k * ldz + i];
+    x[i] += s;
+  }
+}
+
 __global__ void multi_axpy_k(int n, int m, const double *__restrict__ coef, const double *__restrict__ Z,
                              int64_t ldz, double *__restrict__ x) {
   for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
@@ -2028,6 +2040,9 @@ bool k_invert_multi_launch(int na) { return na > kGjLdsMax; }
 int64_t gj_scratch_doubles(int na) {
   // M and M' (na x 2na each), W (na x kGjPanel), dval, partials, used + perm (ints)
   return (int64_t)4 * na * na + (int64_t)(kGjPanel + 1) * na + 512 + (512 + 2 * (int64_t)na + 2) / 2 + 1;
+}
+void k_multi_axpy32(hipStream_t s, int n, int m, const double *coef, const float *Z, int64_t ldz, double *x) {
+  multi_axpy32_k<<<grid_for(n), kBlock, 0, s>>>(n, m, coef, Z, ldz, x);
 }
 void k_multi_axpy(hipStream_t s, int n, int m, const double *coef, const double *Z, int64_t ldz, double *x) {
   if (al16(Z) && al16(x) && ldz % 2 == 0) multi_axpy2_k<<<grid_for((n + 1) / 2), kBlock, 0, s>>>(n, m, coef, Z, ldz, x);

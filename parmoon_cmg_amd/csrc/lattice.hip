@@ -574,14 +574,16 @@ __host__ __device__ inline bool quad_pos(const LatArgs &a, long long k, QuadPos 
 }
 
 // the stencil sums of the 4 lines of a quad (cs: the 8 classes' compact
-// stencils, class c at cs + c kCst; xs: the lines' own x values)
-template <typename T, int G>
+// stencils, class c at cs + c kCst; xs: the lines' own x values).  x is read
+// as TX and each value converted to T (FP32 x in an FP64 sum: exact, so the
+// sums are those of the FP64 copy of x)
+template <typename T, int G, typename TX = T>
 __host__ __device__ inline void quad_sums(const LatArgs &a, const QuadPos &p, const T *cs,
-                                          const T *__restrict__ x, T (&s0)[4][G], T (&s1)[4][G],
+                                          const TX *__restrict__ x, T (&s0)[4][G], T (&s1)[4][G],
                                           T (&xs)[4][2 * G]) {
-  typedef typename Vec2<T>::type V2;
-  typedef typename Vec16<T>::type V16;
-  constexpr int vn = Vec16<T>::n;
+  typedef typename Vec2<TX>::type V2;
+  typedef typename Vec16<TX>::type V16;
+  constexpr int vn = Vec16<TX>::n;
   const long long nn = a.nn;
   const int Y0 = 2 * p.by, Z0 = 2 * p.bz;
 #pragma unroll
@@ -595,24 +597,24 @@ __host__ __device__ inline void quad_sums(const LatArgs &a, const QuadPos &p, co
 #pragma unroll
     for (int dy = -2; dy <= 2; dy++) {
       const int yy = Y0 + dy < 0 ? 0 : Y0 + dy;
-      const T *xl = x + 2LL * G * p.g + nn * (yy + nn * zz);
+      const TX *xl = x + 2LL * G * p.g + nn * (yy + nn * zz);
       T w[2 * G + 3];
       const V2 lft = *(const V2 *)(xl + lft_off);
-      w[0] = lft.x;
-      w[1] = lft.y;
+      w[0] = (T)lft.x;
+      w[1] = (T)lft.y;
 #pragma unroll
       for (int h = 0; h < 2 * G / vn; h++) {
         const V16 m = *(const V16 *)(xl + vn * h);
 #pragma unroll
-        for (int e = 0; e < vn; e++) w[2 + vn * h + e] = m[e];
+        for (int e = 0; e < vn; e++) w[2 + vn * h + e] = (T)m[e];
       }
       if constexpr (2 * G < vn) {
         // one pair per line in FP32: the pair in one 8-byte load
         const V2 m = *(const V2 *)xl;
-        w[2] = m.x;
-        w[3] = m.y;
+        w[2] = (T)m.x;
+        w[3] = (T)m.y;
       }
-      w[2 * G + 2] = xl[2 * G];
+      w[2 * G + 2] = (T)xl[2 * G];
 #pragma unroll
       for (int oz = 0; oz < 2; oz++) {
         const int dzr = dz - oz, loz = oz ? -1 : -2, hiz = oz ? 1 : 2;
@@ -1682,19 +1684,26 @@ __global__ __launch_bounds__(kB) void lat64_pair_gk(LatArgs a, const double *__r
 
 // The FP64 lattice operator by quads of lines (quad_sums over the FP64
 // compact stencils); blocks from nquad on take the Dirichlet rows (identity).
-// Per node the same sums as lat64_pair_group.
-template <int G, int MODE>
+// Per node the same sums as lat64_pair_group.  TX = float: x is the FP32
+// V-cycle output, converted exactly, and xcopy (when given) receives x at
+// every node written (every lattice node), the FP32 Krylov column z_i.
+template <int G, int MODE, typename TX = double>
 __global__ __launch_bounds__(kB) void lat64_quad_gk(LatArgs a, const double *__restrict__ st64,
-                                                    const double *__restrict__ x, const double *__restrict__ f,
+                                                    const TX *__restrict__ x, const double *__restrict__ f,
                                                     double *__restrict__ out, int nquad, int nact, int ndof,
-                                                    const int *__restrict__ dof2node, int xcd) {
+                                                    const int *__restrict__ dof2node, int xcd,
+                                                    TX *__restrict__ xcopy = nullptr) {
   __shared__ double cs[8 * kCst];
   const int bid0 = (int)blockIdx.x;
   if (bid0 >= nquad) {
     const int d = nact + (bid0 - nquad) * kB + threadIdx.x;
     if (d < ndof) {
       const int node = dof2node[d];
-      out[node] = MODE == kL64Defect ? f[node] - x[node] : x[node];
+      const TX xv = x[node];
+      out[node] = MODE == kL64Defect ? f[node] - (double)xv : (double)xv;
+      if constexpr (sizeof(TX) == 4) {
+        if (xcopy) xcopy[node] = xv;
+      }
     }
     return;
   }
@@ -1705,7 +1714,7 @@ __global__ __launch_bounds__(kB) void lat64_quad_gk(LatArgs a, const double *__r
   if (!quad_pos<G>(a, (long long)bid * kB + threadIdx.x, p)) return;
   const long long nn = a.nn;
   double s0[4][G], s1[4][G], xs[4][2 * G];
-  quad_sums<double, G>(a, p, cs, x, s0, s1, xs);
+  quad_sums<double, G, TX>(a, p, cs, x, s0, s1, xs);
 #pragma unroll
   for (int pc = 0; pc < 4; pc++) {
     const int oy = pc & 1, oz = pc >> 1;
@@ -1716,6 +1725,13 @@ __global__ __launch_bounds__(kB) void lat64_quad_gk(LatArgs a, const double *__r
       const long long v = nv + 2 * q;
       if (q > 0 || p.hv) out[v] = MODE == kL64Defect ? f[v] - s0[pc][q] : s0[pc][q];
       out[v + 1] = MODE == kL64Defect ? f[v + 1] - s1[pc][q] : s1[pc][q];
+      if constexpr (sizeof(TX) == 4) {
+        // the node values again (just read: cache hits), not held in registers
+        if (xcopy) {
+          if (q > 0 || p.hv) xcopy[v] = x[v];
+          xcopy[v + 1] = x[v + 1];
+        }
+      }
     }
   }
 }
@@ -2231,6 +2247,27 @@ void kl_affine64_apply(hipStream_t s, const DevLevel &L, int order, double alpha
 }
 void kl_spmv64(hipStream_t s, const DevLevel &L, int order, const double *x, double *y) {
   lat64_apply<kL64Spmv>(s, L, order, x, nullptr, y);
+}
+// the quad form of lat64_apply (the one taken for order 2 on uniform levels
+// above kPairMinNN nodes per side with quads and pair groups on)
+static bool lat64_quad_form(const DevLevel &L, int order, const LatArgs &a) {
+  return !L.lat_aff64 && order == 2 && a.bstart[L.nclass] > 0 && lat_pairs() && L.nn > kPairMinNN &&
+         pair_group(a, 2) > 1 && lat_quads();
+}
+bool kl_spmv64_f32_ok(const DevLevel &L, int order) {
+  const LatArgs a = lat_args(L);
+  const char *eg = getenv("CD3D_QUAD64_G");
+  return lat64_quad_form(L, order, a) && !(eg && atoi(eg) == 2);
+}
+void kl_spmv64_f32(hipStream_t s, const DevLevel &L, int order, const float *x, double *y, float *xcopy) {
+  const LatArgs a = lat_args(L);
+  if (!kl_spmv64_f32_ok(L, order)) throw std::logic_error("FP32-input lattice operator on a level without quads");
+  const char *ex = getenv("CD3D_QUAD_XCD");
+  const int xcd = ex ? atoi(ex) : 1, nd = L.ndof - L.nact;
+  const long long n = a.c[1].cnt[0];
+  const int nq = (int)((n * n * n + kB - 1) / kB), ng = nq + (nd + kB - 1) / kB;
+  lat64_quad_gk<1, kL64Spmv, float><<<ng, kB, 0, s>>>(a, L.lst64, x, nullptr, y, nq, L.nact, L.ndof, L.ldof2node,
+                                                        xcd, xcopy);
 }
 void kl_defect64(hipStream_t s, const DevLevel &L, int order, const double *x, const double *f, double *d) {
   lat64_apply<kL64Defect>(s, L, order, x, f, d);

@@ -466,6 +466,30 @@ def test_restriction_zero_sweep_matches_the_sweep(order, m, k0, levels, pre, cyc
     P.close()
 
 
+@pytest.mark.parametrize("order,m,k0,levels", [(2, 1, 2, 5), (2, 3, 0, 4), (2, 11, 0, 2), (1, 1, 2, 4)])
+def test_fp32_krylov_columns_match_fp64_columns(order, m, k0, levels, monkeypatch):
+    # the mixed lattice FGMRES keeps its columns z_i = M^-1 v_i in FP32 (the
+    # V-cycle's output, exact): the FP64 lattice operator reads them as FP32
+    # and writes the column, and x += Z y converts them back.  Every value
+    # is converted exactly, so the solve is bitwise the one with FP64
+    # columns (CD3D_FGMRES_Z32=0).  Q1 takes no quad form and keeps FP64
+    # columns in both settings
+    P = pm.Problem(order, m, k0, levels, box=(1.0, 1.0, 1.0))
+    P.assemble(pm.coeffs(eps=1e-3, b=(1.0, 0.5, 0.25), supg=1, f_val=1.0))
+    sp = pm.solver_params(**dict(REF_SOLVER, omega=0.5, pre=3, post=3, coarse_smoother=17, rel_tol=1e-10),
+                          precision=pm.MIXED)
+    out = {}
+    for z32 in ("1", "0"):
+        monkeypatch.setenv("CD3D_FGMRES_Z32", z32)
+        x, st = P.solve(sp, check=False)
+        out[z32] = (x, st.iterations, st.status, st.rfinal)
+    say(f"[gpu] FP32 Krylov columns order={order} cells={m * 2 ** (k0 + levels - 1)}: its {out['1'][1]} / "
+        f"{out['0'][1]}, status {out['1'][2]}")
+    assert out["1"][2] == out["0"][2] and out["1"][1] == out["0"][1]
+    assert np.array_equal(out["1"][0], out["0"][0]) and out["1"][3] == out["0"][3]
+    P.close()
+
+
 @pytest.mark.parametrize("order,m,k0,levels", [(1, 1, 2, 4), (2, 1, 1, 3)])
 def test_rotating_field_lattice_fgmres(order, m, k0, levels):
     # Galerkin rotating field: the CSR values are the affine class stencils,
