@@ -343,20 +343,25 @@ __host__ __device__ inline void lat_pair(const LatArgs &a, long long k, const fl
     d0 = hv ? inv[a.c[2 * PY + 4 * PZ].first + q.kv] : 0.f;
     d1 = inv[a.c[2 * PY + 4 * PZ + 1].first + k];
   }
+  float r0, r1;
   if (MODE == kLJacobiZero) {
-    if (hv) out[nv] = omega * fp.x * d0;
-    out[nv + 1] = omega * fp.y * d1;
-    return;
-  }
-  float s0, s1, xv, xm;
-  pair_sums<float, PY, PZ, UNI>(a, q, k, val, cs0, cs1, x, s0, s1, xv, xm);
-  if (MODE == kLDefect) {
-    if (hv) out[nv] = fp.x - s0;
-    out[nv + 1] = fp.y - s1;
+    r0 = omega * fp.x * d0;
+    r1 = omega * fp.y * d1;
   } else {
-    if (hv) out[nv] = xv + omega * (fp.x - s0) * d0;
-    out[nv + 1] = xm + omega * (fp.y - s1) * d1;
+    float s0, s1, xv, xm;
+    pair_sums<float, PY, PZ, UNI>(a, q, k, val, cs0, cs1, x, s0, s1, xv, xm);
+    if (MODE == kLDefect) {
+      r0 = fp.x - s0;
+      r1 = fp.y - s1;
+    } else {
+      r0 = xv + omega * (fp.x - s0) * d0;
+      r1 = xm + omega * (fp.y - s1) * d1;
+    }
   }
+  // vertex and midpoint are adjacent: one 8-byte store unless the vertex is
+  // a boundary node (not written)
+  if (hv) *(f2u *)(out + nv) = f2u{r0, r1};
+  else out[nv + 1] = r1;
 }
 
 // pair k of pair class pc = py + 2 pz; UNI: every class has compact values
@@ -674,18 +679,32 @@ __host__ __device__ __attribute__((always_inline)) inline void lat_quad_node(con
         for (int e = 0; e < 4; e++) fv[4 * h + e] = t[e];
       }
     }
+    float r[2 * G];
 #pragma unroll
     for (int q = 0; q < G; q++) {
       if (MODE == kLJacobiZero) {
-        if (q > 0 || p.hv) out[nv + 2 * q] = omega * fv[2 * q] * d0;
-        out[nv + 2 * q + 1] = omega * fv[2 * q + 1] * d1;
+        r[2 * q] = omega * fv[2 * q] * d0;
+        r[2 * q + 1] = omega * fv[2 * q + 1] * d1;
       } else if (MODE == kLDefect) {
-        if (q > 0 || p.hv) out[nv + 2 * q] = fv[2 * q] - s0[pc][q];
-        out[nv + 2 * q + 1] = fv[2 * q + 1] - s1[pc][q];
+        r[2 * q] = fv[2 * q] - s0[pc][q];
+        r[2 * q + 1] = fv[2 * q + 1] - s1[pc][q];
       } else {
-        if (q > 0 || p.hv) out[nv + 2 * q] = xs[pc][2 * q] + omega * (fv[2 * q] - s0[pc][q]) * d0;
-        out[nv + 2 * q + 1] = xs[pc][2 * q + 1] + omega * (fv[2 * q + 1] - s1[pc][q]) * d1;
+        r[2 * q] = xs[pc][2 * q] + omega * (fv[2 * q] - s0[pc][q]) * d0;
+        r[2 * q + 1] = xs[pc][2 * q + 1] + omega * (fv[2 * q + 1] - s1[pc][q]) * d1;
       }
+    }
+    // the segment's 2G nodes are adjacent: vector stores (4-byte alignment
+    // suffices) unless its first vertex is a boundary node (not written)
+    if (p.hv) {
+      if constexpr (G == 1) {
+        *(f2u *)(out + nv) = f2u{r[0], r[1]};
+      } else {
+#pragma unroll
+        for (int h = 0; h < G / 2; h++) *(f4u *)(out + nv + 4 * h) = f4u{r[4 * h], r[4 * h + 1], r[4 * h + 2], r[4 * h + 3]};
+      }
+    } else {
+#pragma unroll
+      for (int e = 1; e < 2 * G; e++) out[nv + e] = r[e];
     }
   }
 }
@@ -870,11 +889,14 @@ __host__ __device__ inline void restrict_pair(const LatArgs &ca, long long k, in
     s0 += RZ::w(c) * sy0;
     s1 += RZ::w(c) * sy1;
   }
-  if (q.hv) coarse[q.nv] = s0;
-  coarse[q.nv + 1] = s1;
-  if (zs.x) {
-    if (q.hv) zs.x[q.nv] = zs.omega * s0 * dz0;
-    zs.x[q.nv + 1] = zs.omega * s1 * dz1;
+  // vertex and midpoint are adjacent: one 8-byte store (4-byte alignment
+  // suffices) unless the vertex is a boundary node
+  if (q.hv) {
+    *(f2u *)(coarse + q.nv) = f2u{s0, s1};
+    if (zs.x) *(f2u *)(zs.x + q.nv) = f2u{zs.omega * s0 * dz0, zs.omega * s1 * dz1};
+  } else {
+    coarse[q.nv + 1] = s1;
+    if (zs.x) zs.x[q.nv + 1] = zs.omega * s1 * dz1;
   }
 }
 
@@ -1723,13 +1745,18 @@ __global__ __launch_bounds__(kB) void lat64_quad_gk(LatArgs a, const double *__r
 #pragma unroll
     for (int q = 0; q < G; q++) {
       const long long v = nv + 2 * q;
-      if (q > 0 || p.hv) out[v] = MODE == kL64Defect ? f[v] - s0[pc][q] : s0[pc][q];
-      out[v + 1] = MODE == kL64Defect ? f[v + 1] - s1[pc][q] : s1[pc][q];
+      const double r0 = MODE == kL64Defect ? f[v] - s0[pc][q] : s0[pc][q];
+      const double r1 = MODE == kL64Defect ? f[v + 1] - s1[pc][q] : s1[pc][q];
+      // vertex and midpoint adjacent: paired stores (8-byte alignment of
+      // the doubles suffices) unless the vertex is a boundary node
+      typedef double d2u __attribute__((ext_vector_type(2), aligned(8)));
+      if (q > 0 || p.hv) *(d2u *)(out + v) = d2u{r0, r1};
+      else out[v + 1] = r1;
       if constexpr (sizeof(TX) == 4) {
         // the node values again (just read: cache hits), not held in registers
         if (xcopy) {
-          if (q > 0 || p.hv) xcopy[v] = x[v];
-          xcopy[v + 1] = x[v + 1];
+          if (q > 0 || p.hv) *(f2u *)(xcopy + v) = *(const f2u *)(x + v);
+          else xcopy[v + 1] = x[v + 1];
         }
       }
     }
