@@ -93,7 +93,7 @@ int frame_code(const int8_t *E) {
   return code;
 }
 
-cd3d_status run(cd3d_problem *p, const std::function<void()> &f);
+cd3d_status run(cd3d_problem *p, const std::function<void()> &f, bool join_before = true, bool join_after = true);
 
 }  // namespace
 
@@ -1438,13 +1438,19 @@ void check_params(const cd3d_solver_params &sp) {
   if (!(sp.omega > 0) || !(sp.gmg_damp > 0)) throw ArgError("damping factors must be positive");
 }
 
-cd3d_status run(cd3d_problem *p, const std::function<void()> &f) {
+// join_before / join_after: the main stream waits for a pending coarse
+// inverse at the start / end of the call.  cd3d_assemble leaves its inverse
+// running on the side stream and cd3d_solve, which joins it only before its
+// first coarse solve, does not wait at its start, so the inversion runs
+// beside the solve's first defect, Krylov vector and pre-smoothing; every
+// other call waits at both ends.
+cd3d_status run(cd3d_problem *p, const std::function<void()> &f, bool join_before, bool join_after) {
   if (!p) return CD3D_ERR_ARG;
   try {
     p->fork_recorded = false;  // a fork point never outlives its call
-    coarse_join(*p);           // left pending by a call that failed
+    if (join_before) coarse_join(*p);  // (also: left pending by a call that failed)
     f();
-    coarse_join(*p);
+    if (join_after) coarse_join(*p);
     return CD3D_OK;
   } catch (const ArgError &e) {
     p->err = e.what();
@@ -1693,7 +1699,7 @@ cd3d_status cd3d_assemble(cd3d_problem *p, const cd3d_coeffs *co) {
     HIPCHK(hipStreamSynchronize(p->stream));
     p->assembled = true;
     p->tcd_ready = false;  // the level matrices no longer hold the mass matrices
-  });
+  }, true, false);  // the coarse inverse may still run: the next call joins it
 }
 
 static void host_vec_op(cd3d_problem *p, int level, const double *in, double *out, int nin, int nout,
@@ -1848,7 +1854,7 @@ cd3d_status cd3d_solve(cd3d_problem *p, const cd3d_solver_params *sp, const doub
       st->rfinal = rf;
       st->ms_solve = now_ms() - t0;
     }
-  });
+  }, false, true);  // a coarse inverse left by cd3d_assemble is joined before its first use
   return rc != CD3D_OK ? rc : solve_status;
 }
 
